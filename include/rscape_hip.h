@@ -1,0 +1,191 @@
+/* rscape_hip.h -- C-ABI of the MI355X pairwise-covariation library (librscape_hip.so).
+ *
+ * Plain C types only: pointers, sizes, ints, doubles.  Every entry point returns
+ * RSG_OK (0, Easel's eslOK) or RSG_FAIL (1, eslFAIL) and writes a message into a
+ * caller-provided errbuf of RSG_ERRBUFSIZE bytes (Easel's eslERRBUFSIZE); no C++
+ * exception crosses this boundary.  HIP errors and device-side failures (NaN in a
+ * correction, impossible histogram bins) are turned into RSG_FAIL plus text.
+ *
+ * Threading: one context = one HIP stream; a context is not re-entrant.  Calls are
+ * synchronous at this boundary (work is queued on the context stream, and the call
+ * returns after it finished), except where marked "async".
+ *
+ * Each entry point names the reference interface it replaces (R-scape v2.6.11).
+ * Symbol conventions follow the reference: digital RNA codes 0..3 = A,C,G,U,
+ * 4 = gap, 15 = N (src/msamanip.c:284-330); matrices are L x L row-major with
+ * COV[i][i] = -inf, Eval[i][i] = +inf (src/correlators.c:1254-1268).
+ */
+#ifndef RSCAPE_HIP_H
+#define RSCAPE_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSG_OK          0
+#define RSG_FAIL        1
+#define RSG_ERRBUFSIZE  128
+
+/* COVTYPE values (src/correlators.h:44-89). */
+enum { RSG_CHI = 0, RSG_CHIp, RSG_CHIa, RSG_GT, RSG_GTp, RSG_GTa, RSG_MI, RSG_MIp, RSG_MIa,
+       RSG_MIr, RSG_MIrp, RSG_MIra, RSG_MIg, RSG_MIgp, RSG_MIga, RSG_OMES, RSG_OMESp, RSG_OMESa,
+       RSG_RAF, RSG_RAFp, RSG_RAFa, RSG_RAFS, RSG_RAFSp, RSG_RAFSa, RSG_CCF, RSG_CCFp, RSG_CCFa };
+/* COVCLASS (src/correlators.h:37-42) and ACTYPE (:91-94). */
+enum { RSG_C16 = 0, RSG_C2 = 1, RSG_CWC = 2, RSG_CSELECT = 3 };
+enum { RSG_APC = 0, RSG_ASC = 1 };
+
+/* How pair counts are formed. */
+enum {
+  RSG_COUNTS_INT8  = 0,   /* unit weights: int8 one-hot Gram, int32 accumulation, bit-exact */
+  RSG_COUNTS_BF16  = 1,   /* sequence weights: split-BF16 (hi+lo) Gram, FP32 accumulation  */
+  RSG_COUNTS_BF16X3 = 2   /* sequence weights: 3-term BF16 split, FP32 accumulation        */
+};
+/* Arithmetic of the per-pair statistics. */
+enum { RSG_STAT_FP32 = 0, RSG_STAT_FP64 = 1 };
+
+typedef struct rsg_ctx rsg_ctx;   /* device, stream, scratch, timers */
+typedef struct rsg_msa rsg_msa;   /* one alignment resident in HBM + its pair tables */
+
+/* ---- context ---- */
+rsg_ctx *rsg_ctx_create(int device, char *errbuf);
+void     rsg_ctx_destroy(rsg_ctx *ctx);
+/* Per-kernel timing of the next calls (HIP events on the context stream).  The
+ * durations of the most recent launches, in ms, are returned by rsg_ctx_timings:
+ * names[k] / ms[k] for k < *n. */
+int      rsg_ctx_set_timing(rsg_ctx *ctx, int on);
+int      rsg_ctx_timings(rsg_ctx *ctx, const char **names, double *ms, int *n, int cap);
+/* Gram tile shape: 0 = autotuned / default, else an index into the shape table
+ * (rsg_gram_shape_name gives its text).  Used by the autotuner. */
+int         rsg_gram_nshapes(void);
+const char *rsg_gram_shape_name(int shape);
+int         rsg_ctx_set_gram_shape(rsg_ctx *ctx, int shape);
+
+/* ---- alignment upload (replaces the ESL_MSA the corr_* calls read: ax[s][1..L], wgt[s]) ----
+ * ax: nseq x alen digital codes, row-major, 0-based columns (msa->ax[s][i+1] in Easel).
+ * wgt: nseq FP64 sequence weights, or NULL for unit weights. */
+int  rsg_msa_create(rsg_ctx *ctx, const uint8_t *ax, const double *wgt, int64_t nseq, int64_t alen,
+                    rsg_msa **ret_msa, char *errbuf);
+/* Replace the symbols / weights of an existing alignment of the same shape (async). */
+int  rsg_msa_update(rsg_ctx *ctx, rsg_msa *msa, const uint8_t *ax, const double *wgt, char *errbuf);
+void rsg_msa_destroy(rsg_ctx *ctx, rsg_msa *msa);
+
+/* ---- corr_Probs (src/correlators.c:1423-1496, correlators.h:478) ----
+ * Pair counts (int8 or split-BF16 Gram), nseff/ngap, pm (corr_Marginals, FP64) and ps
+ * (corr_NaivePS, FP64) on the device.  Validation as corr_ValidateProbs with tol. */
+int rsg_corr_Probs(rsg_ctx *ctx, rsg_msa *msa, int counts_mode, double tol, char *errbuf);
+/* Materialise the probability tables in the reference layout (any pointer may be NULL):
+ * pp L*L*16 (pp[i][j][IDX(x,y,4)], both triangles), pm L*4, ps L*5, nseff L*L, ngap L*L
+ * (ngap written for i<j only, as the reference). */
+int rsg_corr_GetProbs(rsg_ctx *ctx, rsg_msa *msa, double *pp, double *pm, double *ps,
+                      double *nseff, double *ngap, char *errbuf);
+/* Raw pair counts for i<j: cnt[(i*L+j)*16 + x*4+y] (unit weights: integers). */
+int rsg_corr_GetCounts(rsg_ctx *ctx, rsg_msa *msa, double *cnt, char *errbuf);
+
+/* ---- corr_Calculate{CHI,OMES,GT,MI,MIr,MIg,CCF} (src/correlators.c:49-1061) ----
+ * covtype is a base type (RSG_CHI, RSG_GT, ...); covclass C16/C2/CWC/CSELECT with the
+ * CSELECT rule of src/correlators.c:335-341.  allowpair: 4x4 (NULL = WC + GU). */
+int rsg_corr_CalculateCOV(rsg_ctx *ctx, rsg_msa *msa, int covtype, int covclass,
+                          const double *allowpair, int nseqthresh, int alenthresh,
+                          int stat_precision, char *errbuf);
+/* corr_CalculateRAF / corr_CalculateRAFS (src/correlators.c:876-982): unweighted, from counts. */
+int rsg_corr_CalculateRAF(rsg_ctx *ctx, rsg_msa *msa, int rafs, const double *allowpair,
+                          int stat_precision, char *errbuf);
+/* corr_CalculateCOVCorrected (src/correlators.c:1063-1157). */
+int rsg_corr_CalculateCOVCorrected(rsg_ctx *ctx, rsg_msa *msa, int actype, int shiftnonneg, char *errbuf);
+/* mi->COV and mi->minCOV / mi->maxCOV. COV may be NULL. */
+int rsg_corr_GetCOV(rsg_ctx *ctx, rsg_msa *msa, double *COV, double *minCOV, double *maxCOV, char *errbuf);
+
+/* ---- cov_Calculate (src/covariation.c:63-306) without the ranking:
+ * corr_Probs + statistic + optional correction in one call (the hot path). */
+int rsg_cov_Calculate(rsg_ctx *ctx, rsg_msa *msa, int covtype, int covclass, int counts_mode,
+                      int stat_precision, const double *allowpair, int nseqthresh, int alenthresh,
+                      double tol, double *minCOV, double *maxCOV, char *errbuf);
+
+/* ---- ranking, null histogram, tail fit, E-values (src/covariation.c:325-530, 1914-1975,
+ *      2369-2506; src/R-scape.c:1280-1371, 1564-1612) ----
+ * Histograms follow Easel's esl_histogram binning: bin b covers (bmin + b w, bmin + (b+1) w]. */
+typedef struct {
+  double    bmin, bmax, w;
+  int32_t   nb, imin, imax, cmin;
+  double    xmin, xmax, phi;
+  uint64_t  n, Nc, No;
+  int32_t   tailfit;
+  uint64_t *obs;             /* nb counts, owned by the library */
+} rsg_hist;
+
+/* calculate_width_histo (R-scape.c:1353-1360): w from the current COV extremes. */
+double rsg_width(double maxCOV, double minCOV, double bmin, double w_old, int hpts, double tol);
+/* Histogram of max(COV_ij, bmin + w) over i<j, bmax = maxCOV + 5w (covariation.c:414-432). */
+int  rsg_cov_RankHist(rsg_ctx *ctx, rsg_msa *msa, double bmin, double w, double tol,
+                      rsg_hist **ret_hist, char *errbuf);
+/* null_add2cumranklist (R-scape.c:1564-1612): merge h into *cum (created if NULL). */
+int  rsg_hist_NullAdd(rsg_hist **cum, const rsg_hist *h, char *errbuf);
+/* cov_histogram_pmass + SetTailByMass + cov_NullFit{Exponential,Gamma} + SetSurvFitTail.
+ * Runs in FP64 on the device.  fit = {mu, lambda, tau, newmass, pmass}. survfit: 2*nb doubles
+ * owned by the caller (allocated by the library with rsg_free). NULL if the fit is infinite. */
+int  rsg_hist_NullFit(rsg_ctx *ctx, rsg_hist *null, double pmass, double fracfit, int doexpfit,
+                      double **ret_survfit, double *fit, char *errbuf);
+/* cov2evalue for every pair: Eval[i][j] = p * Nc (covariation.c:846-856), FP64 on the device. */
+int  rsg_cov_Evalues(rsg_ctx *ctx, rsg_msa *msa, const rsg_hist *null, const double *survfit,
+                     int64_t Nc, double *Eval, char *errbuf);
+/* evalue2cov (covariation.c:2404-2436). */
+double rsg_evalue2cov(double eval_thresh, int64_t Nc, const rsg_hist *h, const double *survfit);
+double rsg_cov2evalue(double cov, int64_t Nc, const rsg_hist *h, const double *survfit);
+rsg_hist *rsg_hist_create(double bmin, double bmax, double w);
+void rsg_hist_destroy(rsg_hist *h);
+void rsg_free(void *p);
+
+/* ---- sequence weights and preprocessing (src/R-scape.c:1544-1562; src/msamanip.c:284-541) ---- */
+/* esl_msaweight_PB: position-based weights, FP64 on the device. */
+int rsg_weights_PB(rsg_ctx *ctx, const uint8_t *ax, int64_t nseq, int64_t alen, double *wgt, char *errbuf);
+/* esl_msaweight_GSC: identities on the device (a sequence x sequence one-hot Gram),
+ * UPGMA + GSC on the host in FP64. */
+int rsg_weights_GSC(rsg_ctx *ctx, const uint8_t *ax, int64_t nseq, int64_t alen, double *wgt, char *errbuf);
+/* msamanip_RemoveGapColumns column mask (gapthresh, weights as given). Returns the kept count. */
+int rsg_gapcol_mask(const uint8_t *ax, const double *wgt, int64_t nseq, int64_t alen, double gapthresh,
+                    int *useme);
+
+/* ---- batched phylogenetic null (cov_GenerateAlignment, src/cov_simulate.c:60-859) ----
+ * A batch of independent alignments, each evolved down its own tree from its own root
+ * with per-branch conditional tables, scored (statistic + correction) and histogrammed
+ * on the device.  See rsg_null.h-style structs below. */
+typedef struct {
+  int64_t  nleaf;            /* N sequences */
+  int64_t  alen;             /* L columns */
+  const int *left, *right;   /* N-1 internal nodes, Easel ESL_TREE convention */
+  const int *tb_l, *tb_r;    /* branch -> conditional-table index */
+  const uint32_t *cdf;       /* ntables x 4 rows x 3 uint32 thresholds */
+  int      ntables;
+  const uint8_t *root;       /* alen root symbols 0..3 */
+  uint64_t seed;
+  uint32_t stream;
+  double   gapfrac;          /* i.i.d. gap probability per leaf symbol (0 = none) */
+} rsg_null_job;
+
+/* Simulate one alignment; leaves = nleaf x alen codes (the reference's leaf rows). */
+int rsg_null_simulate(rsg_ctx *ctx, const rsg_null_job *job, uint8_t *leaves, char *errbuf);
+/* Simulate and score a batch (statistic + correction, unit weights, CSELECT) and
+ * return per-alignment histograms (bmin, w per job) plus FP64 exponential tail fits
+ * {mu, lambda, newmass}.  use_graph = 1 captures the batch in a hipGraph. */
+int rsg_null_batch(rsg_ctx *ctx, const rsg_null_job *jobs, int njobs, int covtype,
+                   double bmin, const double *w, double pmass, double fracfit,
+                   rsg_hist **hists, double *fits, int use_graph, char *errbuf);
+
+/* ---- learned null-tail surrogate (no reference counterpart) ---- */
+typedef struct rsg_mlp rsg_mlp;
+int  rsg_mlp_create(rsg_ctx *ctx, int nlayers, const int *dims, uint64_t seed, rsg_mlp **ret, char *errbuf);
+void rsg_mlp_destroy(rsg_ctx *ctx, rsg_mlp *m);
+int  rsg_mlp_set_params(rsg_ctx *ctx, rsg_mlp *m, const float *const *W, const float *const *b, char *errbuf);
+int  rsg_mlp_get_params(rsg_ctx *ctx, rsg_mlp *m, float *const *W, float *const *b, char *errbuf);
+int  rsg_mlp_forward(rsg_ctx *ctx, rsg_mlp *m, const float *x, int64_t B, float *y, char *errbuf);
+/* One forward+backward on a batch, returning parameter gradients (FP32) and the MSE loss. */
+int  rsg_mlp_grad(rsg_ctx *ctx, rsg_mlp *m, const float *x, const float *t, int64_t B,
+                  float *const *dW, float *const *db, double *loss, char *errbuf);
+/* Adam training over a dataset resident on the device. */
+int  rsg_mlp_train(rsg_ctx *ctx, rsg_mlp *m, const float *x, const float *t, int64_t n,
+                   int epochs, int batch, double lr, uint64_t seed, double *final_loss, char *errbuf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,208 @@
+/* ref_sim.c -- oracle: counter-based RNG, rate-matrix evolution down a tree,
+ * and R-scape's phylogenetic null (Fitch ancestors + column shuffle + replay).
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ *
+ * The reference draws from Easel's sequential RNG (un-vendored); that stream
+ * cannot be reproduced.  This restatement replaces it with Philox4x32-10
+ * (Salmon, Moraes, Dror & Shaw, SC'11), keyed so that every draw is a pure
+ * function of (seed, stream, node, position, purpose).  The GPU simulator uses
+ * the same keys, so the two agree symbol for symbol; agreement with the
+ * reference itself is distributional only.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "refcpu.h"
+
+static void mulhilo(uint32_t a, uint32_t b, uint32_t *hi, uint32_t *lo)
+{
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  *hi = (uint32_t)(p >> 32); *lo = (uint32_t)p;
+}
+
+void ref_philox4x32(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4])
+{
+  uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; r++) {
+    uint32_t hi0, lo0, hi1, lo1;
+    mulhilo(0xD2511F53u, c0, &hi0, &lo0);
+    mulhilo(0xCD9E8D57u, c2, &hi1, &lo1);
+    uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* One 32-bit draw for (seed, stream, node, pos); purpose tag 0. */
+uint32_t ref_rng_u32(uint64_t seed, uint32_t stream, uint32_t node, uint32_t pos)
+{
+  uint32_t ctr[4] = { pos, node, stream, 0u }, key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) }, o[4];
+  ref_philox4x32(ctr, key, o);
+  return o[0];
+}
+
+static uint32_t rng_tag(uint64_t seed, uint32_t stream, uint32_t node, uint32_t pos, uint32_t tag)
+{
+  uint32_t ctr[4] = { pos, node, stream, tag }, key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) }, o[4];
+  ref_philox4x32(ctr, key, o);
+  return o[0];
+}
+
+/* Inverse CDF over n categories with n-1 uint32 thresholds: the reference's
+ * "pdf += P[i]; if (pdf > x) break" (cov_simulate.c:756-763) with x = u / 2^32. */
+static int icdf(uint32_t u, const uint32_t *cdf, int n)
+{
+  int k = 0;
+  for (int t = 0; t < n - 1; t++) k += (u >= cdf[t]);
+  return k;
+}
+
+/* cov_simulate.c:288-324 + :387-451 + :585-631 + :724-798 (ungapped evolution).
+ * Nodes: internal v = 0..nleaf-2 (root 0, parents before children), rows 0..nleaf-2 of
+ * allseq; leaf l is row nleaf-1+l.  left/right > 0 are internal, <= 0 leaves.
+ * pidx_l/r: index of the branch's 4x4 conditional table; cdf_l/r hold, per table, 4 rows x 3
+ * thresholds.  ct[pos] = partner (0-based) or -1; paired 5' positions draw a 16-state pair from
+ * pcdf (per table 16 rows x 15 thresholds), the 3' partner takes the second symbol. */
+int ref_evolve_tree(int64_t nleaf, const int *left, const int *right,
+                    const int *pidx_l, const int *pidx_r,
+                    const uint32_t *cdf_l, const uint32_t *cdf_r,
+                    const uint8_t *root, int64_t L, const int *ct,
+                    const uint32_t *pcdf_l, const uint32_t *pcdf_r,
+                    uint64_t seed, uint32_t stream, uint8_t *allseq)
+{
+  memcpy(allseq, root, (size_t)L);
+  for (int64_t v = 0; v < nleaf - 1; v++) {
+    for (int side = 0; side < 2; side++) {
+      int d = side ? right[v] : left[v];
+      int64_t row = (d > 0) ? d : (nleaf - 1 - d);
+      int tb = side ? pidx_r[v] : pidx_l[v];
+      const uint32_t *cdf  = (side ? cdf_r : cdf_l) + (size_t)tb * 12;
+      const uint32_t *pcdf = (side ? pcdf_r : pcdf_l);
+      const uint8_t *par = allseq + v * L;
+      uint8_t *ch = allseq + row * L;
+      for (int64_t pos = 0; pos < L; pos++) {
+        int pr = ct ? ct[pos] : -1;
+        if (pr >= 0 && pcdf) {
+          if (pr < pos) continue;                       /* emitted with its 5' partner */
+          int a = par[pos], b = par[pr];
+          uint32_t u = ref_rng_u32(seed, stream, (uint32_t)row, (uint32_t)pos);
+          int k = icdf(u, pcdf + (size_t)tb * 240 + (size_t)(a * 4 + b) * 15, 16);
+          ch[pos] = (uint8_t)(k / 4); ch[pr] = (uint8_t)(k % 4);
+        } else {
+          uint32_t u = ref_rng_u32(seed, stream, (uint32_t)row, (uint32_t)pos);
+          ch[pos] = (uint8_t)icdf(u, cdf + par[pos] * 3, 4);
+        }
+      }
+    }
+  }
+  return REF_OK;
+}
+
+/* R-scape's null (R-scape.c:1650-1697): Fitch ancestral reconstruction per column
+ * (msatree.c:173-229, 1700-1931), one shuffle of all columns of leaves+ancestors
+ * (msamanip.c:1461 -> 1164), then for every branch parent -> child replay the
+ * original branch's (old -> new) substitution counts at random positions of the
+ * shuffled parent holding old (msamanip.c:1597-1780).  Symbols 0..4 (gap = 4).
+ * Tree layout as ref_evolve_tree; allmsa rows follow the reference: leaves 0..N-1,
+ * internal node v at N+v.  out receives the N shuffled leaves (N x L). */
+#define DIM 6   /* residues 0..3, gap 4, "set" flag 5 */
+int ref_fitch_null(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
+                   uint64_t seed, uint8_t *out)
+{
+  int64_t nn = 2 * N - 1;
+  uint8_t *all = malloc((size_t)(nn * L)), *sh = malloc((size_t)(nn * L));
+  int (*S)[DIM] = malloc(sizeof(int[DIM]) * (size_t)nn);
+  int *parent = malloc(sizeof(int) * (size_t)N), *perm = malloc(sizeof(int) * (size_t)L);
+  int *cols = malloc(sizeof(int) * (size_t)L), *cperm = malloc(sizeof(int) * (size_t)L);
+  memcpy(all, ax, (size_t)(N * L));
+  parent[0] = 0;
+  for (int v = 0; v < N - 1; v++) {
+    if (left[v] > 0)  parent[left[v]] = v;
+    if (right[v] > 0) parent[right[v]] = v;
+  }
+  uint32_t draw = 0;
+  for (int64_t c = 0; c < L; c++) {
+    memset(S, 0, sizeof(int[DIM]) * (size_t)nn);
+    for (int64_t n = 0; n < N; n++) {
+      uint8_t a = all[n * L + c];
+      if (a <= 4) S[n][a] = 1;
+      else {                                              /* unknown: random symbol 0..4 */
+        uint32_t u = rng_tag(seed, 1u, (uint32_t)n, (uint32_t)c, 1u);
+        S[n][(int)(((uint64_t)u * 5u) >> 32)] = 1;
+      }
+      S[n][DIM - 1] = 1;
+    }
+    /* upward pass in post-order: children have larger indices than parents */
+    for (int v = (int)N - 2; v >= 0; v--) {
+      int il = (left[v] <= 0) ? -left[v] : (int)N + left[v];
+      int ir = (right[v] <= 0) ? -right[v] : (int)N + right[v];
+      int *Sv = S[N + v], *Sl = S[il], *Sr = S[ir], any = 0;
+      for (int i = 0; i < DIM - 1; i++) if (Sl[i] && Sr[i]) { Sv[i] = 1; any = 1; }
+      if (!any) {
+        for (int i = 0; i < DIM - 2; i++) if (Sl[i] || Sr[i]) Sv[i] = 1;
+        if (Sl[DIM - 2]) for (int i = 0; i < DIM - 2; i++) if (Sr[i]) Sv[i] = 1;
+        if (Sr[DIM - 2]) for (int i = 0; i < DIM - 2; i++) if (Sl[i]) Sv[i] = 1;
+      }
+      Sv[DIM - 1] = 1;
+    }
+    /* choose: uniform among the set members (tree_fitch_choose with frq == NULL) */
+    for (int v = 0; v < N - 1; v++) {
+      int *Sv = S[N + v];
+      if (v > 0) {
+        int pa = all[(N + parent[v]) * L + c];
+        if (Sv[pa]) for (int i = 0; i < DIM - 1; i++) if (i != pa) Sv[i] = 0;
+      }
+      int m = 0, opts[5];
+      for (int i = 0; i < DIM - 1; i++) if (Sv[i]) opts[m++] = i;
+      uint32_t u = rng_tag(seed, 2u, (uint32_t)v, (uint32_t)c, 2u);
+      all[(N + v) * L + c] = (uint8_t)opts[((uint64_t)u * (uint64_t)m) >> 32];
+    }
+  }
+  /* one permutation of all columns (Fisher-Yates, as esl_vec_IShuffle) */
+  for (int64_t c = 0; c < L; c++) perm[c] = (int)c;
+  for (int64_t n = L; n > 1; n--) {
+    uint32_t u = rng_tag(seed, 3u, 0u, (uint32_t)n, 3u);
+    int64_t k = (int64_t)(((uint64_t)u * (uint64_t)n) >> 32);
+    int t = perm[n - 1]; perm[n - 1] = perm[k]; perm[k] = t;
+  }
+  for (int64_t r = 0; r < nn; r++)
+    for (int64_t c = 0; c < L; c++) sh[r * L + c] = all[r * L + perm[c]];
+  /* replay substitutions branch by branch, left then right */
+  for (int v = 0; v < N - 1; v++) {
+    for (int side = 0; side < 2; side++) {
+      int d = side ? right[v] : left[v];
+      int64_t pa = N + v, ch = (d <= 0) ? -d : N + d;
+      int nsub[25] = {0};
+      for (int64_t c = 0; c < L; c++) {
+        uint8_t a = all[pa * L + c], b = all[ch * L + c];
+        if (a != b && a <= 4 && b <= 4) nsub[a * 5 + b]++;
+        sh[ch * L + c] = sh[pa * L + c];
+      }
+      for (int oldc = 0; oldc < 5; oldc++) {
+        int nst = 0;
+        for (int nw = 0; nw < 5; nw++) nst += nsub[oldc * 5 + nw];
+        if (!nst) continue;
+        int ncol = 0;
+        for (int64_t c = 0; c < L; c++) if (sh[pa * L + c] == oldc) cols[ncol++] = (int)c;
+        if (!ncol) continue;
+        for (int k = 0; k < ncol; k++) cperm[k] = k;
+        for (int n = ncol; n > 1; n--) {
+          uint32_t u = rng_tag(seed, 4u, (uint32_t)(ch * 5 + oldc), (uint32_t)n, 4u + draw);
+          int k = (int)(((uint64_t)u * (uint64_t)n) >> 32);
+          int t = cperm[n - 1]; cperm[n - 1] = cperm[k]; cperm[k] = t;
+        }
+        int idx = ncol - 1;
+        for (int nw = 0; nw < 5; nw++) {
+          int s = nsub[oldc * 5 + nw];
+          while (s > 0 && idx >= 0) { sh[ch * L + cols[cperm[idx]]] = (uint8_t)nw; idx--; s--; }
+        }
+      }
+    }
+  }
+  memcpy(out, sh, (size_t)(N * L));
+  free(all); free(sh); free(S); free(parent); free(perm); free(cols); free(cperm);
+  return REF_OK;
+}

@@ -1,0 +1,288 @@
+// rsg_capi.hip -- extern "C" entry points of librscape_hip (declared in include/rscape_hip.h).
+#include "rsg_internal.h"
+#include <cmath>
+#include <cstring>
+#include <new>
+
+static const double kAllowWCGU[16] = {   // R-scape default allowpair: WC + GU (src/R-scape.c:882-887)
+  0, 0, 0, 1,
+  0, 0, 1, 0,
+  0, 1, 0, 1,
+  1, 0, 1, 0 };
+
+extern "C" rsg_ctx *rsg_ctx_create(int device, char *errbuf)
+{
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device) { rsg_fail(errbuf, "no HIP device %d (count %d)", device, n); return nullptr; }
+  if (hipSetDevice(device) != hipSuccess) { rsg_fail(errbuf, "hipSetDevice(%d) failed", device); return nullptr; }
+  rsg_ctx *c = new (std::nothrow) rsg_ctx();
+  if (!c) { rsg_fail(errbuf, "out of memory"); return nullptr; }
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&c->d_flag, 64) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    rsg_fail(errbuf, "context allocation failed");
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+extern "C" void rsg_ctx_destroy(rsg_ctx *c)
+{
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  for (int k = 0; k < RSG_MAX_TIMERS; k++)
+    if (c->timers[k].a) { hipEventDestroy(c->timers[k].a); hipEventDestroy(c->timers[k].b); }
+  if (c->d_red) hipFree(c->d_red);
+  if (c->d_flag) hipFree(c->d_flag);
+  if (c->h_pinned) hipHostFree(c->h_pinned);
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" int rsg_ctx_set_timing(rsg_ctx *c, int on) { c->timing = on; c->ntimers = 0; return RSG_OK; }
+
+extern "C" int rsg_ctx_timings(rsg_ctx *c, const char **names, double *ms, int *n, int cap)
+{
+  hipStreamSynchronize(c->stream);
+  int k = 0;
+  for (; k < c->ntimers && k < cap; k++) {
+    float f = 0.f;
+    hipEventElapsedTime(&f, c->timers[k].a, c->timers[k].b);
+    names[k] = c->timers[k].name; ms[k] = f;
+  }
+  *n = k;
+  c->ntimers = 0;
+  return RSG_OK;
+}
+
+extern "C" int rsg_ctx_set_gram_shape(rsg_ctx *c, int shape) { c->gram_shape = shape; return RSG_OK; }
+
+static void free_msa(rsg_msa *m)
+{
+  void *ptrs[] = { m->d_ax, m->d_axT, m->d_xt, m->d_w, m->d_xw[0], m->d_xw[1], m->d_xw[2], m->d_x1,
+                   m->d_cnt, m->d_cnt_u, m->d_pm, m->d_ps, m->d_lgc, m->d_cov, m->d_tmp, m->d_rowx };
+  for (void *p : ptrs) if (p) hipFree(p);
+  delete m;
+}
+
+extern "C" int rsg_msa_update(rsg_ctx *c, rsg_msa *m, const uint8_t *ax, const double *wgt, char *errbuf)
+{
+  const size_t nl = (size_t)m->N * (size_t)m->L;
+  if (ax) {
+    HIPCHK(hipMemcpyAsync(m->d_ax, ax, nl, hipMemcpyHostToDevice, c->stream));
+    if (rsg_pack(c, m, errbuf) != RSG_OK) return RSG_FAIL;
+  }
+  std::vector<double> w((size_t)m->N, 1.0);
+  if (wgt) memcpy(w.data(), wgt, sizeof(double) * (size_t)m->N);
+  m->weighted = (wgt != nullptr);
+  m->wsum = 0.;
+  for (double v : w) m->wsum += v;
+  HIPCHK(hipMemcpyAsync(m->d_w, w.data(), sizeof(double) * (size_t)m->N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  m->nsplit_ready = 0;
+  m->have_probs = m->have_cov = 0;
+  return RSG_OK;
+}
+
+extern "C" int rsg_msa_create(rsg_ctx *c, const uint8_t *ax, const double *wgt, int64_t nseq, int64_t alen,
+                              rsg_msa **ret, char *errbuf)
+{
+  *ret = nullptr;
+  if (nseq < 1 || alen < 1) return rsg_fail(errbuf, "empty alignment (nseq %lld, alen %lld)", (long long)nseq, (long long)alen);
+  if (nseq > (1ll << 24)) return rsg_fail(errbuf, "nseq %lld exceeds the int32 one-hot Gram range", (long long)nseq);
+  HIPCHK(hipSetDevice(c->device));
+  rsg_msa *m = new (std::nothrow) rsg_msa();
+  if (!m) return rsg_fail(errbuf, "out of memory");
+  m->N = nseq; m->L = alen;
+  m->Npad = rsg_npad(nseq);
+  m->Lpad = ((alen + 63) / 64) * 64;
+  const size_t nl = (size_t)nseq * (size_t)alen, tp = (size_t)m->Lpad * (size_t)m->Npad;
+  if (hipMalloc(&m->d_ax, nl) != hipSuccess || hipMalloc(&m->d_axT, tp) != hipSuccess ||
+      hipMalloc(&m->d_xt, tp) != hipSuccess || hipMalloc(&m->d_w, sizeof(double) * (size_t)nseq) != hipSuccess) {
+    free_msa(m);
+    return rsg_fail(errbuf, "device allocation failed for a %lld x %lld alignment", (long long)nseq, (long long)alen);
+  }
+  if (rsg_msa_update(c, m, ax, wgt, errbuf) != RSG_OK) { free_msa(m); return RSG_FAIL; }
+  *ret = m;
+  return RSG_OK;
+}
+
+extern "C" void rsg_msa_destroy(rsg_ctx *c, rsg_msa *m)
+{
+  if (!m) return;
+  if (c) hipStreamSynchronize(c->stream);
+  free_msa(m);
+}
+
+static int alloc_counts(rsg_msa *m, void **dst, int kind, char *errbuf)
+{
+  size_t esz = (kind == CNT_U16) ? 2 : 4;
+  size_t bytes = esz * 16 * (size_t)m->L * (size_t)m->L;
+  if (*dst && m->cnt_bytes >= bytes && dst == &m->d_cnt) return RSG_OK;
+  if (*dst) hipFree(*dst);
+  *dst = nullptr;
+  HIPCHK(hipMalloc(dst, bytes));
+  if (dst == &m->d_cnt) m->cnt_bytes = bytes;
+  return RSG_OK;
+}
+
+extern "C" int rsg_corr_Probs(rsg_ctx *c, rsg_msa *m, int mode, double tol, char *errbuf)
+{
+  HIPCHK(hipSetDevice(c->device));
+  m->have_probs = m->have_cov = 0;
+  if (mode == RSG_COUNTS_INT8) {
+    if (m->weighted) return rsg_fail(errbuf, "int8 counts need unit weights (alignment has weights)");
+    int kind = (m->N < 65536) ? CNT_U16 : CNT_U32;
+    if (m->cnt_kind != kind) { if (m->d_cnt) hipFree(m->d_cnt); m->d_cnt = nullptr; m->cnt_bytes = 0; }
+    if (alloc_counts(m, &m->d_cnt, kind, errbuf) != RSG_OK) return RSG_FAIL;
+    m->cnt_kind = kind;
+    if (rsg_gram_counts_i8(c, m, m->d_cnt, kind, errbuf) != RSG_OK) return RSG_FAIL;
+  } else if (mode == RSG_COUNTS_BF16 || mode == RSG_COUNTS_BF16X3) {
+    if (m->cnt_kind != CNT_F32) { if (m->d_cnt) hipFree(m->d_cnt); m->d_cnt = nullptr; m->cnt_bytes = 0; }
+    if (alloc_counts(m, &m->d_cnt, CNT_U32, errbuf) != RSG_OK) return RSG_FAIL;   // 4-byte records
+    m->cnt_kind = CNT_F32;
+    if (rsg_gram_counts_bf16(c, m, mode == RSG_COUNTS_BF16X3 ? 3 : 2, (float *)m->d_cnt, errbuf) != RSG_OK) return RSG_FAIL;
+  } else return rsg_fail(errbuf, "unknown counts mode %d", mode);
+  if (rsg_probs_from_counts(c, m, tol, errbuf) != RSG_OK) return RSG_FAIL;
+  int flag = 0;
+  HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  flag = *(int *)c->h_pinned;
+  if (flag) return rsg_fail(errbuf, "pm validation failed");
+  m->have_probs = 1;
+  return RSG_OK;
+}
+
+extern "C" int rsg_corr_GetCounts(rsg_ctx *c, rsg_msa *m, double *cnt, char *errbuf)
+{
+  if (!m->d_cnt) return rsg_fail(errbuf, "no counts");
+  const int64_t L = m->L;
+  const size_t n = 16 * (size_t)L * (size_t)L;
+  size_t esz = (m->cnt_kind == CNT_U16) ? 2 : 4;
+  std::vector<uint8_t> h(n * esz);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(h.data(), m->d_cnt, n * esz, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < L; i++)
+    for (int64_t j = 0; j < L; j++)
+      for (int k = 0; k < 16; k++) {
+        size_t o = ((size_t)i * L + j) * 16 + k;
+        double v = 0.;
+        if (i < j) {
+          if (m->cnt_kind == CNT_U16)      v = ((uint16_t *)h.data())[o];
+          else if (m->cnt_kind == CNT_U32) v = ((uint32_t *)h.data())[o];
+          else                             v = ((float *)h.data())[o];
+        }
+        cnt[o] = v;
+      }
+  return RSG_OK;
+}
+
+extern "C" int rsg_corr_GetProbs(rsg_ctx *c, rsg_msa *m, double *pp, double *pm, double *ps,
+                                 double *nseff, double *ngap, char *errbuf)
+{
+  if (!m->have_probs) return rsg_fail(errbuf, "corr_Probs has not run");
+  const int64_t L = m->L;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (pm) HIPCHK(hipMemcpy(pm, m->d_pm, sizeof(double) * 4 * (size_t)L, hipMemcpyDeviceToHost));
+  if (ps) HIPCHK(hipMemcpy(ps, m->d_ps, sizeof(double) * 5 * (size_t)L, hipMemcpyDeviceToHost));
+  if (pp || nseff || ngap) {
+    std::vector<double> cnt(16 * (size_t)L * (size_t)L);
+    if (rsg_corr_GetCounts(c, m, cnt.data(), errbuf) != RSG_OK) return RSG_FAIL;
+    for (int64_t i = 0; i < L; i++)
+      for (int64_t j = i + 1; j < L; j++) {
+        const double *r = &cnt[((size_t)i * L + j) * 16];
+        double ns = 0., tot = 0.;
+        for (int k = 0; k < 16; k++) { ns += r[k]; tot += r[k] + 1e-10; }
+        if (nseff) nseff[i * L + j] = nseff[j * L + i] = ns;
+        if (ngap) ngap[i * L + j] = m->wsum - ns;
+        if (pp)
+          for (int x = 0; x < 4; x++)
+            for (int y = 0; y < 4; y++) {
+              double v = (r[x * 4 + y] + 1e-10) / tot;
+              pp[((size_t)i * L + j) * 16 + x * 4 + y] = v;
+              pp[((size_t)j * L + i) * 16 + y * 4 + x] = v;
+            }
+      }
+    for (int64_t i = 0; i < L; i++) {
+      if (nseff) nseff[i * L + i] = 0.;
+      if (ngap) ngap[i * L + i] = 0.;
+      if (pp) for (int k = 0; k < 16; k++) pp[((size_t)i * L + i) * 16 + k] = 0.;
+    }
+  }
+  return RSG_OK;
+}
+
+static int resolve_class(const rsg_msa *m, int cls, int nseqthresh, int alenthresh)
+{
+  if (cls == RSG_CSELECT) return (m->N <= nseqthresh || m->L <= alenthresh) ? RSG_C2 : RSG_C16;
+  return cls;
+}
+
+extern "C" int rsg_corr_CalculateCOV(rsg_ctx *c, rsg_msa *m, int covtype, int covclass, const double *allowpair,
+                                     int nseqthresh, int alenthresh, int prec, char *errbuf)
+{
+  HIPCHK(hipSetDevice(c->device));
+  const double *allow = allowpair ? allowpair : kAllowWCGU;
+  int base = covtype - covtype % 3;
+  if (base == RSG_RAF || base == RSG_RAFS)
+    return rsg_corr_CalculateRAF(c, m, base == RSG_RAFS, allow, prec, errbuf);
+  if (!m->have_probs) return rsg_fail(errbuf, "corr_Probs has not run");
+  int cls = resolve_class(m, covclass, nseqthresh, alenthresh);
+  if (base == RSG_CCF) cls = RSG_C16;
+  if (cls == RSG_CWC && base != RSG_GT) return rsg_fail(errbuf, "CWC not implemented for covariation type %d", base);
+  if (cls < RSG_C16 || cls > RSG_CWC) return rsg_fail(errbuf, "bad covariation class %d", covclass);
+  return rsg_stat(c, m, base, cls, allow, prec, errbuf);
+}
+
+extern "C" int rsg_corr_CalculateRAF(rsg_ctx *c, rsg_msa *m, int rafs, const double *allowpair, int prec, char *errbuf)
+{
+  (void)prec;
+  HIPCHK(hipSetDevice(c->device));
+  const double *allow = allowpair ? allowpair : kAllowWCGU;
+  // RAF is defined on unweighted symbols (src/correlators.c:892-921): make sure unit counts exist.
+  if (m->cnt_kind == CNT_F32 || !m->d_cnt) {
+    int kind = (m->N < 65536) ? CNT_U16 : CNT_U32;
+    if (m->cnt_u_kind != kind) { if (m->d_cnt_u) hipFree(m->d_cnt_u); m->d_cnt_u = nullptr; }
+    if (!m->d_cnt_u) HIPCHK(hipMalloc(&m->d_cnt_u, (kind == CNT_U16 ? 2 : 4) * 16 * (size_t)m->L * (size_t)m->L));
+    m->cnt_u_kind = kind;
+    if (rsg_gram_counts_i8(c, m, m->d_cnt_u, kind, errbuf) != RSG_OK) return RSG_FAIL;
+  }
+  return rsg_raf(c, m, rafs, allow, errbuf);
+}
+
+extern "C" int rsg_corr_CalculateCOVCorrected(rsg_ctx *c, rsg_msa *m, int actype, int shiftnonneg, char *errbuf)
+{
+  HIPCHK(hipSetDevice(c->device));
+  if (actype != RSG_APC && actype != RSG_ASC) return rsg_fail(errbuf, "wrong correction type");
+  return rsg_correct(c, m, actype, shiftnonneg, errbuf);
+}
+
+extern "C" int rsg_corr_GetCOV(rsg_ctx *c, rsg_msa *m, double *COV, double *minCOV, double *maxCOV, char *errbuf)
+{
+  if (!m->have_cov) return rsg_fail(errbuf, "no covariation matrix");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (COV) HIPCHK(hipMemcpy(COV, m->d_cov, sizeof(double) * (size_t)m->L * (size_t)m->L, hipMemcpyDeviceToHost));
+  if (minCOV) *minCOV = m->minCOV;
+  if (maxCOV) *maxCOV = m->maxCOV;
+  return RSG_OK;
+}
+
+extern "C" int rsg_cov_Calculate(rsg_ctx *c, rsg_msa *m, int covtype, int covclass, int counts_mode, int prec,
+                                 const double *allowpair, int nseqthresh, int alenthresh, double tol,
+                                 double *minCOV, double *maxCOV, char *errbuf)
+{
+  int base = covtype - covtype % 3, corr = covtype % 3;
+  if (base != RSG_RAF && base != RSG_RAFS)
+    if (rsg_corr_Probs(c, m, counts_mode, tol, errbuf) != RSG_OK) return RSG_FAIL;
+  if (rsg_corr_CalculateCOV(c, m, base, covclass, allowpair, nseqthresh, alenthresh, prec, errbuf) != RSG_OK) return RSG_FAIL;
+  if (corr == 1 && rsg_corr_CalculateCOVCorrected(c, m, RSG_APC, 0, errbuf) != RSG_OK) return RSG_FAIL;
+  if (corr == 2 && rsg_corr_CalculateCOVCorrected(c, m, RSG_ASC, 0, errbuf) != RSG_OK) return RSG_FAIL;
+  if (minCOV) *minCOV = m->minCOV;
+  if (maxCOV) *maxCOV = m->maxCOV;
+  return RSG_OK;
+}
+
+extern "C" void rsg_free(void *p) { free(p); }

@@ -1,0 +1,593 @@
+// rsg_stats.hip -- probabilities, per-pair statistics, background corrections.
+//
+// Reference loops replaced (src/correlators.c, R-scape v2.6.11):
+//   corr_Marginals :1337-1375, mutual_naive_psi :1782-1817, corr_ValidateProbs :1499-1545,
+//   corr_Calculate{CHI,OMES,GT,MI,MIr,MIg}_{C16,C2,CWC} :49-874, CCF :984-1061,
+//   RAF/RAFS :876-982, corr_CalculateCOVCorrected :1063-1157.
+//
+// Pair records hold raw counts c[x][y] (unit or FP32 weighted).  The reference's
+// probabilities are pp = (c + 1e-10) / (nseff + 16e-10) with nseff = sum c, and
+// obs = nseff pp, exp = nseff pm_i[x] pm_j[y]; every statistic is rewritten in terms of
+// pp and q = pm_i pm_j (GT = 2 nseff sum pp ln(pp/q), CHI = nseff sum (pp-q)^2/q, ...),
+// which is the same quantity with the same cell conditions.  In FP64 mode integer counts
+// take ln(c + 1e-10) and ln(nseff + 16e-10) from per-alignment tables.
+#include "rsg_internal.h"
+#include <cmath>
+#include <algorithm>
+
+template <int KIND> __device__ __forceinline__ void load_rec(const void *cnt, int64_t rec, double *c)
+{
+  if (KIND == CNT_U16) {
+    const uint4 *p = (const uint4 *)((const uint16_t *)cnt + rec * 16);
+    uint4 a = p[0], b = p[1];
+    unsigned w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int k = 0; k < 8; k++) { c[2 * k] = (double)(w[k] & 0xffffu); c[2 * k + 1] = (double)(w[k] >> 16); }
+  } else if (KIND == CNT_U32) {
+    const uint4 *p = (const uint4 *)((const uint32_t *)cnt + rec * 16);
+#pragma unroll
+    for (int k = 0; k < 4; k++) { uint4 v = p[k]; c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w; }
+  } else {
+    const float4 *p = (const float4 *)((const float *)cnt + rec * 16);
+#pragma unroll
+    for (int k = 0; k < 4; k++) { float4 v = p[k]; c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w; }
+  }
+}
+
+// ---- pm: corr_Marginals.  One block per column i; fixed-order reduction (deterministic).
+template <int KIND>
+__global__ void __launch_bounds__(256) k_pm(const void *__restrict__ cnt, int64_t L, double tol,
+                                            double *__restrict__ pm, double *__restrict__ lpm, int *__restrict__ flag)
+{
+  __shared__ double red[4][256];
+  const int64_t i = blockIdx.x;
+  double acc[4] = {0., 0., 0., 0.};
+  for (int64_t j = threadIdx.x; j < L; j += 256) {
+    if (j == i) continue;
+    int64_t a = (i < j) ? i : j, b = (i < j) ? j : i;
+    double c[16];
+    load_rec<KIND>(cnt, a * L + b, c);
+    double ns = 0.;
+#pragma unroll
+    for (int k = 0; k < 16; k++) ns += c[k];
+    if (ns > 0) {
+      double inv = 1. / (ns + 16e-10);
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        double r = (i < j) ? (c[x * 4] + c[x * 4 + 1] + c[x * 4 + 2] + c[x * 4 + 3])
+                           : (c[x] + c[4 + x] + c[8 + x] + c[12 + x]);
+        acc[x] += (r + 4e-10) * inv;
+      }
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 4; x++) red[x][threadIdx.x] = acc[x];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int x = 0; x < 4; x++) red[x][threadIdx.x] += red[x][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double v[4] = {red[0][0], red[1][0], red[2][0], red[3][0]};
+    double s = v[0] + v[1] + v[2] + v[3];
+    double tot = 0.;
+    for (int x = 0; x < 4; x++) {
+      v[x] = (s != 0.) ? v[x] / s : 0.25;
+      if (!(v[x] >= 0. && v[x] <= 1.)) atomicOr(flag, 1);
+      tot += v[x];
+      pm[i * 4 + x] = v[x];
+      lpm[i * 4 + x] = (v[x] > 0.) ? log(v[x]) : -INFINITY;
+    }
+    if (fabs(tot - 1.) > tol) atomicOr(flag, 1);
+  }
+}
+
+// ---- ps: mutual_naive_psi, one block per column over the transposed raw codes.
+__global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, const double *__restrict__ w,
+                                            int64_t N, int64_t Npad, double *__restrict__ ps)
+{
+  __shared__ double red[5][256];
+  const int64_t i = blockIdx.x;
+  double acc[5] = {0., 0., 0., 0., 0.};
+  for (int64_t s = threadIdx.x; s < N; s += 256) {
+    uint8_t a = axT[i * Npad + s];
+    if (a < 5) acc[a] += w[s];
+  }
+  for (int x = 0; x < 5; x++) red[x][threadIdx.x] = acc[x];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) for (int x = 0; x < 5; x++) red[x][threadIdx.x] += red[x][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double v[5], s = 0.;
+    for (int x = 0; x < 5; x++) { v[x] = 1e-5 + red[x][0]; s += v[x]; }
+    for (int x = 0; x < 5; x++) ps[i * 5 + x] = v[x] / s;
+  }
+}
+
+// ln(k + 1e-10) and ln(k + 16e-10) for integer k in [0, N]
+__global__ void k_logtab(int64_t N, double *__restrict__ t)
+{
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= N) { t[k] = log((double)k + 1e-10); t[N + 1 + k] = log((double)k + 16e-10); }
+}
+
+int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
+{
+  const int64_t L = m->L;
+  if (!m->d_pm) HIPCHK(hipMalloc(&m->d_pm, sizeof(double) * 8 * (size_t)L));
+  if (!m->d_ps) HIPCHK(hipMalloc(&m->d_ps, sizeof(double) * 5 * (size_t)L));
+  HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+  int t; rsg_tstart(c, "marginals", &t);
+  if (m->cnt_kind == CNT_U16)      hipLaunchKernelGGL(k_pm<CNT_U16>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, tol, m->d_pm, m->d_pm + 4 * L, c->d_flag);
+  else if (m->cnt_kind == CNT_U32) hipLaunchKernelGGL(k_pm<CNT_U32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, tol, m->d_pm, m->d_pm + 4 * L, c->d_flag);
+  else                             hipLaunchKernelGGL(k_pm<CNT_F32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, tol, m->d_pm, m->d_pm + 4 * L, c->d_flag);
+  rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  rsg_tstart(c, "ps", &t);
+  hipLaunchKernelGGL(k_ps, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_axT, m->d_w, m->N, m->Npad, m->d_ps);
+  rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  if (!m->d_lgc) {
+    HIPCHK(hipMalloc(&m->d_lgc, sizeof(double) * 2 * (size_t)(m->N + 1)));
+    hipLaunchKernelGGL(k_logtab, dim3((unsigned)((m->N + 256) / 256)), dim3(256), 0, c->stream, m->N, m->d_lgc);
+    HIPCHK(hipGetLastError());
+  }
+  return RSG_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// per-pair statistics
+struct StatArgs {
+  const void *cnt; const double *pm; const double *lpm; const double *lgc;
+  int64_t L, N; double wsum; double *cov; double *part;  // part: 2 per block (min, max)
+  int type, cls; unsigned allow;                           // allow: 16-bit mask of allowed cells
+  double meanp[4];                                         // CCF
+};
+
+template <typename T> __device__ __forceinline__ T tlog(T v);
+template <> __device__ __forceinline__ double tlog<double>(double v) { return log(v); }
+template <> __device__ __forceinline__ float tlog<float>(float v) { return __logf(v); }
+
+// One pair. c: counts, pmi/pmj: marginals.  Returns the statistic (see the file header).
+template <typename T, bool TAB>
+__device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int64_t j)
+{
+  T c[16], ns = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) { c[k] = (T)cd[k]; ns += c[k]; }
+  double nsd = 0.;
+#pragma unroll
+  for (int k = 0; k < 16; k++) nsd += cd[k];
+  T pmi[4], pmj[4], lpi[4], lpj[4];
+#pragma unroll
+  for (int x = 0; x < 4; x++) {
+    pmi[x] = (T)A.pm[i * 4 + x]; pmj[x] = (T)A.pm[j * 4 + x];
+    lpi[x] = (T)A.lpm[i * 4 + x]; lpj[x] = (T)A.lpm[j * 4 + x];
+  }
+  const T den = ns + (T)16e-10;
+  T lden;
+  if (TAB) lden = (T)A.lgc[A.N + 1 + (int64_t)nsd]; else lden = tlog<T>(den);
+  T pp[16], lpp[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    pp[k] = (c[k] + (T)1e-10) / den;
+    if (TAB) lpp[k] = (T)A.lgc[(int64_t)cd[k]] - lden; else lpp[k] = tlog<T>(c[k] + (T)1e-10) - lden;
+  }
+  const int type = A.type;
+  if (type == RSG_CCF) {
+    T sa = 0, sb = 0;
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      T a = ns * pmi[x] - (T)A.meanp[x], b = ns * pmj[x] - (T)A.meanp[x];
+      sa += a * a; sb += b * b;
+    }
+    return (double)sqrt((double)(sa * sb));
+  }
+  if (A.cls == RSG_C2) {
+    T pw = 0, pn = 0, qw = 0, qn = 0;
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        if ((A.allow >> (x * 4 + y)) & 1u) { pw += pp[x * 4 + y]; qw += pmi[x] * pmj[y]; }
+        else                               { pn += pp[x * 4 + y]; qn += pmi[x] * pmj[y]; }
+      }
+    T v = 0, H = 0;
+    switch (type) {
+    case RSG_CHI:
+      if (ns > 0) { if (qw > 0) v += ns * (pw - qw) * (pw - qw) / qw; if (qn > 0) v += ns * (pn - qn) * (pn - qn) / qn; }
+      break;
+    case RSG_OMES:
+      if (ns > 0) { if (qw > 0) v += ns * (pw - qw) * (pw - qw); if (qn > 0) v += ns * (pn - qn) * (pn - qn); }
+      break;
+    case RSG_GT:
+      if (ns > 0) {
+        if (qw > 0 && pw > 0) v += pw * (tlog<T>(pw) - tlog<T>(qw));
+        if (qn > 0 && pn > 0) v += pn * (tlog<T>(pn) - tlog<T>(qn));
+        v *= 2 * ns;
+      }
+      break;
+    case RSG_MI: case RSG_MIg:
+      if (pw > 0) v += pw * (tlog<T>(pw) - tlog<T>(qw));
+      if (pn > 0) v += pn * (tlog<T>(pn) - tlog<T>(qn));
+      if (type == RSG_MIg && ns > 0) v -= (T)((A.wsum - nsd) / nsd);
+      break;
+    case RSG_MIr:
+      if (pw > 0) H -= pw * tlog<T>(pw);
+      if (pn > 0) H -= pn * tlog<T>(pn);
+      if (pw > 0 && qw > 0) v += pw * (tlog<T>(pw) - tlog<T>(qw));
+      if (pn > 0 && qn > 0) v += pn * (tlog<T>(pn) - tlog<T>(qn));
+      v = (H > (T)1e-2) ? v / H : 0;
+      break;
+    }
+    return (double)v;
+  }
+  // C16 / CWC
+  T s = 0, H = 0;
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const int k = x * 4 + y;
+      const T q = pmi[x] * pmj[y];
+      switch (type) {
+      case RSG_CHI:  if (q > 0) s += (pp[k] - q) * (pp[k] - q) / q; break;
+      case RSG_OMES: if (q > 0) s += (pp[k] - q) * (pp[k] - q); break;
+      case RSG_GT:
+        if (A.cls == RSG_CWC && !((A.allow >> k) & 1u)) break;
+        if (q > 0) s += pp[k] * (lpp[k] - lpi[x] - lpj[y]);
+        break;
+      case RSG_MI: case RSG_MIg: case RSG_MIr:
+        if (pmi[x] > 0 && pmj[y] > 0) s += pp[k] * (lpp[k] - lpi[x] - lpj[y]);
+        if (type == RSG_MIr) H -= pp[k] * lpp[k];
+        break;
+      }
+    }
+  switch (type) {
+  case RSG_CHI: case RSG_OMES: return (ns > 0) ? (double)(ns * s) : 0.;
+  case RSG_GT:   return (ns > 0) ? 2. * (double)ns * (double)s : 0.;
+  case RSG_MI:   return (double)s;
+  case RSG_MIg:  return (double)s - ((nsd > 0) ? (A.wsum - nsd) / nsd : 0.);
+  case RSG_MIr:  return (H > (T)1e-2) ? (double)(s / H) : 0.;
+  }
+  return 0.;
+}
+
+// 32 x 32 pair tile per block; COV written to both triangles through an LDS transpose.
+template <typename T, int KIND, bool TAB>
+__global__ void __launch_bounds__(256) k_stat(StatArgs A, int nb)
+{
+  __shared__ double tile[32][33];
+  __shared__ double rmin[256], rmax[256];
+  const int bi = blockIdx.x % nb, bj = blockIdx.x / nb;
+  const int tid = threadIdx.x;
+  double lmin = INFINITY, lmax = -INFINITY;
+  if (bi <= bj) {
+    const int jj = tid & 31;
+    for (int k = 0; k < 4; k++) {
+      const int ii = (tid >> 5) + 8 * k;
+      const int64_t i = (int64_t)bi * 32 + ii, j = (int64_t)bj * 32 + jj;
+      double v = -INFINITY;
+      if (i < j && j < A.L) {
+        double cd[16];
+        load_rec<KIND>(A.cnt, i * A.L + j, cd);
+        v = pair_stat<T, TAB>(A, cd, i, j);
+        lmin = fmin(lmin, v); lmax = fmax(lmax, v);
+        A.cov[i * A.L + j] = v;
+      } else if (i == j && i < A.L) {
+        A.cov[i * A.L + j] = -INFINITY;
+      }
+      tile[ii][jj] = v;
+    }
+    __syncthreads();
+    const int ii = tid & 31;
+    for (int k = 0; k < 4; k++) {
+      const int jj2 = (tid >> 5) + 8 * k;
+      const int64_t i = (int64_t)bi * 32 + ii, j = (int64_t)bj * 32 + jj2;
+      if (i < j && j < A.L) A.cov[j * A.L + i] = tile[ii][jj2];
+    }
+  }
+  rmin[tid] = lmin; rmax[tid] = lmax;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) { rmin[tid] = fmin(rmin[tid], rmin[tid + s]); rmax[tid] = fmax(rmax[tid], rmax[tid + s]); }
+    __syncthreads();
+  }
+  if (tid == 0) { A.part[2 * blockIdx.x] = rmin[0]; A.part[2 * blockIdx.x + 1] = rmax[0]; }
+}
+
+__global__ void k_reduce_minmax(const double *__restrict__ part, int n, double *__restrict__ out)
+{
+  __shared__ double rmin[256], rmax[256];
+  double a = INFINITY, b = -INFINITY;
+  for (int k = threadIdx.x; k < n; k += 256) { a = fmin(a, part[2 * k]); b = fmax(b, part[2 * k + 1]); }
+  rmin[threadIdx.x] = a; rmax[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) { rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + s]); rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + s]); }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[0] = rmin[0]; out[1] = rmax[0]; }
+}
+
+static int ensure_red(rsg_ctx *c, size_t n, char *errbuf)
+{
+  if (c->red_cap >= n) return RSG_OK;
+  if (c->d_red) hipFree(c->d_red);
+  c->d_red = nullptr;
+  HIPCHK(hipMalloc(&c->d_red, sizeof(double) * n));
+  c->red_cap = n;
+  return RSG_OK;
+}
+
+static int fetch_minmax(rsg_ctx *c, rsg_msa *m, int nparts, char *errbuf)
+{
+  hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(256), 0, c->stream, c->d_red, nparts, c->d_red + 2 * (size_t)nparts);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_red + 2 * (size_t)nparts, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  m->minCOV = c->h_pinned[0]; m->maxCOV = c->h_pinned[1];
+  return RSG_OK;
+}
+
+// CCF global meanp (src/correlators.c:1027-1036): sum over i<j of nseff_ij pm_i[x], normalised.
+template <int KIND>
+__global__ void k_ccf_rows(const void *__restrict__ cnt, int64_t L, const double *__restrict__ pm, double *__restrict__ rows)
+{
+  __shared__ double red[4][256];
+  const int64_t i = blockIdx.x;
+  double ns_sum = 0.;
+  for (int64_t j = i + 1 + threadIdx.x; j < L; j += 256) {
+    double cd[16]; load_rec<KIND>(cnt, i * L + j, cd);
+    for (int k = 0; k < 16; k++) ns_sum += cd[k];
+  }
+  for (int x = 0; x < 4; x++) red[x][threadIdx.x] = ns_sum * pm[i * 4 + x];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) for (int x = 0; x < 4; x++) red[x][threadIdx.x] += red[x][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) for (int x = 0; x < 4; x++) rows[i * 4 + x] = red[x][0];
+}
+
+int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, int prec, char *errbuf)
+{
+  const int64_t L = m->L;
+  if (!m->d_cov) HIPCHK(hipMalloc(&m->d_cov, sizeof(double) * (size_t)(L * L)));
+  const int nb = (int)((L + 31) / 32);
+  const int nblk = nb * nb;
+  if (ensure_red(c, 2 * (size_t)nblk + 8 + 4 * (size_t)L, errbuf) != RSG_OK) return RSG_FAIL;
+  StatArgs A;
+  A.cnt = m->d_cnt; A.pm = m->d_pm; A.lpm = m->d_pm + 4 * L; A.lgc = m->d_lgc;
+  A.L = L; A.N = m->N; A.wsum = m->wsum; A.cov = m->d_cov; A.part = c->d_red;
+  A.type = base; A.cls = cls;
+  A.allow = 0;
+  for (int k = 0; k < 16; k++) if (allow16[k] > 0) A.allow |= 1u << k;
+  for (int x = 0; x < 4; x++) A.meanp[x] = 0.;
+  if (base == RSG_CCF) {
+    double *rows = c->d_red + 2 * (size_t)nblk + 8;
+    if (m->cnt_kind == CNT_U16)      hipLaunchKernelGGL(k_ccf_rows<CNT_U16>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, m->d_pm, rows);
+    else if (m->cnt_kind == CNT_U32) hipLaunchKernelGGL(k_ccf_rows<CNT_U32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, m->d_pm, rows);
+    else                             hipLaunchKernelGGL(k_ccf_rows<CNT_F32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, m->d_pm, rows);
+    std::vector<double> h(4 * (size_t)L);
+    HIPCHK(hipMemcpyAsync(h.data(), rows, sizeof(double) * 4 * (size_t)L, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double mp[4] = {0., 0., 0., 0.}, s = 0.;
+    for (int64_t i = 0; i < L; i++) for (int x = 0; x < 4; x++) mp[x] += h[i * 4 + x];
+    for (int x = 0; x < 4; x++) s += mp[x];
+    for (int x = 0; x < 4; x++) A.meanp[x] = (s != 0.) ? mp[x] / s : 0.25;
+  }
+  const bool tab = (m->cnt_kind != CNT_F32) && prec == RSG_STAT_FP64;
+  int t; rsg_tstart(c, "stat", &t);
+  dim3 grid((unsigned)nblk), blk(256);
+#define LAUNCH(T, K, TB) hipLaunchKernelGGL((k_stat<T, K, TB>), grid, blk, 0, c->stream, A, nb)
+  if (prec == RSG_STAT_FP64) {
+    if (m->cnt_kind == CNT_U16)      { if (tab) LAUNCH(double, CNT_U16, true); else LAUNCH(double, CNT_U16, false); }
+    else if (m->cnt_kind == CNT_U32) { if (tab) LAUNCH(double, CNT_U32, true); else LAUNCH(double, CNT_U32, false); }
+    else                             LAUNCH(double, CNT_F32, false);
+  } else {
+    if (m->cnt_kind == CNT_U16)      LAUNCH(float, CNT_U16, false);
+    else if (m->cnt_kind == CNT_U32) LAUNCH(float, CNT_U32, false);
+    else                             LAUNCH(float, CNT_F32, false);
+  }
+#undef LAUNCH
+  rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  if (fetch_minmax(c, m, nblk, errbuf) != RSG_OK) return RSG_FAIL;
+  m->have_cov = 1;
+  return RSG_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// RAF from unit-weight counts (exact integer algebra; see rsg_capi for the derivation),
+// RAFS as the 3-point anti-diagonal stencil (src/correlators.c:954-960).
+template <int KIND>
+__global__ void __launch_bounds__(256) k_raf(const void *__restrict__ cnt, int64_t L, int64_t N, unsigned allow,
+                                             double *__restrict__ cov)
+{
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= L * L) return;
+  int64_t i = idx / L, j = idx - i * L;
+  if (i == j) { cov[idx] = -INFINITY; return; }
+  int64_t a = (i < j) ? i : j, b = (i < j) ? j : i;
+  double cd[16]; load_rec<KIND>(cnt, a * L + b, cd);
+  double M = 0., mx[4] = {0, 0, 0, 0}, my[4] = {0, 0, 0, 0};
+  for (int x = 0; x < 4; x++)
+    for (int y = 0; y < 4; y++)
+      if ((allow >> (x * 4 + y)) & 1u) { double n = cd[x * 4 + y]; M += n; mx[x] += n; my[y] += n; }
+  double sx = 0., sy = 0.;
+  for (int k = 0; k < 4; k++) { sx += mx[k] * mx[k]; sy += my[k] * my[k]; }
+  double cij = 0.5 * (M * M - sx) + 0.5 * (M * M - sy);
+  cij /= (N > 1) ? (double)N * ((double)N - 1.0) : 1.0;
+  cij *= 2.0;
+  double qij = ((double)N - M) / (double)N;
+  cov[idx] = cij - qij;
+}
+
+__global__ void k_rafs(const double *__restrict__ B, int64_t L, double *__restrict__ cov)
+{
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= L * L) return;
+  int64_t r = idx / L, s = idx - r * L;
+  if (r == s) { cov[idx] = -INFINITY; return; }
+  int64_t i = (r < s) ? r : s, j = (r < s) ? s : r;
+  double v = 2.0 * B[i * L + j];
+  if (i > 0 && j < L - 1)              v += B[(i - 1) * L + (j + 1)];
+  if (j > 0 && i < L - 1 && i < j - 2) v += B[(i + 1) * L + (j - 1)];
+  cov[idx] = 0.25 * v;
+}
+
+template <int KIND>
+__global__ void k_minmax_upper(const double *__restrict__ cov, int64_t L, double *__restrict__ part)
+{
+  __shared__ double rmin[256], rmax[256];
+  double a = INFINITY, b = -INFINITY;
+  int64_t n = L * L;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * 256) {
+    int64_t i = idx / L, j = idx - i * L;
+    if (i < j) { double v = cov[idx]; a = fmin(a, v); b = fmax(b, v); }
+  }
+  rmin[threadIdx.x] = a; rmax[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) { rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + s]); rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + s]); }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rmin[0]; part[2 * blockIdx.x + 1] = rmax[0]; }
+}
+
+int rsg_minmax(rsg_ctx *c, rsg_msa *m, char *errbuf)
+{
+  const int nblk = 512;
+  if (ensure_red(c, 2 * nblk + 8, errbuf) != RSG_OK) return RSG_FAIL;
+  hipLaunchKernelGGL(k_minmax_upper<0>, dim3(nblk), dim3(256), 0, c->stream, m->d_cov, m->L, c->d_red);
+  HIPCHK(hipGetLastError());
+  return fetch_minmax(c, m, nblk, errbuf);
+}
+
+int rsg_raf(rsg_ctx *c, rsg_msa *m, int rafs, const double *allow16, char *errbuf)
+{
+  const int64_t L = m->L;
+  if (!m->d_cov) HIPCHK(hipMalloc(&m->d_cov, sizeof(double) * (size_t)(L * L)));
+  unsigned allow = 0;
+  for (int k = 0; k < 16; k++) if (allow16[k] > 0) allow |= 1u << k;
+  const void *cnt = m->d_cnt_u ? m->d_cnt_u : m->d_cnt;
+  int kind = m->d_cnt_u ? m->cnt_u_kind : m->cnt_kind;
+  if (kind == CNT_F32) return rsg_fail(errbuf, "RAF needs unit-weight counts");
+  double *dst = m->d_cov;
+  if (rafs) {
+    if (!m->d_tmp) HIPCHK(hipMalloc(&m->d_tmp, sizeof(double) * (size_t)(L * L)));
+    dst = m->d_tmp;
+  }
+  unsigned nb = (unsigned)((L * L + 255) / 256);
+  int t; rsg_tstart(c, "raf", &t);
+  if (kind == CNT_U16) hipLaunchKernelGGL(k_raf<CNT_U16>, dim3(nb), dim3(256), 0, c->stream, cnt, L, m->N, allow, dst);
+  else                 hipLaunchKernelGGL(k_raf<CNT_U32>, dim3(nb), dim3(256), 0, c->stream, cnt, L, m->N, allow, dst);
+  if (rafs) hipLaunchKernelGGL(k_rafs, dim3(nb), dim3(256), 0, c->stream, m->d_tmp, L, m->d_cov);
+  rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  if (rsg_minmax(c, m, errbuf) != RSG_OK) return RSG_FAIL;
+  m->have_cov = 1;
+  return RSG_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// corr_CalculateCOVCorrected: row means over j != i (FP64, fixed order), COVavg, then
+// the APC / ASC correction in place with a NaN flag and min / max.
+__global__ void __launch_bounds__(256) k_rowsum(const double *__restrict__ cov, int64_t L, double *__restrict__ rows)
+{
+  __shared__ double red[256];
+  const int64_t i = blockIdx.x;
+  double s = 0.;
+  for (int64_t j = threadIdx.x; j < L; j += 256) if (j != i) s += cov[i * L + j];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rows[L + i] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_rowmean(int64_t L, double *__restrict__ rows)
+{
+  __shared__ double red[256];
+  double s = 0.;
+  for (int64_t i = threadIdx.x; i < L; i += 256) {
+    double r = rows[L + i];
+    s += r;
+    rows[i] = (L > 1) ? r / ((double)L - 1.) : r;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rows[2 * L] = (L > 1) ? red[0] / ((double)L * ((double)L - 1.)) : red[0];
+}
+
+__global__ void __launch_bounds__(256) k_correct(double *__restrict__ cov, int64_t L, const double *__restrict__ rows,
+                                                 int actype, int *__restrict__ flag, double *__restrict__ part)
+{
+  __shared__ double rmin[256], rmax[256];
+  const double avg = rows[2 * L];
+  double a = INFINITY, b = -INFINITY;
+  const int64_t n = L * L;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * 256) {
+    int64_t i = idx / L, j = idx - i * L;
+    if (i == j) continue;
+    double v = cov[idx], xi = rows[i], xj = rows[j];
+    double r = (actype == RSG_APC) ? ((avg != 0.0) ? v - xi * xj / avg : 0.0) : v - (xi + xj - avg);
+    if (isnan(r)) atomicOr(flag, 1);
+    cov[idx] = r;
+    a = fmin(a, r); b = fmax(b, r);
+  }
+  rmin[threadIdx.x] = a; rmax[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) { rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + s]); rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + s]); }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rmin[0]; part[2 * blockIdx.x + 1] = rmax[0]; }
+}
+
+__global__ void k_shift(double *__restrict__ cov, int64_t L, const double *__restrict__ mm)
+{
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= L * L) return;
+  int64_t i = idx / L, j = idx - i * L;
+  if (i != j) cov[idx] -= mm[0];
+}
+
+int rsg_correct(rsg_ctx *c, rsg_msa *m, int actype, int shiftnonneg, char *errbuf)
+{
+  const int64_t L = m->L;
+  if (!m->have_cov) return rsg_fail(errbuf, "no covariation matrix to correct");
+  if (!m->d_rowx) HIPCHK(hipMalloc(&m->d_rowx, sizeof(double) * (2 * (size_t)L + 2)));
+  const int nblk = (int)std::min<int64_t>(1024, (L * L + 255) / 256);
+  if (ensure_red(c, 2 * (size_t)nblk + 8, errbuf) != RSG_OK) return RSG_FAIL;
+  HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+  int t; rsg_tstart(c, "correct", &t);
+  hipLaunchKernelGGL(k_rowsum, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cov, L, m->d_rowx);
+  hipLaunchKernelGGL(k_rowmean, dim3(1), dim3(256), 0, c->stream, L, m->d_rowx);
+  hipLaunchKernelGGL(k_correct, dim3(nblk), dim3(256), 0, c->stream, m->d_cov, L, m->d_rowx, actype, c->d_flag, c->d_red);
+  rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(256), 0, c->stream, c->d_red, nblk, c->d_red + 2 * (size_t)nblk);
+  if (shiftnonneg)
+    hipLaunchKernelGGL(k_shift, dim3((unsigned)((L * L + 255) / 256)), dim3(256), 0, c->stream, m->d_cov, L, c->d_red + 2 * (size_t)nblk);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_red + 2 * (size_t)nblk, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_pinned + 2, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int flag = *(int *)(c->h_pinned + 2);
+  double avg;
+  HIPCHK(hipMemcpy(&avg, m->d_rowx + 2 * L, sizeof(double), hipMemcpyDeviceToHost));
+  if (flag) return rsg_fail(errbuf, "bad covariation COVavg %f", avg);
+  m->minCOV = c->h_pinned[0]; m->maxCOV = c->h_pinned[1];
+  return RSG_OK;
+}

@@ -1,0 +1,232 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+Counts must be bit-exact; FP64 statistics within 1e-10 relative (with the floor
+|ref| >= 1e-3 * (maxCOV - minCOV) used near zero); FP32 statistics within the
+looser bound stated in each test.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import rscape_amd as R
+from rscape_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _msa(N, L, seed, gap=0.15, other=0.03):
+    rng = np.random.default_rng(seed)
+    return synth.random_msa(N, L, rng, gap=gap, other=other)
+
+
+def _upper(a, L):
+    a = np.asarray(a).reshape(L, L, -1) if np.asarray(a).size > L * L else np.asarray(a).reshape(L, L)
+    iu = np.triu_indices(L, 1)
+    return a[iu]
+
+
+def _relerr(got, ref):
+    got, ref = np.asarray(got, float), np.asarray(ref, float)
+    span = (ref.max() - ref.min()) if ref.size else 1.0
+    floor = max(1e-3 * span, 1e-300)
+    return float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), floor))) if ref.size else 0.0
+
+
+@pytest.mark.parametrize("N,L", [(1, 2), (7, 3), (64, 37), (65, 64), (200, 65), (333, 130), (129, 200)])
+def test_int8_counts_bit_exact(ctx, N, L):
+    print("counts case N=%d L=%d" % (N, L), flush=True)
+    ax = _msa(N, L, 1000 + N + L)
+    m = ctx.msa(ax)
+    m.probs(R.COUNTS_INT8)
+    got = m.counts().reshape(L, L, 16)
+    ref = oracle.pair_counts(ax).reshape(L, L, 16)
+    iu = np.triu_indices(L, 1)
+    bad = np.nonzero(np.any(got[iu] != ref[iu], axis=1))[0]
+    for k in bad[:4]:
+        print("mismatch pair (%d,%d)\n got %s\n ref %s" % (iu[0][k], iu[1][k], got[iu][k].astype(int), ref[iu][k]))
+    assert np.array_equal(got[iu], ref[iu].astype(float))
+
+
+@pytest.mark.parametrize("shape", range(1, 7))
+def test_gram_shapes_bit_exact(ctx, shape):
+    print("gram shape %d (%s)" % (shape, R.lib().rsg_gram_shape_name(shape).decode()), flush=True)
+    N, L = 300, 203
+    ax = _msa(N, L, 7 + shape)
+    ctx.set_gram_shape(shape)
+    try:
+        m = ctx.msa(ax)
+        m.probs(R.COUNTS_INT8)
+        got = m.counts().reshape(L, L, 16)
+    finally:
+        ctx.set_gram_shape(0)
+    ref = oracle.pair_counts(ax).reshape(L, L, 16)
+    iu = np.triu_indices(L, 1)
+    assert np.array_equal(got[iu], ref[iu].astype(float))
+
+
+def test_counts_u32_path(ctx):
+    """N >= 65536 switches the records to 32-bit counts."""
+    N, L = 70000, 9
+    print("u32 counts N=%d L=%d" % (N, L), flush=True)
+    ax = _msa(N, L, 5)
+    m = ctx.msa(ax)
+    m.probs(R.COUNTS_INT8)
+    got = m.counts().reshape(L, L, 16)
+    ref = oracle.pair_counts(ax).reshape(L, L, 16)
+    iu = np.triu_indices(L, 1)
+    assert np.array_equal(got[iu], ref[iu].astype(float))
+
+
+def test_probs_match_oracle(ctx):
+    N, L = 157, 71
+    ax = _msa(N, L, 11)
+    m = ctx.msa(ax)
+    m.probs(R.COUNTS_INT8)
+    g = m.get_probs()
+    r = oracle.probs(ax)
+    assert np.allclose(g["pm"], r["pm"], rtol=1e-12, atol=1e-15)
+    assert np.allclose(g["ps"], r["ps"], rtol=1e-12, atol=1e-15)
+    assert np.allclose(g["pp"], r["pp"], rtol=1e-12, atol=1e-18)
+    assert np.array_equal(g["nseff"], r["nseff"])
+    iu = np.triu_indices(L, 1)
+    assert np.allclose(g["ngap"].reshape(L, L)[iu], r["ngap"].reshape(L, L)[iu], rtol=0, atol=1e-9)
+
+
+STATS = [("CHI", oracle.CHI), ("GT", oracle.GT), ("MI", oracle.MI), ("MIr", oracle.MIr),
+         ("MIg", oracle.MIg), ("OMES", oracle.OMES), ("CCF", oracle.CCF)]
+
+
+@pytest.mark.parametrize("name,otype", STATS)
+@pytest.mark.parametrize("cls", [R.C16, R.C2])
+def test_statistics_fp64(ctx, name, otype, cls):
+    N, L = 90, 57
+    print("stat %s class %d" % (name, cls), flush=True)
+    ax = _msa(N, L, 21)
+    m = ctx.msa(ax)
+    m.probs(R.COUNTS_INT8)
+    m.cov(name, cls, prec=R.STAT_FP64)
+    got, mn, mx = m.get_cov()
+    pr = oracle.probs(ax)
+    ref, mm = oracle.cov(pr, otype, cls)
+    assert _relerr(_upper(got, L), _upper(ref, L)) < 1e-10
+    assert np.allclose([mn, mx], mm, rtol=1e-10, atol=1e-12)
+    assert np.all(np.isneginf(np.diag(got)))
+
+
+def test_gt_cwc_and_cselect(ctx):
+    ax = _msa(6, 40, 3)            # nseq <= 8 -> CSELECT resolves to C2
+    m = ctx.msa(ax)
+    m.probs(R.COUNTS_INT8)
+    pr = oracle.probs(ax)
+    for cls in (R.CWC, R.CSELECT):
+        m.cov("GT", cls)
+        got, _, _ = m.get_cov()
+        ref, _ = oracle.cov(pr, oracle.GT, cls)
+        assert _relerr(_upper(got, 40), _upper(ref, 40)) < 1e-10
+
+
+@pytest.mark.parametrize("actype", [R.APC, R.ASC])
+def test_corrections(ctx, actype):
+    N, L = 120, 77
+    ax = _msa(N, L, 31)
+    m = ctx.msa(ax)
+    m.probs(R.COUNTS_INT8)
+    m.cov("GT", R.C16)
+    m.correct(actype)
+    got, mn, mx = m.get_cov()
+    pr = oracle.probs(ax)
+    ref, _ = oracle.cov(pr, oracle.GT, R.C16)
+    ref2, mm = oracle.corrected(ref, L, actype)
+    assert _relerr(_upper(got, L), _upper(ref2, L)) < 1e-10
+    off = ~np.eye(L, dtype=bool)
+    assert _relerr(got[off], ref2.reshape(L, L)[off]) < 1e-10
+    assert np.allclose([mn, mx], mm, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("rafs", [False, True])
+def test_raf_rafs(ctx, rafs):
+    N, L = 40, 33
+    ax = _msa(N, L, 41)
+    m = ctx.msa(ax)
+    m.cov("RAFS" if rafs else "RAF")
+    got, mn, mx = m.get_cov()
+    ref, mm = oracle.raf(ax, rafs)
+    off = ~np.eye(L, dtype=bool)
+    assert np.allclose(got[off], ref.reshape(L, L)[off], rtol=1e-12, atol=1e-14)
+    assert np.allclose([mn, mx], mm, rtol=1e-12, atol=1e-14)
+
+
+def test_fp32_statistics_error_bound(ctx):
+    """FP32 statistics: error against FP64 stated (bound 1e-3 with the near-zero floor)."""
+    N, L = 400, 60
+    ax = _msa(N, L, 51)
+    m = ctx.msa(ax)
+    m.probs(R.COUNTS_INT8)
+    pr = oracle.probs(ax)
+    for name, ot in STATS[:6]:
+        m.cov(name, R.C16, prec=R.STAT_FP32)
+        got, _, _ = m.get_cov()
+        ref, _ = oracle.cov(pr, ot, R.C16)
+        err = _relerr(_upper(got, L), _upper(ref, L))
+        print("fp32 %s max rel err (floored) %.3e" % (name, err))
+        assert err < 1e-3
+
+
+@pytest.mark.parametrize("mode,bound", [(R.COUNTS_BF16, 2e-5), (R.COUNTS_BF16X3, 1e-6)])
+def test_weighted_counts_split_bf16(ctx, mode, bound):
+    N, L = 500, 48
+    rng = np.random.default_rng(61)
+    ax = synth.random_msa(N, L, rng, gap=0.1)
+    w = rng.uniform(0.05, 3.0, size=N)
+    m = ctx.msa(ax, w)
+    m.probs(mode)
+    got = m.counts().reshape(L, L, 16)
+    pr = oracle.probs(ax, w)
+    iu = np.triu_indices(L, 1)
+    # FP64 weighted counts from the oracle probabilities
+    pp = pr["pp"].reshape(L, L, 16)[iu]
+    ns = pr["nseff"].reshape(L, L)[iu]
+    ref = pp * (ns[:, None] + 16e-10) - 1e-10
+    g = got[iu]
+    rel = np.max(np.abs(g - ref) / np.maximum(np.abs(ref), 1.0))
+    print("split-bf16 mode %d: max rel count error %.3e" % (mode, rel))
+    assert rel < bound
+    m.cov("GT", R.C16)
+    gt, _, _ = m.get_cov()
+    ref_gt, _ = oracle.cov(pr, oracle.GT, R.C16)
+    e = _relerr(_upper(gt, L), _upper(ref_gt, L))
+    print("split-bf16 mode %d: GT max rel err (floored) %.3e" % (mode, e))
+    assert e < 50 * bound
+
+
+def test_full_pipeline_calculate(ctx):
+    N, L = 200, 120
+    d = synth.config(0, seed=3)
+    ax = d["ax"]
+    m = ctx.msa(ax)
+    mn, mx = m.calculate("GTp")
+    got, _, _ = m.get_cov()
+    pr = oracle.probs(ax)
+    ref, _ = oracle.cov(pr, oracle.GT, R.CSELECT)
+    ref2, mm = oracle.corrected(ref, L, R.APC)
+    assert _relerr(_upper(got, L), _upper(ref2, L)) < 1e-10
+    assert np.allclose([mn, mx], mm, rtol=1e-10)
+
+
+def test_tutorial_golden_gtp(ctx):
+    """End-to-end against the published tutorial scores (GSC weights, gap filter, GT, APC)."""
+    import json, os
+    from rscape_amd import stockholm
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    names, ax, _ = stockholm.read(os.path.join(here, "updated_Arisong.sto"))
+    gold = json.load(open(os.path.join(here, "arisong_gtp.json")))
+    use = oracle.gapcol_filter(ax, np.ones(len(names)), 0.75)
+    cols = np.nonzero(use)[0]
+    ax2 = oracle.canonicalize(ax[:, cols])
+    w = oracle.weights_gsc(ax2)
+    m = ctx.msa(ax2, w)
+    m.calculate("GTp", counts=R.COUNTS_BF16X3)
+    got, _, mx = m.get_cov()
+    pos = {c + 1: k for k, c in enumerate(cols)}
+    for i, j, sc in gold["pairs"]:
+        assert abs(got[pos[i], pos[j]] - sc) < 0.05, (i, j, got[pos[i], pos[j]], sc)
