@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Benchmark of the pairwise-covariation hot path on one MI355X.
+
+One step = one complete scoring of the resident alignment as R-scape's cov_Calculate
+does it for GTp (src/covariation.c:63-306): pair counts (int8 one-hot MFMA Gram),
+probabilities and marginals (FP64), the GT statistic (FP64), the APC correction and the
+score extremes.  Workload: synthetic rRNA-like alignment N=5000 sequences x L=1500
+columns (400 planted nested pairs, 10% gaps in runs).  Metric: column-pair x sequence
+updates per second, updates = N * L(L-1)/2 per step.
+
+Prints ONE JSON line.  Extra fields report the weighted (split-BF16) variant, the FP32
+statistic variant, a per-kernel time breakdown, the roofline of the dominant kernel
+(the int8 Gram) and the CPU oracle timed on a column window of the same alignment.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "r-scape_amd"))
+
+import numpy as np  # noqa: E402
+
+import rscape_amd as R  # noqa: E402
+from rscape_amd import synth  # noqa: E402
+
+PEAK_INT8_TOPS = 5000.0      # MI355X dense int8 MFMA, TOP/s
+PEAK_BF16_TFLOPS = 2500.0    # MI355X dense bf16 MFMA, TFLOP/s
+PEAK_HBM_GBS = 8000.0
+
+
+def kernel_avgs(timings):
+    acc = {}
+    for name, ms in timings:
+        acc.setdefault(name, []).append(ms)
+    return {k: (float(np.mean(v)), len(v)) for k, v in acc.items()}
+
+
+def run_steps(ctx, m, k, **kw):
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        m.calculate(**kw)
+    t1 = time.perf_counter()
+    tim = ctx.timings()
+    ctx.set_timing(False)
+    return t1 - t0, kernel_avgs(tim)
+
+
+def cpu_baseline(ax, Lwin):
+    """Oracle (serial FP64 restatement of the reference loops) on the first Lwin columns."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        import oracle
+        oracle.lib()
+    except Exception as e:  # the oracle library is optional on the GPU box
+        return {"value": None, "error": str(e)}
+    sub = np.ascontiguousarray(ax[:, :Lwin])
+    N = sub.shape[0]
+    t0 = time.perf_counter()
+    pr = oracle.probs(sub)
+    cv, _ = oracle.cov(pr, oracle.GT, oracle.CSELECT)
+    oracle.corrected(cv, Lwin, oracle.APC)
+    dt = time.perf_counter() - t0
+    upd = N * Lwin * (Lwin - 1) / 2
+    return {"value": upd / dt, "unit": "column-pair*seq updates/s", "cores": 1, "kind": "port",
+            "sample": "GT+APC (counts, probabilities, marginals, GT, APC) on columns 0..%d of the same "
+                      "N=%d alignment, %.1f s" % (Lwin - 1, N, dt)}
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "gram_i8_c1_traffic.json")
+    if os.path.exists(p):
+        try:
+            return json.load(open(p)).get("traffic_bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--N", type=int, default=5000)
+    ap.add_argument("--L", type=int, default=1500)
+    ap.add_argument("--cpu-window", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the design for sharding is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    d = synth.config(1, seed=42, N=args.N, L=args.L)
+    ax = d["ax"]
+    N, L = ax.shape
+    upd = N * L * (L - 1) / 2
+
+    ctx = R.Context(0)
+    m = ctx.msa(ax)
+    kw = dict(covtype="GTp", counts=R.COUNTS_INT8, prec=R.STAT_FP64)
+    for _ in range(args.warmup):
+        m.calculate(**kw)
+    dt, kav = run_steps(ctx, m, args.steps, **kw)
+    value = upd * args.steps / dt
+    g_ms = kav.get("gram_i8", (float("nan"), 0))[0]
+    gram_ops = 2.0 * 16.0 * upd                      # 16 int8 MACs per update (4x4 canonical table)
+    achieved = gram_ops / (g_ms * 1e-3) / 1e12
+    out = {
+        "metric": "column-pair*seq updates/s (GT+APC)",
+        "value": value,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic",
+        "config": {"workload": "N=%d x L=%d synthetic rRNA-like MSA, GTp (unit weights, int8 counts, FP64 "
+                               "statistics), one full scoring per step" % (N, L),
+                   "N": N, "L": L, "covtype": "GTp", "counts": "int8 one-hot Gram", "statistic": "FP64"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
+                     "frac": achieved / PEAK_INT8_TOPS, "traffic": load_traffic(),
+                     "kernel": "k_gram_i8", "kernel_ms": g_ms,
+                     "algorithmic_ops_per_launch": gram_ops},
+        "kernels_ms": {k: v[0] for k, v in kav.items()},
+    }
+    if not args.no_extra:
+        extra = {}
+        m.close()
+        rng = np.random.default_rng(1)
+        w = rng.uniform(0.2, 2.0, size=N)
+        mw = ctx.msa(ax, w)
+        kw2 = dict(covtype="GTp", counts=R.COUNTS_BF16, prec=R.STAT_FP64)
+        mw.calculate(**kw2)
+        dtw, kw_av = run_steps(ctx, mw, max(3, args.steps // 4), **kw2)
+        b_ms = kw_av.get("gram_bf16", (float("nan"), 0))[0]
+        extra["weighted_bf16_hilo"] = {"value": upd * max(3, args.steps // 4) / dtw, "unit": "updates/s",
+                                       "gram_bf16_ms": b_ms,
+                                       "gram_bf16_tflops": 2 * 2 * 16 * upd / (b_ms * 1e-3) / 1e12,
+                                       "peak_tflops": PEAK_BF16_TFLOPS}
+        mw.close()
+        m = ctx.msa(ax)
+        kw3 = dict(covtype="GTp", counts=R.COUNTS_INT8, prec=R.STAT_FP32)
+        m.calculate(**kw3)
+        dt3, _ = run_steps(ctx, m, max(3, args.steps // 4), **kw3)
+        extra["fp32_statistic"] = {"value": upd * max(3, args.steps // 4) / dt3, "unit": "updates/s"}
+        out["extra"] = extra
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ax, min(args.cpu_window, L))
+    m.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
