@@ -62,7 +62,7 @@ extern "C" int rsg_ctx_set_gram_shape(rsg_ctx *c, int shape) { c->gram_shape = s
 
 static void free_msa(rsg_msa *m)
 {
-  void *ptrs[] = { m->d_ax, m->d_axT, m->d_xt, m->d_w, m->d_xw[0], m->d_xw[1], m->d_xw[2], m->d_x1,
+  void *ptrs[] = { m->d_ax, m->d_axT, m->d_xt, m->d_w, m->d_wsplit,
                    m->d_cnt, m->d_cnt_u, m->d_pm, m->d_ps, m->d_lgc, m->d_cov, m->d_tmp, m->d_rowx };
   for (void *p : ptrs) if (p) hipFree(p);
   delete m;
@@ -119,8 +119,8 @@ extern "C" void rsg_msa_destroy(rsg_ctx *c, rsg_msa *m)
 
 static int alloc_counts(rsg_msa *m, void **dst, int kind, char *errbuf)
 {
-  size_t esz = (kind == CNT_U16) ? 2 : 4;
-  size_t bytes = esz * 16 * (size_t)m->L * (size_t)m->L;
+  (void)kind;                                       // both kinds use 4-byte cells
+  size_t bytes = 4 * 16 * (size_t)m->L * (size_t)m->L;
   if (*dst && m->cnt_bytes >= bytes && dst == &m->d_cnt) return RSG_OK;
   if (*dst) hipFree(*dst);
   *dst = nullptr;
@@ -135,14 +135,11 @@ extern "C" int rsg_corr_Probs(rsg_ctx *c, rsg_msa *m, int mode, double tol, char
   m->have_probs = m->have_cov = 0;
   if (mode == RSG_COUNTS_INT8) {
     if (m->weighted) return rsg_fail(errbuf, "int8 counts need unit weights (alignment has weights)");
-    int kind = (m->N < 65536) ? CNT_U16 : CNT_U32;
-    if (m->cnt_kind != kind) { if (m->d_cnt) hipFree(m->d_cnt); m->d_cnt = nullptr; m->cnt_bytes = 0; }
-    if (alloc_counts(m, &m->d_cnt, kind, errbuf) != RSG_OK) return RSG_FAIL;
-    m->cnt_kind = kind;
-    if (rsg_gram_counts_i8(c, m, m->d_cnt, kind, errbuf) != RSG_OK) return RSG_FAIL;
+    if (alloc_counts(m, &m->d_cnt, CNT_I8G, errbuf) != RSG_OK) return RSG_FAIL;
+    m->cnt_kind = CNT_I8G;
+    if (rsg_gram_counts_i8(c, m, m->d_cnt, CNT_I8G, errbuf) != RSG_OK) return RSG_FAIL;
   } else if (mode == RSG_COUNTS_BF16 || mode == RSG_COUNTS_BF16X3) {
-    if (m->cnt_kind != CNT_F32) { if (m->d_cnt) hipFree(m->d_cnt); m->d_cnt = nullptr; m->cnt_bytes = 0; }
-    if (alloc_counts(m, &m->d_cnt, CNT_U32, errbuf) != RSG_OK) return RSG_FAIL;   // 4-byte records
+    if (alloc_counts(m, &m->d_cnt, CNT_F32, errbuf) != RSG_OK) return RSG_FAIL;
     m->cnt_kind = CNT_F32;
     if (rsg_gram_counts_bf16(c, m, mode == RSG_COUNTS_BF16X3 ? 3 : 2, (float *)m->d_cnt, errbuf) != RSG_OK) return RSG_FAIL;
   } else return rsg_fail(errbuf, "unknown counts mode %d", mode);
@@ -161,19 +158,18 @@ extern "C" int rsg_corr_GetCounts(rsg_ctx *c, rsg_msa *m, double *cnt, char *err
   if (!m->d_cnt) return rsg_fail(errbuf, "no counts");
   const int64_t L = m->L;
   const size_t n = 16 * (size_t)L * (size_t)L;
-  size_t esz = (m->cnt_kind == CNT_U16) ? 2 : 4;
-  std::vector<uint8_t> h(n * esz);
+  std::vector<uint8_t> h(n * 4);
   HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpy(h.data(), m->d_cnt, n * esz, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(h.data(), m->d_cnt, n * 4, hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < L; i++)
     for (int64_t j = 0; j < L; j++)
       for (int k = 0; k < 16; k++) {
         size_t o = ((size_t)i * L + j) * 16 + k;
+        size_t r = (size_t)rsg_rec(i, j, L) * 16 + k;
         double v = 0.;
         if (i < j) {
-          if (m->cnt_kind == CNT_U16)      v = ((uint16_t *)h.data())[o];
-          else if (m->cnt_kind == CNT_U32) v = ((uint32_t *)h.data())[o];
-          else                             v = ((float *)h.data())[o];
+          if (m->cnt_kind == CNT_I8G) v = (double)(((uint32_t *)h.data())[r] >> rsg_cell_shift(k));
+          else                        v = ((float *)h.data())[r];
         }
         cnt[o] = v;
       }
@@ -242,14 +238,11 @@ extern "C" int rsg_corr_CalculateRAF(rsg_ctx *c, rsg_msa *m, int rafs, const dou
   (void)prec;
   HIPCHK(hipSetDevice(c->device));
   const double *allow = allowpair ? allowpair : kAllowWCGU;
-  // RAF is defined on unweighted symbols (src/correlators.c:892-921): make sure unit counts exist.
-  if (m->cnt_kind == CNT_F32 || !m->d_cnt) {
-    int kind = (m->N < 65536) ? CNT_U16 : CNT_U32;
-    if (m->cnt_u_kind != kind) { if (m->d_cnt_u) hipFree(m->d_cnt_u); m->d_cnt_u = nullptr; }
-    if (!m->d_cnt_u) HIPCHK(hipMalloc(&m->d_cnt_u, (kind == CNT_U16 ? 2 : 4) * 16 * (size_t)m->L * (size_t)m->L));
-    m->cnt_u_kind = kind;
-    if (rsg_gram_counts_i8(c, m, m->d_cnt_u, kind, errbuf) != RSG_OK) return RSG_FAIL;
-  }
+  // RAF is defined on unweighted symbols (src/correlators.c:892-921) and, like the reference,
+  // does not go through corr_Probs: form the unit-weight counts here.
+  if (!m->d_cnt_u) HIPCHK(hipMalloc(&m->d_cnt_u, 4 * 16 * (size_t)m->L * (size_t)m->L));
+  m->cnt_u_kind = CNT_I8G;
+  if (rsg_gram_counts_i8(c, m, m->d_cnt_u, CNT_I8G, errbuf) != RSG_OK) return RSG_FAIL;
   return rsg_raf(c, m, rafs, allow, errbuf);
 }
 

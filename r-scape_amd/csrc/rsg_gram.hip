@@ -3,21 +3,30 @@
 // Replaces the O(L^2 N) gather loop of mutual_naive_ppij (src/correlators.c:1695-1780,
 // driven by corr_NaivePP :1300-1317).  With GAPASCHAR == 0 (src/rscape_config.h:48) the
 // table of a column pair is 4 x 4: c[x][y] = sum_s w_s [a_si == x][a_sj == y] over
-// canonical symbols.  Writing X for the N x 4L one-hot matrix, all tables at once are
-// the Gram C = X^T diag(w) X; only the upper-triangular column blocks are formed.
+// canonical symbols.  Writing X for the N x 4L one-hot matrix, all tables at once are the
+// Gram C = X^T diag(w) X; only upper-triangular column blocks are formed.
+//
+// Staging: one byte per (column, sequence) holding 1 << symbol (0 for gap / N), columns
+// K-contiguous ("xt", Lpad x Npad).  A wave owns 32 i-columns x 32 j-columns and keeps
+// all 16 symbol cells as 16 MFMA accumulators ("symbol-major" tiles): each lane reads
+// the 16 staged bytes of its column once per 32 sequences and derives the four symbol
+// operands in registers, so LDS traffic is one byte per (column, sequence) per wave.
 //
 // Unit weights: int8 MFMA (v_mfma_i32_32x32x32_i8), int32 accumulation, bit-exact.
-//   The alignment is staged as one byte per (column, sequence) holding 1 << symbol
-//   (0 for gap / N).  A lane that feeds feature (column, symbol s) masks the byte with
-//   0x01 << s, so the operand byte is 2^s or 0; the product of the two operands of a
-//   pair cell (x, y) is 2^(x+y), which the epilogue shifts back out.  One staged byte
-//   therefore feeds all four symbol rows, and LDS / L2 traffic is 1 byte per
-//   (column, sequence) instead of 4.
-// Weights: split-BF16 (w = hi + lo [+ lo2]) MFMA (v_mfma_f32_32x32x16_bf16), FP32
-//   accumulation of all split terms into one accumulator.
+//   The operand for symbol s is the staged byte masked with 0x01 << s, i.e. 2^s or 0;
+//   cell (x, y) therefore accumulates 2^(x+y) * count, which the epilogue shifts out.
+// Weights: split-BF16 (w = hi + lo [+ lo2]) MFMA (v_mfma_f32_32x32x16_bf16); the i-side
+//   operand is 1.0 where the symbol matches, the j-side operand the split weight part;
+//   all parts accumulate into one FP32 accumulator.
 //
-// Output: one 16-entry record per column pair (i < j) at cnt[(i*L + j)*16 + x*4 + y],
-// the layout of pp[i][j][IDX(x,y,4)] in the reference.
+// MFMA 32x32 operand / result lane layout (CDNA3/4): A row m and B column n belong to
+// lanes with lane % 32 == m (resp. n); the K values split across lane / 32; result
+// register r of lane l holds D[8*(r/4) + 4*(l/32) + r%4][l % 32].  A rows are j columns,
+// B columns are i columns.  A Gram uses the same K assignment on both sides, so the
+// order in which the hardware pairs K values need not be known.
+//
+// Output: a 16-cell record per column pair i < j at record rsg_rec(i, j) = j*L + i,
+// cell x*4+y (the layout of pp[i][j][IDX(x,y,4)] in the reference).
 #include "rsg_internal.h"
 #include <map>
 #include <tuple>
@@ -28,13 +37,12 @@ typedef int   v16i __attribute__((ext_vector_type(16)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-#define KT 64            // sequences per LDS stage of the int8 Gram
+#define NPAD_Q 256            // sequence padding quantum (covers every K-tile size below)
 
-int64_t rsg_npad(int64_t N) { return ((N + KT - 1) / KT) * KT; }
+int64_t rsg_npad(int64_t N) { return ((N + NPAD_Q - 1) / NPAD_Q) * NPAD_Q; }
 
 // ------------------------------------------------------------------------------------
 // Pack: ax (N x L, row-major) -> axT (Lpad x Npad raw codes) and xt (Lpad x Npad nibbles).
-// 64 x 64 tiles through LDS so both the read and the write are coalesced.
 __global__ void k_pack(const uint8_t *__restrict__ ax, int64_t N, int64_t L, int64_t Npad,
                        uint8_t *__restrict__ axT, uint8_t *__restrict__ xt)
 {
@@ -65,29 +73,50 @@ int rsg_pack(rsg_ctx *c, rsg_msa *m, char *errbuf)
 }
 
 // ------------------------------------------------------------------------------------
-// int8 Gram.  Work-group tile: BI columns (i side) x BJ columns (j side); WI x WJ waves,
-// each wave TI x TJ MFMA tiles of 8 x 8 columns (32 x 32 features).
-//
-// MFMA 32x32 operand / result lane layout (CDNA3/4): A row m and B column n are held
-// by lanes with lane % 32 == m (resp. n), the 32 K values split in two halves by
-// lane / 32; result register r of lane l holds D[8*(r/4) + 4*(l/32) + r%4][l % 32].
-// A rows are j-side features (m = 4*jj + y), B columns i-side features (n = 4*ii + x),
-// so lane l owns (ii = (l%32)/4, x = l%4) and registers 4q..4q+3 hold y = 0..3 of
-// jj = 2q + l/32.  A Gram uses the same K mapping for A and B, so the assignment of K
-// values to lane halves need not be known.
-template <int TI, int TJ, int WI, int WJ, int KIND>
+// Staging of code bytes for one K-tile: [chunk (16 seqs)][column][16 B], i side after j side.
+// The prefetch registers are a plain local array of the kernel (kept in VGPRs).
+template <int BI, int BJ, int KT, int NT>
+struct CodeStage {
+  static constexpr int NKC = KT / 16;
+  static constexpr int CSA = BJ * 16 + 64, CSB = BI * 16 + 64;
+  static constexpr int BYTES = NKC * (CSA + CSB);
+  static constexpr int NCH = (BI + BJ) * NKC;
+  static constexpr int NR = (NCH + NT - 1) / NT;
+};
+
+// Every thread moves exactly NR chunks (NCH is a multiple of the thread count for all
+// shapes), so the prefetch registers are unconditionally assigned and stay in VGPRs
+// while the MFMAs of the current stage run.
+#define STAGE_GLOAD(St, pre, xt, Npad, i0, j0, k0, tid)                                        \
+  static_assert(St::NCH % NT == 0, "stage chunks must divide evenly among threads");         \
+  _Pragma("unroll") for (int r_ = 0; r_ < St::NR; r_++) {                                      \
+    const int ch_ = (tid) + r_ * NT;                                                           \
+    const int cc_ = ch_ / St::NKC, kc_ = ch_ % St::NKC;                                        \
+    const int64_t col_ = (cc_ < BJ) ? ((j0) + cc_) : ((i0) + cc_ - BJ);                        \
+    pre[r_] = *(const uint4 *)((xt) + col_ * (Npad) + (k0) + kc_ * 16);                        \
+  }
+#define STAGE_SSTORE(St, pre, base, tid)                                                        \
+  _Pragma("unroll") for (int r_ = 0; r_ < St::NR; r_++) {                                      \
+    const int ch_ = (tid) + r_ * NT;                                                           \
+    const int cc_ = ch_ / St::NKC, kc_ = ch_ % St::NKC;                                        \
+    uint8_t *p_ = (cc_ < BJ) ? ((base) + kc_ * St::CSA + cc_ * 16)                             \
+                             : ((base) + St::NKC * St::CSA + kc_ * St::CSB + (cc_ - BJ) * 16); \
+    *(uint4 *)p_ = pre[r_];                                                                    \
+  }
+
+// int8 Gram: WI x WJ waves, each 32 i-columns x 32 j-columns, KT sequences per stage.
+// Records are written as the raw accumulators (count << (x+y)), so the accumulators
+// never pass through VALU and stay in AGPRs.
+template <int WI, int WJ, int KT>
 __global__ void __launch_bounds__(64 * WI * WJ)
-k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, void *__restrict__ out,
+k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, uint32_t *__restrict__ out,
           const int2 *__restrict__ tiles, int ntiles, int per_xcd)
 {
-  constexpr int BI = 8 * TI * WI, BJ = 8 * TJ * WJ, NT = 64 * WI * WJ;
-  constexpr int CSA = BJ * 16 + 128, CSB = BI * 16 + 128;     // LDS chunk-row strides
-  constexpr int STAGE = 4 * (CSA + CSB);
-  constexpr int NCH = (BI + BJ) * (KT / 16);                   // 16-byte chunks per stage
-  constexpr int NR = (NCH + NT - 1) / NT;
-  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * STAGE];
+  constexpr int BI = 32 * WI, BJ = 32 * WJ, NT = 64 * WI * WJ, KS = KT / 32;
+  typedef CodeStage<BI, BJ, KT, NT> St;
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * St::BYTES];
 
-  // XCD-aware mapping: consecutive tiles of the (bi, bj)-sorted list land on one XCD.
+  // XCD-aware mapping: consecutive tiles of the (bi, bj)-sorted list run on one XCD (one L2).
   const int g = blockIdx.x;
   const int t = (g & 7) * per_xcd + (g >> 3);
   if (t >= ntiles) return;
@@ -96,117 +125,210 @@ k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, void *__restr
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = wave % WI, wj = wave / WI;
-  const unsigned mask = 0x01010101u << (lane & 3);
-  const int fcol = (lane & 31) >> 2, khalf = lane >> 5;
+  const int col = lane & 31, khalf = lane >> 5;
+  const int offA = (wj * 32 + col) * 16 + khalf * St::CSA;
+  const int offB = St::NKC * St::CSA + (wi * 32 + col) * 16 + khalf * St::CSB;
 
-  v16i acc[TI][TJ];
+  v16i acc[4][4];
 #pragma unroll
-  for (int a = 0; a < TI; a++)
+  for (int x = 0; x < 4; x++)
 #pragma unroll
-    for (int b = 0; b < TJ; b++)
+    for (int y = 0; y < 4; y++)
 #pragma unroll
-      for (int r = 0; r < 16; r++) acc[a][b][r] = 0;
+      for (int r = 0; r < 16; r++) acc[x][y][r] = 0;
 
-  uint4 pre[NR];
-  auto gload = [&](int64_t k0) {
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-      int ch = tid + r * NT;
-      if (ch < NCH) {
-        int cc = ch >> 2, kc = ch & 3;
-        int64_t col = (cc < BJ) ? (j0 + cc) : (i0 + cc - BJ);
-        pre[r] = *(const uint4 *)(xt + col * Npad + k0 + kc * 16);
-      }
-    }
-  };
-  auto sstore = [&](int buf) {
-    uint8_t *base = lds + buf * STAGE;
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-      int ch = tid + r * NT;
-      if (ch < NCH) {
-        int cc = ch >> 2, kc = ch & 3;
-        uint8_t *p = (cc < BJ) ? (base + kc * CSA + cc * 16) : (base + 4 * CSA + kc * CSB + (cc - BJ) * 16);
-        *(uint4 *)p = pre[r];
-      }
-    }
-  };
-
+  uint4 pre[St::NR];
   const int64_t nkt = Npad / KT;
-  gload(0);
-  sstore(0);
+  STAGE_GLOAD(St, pre, xt, Npad, i0, j0, 0, tid);
+  STAGE_SSTORE(St, pre, lds, tid);
   __syncthreads();
   for (int64_t kt = 0; kt < nkt; kt++) {
     const int buf = (int)(kt & 1);
-    if (kt + 1 < nkt) gload((kt + 1) * KT);
-    const uint8_t *A = lds + buf * STAGE;
-    const uint8_t *B = A + 4 * CSA;
+    // straight-line prefetch (the last stage reloads itself, in bounds and unused), so the
+    // prefetch registers are never conditionally live
+    const int64_t knext = ((kt + 1 < nkt) ? kt + 1 : kt) * KT;
+    STAGE_GLOAD(St, pre, xt, Npad, i0, j0, knext, tid);
+    const uint8_t *base = lds + buf * St::BYTES;
+    uint4 ra = *(const uint4 *)(base + offA);
+    uint4 rb = *(const uint4 *)(base + offB);
 #pragma unroll
-    for (int ks = 0; ks < KT / 32; ks++) {
-      const int kc = ks * 2 + khalf;
-      v4i af[TJ], bf[TI];
+    for (int ks = 0; ks < KS; ks++) {
+      uint4 na = ra, nb = rb;
+      if (ks + 1 < KS) {
+        na = *(const uint4 *)(base + offA + (ks + 1) * 2 * St::CSA);
+        nb = *(const uint4 *)(base + offB + (ks + 1) * 2 * St::CSB);
+      }
+      v4i a[4], b[4];
 #pragma unroll
-      for (int b = 0; b < TJ; b++) {
-        uint4 v = *(const uint4 *)(A + kc * CSA + (wj * 8 * TJ + b * 8 + fcol) * 16);
-        af[b] = (v4i){(int)(v.x & mask), (int)(v.y & mask), (int)(v.z & mask), (int)(v.w & mask)};
+      for (int s = 0; s < 4; s++) {
+        const unsigned mk = 0x01010101u << s;
+        a[s] = (v4i){(int)(ra.x & mk), (int)(ra.y & mk), (int)(ra.z & mk), (int)(ra.w & mk)};
+        b[s] = (v4i){(int)(rb.x & mk), (int)(rb.y & mk), (int)(rb.z & mk), (int)(rb.w & mk)};
       }
 #pragma unroll
-      for (int a = 0; a < TI; a++) {
-        uint4 v = *(const uint4 *)(B + kc * CSB + (wi * 8 * TI + a * 8 + fcol) * 16);
-        bf[a] = (v4i){(int)(v.x & mask), (int)(v.y & mask), (int)(v.z & mask), (int)(v.w & mask)};
-      }
+      for (int x = 0; x < 4; x++)
 #pragma unroll
-      for (int a = 0; a < TI; a++)
-#pragma unroll
-        for (int b = 0; b < TJ; b++)
-          acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[b], bf[a], acc[a][b], 0, 0, 0);
+        for (int y = 0; y < 4; y++)
+          acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[y], b[x], acc[x][y], 0, 0, 0);
+      ra = na; rb = nb;
     }
-    if (kt + 1 < nkt) sstore(buf ^ 1);
+    STAGE_SSTORE(St, pre, lds + (buf ^ 1) * St::BYTES, tid);
     __syncthreads();
   }
 
-  // epilogue: lane owns (ii, x); undo the 2^(x+y) operand scaling
-  const int x = lane & 3, ii = (lane & 31) >> 2;
+  // epilogue: lane owns i column `col`; register r holds j = 8*(r/4) + 4*khalf + r%4.
+  const int64_t i = i0 + wi * 32 + col;
 #pragma unroll
-  for (int a = 0; a < TI; a++) {
-    const int64_t i = i0 + wi * 8 * TI + a * 8 + ii;
+  for (int r = 0; r < 16; r++) {
+    const int64_t j = j0 + wj * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
+    if (i < j && j < L) {
+      uint4 *p = (uint4 *)(out + rsg_rec(i, j, L) * 16);
 #pragma unroll
-    for (int b = 0; b < TJ; b++) {
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int64_t j = j0 + wj * 8 * TJ + b * 8 + 2 * q + khalf;
-        if (i < j && j < L) {
-          const int64_t off = (i * L + j) * 16 + x * 4;
-          unsigned c0 = (unsigned)acc[a][b][4 * q + 0] >> (x + 0);
-          unsigned c1 = (unsigned)acc[a][b][4 * q + 1] >> (x + 1);
-          unsigned c2 = (unsigned)acc[a][b][4 * q + 2] >> (x + 2);
-          unsigned c3 = (unsigned)acc[a][b][4 * q + 3] >> (x + 3);
-          if (KIND == CNT_U16) {
-            *(uint2 *)((uint16_t *)out + off) = make_uint2(c0 | (c1 << 16), c2 | (c3 << 16));
-          } else {
-            *(uint4 *)((uint32_t *)out + off) = make_uint4(c0, c1, c2, c3);
-          }
-        }
-      }
+      for (int x = 0; x < 4; x++)
+        p[x] = make_uint4(acc[x][0][r], acc[x][1][r], acc[x][2][r], acc[x][3][r]);
     }
   }
 }
 
-// Tile-shape table (the autotuner sweeps these).
-struct GramShape { int TI, TJ, WI, WJ; const char *name; };
+// ------------------------------------------------------------------------------------
+// Weighted Gram, split-BF16.  Staged like the int8 Gram plus the split weight parts of
+// the K-tile's sequences (bf16 bits, [part][KT]).
+//
+// Operand expansion for symbol s from 4 staged bytes (one dword d): t = (d >> s) & 0x01010101
+// holds 0/1 per byte; v_perm spreads bytes 0,1 (or 2,3) into the two 16-bit halves and a
+// 24-bit multiply turns each 0/1 half into 0x0000/0xFFFF (mask) or 0x0000/0x3F80 (1.0).
+__device__ __forceinline__ void spread(unsigned d, int s, unsigned &lo, unsigned &hi)
+{
+  unsigned t = (d >> s) & 0x01010101u;
+  lo = __builtin_amdgcn_perm(t, t, 0x0C010C00u);
+  hi = __builtin_amdgcn_perm(t, t, 0x0C030C02u);
+}
+
+template <int WI, int WJ, int KT, int NS>
+__global__ void __launch_bounds__(64 * WI * WJ)
+k_gram_bf16(const uint8_t *__restrict__ xt, const uint16_t *__restrict__ wsplit, int64_t Npad, int64_t L,
+            float *__restrict__ out, const int2 *__restrict__ tiles, int ntiles, int per_xcd)
+{
+  constexpr int BI = 32 * WI, BJ = 32 * WJ, NT = 64 * WI * WJ, KS = KT / 32;
+  typedef CodeStage<BI, BJ, KT, NT> St;
+  constexpr int WBYTES = NS * KT * 2;
+  constexpr int SB = St::BYTES + WBYTES;
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * SB];
+
+  const int g = blockIdx.x;
+  const int t = (g & 7) * per_xcd + (g >> 3);
+  if (t >= ntiles) return;
+  const int2 tl = tiles[t];
+  const int64_t i0 = (int64_t)tl.x * BI, j0 = (int64_t)tl.y * BJ;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wi = wave % WI, wj = wave / WI;
+  const int col = lane & 31, khalf = lane >> 5;
+  const int offA = (wj * 32 + col) * 16 + khalf * St::CSA;
+  const int offB = St::NKC * St::CSA + (wi * 32 + col) * 16 + khalf * St::CSB;
+
+  v16f acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[x][y][r] = 0.f;
+
+  uint4 pre[St::NR];
+  uint4 wpre = make_uint4(0, 0, 0, 0);            // weight chunk of this thread (if any)
+  constexpr int NWCH = WBYTES / 16;               // 16-byte weight chunks per stage
+  auto wload = [&](int64_t k0) {
+    if (tid < NWCH) {
+      int p = tid / (KT / 8), kc = tid % (KT / 8);
+      wpre = *(const uint4 *)(wsplit + p * Npad + k0 + kc * 8);
+    }
+  };
+  auto wstore = [&](uint8_t *base) {
+    if (tid < NWCH) *(uint4 *)(base + St::BYTES + tid * 16) = wpre;
+  };
+  const int64_t nkt = Npad / KT;
+  STAGE_GLOAD(St, pre, xt, Npad, i0, j0, 0, tid);
+  wload(0);
+  STAGE_SSTORE(St, pre, lds, tid);
+  wstore(lds);
+  __syncthreads();
+  for (int64_t kt = 0; kt < nkt; kt++) {
+    const int buf = (int)(kt & 1);
+    const int64_t knext = ((kt + 1 < nkt) ? kt + 1 : kt) * KT;
+    STAGE_GLOAD(St, pre, xt, Npad, i0, j0, knext, tid);
+    wload(knext);
+    const uint8_t *base = lds + buf * SB;
+    const uint16_t *wl = (const uint16_t *)(base + St::BYTES);
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      const uint4 ra = *(const uint4 *)(base + offA + ks * 2 * St::CSA);
+      const uint4 rb = *(const uint4 *)(base + offB + ks * 2 * St::CSB);
+#pragma unroll
+      for (int h = 0; h < 2; h++) {               // two 16-sequence MFMA steps per 32-sequence chunk
+        const unsigned da0 = h ? ra.z : ra.x, da1 = h ? ra.w : ra.y;
+        const unsigned db0 = h ? rb.z : rb.x, db1 = h ? rb.w : rb.y;
+        uint4 wv[NS];
+#pragma unroll
+        for (int p = 0; p < NS; p++)
+          wv[p] = *(const uint4 *)(wl + p * KT + ks * 32 + khalf * 16 + h * 8);
+        v8bf bo[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          unsigned l0, h0, l1, h1;
+          spread(db0, s, l0, h0); spread(db1, s, l1, h1);
+          uint4 v = make_uint4(__umul24(l0, 0x3F80u), __umul24(h0, 0x3F80u),
+                               __umul24(l1, 0x3F80u), __umul24(h1, 0x3F80u));
+          bo[s] = *(v8bf *)&v;
+        }
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+          unsigned l0, h0, l1, h1;
+          spread(da0, y, l0, h0); spread(da1, y, l1, h1);
+          const unsigned m0 = __umul24(l0, 0xFFFFu), m1 = __umul24(h0, 0xFFFFu);
+          const unsigned m2 = __umul24(l1, 0xFFFFu), m3 = __umul24(h1, 0xFFFFu);
+#pragma unroll
+          for (int p = 0; p < NS; p++) {
+            uint4 v = make_uint4(wv[p].x & m0, wv[p].y & m1, wv[p].z & m2, wv[p].w & m3);
+            const v8bf ao = *(v8bf *)&v;
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+              acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ao, bo[x], acc[x][y], 0, 0, 0);
+          }
+        }
+      }
+    }
+    STAGE_SSTORE(St, pre, lds + (buf ^ 1) * SB, tid);
+    wstore(lds + (buf ^ 1) * SB);
+    __syncthreads();
+  }
+
+  const int64_t i = i0 + wi * 32 + col;
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int64_t j = j0 + wj * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
+    if (i < j && j < L) {
+      float4 *p = (float4 *)(out + rsg_rec(i, j, L) * 16);
+#pragma unroll
+      for (int x = 0; x < 4; x++) p[x] = make_float4(acc[x][0][r], acc[x][1][r], acc[x][2][r], acc[x][3][r]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Tile shapes (the autotuner sweeps these): waves per work group and K-tile.
+struct GramShape { int WI, WJ, KT; const char *name; };
 static const GramShape kShapes[] = {
-  {4, 4, 2, 2, "64x64c/4w(4x4)"},
-  {2, 4, 2, 2, "32x64c/4w(2x4)"},
-  {4, 2, 2, 2, "64x32c/4w(4x2)"},
-  {2, 2, 2, 2, "32x32c/4w(2x2)"},
-  {4, 4, 1, 1, "32x32c/1w(4x4)"},
-  {4, 4, 2, 1, "64x32c/2w(4x4)"},
+  {2, 2, 128, "64x64c/4w/kt128"},
+  {2, 2, 64,  "64x64c/4w/kt64"},
+  {2, 2, 256, "64x64c/4w/kt256"},
+  {1, 1, 128, "32x32c/1w/kt128"},
+  {2, 1, 128, "64x32c/2w/kt128"},
+  {1, 2, 128, "32x64c/2w/kt128"},
 };
 static const int kNShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
 
 extern "C" int rsg_gram_nshapes(void) { return kNShapes; }
 extern "C" const char *rsg_gram_shape_name(int s) { return (s >= 1 && s <= kNShapes) ? kShapes[s - 1].name : "auto"; }
-size_t rsg_gram_tile_cols(int s) { const GramShape &g = kShapes[(s >= 1 && s <= kNShapes) ? s - 1 : 0]; return 8 * (size_t)std::max(g.TI * g.WI, g.TJ * g.WJ); }
 
 // Upper-triangular tile list for (L, BI, BJ), cached per process and device.
 static const int2 *tile_list(int device, int64_t L, int BI, int BJ, int *ntiles, char *errbuf)
@@ -230,10 +352,10 @@ static const int2 *tile_list(int device, int64_t L, int BI, int BJ, int *ntiles,
   return d;
 }
 
-template <int TI, int TJ, int WI, int WJ>
+template <int WI, int WJ, int KT>
 static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, char *errbuf)
 {
-  constexpr int BI = 8 * TI * WI, BJ = 8 * TJ * WJ;
+  constexpr int BI = 32 * WI, BJ = 32 * WJ;
   if (m->Lpad % BI || m->Lpad % BJ || m->Npad % KT) return rsg_fail(errbuf, "gram: padding does not fit the tile");
   int ntiles = 0;
   const int2 *tiles = tile_list(c->device, m->L, BI, BJ, &ntiles, errbuf);
@@ -242,135 +364,83 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, char *errbuf)
   int per = (ntiles + 7) / 8;
   dim3 grid((unsigned)(8 * per)), block(64 * WI * WJ);
   int t; rsg_tstart(c, "gram_i8", &t);
-  if (kind == CNT_U16)
-    hipLaunchKernelGGL((k_gram_i8<TI, TJ, WI, WJ, CNT_U16>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->L, out, tiles, ntiles, per);
-  else
-    hipLaunchKernelGGL((k_gram_i8<TI, TJ, WI, WJ, CNT_U32>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->L, out, tiles, ntiles, per);
+  (void)kind;
+  hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->L, (uint32_t *)out,
+                     tiles, ntiles, per);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   return RSG_OK;
 }
 
-int rsg_gram_counts_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, char *errbuf)
+static int pick_shape(const rsg_ctx *c, const rsg_msa *m)
 {
   int s = c->gram_shape;
-  if (s < 1 || s > kNShapes) s = (m->L >= 512) ? 1 : 4;   // small alignments: small tiles
-  switch (s) {
-  case 1: return launch_i8<4, 4, 2, 2>(c, m, out, kind, errbuf);
-  case 2: return launch_i8<2, 4, 2, 2>(c, m, out, kind, errbuf);
-  case 3: return launch_i8<4, 2, 2, 2>(c, m, out, kind, errbuf);
-  case 4: return launch_i8<2, 2, 2, 2>(c, m, out, kind, errbuf);
-  case 5: return launch_i8<4, 4, 1, 1>(c, m, out, kind, errbuf);
-  case 6: return launch_i8<4, 4, 2, 1>(c, m, out, kind, errbuf);
-  }
-  return rsg_fail(errbuf, "bad gram shape %d", s);
+  if (s >= 1 && s <= kNShapes) return s;
+  // autotuned on MI355X (N=5000, L=1500: 64x64 columns, K-tile 256); small alignments take
+  // one-wave 32x32 tiles so the grid still fills the chip
+  return (m->L >= 768) ? 3 : 4;
 }
 
-// ------------------------------------------------------------------------------------
-// Weighted Gram with split-BF16 weights.
-// Operands are expanded once per alignment into feature-major BF16 rows
-// (row 4*col + sym, K = sequences contiguous): xw[p] = split part p of w_s where the
-// symbol matches, x1 = 1.0 where it matches.
-__global__ void k_expand_bf16(const uint8_t *__restrict__ axT, const double *__restrict__ w, int64_t N,
-                              int64_t Npad, int64_t Lpad, int nsplit,
-                              __hip_bfloat16 *__restrict__ x1, __hip_bfloat16 *__restrict__ xw0,
-                              __hip_bfloat16 *__restrict__ xw1, __hip_bfloat16 *__restrict__ xw2)
+int rsg_gram_counts_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, char *errbuf)
 {
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= Lpad * Npad) return;
-  int64_t col = idx / Npad, s = idx - col * Npad;
-  uint8_t a = axT[idx];
-  double ws = (s < N) ? w[s] : 0.;
-  // split: hi = bf16(w), lo = bf16(w - hi), lo2 = bf16(w - hi - lo)
-  __hip_bfloat16 p0 = __float2bfloat16((float)ws);
-  double r1 = ws - (double)__bfloat162float(p0);
-  __hip_bfloat16 p1 = __float2bfloat16((float)r1);
-  double r2 = r1 - (double)__bfloat162float(p1);
-  __hip_bfloat16 p2 = __float2bfloat16((float)r2);
-  __hip_bfloat16 zero = __float2bfloat16(0.f), one = __float2bfloat16(1.f);
-  for (int sym = 0; sym < 4; sym++) {
-    int64_t o = (col * 4 + sym) * Npad + s;
-    bool hit = (a == sym) && (s < N);
-    x1[o]  = hit ? one : zero;
-    xw0[o] = hit ? p0 : zero;
-    xw1[o] = hit ? p1 : zero;
-    if (nsplit > 2) xw2[o] = hit ? p2 : zero;
+  switch (pick_shape(c, m)) {
+  case 1: return launch_i8<2, 2, 128>(c, m, out, kind, errbuf);
+  case 2: return launch_i8<2, 2, 64>(c, m, out, kind, errbuf);
+  case 3: return launch_i8<2, 2, 256>(c, m, out, kind, errbuf);
+  case 4: return launch_i8<1, 1, 128>(c, m, out, kind, errbuf);
+  case 5: return launch_i8<2, 1, 128>(c, m, out, kind, errbuf);
+  case 6: return launch_i8<1, 2, 128>(c, m, out, kind, errbuf);
+  }
+  return rsg_fail(errbuf, "bad gram shape");
+}
+
+// split weights: part p of w_s as bf16 bits, [NS][Npad]; hi = bf16(w), lo = bf16(w - hi), ...
+__global__ void k_split_weights(const double *__restrict__ w, int64_t N, int64_t Npad, int nsplit,
+                                uint16_t *__restrict__ out)
+{
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= Npad) return;
+  double r = (s < N) ? w[s] : 0.;
+  for (int p = 0; p < nsplit; p++) {
+    __hip_bfloat16 b = __float2bfloat16((float)r);
+    r -= (double)__bfloat162float(b);
+    out[p * Npad + s] = __bfloat16_as_ushort(b);
   }
 }
 
 int rsg_upload_weights_bf16(rsg_ctx *c, rsg_msa *m, int nsplit, char *errbuf)
 {
-  if (m->nsplit_ready >= nsplit) return RSG_OK;
-  size_t bytes = sizeof(__hip_bfloat16) * 4 * (size_t)m->Lpad * (size_t)m->Npad;
-  if (!m->d_x1) HIPCHK(hipMalloc(&m->d_x1, bytes));
-  for (int p = 0; p < nsplit; p++) if (!m->d_xw[p]) HIPCHK(hipMalloc(&m->d_xw[p], bytes));
-  int64_t n = m->Lpad * m->Npad;
-  int t; rsg_tstart(c, "expand_bf16", &t);
-  hipLaunchKernelGGL(k_expand_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                     m->d_axT, m->d_w, m->N, m->Npad, m->Lpad, nsplit, m->d_x1, m->d_xw[0], m->d_xw[1],
-                     nsplit > 2 ? m->d_xw[2] : m->d_xw[1]);
-  rsg_tstop(c, t);
+  if (m->nsplit_ready == nsplit) return RSG_OK;
+  if (!m->d_wsplit) HIPCHK(hipMalloc(&m->d_wsplit, sizeof(uint16_t) * 3 * (size_t)m->Npad));
+  hipLaunchKernelGGL(k_split_weights, dim3((unsigned)((m->Npad + 255) / 256)), dim3(256), 0, c->stream,
+                     m->d_w, m->N, m->Npad, nsplit, m->d_wsplit);
   HIPCHK(hipGetLastError());
   m->nsplit_ready = nsplit;
   return RSG_OK;
 }
 
-// One wave per 8 x 8 column block (32 x 32 features), WPB waves per work group; operands
-// are read straight from HBM/L2 (each 16-byte fragment serves one MFMA).
-template <int NS>
-__global__ void __launch_bounds__(256)
-k_gram_bf16(const __hip_bfloat16 *__restrict__ x1, const __hip_bfloat16 *__restrict__ xw0,
-            const __hip_bfloat16 *__restrict__ xw1, const __hip_bfloat16 *__restrict__ xw2,
-            int64_t Npad, int64_t L, float *__restrict__ out, const int2 *__restrict__ tiles, int ntiles)
+template <int WI, int WJ, int KT, int NS>
+static int launch_bf16(rsg_ctx *c, rsg_msa *m, float *out, char *errbuf)
 {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int t = blockIdx.x * 4 + wave;
-  if (t >= ntiles) return;
-  const int2 tl = tiles[t];
-  const int64_t i0 = (int64_t)tl.x * 8, j0 = (int64_t)tl.y * 8;
-  const int khalf = lane >> 5, f = lane & 31;
-  const __hip_bfloat16 *pa0 = xw0 + (j0 * 4 + f) * Npad + khalf * 8;
-  const __hip_bfloat16 *pa1 = xw1 + (j0 * 4 + f) * Npad + khalf * 8;
-  const __hip_bfloat16 *pa2 = xw2 + (j0 * 4 + f) * Npad + khalf * 8;
-  const __hip_bfloat16 *pb  = x1  + (i0 * 4 + f) * Npad + khalf * 8;
-  v16f acc;
-#pragma unroll
-  for (int r = 0; r < 16; r++) acc[r] = 0.f;
-  for (int64_t k = 0; k < Npad; k += 16) {
-    v8bf b  = *(const v8bf *)(pb + k);
-    v8bf a0 = *(const v8bf *)(pa0 + k);
-    v8bf a1 = *(const v8bf *)(pa1 + k);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc, 0, 0, 0);
-    if (NS > 2) {
-      v8bf a2 = *(const v8bf *)(pa2 + k);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, b, acc, 0, 0, 0);
-    }
-  }
-  const int x = lane & 3, ii = (lane & 31) >> 2;
-  const int64_t i = i0 + ii;
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int64_t j = j0 + 2 * q + khalf;
-    if (i < j && j < L)
-      *(float4 *)(out + (i * L + j) * 16 + x * 4) = make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
-  }
+  constexpr int BI = 32 * WI, BJ = 32 * WJ;
+  if (m->Lpad % BI || m->Lpad % BJ || m->Npad % KT) return rsg_fail(errbuf, "gram: padding does not fit the tile");
+  int ntiles = 0;
+  const int2 *tiles = tile_list(c->device, m->L, BI, BJ, &ntiles, errbuf);
+  if (!tiles) return RSG_FAIL;
+  if (ntiles == 0) return RSG_OK;
+  int per = (ntiles + 7) / 8;
+  int t; rsg_tstart(c, "gram_bf16", &t);
+  hipLaunchKernelGGL((k_gram_bf16<WI, WJ, KT, NS>), dim3((unsigned)(8 * per)), dim3(64 * WI * WJ), 0, c->stream,
+                     m->d_xt, m->d_wsplit, m->Npad, m->L, out, tiles, ntiles, per);
+  rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  return RSG_OK;
 }
 
 int rsg_gram_counts_bf16(rsg_ctx *c, rsg_msa *m, int nsplit, float *out, char *errbuf)
 {
   if (rsg_upload_weights_bf16(c, m, nsplit, errbuf) != RSG_OK) return RSG_FAIL;
-  int ntiles = 0;
-  const int2 *tiles = tile_list(c->device, m->L, 8, 8, &ntiles, errbuf);
-  if (!tiles) return RSG_FAIL;
-  if (!ntiles) return RSG_OK;
-  int t; rsg_tstart(c, "gram_bf16", &t);
-  dim3 grid((unsigned)((ntiles + 3) / 4));
-  if (nsplit > 2)
-    hipLaunchKernelGGL(k_gram_bf16<3>, grid, dim3(256), 0, c->stream, m->d_x1, m->d_xw[0], m->d_xw[1], m->d_xw[2], m->Npad, m->L, out, tiles, ntiles);
-  else
-    hipLaunchKernelGGL(k_gram_bf16<2>, grid, dim3(256), 0, c->stream, m->d_x1, m->d_xw[0], m->d_xw[1], m->d_xw[1], m->Npad, m->L, out, tiles, ntiles);
-  rsg_tstop(c, t);
-  HIPCHK(hipGetLastError());
-  return RSG_OK;
+  const bool big = m->L >= 768;
+  if (nsplit > 2) return big ? launch_bf16<2, 2, 64, 3>(c, m, out, errbuf) : launch_bf16<1, 1, 64, 3>(c, m, out, errbuf);
+  return big ? launch_bf16<2, 2, 64, 2>(c, m, out, errbuf) : launch_bf16<1, 1, 64, 2>(c, m, out, errbuf);
 }

@@ -11,8 +11,16 @@
 
 #define RSG_MAX_TIMERS 64
 
+// Pair record of columns i < j (16 cells c[x*4+y]) lives at record index j*L + i, so
+// consecutive i (the MFMA result lanes) store to consecutive records.
+__host__ __device__ static inline int64_t rsg_rec(int64_t i, int64_t j, int64_t L) { return j * L + i; }
+
 // Kinds of stored pair counts.
-enum { CNT_U16 = 0, CNT_U32 = 1, CNT_F32 = 2 };
+//   CNT_I8G: int8-Gram records, cell x*4+y holds count << (x + y) (the MFMA accumulator
+//            as produced; readers shift the scale out), 64 bytes per record.
+//   CNT_F32: split-BF16 weighted records, FP32 cells, 64 bytes per record.
+enum { CNT_I8G = 1, CNT_F32 = 2 };
+__host__ __device__ static inline int rsg_cell_shift(int k) { return (k >> 2) + (k & 3); }
 
 struct rsg_timer { const char *name; hipEvent_t a, b; };
 
@@ -38,9 +46,8 @@ struct rsg_msa {
   double  *d_w   = nullptr;     // N weights (FP64); all 1 when unweighted
   int      weighted = 0;
   double   wsum = 0.;
-  // split-BF16 operands of the weighted Gram: [4*Lpad][Npad]
-  __hip_bfloat16 *d_xw[3] = {nullptr, nullptr, nullptr};
-  __hip_bfloat16 *d_x1 = nullptr;
+  // split-BF16 weight parts of the weighted Gram: [3][Npad] bf16 bits
+  uint16_t *d_wsplit = nullptr;
   int      nsplit_ready = 0;
   // pair tables
   void    *d_cnt = nullptr;     // L x L x 16 (i<j records), kind below
@@ -98,5 +105,4 @@ int rsg_raf(rsg_ctx *c, rsg_msa *m, int rafs, const double *allow16, char *errbu
 int rsg_correct(rsg_ctx *c, rsg_msa *m, int actype, int shiftnonneg, char *errbuf);
 int rsg_minmax(rsg_ctx *c, rsg_msa *m, char *errbuf);
 int rsg_upload_weights_bf16(rsg_ctx *c, rsg_msa *m, int nsplit, char *errbuf);
-size_t rsg_gram_tile_cols(int shape);
 int64_t rsg_npad(int64_t N);

@@ -17,16 +17,14 @@
 
 template <int KIND> __device__ __forceinline__ void load_rec(const void *cnt, int64_t rec, double *c)
 {
-  if (KIND == CNT_U16) {
-    const uint4 *p = (const uint4 *)((const uint16_t *)cnt + rec * 16);
-    uint4 a = p[0], b = p[1];
-    unsigned w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-    for (int k = 0; k < 8; k++) { c[2 * k] = (double)(w[k] & 0xffffu); c[2 * k + 1] = (double)(w[k] >> 16); }
-  } else if (KIND == CNT_U32) {
+  if (KIND == CNT_I8G) {
     const uint4 *p = (const uint4 *)((const uint32_t *)cnt + rec * 16);
 #pragma unroll
-    for (int k = 0; k < 4; k++) { uint4 v = p[k]; c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w; }
+    for (int k = 0; k < 4; k++) {
+      uint4 v = p[k];                               // row x = k: cells y = 0..3, scale 2^(x+y)
+      c[4 * k] = (double)(v.x >> (k + 0)); c[4 * k + 1] = (double)(v.y >> (k + 1));
+      c[4 * k + 2] = (double)(v.z >> (k + 2)); c[4 * k + 3] = (double)(v.w >> (k + 3));
+    }
   } else {
     const float4 *p = (const float4 *)((const float *)cnt + rec * 16);
 #pragma unroll
@@ -46,7 +44,7 @@ __global__ void __launch_bounds__(256) k_pm(const void *__restrict__ cnt, int64_
     if (j == i) continue;
     int64_t a = (i < j) ? i : j, b = (i < j) ? j : i;
     double c[16];
-    load_rec<KIND>(cnt, a * L + b, c);
+    load_rec<KIND>(cnt, rsg_rec(a, b, L), c);
     double ns = 0.;
 #pragma unroll
     for (int k = 0; k < 16; k++) ns += c[k];
@@ -121,9 +119,8 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
   if (!m->d_ps) HIPCHK(hipMalloc(&m->d_ps, sizeof(double) * 5 * (size_t)L));
   HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
   int t; rsg_tstart(c, "marginals", &t);
-  if (m->cnt_kind == CNT_U16)      hipLaunchKernelGGL(k_pm<CNT_U16>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, tol, m->d_pm, m->d_pm + 4 * L, c->d_flag);
-  else if (m->cnt_kind == CNT_U32) hipLaunchKernelGGL(k_pm<CNT_U32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, tol, m->d_pm, m->d_pm + 4 * L, c->d_flag);
-  else                             hipLaunchKernelGGL(k_pm<CNT_F32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, tol, m->d_pm, m->d_pm + 4 * L, c->d_flag);
+  if (m->cnt_kind == CNT_I8G) hipLaunchKernelGGL(k_pm<CNT_I8G>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, tol, m->d_pm, m->d_pm + 4 * L, c->d_flag);
+  else                            hipLaunchKernelGGL(k_pm<CNT_F32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, tol, m->d_pm, m->d_pm + 4 * L, c->d_flag);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   rsg_tstart(c, "ps", &t);
@@ -266,28 +263,30 @@ __global__ void __launch_bounds__(256) k_stat(StatArgs A, int nb)
   const int tid = threadIdx.x;
   double lmin = INFINITY, lmax = -INFINITY;
   if (bi <= bj) {
-    const int jj = tid & 31;
+    // phase 1: consecutive threads take consecutive i (consecutive records j*L + i),
+    // write row j of COV; phase 2 writes row i from the LDS transpose.
+    const int ii = tid & 31;
     for (int k = 0; k < 4; k++) {
-      const int ii = (tid >> 5) + 8 * k;
+      const int jj = (tid >> 5) + 8 * k;
       const int64_t i = (int64_t)bi * 32 + ii, j = (int64_t)bj * 32 + jj;
       double v = -INFINITY;
       if (i < j && j < A.L) {
         double cd[16];
-        load_rec<KIND>(A.cnt, i * A.L + j, cd);
+        load_rec<KIND>(A.cnt, rsg_rec(i, j, A.L), cd);
         v = pair_stat<T, TAB>(A, cd, i, j);
         lmin = fmin(lmin, v); lmax = fmax(lmax, v);
-        A.cov[i * A.L + j] = v;
+        A.cov[j * A.L + i] = v;
       } else if (i == j && i < A.L) {
         A.cov[i * A.L + j] = -INFINITY;
       }
-      tile[ii][jj] = v;
+      tile[jj][ii] = v;
     }
     __syncthreads();
-    const int ii = tid & 31;
+    const int jj = tid & 31;
     for (int k = 0; k < 4; k++) {
-      const int jj2 = (tid >> 5) + 8 * k;
-      const int64_t i = (int64_t)bi * 32 + ii, j = (int64_t)bj * 32 + jj2;
-      if (i < j && j < A.L) A.cov[j * A.L + i] = tile[ii][jj2];
+      const int ii2 = (tid >> 5) + 8 * k;
+      const int64_t i = (int64_t)bi * 32 + ii2, j = (int64_t)bj * 32 + jj;
+      if (i < j && j < A.L) A.cov[i * A.L + j] = tile[jj][ii2];
     }
   }
   rmin[tid] = lmin; rmax[tid] = lmax;
@@ -341,7 +340,7 @@ __global__ void k_ccf_rows(const void *__restrict__ cnt, int64_t L, const double
   const int64_t i = blockIdx.x;
   double ns_sum = 0.;
   for (int64_t j = i + 1 + threadIdx.x; j < L; j += 256) {
-    double cd[16]; load_rec<KIND>(cnt, i * L + j, cd);
+    double cd[16]; load_rec<KIND>(cnt, rsg_rec(i, j, L), cd);
     for (int k = 0; k < 16; k++) ns_sum += cd[k];
   }
   for (int x = 0; x < 4; x++) red[x][threadIdx.x] = ns_sum * pm[i * 4 + x];
@@ -369,9 +368,8 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
   for (int x = 0; x < 4; x++) A.meanp[x] = 0.;
   if (base == RSG_CCF) {
     double *rows = c->d_red + 2 * (size_t)nblk + 8;
-    if (m->cnt_kind == CNT_U16)      hipLaunchKernelGGL(k_ccf_rows<CNT_U16>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, m->d_pm, rows);
-    else if (m->cnt_kind == CNT_U32) hipLaunchKernelGGL(k_ccf_rows<CNT_U32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, m->d_pm, rows);
-    else                             hipLaunchKernelGGL(k_ccf_rows<CNT_F32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, m->d_pm, rows);
+    if (m->cnt_kind == CNT_I8G) hipLaunchKernelGGL(k_ccf_rows<CNT_I8G>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, m->d_pm, rows);
+    else                            hipLaunchKernelGGL(k_ccf_rows<CNT_F32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, m->d_pm, rows);
     std::vector<double> h(4 * (size_t)L);
     HIPCHK(hipMemcpyAsync(h.data(), rows, sizeof(double) * 4 * (size_t)L, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -385,13 +383,11 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
   dim3 grid((unsigned)nblk), blk(256);
 #define LAUNCH(T, K, TB) hipLaunchKernelGGL((k_stat<T, K, TB>), grid, blk, 0, c->stream, A, nb)
   if (prec == RSG_STAT_FP64) {
-    if (m->cnt_kind == CNT_U16)      { if (tab) LAUNCH(double, CNT_U16, true); else LAUNCH(double, CNT_U16, false); }
-    else if (m->cnt_kind == CNT_U32) { if (tab) LAUNCH(double, CNT_U32, true); else LAUNCH(double, CNT_U32, false); }
-    else                             LAUNCH(double, CNT_F32, false);
+    if (m->cnt_kind == CNT_I8G) { if (tab) LAUNCH(double, CNT_I8G, true); else LAUNCH(double, CNT_I8G, false); }
+    else                        LAUNCH(double, CNT_F32, false);
   } else {
-    if (m->cnt_kind == CNT_U16)      LAUNCH(float, CNT_U16, false);
-    else if (m->cnt_kind == CNT_U32) LAUNCH(float, CNT_U32, false);
-    else                             LAUNCH(float, CNT_F32, false);
+    if (m->cnt_kind == CNT_I8G) LAUNCH(float, CNT_I8G, false);
+    else                        LAUNCH(float, CNT_F32, false);
   }
 #undef LAUNCH
   rsg_tstop(c, t);
@@ -413,7 +409,7 @@ __global__ void __launch_bounds__(256) k_raf(const void *__restrict__ cnt, int64
   int64_t i = idx / L, j = idx - i * L;
   if (i == j) { cov[idx] = -INFINITY; return; }
   int64_t a = (i < j) ? i : j, b = (i < j) ? j : i;
-  double cd[16]; load_rec<KIND>(cnt, a * L + b, cd);
+  double cd[16]; load_rec<KIND>(cnt, rsg_rec(a, b, L), cd);
   double M = 0., mx[4] = {0, 0, 0, 0}, my[4] = {0, 0, 0, 0};
   for (int x = 0; x < 4; x++)
     for (int y = 0; y < 4; y++)
@@ -484,8 +480,7 @@ int rsg_raf(rsg_ctx *c, rsg_msa *m, int rafs, const double *allow16, char *errbu
   }
   unsigned nb = (unsigned)((L * L + 255) / 256);
   int t; rsg_tstart(c, "raf", &t);
-  if (kind == CNT_U16) hipLaunchKernelGGL(k_raf<CNT_U16>, dim3(nb), dim3(256), 0, c->stream, cnt, L, m->N, allow, dst);
-  else                 hipLaunchKernelGGL(k_raf<CNT_U32>, dim3(nb), dim3(256), 0, c->stream, cnt, L, m->N, allow, dst);
+  hipLaunchKernelGGL(k_raf<CNT_I8G>, dim3(nb), dim3(256), 0, c->stream, cnt, L, m->N, allow, dst);
   if (rafs) hipLaunchKernelGGL(k_rafs, dim3(nb), dim3(256), 0, c->stream, m->d_tmp, L, m->d_cov);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
