@@ -134,6 +134,10 @@ double rsg_cov2evalue(double cov, int64_t Nc, const rsg_hist *h, const double *s
 rsg_hist *rsg_hist_create(double bmin, double bmax, double w);
 void rsg_hist_destroy(rsg_hist *h);
 void rsg_free(void *p);
+/* cov_WriteNullHistogram / cov_ReadNullHistogram (covariation.c:1717-1868): the .null
+ * text format, one "%f %llu" line per bin. */
+int  rsg_hist_WriteNull(const char *path, const rsg_hist *h, char *errbuf);
+int  rsg_hist_ReadNull(const char *path, rsg_hist **ret, char *errbuf);
 
 /* ---- sequence weights and preprocessing (src/R-scape.c:1544-1562; src/msamanip.c:284-541) ---- */
 /* esl_msaweight_PB: position-based weights, FP64 on the device. */
@@ -145,36 +149,49 @@ int rsg_weights_GSC(rsg_ctx *ctx, const uint8_t *ax, int64_t nseq, int64_t alen,
 int rsg_gapcol_mask(const uint8_t *ax, const double *wgt, int64_t nseq, int64_t alen, double gapthresh,
                     int *useme);
 
-/* ---- batched phylogenetic null (cov_GenerateAlignment, src/cov_simulate.c:60-859) ----
- * A batch of independent alignments, each evolved down its own tree from its own root
- * with per-branch conditional tables, scored (statistic + correction) and histogrammed
- * on the device.  See rsg_null.h-style structs below. */
+/* ---- phylogenetic null simulation (cov_GenerateAlignment, src/cov_simulate.c:60-859;
+ *      trees as R-scape-sim's create_tree, src/R-scape-sim.c:608-680) ----
+ * Trees use the Easel ESL_TREE layout: internal nodes 0..N-2 (root 0, parents first),
+ * children > 0 internal, <= 0 leaf -child.  Substitutions follow a 4-state GTR P(t) per
+ * branch; every draw is Philox4x32-10 keyed by (seed, stream, node row, column). */
+/* GTR model from stationary pi[4] and exchangeabilities exch[6] (AC AG AU CG CU GU), scaled
+ * to one substitution per unit time; model receives 40 doubles (eigendecomposition). */
+int rsg_gtr_model(const double *pi, const double *exch, double *model);
+/* P(t) inverse-CDF tables on the device (FP64): cdf[k*12 + a*3 + b] = floor(2^32 sum_{c<=b} P(t_k)[a][c]). */
+int rsg_gtr_cdf(rsg_ctx *ctx, const double *model, int64_t nt, const double *t, uint32_t *cdf, char *errbuf);
+/* Yule tree with n leaves, branch lengths rescaled to mean mean_bl (esl_tree_Simulate +
+ * esl_tree_er_RescaleAverageBL). */
+int rsg_tree_yule(int64_t n, double mean_bl, uint64_t seed, uint32_t stream,
+                  int *left, int *right, double *ld, double *rd);
 typedef struct {
   int64_t  nleaf;            /* N sequences */
   int64_t  alen;             /* L columns */
-  const int *left, *right;   /* N-1 internal nodes, Easel ESL_TREE convention */
-  const int *tb_l, *tb_r;    /* branch -> conditional-table index */
-  const uint32_t *cdf;       /* ntables x 4 rows x 3 uint32 thresholds */
-  int      ntables;
-  const uint8_t *root;       /* alen root symbols 0..3 */
+  const int *left, *right;   /* N-1 internal nodes */
+  const double *ld, *rd;     /* branch lengths */
+  const double *model;       /* rsg_gtr_model output; the root is drawn from its pi */
   uint64_t seed;
   uint32_t stream;
   double   gapfrac;          /* i.i.d. gap probability per leaf symbol (0 = none) */
 } rsg_null_job;
-
-/* Simulate one alignment; leaves = nleaf x alen codes (the reference's leaf rows). */
+/* Simulate one alignment; leaves = nleaf x alen codes (the leaf rows, leaf order). */
 int rsg_null_simulate(rsg_ctx *ctx, const rsg_null_job *job, uint8_t *leaves, char *errbuf);
-/* Simulate and score a batch (statistic + correction, unit weights, CSELECT) and
- * return per-alignment histograms (bmin, w per job) plus FP64 exponential tail fits
- * {mu, lambda, newmass}.  use_graph = 1 captures the batch in a hipGraph. */
+/* Simulate and score a batch of alignments of any sizes (GT/MI/... + correction, unit
+ * weights, CSELECT) entirely on the device, then per alignment: histogram width
+ * (calculate_width_histo), rank histogram, and an FP64 tail fit.  Outputs per job:
+ * fits[job*8 + {0..7}] = {mu, lambda, tau, newmass, pmass, w, minCOV, maxCOV}; hists (may
+ * be NULL) receives each job's histogram.  use_graph = k >= 1 records the whole batch in one
+ * hipGraph and launches it k times (each launch recomputes everything); 0 runs eagerly. */
 int rsg_null_batch(rsg_ctx *ctx, const rsg_null_job *jobs, int njobs, int covtype,
-                   double bmin, const double *w, double pmass, double fracfit,
+                   double bmin, double pmass, double fracfit, int doexpfit,
                    rsg_hist **hists, double *fits, int use_graph, char *errbuf);
 
 /* ---- learned null-tail surrogate (no reference counterpart) ---- */
 typedef struct rsg_mlp rsg_mlp;
 int  rsg_mlp_create(rsg_ctx *ctx, int nlayers, const int *dims, uint64_t seed, rsg_mlp **ret, char *errbuf);
 void rsg_mlp_destroy(rsg_ctx *ctx, rsg_mlp *m);
+/* bf16 = 1 (default): BF16 weights and activations into v_mfma_f32_32x32x16_bf16, FP32
+ * accumulation; bf16 = 0: FP32 operands (v_mfma_f32_32x32x2f32), for gradient checks. */
+int  rsg_mlp_set_precision(rsg_mlp *m, int bf16);
 int  rsg_mlp_set_params(rsg_ctx *ctx, rsg_mlp *m, const float *const *W, const float *const *b, char *errbuf);
 int  rsg_mlp_get_params(rsg_ctx *ctx, rsg_mlp *m, float *const *W, float *const *b, char *errbuf);
 int  rsg_mlp_forward(rsg_ctx *ctx, rsg_mlp *m, const float *x, int64_t B, float *y, char *errbuf);

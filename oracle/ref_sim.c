@@ -60,44 +60,62 @@ static int icdf(uint32_t u, const uint32_t *cdf, int n)
   return k;
 }
 
-/* cov_simulate.c:288-324 + :387-451 + :585-631 + :724-798 (ungapped evolution).
- * Nodes: internal v = 0..nleaf-2 (root 0, parents before children), rows 0..nleaf-2 of
- * allseq; leaf l is row nleaf-1+l.  left/right > 0 are internal, <= 0 leaves.
- * pidx_l/r: index of the branch's 4x4 conditional table; cdf_l/r hold, per table, 4 rows x 3
- * thresholds.  ct[pos] = partner (0-based) or -1; paired 5' positions draw a 16-state pair from
- * pcdf (per table 16 rows x 15 thresholds), the 3' partner takes the second symbol. */
-int ref_evolve_tree(int64_t nleaf, const int *left, const int *right,
-                    const int *pidx_l, const int *pidx_r,
-                    const uint32_t *cdf_l, const uint32_t *cdf_r,
-                    const uint8_t *root, int64_t L, const int *ct,
-                    const uint32_t *pcdf_l, const uint32_t *pcdf_r,
-                    uint64_t seed, uint32_t stream, uint8_t *allseq)
+/* P(t) row a as cumulative uint32 thresholds from the eigendecomposed model
+ * (model[0..15] = Pi^-1/2 V, [16..31] = V^T Pi^1/2, [32..35] = eigenvalues, [36..39] = pi). */
+static void row_cdf(const double *model, double t, int a, uint32_t *out)
 {
-  memcpy(allseq, root, (size_t)L);
-  for (int64_t v = 0; v < nleaf - 1; v++) {
+  double e[4], cum = 0.;
+  for (int k = 0; k < 4; k++) e[k] = exp(model[32 + k] * t);
+  for (int b = 0; b < 3; b++) {
+    double p = 0.;
+    for (int k = 0; k < 4; k++) p += model[a * 4 + k] * e[k] * model[16 + k * 4 + b];
+    cum += (p > 0.) ? p : 0.;
+    double v = cum * 4294967296.0;
+    out[b] = (v >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)v;
+  }
+}
+
+/* cov_simulate.c:288-324 + :387-451 + :585-631 + :724-798 (ungapped evolution down a tree).
+ * Nodes: internal v = 0..nleaf-2 (root 0, parents before children) are rows 0..nleaf-2;
+ * leaf l is row nleaf-1+l.  left/right > 0 are internal, <= 0 leaves.  The root is drawn
+ * from pi with the key of row 2N-1; each child symbol from P(t) of its branch; a leaf
+ * symbol becomes a gap when the tag-1 draw is below gapthr.  leaves: N x L output. */
+int ref_evolve_gtr(int64_t nleaf, int64_t L, const int *left, const int *right, const double *ld,
+                   const double *rd, const double *model, uint64_t seed, uint32_t stream, uint32_t gapthr,
+                   uint8_t *leaves)
+{
+  int64_t nrow = 2 * nleaf - 1;
+  uint8_t *S = malloc((size_t)(nrow * L));
+  uint32_t pic[3], tab[12];
+  double cum = 0.;
+  for (int b = 0; b < 3; b++) {
+    cum += model[36 + b];
+    double v = cum * 4294967296.0;
+    pic[b] = (v >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)v;
+  }
+  for (int64_t pos = 0; pos < L; pos++) {
+    uint32_t u = ref_rng_u32(seed, stream, (uint32_t)nrow, (uint32_t)pos);
+    int a = 3;
+    for (int b = 0; b < 3; b++) if (u < pic[b]) { a = b; break; }
+    S[pos] = (uint8_t)a;
+  }
+  for (int64_t v = 0; v < nleaf - 1; v++)
     for (int side = 0; side < 2; side++) {
       int d = side ? right[v] : left[v];
       int64_t row = (d > 0) ? d : (nleaf - 1 - d);
-      int tb = side ? pidx_r[v] : pidx_l[v];
-      const uint32_t *cdf  = (side ? cdf_r : cdf_l) + (size_t)tb * 12;
-      const uint32_t *pcdf = (side ? pcdf_r : pcdf_l);
-      const uint8_t *par = allseq + v * L;
-      uint8_t *ch = allseq + row * L;
+      double t = side ? rd[v] : ld[v];
+      for (int a = 0; a < 4; a++) row_cdf(model, t, a, tab + a * 3);
       for (int64_t pos = 0; pos < L; pos++) {
-        int pr = ct ? ct[pos] : -1;
-        if (pr >= 0 && pcdf) {
-          if (pr < pos) continue;                       /* emitted with its 5' partner */
-          int a = par[pos], b = par[pr];
-          uint32_t u = ref_rng_u32(seed, stream, (uint32_t)row, (uint32_t)pos);
-          int k = icdf(u, pcdf + (size_t)tb * 240 + (size_t)(a * 4 + b) * 15, 16);
-          ch[pos] = (uint8_t)(k / 4); ch[pr] = (uint8_t)(k % 4);
-        } else {
-          uint32_t u = ref_rng_u32(seed, stream, (uint32_t)row, (uint32_t)pos);
-          ch[pos] = (uint8_t)icdf(u, cdf + par[pos] * 3, 4);
+        uint32_t u = ref_rng_u32(seed, stream, (uint32_t)row, (uint32_t)pos);
+        int sym = icdf(u, tab + S[v * L + pos] * 3, 4);
+        S[row * L + pos] = (uint8_t)sym;
+        if (d <= 0) {
+          if (gapthr && rng_tag(seed, stream, (uint32_t)row, (uint32_t)pos, 1u) < gapthr) sym = 4;
+          leaves[(-d) * L + pos] = (uint8_t)sym;
         }
       }
     }
-  }
+  free(S);
   return REF_OK;
 }
 

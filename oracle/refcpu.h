@@ -100,12 +100,9 @@ int  ref_nshuffle_default(int64_t nseq, int64_t alen);
 /* ---- counter-based RNG + rate-matrix evolution (cov_simulate.c) ---- */
 void     ref_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 uint32_t ref_rng_u32(uint64_t seed, uint32_t stream, uint32_t node, uint32_t pos);
-int ref_evolve_tree(int64_t nleaf, const int *left, const int *right,
-                    const int *pidx_l, const int *pidx_r,
-                    const uint32_t *cdf_l, const uint32_t *cdf_r,
-                    const uint8_t *root, int64_t L, const int *ct,
-                    const uint32_t *pcdf_l, const uint32_t *pcdf_r,
-                    uint64_t seed, uint32_t stream, uint8_t *allseq);
+int ref_evolve_gtr(int64_t nleaf, int64_t L, const int *left, const int *right, const double *ld,
+                   const double *rd, const double *model, uint64_t seed, uint32_t stream, uint32_t gapthr,
+                   uint8_t *leaves);
 int ref_fitch_null(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
                    uint64_t seed, uint8_t *out);
 

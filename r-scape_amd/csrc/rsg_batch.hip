@@ -1,0 +1,458 @@
+// rsg_batch.hip -- batched phylogenetic null: simulate, score and fit many alignments of
+// uneven size in one pass on the device.
+//
+// Per alignment this is R-scape's per-replicate work (null_rscape, src/R-scape.c:1615-1724
+// with the cov_GenerateAlignment simulator, src/cov_simulate.c:60-190): cov_Calculate of the
+// replicate (corr_Probs + statistic + APC/ASC, src/covariation.c:63-306), the histogram width
+// of calculate_width_histo (src/R-scape.c:1280-1371), the rank histogram
+// (src/covariation.c:414-432) and the tail fit (cov_NullFitExponential / Gamma,
+// :1914-1975 with cov_histogram_pmass :2483-2506).
+//
+// Layout: every alignment ("item") owns slices of a few large buffers, described by an
+// int64 descriptor (RSG_BDESC fields, see rsg_internal.h).  Each kernel below covers all
+// items in one launch; a block finds its item by binary search over a per-kernel block
+// prefix.  No host synchronisation happens between the launches, so the whole batch can
+// be recorded once as a hipGraph and replayed; buffers are cleared by fill kernels, not
+// memset nodes.
+#include "rsg_device.h"
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cstring>
+#include <numeric>
+
+// Histogram geometry, bins and E-values are rounded exactly as the host computes them.
+#pragma clang fp contract(off)
+
+enum { D_N = 0, D_L, D_NPAD, D_LPAD, D_XT, D_CNT, D_COV, D_PM, D_ROWS, D_HIST, D_LEAF, D_BPACK, D_BCOL,
+       D_BSTAT, D_BEL, D_NB32 };
+#define NBMAX 16384          // histogram bins per item (an item needing more is reported as failed)
+#define ELPB 1024            // matrix elements per block in the element-wise kernels
+
+__device__ __forceinline__ int find_item(const int64_t *__restrict__ desc, int n, int field, int64_t blk)
+{
+  int lo = 0, hi = n - 1;
+  while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (desc[mid * RSG_BDESC + field] <= blk) lo = mid; else hi = mid - 1; }
+  return lo;
+}
+
+// leaves (N x L, row-major) -> one-hot nibble codes (Lpad x Npad)
+__global__ void __launch_bounds__(256)
+k_pack_b(const int64_t *__restrict__ desc, int n, const uint8_t *__restrict__ leaves, uint8_t *__restrict__ xt)
+{
+  __shared__ uint8_t tile[64][65];
+  const int it = find_item(desc, n, D_BPACK, blockIdx.x);
+  const int64_t *d = desc + it * RSG_BDESC;
+  const int64_t N = d[D_N], L = d[D_L], Npad = d[D_NPAD], Lpad = d[D_LPAD];
+  const int64_t local = blockIdx.x - d[D_BPACK], nbx = Lpad / 64;
+  const int64_t c0 = (local % nbx) * 64, s0 = (local / nbx) * 64;
+  const uint8_t *ax = leaves + d[D_LEAF];
+  uint8_t *X = xt + d[D_XT];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {
+    int64_t s = s0 + r, c = c0 + tx;
+    tile[r][tx] = (s < N && c < L) ? ax[s * L + c] : (uint8_t)4;
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {
+    uint8_t a = tile[tx][r];
+    X[(c0 + r) * Npad + s0 + tx] = (a < 4) ? (uint8_t)(1u << a) : (uint8_t)0;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_pm_b(const int64_t *__restrict__ desc, int n, const uint32_t *__restrict__ cnt, double *__restrict__ pm,
+       double tol, int *__restrict__ flag)
+{
+  const int it = find_item(desc, n, D_BCOL, blockIdx.x);
+  const int64_t *d = desc + it * RSG_BDESC;
+  const int64_t L = d[D_L];
+  pm_column<CNT_I8G>(cnt + d[D_CNT], L, blockIdx.x - d[D_BCOL], tol, pm + d[D_PM], pm + d[D_PM] + 4 * L, flag);
+}
+
+__global__ void __launch_bounds__(256)
+k_stat_b(const int64_t *__restrict__ desc, int n, StatArgs A0, const uint32_t *__restrict__ cnt,
+         const double *__restrict__ pm, double *__restrict__ cov, const int *__restrict__ cls, int base)
+{
+  const int it = find_item(desc, n, D_BSTAT, blockIdx.x);
+  const int64_t *d = desc + it * RSG_BDESC;
+  const int64_t L = d[D_L], local = blockIdx.x - d[D_BSTAT], nb = d[D_NB32];
+  StatArgs A = A0;
+  A.cnt = cnt + d[D_CNT]; A.pm = pm + d[D_PM]; A.lpm = pm + d[D_PM] + 4 * L; A.L = L;
+  A.wsum = (double)d[D_N]; A.cov = cov + d[D_COV]; A.type = base; A.cls = cls[it];
+  stat_tile<double, CNT_I8G, true>(A, (int)(local % nb), (int)(local / nb), (int64_t)blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256)
+k_rowsum_b(const int64_t *__restrict__ desc, int n, const double *__restrict__ cov, double *__restrict__ rows)
+{
+  __shared__ double red[256];
+  const int it = find_item(desc, n, D_BCOL, blockIdx.x);
+  const int64_t *d = desc + it * RSG_BDESC;
+  const int64_t L = d[D_L], i = blockIdx.x - d[D_BCOL];
+  const double *C = cov + d[D_COV];
+  double s = 0.;
+  for (int64_t j = threadIdx.x; j < L; j += 256) if (j != i) s += C[i * L + j];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) { if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st]; __syncthreads(); }
+  if (threadIdx.x == 0) rows[d[D_ROWS] + L + i] = red[0];
+}
+
+__global__ void __launch_bounds__(256)
+k_rowmean_b(const int64_t *__restrict__ desc, double *__restrict__ rows)
+{
+  __shared__ double red[256];
+  const int64_t *d = desc + blockIdx.x * RSG_BDESC;
+  const int64_t L = d[D_L];
+  double *R = rows + d[D_ROWS];
+  double s = 0.;
+  for (int64_t i = threadIdx.x; i < L; i += 256) { double r = R[L + i]; s += r; R[i] = (L > 1) ? r / ((double)L - 1.) : r; }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) { if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st]; __syncthreads(); }
+  if (threadIdx.x == 0) R[2 * L] = (L > 1) ? red[0] / ((double)L * ((double)L - 1.)) : red[0];
+}
+
+// APC / ASC in place on both triangles; block min / max to part[blockIdx]
+__global__ void __launch_bounds__(256)
+k_correct_b(const int64_t *__restrict__ desc, int n, double *__restrict__ cov, const double *__restrict__ rows,
+            int actype, int *__restrict__ flag, double *__restrict__ part)
+{
+  __shared__ double rmin[256], rmax[256];
+  const int it = find_item(desc, n, D_BEL, blockIdx.x);
+  const int64_t *d = desc + it * RSG_BDESC;
+  const int64_t L = d[D_L], e0 = (blockIdx.x - d[D_BEL]) * ELPB;
+  double *C = cov + d[D_COV];
+  const double *R = rows + d[D_ROWS];
+  const double avg = R[2 * L];
+  double a = INFINITY, b = -INFINITY;
+  for (int64_t e = e0 + threadIdx.x; e < min(e0 + ELPB, L * L); e += 256) {
+    int64_t i = e / L, j = e - i * L;
+    if (i == j) continue;
+    double v = C[e];
+    double r = (actype == RSG_APC) ? ((avg != 0.0) ? v - R[i] * R[j] / avg : 0.0) : v - (R[i] + R[j] - avg);
+    if (isnan(r)) atomicOr(flag, 1);
+    C[e] = r;
+    a = fmin(a, r); b = fmax(b, r);
+  }
+  rmin[threadIdx.x] = a; rmax[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) { rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + s]); rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + s]); }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rmin[0]; part[2 * blockIdx.x + 1] = rmax[0]; }
+}
+
+// per item: reduce block partials [p0, p1) to (min, max)
+__global__ void __launch_bounds__(256)
+k_item_minmax(const int64_t *__restrict__ desc, int field, int64_t nblk_total, int n, const double *__restrict__ part,
+              double *__restrict__ mm)
+{
+  __shared__ double rmin[256], rmax[256];
+  const int it = blockIdx.x;
+  const int64_t p0 = desc[it * RSG_BDESC + field];
+  const int64_t p1 = (it + 1 < n) ? desc[(it + 1) * RSG_BDESC + field] : nblk_total;
+  double a = INFINITY, b = -INFINITY;
+  for (int64_t k = p0 + threadIdx.x; k < p1; k += 256) { a = fmin(a, part[2 * k]); b = fmax(b, part[2 * k + 1]); }
+  rmin[threadIdx.x] = a; rmax[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) { rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + s]); rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + s]); }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { mm[2 * it] = rmin[0]; mm[2 * it + 1] = rmax[0]; }
+}
+
+// histogram geometry per item: calculate_width_histo (w) then cov_CreateRankList bounds
+struct HGeo { double w, bmax, xmin, xmax; int nb, imin, imax, ok; };
+
+__global__ void k_width_b(int n, const double *__restrict__ mm, double bmin, double w0, int hpts, double tol,
+                          HGeo *__restrict__ geo)
+{
+  int it = blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= n) return;
+  HGeo g; g.ok = 0;
+  double mn = mm[2 * it], mx = mm[2 * it + 1];
+  double lo = fmax(bmin, mn);
+  double w = fmin(w0, (mx - lo) / (double)hpts);
+  if (w < tol) w = 0.;
+  g.w = w; g.bmax = 0.; g.xmin = g.xmax = 0.; g.nb = 0; g.imin = 0; g.imax = -1;
+  if (w > 0. && mx > bmin) {
+    double bmax = mx + 5 * w;
+    while (fabs(bmax - bmin) < tol) bmax += w;
+    double nbd = (bmax - bmin) / w;
+    if (nbd < (double)NBMAX) {
+      g.nb = (int)nbd; g.bmax = bmax;
+      g.xmin = fmax(mn, bmin + w); g.xmax = fmax(mx, bmin + w);
+      g.imin = (int)ceil((g.xmin - bmin) / w - 1.); g.imax = (int)ceil((g.xmax - bmin) / w - 1.);
+      g.ok = (g.imin >= 0 && g.imax < g.nb) ? 1 : 0;
+    }
+  }
+  geo[it] = g;
+}
+
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, int64_t n)
+{
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0ull;
+}
+
+__global__ void k_fill_int(int *__restrict__ p, int n)
+{
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0;
+}
+
+__global__ void __launch_bounds__(256)
+k_hist_b(const int64_t *__restrict__ desc, int n, const double *__restrict__ cov, const HGeo *__restrict__ geo,
+         double bmin, unsigned long long *__restrict__ hist)
+{
+  const int it = find_item(desc, n, D_BEL, blockIdx.x);
+  const HGeo g = geo[it];
+  if (!g.ok) return;
+  const int64_t *d = desc + it * RSG_BDESC;
+  const int64_t L = d[D_L], e0 = (blockIdx.x - d[D_BEL]) * ELPB;
+  const double *C = cov + d[D_COV];
+  unsigned long long *H = hist + (int64_t)it * NBMAX;
+  const double fl = bmin + g.w;
+  for (int64_t e = e0 + threadIdx.x; e < min(e0 + ELPB, L * L); e += 256) {
+    int64_t i = e / L, j = e - i * L;
+    if (i >= j) continue;
+    double x = C[e];
+    double add = (x > fl) ? x : fl;
+    int b = (int)ceil(((add - bmin) / g.w) - 1.);
+    b = (b < 0) ? 0 : ((b >= g.nb) ? g.nb - 1 : b);
+    atomicAdd(&H[b], 1ull);
+  }
+}
+
+// tail fit per item (one 1024-thread block): see rsg_rank.hip for the method
+
+__global__ void __launch_bounds__(1024)
+k_fit_b(const int64_t *__restrict__ desc, const HGeo *__restrict__ geo, const unsigned long long *__restrict__ hist,
+        unsigned long long *__restrict__ S, int *__restrict__ NZ, double bmin, double pmass, double fracfit, int doexp,
+        double *__restrict__ fits)
+{
+  const int it = blockIdx.x;
+  const HGeo g = geo[it];
+  double *o = fits + it * 8;
+  if (!g.ok) {
+    if (threadIdx.x == 0) { o[0] = NAN; o[1] = INFINITY; o[2] = 0.; o[3] = 0.; o[4] = NAN; o[5] = g.w; }
+    return;
+  }
+  const int64_t L = desc[it * RSG_BDESC + D_L];
+  const double npairs = (double)(L * (L - 1) / 2);
+  nullfit_body(hist + (int64_t)it * NBMAX, g.nb, g.imin, g.imax, npairs, npairs, bmin, g.w, pmass, fracfit, doexp,
+               S + (int64_t)it * NBMAX, NZ + (int64_t)it * NBMAX, nullptr, o);
+  if (threadIdx.x == 0) o[5] = g.w;
+}
+
+__global__ void k_logtab_b(int64_t NM, double *__restrict__ t)
+{
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= NM) { t[k] = log((double)k + 1e-10); t[NM + 1 + k] = log((double)k + 16e-10); }
+}
+
+__global__ void k_copy_mm(int n, const double *__restrict__ mm, double *__restrict__ fits)
+{
+  int it = blockIdx.x * blockDim.x + threadIdx.x;
+  if (it < n) { fits[it * 8 + 6] = mm[2 * it]; fits[it * 8 + 7] = mm[2 * it + 1]; }
+}
+
+// ------------------------------------------------------------------------------- host side
+namespace {
+struct Dev {
+  std::vector<void *> ptrs;
+  template <typename T> T *get(size_t count, char *errbuf, int *rc)
+  {
+    void *p = nullptr;
+    if (*rc != RSG_OK) return nullptr;
+    if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+      *rc = rsg_fail(errbuf, "batch: out of device memory (%zu bytes)", count * sizeof(T));
+      return nullptr;
+    }
+    ptrs.push_back(p);
+    return (T *)p;
+  }
+  ~Dev() { for (void *p : ptrs) hipFree(p); }
+};
+}
+
+extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int covtype, double bmin, double pmass,
+                              double fracfit, int doexpfit, rsg_hist **hists, double *fits, int use_graph, char *errbuf)
+{
+  HIPCHK(hipSetDevice(c->device));
+  if (nj < 1) return rsg_fail(errbuf, "empty batch");
+  const int base = covtype - covtype % 3, corr = covtype % 3;
+  if (base == RSG_RAF || base == RSG_RAFS || base == RSG_CCF)
+    return rsg_fail(errbuf, "batched null supports CHI/GT/MI/MIr/MIg/OMES statistics");
+  // ---- layout
+  std::vector<int64_t> desc((size_t)nj * RSG_BDESC, 0);
+  std::vector<SimItem> sim((size_t)nj);
+  std::vector<int> cls((size_t)nj);
+  int64_t o_xt = 0, o_cnt = 0, o_cov = 0, o_pm = 0, o_rows = 0, o_leaf = 0, nodes = 0, seqs = 0;
+  int64_t b_pack = 0, b_col = 0, b_stat = 0, b_el = 0, b_sim = 0, nmax = 1;
+  std::vector<int4> tiles;
+  for (int k = 0; k < nj; k++) {
+    const rsg_null_job &J = jobs[k];
+    if (J.nleaf < 2 || J.alen < 2) return rsg_fail(errbuf, "job %d: needs >= 2 sequences and >= 2 columns", k);
+    const int64_t N = J.nleaf, L = J.alen, Npad = rsg_npad(N), Lpad = ((L + 63) / 64) * 64;
+    int64_t *d = &desc[(size_t)k * RSG_BDESC];
+    d[D_N] = N; d[D_L] = L; d[D_NPAD] = Npad; d[D_LPAD] = Lpad;
+    d[D_XT] = o_xt; d[D_CNT] = o_cnt; d[D_COV] = o_cov; d[D_PM] = o_pm; d[D_ROWS] = o_rows;
+    d[D_HIST] = (int64_t)k * NBMAX; d[D_LEAF] = o_leaf;
+    d[D_BPACK] = b_pack; d[D_BCOL] = b_col; d[D_BSTAT] = b_stat; d[D_BEL] = b_el;
+    const int64_t nb32 = (L + 31) / 32;
+    d[D_NB32] = nb32;
+    cls[(size_t)k] = (N <= 8 || L <= 50) ? RSG_C2 : RSG_C16;           // CSELECT (correlators.c:335-341)
+    SimItem &s = sim[(size_t)k];
+    s.N = N; s.L = L; s.tree_off = nodes; s.seq_off = seqs; s.leaf_off = o_leaf; s.seed = J.seed; s.stream = J.stream;
+    s.gapthr = (J.gapfrac <= 0.) ? 0u : (J.gapfrac >= 1. ? 0xFFFFFFFFu : (uint32_t)(J.gapfrac * 4294967296.0));
+    s.col_blk0 = b_sim;
+    for (int64_t bi = 0; bi < nb32; bi++)
+      for (int64_t bj = bi; bj < nb32; bj++) tiles.push_back(make_int4(k, (int)bi, (int)bj, 0));
+    o_xt += Lpad * Npad; o_cnt += 16 * L * L; o_cov += L * L; o_pm += 8 * L; o_rows += 2 * L + 2; o_leaf += N * L;
+    nodes += N - 1; seqs += (2 * N - 1) * L;
+    b_pack += (Lpad / 64) * (Npad / 64); b_col += L; b_stat += nb32 * nb32; b_el += (L * L + ELPB - 1) / ELPB;
+    b_sim += (L + 63) / 64;
+    nmax = std::max(nmax, N);
+  }
+  // longest K loops first so the tail of the launch is short work
+  std::stable_sort(tiles.begin(), tiles.end(), [&](const int4 &a, const int4 &b) {
+    return desc[(size_t)a.x * RSG_BDESC + D_NPAD] > desc[(size_t)b.x * RSG_BDESC + D_NPAD];
+  });
+  std::vector<int> left((size_t)nodes), right((size_t)nodes);
+  std::vector<double> ld((size_t)nodes), rd((size_t)nodes);
+  for (int k = 0; k < nj; k++) {
+    const int64_t off = sim[(size_t)k].tree_off, m1 = jobs[k].nleaf - 1;
+    memcpy(&left[(size_t)off], jobs[k].left, sizeof(int) * m1);
+    memcpy(&right[(size_t)off], jobs[k].right, sizeof(int) * m1);
+    memcpy(&ld[(size_t)off], jobs[k].ld, sizeof(double) * m1);
+    memcpy(&rd[(size_t)off], jobs[k].rd, sizeof(double) * m1);
+  }
+  // ---- device buffers
+  int rc = RSG_OK;
+  Dev D;
+  int64_t *d_desc = D.get<int64_t>(desc.size(), errbuf, &rc);
+  SimItem *d_sim = D.get<SimItem>((size_t)nj, errbuf, &rc);
+  int *d_cls = D.get<int>((size_t)nj, errbuf, &rc);
+  int4 *d_tiles = D.get<int4>(tiles.size(), errbuf, &rc);
+  int *d_left = D.get<int>((size_t)nodes, errbuf, &rc), *d_right = D.get<int>((size_t)nodes, errbuf, &rc);
+  double *d_ld = D.get<double>((size_t)nodes, errbuf, &rc), *d_rd = D.get<double>((size_t)nodes, errbuf, &rc);
+  double *d_model = D.get<double>(40, errbuf, &rc);
+  uint32_t *d_cdf = D.get<uint32_t>((size_t)nodes * 24, errbuf, &rc);
+  uint8_t *d_seqs = D.get<uint8_t>((size_t)seqs, errbuf, &rc);
+  uint8_t *d_leaves = D.get<uint8_t>((size_t)o_leaf, errbuf, &rc);
+  uint8_t *d_xt = D.get<uint8_t>((size_t)o_xt, errbuf, &rc);
+  uint32_t *d_cnt = D.get<uint32_t>((size_t)o_cnt, errbuf, &rc);
+  double *d_cov = D.get<double>((size_t)o_cov, errbuf, &rc);
+  double *d_pm = D.get<double>((size_t)o_pm, errbuf, &rc);
+  double *d_rows = D.get<double>((size_t)o_rows, errbuf, &rc);
+  double *d_lgc = D.get<double>(2 * (size_t)(nmax + 1), errbuf, &rc);
+  double *d_part_s = D.get<double>(2 * (size_t)b_stat, errbuf, &rc);
+  double *d_part_e = D.get<double>(2 * (size_t)b_el, errbuf, &rc);
+  double *d_mm = D.get<double>(2 * (size_t)nj, errbuf, &rc);
+  HGeo *d_geo = D.get<HGeo>((size_t)nj, errbuf, &rc);
+  unsigned long long *d_hist = D.get<unsigned long long>((size_t)nj * NBMAX, errbuf, &rc);
+  unsigned long long *d_S = D.get<unsigned long long>((size_t)nj * NBMAX, errbuf, &rc);
+  int *d_NZ = D.get<int>((size_t)nj * NBMAX, errbuf, &rc);
+  double *d_fits = D.get<double>((size_t)nj * 8, errbuf, &rc);
+  if (rc != RSG_OK) return rc;
+  HIPCHK(hipMemcpyAsync(d_desc, desc.data(), sizeof(int64_t) * desc.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_sim, sim.data(), sizeof(SimItem) * nj, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_cls, cls.data(), sizeof(int) * nj, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_left, left.data(), sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_right, right.data(), sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_ld, ld.data(), sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_rd, rd.data(), sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_model, jobs[0].model, sizeof(double) * 40, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 1; k < nj; k++)
+    if (memcmp(jobs[k].model, jobs[0].model, sizeof(double) * 40))
+      return rsg_fail(errbuf, "all jobs of a batch share one substitution model");
+
+  StatArgs A0;
+  memset(&A0, 0, sizeof(A0));
+  A0.lgc = d_lgc; A0.N = nmax; A0.part = d_part_s;
+  for (int k = 0; k < 16; k++) {                     // WC + GU (src/R-scape.c:882-887)
+    int x = k / 4, y = k % 4;
+    if ((x == 0 && y == 3) || (x == 3 && y == 0) || (x == 1 && y == 2) || (x == 2 && y == 1) ||
+        (x == 2 && y == 3) || (x == 3 && y == 2)) A0.allow |= 1u << k;
+  }
+
+  auto launch_all = [&]() -> int {
+    if (rsg_sim_launch(c, d_sim, nj, nodes, b_sim, d_left, d_right, d_ld, d_rd, d_model, d_cdf, d_seqs, d_leaves) != RSG_OK) return RSG_FAIL;
+    hipLaunchKernelGGL(k_pack_b, dim3((unsigned)b_pack), dim3(256), 0, c->stream, d_desc, nj, d_leaves, d_xt);
+    if (rsg_gram_batch_launch(c, d_tiles, (int)tiles.size(), d_desc, d_xt, d_cnt) != RSG_OK) return RSG_FAIL;
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_flag, 16);
+    hipLaunchKernelGGL(k_logtab_b, dim3((unsigned)((nmax + 256) / 256)), dim3(256), 0, c->stream, nmax, d_lgc);
+    hipLaunchKernelGGL(k_pm_b, dim3((unsigned)b_col), dim3(256), 0, c->stream, d_desc, nj, d_cnt, d_pm, 1e-6, c->d_flag);
+    hipLaunchKernelGGL(k_stat_b, dim3((unsigned)b_stat), dim3(256), 0, c->stream, d_desc, nj, A0, d_cnt, d_pm, d_cov, d_cls, base);
+    if (corr) {
+      hipLaunchKernelGGL(k_rowsum_b, dim3((unsigned)b_col), dim3(256), 0, c->stream, d_desc, nj, d_cov, d_rows);
+      hipLaunchKernelGGL(k_rowmean_b, dim3((unsigned)nj), dim3(256), 0, c->stream, d_desc, d_rows);
+      hipLaunchKernelGGL(k_correct_b, dim3((unsigned)b_el), dim3(256), 0, c->stream, d_desc, nj, d_cov, d_rows,
+                         corr == 1 ? RSG_APC : RSG_ASC, c->d_flag + 1, d_part_e);
+      hipLaunchKernelGGL(k_item_minmax, dim3((unsigned)nj), dim3(256), 0, c->stream, d_desc, (int)D_BEL, b_el, nj, d_part_e, d_mm);
+    } else {
+      hipLaunchKernelGGL(k_item_minmax, dim3((unsigned)nj), dim3(256), 0, c->stream, d_desc, (int)D_BSTAT, b_stat, nj, d_part_s, d_mm);
+    }
+    hipLaunchKernelGGL(k_width_b, dim3((unsigned)((nj + 63) / 64)), dim3(64), 0, c->stream, nj, d_mm, bmin, 0.05, 400, 1e-6, d_geo);
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)(((int64_t)nj * NBMAX + 255) / 256)), dim3(256), 0, c->stream, d_hist, (int64_t)nj * NBMAX);
+    hipLaunchKernelGGL(k_hist_b, dim3((unsigned)b_el), dim3(256), 0, c->stream, d_desc, nj, d_cov, d_geo, bmin, d_hist);
+    hipLaunchKernelGGL(k_fit_b, dim3((unsigned)nj), dim3(1024), 0, c->stream, d_desc, d_geo, d_hist, d_S, d_NZ,
+                       bmin, pmass, fracfit, doexpfit, d_fits);
+    hipLaunchKernelGGL(k_copy_mm, dim3((unsigned)((nj + 63) / 64)), dim3(64), 0, c->stream, nj, d_mm, d_fits);
+    return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
+  };
+
+  int t; rsg_tstart(c, "null_batch", &t);
+  if (use_graph) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int lr = launch_all();
+    hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    if (lr != RSG_OK || ec != hipSuccess) { if (graph) hipGraphDestroy(graph); return rsg_fail(errbuf, "batch graph capture failed"); }
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) { hipGraphDestroy(graph); return rsg_fail(errbuf, "graph instantiate failed"); }
+    hipError_t el = hipSuccess, es = hipSuccess;
+    for (int rep = 0; rep < use_graph && el == hipSuccess && es == hipSuccess; rep++) {
+      el = hipGraphLaunch(exec, c->stream);                 // every replay recomputes from scratch
+      if (rep + 1 < use_graph) es = hipStreamSynchronize(c->stream);
+    }
+    rsg_tstop(c, t);
+    if (es == hipSuccess) es = hipStreamSynchronize(c->stream);
+    hipGraphExecDestroy(exec); hipGraphDestroy(graph);
+    if (el != hipSuccess || es != hipSuccess) return rsg_fail(errbuf, "batch graph launch failed");
+  } else {
+    if (launch_all() != RSG_OK) return rsg_fail(errbuf, "batch launch failed");
+    rsg_tstop(c, t);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  int flags[2];
+  HIPCHK(hipMemcpy(flags, c->d_flag, sizeof(flags), hipMemcpyDeviceToHost));
+  if (flags[0]) return rsg_fail(errbuf, "pm validation failed in a batch item");
+  if (flags[1]) return rsg_fail(errbuf, "bad covariation (NaN after correction) in a batch item");
+  HIPCHK(hipMemcpy(fits, d_fits, sizeof(double) * 8 * nj, hipMemcpyDeviceToHost));
+  if (hists) {
+    std::vector<HGeo> geo((size_t)nj);
+    HIPCHK(hipMemcpy(geo.data(), d_geo, sizeof(HGeo) * nj, hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> h((size_t)NBMAX);
+    for (int k = 0; k < nj; k++) {
+      hists[k] = nullptr;
+      const HGeo &g = geo[(size_t)k];
+      if (!g.ok) continue;
+      rsg_hist *H = rsg_hist_create(bmin, g.bmax, g.w);
+      HIPCHK(hipMemcpy(h.data(), d_hist + (size_t)k * NBMAX, sizeof(unsigned long long) * g.nb, hipMemcpyDeviceToHost));
+      for (int b = 0; b < g.nb; b++) H->obs[b] = h[(size_t)b];
+      const int64_t L = desc[(size_t)k * RSG_BDESC + D_L];
+      H->n = H->Nc = H->No = (uint64_t)(L * (L - 1) / 2);
+      H->imin = H->cmin = g.imin; H->imax = g.imax; H->xmin = g.xmin; H->xmax = g.xmax;
+      hists[k] = H;
+    }
+  }
+  return RSG_OK;
+}

@@ -1,0 +1,390 @@
+// rsg_device.h -- device helpers shared by the single-alignment and batched kernels.
+#pragma once
+#include "rsg_internal.h"
+#include <cmath>
+
+template <int KIND> __device__ __forceinline__ void load_rec(const void *cnt, int64_t rec, double *c)
+{
+  if (KIND == CNT_I8G) {
+    const uint4 *p = (const uint4 *)((const uint32_t *)cnt + rec * 16);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      uint4 v = p[k];                               // row x = k: cells y = 0..3, scale 2^(x+y)
+      c[4 * k] = (double)(v.x >> (k + 0)); c[4 * k + 1] = (double)(v.y >> (k + 1));
+      c[4 * k + 2] = (double)(v.z >> (k + 2)); c[4 * k + 3] = (double)(v.w >> (k + 3));
+    }
+  } else {
+    const float4 *p = (const float4 *)((const float *)cnt + rec * 16);
+#pragma unroll
+    for (int k = 0; k < 4; k++) { float4 v = p[k]; c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w; }
+  }
+}
+
+
+// per-pair statistics
+struct StatArgs {
+  const void *cnt; const double *pm; const double *lpm; const double *lgc;
+  int64_t L, N; double wsum; double *cov; double *part;  // part: 2 per block (min, max)
+  int type, cls; unsigned allow;                           // allow: 16-bit mask of allowed cells
+  double meanp[4];                                         // CCF
+};
+
+template <typename T> __device__ __forceinline__ T tlog(T v);
+template <> __device__ __forceinline__ double tlog<double>(double v) { return log(v); }
+template <> __device__ __forceinline__ float tlog<float>(float v) { return logf(v); }
+
+// One pair. c: counts, pmi/pmj: marginals.  Returns the statistic (see the file header).
+template <typename T, bool TAB>
+__device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int64_t j)
+{
+  T c[16], ns = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) { c[k] = (T)cd[k]; ns += c[k]; }
+  double nsd = 0.;
+#pragma unroll
+  for (int k = 0; k < 16; k++) nsd += cd[k];
+  T pmi[4], pmj[4], lpi[4], lpj[4];
+#pragma unroll
+  for (int x = 0; x < 4; x++) {
+    pmi[x] = (T)A.pm[i * 4 + x]; pmj[x] = (T)A.pm[j * 4 + x];
+    lpi[x] = (T)A.lpm[i * 4 + x]; lpj[x] = (T)A.lpm[j * 4 + x];
+  }
+  const T den = ns + (T)16e-10;
+  T lden;
+  if (TAB) lden = (T)A.lgc[A.N + 1 + (int64_t)nsd]; else lden = tlog<T>(den);
+  T pp[16], lpp[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    pp[k] = (c[k] + (T)1e-10) / den;
+    if (TAB) lpp[k] = (T)A.lgc[(int64_t)cd[k]] - lden; else lpp[k] = tlog<T>(c[k] + (T)1e-10) - lden;
+  }
+  const int type = A.type;
+  if (type == RSG_CCF) {
+    T sa = 0, sb = 0;
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      T a = ns * pmi[x] - (T)A.meanp[x], b = ns * pmj[x] - (T)A.meanp[x];
+      sa += a * a; sb += b * b;
+    }
+    return (double)sqrt((double)(sa * sb));
+  }
+  if (A.cls == RSG_C2) {
+    T pw = 0, pn = 0, qw = 0, qn = 0;
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        if ((A.allow >> (x * 4 + y)) & 1u) { pw += pp[x * 4 + y]; qw += pmi[x] * pmj[y]; }
+        else                               { pn += pp[x * 4 + y]; qn += pmi[x] * pmj[y]; }
+      }
+    T v = 0, H = 0;
+    switch (type) {
+    case RSG_CHI:
+      if (ns > 0) { if (qw > 0) v += ns * (pw - qw) * (pw - qw) / qw; if (qn > 0) v += ns * (pn - qn) * (pn - qn) / qn; }
+      break;
+    case RSG_OMES:
+      if (ns > 0) { if (qw > 0) v += ns * (pw - qw) * (pw - qw); if (qn > 0) v += ns * (pn - qn) * (pn - qn); }
+      break;
+    case RSG_GT:
+      if (ns > 0) {
+        if (qw > 0 && pw > 0) v += pw * (tlog<T>(pw) - tlog<T>(qw));
+        if (qn > 0 && pn > 0) v += pn * (tlog<T>(pn) - tlog<T>(qn));
+        v *= 2 * ns;
+      }
+      break;
+    case RSG_MI: case RSG_MIg:
+      if (pw > 0) v += pw * (tlog<T>(pw) - tlog<T>(qw));
+      if (pn > 0) v += pn * (tlog<T>(pn) - tlog<T>(qn));
+      if (type == RSG_MIg && ns > 0) v -= (T)((A.wsum - nsd) / nsd);
+      break;
+    case RSG_MIr:
+      if (pw > 0) H -= pw * tlog<T>(pw);
+      if (pn > 0) H -= pn * tlog<T>(pn);
+      if (pw > 0 && qw > 0) v += pw * (tlog<T>(pw) - tlog<T>(qw));
+      if (pn > 0 && qn > 0) v += pn * (tlog<T>(pn) - tlog<T>(qn));
+      v = (H > (T)1e-2) ? v / H : 0;
+      break;
+    }
+    return (double)v;
+  }
+  // C16 / CWC
+  T s = 0, H = 0;
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const int k = x * 4 + y;
+      const T q = pmi[x] * pmj[y];
+      switch (type) {
+      case RSG_CHI:  if (q > 0) s += (pp[k] - q) * (pp[k] - q) / q; break;
+      case RSG_OMES: if (q > 0) s += (pp[k] - q) * (pp[k] - q); break;
+      case RSG_GT:
+        if (A.cls == RSG_CWC && !((A.allow >> k) & 1u)) break;
+        if (q > 0) s += pp[k] * (lpp[k] - lpi[x] - lpj[y]);
+        break;
+      case RSG_MI: case RSG_MIg: case RSG_MIr:
+        if (pmi[x] > 0 && pmj[y] > 0) s += pp[k] * (lpp[k] - lpi[x] - lpj[y]);
+        if (type == RSG_MIr) H -= pp[k] * lpp[k];
+        break;
+      }
+    }
+  switch (type) {
+  case RSG_CHI: case RSG_OMES: return (ns > 0) ? (double)(ns * s) : 0.;
+  case RSG_GT:   return (ns > 0) ? 2. * (double)ns * (double)s : 0.;
+  case RSG_MI:   return (double)s;
+  case RSG_MIg:  return (double)s - ((nsd > 0) ? (A.wsum - nsd) / nsd : 0.);
+  case RSG_MIr:  return (H > (T)1e-2) ? (double)(s / H) : 0.;
+  }
+  return 0.;
+}
+
+// corr_Marginals for one column i (one 256-thread block; fixed-order reduction).
+template <int KIND>
+__device__ void pm_column(const void *__restrict__ cnt, int64_t L, int64_t i, double tol,
+                          double *__restrict__ pm, double *__restrict__ lpm, int *__restrict__ flag)
+{
+  __shared__ double red[4][256];
+  double acc[4] = {0., 0., 0., 0.};
+  for (int64_t j = threadIdx.x; j < L; j += 256) {
+    if (j == i) continue;
+    int64_t a = (i < j) ? i : j, b = (i < j) ? j : i;
+    double c[16];
+    load_rec<KIND>(cnt, rsg_rec(a, b, L), c);
+    double ns = 0.;
+#pragma unroll
+    for (int k = 0; k < 16; k++) ns += c[k];
+    if (ns > 0) {
+      double inv = 1. / (ns + 16e-10);
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        double r = (i < j) ? (c[x * 4] + c[x * 4 + 1] + c[x * 4 + 2] + c[x * 4 + 3])
+                           : (c[x] + c[4 + x] + c[8 + x] + c[12 + x]);
+        acc[x] += (r + 4e-10) * inv;
+      }
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 4; x++) red[x][threadIdx.x] = acc[x];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int x = 0; x < 4; x++) red[x][threadIdx.x] += red[x][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double v[4] = {red[0][0], red[1][0], red[2][0], red[3][0]};
+    double s = v[0] + v[1] + v[2] + v[3];
+    double tot = 0.;
+    for (int x = 0; x < 4; x++) {
+      v[x] = (s != 0.) ? v[x] / s : 0.25;
+      if (!(v[x] >= 0. && v[x] <= 1.)) atomicOr(flag, 1);
+      tot += v[x];
+      pm[i * 4 + x] = v[x];
+      lpm[i * 4 + x] = (v[x] > 0.) ? log(v[x]) : -INFINITY;
+    }
+    if (fabs(tot - 1.) > tol) atomicOr(flag, 1);
+  }
+}
+
+// 32 x 32 pair tile; COV written to both triangles through an LDS transpose; block min/max
+// to part[2*part_idx].
+template <typename T, int KIND, bool TAB>
+__device__ void stat_tile(const StatArgs &A, int bi, int bj, int64_t part_idx)
+{
+  __shared__ double tile[32][33];
+  __shared__ double rmin[256], rmax[256];
+  const int tid = threadIdx.x;
+  double lmin = INFINITY, lmax = -INFINITY;
+  if (bi <= bj) {
+    // phase 1: consecutive threads take consecutive i (consecutive records j*L + i),
+    // write row j of COV; phase 2 writes row i from the LDS transpose.
+    const int ii = tid & 31;
+    for (int k = 0; k < 4; k++) {
+      const int jj = (tid >> 5) + 8 * k;
+      const int64_t i = (int64_t)bi * 32 + ii, j = (int64_t)bj * 32 + jj;
+      double v = -INFINITY;
+      if (i < j && j < A.L) {
+        double cd[16];
+        load_rec<KIND>(A.cnt, rsg_rec(i, j, A.L), cd);
+        v = pair_stat<T, TAB>(A, cd, i, j);
+        lmin = fmin(lmin, v); lmax = fmax(lmax, v);
+        A.cov[j * A.L + i] = v;
+      } else if (i == j && i < A.L) {
+        A.cov[i * A.L + j] = -INFINITY;
+      }
+      tile[jj][ii] = v;
+    }
+    __syncthreads();
+    const int jj = tid & 31;
+    for (int k = 0; k < 4; k++) {
+      const int ii2 = (tid >> 5) + 8 * k;
+      const int64_t i = (int64_t)bi * 32 + ii2, j = (int64_t)bj * 32 + jj;
+      if (i < j && j < A.L) A.cov[i * A.L + j] = tile[jj][ii2];
+    }
+  }
+  rmin[tid] = lmin; rmax[tid] = lmax;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) { rmin[tid] = fmin(rmin[tid], rmin[tid + s]); rmax[tid] = fmax(rmax[tid], rmax[tid + s]); }
+    __syncthreads();
+  }
+  if (tid == 0) { A.part[2 * part_idx] = rmin[0]; A.part[2 * part_idx + 1] = rmax[0]; }
+}
+
+// ---- FP64 null tail fit (cov_histogram_pmass, SetTailByMass, binned exponential / Gamma ML,
+// SetSurvFitTail); one work group; out = {mu, lambda, tau, newmass, pmass, phi, cmin, No}.
+#include <cfloat>
+#include <climits>
+__device__ double d_digamma(double x)
+{
+  double r = 0.;
+  while (x < 6.) { r -= 1. / x; x += 1.; }
+  double f = 1. / (x * x);
+  return r + log(x) - 0.5 / x - f * (1. / 12 - f * (1. / 120 - f * (1. / 252 - f * (1. / 240 - f / 132))));
+}
+
+// Regularized upper incomplete gamma (series / Lentz continued fraction).
+__device__ double d_gammq(double a, double x)
+{
+  if (x <= 0.) return 1.0;
+  double gln = lgamma(a);
+  if (x < a + 1.) {
+    double ap = a, sum = 1. / a, del = sum;
+    for (int n = 0; n < 1000; n++) { ap += 1.; del *= x / ap; sum += del; if (fabs(del) < fabs(sum) * 1e-16) break; }
+    return 1. - sum * exp(-x + a * log(x) - gln);
+  }
+  double b = x + 1. - a, cc = 1. / DBL_MIN, d = 1. / b, hh = d;
+  for (int i = 1; i < 1000; i++) {
+    double an = -i * (i - a);
+    b += 2.;
+    d = an * d + b; if (fabs(d) < DBL_MIN) d = DBL_MIN;
+    cc = b + an / cc; if (fabs(cc) < DBL_MIN) cc = DBL_MIN;
+    d = 1. / d;
+    double del = d * cc; hh *= del;
+    if (fabs(del - 1.) < 1e-16) break;
+  }
+  return exp(-x + a * log(x) - gln) * hh;
+}
+
+// Block-wide reductions over 1024 threads in a fixed order (deterministic).
+template <typename T, typename Op>
+__device__ T block_reduce(T v, T *sh, Op op)
+{
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = op(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  T r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// One work group.  S[b] = sum_{i >= b} obs[i] and NZ[b] = #nonzero bins >= b over [imin, imax]
+// (suffix sums), so the reference's downward scans become "largest bin satisfying a monotone
+// condition" reductions.
+__device__ void nullfit_body(const unsigned long long *obs, int nb, int imin, int imax, double n, double Nc,
+                             double bmin, double w, double target_pmass, double fracfit, int doexp,
+                             unsigned long long *S, int *NZ, double *survfit, double *out)
+{
+#pragma clang fp contract(off)   // bin bounds and sums rounded as the host computes them (no FMA)
+  __shared__ double shd[1024];
+  __shared__ long long shl[1024];
+  const int T = blockDim.x, tid = threadIdx.x;
+  // suffix sums by chunks: each thread owns a contiguous chunk of bins [imin, imax]
+  const int nbin = (imax >= imin) ? imax - imin + 1 : 0;
+  const int chunk = (nbin + T - 1) / T;
+  const int c0 = imin + tid * chunk, c1 = min(imin + (tid + 1) * chunk, imax + 1);
+  unsigned long long csum = 0; long long cnz = 0;
+  for (int b = c0; b < c1; b++) { csum += obs[b]; cnz += (obs[b] > 0); }
+  // exclusive suffix of chunk totals: sum over threads with larger index
+  shl[tid] = (long long)csum;
+  __syncthreads();
+  long long above = 0, abovenz = 0;
+  for (int k = tid + 1; k < T; k++) above += shl[k];
+  __syncthreads();
+  shl[tid] = cnz;
+  __syncthreads();
+  for (int k = tid + 1; k < T; k++) abovenz += shl[k];
+  __syncthreads();
+  unsigned long long run = (unsigned long long)above; long long runnz = abovenz;
+  for (int b = c1 - 1; b >= c0; b--) { run += obs[b]; runnz += (obs[b] > 0); S[b] = run; NZ[b] = (int)runnz; }
+  __syncthreads();
+  const long long tp = (nbin > 0) ? NZ[imin] : 0;
+  // cov_histogram_pmass: the largest nonzero bin where NZ/tp >= fracfit or S/Nc >= target
+  long long bsel = LLONG_MIN;
+  for (int b = imin + tid; b <= imax; b += T)
+    if (obs[b] > 0 && ((double)NZ[b] / (double)tp >= fracfit || (double)S[b] / Nc >= target_pmass))
+      bsel = max(bsel, (long long)b);
+  bsel = block_reduce<long long>(bsel, shl, [](long long a, long long b) { return a > b ? a : b; });
+  double pmass;
+  if (bsel == LLONG_MIN) {                       // the scan ran through: pmass of the last nonzero bin
+    long long lo = LLONG_MAX;
+    for (int b = imin + tid; b <= imax; b += T) if (obs[b] > 0) lo = min(lo, (long long)b);
+    lo = block_reduce<long long>(lo, shl, [](long long a, long long b) { return a < b ? a : b; });
+    pmass = (lo == LLONG_MAX) ? NAN : (double)S[lo] / Nc;
+  } else pmass = (double)S[bsel] / Nc;
+  // SetTailByMass: the largest b in [imin, imax] with S[b] >= pmass * n
+  long long bt = LLONG_MIN;
+  for (int b = imin + tid; b <= imax; b += T) if ((double)S[b] >= pmass * n) bt = max(bt, (long long)b);
+  bt = block_reduce<long long>(bt, shl, [](long long a, long long b) { return a > b ? a : b; });
+  long long cmin = (bt == LLONG_MIN) ? (long long)imin - 1 : bt;
+  unsigned long long kept = (bt == LLONG_MIN) ? ((nbin > 0) ? S[imin] : 0ull) : S[bt];
+  const double phi = w * (double)cmin + bmin;
+  const double newmass = (double)kept / n;
+  // fits over bins [cmin, imax]
+  double mu = phi, lambda = INFINITY, tau = 0.;
+  if (doexp) {
+    double sa = 0., sb = 0.;
+    for (int b = (int)max(cmin, (long long)imin) + tid; b <= imax; b += T) {
+      double ai = w * b + bmin, bi = w * (b + 1) + bmin;
+      sa += (double)obs[b] * (ai - mu); sb += (double)obs[b] * (bi - mu);
+    }
+    sa = block_reduce<double>(sa, shd, [](double a, double b) { return a + b; });
+    sb = block_reduce<double>(sb, shd, [](double a, double b) { return a + b; });
+    lambda = 1. / w * (log(sb) - log(sa));
+  } else {
+    double cnt = 0., mean = 0., lsum = 0.;
+    for (int b = (int)max(cmin, (long long)imin) + tid; b <= imax; b += T) {
+      if (!obs[b]) continue;
+      double x = 0.5 * ((w * b + bmin) + (w * (b + 1) + bmin)) - mu;
+      if (x <= 0.) continue;
+      cnt += (double)obs[b]; mean += (double)obs[b] * x; lsum += (double)obs[b] * log(x);
+    }
+    cnt  = block_reduce<double>(cnt, shd, [](double a, double b) { return a + b; });
+    mean = block_reduce<double>(mean, shd, [](double a, double b) { return a + b; });
+    lsum = block_reduce<double>(lsum, shd, [](double a, double b) { return a + b; });
+    if (cnt > 0.) {
+      mean /= cnt; lsum /= cnt;
+      double s = log(mean) - lsum;
+      if (s > 1e-12) {
+        double lo = 1e-6, hi = 1e6;
+        for (int it = 0; it < 200; it++) {
+          double mid = sqrt(lo * hi);
+          double f = log(mid) - d_digamma(mid) - s;
+          if (f > 0) lo = mid; else hi = mid;
+          if (hi / lo < 1. + 1e-14) break;
+        }
+        tau = sqrt(lo * hi);
+        lambda = tau / mean;
+      } else { lambda = INFINITY; tau = INFINITY; }
+    }
+  }
+  // survival fit: survfit[b] = newmass * S(UBound(b)) for b in [cmin, 2 nb)
+  const bool have = !isinf(lambda);
+  for (int b = tid; survfit && b < 2 * nb; b += T) {
+    double v = 0.;
+    if (have && b >= cmin) {
+      double x = w * (b + 1) + bmin;
+      double sv = (x < mu) ? 1.0 : (doexp ? exp(-lambda * (x - mu)) : d_gammq(tau, lambda * (x - mu)));
+      v = newmass * sv;
+    }
+    survfit[b] = v;
+  }
+  if (tid == 0) {
+    out[0] = mu; out[1] = lambda; out[2] = tau; out[3] = newmass; out[4] = pmass;
+    out[5] = phi; out[6] = (double)cmin; out[7] = (double)kept;
+  }
+}
+

@@ -107,22 +107,13 @@ struct CodeStage {
 // int8 Gram: WI x WJ waves, each 32 i-columns x 32 j-columns, KT sequences per stage.
 // Records are written as the raw accumulators (count << (x+y)), so the accumulators
 // never pass through VALU and stay in AGPRs.
+// One work-group tile (i0, j0) of one alignment; `lds` holds 2 * St::BYTES.
 template <int WI, int WJ, int KT>
-__global__ void __launch_bounds__(64 * WI * WJ)
-k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, uint32_t *__restrict__ out,
-          const int2 *__restrict__ tiles, int ntiles, int per_xcd)
+__device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L,
+                                             uint32_t *__restrict__ out, int64_t i0, int64_t j0, uint8_t *lds)
 {
   constexpr int BI = 32 * WI, BJ = 32 * WJ, NT = 64 * WI * WJ, KS = KT / 32;
   typedef CodeStage<BI, BJ, KT, NT> St;
-  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * St::BYTES];
-
-  // XCD-aware mapping: consecutive tiles of the (bi, bj)-sorted list run on one XCD (one L2).
-  const int g = blockIdx.x;
-  const int t = (g & 7) * per_xcd + (g >> 3);
-  if (t >= ntiles) return;
-  const int2 tl = tiles[t];
-  const int64_t i0 = (int64_t)tl.x * BI, j0 = (int64_t)tl.y * BJ;
-
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = wave % WI, wj = wave / WI;
   const int col = lane & 31, khalf = lane >> 5;
@@ -188,6 +179,42 @@ k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, uint32_t *__r
         p[x] = make_uint4(acc[x][0][r], acc[x][1][r], acc[x][2][r], acc[x][3][r]);
     }
   }
+}
+
+template <int WI, int WJ, int KT>
+__global__ void __launch_bounds__(64 * WI * WJ)
+k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, uint32_t *__restrict__ out,
+          const int2 *__restrict__ tiles, int ntiles, int per_xcd)
+{
+  typedef CodeStage<32 * WI, 32 * WJ, KT, 64 * WI * WJ> St;
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * St::BYTES];
+  // XCD-aware mapping: consecutive tiles of the (bi, bj)-sorted list run on one XCD (one L2).
+  const int g = blockIdx.x;
+  const int t = (g & 7) * per_xcd + (g >> 3);
+  if (t >= ntiles) return;
+  const int2 tl = tiles[t];
+  gram_tile_i8<WI, WJ, KT>(xt, Npad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds);
+}
+
+// Batched int8 Gram over many alignments: one work group per (item, tile), one wave per
+// 32 x 32-column tile; tiles are pre-sorted by decreasing sequence count (cost).
+__global__ void __launch_bounds__(64)
+k_gram_i8_batch(const int4 *__restrict__ tiles, const int64_t *__restrict__ desc, const uint8_t *__restrict__ xt,
+                uint32_t *__restrict__ cnt)
+{
+  typedef CodeStage<32, 32, 64, 64> St;
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * St::BYTES];
+  const int4 tl = tiles[blockIdx.x];                    // (item, bi, bj, -)
+  const int64_t *d = desc + (int64_t)tl.x * RSG_BDESC;  // item descriptor (see rsg_batch.hip)
+  gram_tile_i8<1, 1, 64>(xt + d[4], d[2], d[1], cnt + d[5], (int64_t)tl.y * 32, (int64_t)tl.z * 32, lds);
+}
+
+int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_tiles, int ntiles, const int64_t *d_desc, const uint8_t *d_xt,
+                          uint32_t *d_cnt)
+{
+  if (ntiles > 0)
+    hipLaunchKernelGGL(k_gram_i8_batch, dim3((unsigned)ntiles), dim3(64), 0, c->stream, d_tiles, d_desc, d_xt, d_cnt);
+  return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
 }
 
 // ------------------------------------------------------------------------------------

@@ -15,6 +15,24 @@
 // consecutive i (the MFMA result lanes) store to consecutive records.
 __host__ __device__ static inline int64_t rsg_rec(int64_t i, int64_t j, int64_t L) { return j * L + i; }
 
+// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror & Shaw, SC'11).  A draw is a
+// pure function of (seed, stream, node, position, tag), so simulation is order-free.
+__host__ __device__ static inline uint32_t rsg_philox(uint64_t seed, uint32_t stream, uint32_t node,
+                                                     uint32_t pos, uint32_t tag)
+{
+  uint32_t c0 = pos, c1 = node, c2 = stream, c3 = tag;
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0, hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+  return c0;
+}
+
 // Kinds of stored pair counts.
 //   CNT_I8G: int8-Gram records, cell x*4+y holds count << (x + y) (the MFMA accumulator
 //            as produced; readers shift the scale out), 64 bytes per record.
@@ -106,3 +124,27 @@ int rsg_correct(rsg_ctx *c, rsg_msa *m, int actype, int shiftnonneg, char *errbu
 int rsg_minmax(rsg_ctx *c, rsg_msa *m, char *errbuf);
 int rsg_upload_weights_bf16(rsg_ctx *c, rsg_msa *m, int nsplit, char *errbuf);
 int64_t rsg_npad(int64_t N);
+
+// One simulated alignment: tree + output rows (rsg_sim.hip).
+struct SimItem {
+  int64_t N, L;           // leaves, columns
+  int64_t tree_off;       // first internal node of this item in the concatenated tree arrays
+  int64_t seq_off;        // offset of its (2N-1) x L symbol rows
+  int64_t leaf_off;       // offset of its N x L leaf output
+  uint64_t seed; uint32_t stream; uint32_t gapthr;   // a leaf symbol is gapped if its tag-1 draw < gapthr
+  int64_t col_blk0;       // first 64-column block of this item in the launch
+};
+int rsg_sim_launch(rsg_ctx *c, const SimItem *d_items, int nitems, int64_t nodes, int64_t blocks,
+                   const int *d_left, const int *d_right, const double *d_ld, const double *d_rd,
+                   const double *d_model, uint32_t *d_cdf, uint8_t *d_seqs, uint8_t *d_leaves);
+int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t *L,
+                        const int *left, const int *right, const double *ld, const double *rd,
+                        const double *model, const uint64_t *seed, const uint32_t *stream, const double *gapfrac,
+                        uint8_t *d_leaves, const int64_t *leaf_off, char *errbuf);
+
+// Batched pipeline: per-item descriptor of RSG_BDESC int64 fields
+// {N, L, Npad, Lpad, off_xt, off_cnt, off_cov, off_pm, off_rows, off_hist, off_leaf, blk0_pack,
+//  blk0_col, blk0_stat, blk0_el, nb32}
+#define RSG_BDESC 16
+int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_tiles, int ntiles, const int64_t *d_desc, const uint8_t *d_xt,
+                          uint32_t *d_cnt);

@@ -1,0 +1,419 @@
+// rsg_rank.hip -- score histograms, null histogram merge, tail fits and E-values.
+//
+// Reference (R-scape v2.6.11): cov_SignificantPairs_Ranking src/covariation.c:325-530,
+// cov_CreateRankList :640-665, cov_GrowRankList :682-736, null histogram I/O :1717-1868,
+// cov_NullFitExponential :1914-1941, cov_NullFitGamma :1943-1975, SetSurvFitTail :1676-1699,
+// cov2evalue / evalue2cov :2369-2436, cov_histogram_pmass :2483-2506,
+// cov_ranklist_Bin2Bin :2333-2364; calculate_width_histo src/R-scape.c:1280-1371 and
+// null_add2cumranklist :1564-1612.
+//
+// The histogram follows Easel's esl_histogram (not vendored in the reference tree):
+// nb = (int)((bmax - bmin)/w) bins, bin b = ceil((x - bmin)/w - 1) covers
+// (bmin + b w, bmin + (b+1) w], growth by 2x over-allocation below or above, and
+// SetTailByMass censors virtually at the lower bound of the bin where the kept mass
+// first reaches pmass * n.  The exponential tail fit is the binned ML closed form
+// lambda = (ln Sb - ln Sa)/w; the Gamma fit is ML on bin midpoints above phi.
+//
+// Device work (FP64): the histogram of a COV matrix, the pmass / tail / fit / survival
+// computation (one work group, parallel scans and reductions), and the E-value of every
+// pair.  Histogram merging and the .null text format are host code.
+#include "rsg_device.h"
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <climits>
+#include <cfloat>
+#include <algorithm>
+
+// Histogram geometry, bins and E-values are rounded exactly as the host computes them.
+#pragma clang fp contract(off)
+
+// ---------------------------------------------------------------- host histogram object
+extern "C" rsg_hist *rsg_hist_create(double bmin, double bmax, double w)
+{
+  rsg_hist *h = (rsg_hist *)calloc(1, sizeof(rsg_hist));
+  if (!h) return nullptr;
+  h->bmin = bmin; h->bmax = bmax; h->w = w;
+  h->nb = (int32_t)((bmax - bmin) / w);
+  h->obs = (uint64_t *)calloc((size_t)std::max(h->nb, 1), sizeof(uint64_t));
+  h->imin = h->nb; h->imax = -1; h->cmin = h->imin;
+  h->xmin = DBL_MAX; h->xmax = -DBL_MAX; h->phi = 0.;
+  return h;
+}
+
+extern "C" void rsg_hist_destroy(rsg_hist *h) { if (h) { free(h->obs); free(h); } }
+
+static int score2bin(const rsg_hist *h, double x, int *b)
+{
+  if (!std::isfinite(x)) return RSG_FAIL;
+  double v = ceil(((x - h->bmin) / h->w) - 1.);
+  if (v < (double)INT_MIN || v > (double)INT_MAX) return RSG_FAIL;
+  *b = (int)v;
+  return RSG_OK;
+}
+static double lbound(const rsg_hist *h, int b) { return h->w * b + h->bmin; }
+static double ubound(const rsg_hist *h, int b) { return h->w * (b + 1) + h->bmin; }
+
+// make bin b exist (growing below or above as Easel does); returns the shifted bin index
+static int grow_to(rsg_hist *h, int b)
+{
+  if (b < 0) {
+    int nnew = -b * 2;
+    uint64_t *o = (uint64_t *)calloc((size_t)(nnew + h->nb), sizeof(uint64_t));
+    memcpy(o + nnew, h->obs, sizeof(uint64_t) * (size_t)h->nb);
+    free(h->obs); h->obs = o;
+    h->nb += nnew; b += nnew; h->bmin -= nnew * h->w;
+    h->imin += nnew; h->cmin += nnew;
+    if (h->imax > -1) h->imax += nnew;
+  } else if (b >= h->nb) {
+    int nnew = (b - h->nb + 1) * 2;
+    uint64_t *o = (uint64_t *)calloc((size_t)(nnew + h->nb), sizeof(uint64_t));
+    memcpy(o, h->obs, sizeof(uint64_t) * (size_t)h->nb);
+    free(h->obs); h->obs = o;
+    if (h->imin == h->nb) { h->imin += nnew; h->cmin += nnew; }
+    h->bmax += nnew * h->w;
+    h->nb += nnew;
+  }
+  return b;
+}
+
+extern "C" double rsg_width(double maxCOV, double minCOV, double bmin, double w_old, int hpts, double tol)
+{
+  double lo = std::max(bmin, minCOV);
+  double w = std::min(w_old, (maxCOV - lo) / (double)hpts);
+  return (w < tol) ? 0.0 : w;
+}
+
+// ---------------------------------------------------------------- device histogram
+__global__ void __launch_bounds__(256)
+k_hist(const double *__restrict__ cov, int64_t L, double bmin, double w, int nb, int use_lds,
+       unsigned long long *__restrict__ obs)
+{
+  extern __shared__ unsigned sh[];
+  if (use_lds) {
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) sh[b] = 0;
+    __syncthreads();
+  }
+  const double floor_v = bmin + w;
+  const int64_t n = L * L;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = idx / L, j = idx - i * L;
+    if (i >= j) continue;
+    double x = cov[idx];
+    double add = (x > floor_v) ? x : floor_v;                 // ESL_MAX(COV, bmin + w)
+    int b = (int)ceil(((add - bmin) / w) - 1.);
+    b = (b < 0) ? 0 : ((b >= nb) ? nb - 1 : b);               // cannot happen: bmax = maxCOV + 5w
+    if (use_lds) atomicAdd(&sh[b], 1u);
+    else         atomicAdd(&obs[b], 1ull);
+  }
+  if (use_lds) {
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+      if (sh[b]) atomicAdd(&obs[b], (unsigned long long)sh[b]);
+  }
+}
+
+extern "C" int rsg_cov_RankHist(rsg_ctx *c, rsg_msa *m, double bmin, double w, double tol, rsg_hist **ret, char *errbuf)
+{
+  *ret = nullptr;
+  if (!m->have_cov) return rsg_fail(errbuf, "no covariation matrix to rank");
+  if (!(w > 0.)) return rsg_fail(errbuf, "histogram width must be positive (w = %g)", w);
+  double bmax = m->maxCOV + 5 * w;
+  while (fabs(bmax - bmin) < tol) bmax += w;
+  rsg_hist *h = rsg_hist_create(bmin, bmax, w);
+  if (!h) return rsg_fail(errbuf, "out of memory");
+  const int nb = h->nb;
+  unsigned long long *d_obs = nullptr;
+  HIPCHK(hipMalloc(&d_obs, sizeof(unsigned long long) * (size_t)nb));
+  HIPCHK(hipMemsetAsync(d_obs, 0, sizeof(unsigned long long) * (size_t)nb, c->stream));
+  const int use_lds = (nb <= 16384);          // LDS-privatised bins up to 64 KB per work group
+  const int64_t L = m->L;
+  int blocks = (int)std::min<int64_t>(2048, (L * L + 255) / 256);
+  int t; rsg_tstart(c, "hist", &t);
+  hipLaunchKernelGGL(k_hist, dim3(blocks), dim3(256), use_lds ? sizeof(unsigned) * nb : 0, c->stream,
+                     m->d_cov, L, bmin, w, nb, use_lds, d_obs);
+  rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(h->obs, d_obs, sizeof(uint64_t) * (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hipFree(d_obs);
+  // scalar fields: the added values are max(COV, bmin + w), so their extremes follow
+  // from the COV extremes of the upper triangle
+  uint64_t npairs = (uint64_t)(L * (L - 1) / 2);
+  double fl = bmin + w;
+  h->xmin = std::max(m->minCOV, fl);
+  h->xmax = std::max(m->maxCOV, fl);
+  int bmin_b, bmax_b;
+  score2bin(h, h->xmin, &bmin_b); score2bin(h, h->xmax, &bmax_b);
+  h->imin = bmin_b; h->imax = bmax_b; h->cmin = bmin_b;
+  h->n = h->Nc = h->No = npairs;
+  if (npairs == 0) { h->imin = h->nb; h->imax = -1; h->cmin = h->nb; h->xmin = DBL_MAX; h->xmax = -DBL_MAX; }
+  *ret = h;
+  return RSG_OK;
+}
+
+// ---------------------------------------------------------------- null histogram merge
+static int bin2bin(const rsg_hist *h, int b, const rsg_hist *nw, int *ret)
+{
+  double x = lbound(h, b);
+  *ret = -1;
+  if (!std::isfinite(x)) return RSG_FAIL;
+  x = round((x - nw->bmin) / nw->w);
+  if (x < (double)INT_MIN || x > (double)INT_MAX) return RSG_FAIL;
+  int nb = (int)x;
+  if (nb > nw->nb) return RSG_FAIL;
+  *ret = nb;
+  return RSG_OK;
+}
+
+extern "C" int rsg_hist_NullAdd(rsg_hist **pcum, const rsg_hist *h, char *errbuf)
+{
+  if (!h) return RSG_OK;
+  rsg_hist *cum = *pcum;
+  if (!cum) {
+    cum = rsg_hist_create(h->bmin, h->bmax, h->w);
+    if (!cum) return rsg_fail(errbuf, "out of memory");
+    cum->n = h->n; cum->xmin = h->xmin; cum->xmax = h->xmax; cum->imin = h->imin; cum->imax = h->imax;
+  } else {
+    // cov_GrowRankList: a new histogram covering both, bins carried over by Bin2Bin
+    double new_bmin = cum->bmin;
+    if (h->bmin < cum->bmin) new_bmin -= fabs(h->bmin) * 2. * cum->w;
+    rsg_hist *nw = rsg_hist_create(new_bmin, std::max(h->bmax, cum->bmax), cum->w);
+    if (!nw) return rsg_fail(errbuf, "out of memory");
+    nw->n = cum->n; nw->xmin = cum->xmin; nw->xmax = cum->xmax; nw->imin = cum->imin; nw->imax = cum->imax;
+    nw->Nc = cum->Nc; nw->No = cum->No;
+    for (int b = cum->imin; b <= cum->imax; b++) {
+      int nb2;
+      bin2bin(cum, b, nw, &nb2);
+      if (nb2 >= 0 && nb2 < nw->nb) nw->obs[nb2] = cum->obs[b];
+    }
+    rsg_hist_destroy(cum);
+    cum = nw;
+    cum->n += h->n;
+    cum->xmin = std::min(cum->xmin, h->xmin);
+    cum->xmax = std::max(cum->xmax, h->xmax);
+    cum->imin = std::min(cum->imin, h->imin);
+    cum->imax = std::max(cum->imax, h->imax);
+  }
+  for (int b = h->imin; b <= h->imax; b++) {
+    int cb;
+    bin2bin(h, b, cum, &cb);
+    if (cb >= 0 && cb < cum->nb) { cum->obs[cb] += h->obs[b]; cum->Nc += h->obs[b]; cum->No += h->obs[b]; }
+  }
+  *pcum = cum;
+  return RSG_OK;
+}
+
+// ---------------------------------------------------------------- FP64 tail fit on the device
+// (body in rsg_device.h: nullfit_body)
+__global__ void __launch_bounds__(1024)
+k_nullfit(const unsigned long long *__restrict__ obs, int nb, int imin, int imax, double n, double Nc,
+          double bmin, double w, double target_pmass, double fracfit, int doexp,
+          unsigned long long *__restrict__ S, int *__restrict__ NZ, double *__restrict__ survfit, double *out)
+{
+  nullfit_body(obs, nb, imin, imax, n, Nc, bmin, w, target_pmass, fracfit, doexp, S, NZ, survfit, out);
+}
+
+extern "C" int rsg_hist_NullFit(rsg_ctx *c, rsg_hist *h, double pmass, double fracfit, int doexpfit,
+                                double **ret_survfit, double *fit, char *errbuf)
+{
+  *ret_survfit = nullptr;
+  if (!h || h->imax < h->imin) return rsg_fail(errbuf, "empty null histogram");
+  const int nb = h->nb;
+  unsigned long long *d_obs = nullptr, *d_S = nullptr;
+  int *d_NZ = nullptr;
+  double *d_sf = nullptr;
+  double *d_out = nullptr;
+  HIPCHK(hipMalloc(&d_obs, sizeof(unsigned long long) * nb));
+  HIPCHK(hipMalloc(&d_S, sizeof(unsigned long long) * nb));
+  HIPCHK(hipMalloc(&d_NZ, sizeof(int) * nb));
+  HIPCHK(hipMalloc(&d_sf, sizeof(double) * 2 * (size_t)nb));
+  HIPCHK(hipMalloc(&d_out, sizeof(double) * 8));
+  HIPCHK(hipMemcpyAsync(d_obs, h->obs, sizeof(uint64_t) * nb, hipMemcpyHostToDevice, c->stream));
+  int t; rsg_tstart(c, "nullfit", &t);
+  hipLaunchKernelGGL(k_nullfit, dim3(1), dim3(1024), 0, c->stream, d_obs, nb, h->imin, h->imax, (double)h->n,
+                     (double)h->Nc, h->bmin, h->w, pmass, fracfit, doexpfit, d_S, d_NZ, d_sf, d_out);
+  rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  double o[8];
+  double *sf = (double *)malloc(sizeof(double) * 2 * (size_t)nb);
+  HIPCHK(hipMemcpyAsync(o, d_out, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(sf, d_sf, sizeof(double) * 2 * (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hipFree(d_obs); hipFree(d_S); hipFree(d_NZ); hipFree(d_sf); hipFree(d_out);
+  if (std::isnan(o[4])) { free(sf); return rsg_fail(errbuf, "bad Null histogram fit, pmass is nan."); }
+  h->phi = o[5]; h->cmin = (int32_t)o[6]; h->Nc = h->n; h->No = (uint64_t)o[7]; h->tailfit = 1;
+  for (int k = 0; k < 5; k++) fit[k] = o[k];
+  if (!std::isinf(o[1])) *ret_survfit = sf; else free(sf);
+  return RSG_OK;
+}
+
+// ---------------------------------------------------------------- E-values
+struct EvArgs {
+  const unsigned long long *S;   // suffix sums of the null histogram
+  const double *sf;              // survfit (2 nb) or null
+  double bmin, w, phi, xmax, NcNull, Nc;
+  int nb, imin, imax;
+};
+
+__device__ double d_cov2evalue(double cov, const EvArgs &A, int *bad)
+{
+  int icov = (int)ceil(((cov - A.bmin) / A.w) - 1.);
+  if (A.sf && icov >= 2 * A.nb - 1) return A.sf[2 * A.nb - 1];
+  if (A.sf && cov >= A.phi)         return A.sf[icov + 1];
+  if (cov >= A.xmax) return 1.0 / A.NcNull;
+  if (icov <= A.imax) {
+    if (icov < A.imin) icov = A.imin;
+    if (icov >= A.imax - 1) return 1.0 / A.NcNull;
+    return (double)A.S[icov] / A.NcNull;
+  }
+  *bad = 1;
+  return NAN;
+}
+
+__global__ void k_evalues(const double *__restrict__ cov, int64_t L, EvArgs A, double *__restrict__ ev, int *flag)
+{
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= L * L) return;
+  int64_t i = idx / L, j = idx - i * L;
+  if (i == j) { ev[idx] = INFINITY; return; }
+  int bad = 0;
+  double p = d_cov2evalue(cov[idx], A, &bad);
+  if (bad) atomicOr(flag, 1);
+  ev[idx] = p * A.Nc;
+}
+
+__global__ void k_suffix(const unsigned long long *__restrict__ obs, int imin, int imax, unsigned long long *__restrict__ S)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    unsigned long long run = 0;
+    for (int b = imax; b >= imin; b--) { run += obs[b]; S[b] = run; }
+  }
+}
+
+extern "C" int rsg_cov_Evalues(rsg_ctx *c, rsg_msa *m, const rsg_hist *null, const double *survfit, int64_t Nc,
+                               double *Eval, char *errbuf)
+{
+  if (!m->have_cov) return rsg_fail(errbuf, "no covariation matrix");
+  const int nb = null->nb;
+  const int64_t L = m->L;
+  unsigned long long *d_obs = nullptr, *d_S = nullptr;
+  double *d_sf = nullptr, *d_ev = nullptr;
+  HIPCHK(hipMalloc(&d_obs, sizeof(unsigned long long) * nb));
+  HIPCHK(hipMalloc(&d_S, sizeof(unsigned long long) * nb));
+  HIPCHK(hipMalloc(&d_ev, sizeof(double) * (size_t)(L * L)));
+  HIPCHK(hipMemcpyAsync(d_obs, null->obs, sizeof(uint64_t) * nb, hipMemcpyHostToDevice, c->stream));
+  if (survfit) {
+    HIPCHK(hipMalloc(&d_sf, sizeof(double) * 2 * (size_t)nb));
+    HIPCHK(hipMemcpyAsync(d_sf, survfit, sizeof(double) * 2 * (size_t)nb, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+  hipLaunchKernelGGL(k_suffix, dim3(1), dim3(64), 0, c->stream, d_obs, null->imin, null->imax, d_S);
+  EvArgs A;
+  A.S = d_S; A.sf = d_sf; A.bmin = null->bmin; A.w = null->w; A.phi = null->phi; A.xmax = null->xmax;
+  A.NcNull = (double)null->Nc; A.Nc = (double)Nc; A.nb = nb; A.imin = null->imin; A.imax = null->imax;
+  int t; rsg_tstart(c, "evalues", &t);
+  hipLaunchKernelGGL(k_evalues, dim3((unsigned)((L * L + 255) / 256)), dim3(256), 0, c->stream, m->d_cov, L, A, d_ev, c->d_flag);
+  rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  if (Eval) HIPCHK(hipMemcpyAsync(Eval, d_ev, sizeof(double) * (size_t)(L * L), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hipFree(d_obs); hipFree(d_S); hipFree(d_ev); if (d_sf) hipFree(d_sf);
+  if (*(int *)c->h_pinned) return rsg_fail(errbuf, "cannot find evalue for a covariation score");
+  return RSG_OK;
+}
+
+// ---------------------------------------------------------------- scalar helpers (host)
+extern "C" double rsg_cov2evalue(double cov, int64_t Nc, const rsg_hist *h, const double *survfit)
+{
+  int icov;
+  if (score2bin(h, cov, &icov) != RSG_OK) return NAN;
+  if (survfit && icov >= 2 * h->nb - 1) return survfit[2 * h->nb - 1] * (double)Nc;
+  if (survfit && cov >= h->phi)         return survfit[icov + 1] * (double)Nc;
+  if (cov >= h->xmax) return (double)Nc / (double)h->Nc;
+  if (icov <= h->imax) {
+    if (icov < h->imin) icov = h->imin;
+    if (icov >= h->imax - 1) return (double)Nc / (double)h->Nc;
+    uint64_t cc = 0;
+    for (int i = h->imax; i >= icov; i--) cc += h->obs[i];
+    return (double)cc * (double)Nc / (double)h->Nc;
+  }
+  return NAN;
+}
+
+extern "C" double rsg_evalue2cov(double eval_thresh, int64_t Nc, const rsg_hist *h, const double *survfit)
+{
+  double cov = -INFINITY, eval;
+  int b = h->cmin - 1, i;
+  if (h->No >= 10 && survfit) {
+    for (b = 2 * h->nb - 1; b >= h->cmin; b--) {
+      eval = survfit[b] * (double)Nc;
+      if (eval >= eval_thresh) break;
+    }
+    cov = lbound(h, b);
+  }
+  if (b == h->cmin - 1) {
+    uint64_t cc = 0;
+    for (i = h->imax; i >= h->imin; i--) {
+      cc += h->obs[i];
+      eval = (double)cc * (double)Nc / (double)h->Nc;
+      if (eval >= eval_thresh) break;
+    }
+    cov = ubound(h, i);
+  }
+  return cov;
+}
+
+// ---------------------------------------------------------------- .null text format
+// cov_WriteNullHistogram: "%f %llu\n" per bin, x = bmin + i w (covariation.c:1845-1868).
+extern "C" int rsg_hist_WriteNull(const char *path, const rsg_hist *h, char *errbuf)
+{
+  FILE *fp = fopen(path, "w");
+  if (!fp) return rsg_fail(errbuf, "could not open nullhisfile %s", path);
+  for (int i = 0; i < h->nb; i++) fprintf(fp, "%f %llu\n", h->bmin + i * h->w, (unsigned long long)h->obs[i]);
+  fclose(fp);
+  return RSG_OK;
+}
+
+// cov_ReadNullHistogram (covariation.c:1717-1843): w = smallest gap between successive
+// scores, bmin / bmax = score extremes, then every "score count" line sets its bin.
+extern "C" int rsg_hist_ReadNull(const char *path, rsg_hist **ret, char *errbuf)
+{
+  *ret = nullptr;
+  FILE *fp = fopen(path, "r");
+  if (!fp) return rsg_fail(errbuf, "file open failed");
+  std::vector<double> sc; std::vector<unsigned long long> cnt;
+  char line[1024];
+  while (fgets(line, sizeof line, fp)) {
+    char *p = line;
+    while (*p == ' ' || *p == '\t') p++;
+    if (*p == '#' || *p == '\n' || *p == 0) continue;
+    char *tok1 = strtok(p, " \t\n"), *tok2 = strtok(nullptr, " \t\n");
+    if (!tok1) continue;
+    sc.push_back(atof(tok1));
+    cnt.push_back(tok2 ? (unsigned long long)atoi(tok2) : 0ull);
+    if (!tok2) cnt.back() = ~0ull;                          // marks a line without a count
+  }
+  fclose(fp);
+  if (sc.empty()) return rsg_fail(errbuf, "empty null histogram file");
+  double w = INFINITY, prv = -INFINITY, mx = -INFINITY, mn = INFINITY;
+  for (double s : sc) { w = std::min(w, fabs(s - prv)); prv = s; mx = std::max(mx, s); mn = std::min(mn, s); }
+  double bmax = mx, bmin = mn;
+  while (fabs(bmax - bmin) < 1e-2) bmax += w;
+  rsg_hist *h = rsg_hist_create(bmin, bmax, w);
+  for (size_t k = 0; k < sc.size(); k++) {
+    if (cnt[k] == ~0ull) continue;
+    int b;
+    if (score2bin(h, sc[k], &b) != RSG_OK) { rsg_hist_destroy(h); return rsg_fail(errbuf, "bad score %f", sc[k]); }
+    b = grow_to(h, b);
+    if (b < h->imin) { h->imin = b; h->cmin = b; }
+    if (b > h->imax) h->imax = b;
+    if (sc[k] < h->xmin) h->xmin = sc[k];
+    if (sc[k] > h->xmax) h->xmax = sc[k];
+    h->obs[b] = cnt[k];
+    h->n += cnt[k]; h->Nc += cnt[k]; h->No += cnt[k];
+  }
+  *ret = h;
+  return RSG_OK;
+}

@@ -1,0 +1,320 @@
+// rsg_sim.hip -- phylogenetic null simulation on the device.
+//
+// Reference: cov_GenerateAlignment (src/cov_simulate.c:60-86) -> Ungapped :88-159 ->
+// cov_evolve_root_ungapped_tree :288-324 -> per branch P(t) = exp(tQ)
+// (src/ratematrix.c:185-233, esl_dmx_Exp) and inverse-CDF draws cov_addres / cov_addpair
+// (:756-798); trees from esl_tree_Simulate rescaled to a mean branch length
+// (src/R-scape-sim.c:608-680).
+//
+// Design: a tree is the Easel ESL_TREE layout (internal nodes 0..N-2, root 0, parents
+// before children; children > 0 internal, <= 0 leaf -child).  Sequence rows: internal
+// node v -> row v, leaf l -> row N-1+l.  Every symbol draw is Philox(seed, stream,
+// row, column, tag) compared against uint32 CDF thresholds of the branch's P(t) row, so a
+// column's history is independent of every other column: one thread owns one column and
+// walks the nodes in index order (parents first), no synchronisation needed.
+// P(t) = Pi^-1/2 V diag(exp(lambda t)) V^T Pi^1/2 from the eigendecomposition of the
+// symmetrised GTR generator (host, FP64 Jacobi); the per-branch tables are built on the
+// device in FP64.
+#include "rsg_internal.h"
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+
+// Transition tables are rounded as the host computes them (no FMA contraction).
+#pragma clang fp contract(off)
+
+// ---------------------------------------------------------------- host: model
+// Eigendecomposition of a symmetric 4x4 (cyclic Jacobi).
+static void jacobi4(double A[4][4], double V[4][4], double lam[4])
+{
+  for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) V[i][j] = (i == j);
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.;
+    for (int p = 0; p < 4; p++) for (int q = p + 1; q < 4; q++) off += A[p][q] * A[p][q];
+    if (off < 1e-30) break;
+    for (int p = 0; p < 4; p++)
+      for (int q = p + 1; q < 4; q++) {
+        if (fabs(A[p][q]) < 1e-300) continue;
+        double th = (A[q][q] - A[p][p]) / (2 * A[p][q]);
+        double t = (th >= 0 ? 1. : -1.) / (fabs(th) + sqrt(th * th + 1.));
+        double cs = 1. / sqrt(t * t + 1.), sn = t * cs;
+        for (int k = 0; k < 4; k++) {
+          double akp = A[k][p], akq = A[k][q];
+          A[k][p] = cs * akp - sn * akq; A[k][q] = sn * akp + cs * akq;
+        }
+        for (int k = 0; k < 4; k++) {
+          double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = cs * apk - sn * aqk; A[q][k] = sn * apk + cs * aqk;
+        }
+        for (int k = 0; k < 4; k++) {
+          double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = cs * vkp - sn * vkq; V[k][q] = sn * vkp + cs * vkq;
+        }
+      }
+  }
+  for (int i = 0; i < 4; i++) lam[i] = A[i][i];
+}
+
+// Model packed for the device: [0..15] M1 = Pi^-1/2 V, [16..31] M2 = V^T Pi^1/2, [32..35] lambda,
+// [36..39] pi.  P(t)[a][b] = sum_k M1[a][k] exp(lambda_k t) M2[k][b].
+extern "C" int rsg_gtr_model(const double *pi, const double *exch, double *model)
+{
+  double Q[4][4] = {{0}};
+  const int pr[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+  for (int k = 0; k < 6; k++) {
+    int a = pr[k][0], b = pr[k][1];
+    Q[a][b] = exch[k] * pi[b]; Q[b][a] = exch[k] * pi[a];
+  }
+  double mu = 0.;
+  for (int a = 0; a < 4; a++) {
+    double s = 0.;
+    for (int b = 0; b < 4; b++) if (b != a) s += Q[a][b];
+    Q[a][a] = -s; mu += pi[a] * s;
+  }
+  if (!(mu > 0.)) return RSG_FAIL;
+  for (int a = 0; a < 4; a++) for (int b = 0; b < 4; b++) Q[a][b] /= mu;   // one substitution per unit time
+  double S[4][4], V[4][4], lam[4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) S[a][b] = sqrt(pi[a]) * Q[a][b] / sqrt(pi[b]);
+  jacobi4(S, V, lam);
+  for (int a = 0; a < 4; a++)
+    for (int k = 0; k < 4; k++) {
+      model[a * 4 + k] = V[a][k] / sqrt(pi[a]);
+      model[16 + k * 4 + a] = V[a][k] * sqrt(pi[a]);
+    }
+  for (int k = 0; k < 4; k++) { model[32 + k] = lam[k]; model[36 + k] = pi[k]; }
+  return RSG_OK;
+}
+
+// Yule tree: lineages split one at a time (uniform choice), waiting times Exp(k); then the
+// branch lengths are rescaled to mean branch length mean_bl.  Draws from Philox(seed, stream).
+extern "C" int rsg_tree_yule(int64_t n, double mean_bl, uint64_t seed, uint32_t stream,
+                             int *left, int *right, double *ld, double *rd)
+{
+  if (n < 2) return RSG_FAIL;
+  auto unif = [&](uint32_t ctr, uint32_t tag) {
+    return ((double)rsg_philox(seed, stream, 0xFFFFFFF0u, ctr, tag) + 0.5) / 4294967296.0;
+  };
+  std::vector<double> birth((size_t)(n - 1));
+  std::vector<std::pair<int, int>> lin;            // (parent internal node, side)
+  lin.reserve((size_t)n);
+  lin.push_back(std::make_pair(-1, 0));
+  double t = 0.;
+  for (int64_t v = 0; v < n - 1; v++) {
+    double k = (double)lin.size();
+    t += -log(unif((uint32_t)v, 1u)) / k;
+    size_t idx = (size_t)(unif((uint32_t)v, 2u) * k);
+    if (idx >= lin.size()) idx = lin.size() - 1;
+    std::pair<int, int> pl = lin[idx];
+    lin[idx] = lin.back(); lin.pop_back();
+    birth[(size_t)v] = t;
+    if (pl.first >= 0) { if (pl.second == 0) left[pl.first] = (int)v; else right[pl.first] = (int)v; }
+    lin.push_back(std::make_pair((int)v, 0));
+    lin.push_back(std::make_pair((int)v, 1));
+  }
+  double t_end = t - log(unif((uint32_t)n, 3u)) / (double)lin.size();
+  int leaf = 0;
+  for (auto &pl : lin) {
+    if (pl.second == 0) left[pl.first] = -leaf; else right[pl.first] = -leaf;
+    leaf++;
+  }
+  double sum = 0.;
+  for (int64_t v = 0; v < n - 1; v++) {
+    double tl = (left[v] > 0) ? birth[(size_t)left[v]] : t_end;
+    double tr = (right[v] > 0) ? birth[(size_t)right[v]] : t_end;
+    ld[v] = tl - birth[(size_t)v]; rd[v] = tr - birth[(size_t)v];
+    sum += ld[v] + rd[v];
+  }
+  double scale = mean_bl / (sum / (2. * (double)(n - 1)));
+  for (int64_t v = 0; v < n - 1; v++) { ld[v] *= scale; rd[v] *= scale; }
+  return RSG_OK;
+}
+
+// ---------------------------------------------------------------- device: tables + evolution
+// CDF thresholds of one P(t) row: c_k = floor(2^32 * sum_{b<=k} P[a][b]) for k = 0..2.
+__device__ void row_cdf(const double *model, double t, int a, uint32_t *out)
+{
+  double e[4];
+  for (int k = 0; k < 4; k++) e[k] = exp(model[32 + k] * t);
+  double cum = 0.;
+  for (int b = 0; b < 3; b++) {
+    double p = 0.;
+    for (int k = 0; k < 4; k++) p += model[a * 4 + k] * e[k] * model[16 + k * 4 + b];
+    cum += (p > 0.) ? p : 0.;
+    double v = cum * 4294967296.0;
+    out[b] = (v >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)v;
+  }
+}
+
+__global__ void k_cdf_list(const double *__restrict__ model, const double *__restrict__ t, int64_t nt,
+                           uint32_t *__restrict__ cdf)
+{
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nt * 4) return;
+  row_cdf(model, t[idx / 4], (int)(idx % 4), cdf + (idx / 4) * 12 + (idx % 4) * 3);
+}
+
+extern "C" int rsg_gtr_cdf(rsg_ctx *c, const double *model, int64_t nt, const double *t, uint32_t *cdf, char *errbuf)
+{
+  double *d_m = nullptr, *d_t = nullptr; uint32_t *d_c = nullptr;
+  HIPCHK(hipMalloc(&d_m, sizeof(double) * 40));
+  HIPCHK(hipMalloc(&d_t, sizeof(double) * (size_t)std::max<int64_t>(nt, 1)));
+  HIPCHK(hipMalloc(&d_c, sizeof(uint32_t) * 12 * (size_t)std::max<int64_t>(nt, 1)));
+  HIPCHK(hipMemcpy(d_m, model, sizeof(double) * 40, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_t, t, sizeof(double) * (size_t)nt, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_cdf_list, dim3((unsigned)((nt * 4 + 255) / 256)), dim3(256), 0, c->stream, d_m, d_t, nt, d_c);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(cdf, d_c, sizeof(uint32_t) * 12 * (size_t)nt, hipMemcpyDeviceToHost));
+  hipFree(d_m); hipFree(d_t); hipFree(d_c);
+  return RSG_OK;
+}
+
+__device__ __forceinline__ int icdf3(uint32_t u, const uint32_t *c)
+{
+  return (u >= c[0]) + (u >= c[1]) + (u >= c[2]);
+}
+
+// grid: column blocks of 64 over all items (col_blk0 prefix); one thread per column.
+__global__ void __launch_bounds__(64)
+k_evolve(const SimItem *__restrict__ items, int nitems, const int *__restrict__ left, const int *__restrict__ right,
+         const uint32_t *__restrict__ cdf, const double *__restrict__ model, uint8_t *__restrict__ seqs,
+         uint8_t *__restrict__ leaves)
+{
+  // locate the item of this block (binary search on col_blk0)
+  int lo = 0, hi = nitems - 1;
+  const int64_t blk = blockIdx.x;
+  while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (items[mid].col_blk0 <= blk) lo = mid; else hi = mid - 1; }
+  const SimItem it = items[lo];
+  const int64_t col = (blk - it.col_blk0) * 64 + threadIdx.x;
+  if (col >= it.L) return;
+  const int64_t N = it.N, L = it.L;
+  const int *lf = left + it.tree_off, *rt = right + it.tree_off;
+  const uint32_t *cd = cdf + it.tree_off * 24;               // per internal node: left 12, right 12 (see launcher)
+  uint8_t *S = seqs + it.seq_off;
+  // root from the stationary distribution
+  {
+    uint32_t u = rsg_philox(it.seed, it.stream, (uint32_t)(2 * N - 1), (uint32_t)col, 0u);
+    double cum = 0.; int a = 3;
+    for (int b = 0; b < 3; b++) { cum += model[36 + b]; double v = cum * 4294967296.0; uint32_t th = (v >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)v; if (u < th) { a = b; break; } }
+    S[col] = (uint8_t)a;
+  }
+  for (int64_t v = 0; v < N - 1; v++) {
+    const int par = S[v * L + col];
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      const int d = side ? rt[v] : lf[v];
+      const int64_t row = (d > 0) ? d : (N - 1 - d);
+      const uint32_t u = rsg_philox(it.seed, it.stream, (uint32_t)row, (uint32_t)col, 0u);
+      int sym = icdf3(u, cd + v * 24 + side * 12 + par * 3);
+      S[row * L + col] = (uint8_t)sym;
+      if (d <= 0) {
+        if (it.gapthr && rsg_philox(it.seed, it.stream, (uint32_t)row, (uint32_t)col, 1u) < it.gapthr) sym = 4;
+        leaves[it.leaf_off + (-d) * L + col] = (uint8_t)sym;
+      }
+    }
+  }
+}
+
+// Per-item branch tables into the [node][side][4][3] layout used by k_evolve.
+__global__ void k_branch_tables(const SimItem *__restrict__ items, int nitems, const double *__restrict__ ld,
+                                const double *__restrict__ rd, const double *__restrict__ model,
+                                int64_t total_nodes, uint32_t *__restrict__ cdf)
+{
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;     // (node, side, row)
+  if (idx >= total_nodes * 8) return;
+  int64_t node = idx / 8; int side = (int)((idx / 4) & 1); int a = (int)(idx & 3);
+  double t = side ? rd[node] : ld[node];
+  row_cdf(model, t, a, cdf + node * 24 + side * 12 + a * 3);
+}
+
+// Launch-only entry (no allocation, no synchronisation; usable under stream capture).
+int rsg_sim_launch(rsg_ctx *c, const SimItem *d_items, int nitems, int64_t nodes, int64_t blocks,
+                   const int *d_left, const int *d_right, const double *d_ld, const double *d_rd,
+                   const double *d_model, uint32_t *d_cdf, uint8_t *d_seqs, uint8_t *d_leaves)
+{
+  if (nodes > 0)
+    hipLaunchKernelGGL(k_branch_tables, dim3((unsigned)((nodes * 8 + 255) / 256)), dim3(256), 0, c->stream,
+                       d_items, nitems, d_ld, d_rd, d_model, nodes, d_cdf);
+  if (blocks > 0)
+    hipLaunchKernelGGL(k_evolve, dim3((unsigned)blocks), dim3(64), 0, c->stream, d_items, nitems, d_left, d_right,
+                       d_cdf, d_model, d_seqs, d_leaves);
+  return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
+}
+
+// ---------------------------------------------------------------- host API
+struct SimBuffers {
+  int *d_left = nullptr, *d_right = nullptr; double *d_ld = nullptr, *d_rd = nullptr;
+  uint32_t *d_cdf = nullptr; double *d_model = nullptr; uint8_t *d_seqs = nullptr; SimItem *d_items = nullptr;
+};
+
+static void sim_free(SimBuffers &b)
+{
+  void *p[] = {b.d_left, b.d_right, b.d_ld, b.d_rd, b.d_cdf, b.d_model, b.d_seqs, b.d_items};
+  for (void *q : p) if (q) hipFree(q);
+}
+
+// Simulate a set of alignments into device leaves (row-major N x L per item at leaf offsets).
+// Trees come from the caller (host arrays, concatenated per item); model = rsg_gtr_model output.
+int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t *L,
+                        const int *left, const int *right, const double *ld, const double *rd,
+                        const double *model, const uint64_t *seed, const uint32_t *stream, const double *gapfrac,
+                        uint8_t *d_leaves, const int64_t *leaf_off, char *errbuf)
+{
+  std::vector<SimItem> items((size_t)nitems);
+  int64_t nodes = 0, seq = 0, blocks = 0;
+  for (int k = 0; k < nitems; k++) {
+    SimItem &it = items[(size_t)k];
+    it.N = N[k]; it.L = L[k]; it.tree_off = nodes; it.seq_off = seq; it.leaf_off = leaf_off[k];
+    it.seed = seed[k]; it.stream = stream[k];
+    double g = gapfrac ? gapfrac[k] : 0.;
+    it.gapthr = (g <= 0.) ? 0u : (g >= 1. ? 0xFFFFFFFFu : (uint32_t)(g * 4294967296.0));
+    it.col_blk0 = blocks;
+    nodes += N[k] - 1; seq += (2 * N[k] - 1) * L[k]; blocks += (L[k] + 63) / 64;
+  }
+  SimBuffers b;
+  int rc = RSG_FAIL;
+  do {
+    if (hipMalloc(&b.d_left, sizeof(int) * nodes) || hipMalloc(&b.d_right, sizeof(int) * nodes) ||
+        hipMalloc(&b.d_ld, sizeof(double) * nodes) || hipMalloc(&b.d_rd, sizeof(double) * nodes) ||
+        hipMalloc(&b.d_cdf, sizeof(uint32_t) * 24 * nodes) || hipMalloc(&b.d_model, sizeof(double) * 40) ||
+        hipMalloc(&b.d_seqs, (size_t)seq) || hipMalloc(&b.d_items, sizeof(SimItem) * nitems)) {
+      rsg_fail(errbuf, "simulation buffers: out of device memory"); break;
+    }
+    hipMemcpyAsync(b.d_left, left, sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(b.d_right, right, sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(b.d_ld, ld, sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(b.d_rd, rd, sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(b.d_model, model, sizeof(double) * 40, hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(b.d_items, items.data(), sizeof(SimItem) * nitems, hipMemcpyHostToDevice, c->stream);
+    int t; rsg_tstart(c, "simulate", &t);
+    hipLaunchKernelGGL(k_branch_tables, dim3((unsigned)((nodes * 8 + 255) / 256)), dim3(256), 0, c->stream,
+                       b.d_items, nitems, b.d_ld, b.d_rd, b.d_model, nodes, b.d_cdf);
+    hipLaunchKernelGGL(k_evolve, dim3((unsigned)blocks), dim3(64), 0, c->stream, b.d_items, nitems, b.d_left,
+                       b.d_right, b.d_cdf, b.d_model, b.d_seqs, d_leaves);
+    rsg_tstop(c, t);
+    if (hipGetLastError() != hipSuccess) { rsg_fail(errbuf, "simulation launch failed"); break; }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { rsg_fail(errbuf, "simulation failed"); break; }
+    rc = RSG_OK;
+  } while (0);
+  sim_free(b);
+  return rc;
+}
+
+// Simulate one alignment and return its leaves (N x L) to the host.
+extern "C" int rsg_null_simulate(rsg_ctx *c, const rsg_null_job *job, uint8_t *leaves, char *errbuf)
+{
+  HIPCHK(hipSetDevice(c->device));
+  if (job->nleaf < 2 || job->alen < 1) return rsg_fail(errbuf, "simulation needs >= 2 leaves and >= 1 column");
+  uint8_t *d_leaves = nullptr;
+  HIPCHK(hipMalloc(&d_leaves, (size_t)(job->nleaf * job->alen)));
+  int64_t off = 0;
+  int rc = rsg_simulate_device(c, 1, &job->nleaf, &job->alen, job->left, job->right, job->ld, job->rd,
+                               job->model, &job->seed, &job->stream, &job->gapfrac, d_leaves, &off, errbuf);
+  if (rc == RSG_OK) {
+    if (hipMemcpy(leaves, d_leaves, (size_t)(job->nleaf * job->alen), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = rsg_fail(errbuf, "copy of simulated leaves failed");
+  }
+  hipFree(d_leaves);
+  return rc;
+}

@@ -11,75 +11,18 @@
 // pp and q = pm_i pm_j (GT = 2 nseff sum pp ln(pp/q), CHI = nseff sum (pp-q)^2/q, ...),
 // which is the same quantity with the same cell conditions.  In FP64 mode integer counts
 // take ln(c + 1e-10) and ln(nseff + 16e-10) from per-alignment tables.
-#include "rsg_internal.h"
+#include "rsg_device.h"
 #include <cmath>
 #include <algorithm>
-
-template <int KIND> __device__ __forceinline__ void load_rec(const void *cnt, int64_t rec, double *c)
-{
-  if (KIND == CNT_I8G) {
-    const uint4 *p = (const uint4 *)((const uint32_t *)cnt + rec * 16);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      uint4 v = p[k];                               // row x = k: cells y = 0..3, scale 2^(x+y)
-      c[4 * k] = (double)(v.x >> (k + 0)); c[4 * k + 1] = (double)(v.y >> (k + 1));
-      c[4 * k + 2] = (double)(v.z >> (k + 2)); c[4 * k + 3] = (double)(v.w >> (k + 3));
-    }
-  } else {
-    const float4 *p = (const float4 *)((const float *)cnt + rec * 16);
-#pragma unroll
-    for (int k = 0; k < 4; k++) { float4 v = p[k]; c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w; }
-  }
-}
 
 // ---- pm: corr_Marginals.  One block per column i; fixed-order reduction (deterministic).
 template <int KIND>
 __global__ void __launch_bounds__(256) k_pm(const void *__restrict__ cnt, int64_t L, double tol,
                                             double *__restrict__ pm, double *__restrict__ lpm, int *__restrict__ flag)
 {
-  __shared__ double red[4][256];
-  const int64_t i = blockIdx.x;
-  double acc[4] = {0., 0., 0., 0.};
-  for (int64_t j = threadIdx.x; j < L; j += 256) {
-    if (j == i) continue;
-    int64_t a = (i < j) ? i : j, b = (i < j) ? j : i;
-    double c[16];
-    load_rec<KIND>(cnt, rsg_rec(a, b, L), c);
-    double ns = 0.;
-#pragma unroll
-    for (int k = 0; k < 16; k++) ns += c[k];
-    if (ns > 0) {
-      double inv = 1. / (ns + 16e-10);
-#pragma unroll
-      for (int x = 0; x < 4; x++) {
-        double r = (i < j) ? (c[x * 4] + c[x * 4 + 1] + c[x * 4 + 2] + c[x * 4 + 3])
-                           : (c[x] + c[4 + x] + c[8 + x] + c[12 + x]);
-        acc[x] += (r + 4e-10) * inv;
-      }
-    }
-  }
-#pragma unroll
-  for (int x = 0; x < 4; x++) red[x][threadIdx.x] = acc[x];
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      for (int x = 0; x < 4; x++) red[x][threadIdx.x] += red[x][threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    double v[4] = {red[0][0], red[1][0], red[2][0], red[3][0]};
-    double s = v[0] + v[1] + v[2] + v[3];
-    double tot = 0.;
-    for (int x = 0; x < 4; x++) {
-      v[x] = (s != 0.) ? v[x] / s : 0.25;
-      if (!(v[x] >= 0. && v[x] <= 1.)) atomicOr(flag, 1);
-      tot += v[x];
-      pm[i * 4 + x] = v[x];
-      lpm[i * 4 + x] = (v[x] > 0.) ? log(v[x]) : -INFINITY;
-    }
-    if (fabs(tot - 1.) > tol) atomicOr(flag, 1);
-  }
+  pm_column<KIND>(cnt, L, (int64_t)blockIdx.x, tol, pm, lpm, flag);
 }
+
 
 // ---- ps: mutual_naive_psi, one block per column over the transposed raw codes.
 __global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, const double *__restrict__ w,
@@ -136,166 +79,11 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
 }
 
 // ------------------------------------------------------------------------------------
-// per-pair statistics
-struct StatArgs {
-  const void *cnt; const double *pm; const double *lpm; const double *lgc;
-  int64_t L, N; double wsum; double *cov; double *part;  // part: 2 per block (min, max)
-  int type, cls; unsigned allow;                           // allow: 16-bit mask of allowed cells
-  double meanp[4];                                         // CCF
-};
-
-template <typename T> __device__ __forceinline__ T tlog(T v);
-template <> __device__ __forceinline__ double tlog<double>(double v) { return log(v); }
-template <> __device__ __forceinline__ float tlog<float>(float v) { return __logf(v); }
-
-// One pair. c: counts, pmi/pmj: marginals.  Returns the statistic (see the file header).
-template <typename T, bool TAB>
-__device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int64_t j)
-{
-  T c[16], ns = 0;
-#pragma unroll
-  for (int k = 0; k < 16; k++) { c[k] = (T)cd[k]; ns += c[k]; }
-  double nsd = 0.;
-#pragma unroll
-  for (int k = 0; k < 16; k++) nsd += cd[k];
-  T pmi[4], pmj[4], lpi[4], lpj[4];
-#pragma unroll
-  for (int x = 0; x < 4; x++) {
-    pmi[x] = (T)A.pm[i * 4 + x]; pmj[x] = (T)A.pm[j * 4 + x];
-    lpi[x] = (T)A.lpm[i * 4 + x]; lpj[x] = (T)A.lpm[j * 4 + x];
-  }
-  const T den = ns + (T)16e-10;
-  T lden;
-  if (TAB) lden = (T)A.lgc[A.N + 1 + (int64_t)nsd]; else lden = tlog<T>(den);
-  T pp[16], lpp[16];
-#pragma unroll
-  for (int k = 0; k < 16; k++) {
-    pp[k] = (c[k] + (T)1e-10) / den;
-    if (TAB) lpp[k] = (T)A.lgc[(int64_t)cd[k]] - lden; else lpp[k] = tlog<T>(c[k] + (T)1e-10) - lden;
-  }
-  const int type = A.type;
-  if (type == RSG_CCF) {
-    T sa = 0, sb = 0;
-#pragma unroll
-    for (int x = 0; x < 4; x++) {
-      T a = ns * pmi[x] - (T)A.meanp[x], b = ns * pmj[x] - (T)A.meanp[x];
-      sa += a * a; sb += b * b;
-    }
-    return (double)sqrt((double)(sa * sb));
-  }
-  if (A.cls == RSG_C2) {
-    T pw = 0, pn = 0, qw = 0, qn = 0;
-#pragma unroll
-    for (int x = 0; x < 4; x++)
-#pragma unroll
-      for (int y = 0; y < 4; y++) {
-        if ((A.allow >> (x * 4 + y)) & 1u) { pw += pp[x * 4 + y]; qw += pmi[x] * pmj[y]; }
-        else                               { pn += pp[x * 4 + y]; qn += pmi[x] * pmj[y]; }
-      }
-    T v = 0, H = 0;
-    switch (type) {
-    case RSG_CHI:
-      if (ns > 0) { if (qw > 0) v += ns * (pw - qw) * (pw - qw) / qw; if (qn > 0) v += ns * (pn - qn) * (pn - qn) / qn; }
-      break;
-    case RSG_OMES:
-      if (ns > 0) { if (qw > 0) v += ns * (pw - qw) * (pw - qw); if (qn > 0) v += ns * (pn - qn) * (pn - qn); }
-      break;
-    case RSG_GT:
-      if (ns > 0) {
-        if (qw > 0 && pw > 0) v += pw * (tlog<T>(pw) - tlog<T>(qw));
-        if (qn > 0 && pn > 0) v += pn * (tlog<T>(pn) - tlog<T>(qn));
-        v *= 2 * ns;
-      }
-      break;
-    case RSG_MI: case RSG_MIg:
-      if (pw > 0) v += pw * (tlog<T>(pw) - tlog<T>(qw));
-      if (pn > 0) v += pn * (tlog<T>(pn) - tlog<T>(qn));
-      if (type == RSG_MIg && ns > 0) v -= (T)((A.wsum - nsd) / nsd);
-      break;
-    case RSG_MIr:
-      if (pw > 0) H -= pw * tlog<T>(pw);
-      if (pn > 0) H -= pn * tlog<T>(pn);
-      if (pw > 0 && qw > 0) v += pw * (tlog<T>(pw) - tlog<T>(qw));
-      if (pn > 0 && qn > 0) v += pn * (tlog<T>(pn) - tlog<T>(qn));
-      v = (H > (T)1e-2) ? v / H : 0;
-      break;
-    }
-    return (double)v;
-  }
-  // C16 / CWC
-  T s = 0, H = 0;
-#pragma unroll
-  for (int x = 0; x < 4; x++)
-#pragma unroll
-    for (int y = 0; y < 4; y++) {
-      const int k = x * 4 + y;
-      const T q = pmi[x] * pmj[y];
-      switch (type) {
-      case RSG_CHI:  if (q > 0) s += (pp[k] - q) * (pp[k] - q) / q; break;
-      case RSG_OMES: if (q > 0) s += (pp[k] - q) * (pp[k] - q); break;
-      case RSG_GT:
-        if (A.cls == RSG_CWC && !((A.allow >> k) & 1u)) break;
-        if (q > 0) s += pp[k] * (lpp[k] - lpi[x] - lpj[y]);
-        break;
-      case RSG_MI: case RSG_MIg: case RSG_MIr:
-        if (pmi[x] > 0 && pmj[y] > 0) s += pp[k] * (lpp[k] - lpi[x] - lpj[y]);
-        if (type == RSG_MIr) H -= pp[k] * lpp[k];
-        break;
-      }
-    }
-  switch (type) {
-  case RSG_CHI: case RSG_OMES: return (ns > 0) ? (double)(ns * s) : 0.;
-  case RSG_GT:   return (ns > 0) ? 2. * (double)ns * (double)s : 0.;
-  case RSG_MI:   return (double)s;
-  case RSG_MIg:  return (double)s - ((nsd > 0) ? (A.wsum - nsd) / nsd : 0.);
-  case RSG_MIr:  return (H > (T)1e-2) ? (double)(s / H) : 0.;
-  }
-  return 0.;
-}
-
 // 32 x 32 pair tile per block; COV written to both triangles through an LDS transpose.
 template <typename T, int KIND, bool TAB>
 __global__ void __launch_bounds__(256) k_stat(StatArgs A, int nb)
 {
-  __shared__ double tile[32][33];
-  __shared__ double rmin[256], rmax[256];
-  const int bi = blockIdx.x % nb, bj = blockIdx.x / nb;
-  const int tid = threadIdx.x;
-  double lmin = INFINITY, lmax = -INFINITY;
-  if (bi <= bj) {
-    // phase 1: consecutive threads take consecutive i (consecutive records j*L + i),
-    // write row j of COV; phase 2 writes row i from the LDS transpose.
-    const int ii = tid & 31;
-    for (int k = 0; k < 4; k++) {
-      const int jj = (tid >> 5) + 8 * k;
-      const int64_t i = (int64_t)bi * 32 + ii, j = (int64_t)bj * 32 + jj;
-      double v = -INFINITY;
-      if (i < j && j < A.L) {
-        double cd[16];
-        load_rec<KIND>(A.cnt, rsg_rec(i, j, A.L), cd);
-        v = pair_stat<T, TAB>(A, cd, i, j);
-        lmin = fmin(lmin, v); lmax = fmax(lmax, v);
-        A.cov[j * A.L + i] = v;
-      } else if (i == j && i < A.L) {
-        A.cov[i * A.L + j] = -INFINITY;
-      }
-      tile[jj][ii] = v;
-    }
-    __syncthreads();
-    const int jj = tid & 31;
-    for (int k = 0; k < 4; k++) {
-      const int ii2 = (tid >> 5) + 8 * k;
-      const int64_t i = (int64_t)bi * 32 + ii2, j = (int64_t)bj * 32 + jj;
-      if (i < j && j < A.L) A.cov[i * A.L + j] = tile[jj][ii2];
-    }
-  }
-  rmin[tid] = lmin; rmax[tid] = lmax;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s) { rmin[tid] = fmin(rmin[tid], rmin[tid + s]); rmax[tid] = fmax(rmax[tid], rmax[tid + s]); }
-    __syncthreads();
-  }
-  if (tid == 0) { A.part[2 * blockIdx.x] = rmin[0]; A.part[2 * blockIdx.x + 1] = rmax[0]; }
+  stat_tile<T, KIND, TAB>(A, (int)(blockIdx.x % nb), (int)(blockIdx.x / nb), (int64_t)blockIdx.x);
 }
 
 __global__ void k_reduce_minmax(const double *__restrict__ part, int n, double *__restrict__ out)

@@ -68,10 +68,118 @@ def _declare(L):
         "rsg_cov_Calculate": (i32, [vp, vp, i32, i32, i32, i32, vp, i32, i32, d,
                                     C.POINTER(d), C.POINTER(d), cp]),
     }
+    hp = C.POINTER(Hist)
+    sig.update({
+        "rsg_width": (d, [d, d, d, d, i32, d]),
+        "rsg_cov_RankHist": (i32, [vp, vp, d, d, d, C.POINTER(hp), cp]),
+        "rsg_hist_NullAdd": (i32, [C.POINTER(hp), hp, cp]),
+        "rsg_hist_NullFit": (i32, [vp, hp, d, d, i32, C.POINTER(C.POINTER(d)), C.POINTER(d), cp]),
+        "rsg_cov_Evalues": (i32, [vp, vp, hp, C.POINTER(d), i64, vp, cp]),
+        "rsg_cov2evalue": (d, [d, i64, hp, C.POINTER(d)]),
+        "rsg_evalue2cov": (d, [d, i64, hp, C.POINTER(d)]),
+        "rsg_hist_create": (hp, [d, d, d]),
+        "rsg_hist_destroy": (None, [hp]),
+        "rsg_free": (None, [vp]),
+        "rsg_hist_WriteNull": (i32, [cp, hp, cp]),
+        "rsg_hist_ReadNull": (i32, [cp, C.POINTER(hp), cp]),
+        "rsg_weights_PB": (i32, [vp, vp, i64, i64, vp, cp]),
+        "rsg_weights_GSC": (i32, [vp, vp, i64, i64, vp, cp]),
+        "rsg_gapcol_mask": (i32, [vp, vp, i64, i64, d, vp]),
+    })
     for name, (res, args) in sig.items():
         if hasattr(L, name):
             f = getattr(L, name)
             f.restype, f.argtypes = res, args
+
+
+class Hist(C.Structure):
+    """rsg_hist (include/rscape_hip.h): Easel-style score histogram."""
+    _fields_ = [("bmin", C.c_double), ("bmax", C.c_double), ("w", C.c_double),
+                ("nb", C.c_int32), ("imin", C.c_int32), ("imax", C.c_int32), ("cmin", C.c_int32),
+                ("xmin", C.c_double), ("xmax", C.c_double), ("phi", C.c_double),
+                ("n", C.c_uint64), ("Nc", C.c_uint64), ("No", C.c_uint64), ("tailfit", C.c_int32),
+                ("obs", C.POINTER(C.c_uint64))]
+
+    def counts(self):
+        return np.array([self.obs[b] for b in range(self.nb)], dtype=np.uint64)
+
+
+class Histogram:
+    """Owner of an rsg_hist allocated by the library."""
+
+    def __init__(self, ptr):
+        self.p = ptr
+
+    @property
+    def h(self):
+        return self.p.contents
+
+    def __del__(self):
+        try:
+            if self.p:
+                lib().rsg_hist_destroy(self.p)
+                self.p = None
+        except Exception:
+            pass
+
+    def null_add(self_or_none, other):
+        """Merge `other` into this null histogram (null_add2cumranklist); returns the merged one."""
+        err = _err()
+        p = C.POINTER(Hist)() if self_or_none is None else self_or_none.p
+        _check(lib().rsg_hist_NullAdd(C.byref(p), other.p, err), err, "rsg_hist_NullAdd")
+        if self_or_none is None:
+            return Histogram(p)
+        self_or_none.p = p
+        return self_or_none
+
+    def fit(self, ctx, pmass=5e-4, fracfit=1.0, expo=True):
+        """Tail fit on the device; returns (fit dict, survfit array or None)."""
+        err = _err()
+        sf = C.POINTER(C.c_double)()
+        out = (C.c_double * 5)()
+        _check(lib().rsg_hist_NullFit(ctx.h, self.p, pmass, fracfit, 1 if expo else 0, C.byref(sf), out, err),
+               err, "rsg_hist_NullFit")
+        fit = dict(mu=out[0], **{"lambda": out[1]}, tau=out[2], newmass=out[3], pmass=out[4])
+        if sf:
+            arr = np.ctypeslib.as_array(sf, shape=(2 * self.h.nb,)).copy()
+            lib().rsg_free(C.cast(sf, C.c_void_p))
+            return fit, arr
+        return fit, None
+
+    def cov2evalue(self, cov, Nc, survfit=None):
+        sfp = None if survfit is None else survfit.ctypes.data_as(C.POINTER(C.c_double))
+        return lib().rsg_cov2evalue(cov, Nc, self.p, sfp)
+
+    def evalue2cov(self, E, Nc, survfit=None):
+        sfp = None if survfit is None else survfit.ctypes.data_as(C.POINTER(C.c_double))
+        return lib().rsg_evalue2cov(E, Nc, self.p, sfp)
+
+    def write_null(self, path):
+        err = _err()
+        _check(lib().rsg_hist_WriteNull(path.encode(), self.p, err), err, "rsg_hist_WriteNull")
+
+    @staticmethod
+    def read_null(path):
+        err = _err()
+        p = C.POINTER(Hist)()
+        _check(lib().rsg_hist_ReadNull(path.encode(), C.byref(p), err), err, "rsg_hist_ReadNull")
+        return Histogram(p)
+
+
+def width(maxCOV, minCOV, bmin=-10.0, w_old=0.05, hpts=400, tol=1e-6):
+    """calculate_width_histo's histogram step (R-scape.c:1353-1360)."""
+    return lib().rsg_width(maxCOV, minCOV, bmin, w_old, hpts, tol)
+
+
+def gapcol_mask(ax, wgt=None, gapthresh=0.75):
+    """msamanip_RemoveGapColumns' column test: (useme bool array, kept count)."""
+    ax = np.ascontiguousarray(ax, dtype=np.uint8)
+    useme = np.zeros(ax.shape[1], dtype=np.int32)
+    w = None if wgt is None else np.ascontiguousarray(wgt, dtype=np.float64)
+    kept = lib().rsg_gapcol_mask(_p(ax), _p(w), ax.shape[0], ax.shape[1], gapthresh, _p(useme))
+    if kept < 0:
+        raise RscapeError("rsg_gapcol_mask: bad arguments")
+    return useme.astype(bool), kept
 
 
 def exported_symbols():
@@ -131,6 +239,19 @@ class Context:
 
     def msa(self, ax, wgt=None):
         return Msa(self, ax, wgt)
+
+    def weights(self, ax, method="PB"):
+        """esl_msaweight_PB / esl_msaweight_GSC weights of an N x L code array (FP64, sum N)."""
+        ax = np.ascontiguousarray(ax, dtype=np.uint8)
+        w = np.zeros(ax.shape[0])
+        err = _err()
+        f = {"PB": self.L.rsg_weights_PB, "GSC": self.L.rsg_weights_GSC}[method]
+        _check(f(self.h, _p(ax), ax.shape[0], ax.shape[1], _p(w), err), err, "rsg_weights_" + method)
+        return w
+
+    def msaweight(self, ax, maxsq_gsc=1000):
+        """R-scape's choice (R-scape.c:1554-1557): GSC up to maxsq_gsc sequences, PB above."""
+        return self.weights(ax, "GSC" if np.asarray(ax).shape[0] <= maxsq_gsc else "PB")
 
 
 class Msa:
@@ -204,6 +325,23 @@ class Msa:
         _check(self.ctx.L.rsg_corr_GetCOV(self.ctx.h, self.h, _p(out), C.byref(mn), C.byref(mx), err),
                err, "rsg_corr_GetCOV")
         return (None if out is None else out.reshape(L, L)), mn.value, mx.value
+
+    def rank_hist(self, bmin=-10.0, w=0.05, tol=1e-6):
+        """Histogram of max(COV_ij, bmin + w) over i<j (cov_SignificantPairs_Ranking)."""
+        err = _err()
+        p = C.POINTER(Hist)()
+        _check(self.ctx.L.rsg_cov_RankHist(self.ctx.h, self.h, bmin, w, tol, C.byref(p), err), err, "rsg_cov_RankHist")
+        return Histogram(p)
+
+    def evalues(self, null, survfit, Nc=None):
+        """E-value of every pair against a fitted null (cov2evalue * Nc)."""
+        L = self.Lc
+        Nc = L * (L - 1) // 2 if Nc is None else Nc
+        out = np.zeros(L * L)
+        err = _err()
+        sfp = None if survfit is None else survfit.ctypes.data_as(C.POINTER(C.c_double))
+        _check(self.ctx.L.rsg_cov_Evalues(self.ctx.h, self.h, null.p, sfp, Nc, _p(out), err), err, "rsg_cov_Evalues")
+        return out.reshape(L, L)
 
     def calculate(self, covtype="GTp", covclass=CSELECT, counts=COUNTS_INT8, prec=STAT_FP64, allowpair=None,
                   nseqthresh=8, alenthresh=50, tol=1e-6):

@@ -1,0 +1,116 @@
+"""Phylogenetic null simulation and the batched null pipeline (ctypes binding).
+
+Trees follow the Easel ESL_TREE layout (internal nodes 0..N-2, root 0, parents first;
+children > 0 internal, <= 0 leaf -child).  Substitution model: 4-state GTR scaled to one
+expected substitution per unit branch length.  All arithmetic runs in librscape_hip.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _check, _err, _p, lib, COV
+
+# R-scape-sim-like defaults: uniform composition, transitions twice transversions
+UNIFORM = (0.25, 0.25, 0.25, 0.25)
+EXCH = (1.0, 2.0, 1.0, 1.0, 2.0, 1.0)          # AC AG AU CG CU GU
+
+
+class NullJob(C.Structure):
+    _fields_ = [("nleaf", C.c_int64), ("alen", C.c_int64),
+                ("left", C.POINTER(C.c_int)), ("right", C.POINTER(C.c_int)),
+                ("ld", C.POINTER(C.c_double)), ("rd", C.POINTER(C.c_double)),
+                ("model", C.POINTER(C.c_double)), ("seed", C.c_uint64), ("stream", C.c_uint32),
+                ("gapfrac", C.c_double)]
+
+
+def _declare():
+    L = lib()
+    if getattr(L, "_null_declared", False):
+        return L
+    d, i64, vp, cp, i32 = C.c_double, C.c_int64, C.c_void_p, C.c_char_p, C.c_int
+    L.rsg_gtr_model.restype, L.rsg_gtr_model.argtypes = i32, [vp, vp, vp]
+    L.rsg_gtr_cdf.restype, L.rsg_gtr_cdf.argtypes = i32, [vp, vp, i64, vp, vp, cp]
+    L.rsg_tree_yule.restype = i32
+    L.rsg_tree_yule.argtypes = [i64, d, C.c_uint64, C.c_uint32, vp, vp, vp, vp]
+    L.rsg_null_simulate.restype, L.rsg_null_simulate.argtypes = i32, [vp, C.POINTER(NullJob), vp, cp]
+    L.rsg_null_batch.restype = i32
+    L.rsg_null_batch.argtypes = [vp, C.POINTER(NullJob), i32, i32, d, d, d, i32, vp, vp, i32, cp]
+    L._null_declared = True
+    return L
+
+
+def gtr_model(pi=UNIFORM, exch=EXCH):
+    """Eigendecomposed GTR model (40 doubles) for the device."""
+    L = _declare()
+    m = np.zeros(40)
+    p = np.ascontiguousarray(pi, dtype=np.float64)
+    e = np.ascontiguousarray(exch, dtype=np.float64)
+    if L.rsg_gtr_model(_p(p), _p(e), _p(m)) != 0:
+        raise ValueError("bad GTR parameters")
+    return m
+
+
+def gtr_cdf(ctx, model, t):
+    """Device-built inverse-CDF thresholds of P(t) for each t: array (len(t), 4, 3) uint32."""
+    L = _declare()
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    out = np.zeros(len(t) * 12, dtype=np.uint32)
+    err = _err()
+    _check(L.rsg_gtr_cdf(ctx.h, _p(model), len(t), _p(t), _p(out), err), err, "rsg_gtr_cdf")
+    return out.reshape(len(t), 4, 3)
+
+
+def yule_tree(n, mean_bl, seed=42, stream=0):
+    """Yule tree (dict with left, right, ld, rd, n) from the library's generator."""
+    L = _declare()
+    left = np.zeros(n - 1, dtype=np.int32); right = np.zeros(n - 1, dtype=np.int32)
+    ld = np.zeros(n - 1); rd = np.zeros(n - 1)
+    if L.rsg_tree_yule(n, mean_bl, seed, stream, _p(left), _p(right), _p(ld), _p(rd)) != 0:
+        raise ValueError("tree needs n >= 2")
+    return dict(left=left, right=right, ld=ld, rd=rd, n=n)
+
+
+class Job:
+    """One alignment to simulate: tree, length, model and RNG key."""
+
+    def __init__(self, tree, L, model, seed=42, stream=0, gapfrac=0.0):
+        self.tree = {k: (np.ascontiguousarray(v, dtype=np.int32) if k in ("left", "right") else
+                         np.ascontiguousarray(v, dtype=np.float64) if k in ("ld", "rd") else v)
+                     for k, v in tree.items()}
+        self.L, self.model = int(L), np.ascontiguousarray(model, dtype=np.float64)
+        self.seed, self.stream, self.gapfrac = seed, stream, gapfrac
+
+    def struct(self):
+        t = self.tree
+        ip = C.POINTER(C.c_int); dp = C.POINTER(C.c_double)
+        return NullJob(t["n"], self.L, t["left"].ctypes.data_as(ip), t["right"].ctypes.data_as(ip),
+                       t["ld"].ctypes.data_as(dp), t["rd"].ctypes.data_as(dp), self.model.ctypes.data_as(dp),
+                       self.seed, self.stream, self.gapfrac)
+
+
+def simulate(ctx, job):
+    """Leaves (N x L uint8) of one simulated alignment."""
+    L = _declare()
+    out = np.zeros((job.tree["n"], job.L), dtype=np.uint8)
+    s = job.struct()
+    err = _err()
+    _check(L.rsg_null_simulate(ctx.h, C.byref(s), _p(out), err), err, "rsg_null_simulate")
+    return out
+
+
+def batch(ctx, jobs, covtype="GTp", bmin=-10.0, pmass=5e-4, fracfit=1.0, expo=True, graph=0, hists=False):
+    """Simulate + score + fit a batch on the device.  Returns (fits array (n, 8), histograms).
+    fits columns: mu, lambda, tau, newmass, pmass, w, minCOV, maxCOV."""
+    from . import Hist, Histogram
+    L = _declare()
+    n = len(jobs)
+    arr = (NullJob * n)(*[j.struct() for j in jobs])
+    fits = np.zeros(n * 8)
+    hp = (C.POINTER(Hist) * n)() if hists else None
+    err = _err()
+    t = COV[covtype] if isinstance(covtype, str) else covtype
+    _check(L.rsg_null_batch(ctx.h, arr, n, t, bmin, pmass, fracfit, 1 if expo else 0,
+                            C.cast(hp, C.c_void_p) if hists else None, _p(fits), int(graph), err),
+           err, "rsg_null_batch")
+    hs = [Histogram(hp[k]) if (hists and hp[k]) else None for k in range(n)] if hists else None
+    return fits.reshape(n, 8), hs

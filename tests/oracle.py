@@ -127,3 +127,89 @@ def canonicalize(ax):
     ax = np.ascontiguousarray(ax, dtype=np.uint8).copy()
     lib().ref_canonicalize(P(ax), C.c_int64(ax.size))
     return ax
+
+
+class RefHist(C.Structure):
+    """ref_hist of oracle/refcpu.h."""
+    _fields_ = [("bmin", C.c_double), ("bmax", C.c_double), ("w", C.c_double), ("nb", C.c_int),
+                ("obs", C.POINTER(C.c_uint64)), ("imin", C.c_int), ("imax", C.c_int), ("cmin", C.c_int),
+                ("xmin", C.c_double), ("xmax", C.c_double), ("phi", C.c_double),
+                ("n", C.c_uint64), ("Nc", C.c_uint64), ("No", C.c_uint64), ("tailfit", C.c_int)]
+
+    def counts(self):
+        return np.array([self.obs[b] for b in range(self.nb)], dtype=np.uint64)
+
+
+def _hp(p):
+    return C.cast(p, C.POINTER(RefHist))
+
+
+def rank_hist(COV, L, maxCOV, bmin=-10.0, w=0.05, tol=1e-6):
+    p = lib().ref_rank_hist(C.c_int64(L), P(np.ascontiguousarray(COV, dtype=np.float64)), maxCOV, bmin, w, tol)
+    return p
+
+
+def null_add(cum, h):
+    cp = C.c_void_p(cum)
+    lib().ref_null_add(C.byref(cp), C.c_void_p(h))
+    return cp.value
+
+
+def null_fit(h, pmass=5e-4, fracfit=1.0, expo=True):
+    sf = C.c_void_p()
+    fit = np.zeros(5)
+    lib().ref_null_fit.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_void_p), C.c_void_p]
+    r = lib().ref_null_fit(C.c_void_p(h), pmass, fracfit, 1 if expo else 0, C.byref(sf), P(fit))
+    if r:
+        raise RuntimeError("ref_null_fit failed")
+    nb = _hp(h).contents.nb
+    arr = None
+    if sf.value:
+        arr = np.ctypeslib.as_array(C.cast(sf, C.POINTER(C.c_double)), shape=(2 * nb,)).copy()
+    return dict(mu=fit[0], **{"lambda": fit[1]}, tau=fit[2], newmass=fit[3], pmass=fit[4]), arr
+
+
+def evalues(COV, L, Nc, h, survfit):
+    Ev = np.zeros(L * L)
+    lib().ref_evalues.argtypes = [C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib().ref_evalues(L, P(np.ascontiguousarray(COV, dtype=np.float64)), Nc, C.c_void_p(h),
+                      None if survfit is None else P(survfit), P(Ev))
+    return Ev.reshape(L, L)
+
+
+def evolve_gtr(tree, L, model, seed, stream, gapthr=0):
+    n = tree["n"]
+    out = np.zeros((n, L), dtype=np.uint8)
+    f = lib().ref_evolve_gtr
+    f.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p]
+    left = np.ascontiguousarray(tree["left"], dtype=np.int32); right = np.ascontiguousarray(tree["right"], dtype=np.int32)
+    ld = np.ascontiguousarray(tree["ld"], dtype=np.float64); rd = np.ascontiguousarray(tree["rd"], dtype=np.float64)
+    f(n, L, P(left), P(right), P(ld), P(rd), P(np.ascontiguousarray(model, dtype=np.float64)), seed, stream, gapthr, P(out))
+    return out
+
+
+def mlp_forward_backward(W, b, x, dy=None):
+    """FP64 oracle forward (and backward for dy) of the surrogate MLP."""
+    nl = len(W)
+    dims = [W[0].shape[1]] + [w.shape[0] for w in W]
+    d = (C.c_int * (nl + 1))(*dims)
+    Wd = [np.ascontiguousarray(w, dtype=np.float64) for w in W]
+    bd = [np.ascontiguousarray(v, dtype=np.float64) for v in b]
+    Wp = (C.c_void_p * nl)(*[w.ctypes.data for w in Wd])
+    bp = (C.c_void_p * nl)(*[v.ctypes.data for v in bd])
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    B = x.shape[0]
+    aw = sum(dims[l + 1] + (dims[l + 1] if l < nl - 1 else 0) for l in range(nl))
+    acts = np.zeros(B * aw)
+    y = np.zeros((B, dims[-1]))
+    L = lib()
+    L.ref_mlp_forward(nl, d, Wp, bp, P(x), C.c_int64(B), P(y), P(acts))
+    if dy is None:
+        return y
+    dW = [np.zeros_like(w) for w in Wd]
+    db = [np.zeros_like(v) for v in bd]
+    dWp = (C.c_void_p * nl)(*[w.ctypes.data for w in dW])
+    dbp = (C.c_void_p * nl)(*[v.ctypes.data for v in db])
+    L.ref_mlp_backward(nl, d, Wp, bp, P(x), C.c_int64(B), P(acts), P(np.ascontiguousarray(dy, dtype=np.float64)), dWp, dbp)
+    return y, dW, db

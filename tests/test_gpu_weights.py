@@ -1,0 +1,83 @@
+"""Sequence weights on the device against the oracle.
+
+GSC: the identity counts are integers and the UPGMA/GSC arithmetic is the same FP64 sequence,
+so the weights match the oracle to the last bit, including alignments with many tied
+distances (duplicated sequences).  PB: the per-sequence sums are reduced in another order,
+so they agree to 1e-12 relative.  The tutorial alignment pins GSC end to end.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from rscape_amd import stockholm
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rand_msa(rng, N, L, gap=0.1, degen=0.0):
+    ax = rng.integers(0, 4, size=(N, L), dtype=np.uint8)
+    ax[rng.random((N, L)) < gap] = 4
+    if degen:
+        ax[rng.random((N, L)) < degen] = rng.integers(5, 18, size=1)[0]
+    return ax
+
+
+@pytest.mark.parametrize("N,L,seed", [(1, 10, 0), (2, 5, 1), (63, 40, 2), (64, 33, 3), (65, 100, 4), (300, 257, 5)])
+def test_gsc_matches_oracle(ctx, N, L, seed):
+    rng = np.random.default_rng(seed)
+    ax = rand_msa(rng, N, L)
+    got = ctx.weights(ax, "GSC")
+    ref = oracle.weights_gsc(ax)
+    assert np.array_equal(got, ref), np.max(np.abs(got - ref))
+    assert abs(got.sum() - N) < 1e-9 * N
+
+
+def test_gsc_ties_and_duplicates(ctx):
+    rng = np.random.default_rng(7)
+    base = rand_msa(rng, 40, 60, gap=0.05)
+    ax = np.concatenate([base, base[:15], base[5:25]], axis=0)   # exact duplicates -> D = 0 ties
+    ax = ax[rng.permutation(len(ax))]
+    got = ctx.weights(ax, "GSC")
+    assert np.array_equal(got, oracle.weights_gsc(ax))
+    ax2 = np.zeros((50, 30), np.uint8)                           # all identical: equal weights
+    assert np.allclose(ctx.weights(ax2, "GSC"), 1.0)
+
+
+@pytest.mark.parametrize("N,L,seed", [(1, 10, 0), (7, 3, 1), (500, 300, 2), (5000, 1500, 3)])
+def test_pb_matches_oracle(ctx, N, L, seed):
+    rng = np.random.default_rng(seed)
+    ax = rand_msa(rng, N, L, gap=0.2, degen=0.02)
+    got = ctx.weights(ax, "PB")
+    ref = oracle.weights_pb(ax)
+    assert np.allclose(got, ref, rtol=1e-12, atol=0)
+
+
+def test_pb_all_gap_rows(ctx):
+    ax = np.full((6, 8), 4, np.uint8)
+    ax[0, :4] = [0, 1, 2, 3]
+    got = ctx.weights(ax, "PB")
+    assert np.allclose(got, oracle.weights_pb(ax), rtol=1e-12)
+
+
+def test_tutorial_scores_through_device_weights(ctx):
+    """GSC weights on the device + gap filter + GTp reproduce the tutorial's published scores."""
+    import rscape_amd as R
+    names, ax, _ = stockholm.read(os.path.join(GOLD, "updated_Arisong.sto"))
+    gold = json.load(open(os.path.join(GOLD, "arisong_gtp.json")))
+    use, kept = R.gapcol_mask(ax, np.ones(len(names)), 0.75)
+    assert kept == gold["alen_analysed"]
+    assert np.array_equal(use, oracle.gapcol_filter(ax, np.ones(len(names)), 0.75))
+    sub = np.ascontiguousarray(ax[:, use])
+    w = ctx.msaweight(sub)
+    assert np.array_equal(w, oracle.weights_gsc(sub))
+    msa = ctx.msa(sub, w)
+    msa.calculate("GTp", counts=R.COUNTS_BF16X3)
+    cov, _, _ = msa.get_cov()
+    cols = np.nonzero(use)[0]
+    pos = {c + 1: k for k, c in enumerate(cols)}
+    errs = [abs(cov[pos[i], pos[j]] - s) for i, j, s in gold["pairs"]]
+    assert max(errs) < 0.05 and sorted(errs)[-2] < 0.005
