@@ -29,6 +29,7 @@ from rscape_amd import synth  # noqa: E402
 PEAK_INT8_TOPS = 5000.0      # MI355X dense int8 MFMA, TOP/s
 PEAK_BF16_TFLOPS = 2500.0    # MI355X dense bf16 MFMA, TFLOP/s
 PEAK_HBM_GBS = 8000.0
+METRIC = "column-pair\u00b7seq updates/s (GT+APC); E-value runs/s, surrogate vs full null"
 
 
 def kernel_avgs(timings):
@@ -70,14 +71,29 @@ def cpu_baseline(ax, Lwin):
                       "N=%d alignment, %.1f s" % (Lwin - 1, N, dt)}
 
 
-def load_traffic():
-    p = os.path.join(ROOT, "profiles", "gram_i8_c1_traffic.json")
+def load_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 counter summary (or None)."""
+    p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
-        try:
-            return json.load(open(p)).get("traffic_bytes_per_launch")
-        except Exception:
-            return None
+        for line in open(p):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
     return None
+
+
+CALIB_BYTES = 256 << 20
+
+
+def calibration_kernel(ctx):
+    """The library's plain copy kernel (k_copy_probe: reads and writes CALIB_BYTES per launch),
+    run outside the timed region so rocprofv3 counter runs can convert FETCH_SIZE /
+    WRITE_SIZE to bytes."""
+    import ctypes
+    err = ctypes.create_string_buffer(128)
+    L = R.lib()
+    L.rsg_ctx_copy_probe.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p]
+    if L.rsg_ctx_copy_probe(ctx.h, CALIB_BYTES, err) != 0:
+        raise RuntimeError(err.value.decode())
 
 
 def main():
@@ -90,6 +106,7 @@ def main():
     ap.add_argument("--cpu-window", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--calib", action="store_true", help="also run the counter calibration kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -112,7 +129,7 @@ def main():
     gram_ops = 2.0 * 16.0 * upd                      # 16 int8 MACs per update (4x4 canonical table)
     achieved = gram_ops / (g_ms * 1e-3) / 1e12
     out = {
-        "metric": "column-pair*seq updates/s (GT+APC)",
+        "metric": METRIC,
         "value": value,
         "unit": "updates/s",
         "n_gpus": 1,
@@ -128,7 +145,8 @@ def main():
                                "statistics), one full scoring per step" % (N, L),
                    "N": N, "L": L, "covtype": "GTp", "counts": "int8 one-hot Gram", "statistic": "FP64"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
-                     "frac": achieved / PEAK_INT8_TOPS, "traffic": load_traffic(),
+                     "frac": achieved / PEAK_INT8_TOPS,
+                     "traffic": load_traffic("gram_i8_n5000_l1500_traffic.txt"),
                      "kernel": "k_gram_i8", "kernel_ms": g_ms,
                      "algorithmic_ops_per_launch": gram_ops},
         "kernels_ms": {k: v[0] for k, v in kav.items()},
@@ -154,6 +172,8 @@ def main():
         dt3, _ = run_steps(ctx, m, max(3, args.steps // 4), **kw3)
         extra["fp32_statistic"] = {"value": upd * max(3, args.steps // 4) / dt3, "unit": "updates/s"}
         out["extra"] = extra
+    if args.calib:
+        calibration_kernel(ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ax, min(args.cpu_window, L))
     m.close()

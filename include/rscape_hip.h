@@ -59,6 +59,10 @@ int      rsg_ctx_timings(rsg_ctx *ctx, const char **names, double *ms, int *n, i
 int         rsg_gram_nshapes(void);
 const char *rsg_gram_shape_name(int shape);
 int         rsg_ctx_set_gram_shape(rsg_ctx *ctx, int shape);
+/* Counter calibration (no reference counterpart): runs the plain copy kernel
+ * k_copy_probe over `bytes` (multiple of 16) three times, reading and writing `bytes` per
+ * launch, so HBM counters of other kernels can be converted to bytes. */
+int         rsg_ctx_copy_probe(rsg_ctx *ctx, size_t bytes, char *errbuf);
 
 /* ---- alignment upload (replaces the ESL_MSA the corr_* calls read: ax[s][1..L], wgt[s]) ----
  * ax: nseq x alen digital codes, row-major, 0-based columns (msa->ax[s][i+1] in Easel).
@@ -172,14 +176,18 @@ typedef struct {
   uint64_t seed;
   uint32_t stream;
   double   gapfrac;          /* i.i.d. gap probability per leaf symbol (0 = none) */
+  int32_t  group;            /* rsg_null_batch: replicates of one null share a group id */
 } rsg_null_job;
 /* Simulate one alignment; leaves = nleaf x alen codes (the leaf rows, leaf order). */
 int rsg_null_simulate(rsg_ctx *ctx, const rsg_null_job *job, uint8_t *leaves, char *errbuf);
 /* Simulate and score a batch of alignments of any sizes (GT/MI/... + correction, unit
- * weights, CSELECT) entirely on the device, then per alignment: histogram width
- * (calculate_width_histo), rank histogram, and an FP64 tail fit.  Outputs per job:
- * fits[job*8 + {0..7}] = {mu, lambda, tau, newmass, pmass, w, minCOV, maxCOV}; hists (may
- * be NULL) receives each job's histogram.  use_graph = k >= 1 records the whole batch in one
+ * weights, CSELECT) entirely on the device, then build one null per group of jobs: jobs that
+ * share a group id (ids contiguous, ascending from 0) are the replicates of one null, as
+ * R-scape's nshuffle loop makes them: the histogram width comes from the group's first job
+ * (calculate_width_histo), every replicate's pairs accumulate into one histogram
+ * (null_add2cumranklist) and one FP64 tail fit is made per group.  Outputs per group g:
+ * fits[g*8 + {0..7}] = {mu, lambda, tau, newmass, pmass, w, minCOV, maxCOV}; hists (may be
+ * NULL) receives each group's histogram.  use_graph = k >= 1 records the whole batch in one
  * hipGraph and launches it k times (each launch recomputes everything); 0 runs eagerly. */
 int rsg_null_batch(rsg_ctx *ctx, const rsg_null_job *jobs, int njobs, int covtype,
                    double bmin, double pmass, double fracfit, int doexpfit,

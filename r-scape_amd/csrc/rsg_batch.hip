@@ -25,7 +25,7 @@
 #pragma clang fp contract(off)
 
 enum { D_N = 0, D_L, D_NPAD, D_LPAD, D_XT, D_CNT, D_COV, D_PM, D_ROWS, D_HIST, D_LEAF, D_BPACK, D_BCOL,
-       D_BSTAT, D_BEL, D_NB32 };
+       D_BSTAT, D_BEL, D_NB32, D_GROUP };
 #define NBMAX 16384          // histogram bins per item (an item needing more is reported as failed)
 #define ELPB 1024            // matrix elements per block in the element-wise kernels
 
@@ -165,32 +165,43 @@ k_item_minmax(const int64_t *__restrict__ desc, int field, int64_t nblk_total, i
   if (threadIdx.x == 0) { mm[2 * it] = rmin[0]; mm[2 * it + 1] = rmax[0]; }
 }
 
-// histogram geometry per item: calculate_width_histo (w) then cov_CreateRankList bounds
-struct HGeo { double w, bmax, xmin, xmax; int nb, imin, imax, ok; };
+// Histogram geometry per null (group of replicates): the width comes from the group's first
+// replicate (calculate_width_histo runs on the first null replicate, src/R-scape.c:1353);
+// the bounds cover every replicate, as the cumulative histogram grows to do
+// (null_add2cumranklist, src/R-scape.c:1564-1612).
+struct HGeo { double w, bmax, xmin, xmax, npairs, mn, mx; int nb, imin, imax, ok; };
 
-__global__ void k_width_b(int n, const double *__restrict__ mm, double bmin, double w0, int hpts, double tol,
+__global__ void k_width_b(int ng, const int *__restrict__ gfirst, const int64_t *__restrict__ desc,
+                          const double *__restrict__ mm, double bmin, double w0, int hpts, double tol,
                           HGeo *__restrict__ geo)
 {
-  int it = blockIdx.x * blockDim.x + threadIdx.x;
-  if (it >= n) return;
-  HGeo g; g.ok = 0;
-  double mn = mm[2 * it], mx = mm[2 * it + 1];
-  double lo = fmax(bmin, mn);
-  double w = fmin(w0, (mx - lo) / (double)hpts);
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int j0 = gfirst[g], j1 = gfirst[g + 1];
+  HGeo G; G.ok = 0;
+  const double lo = fmax(bmin, mm[2 * j0]);
+  double w = fmin(w0, (mm[2 * j0 + 1] - lo) / (double)hpts);
   if (w < tol) w = 0.;
-  g.w = w; g.bmax = 0.; g.xmin = g.xmax = 0.; g.nb = 0; g.imin = 0; g.imax = -1;
+  double mn = INFINITY, mx = -INFINITY, np = 0.;
+  for (int j = j0; j < j1; j++) {
+    mn = fmin(mn, mm[2 * j]); mx = fmax(mx, mm[2 * j + 1]);
+    const int64_t L = desc[j * RSG_BDESC + D_L];
+    np += (double)(L * (L - 1) / 2);
+  }
+  G.w = w; G.bmax = 0.; G.xmin = G.xmax = 0.; G.nb = 0; G.imin = 0; G.imax = -1;
+  G.npairs = np; G.mn = mn; G.mx = mx;
   if (w > 0. && mx > bmin) {
     double bmax = mx + 5 * w;
     while (fabs(bmax - bmin) < tol) bmax += w;
-    double nbd = (bmax - bmin) / w;
+    const double nbd = (bmax - bmin) / w;
     if (nbd < (double)NBMAX) {
-      g.nb = (int)nbd; g.bmax = bmax;
-      g.xmin = fmax(mn, bmin + w); g.xmax = fmax(mx, bmin + w);
-      g.imin = (int)ceil((g.xmin - bmin) / w - 1.); g.imax = (int)ceil((g.xmax - bmin) / w - 1.);
-      g.ok = (g.imin >= 0 && g.imax < g.nb) ? 1 : 0;
+      G.nb = (int)nbd; G.bmax = bmax;
+      G.xmin = fmax(mn, bmin + w); G.xmax = fmax(mx, bmin + w);
+      G.imin = (int)ceil((G.xmin - bmin) / w - 1.); G.imax = (int)ceil((G.xmax - bmin) / w - 1.);
+      G.ok = (G.imin >= 0 && G.imax < G.nb) ? 1 : 0;
     }
   }
-  geo[it] = g;
+  geo[g] = G;
 }
 
 __global__ void k_fill_u64(unsigned long long *__restrict__ p, int64_t n)
@@ -210,12 +221,12 @@ k_hist_b(const int64_t *__restrict__ desc, int n, const double *__restrict__ cov
          double bmin, unsigned long long *__restrict__ hist)
 {
   const int it = find_item(desc, n, D_BEL, blockIdx.x);
-  const HGeo g = geo[it];
-  if (!g.ok) return;
   const int64_t *d = desc + it * RSG_BDESC;
+  const HGeo g = geo[d[D_GROUP]];
+  if (!g.ok) return;
   const int64_t L = d[D_L], e0 = (blockIdx.x - d[D_BEL]) * ELPB;
   const double *C = cov + d[D_COV];
-  unsigned long long *H = hist + (int64_t)it * NBMAX;
+  unsigned long long *H = hist + d[D_GROUP] * NBMAX;
   const double fl = bmin + g.w;
   for (int64_t e = e0 + threadIdx.x; e < min(e0 + ELPB, L * L); e += 256) {
     int64_t i = e / L, j = e - i * L;
@@ -228,37 +239,29 @@ k_hist_b(const int64_t *__restrict__ desc, int n, const double *__restrict__ cov
   }
 }
 
-// tail fit per item (one 1024-thread block): see rsg_rank.hip for the method
-
+// tail fit per null (one 1024-thread block per group): see rsg_rank.hip for the method
 __global__ void __launch_bounds__(1024)
-k_fit_b(const int64_t *__restrict__ desc, const HGeo *__restrict__ geo, const unsigned long long *__restrict__ hist,
+k_fit_b(const HGeo *__restrict__ geo, const unsigned long long *__restrict__ hist,
         unsigned long long *__restrict__ S, int *__restrict__ NZ, double bmin, double pmass, double fracfit, int doexp,
         double *__restrict__ fits)
 {
-  const int it = blockIdx.x;
-  const HGeo g = geo[it];
-  double *o = fits + it * 8;
-  if (!g.ok) {
-    if (threadIdx.x == 0) { o[0] = NAN; o[1] = INFINITY; o[2] = 0.; o[3] = 0.; o[4] = NAN; o[5] = g.w; }
+  const int g = blockIdx.x;
+  const HGeo G = geo[g];
+  double *o = fits + g * 8;
+  if (threadIdx.x == 0) { o[6] = G.mn; o[7] = G.mx; }
+  if (!G.ok) {
+    if (threadIdx.x == 0) { o[0] = NAN; o[1] = INFINITY; o[2] = 0.; o[3] = 0.; o[4] = NAN; o[5] = G.w; }
     return;
   }
-  const int64_t L = desc[it * RSG_BDESC + D_L];
-  const double npairs = (double)(L * (L - 1) / 2);
-  nullfit_body(hist + (int64_t)it * NBMAX, g.nb, g.imin, g.imax, npairs, npairs, bmin, g.w, pmass, fracfit, doexp,
-               S + (int64_t)it * NBMAX, NZ + (int64_t)it * NBMAX, nullptr, o);
-  if (threadIdx.x == 0) o[5] = g.w;
+  nullfit_body(hist + (int64_t)g * NBMAX, G.nb, G.imin, G.imax, G.npairs, G.npairs, bmin, G.w, pmass, fracfit, doexp,
+               S + (int64_t)g * NBMAX, NZ + (int64_t)g * NBMAX, nullptr, o);
+  if (threadIdx.x == 0) { o[5] = G.w; o[6] = G.mn; o[7] = G.mx; }
 }
 
 __global__ void k_logtab_b(int64_t NM, double *__restrict__ t)
 {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k <= NM) { t[k] = log((double)k + 1e-10); t[NM + 1 + k] = log((double)k + 16e-10); }
-}
-
-__global__ void k_copy_mm(int n, const double *__restrict__ mm, double *__restrict__ fits)
-{
-  int it = blockIdx.x * blockDim.x + threadIdx.x;
-  if (it < n) { fits[it * 8 + 6] = mm[2 * it]; fits[it * 8 + 7] = mm[2 * it + 1]; }
 }
 
 // ------------------------------------------------------------------------------- host side
@@ -286,6 +289,16 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   HIPCHK(hipSetDevice(c->device));
   if (nj < 1) return rsg_fail(errbuf, "empty batch");
   const int base = covtype - covtype % 3, corr = covtype % 3;
+  // groups: contiguous, ascending from 0
+  std::vector<int> gfirst;
+  for (int k = 0; k < nj; k++) {
+    const int g = jobs[k].group;
+    if (k == 0 ? g != 0 : (g != jobs[k - 1].group && g != jobs[k - 1].group + 1))
+      return rsg_fail(errbuf, "job %d: group ids must start at 0 and be contiguous and ascending", k);
+    if (k == 0 || g != jobs[k - 1].group) gfirst.push_back(k);
+  }
+  const int ng = (int)gfirst.size();
+  gfirst.push_back(nj);
   if (base == RSG_RAF || base == RSG_RAFS || base == RSG_CCF)
     return rsg_fail(errbuf, "batched null supports CHI/GT/MI/MIr/MIg/OMES statistics");
   // ---- layout
@@ -305,16 +318,16 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     d[D_HIST] = (int64_t)k * NBMAX; d[D_LEAF] = o_leaf;
     d[D_BPACK] = b_pack; d[D_BCOL] = b_col; d[D_BSTAT] = b_stat; d[D_BEL] = b_el;
     const int64_t nb32 = (L + 31) / 32;
-    d[D_NB32] = nb32;
+    d[D_NB32] = nb32; d[D_GROUP] = J.group;
     cls[(size_t)k] = (N <= 8 || L <= 50) ? RSG_C2 : RSG_C16;           // CSELECT (correlators.c:335-341)
     SimItem &s = sim[(size_t)k];
     s.N = N; s.L = L; s.tree_off = nodes; s.seq_off = seqs; s.leaf_off = o_leaf; s.seed = J.seed; s.stream = J.stream;
     s.gapthr = (J.gapfrac <= 0.) ? 0u : (J.gapfrac >= 1. ? 0xFFFFFFFFu : (uint32_t)(J.gapfrac * 4294967296.0));
-    s.col_blk0 = b_sim;
+    s.col_blk0 = b_sim; s.model_off = 40 * (int64_t)k;
     for (int64_t bi = 0; bi < nb32; bi++)
       for (int64_t bj = bi; bj < nb32; bj++) tiles.push_back(make_int4(k, (int)bi, (int)bj, 0));
     o_xt += Lpad * Npad; o_cnt += 16 * L * L; o_cov += L * L; o_pm += 8 * L; o_rows += 2 * L + 2; o_leaf += N * L;
-    nodes += N - 1; seqs += (2 * N - 1) * L;
+    nodes += N - 1; seqs += (N - 1) * L;
     b_pack += (Lpad / 64) * (Npad / 64); b_col += L; b_stat += nb32 * nb32; b_el += (L * L + ELPB - 1) / ELPB;
     b_sim += (L + 63) / 64;
     nmax = std::max(nmax, N);
@@ -324,14 +337,17 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     return desc[(size_t)a.x * RSG_BDESC + D_NPAD] > desc[(size_t)b.x * RSG_BDESC + D_NPAD];
   });
   std::vector<int> left((size_t)nodes), right((size_t)nodes);
-  std::vector<double> ld((size_t)nodes), rd((size_t)nodes);
+  std::vector<double> ld((size_t)nodes), rd((size_t)nodes), models((size_t)nj * 40);
   for (int k = 0; k < nj; k++) {
     const int64_t off = sim[(size_t)k].tree_off, m1 = jobs[k].nleaf - 1;
     memcpy(&left[(size_t)off], jobs[k].left, sizeof(int) * m1);
     memcpy(&right[(size_t)off], jobs[k].right, sizeof(int) * m1);
     memcpy(&ld[(size_t)off], jobs[k].ld, sizeof(double) * m1);
     memcpy(&rd[(size_t)off], jobs[k].rd, sizeof(double) * m1);
+    memcpy(&models[(size_t)k * 40], jobs[k].model, sizeof(double) * 40);
   }
+  SimLevels lv;
+  if (rsg_sim_levels(sim.data(), nj, left.data(), right.data(), lv, errbuf) != RSG_OK) return RSG_FAIL;
   // ---- device buffers
   int rc = RSG_OK;
   Dev D;
@@ -341,7 +357,9 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   int4 *d_tiles = D.get<int4>(tiles.size(), errbuf, &rc);
   int *d_left = D.get<int>((size_t)nodes, errbuf, &rc), *d_right = D.get<int>((size_t)nodes, errbuf, &rc);
   double *d_ld = D.get<double>((size_t)nodes, errbuf, &rc), *d_rd = D.get<double>((size_t)nodes, errbuf, &rc);
-  double *d_model = D.get<double>(40, errbuf, &rc);
+  double *d_models = D.get<double>(40 * (size_t)nj, errbuf, &rc);
+  int4 *d_ent = D.get<int4>(lv.ent.size(), errbuf, &rc);
+  int *d_gfirst = D.get<int>((size_t)ng + 1, errbuf, &rc);
   uint32_t *d_cdf = D.get<uint32_t>((size_t)nodes * 24, errbuf, &rc);
   uint8_t *d_seqs = D.get<uint8_t>((size_t)seqs, errbuf, &rc);
   uint8_t *d_leaves = D.get<uint8_t>((size_t)o_leaf, errbuf, &rc);
@@ -354,11 +372,11 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   double *d_part_s = D.get<double>(2 * (size_t)b_stat, errbuf, &rc);
   double *d_part_e = D.get<double>(2 * (size_t)b_el, errbuf, &rc);
   double *d_mm = D.get<double>(2 * (size_t)nj, errbuf, &rc);
-  HGeo *d_geo = D.get<HGeo>((size_t)nj, errbuf, &rc);
-  unsigned long long *d_hist = D.get<unsigned long long>((size_t)nj * NBMAX, errbuf, &rc);
-  unsigned long long *d_S = D.get<unsigned long long>((size_t)nj * NBMAX, errbuf, &rc);
-  int *d_NZ = D.get<int>((size_t)nj * NBMAX, errbuf, &rc);
-  double *d_fits = D.get<double>((size_t)nj * 8, errbuf, &rc);
+  HGeo *d_geo = D.get<HGeo>((size_t)ng, errbuf, &rc);
+  unsigned long long *d_hist = D.get<unsigned long long>((size_t)ng * NBMAX, errbuf, &rc);
+  unsigned long long *d_S = D.get<unsigned long long>((size_t)ng * NBMAX, errbuf, &rc);
+  int *d_NZ = D.get<int>((size_t)ng * NBMAX, errbuf, &rc);
+  double *d_fits = D.get<double>((size_t)ng * 8, errbuf, &rc);
   if (rc != RSG_OK) return rc;
   HIPCHK(hipMemcpyAsync(d_desc, desc.data(), sizeof(int64_t) * desc.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_sim, sim.data(), sizeof(SimItem) * nj, hipMemcpyHostToDevice, c->stream));
@@ -368,11 +386,10 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   HIPCHK(hipMemcpyAsync(d_right, right.data(), sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_ld, ld.data(), sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_rd, rd.data(), sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_model, jobs[0].model, sizeof(double) * 40, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_models, models.data(), sizeof(double) * models.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_ent, lv.ent.data(), sizeof(int4) * lv.ent.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_gfirst, gfirst.data(), sizeof(int) * gfirst.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (int k = 1; k < nj; k++)
-    if (memcmp(jobs[k].model, jobs[0].model, sizeof(double) * 40))
-      return rsg_fail(errbuf, "all jobs of a batch share one substitution model");
 
   StatArgs A0;
   memset(&A0, 0, sizeof(A0));
@@ -384,7 +401,8 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   }
 
   auto launch_all = [&]() -> int {
-    if (rsg_sim_launch(c, d_sim, nj, nodes, b_sim, d_left, d_right, d_ld, d_rd, d_model, d_cdf, d_seqs, d_leaves) != RSG_OK) return RSG_FAIL;
+    if (rsg_sim_launch(c, d_sim, nj, nodes, b_sim, lv, d_ent, d_left, d_right, d_ld, d_rd, d_models, d_cdf, d_seqs,
+                       d_leaves) != RSG_OK) return RSG_FAIL;
     hipLaunchKernelGGL(k_pack_b, dim3((unsigned)b_pack), dim3(256), 0, c->stream, d_desc, nj, d_leaves, d_xt);
     if (rsg_gram_batch_launch(c, d_tiles, (int)tiles.size(), d_desc, d_xt, d_cnt) != RSG_OK) return RSG_FAIL;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_flag, 16);
@@ -400,12 +418,12 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     } else {
       hipLaunchKernelGGL(k_item_minmax, dim3((unsigned)nj), dim3(256), 0, c->stream, d_desc, (int)D_BSTAT, b_stat, nj, d_part_s, d_mm);
     }
-    hipLaunchKernelGGL(k_width_b, dim3((unsigned)((nj + 63) / 64)), dim3(64), 0, c->stream, nj, d_mm, bmin, 0.05, 400, 1e-6, d_geo);
-    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)(((int64_t)nj * NBMAX + 255) / 256)), dim3(256), 0, c->stream, d_hist, (int64_t)nj * NBMAX);
+    hipLaunchKernelGGL(k_width_b, dim3((unsigned)((ng + 63) / 64)), dim3(64), 0, c->stream, ng, d_gfirst, d_desc, d_mm,
+                       bmin, 0.05, 400, 1e-6, d_geo);
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)(((int64_t)ng * NBMAX + 255) / 256)), dim3(256), 0, c->stream, d_hist, (int64_t)ng * NBMAX);
     hipLaunchKernelGGL(k_hist_b, dim3((unsigned)b_el), dim3(256), 0, c->stream, d_desc, nj, d_cov, d_geo, bmin, d_hist);
-    hipLaunchKernelGGL(k_fit_b, dim3((unsigned)nj), dim3(1024), 0, c->stream, d_desc, d_geo, d_hist, d_S, d_NZ,
+    hipLaunchKernelGGL(k_fit_b, dim3((unsigned)ng), dim3(1024), 0, c->stream, d_geo, d_hist, d_S, d_NZ,
                        bmin, pmass, fracfit, doexpfit, d_fits);
-    hipLaunchKernelGGL(k_copy_mm, dim3((unsigned)((nj + 63) / 64)), dim3(64), 0, c->stream, nj, d_mm, d_fits);
     return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
   };
 
@@ -436,20 +454,19 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   HIPCHK(hipMemcpy(flags, c->d_flag, sizeof(flags), hipMemcpyDeviceToHost));
   if (flags[0]) return rsg_fail(errbuf, "pm validation failed in a batch item");
   if (flags[1]) return rsg_fail(errbuf, "bad covariation (NaN after correction) in a batch item");
-  HIPCHK(hipMemcpy(fits, d_fits, sizeof(double) * 8 * nj, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(fits, d_fits, sizeof(double) * 8 * ng, hipMemcpyDeviceToHost));
   if (hists) {
-    std::vector<HGeo> geo((size_t)nj);
-    HIPCHK(hipMemcpy(geo.data(), d_geo, sizeof(HGeo) * nj, hipMemcpyDeviceToHost));
+    std::vector<HGeo> geo((size_t)ng);
+    HIPCHK(hipMemcpy(geo.data(), d_geo, sizeof(HGeo) * ng, hipMemcpyDeviceToHost));
     std::vector<unsigned long long> h((size_t)NBMAX);
-    for (int k = 0; k < nj; k++) {
+    for (int k = 0; k < ng; k++) {
       hists[k] = nullptr;
       const HGeo &g = geo[(size_t)k];
       if (!g.ok) continue;
       rsg_hist *H = rsg_hist_create(bmin, g.bmax, g.w);
       HIPCHK(hipMemcpy(h.data(), d_hist + (size_t)k * NBMAX, sizeof(unsigned long long) * g.nb, hipMemcpyDeviceToHost));
       for (int b = 0; b < g.nb; b++) H->obs[b] = h[(size_t)b];
-      const int64_t L = desc[(size_t)k * RSG_BDESC + D_L];
-      H->n = H->Nc = H->No = (uint64_t)(L * (L - 1) / 2);
+      H->n = H->Nc = H->No = (uint64_t)g.npairs;
       H->imin = H->cmin = g.imin; H->imax = g.imax; H->xmin = g.xmin; H->xmax = g.xmax;
       hists[k] = H;
     }

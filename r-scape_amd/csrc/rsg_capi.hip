@@ -42,6 +42,32 @@ extern "C" void rsg_ctx_destroy(rsg_ctx *c)
   delete c;
 }
 
+__global__ void __launch_bounds__(256) k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n)
+{
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) dst[k] = src[k];
+}
+
+extern "C" int rsg_ctx_copy_probe(rsg_ctx *c, size_t bytes, char *errbuf)
+{
+  if (!c || bytes == 0 || bytes % 16) return rsg_fail(errbuf, "rsg_ctx_copy_probe: bytes must be a positive multiple of 16");
+  HIPCHK(hipSetDevice(c->device));
+  uint4 *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) { hipFree(a); return rsg_fail(errbuf, "rsg_ctx_copy_probe: out of device memory"); }
+  int st = RSG_OK;
+  if (hipMemsetAsync(a, 1, bytes, c->stream) != hipSuccess) st = rsg_fail(errbuf, "rsg_ctx_copy_probe: memset failed");
+  const int64_t n = (int64_t)(bytes / 16);
+  for (int r = 0; r < 3 && st == RSG_OK; r++) {
+    int t; rsg_tstart(c, "copy_probe", &t);
+    hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(256), 0, c->stream, a, b, n);
+    rsg_tstop(c, t);
+    if (hipGetLastError() != hipSuccess) st = rsg_fail(errbuf, "rsg_ctx_copy_probe: launch failed");
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess && st == RSG_OK) st = rsg_fail(errbuf, "rsg_ctx_copy_probe: failed");
+  hipFree(a); hipFree(b);
+  return st;
+}
+
 extern "C" int rsg_ctx_set_timing(rsg_ctx *c, int on) { c->timing = on; c->ntimers = 0; return RSG_OK; }
 
 extern "C" int rsg_ctx_timings(rsg_ctx *c, const char **names, double *ms, int *n, int cap)

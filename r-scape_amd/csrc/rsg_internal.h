@@ -129,22 +129,32 @@ int64_t rsg_npad(int64_t N);
 struct SimItem {
   int64_t N, L;           // leaves, columns
   int64_t tree_off;       // first internal node of this item in the concatenated tree arrays
-  int64_t seq_off;        // offset of its (2N-1) x L symbol rows
+  int64_t seq_off;        // offset of its (N-1) x L internal-node symbol rows
   int64_t leaf_off;       // offset of its N x L leaf output
   uint64_t seed; uint32_t stream; uint32_t gapthr;   // a leaf symbol is gapped if its tag-1 draw < gapthr
   int64_t col_blk0;       // first 64-column block of this item in the launch
+  int64_t model_off;      // offset (doubles) of its 40-double GTR model
 };
+// Tree levels of a set of items: entries {item, internal node, first block within the level, 0}
+// grouped by depth.
+struct SimLevels {
+  std::vector<int4> ent;
+  std::vector<int64_t> lev_ent0;     // entry offset of each level (size nlev + 1)
+  std::vector<int64_t> lev_blocks;   // 64-column blocks of each level
+};
+int rsg_sim_levels(const SimItem *items, int nitems, const int *left, const int *right, SimLevels &lv, char *errbuf);
 int rsg_sim_launch(rsg_ctx *c, const SimItem *d_items, int nitems, int64_t nodes, int64_t blocks,
-                   const int *d_left, const int *d_right, const double *d_ld, const double *d_rd,
-                   const double *d_model, uint32_t *d_cdf, uint8_t *d_seqs, uint8_t *d_leaves);
+                   const SimLevels &lv, const int4 *d_ent, const int *d_left, const int *d_right,
+                   const double *d_ld, const double *d_rd, const double *d_models, uint32_t *d_cdf,
+                   uint8_t *d_seqs, uint8_t *d_leaves);
 int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t *L,
                         const int *left, const int *right, const double *ld, const double *rd,
-                        const double *model, const uint64_t *seed, const uint32_t *stream, const double *gapfrac,
+                        const double *models, const uint64_t *seed, const uint32_t *stream, const double *gapfrac,
                         uint8_t *d_leaves, const int64_t *leaf_off, char *errbuf);
 
 // Batched pipeline: per-item descriptor of RSG_BDESC int64 fields
 // {N, L, Npad, Lpad, off_xt, off_cnt, off_cov, off_pm, off_rows, off_hist, off_leaf, blk0_pack,
-//  blk0_col, blk0_stat, blk0_el, nb32}
-#define RSG_BDESC 16
+//  blk0_col, blk0_stat, blk0_el, nb32, group, spare}
+#define RSG_BDESC 18
 int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_tiles, int ntiles, const int64_t *d_desc, const uint8_t *d_xt,
                           uint32_t *d_cnt);

@@ -10,8 +10,10 @@
 // before children; children > 0 internal, <= 0 leaf -child).  Sequence rows: internal
 // node v -> row v, leaf l -> row N-1+l.  Every symbol draw is Philox(seed, stream,
 // row, column, tag) compared against uint32 CDF thresholds of the branch's P(t) row, so a
-// column's history is independent of every other column: one thread owns one column and
-// walks the nodes in index order (parents first), no synchronisation needed.
+// draw depends only on the parent's symbol in the same column: the evolution runs one
+// launch per tree depth (Yule trees are O(log N) deep), each covering every (node at that
+// depth, column) of every alignment in the batch.  Internal rows live in HBM between levels;
+// leaves go straight to the output.
 // P(t) = Pi^-1/2 V diag(exp(lambda t)) V^T Pi^1/2 from the eigendecomposition of the
 // symmetrised GTR generator (host, FP64 Jacobi); the per-branch tables are built on the
 // device in FP64.
@@ -19,6 +21,8 @@
 #include <cmath>
 #include <cstring>
 #include <algorithm>
+#include <climits>
+#include <vector>
 
 // Transition tables are rounded as the host computes them (no FMA contraction).
 #pragma clang fp contract(off)
@@ -175,90 +179,162 @@ __device__ __forceinline__ int icdf3(uint32_t u, const uint32_t *c)
   return (u >= c[0]) + (u >= c[1]) + (u >= c[2]);
 }
 
-// grid: column blocks of 64 over all items (col_blk0 prefix); one thread per column.
-__global__ void __launch_bounds__(64)
-k_evolve(const SimItem *__restrict__ items, int nitems, const int *__restrict__ left, const int *__restrict__ right,
-         const uint32_t *__restrict__ cdf, const double *__restrict__ model, uint8_t *__restrict__ seqs,
-         uint8_t *__restrict__ leaves)
+__device__ __forceinline__ int find_sim_item(const SimItem *__restrict__ items, int nitems, int64_t blk)
 {
-  // locate the item of this block (binary search on col_blk0)
   int lo = 0, hi = nitems - 1;
-  const int64_t blk = blockIdx.x;
   while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (items[mid].col_blk0 <= blk) lo = mid; else hi = mid - 1; }
-  const SimItem it = items[lo];
-  const int64_t col = (blk - it.col_blk0) * 64 + threadIdx.x;
+  return lo;
+}
+
+// Root row of every item from the stationary distribution; one thread per column.
+__global__ void __launch_bounds__(64)
+k_root(const SimItem *__restrict__ items, int nitems, const double *__restrict__ models, uint8_t *__restrict__ seqs)
+{
+  const SimItem it = items[find_sim_item(items, nitems, blockIdx.x)];
+  const int64_t col = (blockIdx.x - it.col_blk0) * 64 + threadIdx.x;
   if (col >= it.L) return;
-  const int64_t N = it.N, L = it.L;
-  const int *lf = left + it.tree_off, *rt = right + it.tree_off;
-  const uint32_t *cd = cdf + it.tree_off * 24;               // per internal node: left 12, right 12 (see launcher)
-  uint8_t *S = seqs + it.seq_off;
-  // root from the stationary distribution
-  {
-    uint32_t u = rsg_philox(it.seed, it.stream, (uint32_t)(2 * N - 1), (uint32_t)col, 0u);
-    double cum = 0.; int a = 3;
-    for (int b = 0; b < 3; b++) { cum += model[36 + b]; double v = cum * 4294967296.0; uint32_t th = (v >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)v; if (u < th) { a = b; break; } }
-    S[col] = (uint8_t)a;
+  const double *pi = models + it.model_off + 36;
+  const uint32_t u = rsg_philox(it.seed, it.stream, (uint32_t)(2 * it.N - 1), (uint32_t)col, 0u);
+  double cum = 0.; int a = 3;
+  for (int b = 0; b < 3; b++) {
+    cum += pi[b];
+    const double v = cum * 4294967296.0;
+    const uint32_t th = (v >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)v;
+    if (u < th) { a = b; break; }
   }
-  for (int64_t v = 0; v < N - 1; v++) {
-    const int par = S[v * L + col];
+  seqs[it.seq_off + col] = (uint8_t)a;
+}
+
+// One tree level: every internal node v at this depth draws its two children for 64
+// columns per block.  ent[k] = {item, v, first block of the entry within the level}.
+__global__ void __launch_bounds__(64)
+k_evolve_level(const SimItem *__restrict__ items, const int4 *__restrict__ ent, int nent,
+               const int *__restrict__ left, const int *__restrict__ right, const uint32_t *__restrict__ cdf,
+               uint8_t *__restrict__ seqs, uint8_t *__restrict__ leaves)
+{
+  int lo = 0, hi = nent - 1;
+  const int blk = (int)blockIdx.x;
+  while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (ent[mid].z <= blk) lo = mid; else hi = mid - 1; }
+  const int4 e = ent[lo];
+  const SimItem it = items[e.x];
+  const int64_t col = (int64_t)(blk - e.z) * 64 + threadIdx.x;
+  if (col >= it.L) return;
+  const int64_t N = it.N, L = it.L, v = e.y;
+  uint8_t *S = seqs + it.seq_off;
+  const uint32_t *cd = cdf + (it.tree_off + v) * 24;
+  const int par = S[v * L + col];
 #pragma unroll
-    for (int side = 0; side < 2; side++) {
-      const int d = side ? rt[v] : lf[v];
-      const int64_t row = (d > 0) ? d : (N - 1 - d);
-      const uint32_t u = rsg_philox(it.seed, it.stream, (uint32_t)row, (uint32_t)col, 0u);
-      int sym = icdf3(u, cd + v * 24 + side * 12 + par * 3);
-      S[row * L + col] = (uint8_t)sym;
-      if (d <= 0) {
-        if (it.gapthr && rsg_philox(it.seed, it.stream, (uint32_t)row, (uint32_t)col, 1u) < it.gapthr) sym = 4;
-        leaves[it.leaf_off + (-d) * L + col] = (uint8_t)sym;
-      }
+  for (int side = 0; side < 2; side++) {
+    const int d = side ? right[it.tree_off + v] : left[it.tree_off + v];
+    const int64_t row = (d > 0) ? d : (N - 1 - d);
+    const uint32_t u = rsg_philox(it.seed, it.stream, (uint32_t)row, (uint32_t)col, 0u);
+    int sym = icdf3(u, cd + side * 12 + par * 3);
+    if (d > 0) S[(int64_t)d * L + col] = (uint8_t)sym;
+    else {
+      if (it.gapthr && rsg_philox(it.seed, it.stream, (uint32_t)row, (uint32_t)col, 1u) < it.gapthr) sym = 4;
+      leaves[it.leaf_off + (int64_t)(-d) * L + col] = (uint8_t)sym;
     }
   }
 }
 
-// Per-item branch tables into the [node][side][4][3] layout used by k_evolve.
+// Per-item branch tables into the [node][side][4][3] layout used by k_evolve_level.
 __global__ void k_branch_tables(const SimItem *__restrict__ items, int nitems, const double *__restrict__ ld,
-                                const double *__restrict__ rd, const double *__restrict__ model,
+                                const double *__restrict__ rd, const double *__restrict__ models,
                                 int64_t total_nodes, uint32_t *__restrict__ cdf)
 {
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;     // (node, side, row)
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;     // (node, side, row)
   if (idx >= total_nodes * 8) return;
-  int64_t node = idx / 8; int side = (int)((idx / 4) & 1); int a = (int)(idx & 3);
-  double t = side ? rd[node] : ld[node];
-  row_cdf(model, t, a, cdf + node * 24 + side * 12 + a * 3);
+  const int64_t node = idx / 8; const int side = (int)((idx / 4) & 1); const int a = (int)(idx & 3);
+  int lo = 0, hi = nitems - 1;
+  while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (items[mid].tree_off <= node) lo = mid; else hi = mid - 1; }
+  const double t = side ? rd[node] : ld[node];
+  row_cdf(models + items[lo].model_off, t, a, cdf + node * 24 + side * 12 + a * 3);
+}
+
+// Tree levels of all items (host).  Depths come from a walk down from each root, so any
+// node order is accepted; a node reached twice or never is an error.
+int rsg_sim_levels(const SimItem *items, int nitems, const int *left, const int *right, SimLevels &lv, char *errbuf)
+{
+  std::vector<std::vector<int4>> per;                  // per level: entries without block offsets
+  std::vector<int> depth, stack;
+  for (int k = 0; k < nitems; k++) {
+    const SimItem &it = items[k];
+    const int64_t m = it.N - 1;
+    depth.assign((size_t)m, -1);
+    stack.assign(1, 0);
+    depth[0] = 0;
+    int64_t seen = 1;
+    while (!stack.empty()) {
+      const int v = stack.back(); stack.pop_back();
+      if ((int)per.size() <= depth[(size_t)v]) per.resize((size_t)depth[(size_t)v] + 1);
+      per[(size_t)depth[(size_t)v]].push_back(make_int4(k, v, 0, 0));
+      for (int side = 0; side < 2; side++) {
+        const int d = side ? right[it.tree_off + v] : left[it.tree_off + v];
+        if (d > 0) {
+          if (d >= m || depth[(size_t)d] >= 0) return rsg_fail(errbuf, "item %d: malformed tree at node %d", k, v);
+          depth[(size_t)d] = depth[(size_t)v] + 1; stack.push_back(d); seen++;
+        } else if (-d >= it.N) return rsg_fail(errbuf, "item %d: leaf index %d out of range", k, -d);
+      }
+    }
+    if (seen != m) return rsg_fail(errbuf, "item %d: %lld internal nodes unreachable from the root", k, (long long)(m - seen));
+  }
+  lv.ent.clear(); lv.lev_ent0.assign(1, 0); lv.lev_blocks.clear();
+  for (auto &lev : per) {
+    std::sort(lev.begin(), lev.end(), [](const int4 &a, const int4 &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+    int64_t blk = 0;
+    for (int4 &e : lev) {
+      if (blk > INT32_MAX) return rsg_fail(errbuf, "too many column blocks in one tree level");
+      e.z = (int)blk;
+      blk += (items[e.x].L + 63) / 64;
+      lv.ent.push_back(e);
+    }
+    lv.lev_ent0.push_back((int64_t)lv.ent.size());
+    lv.lev_blocks.push_back(blk);
+  }
+  return RSG_OK;
 }
 
 // Launch-only entry (no allocation, no synchronisation; usable under stream capture).
 int rsg_sim_launch(rsg_ctx *c, const SimItem *d_items, int nitems, int64_t nodes, int64_t blocks,
-                   const int *d_left, const int *d_right, const double *d_ld, const double *d_rd,
-                   const double *d_model, uint32_t *d_cdf, uint8_t *d_seqs, uint8_t *d_leaves)
+                   const SimLevels &lv, const int4 *d_ent, const int *d_left, const int *d_right,
+                   const double *d_ld, const double *d_rd, const double *d_models, uint32_t *d_cdf,
+                   uint8_t *d_seqs, uint8_t *d_leaves)
 {
   if (nodes > 0)
     hipLaunchKernelGGL(k_branch_tables, dim3((unsigned)((nodes * 8 + 255) / 256)), dim3(256), 0, c->stream,
-                       d_items, nitems, d_ld, d_rd, d_model, nodes, d_cdf);
+                       d_items, nitems, d_ld, d_rd, d_models, nodes, d_cdf);
   if (blocks > 0)
-    hipLaunchKernelGGL(k_evolve, dim3((unsigned)blocks), dim3(64), 0, c->stream, d_items, nitems, d_left, d_right,
-                       d_cdf, d_model, d_seqs, d_leaves);
+    hipLaunchKernelGGL(k_root, dim3((unsigned)blocks), dim3(64), 0, c->stream, d_items, nitems, d_models, d_seqs);
+  for (size_t l = 0; l < lv.lev_blocks.size(); l++) {
+    const int64_t e0 = lv.lev_ent0[l], ne = lv.lev_ent0[l + 1] - e0;
+    if (lv.lev_blocks[l] > 0)
+      hipLaunchKernelGGL(k_evolve_level, dim3((unsigned)lv.lev_blocks[l]), dim3(64), 0, c->stream, d_items,
+                         d_ent + e0, (int)ne, d_left, d_right, d_cdf, d_seqs, d_leaves);
+  }
   return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
 }
 
 // ---------------------------------------------------------------- host API
+namespace {
 struct SimBuffers {
-  int *d_left = nullptr, *d_right = nullptr; double *d_ld = nullptr, *d_rd = nullptr;
-  uint32_t *d_cdf = nullptr; double *d_model = nullptr; uint8_t *d_seqs = nullptr; SimItem *d_items = nullptr;
+  std::vector<void *> p;
+  template <typename T> bool get(T **out, size_t count)
+  {
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return false;
+    p.push_back(q); *out = (T *)q;
+    return true;
+  }
+  ~SimBuffers() { for (void *q : p) hipFree(q); }
 };
-
-static void sim_free(SimBuffers &b)
-{
-  void *p[] = {b.d_left, b.d_right, b.d_ld, b.d_rd, b.d_cdf, b.d_model, b.d_seqs, b.d_items};
-  for (void *q : p) if (q) hipFree(q);
 }
 
 // Simulate a set of alignments into device leaves (row-major N x L per item at leaf offsets).
-// Trees come from the caller (host arrays, concatenated per item); model = rsg_gtr_model output.
+// Trees come from the caller (host arrays, concatenated per item); models = nitems x 40
+// doubles of rsg_gtr_model output.
 int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t *L,
                         const int *left, const int *right, const double *ld, const double *rd,
-                        const double *model, const uint64_t *seed, const uint32_t *stream, const double *gapfrac,
+                        const double *models, const uint64_t *seed, const uint32_t *stream, const double *gapfrac,
                         uint8_t *d_leaves, const int64_t *leaf_off, char *errbuf)
 {
   std::vector<SimItem> items((size_t)nitems);
@@ -266,39 +342,34 @@ int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t 
   for (int k = 0; k < nitems; k++) {
     SimItem &it = items[(size_t)k];
     it.N = N[k]; it.L = L[k]; it.tree_off = nodes; it.seq_off = seq; it.leaf_off = leaf_off[k];
-    it.seed = seed[k]; it.stream = stream[k];
-    double g = gapfrac ? gapfrac[k] : 0.;
+    it.seed = seed[k]; it.stream = stream[k]; it.model_off = 40 * (int64_t)k;
+    const double g = gapfrac ? gapfrac[k] : 0.;
     it.gapthr = (g <= 0.) ? 0u : (g >= 1. ? 0xFFFFFFFFu : (uint32_t)(g * 4294967296.0));
     it.col_blk0 = blocks;
-    nodes += N[k] - 1; seq += (2 * N[k] - 1) * L[k]; blocks += (L[k] + 63) / 64;
+    nodes += N[k] - 1; seq += (N[k] - 1) * L[k]; blocks += (L[k] + 63) / 64;
   }
+  SimLevels lv;
+  if (rsg_sim_levels(items.data(), nitems, left, right, lv, errbuf) != RSG_OK) return RSG_FAIL;
   SimBuffers b;
-  int rc = RSG_FAIL;
-  do {
-    if (hipMalloc(&b.d_left, sizeof(int) * nodes) || hipMalloc(&b.d_right, sizeof(int) * nodes) ||
-        hipMalloc(&b.d_ld, sizeof(double) * nodes) || hipMalloc(&b.d_rd, sizeof(double) * nodes) ||
-        hipMalloc(&b.d_cdf, sizeof(uint32_t) * 24 * nodes) || hipMalloc(&b.d_model, sizeof(double) * 40) ||
-        hipMalloc(&b.d_seqs, (size_t)seq) || hipMalloc(&b.d_items, sizeof(SimItem) * nitems)) {
-      rsg_fail(errbuf, "simulation buffers: out of device memory"); break;
-    }
-    hipMemcpyAsync(b.d_left, left, sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(b.d_right, right, sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(b.d_ld, ld, sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(b.d_rd, rd, sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(b.d_model, model, sizeof(double) * 40, hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(b.d_items, items.data(), sizeof(SimItem) * nitems, hipMemcpyHostToDevice, c->stream);
-    int t; rsg_tstart(c, "simulate", &t);
-    hipLaunchKernelGGL(k_branch_tables, dim3((unsigned)((nodes * 8 + 255) / 256)), dim3(256), 0, c->stream,
-                       b.d_items, nitems, b.d_ld, b.d_rd, b.d_model, nodes, b.d_cdf);
-    hipLaunchKernelGGL(k_evolve, dim3((unsigned)blocks), dim3(64), 0, c->stream, b.d_items, nitems, b.d_left,
-                       b.d_right, b.d_cdf, b.d_model, b.d_seqs, d_leaves);
-    rsg_tstop(c, t);
-    if (hipGetLastError() != hipSuccess) { rsg_fail(errbuf, "simulation launch failed"); break; }
-    if (hipStreamSynchronize(c->stream) != hipSuccess) { rsg_fail(errbuf, "simulation failed"); break; }
-    rc = RSG_OK;
-  } while (0);
-  sim_free(b);
-  return rc;
+  int *d_left, *d_right; double *d_ld, *d_rd, *d_models; uint32_t *d_cdf; uint8_t *d_seqs; SimItem *d_items; int4 *d_ent;
+  if (!b.get(&d_left, (size_t)nodes) || !b.get(&d_right, (size_t)nodes) || !b.get(&d_ld, (size_t)nodes) ||
+      !b.get(&d_rd, (size_t)nodes) || !b.get(&d_cdf, 24 * (size_t)nodes) || !b.get(&d_models, 40 * (size_t)nitems) ||
+      !b.get(&d_seqs, (size_t)seq) || !b.get(&d_items, (size_t)nitems) || !b.get(&d_ent, lv.ent.size()))
+    return rsg_fail(errbuf, "simulation buffers: out of device memory");
+  HIPCHK(hipMemcpyAsync(d_left, left, sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_right, right, sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_ld, ld, sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_rd, rd, sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_models, models, sizeof(double) * 40 * nitems, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_items, items.data(), sizeof(SimItem) * nitems, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_ent, lv.ent.data(), sizeof(int4) * lv.ent.size(), hipMemcpyHostToDevice, c->stream));
+  int t; rsg_tstart(c, "simulate", &t);
+  const int lr = rsg_sim_launch(c, d_items, nitems, nodes, blocks, lv, d_ent, d_left, d_right, d_ld, d_rd, d_models,
+                                d_cdf, d_seqs, d_leaves);
+  rsg_tstop(c, t);
+  if (lr != RSG_OK) return rsg_fail(errbuf, "simulation launch failed");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return RSG_OK;
 }
 
 // Simulate one alignment and return its leaves (N x L) to the host.

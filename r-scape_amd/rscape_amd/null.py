@@ -20,7 +20,7 @@ class NullJob(C.Structure):
                 ("left", C.POINTER(C.c_int)), ("right", C.POINTER(C.c_int)),
                 ("ld", C.POINTER(C.c_double)), ("rd", C.POINTER(C.c_double)),
                 ("model", C.POINTER(C.c_double)), ("seed", C.c_uint64), ("stream", C.c_uint32),
-                ("gapfrac", C.c_double)]
+                ("gapfrac", C.c_double), ("group", C.c_int32)]
 
 
 def _declare():
@@ -73,19 +73,19 @@ def yule_tree(n, mean_bl, seed=42, stream=0):
 class Job:
     """One alignment to simulate: tree, length, model and RNG key."""
 
-    def __init__(self, tree, L, model, seed=42, stream=0, gapfrac=0.0):
+    def __init__(self, tree, L, model, seed=42, stream=0, gapfrac=0.0, group=None):
         self.tree = {k: (np.ascontiguousarray(v, dtype=np.int32) if k in ("left", "right") else
                          np.ascontiguousarray(v, dtype=np.float64) if k in ("ld", "rd") else v)
                      for k, v in tree.items()}
         self.L, self.model = int(L), np.ascontiguousarray(model, dtype=np.float64)
-        self.seed, self.stream, self.gapfrac = seed, stream, gapfrac
+        self.seed, self.stream, self.gapfrac, self.group = seed, stream, gapfrac, group
 
-    def struct(self):
+    def struct(self, group=0):
         t = self.tree
         ip = C.POINTER(C.c_int); dp = C.POINTER(C.c_double)
         return NullJob(t["n"], self.L, t["left"].ctypes.data_as(ip), t["right"].ctypes.data_as(ip),
                        t["ld"].ctypes.data_as(dp), t["rd"].ctypes.data_as(dp), self.model.ctypes.data_as(dp),
-                       self.seed, self.stream, self.gapfrac)
+                       self.seed, self.stream, self.gapfrac, group if self.group is None else self.group)
 
 
 def simulate(ctx, job):
@@ -99,18 +99,22 @@ def simulate(ctx, job):
 
 
 def batch(ctx, jobs, covtype="GTp", bmin=-10.0, pmass=5e-4, fracfit=1.0, expo=True, graph=0, hists=False):
-    """Simulate + score + fit a batch on the device.  Returns (fits array (n, 8), histograms).
-    fits columns: mu, lambda, tau, newmass, pmass, w, minCOV, maxCOV."""
+    """Simulate + score + fit a batch on the device.  Jobs with group=None are nulls of one
+    replicate each; jobs sharing a group id are the replicates of one null.  Returns (fits
+    array (groups, 8), histograms); fits columns: mu, lambda, tau, newmass, pmass, w,
+    minCOV, maxCOV."""
     from . import Hist, Histogram
     L = _declare()
+    structs = [j.struct(k) for k, j in enumerate(jobs)]
     n = len(jobs)
-    arr = (NullJob * n)(*[j.struct() for j in jobs])
-    fits = np.zeros(n * 8)
-    hp = (C.POINTER(Hist) * n)() if hists else None
+    arr = (NullJob * n)(*structs)
+    ng = max(s.group for s in structs) + 1
+    fits = np.zeros(ng * 8)
+    hp = (C.POINTER(Hist) * ng)() if hists else None
     err = _err()
     t = COV[covtype] if isinstance(covtype, str) else covtype
     _check(L.rsg_null_batch(ctx.h, arr, n, t, bmin, pmass, fracfit, 1 if expo else 0,
                             C.cast(hp, C.c_void_p) if hists else None, _p(fits), int(graph), err),
            err, "rsg_null_batch")
-    hs = [Histogram(hp[k]) if (hists and hp[k]) else None for k in range(n)] if hists else None
-    return fits.reshape(n, 8), hs
+    hs = [Histogram(hp[k]) if hp[k] else None for k in range(ng)] if hists else None
+    return fits.reshape(ng, 8), hs

@@ -102,3 +102,47 @@ def test_batch_graph_replays_equal_eager(ctx):
         assert np.array_equal(np.nan_to_num(g), np.nan_to_num(eager))
         for a, b in zip(he, hg):
             assert np.array_equal(a.h.counts(), b.h.counts())
+
+
+def test_batch_groups_match_cumulative_null(ctx):
+    """A group of replicates = R-scape's null loop: width from the first replicate, every
+    replicate's histogram merged (null_add2cumranklist), one fit on the merged histogram."""
+    m1, m2 = NL.gtr_model(PI), NL.gtr_model((0.4, 0.1, 0.1, 0.4))
+    tree_a, tree_b = NL.yule_tree(120, 0.1, seed=1), NL.yule_tree(40, 0.25, seed=2)
+    jobs = [NL.Job(tree_a, 90, m1, seed=77, stream=r, gapfrac=0.05, group=0) for r in range(5)]
+    jobs += [NL.Job(tree_b, 150, m2, seed=78, stream=r, group=1) for r in range(3)]
+    jobs += [NL.Job(tree_a, 61, m2, seed=79, stream=0, group=2)]
+    fits, hs = NL.batch(ctx, jobs, "GTp", pmass=0.01, hists=True)
+    assert fits.shape == (3, 8)
+    for g, members in enumerate([jobs[:5], jobs[5:8], jobs[8:]]):
+        cum, w = None, None
+        mn, mx = np.inf, -np.inf
+        for job in members:
+            msa = ctx.msa(NL.simulate(ctx, job))
+            a, b = msa.calculate("GTp")
+            mn, mx = min(mn, a), max(mx, b)
+            if w is None:
+                w = R.width(b, a)
+            h = msa.rank_hist(-10.0, w)
+            cum = R.Histogram.null_add(cum, h)
+        fit, _ = cum.fit(ctx, 0.01, 1.0, True)
+        assert fits[g][5] == w and fits[g][6] == mn and fits[g][7] == mx
+        got = hs[g].h
+        assert (got.n, got.Nc, got.imin, got.imax) == (cum.h.n, cum.h.Nc, cum.h.imin, cum.h.imax)
+        assert np.array_equal(got.counts()[got.imin:got.imax + 1], cum.h.counts()[cum.h.imin:cum.h.imax + 1])
+        for i, key in enumerate(["mu", "lambda", "tau", "newmass", "pmass"]):
+            assert np.isclose(fits[g][i], fit[key], rtol=1e-9, atol=1e-12), (g, key)
+
+
+def test_batch_rejects_bad_groups_and_trees(ctx):
+    m = NL.gtr_model(PI)
+    t = NL.yule_tree(10, 0.1)
+    with pytest.raises(R.RscapeError):
+        NL.batch(ctx, [NL.Job(t, 20, m, group=1)])
+    with pytest.raises(R.RscapeError):
+        NL.batch(ctx, [NL.Job(t, 20, m, group=0), NL.Job(t, 20, m, group=2)])
+    c = t["left"][0] if t["left"][0] > 0 else t["right"][0]
+    bad = dict(t); bad["left"] = t["left"].copy(); bad["right"] = t["right"].copy()
+    bad["left"][0] = bad["right"][0] = c                                   # one internal node reached twice
+    with pytest.raises(R.RscapeError):
+        NL.simulate(ctx, NL.Job(bad, 20, m))
