@@ -187,19 +187,38 @@ int rsg_null_simulate(rsg_ctx *ctx, const rsg_null_job *job, uint8_t *leaves, ch
  * (calculate_width_histo), every replicate's pairs accumulate into one histogram
  * (null_add2cumranklist) and one FP64 tail fit is made per group.  Outputs per group g:
  * fits[g*8 + {0..7}] = {mu, lambda, tau, newmass, pmass, w, minCOV, maxCOV}; hists (may be
- * NULL) receives each group's histogram.  use_graph = k >= 1 records the whole batch in one
+ * NULL) receives each group's histogram; feats (may be NULL) receives 48 surrogate features
+ * per group (rsg_null_features of its first replicate).  use_graph = k >= 1 records the whole batch in one
  * hipGraph and launches it k times (each launch recomputes everything); 0 runs eagerly. */
 int rsg_null_batch(rsg_ctx *ctx, const rsg_null_job *jobs, int njobs, int covtype,
                    double bmin, double pmass, double fracfit, int doexpfit,
-                   rsg_hist **hists, double *fits, int use_graph, char *errbuf);
+                   rsg_hist **hists, double *fits, float *feats, int use_graph, char *errbuf);
 
 /* ---- learned null-tail surrogate (no reference counterpart) ---- */
+/* The surrogate's 48 inputs of an alignment and its tree (column statistics on the device,
+ * tree moments on the host): ln N, ln L, ln total branch length, 16-bin column-entropy and
+ * 16-bin gap histograms, A/C/G composition, 10 tree-shape moments. */
+int  rsg_null_features(rsg_ctx *ctx, rsg_msa *msa, const int *left, const int *right,
+                       const double *ld, const double *rd, float *feat, char *errbuf);
+/* E-values from a parametric tail P(score > x) = p0 exp(-lambda (x - m)) (the surrogate's
+ * output): Eval[i][j] = Nc * min(1, p0 exp(-lambda (COV[i][j] - m))) for i != j, +inf on
+ * the diagonal (cov2evalue's role, src/covariation.c:2369-2436, for a fitted tail). */
+int  rsg_cov_EvaluesTail(rsg_ctx *ctx, rsg_msa *msa, double m, double lambda, double p0, int64_t Nc,
+                         double *Eval, char *errbuf);
 typedef struct rsg_mlp rsg_mlp;
 int  rsg_mlp_create(rsg_ctx *ctx, int nlayers, const int *dims, uint64_t seed, rsg_mlp **ret, char *errbuf);
 void rsg_mlp_destroy(rsg_ctx *ctx, rsg_mlp *m);
 /* bf16 = 1 (default): BF16 weights and activations into v_mfma_f32_32x32x16_bf16, FP32
  * accumulation; bf16 = 0: FP32 operands (v_mfma_f32_32x32x2f32), for gradient checks. */
 int  rsg_mlp_set_precision(rsg_mlp *m, int bf16);
+/* Input / output scalers kept with the network: the kernels use x' = (x - xmean) / xstd and
+ * y = y' * ystd + ymean (targets are scaled the same way for training).  NULL = identity. */
+int  rsg_mlp_set_scaling(rsg_ctx *ctx, rsg_mlp *m, const float *xmean, const float *xstd,
+                         const float *ymean, const float *ystd, char *errbuf);
+/* Training range of the inputs; rsg_mlp_in_range is 1 when every input lies inside it (the
+ * caller falls back to the full simulated null otherwise).  NULL clears the range. */
+int  rsg_mlp_set_range(rsg_mlp *m, const float *xmin, const float *xmax);
+int  rsg_mlp_in_range(const rsg_mlp *m, const float *x);
 int  rsg_mlp_set_params(rsg_ctx *ctx, rsg_mlp *m, const float *const *W, const float *const *b, char *errbuf);
 int  rsg_mlp_get_params(rsg_ctx *ctx, rsg_mlp *m, float *const *W, float *const *b, char *errbuf);
 int  rsg_mlp_forward(rsg_ctx *ctx, rsg_mlp *m, const float *x, int64_t B, float *y, char *errbuf);

@@ -284,7 +284,8 @@ struct Dev {
 }
 
 extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int covtype, double bmin, double pmass,
-                              double fracfit, int doexpfit, rsg_hist **hists, double *fits, int use_graph, char *errbuf)
+                              double fracfit, int doexpfit, rsg_hist **hists, double *fits, float *feats,
+                              int use_graph, char *errbuf)
 {
   HIPCHK(hipSetDevice(c->device));
   if (nj < 1) return rsg_fail(errbuf, "empty batch");
@@ -348,6 +349,15 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   }
   SimLevels lv;
   if (rsg_sim_levels(sim.data(), nj, left.data(), right.data(), lv, errbuf) != RSG_OK) return RSG_FAIL;
+  // surrogate features of each group's first replicate
+  std::vector<int64_t> fitems((size_t)ng * 4);
+  int64_t fblocks = 0;
+  for (int g = 0; g < ng; g++) {
+    const SimItem &s0 = sim[(size_t)gfirst[(size_t)g]];
+    fitems[(size_t)g * 4 + 0] = s0.leaf_off; fitems[(size_t)g * 4 + 1] = s0.N; fitems[(size_t)g * 4 + 2] = s0.L;
+    fitems[(size_t)g * 4 + 3] = fblocks;
+    fblocks += (s0.L + 255) / 256;
+  }
   // ---- device buffers
   int rc = RSG_OK;
   Dev D;
@@ -360,6 +370,8 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   double *d_models = D.get<double>(40 * (size_t)nj, errbuf, &rc);
   int4 *d_ent = D.get<int4>(lv.ent.size(), errbuf, &rc);
   int *d_gfirst = D.get<int>((size_t)ng + 1, errbuf, &rc);
+  int64_t *d_fitems = D.get<int64_t>(fitems.size(), errbuf, &rc);
+  unsigned long long *d_fcnt = D.get<unsigned long long>((size_t)ng * 36, errbuf, &rc);
   uint32_t *d_cdf = D.get<uint32_t>((size_t)nodes * 24, errbuf, &rc);
   uint8_t *d_seqs = D.get<uint8_t>((size_t)seqs, errbuf, &rc);
   uint8_t *d_leaves = D.get<uint8_t>((size_t)o_leaf, errbuf, &rc);
@@ -389,6 +401,7 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   HIPCHK(hipMemcpyAsync(d_models, models.data(), sizeof(double) * models.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_ent, lv.ent.data(), sizeof(int4) * lv.ent.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_gfirst, gfirst.data(), sizeof(int) * gfirst.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_fitems, fitems.data(), sizeof(int64_t) * fitems.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
 
   StatArgs A0;
@@ -404,6 +417,7 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     if (rsg_sim_launch(c, d_sim, nj, nodes, b_sim, lv, d_ent, d_left, d_right, d_ld, d_rd, d_models, d_cdf, d_seqs,
                        d_leaves) != RSG_OK) return RSG_FAIL;
     hipLaunchKernelGGL(k_pack_b, dim3((unsigned)b_pack), dim3(256), 0, c->stream, d_desc, nj, d_leaves, d_xt);
+    if (feats && rsg_feat_launch(c, d_fitems, ng, fblocks, d_leaves, d_fcnt) != RSG_OK) return RSG_FAIL;
     if (rsg_gram_batch_launch(c, d_tiles, (int)tiles.size(), d_desc, d_xt, d_cnt) != RSG_OK) return RSG_FAIL;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_flag, 16);
     hipLaunchKernelGGL(k_logtab_b, dim3((unsigned)((nmax + 256) / 256)), dim3(256), 0, c->stream, nmax, d_lgc);
@@ -455,6 +469,14 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   if (flags[0]) return rsg_fail(errbuf, "pm validation failed in a batch item");
   if (flags[1]) return rsg_fail(errbuf, "bad covariation (NaN after correction) in a batch item");
   HIPCHK(hipMemcpy(fits, d_fits, sizeof(double) * 8 * ng, hipMemcpyDeviceToHost));
+  if (feats) {
+    std::vector<unsigned long long> cnt((size_t)ng * 36);
+    HIPCHK(hipMemcpy(cnt.data(), d_fcnt, sizeof(unsigned long long) * cnt.size(), hipMemcpyDeviceToHost));
+    for (int g = 0; g < ng; g++) {
+      const rsg_null_job &J = jobs[gfirst[(size_t)g]];
+      rsg_feat_finish(&cnt[(size_t)g * 36], J.nleaf, J.alen, J.left, J.right, J.ld, J.rd, feats + (size_t)g * 48);
+    }
+  }
   if (hists) {
     std::vector<HGeo> geo((size_t)ng);
     HIPCHK(hipMemcpy(geo.data(), d_geo, sizeof(HGeo) * ng, hipMemcpyDeviceToHost));

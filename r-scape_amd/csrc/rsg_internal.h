@@ -156,5 +156,11 @@ int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t 
 // {N, L, Npad, Lpad, off_xt, off_cnt, off_cov, off_pm, off_rows, off_hist, off_leaf, blk0_pack,
 //  blk0_col, blk0_stat, blk0_el, nb32, group, spare}
 #define RSG_BDESC 18
+// Surrogate features (rsg_features.hip): column statistics of items {ax offset, N, L, first
+// 256-column block} into 36 uint64 counters per item; the host turns them into 48 floats.
+int rsg_feat_launch(rsg_ctx *c, const int64_t *d_items, int nitems, int64_t nblocks, const uint8_t *d_base,
+                    unsigned long long *d_out);
+void rsg_feat_finish(const unsigned long long *cnt, int64_t N, int64_t L, const int *left, const int *right,
+                     const double *ld, const double *rd, float *f);
 int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_tiles, int ntiles, const int64_t *d_desc, const uint8_t *d_xt,
                           uint32_t *d_cnt);

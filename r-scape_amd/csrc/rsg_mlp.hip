@@ -36,6 +36,8 @@ struct rsg_mlp {
   float   *d_part = nullptr;  int64_t part_blocks = 0;
   int64_t  step = 0;                       // Adam step counter
   size_t   lds_bytes = 0;
+  float   *d_scale = nullptr;                // input mean | input 1/std | output mean | output std
+  std::vector<float> xmin, xmax;             // training range (empty: unbounded)
 };
 
 namespace {
@@ -45,6 +47,7 @@ struct MlpArgs {
   int64_t offW[MLP_MAXL], offB[MLP_MAXL], P;
   int ldsZ[MLP_MAXL], ldsX, ldsG;             // float offsets in LDS
   const float *p; const uint16_t *wb;
+  const float *xm, *xi, *ym, *ys;            // scalers: x' = (x - xm) * xi, y = y' * ys + ym
   const float *x, *t; const int *idx;
   int64_t nrows, first;                       // samples of this launch: rows first .. first+nrows-1 of idx/x
   float *y, *part;
@@ -113,7 +116,7 @@ __global__ void __launch_bounds__(256) k_mlp(MlpArgs a)
     float v = 0.f;
     if (s < nvalid) {
       const int64_t row = a.idx ? a.idx[a.first + s0 + s] : a.first + s0 + s;
-      v = a.x[row * d0 + i];
+      v = (a.x[row * d0 + i] - a.xm[i]) * a.xi[i];
     }
     X[s * ldl(d0) + i] = v;
   }
@@ -147,7 +150,7 @@ __global__ void __launch_bounds__(256) k_mlp(MlpArgs a)
   if constexpr (!BWD) {
     for (int e = tid; e < nvalid * dl; e += 256) {
       const int s = e / dl, o = e % dl;
-      a.y[(s0 + s) * dl + o] = G[s * ldl(dl) + o];
+      a.y[(s0 + s) * dl + o] = G[s * ldl(dl) + o] * a.ys[o] + a.ym[o];
     }
     return;
   } else {
@@ -160,7 +163,7 @@ __global__ void __launch_bounds__(256) k_mlp(MlpArgs a)
       float g = 0.f;
       if (s < nvalid) {
         const int64_t row = a.idx ? a.idx[a.first + s0 + s] : a.first + s0 + s;
-        const float d = G[s * ldl(dl) + o] - a.t[row * dl + o];
+        const float d = G[s * ldl(dl) + o] - (a.t[row * dl + o] - a.ym[o]) / a.ys[o];
         ls += d * d;
         g = a.scale * d;
       }
@@ -270,6 +273,8 @@ MlpArgs make_args(const rsg_mlp *m)
   for (int l = 0; l + 1 < m->nl; l++) { a.ldsZ[l] = off; off += MLP_SB * ldl(m->dims[l + 1]); }
   a.ldsG = off; off += MLP_SB * ldl(m->dims[m->nl]);
   a.p = m->d_p; a.wb = m->d_wb;
+  const int d0 = m->dims[0], dl = m->dims[m->nl];
+  a.xm = m->d_scale; a.xi = m->d_scale + d0; a.ym = m->d_scale + 2 * d0; a.ys = m->d_scale + 2 * d0 + dl;
   return a;
 }
 
@@ -343,11 +348,13 @@ extern "C" int rsg_mlp_create(rsg_ctx *c, int nl, const int *dims, uint64_t seed
   if (hipMalloc(&m->d_p, pb) != hipSuccess || hipMalloc(&m->d_m, pb) != hipSuccess ||
       hipMalloc(&m->d_v, pb) != hipSuccess || hipMalloc(&m->d_g, pb) != hipSuccess ||
       hipMalloc(&m->d_wb, sizeof(uint16_t) * (size_t)m->P) != hipSuccess ||
+      hipMalloc(&m->d_scale, sizeof(float) * 2 * (size_t)(dims[0] + dims[nl])) != hipSuccess ||
       hipMemcpy(m->d_p, h.data(), pb, hipMemcpyHostToDevice) != hipSuccess) {
     rsg_mlp_destroy(c, m);
     return rsg_fail(errbuf, "rsg_mlp_create: device allocation failed");
   }
-  if (rsg_mlp_set_params(c, m, nullptr, nullptr, errbuf) != RSG_OK) { rsg_mlp_destroy(c, m); return RSG_FAIL; }
+  if (rsg_mlp_set_params(c, m, nullptr, nullptr, errbuf) != RSG_OK ||
+      rsg_mlp_set_scaling(c, m, nullptr, nullptr, nullptr, nullptr, errbuf) != RSG_OK) { rsg_mlp_destroy(c, m); return RSG_FAIL; }
   *ret = m;
   return RSG_OK;
 }
@@ -356,9 +363,49 @@ extern "C" void rsg_mlp_destroy(rsg_ctx *c, rsg_mlp *m)
 {
   if (!m) return;
   if (c) { hipSetDevice(c->device); hipStreamSynchronize(c->stream); }
-  hipFree(m->d_p); hipFree(m->d_m); hipFree(m->d_v); hipFree(m->d_g); hipFree(m->d_wb);
+  hipFree(m->d_p); hipFree(m->d_m); hipFree(m->d_v); hipFree(m->d_g); hipFree(m->d_wb); hipFree(m->d_scale);
   if (m->d_part) hipFree(m->d_part);
   delete m;
+}
+
+extern "C" int rsg_mlp_set_scaling(rsg_ctx *c, rsg_mlp *m, const float *xmean, const float *xstd,
+                                   const float *ymean, const float *ystd, char *errbuf)
+{
+  if (!c || !m) return rsg_fail(errbuf, "rsg_mlp_set_scaling: bad arguments");
+  const int d0 = m->dims[0], dl = m->dims[m->nl];
+  std::vector<float> h((size_t)(2 * (d0 + dl)));
+  for (int i = 0; i < d0; i++) {
+    h[(size_t)i] = xmean ? xmean[i] : 0.f;
+    const float sd = xstd ? xstd[i] : 1.f;
+    if (!(sd > 0.f)) return rsg_fail(errbuf, "rsg_mlp_set_scaling: input std %d must be > 0", i);
+    h[(size_t)(d0 + i)] = 1.f / sd;
+  }
+  for (int o = 0; o < dl; o++) {
+    h[(size_t)(2 * d0 + o)] = ymean ? ymean[o] : 0.f;
+    const float sd = ystd ? ystd[o] : 1.f;
+    if (!(sd > 0.f)) return rsg_fail(errbuf, "rsg_mlp_set_scaling: output std %d must be > 0", o);
+    h[(size_t)(2 * d0 + dl + o)] = sd;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpy(m->d_scale, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+  return RSG_OK;
+}
+
+extern "C" int rsg_mlp_set_range(rsg_mlp *m, const float *xmin, const float *xmax)
+{
+  if (!m) return RSG_FAIL;
+  if (!xmin || !xmax) { m->xmin.clear(); m->xmax.clear(); return RSG_OK; }
+  m->xmin.assign(xmin, xmin + m->dims[0]);
+  m->xmax.assign(xmax, xmax + m->dims[0]);
+  return RSG_OK;
+}
+
+extern "C" int rsg_mlp_in_range(const rsg_mlp *m, const float *x)
+{
+  if (!m || !x) return 0;
+  for (size_t i = 0; i < m->xmin.size(); i++)
+    if (!(x[i] >= m->xmin[i] && x[i] <= m->xmax[i])) return 0;
+  return 1;
 }
 
 extern "C" int rsg_mlp_set_precision(rsg_mlp *m, int bf16)

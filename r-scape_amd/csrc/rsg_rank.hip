@@ -324,6 +324,39 @@ extern "C" int rsg_cov_Evalues(rsg_ctx *c, rsg_msa *m, const rsg_hist *null, con
   return RSG_OK;
 }
 
+__global__ void k_evtail(const double *__restrict__ cov, int64_t L, double m, double lambda, double p0, double Nc,
+                         double *__restrict__ ev)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= L * L) return;
+  const int64_t i = e / L, j = e - i * L;
+  if (i == j) { ev[e] = INFINITY; return; }
+  const double p = p0 * exp(-lambda * (cov[e] - m));
+  ev[e] = Nc * (p < 1.0 ? p : 1.0);
+}
+
+extern "C" int rsg_cov_EvaluesTail(rsg_ctx *c, rsg_msa *m, double mu, double lambda, double p0, int64_t Nc,
+                                   double *Eval, char *errbuf)
+{
+  if (!c || !m || !Eval) return rsg_fail(errbuf, "rsg_cov_EvaluesTail: bad arguments");
+  if (!m->have_cov) return rsg_fail(errbuf, "no covariation matrix");
+  if (!(lambda > 0.) || !(p0 > 0.)) return rsg_fail(errbuf, "rsg_cov_EvaluesTail: needs lambda > 0 and p0 > 0");
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t L = m->L;
+  double *d_ev = nullptr;
+  HIPCHK(hipMalloc(&d_ev, sizeof(double) * (size_t)(L * L)));
+  int t; rsg_tstart(c, "evalues_tail", &t);
+  hipLaunchKernelGGL(k_evtail, dim3((unsigned)((L * L + 255) / 256)), dim3(256), 0, c->stream, m->d_cov, L, mu, lambda,
+                     p0, (double)Nc, d_ev);
+  rsg_tstop(c, t);
+  int st = RSG_OK;
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(Eval, d_ev, sizeof(double) * (size_t)(L * L), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) st = rsg_fail(errbuf, "rsg_cov_EvaluesTail failed");
+  hipFree(d_ev);
+  return st;
+}
+
 // ---------------------------------------------------------------- scalar helpers (host)
 extern "C" double rsg_cov2evalue(double cov, int64_t Nc, const rsg_hist *h, const double *survfit)
 {

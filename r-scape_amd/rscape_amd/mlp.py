@@ -15,6 +15,9 @@ def _declare():
     L.rsg_mlp_create.restype, L.rsg_mlp_create.argtypes = i32, [vp, i32, vp, C.c_uint64, C.POINTER(vp), cp]
     L.rsg_mlp_destroy.restype, L.rsg_mlp_destroy.argtypes = None, [vp, vp]
     L.rsg_mlp_set_precision.restype, L.rsg_mlp_set_precision.argtypes = i32, [vp, i32]
+    L.rsg_mlp_set_scaling.restype, L.rsg_mlp_set_scaling.argtypes = i32, [vp, vp, vp, vp, vp, vp, cp]
+    L.rsg_mlp_set_range.restype, L.rsg_mlp_set_range.argtypes = i32, [vp, vp, vp]
+    L.rsg_mlp_in_range.restype, L.rsg_mlp_in_range.argtypes = i32, [vp, vp]
     L.rsg_mlp_set_params.restype, L.rsg_mlp_set_params.argtypes = i32, [vp, vp, vp, vp, cp]
     L.rsg_mlp_get_params.restype, L.rsg_mlp_get_params.argtypes = i32, [vp, vp, vp, vp, cp]
     L.rsg_mlp_forward.restype, L.rsg_mlp_forward.argtypes = i32, [vp, vp, vp, i64, vp, cp]
@@ -61,6 +64,24 @@ class MLP:
     def set_precision(self, bf16):
         self.bf16 = bool(bf16)
         self.L.rsg_mlp_set_precision(self.h, 1 if bf16 else 0)
+
+    def set_scaling(self, xmean=None, xstd=None, ymean=None, ystd=None):
+        arrs = [None if a is None else _f32(a) for a in (xmean, xstd, ymean, ystd)]
+        err = _err()
+        _check(self.L.rsg_mlp_set_scaling(self.ctx.h, self.h, *[None if a is None else a.ctypes.data for a in arrs], err),
+               err, "rsg_mlp_set_scaling")
+        self._scaling = arrs
+
+    def set_range(self, xmin=None, xmax=None):
+        if xmin is None:
+            self.L.rsg_mlp_set_range(self.h, None, None)
+            return
+        self._range = (_f32(xmin), _f32(xmax))
+        self.L.rsg_mlp_set_range(self.h, self._range[0].ctypes.data, self._range[1].ctypes.data)
+
+    def in_range(self, x):
+        x = _f32(x)
+        return bool(self.L.rsg_mlp_in_range(self.h, x.ctypes.data))
 
     def _shapes(self):
         return [(self.dims[l + 1], self.dims[l]) for l in range(self.nl)]

@@ -34,7 +34,11 @@ def _declare():
     L.rsg_tree_yule.argtypes = [i64, d, C.c_uint64, C.c_uint32, vp, vp, vp, vp]
     L.rsg_null_simulate.restype, L.rsg_null_simulate.argtypes = i32, [vp, C.POINTER(NullJob), vp, cp]
     L.rsg_null_batch.restype = i32
-    L.rsg_null_batch.argtypes = [vp, C.POINTER(NullJob), i32, i32, d, d, d, i32, vp, vp, i32, cp]
+    L.rsg_null_batch.argtypes = [vp, C.POINTER(NullJob), i32, i32, d, d, d, i32, vp, vp, vp, i32, cp]
+    L.rsg_null_features.restype = i32
+    L.rsg_null_features.argtypes = [vp, vp, vp, vp, vp, vp, vp, cp]
+    L.rsg_cov_EvaluesTail.restype = i32
+    L.rsg_cov_EvaluesTail.argtypes = [vp, vp, d, d, d, i64, vp, cp]
     L._null_declared = True
     return L
 
@@ -98,11 +102,12 @@ def simulate(ctx, job):
     return out
 
 
-def batch(ctx, jobs, covtype="GTp", bmin=-10.0, pmass=5e-4, fracfit=1.0, expo=True, graph=0, hists=False):
+def batch(ctx, jobs, covtype="GTp", bmin=-10.0, pmass=5e-4, fracfit=1.0, expo=True, graph=0, hists=False,
+          feats=False):
     """Simulate + score + fit a batch on the device.  Jobs with group=None are nulls of one
     replicate each; jobs sharing a group id are the replicates of one null.  Returns (fits
     array (groups, 8), histograms); fits columns: mu, lambda, tau, newmass, pmass, w,
-    minCOV, maxCOV."""
+    minCOV, maxCOV.  feats=True also returns the (groups, 48) surrogate features."""
     from . import Hist, Histogram
     L = _declare()
     structs = [j.struct(k) for k, j in enumerate(jobs)]
@@ -111,10 +116,36 @@ def batch(ctx, jobs, covtype="GTp", bmin=-10.0, pmass=5e-4, fracfit=1.0, expo=Tr
     ng = max(s.group for s in structs) + 1
     fits = np.zeros(ng * 8)
     hp = (C.POINTER(Hist) * ng)() if hists else None
+    fe = np.zeros((ng, 48), np.float32) if feats else None
     err = _err()
     t = COV[covtype] if isinstance(covtype, str) else covtype
     _check(L.rsg_null_batch(ctx.h, arr, n, t, bmin, pmass, fracfit, 1 if expo else 0,
-                            C.cast(hp, C.c_void_p) if hists else None, _p(fits), int(graph), err),
+                            C.cast(hp, C.c_void_p) if hists else None, _p(fits), _p(fe), int(graph), err),
            err, "rsg_null_batch")
     hs = [Histogram(hp[k]) if hp[k] else None for k in range(ng)] if hists else None
+    if feats:
+        return fits.reshape(ng, 8), hs, fe
     return fits.reshape(ng, 8), hs
+
+
+def features(ctx, msa, tree):
+    """The surrogate's 48 inputs of a resident alignment and its tree."""
+    L = _declare()
+    t = {k: (np.ascontiguousarray(v, dtype=np.int32) if k in ("left", "right") else
+             np.ascontiguousarray(v, dtype=np.float64)) for k, v in tree.items() if k != "n"}
+    f = np.zeros(48, np.float32)
+    err = _err()
+    _check(L.rsg_null_features(ctx.h, msa.h, _p(t["left"]), _p(t["right"]), _p(t["ld"]), _p(t["rd"]), _p(f), err),
+           err, "rsg_null_features")
+    return f
+
+
+def evalues_tail(ctx, msa, m, lam, p0, Nc=None):
+    """E-values of the resident scores from a parametric tail p0 exp(-lam (x - m))."""
+    L = _declare()
+    n = msa.Lc
+    ev = np.zeros((n, n))
+    err = _err()
+    _check(L.rsg_cov_EvaluesTail(ctx.h, msa.h, m, lam, p0, n * (n - 1) // 2 if Nc is None else Nc, _p(ev), err),
+           err, "rsg_cov_EvaluesTail")
+    return ev

@@ -1,0 +1,170 @@
+"""Learned null-tail surrogate.
+
+A small MLP (48 -> 128 -> 128 -> 128 -> 2, GELU; rsg_mlp_* kernels) maps the features of an
+alignment and its tree (rsg_null_features) to the exponential tail of its phylogenetic null,
+written as P(score > x) = P0 * exp(-lambda (x - m)) with outputs (m, ln lambda).  It is
+trained on tail fits that the batched GPU null simulator produces (rsg_null_batch with
+replicate groups and features), and falls back to the full simulated null when an
+alignment's features leave the training range.
+
+Training data: random setups with N log-uniform in [20, 6000], L log-uniform in [50, 3500],
+Yule trees with mean branch length U(0.05, 0.3), random base composition and gap fraction.
+Each setup is simulated as a group of K replicates, K = clamp(ceil(2e5 / pairs), 1, 200), so
+every target fit sees at least ~100 pairs in a 5e-4 tail; the fit (exponential, pmass 5e-4)
+is moved to the reference mass P0 by m = mu + ln(newmass / P0) / lambda.
+"""
+import math
+import os
+
+import numpy as np
+
+from . import null as NL
+from .mlp import MLP
+
+P0 = 5e-4
+DIMS = (48, 128, 128, 128, 2)
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_gtp.npz")
+
+
+def tail_params(mu, lam, newmass):
+    """(m, lambda) of P(score > x) = P0 exp(-lambda (x - m)) from an exponential tail fit."""
+    return mu + math.log(newmass / P0) / lam, lam
+
+
+def replicates_for(L):
+    pairs = L * (L - 1) // 2
+    return int(min(200, max(1, math.ceil(2e5 / pairs))))
+
+
+def random_setup(rng, Nr=(20, 6000), Lr=(50, 3500)):
+    N = int(round(math.exp(rng.uniform(math.log(Nr[0]), math.log(Nr[1])))))
+    L = int(round(math.exp(rng.uniform(math.log(Lr[0]), math.log(Lr[1])))))
+    pi = rng.dirichlet(np.full(4, 8.0))
+    return dict(N=N, L=L, bl=float(rng.uniform(0.05, 0.3)), pi=tuple(float(p) for p in pi),
+                gap=float(rng.uniform(0.0, 0.3)), tseed=int(rng.integers(1 << 31)), sseed=int(rng.integers(1 << 31)))
+
+
+def _record_bytes(L, N, K):
+    return K * (80 * L * L + 3 * N * L)
+
+
+def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1, log=None, covtype="GTp"):
+    """n training pairs: X (n, 48) features, Y (n, 2) = (m, ln lambda); setups whose fit failed
+    are dropped.  Returns X, Y, list of setups."""
+    rng = np.random.default_rng(seed)
+    X, Y, S = [], [], []
+    pending, used = [], 0.0
+
+    def flush():
+        nonlocal pending, used
+        if not pending:
+            return
+        jobs = []
+        for g, (st, tree, model) in enumerate(pending):
+            K = replicates_for(st["L"])
+            jobs += [NL.Job(tree, st["L"], model, seed=st["sseed"], stream=r, gapfrac=st["gap"], group=g) for r in range(K)]
+        fits, _, fe = NL.batch(ctx, jobs, covtype, pmass=5e-4, expo=True, graph=graph, feats=True)
+        for g, (st, _, _) in enumerate(pending):
+            mu, lam, newmass = fits[g][0], fits[g][1], fits[g][3]
+            if not (np.isfinite(mu) and np.isfinite(lam) and lam > 0 and newmass > 0):
+                continue
+            m, lam = tail_params(mu, lam, newmass)
+            X.append(fe[g]); Y.append((m, math.log(lam))); S.append(st)
+        if log:
+            log("generated %d / %d" % (len(X), n))
+        pending, used = [], 0.0
+
+    made = 0
+    while made < n:
+        st = random_setup(rng, Nr, Lr)
+        tree = NL.yule_tree(st["N"], st["bl"], seed=st["tseed"])
+        model = NL.gtr_model(st["pi"])
+        b = _record_bytes(st["L"], st["N"], replicates_for(st["L"]))
+        if used + b > budget:
+            flush()
+        pending.append((st, tree, model)); used += b; made += 1
+    flush()
+    return np.array(X, np.float32), np.array(Y, np.float32), S
+
+
+class Surrogate:
+    """The trained network with its scalers and training range."""
+
+    def __init__(self, ctx, net=None, seed=1):
+        self.ctx = ctx
+        self.net = net or MLP(ctx, DIMS, seed=seed, bf16=True)
+        self.xmin = self.xmax = None
+
+    def fit(self, X, Y, epochs=60, batch=512, lr=1e-3, seed=7):
+        xm, xs = X.mean(0), X.std(0)
+        xs = np.where(xs > 1e-6, xs, 1.0)
+        ym, ys = Y.mean(0), Y.std(0)
+        self.net.set_scaling(xm, xs, ym, ys)
+        self.xmin, self.xmax = X.min(0), X.max(0)
+        self.net.set_range(self.xmin, self.xmax)
+        self.scaling = (xm, xs, ym, ys)
+        return self.net.train(X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+
+    def predict(self, feats):
+        """(m, lambda) per row of feats; rows outside the training range get NaN."""
+        feats = np.atleast_2d(np.asarray(feats, np.float32))
+        out = self.net.forward(feats).astype(np.float64)
+        ok = np.array([self.net.in_range(f) for f in feats])
+        m, lam = out[:, 0], np.exp(out[:, 1])
+        m[~ok] = np.nan; lam[~ok] = np.nan
+        return m, lam
+
+    def save(self, path=DATA):
+        W, b = self.net.get_params()
+        xm, xs, ym, ys = self.scaling
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        np.savez(path, dims=np.array(DIMS), xm=xm, xs=xs, ym=ym, ys=ys, xmin=self.xmin, xmax=self.xmax,
+                 **{"W%d" % k: w for k, w in enumerate(W)}, **{"b%d" % k: v for k, v in enumerate(b)})
+
+    @classmethod
+    def load(cls, ctx, path=DATA):
+        z = np.load(path, allow_pickle=False)
+        s = cls(ctx)
+        nl = len(z["dims"]) - 1
+        s.net.set_params([z["W%d" % k] for k in range(nl)], [z["b%d" % k] for k in range(nl)])
+        s.scaling = (z["xm"], z["xs"], z["ym"], z["ys"])
+        s.net.set_scaling(*s.scaling)
+        s.xmin, s.xmax = z["xmin"], z["xmax"]
+        s.net.set_range(s.xmin, s.xmax)
+        return s
+
+
+# ------------------------------------------------------------------ significance runs
+def full_null(ctx, tree, L, pi, gapfrac, K=20, seed=5, covtype="GTp", graph=1, pmass=5e-4):
+    """The simulated null of one alignment: K replicates on its tree (one batched group).
+    Returns (histogram, survfit, fit row)."""
+    model = NL.gtr_model(pi)
+    jobs = [NL.Job(tree, L, model, seed=seed, stream=r, gapfrac=gapfrac, group=0) for r in range(K)]
+    fits, hs = NL.batch(ctx, jobs, covtype, pmass=pmass, expo=True, graph=graph, hists=True)
+    h = hs[0]
+    fit, sf = h.fit(ctx, pmass, 1.0, True)
+    return h, sf, fits[0]
+
+
+def msa_null_params(ax):
+    """Composition and gap fraction of an alignment (the null simulator's inputs)."""
+    ax = np.asarray(ax)
+    cnt = np.array([(ax == x).sum() for x in range(4)], np.float64)
+    pi = cnt / cnt.sum()
+    pi = np.maximum(pi, 1e-3); pi /= pi.sum()
+    gap = 1.0 - cnt.sum() / ax.size
+    return tuple(pi), float(gap)
+
+
+def nshuffle_default(N, L):
+    """R-scape's nshuffle rule (src/R-scape.c:2449-2456)."""
+    ns = 20
+    if N * L < 1e3:
+        ns = 200
+    if L < 50:
+        ns = 200
+    if L < 40:
+        ns = 300
+    if N < 4:
+        ns = 300
+    return ns
