@@ -71,6 +71,103 @@ def cpu_baseline(ax, Lwin):
                       "N=%d alignment, %.1f s" % (Lwin - 1, N, dt)}
 
 
+def evalue_runs(ctx, d, K=20, reps=3):
+    """Complete significance runs of the alignment (GTp scoring + null + exponential tail fit +
+    every pair's E-value), three ways: the plain per-replicate GPU path (simulate one replicate,
+    score it, bin it, merge on the host; R-scape's loop with the GPU doing each step), the
+    batched null (all K replicates in one captured graph), and the learned surrogate (features
+    + MLP + parametric tail; full null when the features leave the training range)."""
+    from rscape_amd import null as NL
+    from rscape_amd import surrogate as S
+    ax, tree = d["ax"], d["tree"]
+    N, L = ax.shape
+    Nc = L * (L - 1) // 2
+    pi, gap = S.msa_null_params(ax)
+    model = NL.gtr_model(pi)
+    m = ctx.msa(ax)
+    sur = S.Surrogate.load(ctx) if os.path.exists(S.DATA) else None
+
+    def plain():
+        m.calculate("GTp")
+        cum, w = None, None
+        for r in range(K):
+            leaves = NL.simulate(ctx, NL.Job(tree, L, model, seed=7, stream=r, gapfrac=gap))
+            mr = ctx.msa(leaves)
+            a, b = mr.calculate("GTp")
+            if w is None:
+                w = R.width(b, a)
+            cum = R.Histogram.null_add(cum, mr.rank_hist(-10.0, w))
+            mr.close()
+        _, sf = cum.fit(ctx, 5e-4, 1.0, True)
+        return m.evalues(cum, sf, Nc)
+
+    def batched():
+        m.calculate("GTp")
+        h, sf, _ = S.full_null(ctx, tree, L, pi, gap, K=K, seed=7, graph=1)
+        return m.evalues(h, sf, Nc)
+
+    fallback = [False]
+
+    def surrogate():
+        m.calculate("GTp")
+        mu, lam = sur.predict(NL.features(ctx, m, tree))
+        if not np.isfinite(mu[0]):
+            fallback[0] = True
+            return batched()
+        return NL.evalues_tail(ctx, m, float(mu[0]), float(lam[0]), S.P0, Nc)
+
+    def timed(fn):
+        ev = fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), ev
+
+    out = {"unit": "runs/s", "replicates": K, "workload": "N=%d x L=%d, GTp, exponential tail, pmass 5e-4" % (N, L)}
+    tp, ep = timed(plain)
+    tb, eb = timed(batched)
+    iu = np.triu_indices(L, 1)
+    sig = eb[iu] < 0.05
+    out["plain_per_replicate"] = 1.0 / tp
+    out["batched_null"] = 1.0 / tb
+    out["plain_vs_batched_max_abs_log10E"] = float(np.max(np.abs(np.log10(np.maximum(ep[iu][sig], 1e-300)) -
+                                                              np.log10(np.maximum(eb[iu][sig], 1e-300))))) if sig.any() else 0.0
+    if sur is not None:
+        ts, es = timed(surrogate)
+        err = np.abs(np.log10(np.maximum(es[iu][sig], 1e-300)) - np.log10(np.maximum(eb[iu][sig], 1e-300)))
+        out["surrogate"] = 1.0 / ts
+        out["surrogate_fallback"] = fallback[0]
+        out["speedup_surrogate_vs_plain"] = tp / ts
+        out["speedup_batched_vs_plain"] = tp / tb
+        out["sig_pairs"] = int(sig.sum())
+        out["surrogate_log10E_err_median"] = float(np.median(err)) if len(err) else None
+        out["surrogate_log10E_err_frac_within_0p3"] = float(np.mean(err <= 0.3)) if len(err) else None
+        out["call_agreement"] = float(np.mean((es[iu] < 0.05) == sig))
+    m.close()
+    return out
+
+
+def fp32_errors(ctx, ax):
+    """Max relative error of each FP32 statistic against the FP64 path on the same counts
+    (floored at 1e-3 of the largest |score|, as the parity tests do)."""
+    m = ctx.msa(ax)
+    m.probs(R.COUNTS_INT8)
+    L = ax.shape[1]
+    iu = np.triu_indices(L, 1)
+    out = {}
+    for name in ("GT", "MI", "MIr", "MIg", "CHI"):
+        m.cov(name, R.C16, prec=R.STAT_FP64)
+        ref = m.get_cov()[0][iu]
+        m.cov(name, R.C16, prec=R.STAT_FP32)
+        got = m.get_cov()[0][iu]
+        floor = 1e-3 * np.max(np.abs(ref))
+        out[name] = float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), floor)))
+    m.close()
+    return out
+
+
 def load_traffic(name):
     """HBM bytes per launch from a committed rocprofv3 counter summary (or None)."""
     p = os.path.join(ROOT, "profiles", name)
@@ -107,6 +204,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--calib", action="store_true", help="also run the counter calibration kernel")
+    ap.add_argument("--no-evalue", action="store_true", help="skip the significance-run timing")
+    ap.add_argument("--evalue-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -170,8 +269,20 @@ def main():
         kw3 = dict(covtype="GTp", counts=R.COUNTS_INT8, prec=R.STAT_FP32)
         m.calculate(**kw3)
         dt3, _ = run_steps(ctx, m, max(3, args.steps // 4), **kw3)
-        extra["fp32_statistic"] = {"value": upd * max(3, args.steps // 4) / dt3, "unit": "updates/s"}
+        extra["fp32_statistic"] = {"value": upd * max(3, args.steps // 4) / dt3, "unit": "updates/s",
+                                   "max_rel_err_vs_fp64": fp32_errors(ctx, ax)}
+        # PCIe-inclusive: the alignment uploaded from host memory before every scoring
+        m.calculate(**kw)
+        k = max(3, args.steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            m.update(ax)
+            m.calculate(**kw)
+        extra["pcie_inclusive"] = {"value": upd * k / (time.perf_counter() - t0), "unit": "updates/s",
+                                   "note": "N x L codes copied host->device and re-packed before each step"}
         out["extra"] = extra
+    if not args.no_evalue:
+        out["evalue_runs"] = evalue_runs(ctx, d, reps=args.evalue_reps)
     if args.calib:
         calibration_kernel(ctx)
     if not args.no_cpu:

@@ -59,6 +59,10 @@ int      rsg_ctx_timings(rsg_ctx *ctx, const char **names, double *ms, int *n, i
 int         rsg_gram_nshapes(void);
 const char *rsg_gram_shape_name(int shape);
 int         rsg_ctx_set_gram_shape(rsg_ctx *ctx, int shape);
+/* 1 (default): when the tile count leaves a last partial wave of work groups, those tiles
+ * are split along the sequence axis over the idle CUs and their partial counts summed
+ * (integers, so the result is identical); 0: one work group per tile. */
+int         rsg_ctx_set_gram_split(rsg_ctx *ctx, int on);
 /* Counter calibration (no reference counterpart): runs the plain copy kernel
  * k_copy_probe over `bytes` (multiple of 16) three times, reading and writing `bytes` per
  * launch, so HBM counters of other kernels can be converted to bytes. */

@@ -18,6 +18,7 @@ extern "C" rsg_ctx *rsg_ctx_create(int device, char *errbuf)
   rsg_ctx *c = new (std::nothrow) rsg_ctx();
   if (!c) { rsg_fail(errbuf, "out of memory"); return nullptr; }
   c->device = device;
+  hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->d_flag, 64) != hipSuccess ||
       hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
@@ -36,6 +37,7 @@ extern "C" void rsg_ctx_destroy(rsg_ctx *c)
   for (int k = 0; k < RSG_MAX_TIMERS; k++)
     if (c->timers[k].a) { hipEventDestroy(c->timers[k].a); hipEventDestroy(c->timers[k].b); }
   if (c->d_red) hipFree(c->d_red);
+  if (c->d_gscr) hipFree(c->d_gscr);
   if (c->d_flag) hipFree(c->d_flag);
   if (c->h_pinned) hipHostFree(c->h_pinned);
   hipStreamDestroy(c->stream);
@@ -85,6 +87,7 @@ extern "C" int rsg_ctx_timings(rsg_ctx *c, const char **names, double *ms, int *
 }
 
 extern "C" int rsg_ctx_set_gram_shape(rsg_ctx *c, int shape) { c->gram_shape = shape; return RSG_OK; }
+extern "C" int rsg_ctx_set_gram_split(rsg_ctx *c, int on) { c->gram_split = on ? 1 : 0; return RSG_OK; }
 
 static void free_msa(rsg_msa *m)
 {

@@ -107,10 +107,14 @@ struct CodeStage {
 // int8 Gram: WI x WJ waves, each 32 i-columns x 32 j-columns, KT sequences per stage.
 // Records are written as the raw accumulators (count << (x+y)), so the accumulators
 // never pass through VALU and stay in AGPRs.
-// One work-group tile (i0, j0) of one alignment; `lds` holds 2 * St::BYTES.
+// One work-group tile (i0, j0) of one alignment over K-tiles [kt0, kt1); `lds` holds
+// 2 * St::BYTES.  With part == nullptr the records go to `out`; otherwise the tile's partial
+// sums go, unconditionally, to part[(jj * BI + ii) * 16 + cell] (ii, jj: column offsets in
+// the tile) for a later reduction.
 template <int WI, int WJ, int KT>
 __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L,
-                                             uint32_t *__restrict__ out, int64_t i0, int64_t j0, uint8_t *lds)
+                                             uint32_t *__restrict__ out, int64_t i0, int64_t j0, uint8_t *lds,
+                                             int64_t kt0, int64_t kt1, uint32_t *__restrict__ part)
 {
   constexpr int BI = 32 * WI, BJ = 32 * WJ, NT = 64 * WI * WJ, KS = KT / 32;
   typedef CodeStage<BI, BJ, KT, NT> St;
@@ -129,15 +133,14 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
       for (int r = 0; r < 16; r++) acc[x][y][r] = 0;
 
   uint4 pre[St::NR];
-  const int64_t nkt = Npad / KT;
-  STAGE_GLOAD(St, pre, xt, Npad, i0, j0, 0, tid);
+  STAGE_GLOAD(St, pre, xt, Npad, i0, j0, kt0 * KT, tid);
   STAGE_SSTORE(St, pre, lds, tid);
   __syncthreads();
-  for (int64_t kt = 0; kt < nkt; kt++) {
-    const int buf = (int)(kt & 1);
+  for (int64_t kt = kt0; kt < kt1; kt++) {
+    const int buf = (int)((kt - kt0) & 1);
     // straight-line prefetch (the last stage reloads itself, in bounds and unused), so the
     // prefetch registers are never conditionally live
-    const int64_t knext = ((kt + 1 < nkt) ? kt + 1 : kt) * KT;
+    const int64_t knext = ((kt + 1 < kt1) ? kt + 1 : kt) * KT;
     STAGE_GLOAD(St, pre, xt, Npad, i0, j0, knext, tid);
     const uint8_t *base = lds + buf * St::BYTES;
     uint4 ra = *(const uint4 *)(base + offA);
@@ -168,12 +171,16 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
   }
 
   // epilogue: lane owns i column `col`; register r holds j = 8*(r/4) + 4*khalf + r%4.
-  const int64_t i = i0 + wi * 32 + col;
+  const int ii = wi * 32 + col;
+  const int64_t i = i0 + ii;
 #pragma unroll
   for (int r = 0; r < 16; r++) {
-    const int64_t j = j0 + wj * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
-    if (i < j && j < L) {
-      uint4 *p = (uint4 *)(out + rsg_rec(i, j, L) * 16);
+    const int jj = wj * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
+    const int64_t j = j0 + jj;
+    uint4 *p = nullptr;
+    if (part) p = (uint4 *)(part + ((int64_t)jj * BI + ii) * 16);
+    else if (i < j && j < L) p = (uint4 *)(out + rsg_rec(i, j, L) * 16);
+    if (p) {
 #pragma unroll
       for (int x = 0; x < 4; x++)
         p[x] = make_uint4(acc[x][0][r], acc[x][1][r], acc[x][2][r], acc[x][3][r]);
@@ -181,19 +188,58 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
   }
 }
 
+// Grid: 8 * per_xcd work groups for the first ndp tiles (whole K range), then nsplit work
+// groups for each of the remaining ntiles - ndp tiles, each over one K segment, writing
+// partial sums for k_gram_reduce.  The split keeps the last, partial wave of tiles from
+// leaving most CUs idle.
 template <int WI, int WJ, int KT>
 __global__ void __launch_bounds__(64 * WI * WJ)
 k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, uint32_t *__restrict__ out,
-          const int2 *__restrict__ tiles, int ntiles, int per_xcd)
+          const int2 *__restrict__ tiles, int ndp, int per_xcd, int ntiles, int nsplit,
+          uint32_t *__restrict__ part)
 {
   typedef CodeStage<32 * WI, 32 * WJ, KT, 64 * WI * WJ> St;
   __shared__ __attribute__((aligned(16))) uint8_t lds[2 * St::BYTES];
-  // XCD-aware mapping: consecutive tiles of the (bi, bj)-sorted list run on one XCD (one L2).
   const int g = blockIdx.x;
-  const int t = (g & 7) * per_xcd + (g >> 3);
-  if (t >= ntiles) return;
-  const int2 tl = tiles[t];
-  gram_tile_i8<WI, WJ, KT>(xt, Npad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds);
+  const int64_t nkt = Npad / KT;
+  if (g < 8 * per_xcd) {
+    // XCD-aware mapping: consecutive tiles of the (bi, bj)-sorted list run on one XCD (one L2).
+    const int t = (g & 7) * per_xcd + (g >> 3);
+    if (t >= ndp) return;
+    const int2 tl = tiles[t];
+    gram_tile_i8<WI, WJ, KT>(xt, Npad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, 0, nkt, nullptr);
+  } else {
+    const int u = g - 8 * per_xcd, t = ndp + u / nsplit, sg = u % nsplit;
+    if (t >= ntiles) return;
+    const int2 tl = tiles[t];
+    const int64_t k0 = nkt * sg / nsplit, k1 = nkt * (sg + 1) / nsplit;
+    gram_tile_i8<WI, WJ, KT>(xt, Npad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, k0, k1,
+                             part + (int64_t)u * (32 * WI) * (32 * WJ) * 16);
+  }
+}
+
+// Sum the nsplit partial tiles of each split tile into its records (integer sums: exact).
+template <int BI, int BJ>
+__global__ void __launch_bounds__(256)
+k_gram_reduce(const uint32_t *__restrict__ part, const int2 *__restrict__ tiles, int ndp, int nsplit, int64_t L,
+              uint32_t *__restrict__ out)
+{
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;       // (split tile, pair, quarter)
+  const int64_t per_tile = (int64_t)BI * BJ * 4;
+  const int t = (int)(q / per_tile);
+  const int64_t rem = q % per_tile, pr = rem >> 2;
+  const int quarter = (int)(rem & 3);
+  const int ii = (int)(pr % BI), jj = (int)(pr / BI);
+  const int2 tl = tiles[ndp + t];
+  const int64_t i = (int64_t)tl.x * BI + ii, j = (int64_t)tl.y * BJ + jj;
+  if (!(i < j && j < L)) return;
+  const uint4 *p = (const uint4 *)(part + ((int64_t)t * nsplit * BI * BJ + pr) * 16) + quarter;
+  uint4 s = make_uint4(0, 0, 0, 0);
+  for (int k = 0; k < nsplit; k++) {
+    const uint4 v = p[(int64_t)k * BI * BJ * 4];
+    s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+  }
+  ((uint4 *)(out + rsg_rec(i, j, L) * 16))[quarter] = s;
 }
 
 // Batched int8 Gram over many alignments: one work group per (item, tile), one wave per
@@ -206,7 +252,8 @@ k_gram_i8_batch(const int4 *__restrict__ tiles, const int64_t *__restrict__ desc
   __shared__ __attribute__((aligned(16))) uint8_t lds[2 * St::BYTES];
   const int4 tl = tiles[blockIdx.x];                    // (item, bi, bj, -)
   const int64_t *d = desc + (int64_t)tl.x * RSG_BDESC;  // item descriptor (see rsg_batch.hip)
-  gram_tile_i8<1, 1, 64>(xt + d[4], d[2], d[1], cnt + d[5], (int64_t)tl.y * 32, (int64_t)tl.z * 32, lds);
+  gram_tile_i8<1, 1, 64>(xt + d[4], d[2], d[1], cnt + d[5], (int64_t)tl.y * 32, (int64_t)tl.z * 32, lds,
+                         0, d[2] / 64, nullptr);
 }
 
 int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_tiles, int ntiles, const int64_t *d_desc, const uint8_t *d_xt,
@@ -388,12 +435,39 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, char *errbuf)
   const int2 *tiles = tile_list(c->device, m->L, BI, BJ, &ntiles, errbuf);
   if (!tiles) return RSG_FAIL;
   if (ntiles == 0) return RSG_OK;
-  int per = (ntiles + 7) / 8;
-  dim3 grid((unsigned)(8 * per)), block(64 * WI * WJ);
-  int t; rsg_tstart(c, "gram_i8", &t);
   (void)kind;
+  // whole waves of tiles run over the full K range; the last partial wave is split along K
+  // into enough segments to occupy every CU, then reduced
+  const int ncu = c->ncu > 0 ? c->ncu : 256;
+  const int64_t nkt = m->Npad / KT;
+  int ndp = ntiles, nsplit = 1;
+  const int rem = ntiles % ncu;
+  if (c->gram_split && rem > 0 && rem <= ncu / 2 && nkt >= 2) {
+    ndp = ntiles - rem;
+    nsplit = (int)std::min<int64_t>(nkt, ncu / rem);
+  }
+  const int nrem = ntiles - ndp;
+  uint32_t *part = nullptr;
+  if (nrem > 0) {
+    const size_t need = (size_t)nrem * nsplit * BI * BJ * 64;
+    if (c->gscr_cap < need) {
+      if (c->d_gscr) hipFree(c->d_gscr);
+      c->d_gscr = nullptr; c->gscr_cap = 0;
+      HIPCHK(hipMalloc(&c->d_gscr, need));
+      c->gscr_cap = need;
+    }
+    part = (uint32_t *)c->d_gscr;
+  }
+  const int per = (ndp + 7) / 8;
+  dim3 grid((unsigned)(8 * per + nrem * nsplit)), block(64 * WI * WJ);
+  int t; rsg_tstart(c, "gram_i8", &t);
   hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->L, (uint32_t *)out,
-                     tiles, ntiles, per);
+                     tiles, ndp, per, ntiles, nsplit, part);
+  if (nrem > 0) {
+    const int64_t nq = (int64_t)nrem * BI * BJ * 4;
+    hipLaunchKernelGGL((k_gram_reduce<BI, BJ>), dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, c->stream, part, tiles,
+                       ndp, nsplit, m->L, (uint32_t *)out);
+  }
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   return RSG_OK;

@@ -49,6 +49,9 @@ struct rsg_ctx {
   int          ntimers = 0;
   rsg_timer    timers[RSG_MAX_TIMERS];
   int          gram_shape = 0;          // 0 = automatic
+  int          gram_split = 1;          // split the last partial wave of Gram tiles along K
+  int          ncu = 0;                 // compute units of the device
+  void        *d_gscr = nullptr;  size_t gscr_cap = 0;   // Gram split partials
   // small reusable device scratch (reductions)
   double      *d_red = nullptr;  size_t red_cap = 0;
   int         *d_flag = nullptr;

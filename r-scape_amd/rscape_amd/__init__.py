@@ -55,6 +55,7 @@ def _declare(L):
         "rsg_gram_nshapes": (i32, []),
         "rsg_gram_shape_name": (cp, [i32]),
         "rsg_ctx_set_gram_shape": (i32, [vp, i32]),
+        "rsg_ctx_set_gram_split": (i32, [vp, i32]),
         "rsg_msa_create": (i32, [vp, vp, vp, i64, i64, C.POINTER(vp), cp]),
         "rsg_msa_update": (i32, [vp, vp, vp, vp, cp]),
         "rsg_msa_destroy": (None, [vp, vp]),
@@ -236,6 +237,9 @@ class Context:
 
     def set_gram_shape(self, shape):
         self.L.rsg_ctx_set_gram_shape(self.h, int(shape))
+
+    def set_gram_split(self, on):
+        self.L.rsg_ctx_set_gram_split(self.h, 1 if on else 0)
 
     def msa(self, ax, wgt=None):
         return Msa(self, ax, wgt)

@@ -2,16 +2,19 @@
 
 A small MLP (48 -> 128 -> 128 -> 128 -> 2, GELU; rsg_mlp_* kernels) maps the features of an
 alignment and its tree (rsg_null_features) to the exponential tail of its phylogenetic null,
-written as P(score > x) = P0 * exp(-lambda (x - m)) with outputs (m, ln lambda).  It is
+written as P(score > x) = P0 * exp(-lambda (x - m)).  The network outputs (lambda m, ln lambda):
+an error in lambda m moves log E by the same amount at every score, where an error in m
+would be weighted by lambda, and lambda ranges over more than an order of magnitude.  It is
 trained on tail fits that the batched GPU null simulator produces (rsg_null_batch with
 replicate groups and features), and falls back to the full simulated null when an
 alignment's features leave the training range.
 
 Training data: random setups with N log-uniform in [20, 6000], L log-uniform in [50, 3500],
 Yule trees with mean branch length U(0.05, 0.3), random base composition and gap fraction.
-Each setup is simulated as a group of K replicates, K = clamp(ceil(2e5 / pairs), 1, 200), so
-every target fit sees at least ~100 pairs in a 5e-4 tail; the fit (exponential, pmass 5e-4)
-is moved to the reference mass P0 by m = mu + ln(newmass / P0) / lambda.
+Each setup is simulated as a group of K replicates, K = clamp(ceil(T / pairs), 1, 200) with
+T = 2e5 null pairs by default, so a target fit sees ~100 pairs in a 5e-4 tail (fewer only for
+the smallest alignments); the fit (exponential, pmass 5e-4) is moved to the reference mass P0
+by m = mu + ln(newmass / P0) / lambda.
 """
 import math
 import os
@@ -31,9 +34,9 @@ def tail_params(mu, lam, newmass):
     return mu + math.log(newmass / P0) / lam, lam
 
 
-def replicates_for(L):
+def replicates_for(L, target_pairs=2e5, cap=200):
     pairs = L * (L - 1) // 2
-    return int(min(200, max(1, math.ceil(2e5 / pairs))))
+    return int(min(cap, max(1, math.ceil(target_pairs / pairs))))
 
 
 def random_setup(rng, Nr=(20, 6000), Lr=(50, 3500)):
@@ -48,9 +51,10 @@ def _record_bytes(L, N, K):
     return K * (80 * L * L + 3 * N * L)
 
 
-def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1, log=None, covtype="GTp"):
-    """n training pairs: X (n, 48) features, Y (n, 2) = (m, ln lambda); setups whose fit failed
-    are dropped.  Returns X, Y, list of setups."""
+def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1, log=None, covtype="GTp",
+             target_pairs=2e5):
+    """n training pairs: X (n, 48) features, Y (n, 2) = (lambda m, ln lambda); setups whose fit
+    failed are dropped.  Returns X, Y, list of setups."""
     rng = np.random.default_rng(seed)
     X, Y, S = [], [], []
     pending, used = [], 0.0
@@ -61,7 +65,7 @@ def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1,
             return
         jobs = []
         for g, (st, tree, model) in enumerate(pending):
-            K = replicates_for(st["L"])
+            K = replicates_for(st["L"], target_pairs)
             jobs += [NL.Job(tree, st["L"], model, seed=st["sseed"], stream=r, gapfrac=st["gap"], group=g) for r in range(K)]
         fits, _, fe = NL.batch(ctx, jobs, covtype, pmass=5e-4, expo=True, graph=graph, feats=True)
         for g, (st, _, _) in enumerate(pending):
@@ -69,7 +73,7 @@ def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1,
             if not (np.isfinite(mu) and np.isfinite(lam) and lam > 0 and newmass > 0):
                 continue
             m, lam = tail_params(mu, lam, newmass)
-            X.append(fe[g]); Y.append((m, math.log(lam))); S.append(st)
+            X.append(fe[g]); Y.append((lam * m, math.log(lam))); S.append(st)
         if log:
             log("generated %d / %d" % (len(X), n))
         pending, used = [], 0.0
@@ -79,7 +83,7 @@ def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1,
         st = random_setup(rng, Nr, Lr)
         tree = NL.yule_tree(st["N"], st["bl"], seed=st["tseed"])
         model = NL.gtr_model(st["pi"])
-        b = _record_bytes(st["L"], st["N"], replicates_for(st["L"]))
+        b = _record_bytes(st["L"], st["N"], replicates_for(st["L"], target_pairs))
         if used + b > budget:
             flush()
         pending.append((st, tree, model)); used += b; made += 1
@@ -110,7 +114,8 @@ class Surrogate:
         feats = np.atleast_2d(np.asarray(feats, np.float32))
         out = self.net.forward(feats).astype(np.float64)
         ok = np.array([self.net.in_range(f) for f in feats])
-        m, lam = out[:, 0], np.exp(out[:, 1])
+        lam = np.exp(out[:, 1])
+        m = out[:, 0] / lam
         m[~ok] = np.nan; lam[~ok] = np.nan
         return m, lam
 

@@ -64,6 +64,26 @@ def test_gram_shapes_bit_exact(ctx, shape):
     assert np.array_equal(got[iu], ref[iu].astype(float))
 
 
+@pytest.mark.parametrize("N,L", [(600, 1500), (300, 203), (5000, 90)])
+def test_gram_split_matches_unsplit(ctx, N, L):
+    """Tiles of the last partial wave split along K (partials summed) give the same counts."""
+    print("gram split N=%d L=%d" % (N, L), flush=True)
+    ax = _msa(N, L, 31 + L)
+    m = ctx.msa(ax)
+    m.probs(R.COUNTS_INT8)
+    a = m.counts()
+    ctx.set_gram_split(False)
+    try:
+        m.probs(R.COUNTS_INT8)
+        b = m.counts()
+    finally:
+        ctx.set_gram_split(True)
+    assert np.array_equal(a, b)
+    ref = oracle.pair_counts(ax).reshape(L, L, 16)
+    iu = np.triu_indices(L, 1)
+    assert np.array_equal(a.reshape(L, L, 16)[iu], ref[iu].astype(float))
+
+
 def test_counts_u32_path(ctx):
     """N >= 65536 switches the records to 32-bit counts."""
     N, L = 70000, 9

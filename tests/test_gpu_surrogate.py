@@ -136,7 +136,7 @@ def test_surrogate_beats_constant_tail(ctx):
     pm, pl = sur.predict(X[va])
     ok = np.isfinite(pm)
     assert ok.mean() > 0.7
-    rm = np.sqrt(np.mean((pm[ok] - Y[va, 0][ok]) ** 2)); cm = np.sqrt(np.mean((Y[tr, 0].mean() - Y[va, 0][ok]) ** 2))
+    rm = np.sqrt(np.mean((pm[ok] * pl[ok] - Y[va, 0][ok]) ** 2)); cm = np.sqrt(np.mean((Y[tr, 0].mean() - Y[va, 0][ok]) ** 2))
     rl = np.sqrt(np.mean((np.log(pl[ok]) - Y[va, 1][ok]) ** 2)); cl = np.sqrt(np.mean((Y[tr, 1].mean() - Y[va, 1][ok]) ** 2))
-    print("rmse m %.3f (const %.3f)  rmse ln lambda %.3f (const %.3f)" % (rm, cm, rl, cl))
+    print("rmse lambda*m %.3f (const %.3f)  rmse ln lambda %.3f (const %.3f)" % (rm, cm, rl, cl))
     assert rm < 0.7 * cm and rl < 0.7 * cl
