@@ -133,9 +133,15 @@ int  rsg_hist_NullAdd(rsg_hist **cum, const rsg_hist *h, char *errbuf);
  * owned by the caller (allocated by the library with rsg_free). NULL if the fit is infinite. */
 int  rsg_hist_NullFit(rsg_ctx *ctx, rsg_hist *null, double pmass, double fracfit, int doexpfit,
                       double **ret_survfit, double *fit, char *errbuf);
-/* cov2evalue for every pair: Eval[i][j] = p * Nc (covariation.c:846-856), FP64 on the device. */
+/* cov2evalue for every pair: Eval[i][j] = p * Nc (covariation.c:846-856), FP64 on the device.
+ * The matrix stays on the device for rsg_cov_Hits; Eval (may be NULL) receives a copy. */
 int  rsg_cov_Evalues(rsg_ctx *ctx, rsg_msa *msa, const rsg_hist *null, const double *survfit,
                      int64_t Nc, double *Eval, char *errbuf);
+/* The significant pairs of the last E-value call (cov_CreateHitList's selection,
+ * src/covariation.c:778-1006): pairs i < j with E <= thresh, sorted by (i, j), up to cap of
+ * them; *nhits receives the total (RSG_FAIL with the first cap returned if it exceeds cap). */
+int  rsg_cov_Hits(rsg_ctx *ctx, rsg_msa *msa, double thresh, int cap, int *nhits, int *i, int *j,
+                  double *score, double *evalue, char *errbuf);
 /* evalue2cov (covariation.c:2404-2436). */
 double rsg_evalue2cov(double eval_thresh, int64_t Nc, const rsg_hist *h, const double *survfit);
 double rsg_cov2evalue(double cov, int64_t Nc, const rsg_hist *h, const double *survfit);
@@ -206,7 +212,8 @@ int  rsg_null_features(rsg_ctx *ctx, rsg_msa *msa, const int *left, const int *r
                        const double *ld, const double *rd, float *feat, char *errbuf);
 /* E-values from a parametric tail P(score > x) = p0 exp(-lambda (x - m)) (the surrogate's
  * output): Eval[i][j] = Nc * min(1, p0 exp(-lambda (COV[i][j] - m))) for i != j, +inf on
- * the diagonal (cov2evalue's role, src/covariation.c:2369-2436, for a fitted tail). */
+ * the diagonal (cov2evalue's role, src/covariation.c:2369-2436, for a fitted tail).  Kept on
+ * the device for rsg_cov_Hits; Eval may be NULL. */
 int  rsg_cov_EvaluesTail(rsg_ctx *ctx, rsg_msa *msa, double m, double lambda, double p0, int64_t Nc,
                          double *Eval, char *errbuf);
 typedef struct rsg_mlp rsg_mlp;

@@ -25,7 +25,8 @@
 #pragma clang fp contract(off)
 
 enum { D_N = 0, D_L, D_NPAD, D_LPAD, D_XT, D_CNT, D_COV, D_PM, D_ROWS, D_HIST, D_LEAF, D_BPACK, D_BCOL,
-       D_BSTAT, D_BEL, D_NB32, D_GROUP };
+       D_BSTAT, D_BEL, D_NB32, D_GROUP, D_BHIST };
+#define HIST_EPB 65536       // matrix elements per histogram block
 #define NBMAX 16384          // histogram bins per item (an item needing more is reported as failed)
 #define ELPB 1024            // matrix elements per block in the element-wise kernels
 
@@ -216,27 +217,37 @@ __global__ void k_fill_int(int *__restrict__ p, int n)
   if (k < n) p[k] = 0;
 }
 
+// Rank histogram of every item into its group's bins: each block bins up to HIST_EPB
+// matrix elements into an LDS histogram (32-bit, at most HIST_EPB per bin), then adds its
+// nonzero bins to the group histogram.
 __global__ void __launch_bounds__(256)
 k_hist_b(const int64_t *__restrict__ desc, int n, const double *__restrict__ cov, const HGeo *__restrict__ geo,
          double bmin, unsigned long long *__restrict__ hist)
 {
-  const int it = find_item(desc, n, D_BEL, blockIdx.x);
+  __shared__ unsigned int lh[NBMAX];
+  const int it = find_item(desc, n, D_BHIST, blockIdx.x);
   const int64_t *d = desc + it * RSG_BDESC;
   const HGeo g = geo[d[D_GROUP]];
   if (!g.ok) return;
-  const int64_t L = d[D_L], e0 = (blockIdx.x - d[D_BEL]) * ELPB;
+  for (int b = threadIdx.x; b < g.nb; b += 256) lh[b] = 0u;
+  __syncthreads();
+  const int64_t L = d[D_L], e0 = (blockIdx.x - d[D_BHIST]) * (int64_t)HIST_EPB;
+  const int64_t e1 = min(e0 + (int64_t)HIST_EPB, L * L);
   const double *C = cov + d[D_COV];
-  unsigned long long *H = hist + d[D_GROUP] * NBMAX;
   const double fl = bmin + g.w;
-  for (int64_t e = e0 + threadIdx.x; e < min(e0 + ELPB, L * L); e += 256) {
-    int64_t i = e / L, j = e - i * L;
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+    const int64_t i = e / L, j = e - i * L;
     if (i >= j) continue;
-    double x = C[e];
-    double add = (x > fl) ? x : fl;
+    const double x = C[e];
+    const double add = (x > fl) ? x : fl;
     int b = (int)ceil(((add - bmin) / g.w) - 1.);
     b = (b < 0) ? 0 : ((b >= g.nb) ? g.nb - 1 : b);
-    atomicAdd(&H[b], 1ull);
+    atomicAdd(&lh[b], 1u);
   }
+  __syncthreads();
+  unsigned long long *H = hist + d[D_GROUP] * NBMAX;
+  for (int b = threadIdx.x; b < g.nb; b += 256)
+    if (lh[b]) atomicAdd(&H[b], (unsigned long long)lh[b]);
 }
 
 // tail fit per null (one 1024-thread block per group): see rsg_rank.hip for the method
@@ -266,20 +277,17 @@ __global__ void k_logtab_b(int64_t NM, double *__restrict__ t)
 
 // ------------------------------------------------------------------------------- host side
 namespace {
-struct Dev {
-  std::vector<void *> ptrs;
-  template <typename T> T *get(size_t count, char *errbuf, int *rc)
+// Bump allocation over a base pointer (256-byte aligned slices); with base == nullptr it only
+// measures.
+struct Arena {
+  uint8_t *base = nullptr;
+  size_t off = 0;
+  template <typename T> T *get(size_t count)
   {
-    void *p = nullptr;
-    if (*rc != RSG_OK) return nullptr;
-    if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
-      *rc = rsg_fail(errbuf, "batch: out of device memory (%zu bytes)", count * sizeof(T));
-      return nullptr;
-    }
-    ptrs.push_back(p);
-    return (T *)p;
+    T *p = base ? (T *)(base + off) : nullptr;
+    off += (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
+    return p;
   }
-  ~Dev() { for (void *p : ptrs) hipFree(p); }
 };
 }
 
@@ -307,8 +315,8 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   std::vector<SimItem> sim((size_t)nj);
   std::vector<int> cls((size_t)nj);
   int64_t o_xt = 0, o_cnt = 0, o_cov = 0, o_pm = 0, o_rows = 0, o_leaf = 0, nodes = 0, seqs = 0;
-  int64_t b_pack = 0, b_col = 0, b_stat = 0, b_el = 0, b_sim = 0, nmax = 1;
-  std::vector<int4> tiles;
+  int64_t b_pack = 0, b_col = 0, b_stat = 0, b_el = 0, b_sim = 0, b_hist = 0, nmax = 1;
+  std::vector<int4> tiles, btiles;        // 32-column tiles (small alignments), 64-column tiles (large)
   for (int k = 0; k < nj; k++) {
     const rsg_null_job &J = jobs[k];
     if (J.nleaf < 2 || J.alen < 2) return rsg_fail(errbuf, "job %d: needs >= 2 sequences and >= 2 columns", k);
@@ -319,24 +327,32 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     d[D_HIST] = (int64_t)k * NBMAX; d[D_LEAF] = o_leaf;
     d[D_BPACK] = b_pack; d[D_BCOL] = b_col; d[D_BSTAT] = b_stat; d[D_BEL] = b_el;
     const int64_t nb32 = (L + 31) / 32;
-    d[D_NB32] = nb32; d[D_GROUP] = J.group;
+    d[D_NB32] = nb32; d[D_GROUP] = J.group; d[D_BHIST] = b_hist;
     cls[(size_t)k] = (N <= 8 || L <= 50) ? RSG_C2 : RSG_C16;           // CSELECT (correlators.c:335-341)
     SimItem &s = sim[(size_t)k];
     s.N = N; s.L = L; s.tree_off = nodes; s.seq_off = seqs; s.leaf_off = o_leaf; s.seed = J.seed; s.stream = J.stream;
     s.gapthr = (J.gapfrac <= 0.) ? 0u : (J.gapfrac >= 1. ? 0xFFFFFFFFu : (uint32_t)(J.gapfrac * 4294967296.0));
     s.col_blk0 = b_sim; s.model_off = 40 * (int64_t)k;
-    for (int64_t bi = 0; bi < nb32; bi++)
-      for (int64_t bj = bi; bj < nb32; bj++) tiles.push_back(make_int4(k, (int)bi, (int)bj, 0));
+    if (rsg_batch_big_tiles(N, L)) {
+      const int64_t nb64 = Lpad / 64;
+      for (int64_t bi = 0; bi < nb64; bi++)
+        for (int64_t bj = bi; bj < nb64; bj++) btiles.push_back(make_int4(k, (int)bi, (int)bj, 0));
+    } else {
+      for (int64_t bi = 0; bi < nb32; bi++)
+        for (int64_t bj = bi; bj < nb32; bj++) tiles.push_back(make_int4(k, (int)bi, (int)bj, 0));
+    }
     o_xt += Lpad * Npad; o_cnt += 16 * L * L; o_cov += L * L; o_pm += 8 * L; o_rows += 2 * L + 2; o_leaf += N * L;
     nodes += N - 1; seqs += (N - 1) * L;
     b_pack += (Lpad / 64) * (Npad / 64); b_col += L; b_stat += nb32 * nb32; b_el += (L * L + ELPB - 1) / ELPB;
-    b_sim += (L + 63) / 64;
+    b_sim += (L + 63) / 64; b_hist += (L * L + HIST_EPB - 1) / HIST_EPB;
     nmax = std::max(nmax, N);
   }
-  // longest K loops first so the tail of the launch is short work
-  std::stable_sort(tiles.begin(), tiles.end(), [&](const int4 &a, const int4 &b) {
+  // longest K loops first so the tail of each launch is short work
+  auto by_k = [&](const int4 &a, const int4 &b) {
     return desc[(size_t)a.x * RSG_BDESC + D_NPAD] > desc[(size_t)b.x * RSG_BDESC + D_NPAD];
-  });
+  };
+  std::stable_sort(tiles.begin(), tiles.end(), by_k);
+  std::stable_sort(btiles.begin(), btiles.end(), by_k);
   std::vector<int> left((size_t)nodes), right((size_t)nodes);
   std::vector<double> ld((size_t)nodes), rd((size_t)nodes), models((size_t)nj * 40);
   for (int k = 0; k < nj; k++) {
@@ -350,50 +366,54 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   SimLevels lv;
   if (rsg_sim_levels(sim.data(), nj, left.data(), right.data(), lv, errbuf) != RSG_OK) return RSG_FAIL;
   // surrogate features of each group's first replicate
-  std::vector<int64_t> fitems((size_t)ng * 4);
+  std::vector<int64_t> fitems((size_t)ng * 5);
   int64_t fblocks = 0;
   for (int g = 0; g < ng; g++) {
-    const SimItem &s0 = sim[(size_t)gfirst[(size_t)g]];
-    fitems[(size_t)g * 4 + 0] = s0.leaf_off; fitems[(size_t)g * 4 + 1] = s0.N; fitems[(size_t)g * 4 + 2] = s0.L;
-    fitems[(size_t)g * 4 + 3] = fblocks;
-    fblocks += (s0.L + 255) / 256;
+    const int64_t *d0 = &desc[(size_t)gfirst[(size_t)g] * RSG_BDESC];
+    int64_t *f = &fitems[(size_t)g * 5];
+    f[0] = d0[D_XT]; f[1] = d0[D_N]; f[2] = d0[D_NPAD]; f[3] = d0[D_L]; f[4] = fblocks;
+    fblocks += (d0[D_L] + 3) / 4;
   }
-  // ---- device buffers
-  int rc = RSG_OK;
-  Dev D;
-  int64_t *d_desc = D.get<int64_t>(desc.size(), errbuf, &rc);
-  SimItem *d_sim = D.get<SimItem>((size_t)nj, errbuf, &rc);
-  int *d_cls = D.get<int>((size_t)nj, errbuf, &rc);
-  int4 *d_tiles = D.get<int4>(tiles.size(), errbuf, &rc);
-  int *d_left = D.get<int>((size_t)nodes, errbuf, &rc), *d_right = D.get<int>((size_t)nodes, errbuf, &rc);
-  double *d_ld = D.get<double>((size_t)nodes, errbuf, &rc), *d_rd = D.get<double>((size_t)nodes, errbuf, &rc);
-  double *d_models = D.get<double>(40 * (size_t)nj, errbuf, &rc);
-  int4 *d_ent = D.get<int4>(lv.ent.size(), errbuf, &rc);
-  int *d_gfirst = D.get<int>((size_t)ng + 1, errbuf, &rc);
-  int64_t *d_fitems = D.get<int64_t>(fitems.size(), errbuf, &rc);
-  unsigned long long *d_fcnt = D.get<unsigned long long>((size_t)ng * 36, errbuf, &rc);
-  uint32_t *d_cdf = D.get<uint32_t>((size_t)nodes * 24, errbuf, &rc);
-  uint8_t *d_seqs = D.get<uint8_t>((size_t)seqs, errbuf, &rc);
-  uint8_t *d_leaves = D.get<uint8_t>((size_t)o_leaf, errbuf, &rc);
-  uint8_t *d_xt = D.get<uint8_t>((size_t)o_xt, errbuf, &rc);
-  uint32_t *d_cnt = D.get<uint32_t>((size_t)o_cnt, errbuf, &rc);
-  double *d_cov = D.get<double>((size_t)o_cov, errbuf, &rc);
-  double *d_pm = D.get<double>((size_t)o_pm, errbuf, &rc);
-  double *d_rows = D.get<double>((size_t)o_rows, errbuf, &rc);
-  double *d_lgc = D.get<double>(2 * (size_t)(nmax + 1), errbuf, &rc);
-  double *d_part_s = D.get<double>(2 * (size_t)b_stat, errbuf, &rc);
-  double *d_part_e = D.get<double>(2 * (size_t)b_el, errbuf, &rc);
-  double *d_mm = D.get<double>(2 * (size_t)nj, errbuf, &rc);
-  HGeo *d_geo = D.get<HGeo>((size_t)ng, errbuf, &rc);
-  unsigned long long *d_hist = D.get<unsigned long long>((size_t)ng * NBMAX, errbuf, &rc);
-  unsigned long long *d_S = D.get<unsigned long long>((size_t)ng * NBMAX, errbuf, &rc);
-  int *d_NZ = D.get<int>((size_t)ng * NBMAX, errbuf, &rc);
-  double *d_fits = D.get<double>((size_t)ng * 8, errbuf, &rc);
-  if (rc != RSG_OK) return rc;
+  // ---- device buffers: one grow-only arena of the context, carved in two passes
+  int64_t *d_desc; SimItem *d_sim; int *d_cls; int4 *d_tiles, *d_btiles; int *d_left, *d_right;
+  double *d_ld, *d_rd, *d_models; int4 *d_ent; int *d_gfirst; int64_t *d_fitems; unsigned long long *d_fcnt;
+  uint32_t *d_cdf; uint8_t *d_seqs, *d_leaves, *d_xt; uint32_t *d_cnt; double *d_cov, *d_pm, *d_rows, *d_lgc;
+  double *d_part_s, *d_part_e, *d_mm; HGeo *d_geo; unsigned long long *d_hist, *d_S; int *d_NZ; double *d_fits;
+  auto carve = [&](Arena &A) {
+    d_desc = A.get<int64_t>(desc.size()); d_sim = A.get<SimItem>((size_t)nj); d_cls = A.get<int>((size_t)nj);
+    d_tiles = A.get<int4>(tiles.size()); d_btiles = A.get<int4>(btiles.size());
+    d_left = A.get<int>((size_t)nodes); d_right = A.get<int>((size_t)nodes);
+    d_ld = A.get<double>((size_t)nodes); d_rd = A.get<double>((size_t)nodes);
+    d_models = A.get<double>(40 * (size_t)nj); d_ent = A.get<int4>(lv.ent.size());
+    d_gfirst = A.get<int>((size_t)ng + 1); d_fitems = A.get<int64_t>(fitems.size());
+    d_fcnt = A.get<unsigned long long>((size_t)ng * 36);
+    d_cdf = A.get<uint32_t>((size_t)nodes * 24); d_seqs = A.get<uint8_t>((size_t)seqs);
+    d_leaves = A.get<uint8_t>((size_t)o_leaf); d_xt = A.get<uint8_t>((size_t)o_xt);
+    d_cnt = A.get<uint32_t>((size_t)o_cnt); d_cov = A.get<double>((size_t)o_cov); d_pm = A.get<double>((size_t)o_pm);
+    d_rows = A.get<double>((size_t)o_rows); d_lgc = A.get<double>(2 * (size_t)(nmax + 1));
+    d_part_s = A.get<double>(2 * (size_t)b_stat); d_part_e = A.get<double>(2 * (size_t)b_el);
+    d_mm = A.get<double>(2 * (size_t)nj); d_geo = A.get<HGeo>((size_t)ng);
+    d_hist = A.get<unsigned long long>((size_t)ng * NBMAX); d_S = A.get<unsigned long long>((size_t)ng * NBMAX);
+    d_NZ = A.get<int>((size_t)ng * NBMAX); d_fits = A.get<double>((size_t)ng * 8);
+  };
+  {
+    Arena plan;
+    carve(plan);
+    if (c->arena_cap < plan.off) {
+      if (c->d_arena) hipFree(c->d_arena);
+      c->d_arena = nullptr; c->arena_cap = 0;
+      if (hipMalloc(&c->d_arena, plan.off) != hipSuccess)
+        return rsg_fail(errbuf, "batch: out of device memory (%zu bytes)", plan.off);
+      c->arena_cap = plan.off;
+    }
+    Arena A; A.base = (uint8_t *)c->d_arena;
+    carve(A);
+  }
   HIPCHK(hipMemcpyAsync(d_desc, desc.data(), sizeof(int64_t) * desc.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_sim, sim.data(), sizeof(SimItem) * nj, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_cls, cls.data(), sizeof(int) * nj, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, c->stream));
+  if (!tiles.empty()) HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, c->stream));
+  if (!btiles.empty()) HIPCHK(hipMemcpyAsync(d_btiles, btiles.data(), sizeof(int4) * btiles.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_left, left.data(), sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_right, right.data(), sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_ld, ld.data(), sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
@@ -417,8 +437,9 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     if (rsg_sim_launch(c, d_sim, nj, nodes, b_sim, lv, d_ent, d_left, d_right, d_ld, d_rd, d_models, d_cdf, d_seqs,
                        d_leaves) != RSG_OK) return RSG_FAIL;
     hipLaunchKernelGGL(k_pack_b, dim3((unsigned)b_pack), dim3(256), 0, c->stream, d_desc, nj, d_leaves, d_xt);
-    if (feats && rsg_feat_launch(c, d_fitems, ng, fblocks, d_leaves, d_fcnt) != RSG_OK) return RSG_FAIL;
-    if (rsg_gram_batch_launch(c, d_tiles, (int)tiles.size(), d_desc, d_xt, d_cnt) != RSG_OK) return RSG_FAIL;
+    if (feats && rsg_feat_launch(c, d_fitems, ng, fblocks, d_xt, d_fcnt) != RSG_OK) return RSG_FAIL;
+    if (rsg_gram_batch_launch(c, d_btiles, (int)btiles.size(), d_tiles, (int)tiles.size(), d_desc, d_xt, d_cnt) != RSG_OK)
+      return RSG_FAIL;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_flag, 16);
     hipLaunchKernelGGL(k_logtab_b, dim3((unsigned)((nmax + 256) / 256)), dim3(256), 0, c->stream, nmax, d_lgc);
     hipLaunchKernelGGL(k_pm_b, dim3((unsigned)b_col), dim3(256), 0, c->stream, d_desc, nj, d_cnt, d_pm, 1e-6, c->d_flag);
@@ -435,7 +456,7 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     hipLaunchKernelGGL(k_width_b, dim3((unsigned)((ng + 63) / 64)), dim3(64), 0, c->stream, ng, d_gfirst, d_desc, d_mm,
                        bmin, 0.05, 400, 1e-6, d_geo);
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)(((int64_t)ng * NBMAX + 255) / 256)), dim3(256), 0, c->stream, d_hist, (int64_t)ng * NBMAX);
-    hipLaunchKernelGGL(k_hist_b, dim3((unsigned)b_el), dim3(256), 0, c->stream, d_desc, nj, d_cov, d_geo, bmin, d_hist);
+    hipLaunchKernelGGL(k_hist_b, dim3((unsigned)b_hist), dim3(256), 0, c->stream, d_desc, nj, d_cov, d_geo, bmin, d_hist);
     hipLaunchKernelGGL(k_fit_b, dim3((unsigned)ng), dim3(1024), 0, c->stream, d_geo, d_hist, d_S, d_NZ,
                        bmin, pmass, fracfit, doexpfit, d_fits);
     return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;

@@ -20,7 +20,7 @@ extern "C" rsg_ctx *rsg_ctx_create(int device, char *errbuf)
   c->device = device;
   hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&c->d_flag, 64) != hipSuccess ||
+      hipMalloc(&c->d_flag, 64) != hipSuccess || hipMalloc(&c->d_small, 65536) != hipSuccess ||
       hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     rsg_fail(errbuf, "context allocation failed");
     delete c;
@@ -38,7 +38,9 @@ extern "C" void rsg_ctx_destroy(rsg_ctx *c)
     if (c->timers[k].a) { hipEventDestroy(c->timers[k].a); hipEventDestroy(c->timers[k].b); }
   if (c->d_red) hipFree(c->d_red);
   if (c->d_gscr) hipFree(c->d_gscr);
+  if (c->d_arena) hipFree(c->d_arena);
   if (c->d_flag) hipFree(c->d_flag);
+  if (c->d_small) hipFree(c->d_small);
   if (c->h_pinned) hipHostFree(c->h_pinned);
   hipStreamDestroy(c->stream);
   delete c;
@@ -92,7 +94,7 @@ extern "C" int rsg_ctx_set_gram_split(rsg_ctx *c, int on) { c->gram_split = on ?
 static void free_msa(rsg_msa *m)
 {
   void *ptrs[] = { m->d_ax, m->d_axT, m->d_xt, m->d_w, m->d_wsplit,
-                   m->d_cnt, m->d_cnt_u, m->d_pm, m->d_ps, m->d_lgc, m->d_cov, m->d_tmp, m->d_rowx };
+                   m->d_cnt, m->d_cnt_u, m->d_pm, m->d_ps, m->d_lgc, m->d_cov, m->d_tmp, m->d_rowx, m->d_eval };
   for (void *p : ptrs) if (p) hipFree(p);
   delete m;
 }
@@ -112,7 +114,7 @@ extern "C" int rsg_msa_update(rsg_ctx *c, rsg_msa *m, const uint8_t *ax, const d
   HIPCHK(hipMemcpyAsync(m->d_w, w.data(), sizeof(double) * (size_t)m->N, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   m->nsplit_ready = 0;
-  m->have_probs = m->have_cov = 0;
+  m->have_probs = m->have_cov = m->have_eval = 0;
   return RSG_OK;
 }
 
@@ -161,7 +163,7 @@ static int alloc_counts(rsg_msa *m, void **dst, int kind, char *errbuf)
 extern "C" int rsg_corr_Probs(rsg_ctx *c, rsg_msa *m, int mode, double tol, char *errbuf)
 {
   HIPCHK(hipSetDevice(c->device));
-  m->have_probs = m->have_cov = 0;
+  m->have_probs = m->have_cov = m->have_eval = 0;
   if (mode == RSG_COUNTS_INT8) {
     if (m->weighted) return rsg_fail(errbuf, "int8 counts need unit weights (alignment has weights)");
     if (alloc_counts(m, &m->d_cnt, CNT_I8G, errbuf) != RSG_OK) return RSG_FAIL;

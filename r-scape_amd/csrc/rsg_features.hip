@@ -12,15 +12,18 @@
 //              mean branch; mean internal branch / mean branch; mean root-to-leaf edge count /
 //              log2 N; cherries / (N/2); Colless imbalance / ((N-1)(N-2)/2)
 //
-// The column statistics are one pass over the N x L codes on the device (one thread per
-// column, coalesced across columns; block histograms in LDS, one atomic per bin per
-// block); the tree moments are O(N) on the host.
+// The column statistics are one pass over the one-hot Gram operand on the device (one wave
+// per column; block histograms in LDS, one atomic per bin per block); the tree moments are
+// O(N) on the host.
 #include "rsg_internal.h"
 #include <cmath>
+#include <cstring>
 #include <vector>
 
-// Per item descriptor {ax offset, N, L, first block}; 36 uint64 counters per item:
-// [0..15] entropy bins, [16..31] gap bins, [32..35] residue totals.
+// Per item descriptor {one-hot panel offset, N, Npad, L, first block}; the panel is the
+// Gram operand (Lpad x Npad bytes, 1 << symbol for canonical residues, 0 otherwise).  One
+// wave per column (4 per block) reads the column's Npad bytes 16 at a time; 36 uint64
+// counters per item: [0..15] entropy bins, [16..31] gap bins, [32..35] residue totals.
 __global__ void __launch_bounds__(256)
 k_colfeat(const int64_t *__restrict__ items, int nitems, const uint8_t *__restrict__ base,
           unsigned long long *__restrict__ out)
@@ -28,28 +31,38 @@ k_colfeat(const int64_t *__restrict__ items, int nitems, const uint8_t *__restri
   __shared__ unsigned int eh[16], gh[16];
   __shared__ unsigned long long comp[4];
   int lo = 0, hi = nitems - 1;
-  while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (items[mid * 4 + 3] <= (int64_t)blockIdx.x) lo = mid; else hi = mid - 1; }
-  const int64_t *it = items + lo * 4;
-  const uint8_t *ax = base + it[0];
-  const int64_t N = it[1], L = it[2];
-  const int64_t c = ((int64_t)blockIdx.x - it[3]) * 256 + threadIdx.x;
+  while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (items[mid * 5 + 4] <= (int64_t)blockIdx.x) lo = mid; else hi = mid - 1; }
+  const int64_t *it = items + lo * 5;
+  const int64_t N = it[1], Npad = it[2], L = it[3];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t c = ((int64_t)blockIdx.x - it[4]) * 4 + wave;
   if (threadIdx.x < 16) { eh[threadIdx.x] = 0; gh[threadIdx.x] = 0; }
   if (threadIdx.x < 4) comp[threadIdx.x] = 0;
   __syncthreads();
   if (c < L) {
+    const uint4 *col = (const uint4 *)(base + it[0] + c * Npad);
     unsigned int n[4] = {0, 0, 0, 0};
-    for (int64_t s = 0; s < N; s++) {
-      const uint8_t a = ax[s * L + c];
-      if (a < 4) n[a]++;
+    for (int64_t k = lane; k < Npad / 16; k += 64) {
+      const uint4 v = col[k];
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        const unsigned mk = 0x01010101u << x;
+        n[x] += __popc(v.x & mk) + __popc(v.y & mk) + __popc(v.z & mk) + __popc(v.w & mk);
+      }
     }
-    const unsigned int nc = n[0] + n[1] + n[2] + n[3];
-    double H = 0.;
+#pragma unroll
     for (int x = 0; x < 4; x++)
-      if (n[x]) { const double p = (double)n[x] / (double)nc; H -= p * log2(p); }
-    const double g = (double)(N - nc) / (double)N;
-    atomicAdd(&eh[min(15, (int)(H * 8.0))], 1u);
-    atomicAdd(&gh[min(15, (int)(g * 16.0))], 1u);
-    for (int x = 0; x < 4; x++) atomicAdd(&comp[x], (unsigned long long)n[x]);
+      for (int o = 32; o > 0; o >>= 1) n[x] += __shfl_down(n[x], o, 64);
+    if (lane == 0) {
+      const unsigned int nc = n[0] + n[1] + n[2] + n[3];
+      double H = 0.;
+      for (int x = 0; x < 4; x++)
+        if (n[x]) { const double p = (double)n[x] / (double)nc; H -= p * log2(p); }
+      const double g = (double)(N - nc) / (double)N;
+      atomicAdd(&eh[min(15, (int)(H * 8.0))], 1u);
+      atomicAdd(&gh[min(15, (int)(g * 16.0))], 1u);
+      for (int x = 0; x < 4; x++) atomicAdd(&comp[x], (unsigned long long)n[x]);
+    }
   }
   __syncthreads();
   unsigned long long *o = out + (int64_t)lo * 36;
@@ -151,20 +164,20 @@ extern "C" int rsg_null_features(rsg_ctx *c, rsg_msa *msa, const int *left, cons
   if (!c || !msa || !left || !right || !ld || !rd || !feat) return rsg_fail(errbuf, "rsg_null_features: bad arguments");
   if (msa->N < 2) return rsg_fail(errbuf, "rsg_null_features: needs >= 2 sequences");
   HIPCHK(hipSetDevice(c->device));
-  const int64_t nblk = (msa->L + 255) / 256;
-  const int64_t item[4] = {0, msa->N, msa->L, 0};
-  int64_t *d_item = nullptr; unsigned long long *d_out = nullptr;
-  HIPCHK(hipMalloc(&d_item, sizeof(item)));
-  if (hipMalloc(&d_out, 36 * sizeof(unsigned long long)) != hipSuccess) { hipFree(d_item); return rsg_fail(errbuf, "out of device memory"); }
+  const int64_t nblk = (msa->L + 3) / 4;
+  const int64_t item[5] = {0, msa->N, msa->Npad, msa->L, 0};
+  int64_t *d_item = (int64_t *)c->d_small;                      // 40 B of context scratch
+  unsigned long long *d_out = (unsigned long long *)c->d_small + 8;
+  unsigned long long *h = (unsigned long long *)c->h_pinned;   // 36 of the 64 pinned doubles
   unsigned long long cnt[36];
   int st = RSG_OK;
   if (hipMemcpyAsync(d_item, item, sizeof(item), hipMemcpyHostToDevice, c->stream) != hipSuccess) st = rsg_fail(errbuf, "upload failed");
   int t; rsg_tstart(c, "features", &t);
-  if (st == RSG_OK && rsg_feat_launch(c, d_item, 1, nblk, msa->d_ax, d_out) != RSG_OK) st = rsg_fail(errbuf, "feature launch failed");
+  if (st == RSG_OK && rsg_feat_launch(c, d_item, 1, nblk, msa->d_xt, d_out) != RSG_OK) st = rsg_fail(errbuf, "feature launch failed");
   rsg_tstop(c, t);
-  if (st == RSG_OK && (hipMemcpyAsync(cnt, d_out, sizeof(cnt), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+  if (st == RSG_OK && (hipMemcpyAsync(h, d_out, sizeof(cnt), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                        hipStreamSynchronize(c->stream) != hipSuccess)) st = rsg_fail(errbuf, "feature download failed");
-  hipFree(d_item); hipFree(d_out);
+  if (st == RSG_OK) memcpy(cnt, h, sizeof(cnt));
   if (st == RSG_OK) rsg_feat_finish(cnt, msa->N, msa->L, left, right, ld, rd, feat);
   return st;
 }

@@ -136,11 +136,12 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
   STAGE_GLOAD(St, pre, xt, Npad, i0, j0, kt0 * KT, tid);
   STAGE_SSTORE(St, pre, lds, tid);
   __syncthreads();
-  for (int64_t kt = kt0; kt < kt1; kt++) {
-    const int buf = (int)((kt - kt0) & 1);
+  const int nit = (int)(kt1 - kt0);
+  for (int it = 0; it < nit; it++) {
+    const int buf = it & 1;
     // straight-line prefetch (the last stage reloads itself, in bounds and unused), so the
     // prefetch registers are never conditionally live
-    const int64_t knext = ((kt + 1 < kt1) ? kt + 1 : kt) * KT;
+    const int64_t knext = (kt0 + ((it + 1 < nit) ? it + 1 : it)) * KT;
     STAGE_GLOAD(St, pre, xt, Npad, i0, j0, knext, tid);
     const uint8_t *base = lds + buf * St::BYTES;
     uint4 ra = *(const uint4 *)(base + offA);
@@ -171,16 +172,18 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
   }
 
   // epilogue: lane owns i column `col`; register r holds j = 8*(r/4) + 4*khalf + r%4.
+  // Both destinations are affine in (ii, jj): base + jj * sj + ii * 16 words.
   const int ii = wi * 32 + col;
   const int64_t i = i0 + ii;
+  const bool all = part != nullptr;
+  uint32_t *const base = all ? part : out + rsg_rec(i0, j0, L) * 16;
+  const int64_t sj = all ? (int64_t)BI * 16 : L * 16;
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     const int jj = wj * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
     const int64_t j = j0 + jj;
-    uint4 *p = nullptr;
-    if (part) p = (uint4 *)(part + ((int64_t)jj * BI + ii) * 16);
-    else if (i < j && j < L) p = (uint4 *)(out + rsg_rec(i, j, L) * 16);
-    if (p) {
+    if (all || (i < j && j < L)) {
+      uint4 *p = (uint4 *)(base + jj * sj + ii * 16);
 #pragma unroll
       for (int x = 0; x < 4; x++)
         p[x] = make_uint4(acc[x][0][r], acc[x][1][r], acc[x][2][r], acc[x][3][r]);
@@ -202,20 +205,22 @@ k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, uint32_t *__r
   __shared__ __attribute__((aligned(16))) uint8_t lds[2 * St::BYTES];
   const int g = blockIdx.x;
   const int64_t nkt = Npad / KT;
+  int t;
+  int64_t k0 = 0, k1 = nkt;
+  uint32_t *p = nullptr;
   if (g < 8 * per_xcd) {
     // XCD-aware mapping: consecutive tiles of the (bi, bj)-sorted list run on one XCD (one L2).
-    const int t = (g & 7) * per_xcd + (g >> 3);
+    t = (g & 7) * per_xcd + (g >> 3);
     if (t >= ndp) return;
-    const int2 tl = tiles[t];
-    gram_tile_i8<WI, WJ, KT>(xt, Npad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, 0, nkt, nullptr);
   } else {
-    const int u = g - 8 * per_xcd, t = ndp + u / nsplit, sg = u % nsplit;
+    const int u = g - 8 * per_xcd, sg = u % nsplit;
+    t = ndp + u / nsplit;
     if (t >= ntiles) return;
-    const int2 tl = tiles[t];
-    const int64_t k0 = nkt * sg / nsplit, k1 = nkt * (sg + 1) / nsplit;
-    gram_tile_i8<WI, WJ, KT>(xt, Npad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, k0, k1,
-                             part + (int64_t)u * (32 * WI) * (32 * WJ) * 16);
+    k0 = nkt * sg / nsplit; k1 = nkt * (sg + 1) / nsplit;
+    p = part + (int64_t)u * (32 * WI) * (32 * WJ) * 16;
   }
+  const int2 tl = tiles[t];
+  gram_tile_i8<WI, WJ, KT>(xt, Npad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, k0, k1, p);
 }
 
 // Sum the nsplit partial tiles of each split tile into its records (integer sums: exact).
@@ -242,25 +247,29 @@ k_gram_reduce(const uint32_t *__restrict__ part, const int2 *__restrict__ tiles,
   ((uint4 *)(out + rsg_rec(i, j, L) * 16))[quarter] = s;
 }
 
-// Batched int8 Gram over many alignments: one work group per (item, tile), one wave per
-// 32 x 32-column tile; tiles are pre-sorted by decreasing sequence count (cost).
-__global__ void __launch_bounds__(64)
+// Batched int8 Gram over many alignments: one work group per (item, tile); tiles are
+// pre-sorted by decreasing sequence count (cost).  Large alignments: 64 x 64 columns, 4
+// waves, 256-sequence K tiles; small ones: 32 x 32 columns, one wave, 64-sequence K tiles.
+template <int W, int KT>
+__global__ void __launch_bounds__(64 * W * W)
 k_gram_i8_batch(const int4 *__restrict__ tiles, const int64_t *__restrict__ desc, const uint8_t *__restrict__ xt,
                 uint32_t *__restrict__ cnt)
 {
-  typedef CodeStage<32, 32, 64, 64> St;
+  typedef CodeStage<32 * W, 32 * W, KT, 64 * W * W> St;
   __shared__ __attribute__((aligned(16))) uint8_t lds[2 * St::BYTES];
   const int4 tl = tiles[blockIdx.x];                    // (item, bi, bj, -)
   const int64_t *d = desc + (int64_t)tl.x * RSG_BDESC;  // item descriptor (see rsg_batch.hip)
-  gram_tile_i8<1, 1, 64>(xt + d[4], d[2], d[1], cnt + d[5], (int64_t)tl.y * 32, (int64_t)tl.z * 32, lds,
-                         0, d[2] / 64, nullptr);
+  gram_tile_i8<W, W, KT>(xt + d[4], d[2], d[1], cnt + d[5], (int64_t)tl.y * 32 * W, (int64_t)tl.z * 32 * W, lds,
+                         0, d[2] / KT, nullptr);
 }
 
-int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_tiles, int ntiles, const int64_t *d_desc, const uint8_t *d_xt,
-                          uint32_t *d_cnt)
+int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_big, int nbig, const int4 *d_small, int nsmall,
+                          const int64_t *d_desc, const uint8_t *d_xt, uint32_t *d_cnt)
 {
-  if (ntiles > 0)
-    hipLaunchKernelGGL(k_gram_i8_batch, dim3((unsigned)ntiles), dim3(64), 0, c->stream, d_tiles, d_desc, d_xt, d_cnt);
+  if (nbig > 0)
+    hipLaunchKernelGGL((k_gram_i8_batch<2, 256>), dim3((unsigned)nbig), dim3(256), 0, c->stream, d_big, d_desc, d_xt, d_cnt);
+  if (nsmall > 0)
+    hipLaunchKernelGGL((k_gram_i8_batch<1, 64>), dim3((unsigned)nsmall), dim3(64), 0, c->stream, d_small, d_desc, d_xt, d_cnt);
   return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
 }
 

@@ -52,10 +52,12 @@ struct rsg_ctx {
   int          gram_split = 1;          // split the last partial wave of Gram tiles along K
   int          ncu = 0;                 // compute units of the device
   void        *d_gscr = nullptr;  size_t gscr_cap = 0;   // Gram split partials
+  void        *d_arena = nullptr; size_t arena_cap = 0;  // batched-null buffers (grow-only)
   // small reusable device scratch (reductions)
   double      *d_red = nullptr;  size_t red_cap = 0;
   int         *d_flag = nullptr;
   double      *h_pinned = nullptr;      // 64 doubles of pinned host staging
+  void        *d_small = nullptr;       // 64 KB of device scratch for small per-call tables
 };
 
 struct rsg_msa {
@@ -82,6 +84,8 @@ struct rsg_msa {
   double  *d_cov = nullptr;     // L x L (FP64)
   double  *d_tmp = nullptr;     // L x L scratch (RAFS source)
   double  *d_rowx = nullptr;    // L + 1 (row means, then COVavg)
+  double  *d_eval = nullptr;    // L x L E-values (kept on the device for the hit list)
+  int      have_eval = 0;
   double   minCOV = 0., maxCOV = 0.;
   int      have_probs = 0, have_cov = 0;
 };
@@ -157,13 +161,17 @@ int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t 
 
 // Batched pipeline: per-item descriptor of RSG_BDESC int64 fields
 // {N, L, Npad, Lpad, off_xt, off_cnt, off_cov, off_pm, off_rows, off_hist, off_leaf, blk0_pack,
-//  blk0_col, blk0_stat, blk0_el, nb32, group, spare}
+//  blk0_col, blk0_stat, blk0_el, nb32, group, blk0_hist}
 #define RSG_BDESC 18
-// Surrogate features (rsg_features.hip): column statistics of items {ax offset, N, L, first
-// 256-column block} into 36 uint64 counters per item; the host turns them into 48 floats.
+// Surrogate features (rsg_features.hip): column statistics of items {one-hot panel offset, N,
+// Npad, L, first 4-column block} into 36 uint64 counters per item; the host turns them into
+// 48 floats.
 int rsg_feat_launch(rsg_ctx *c, const int64_t *d_items, int nitems, int64_t nblocks, const uint8_t *d_base,
                     unsigned long long *d_out);
 void rsg_feat_finish(const unsigned long long *cnt, int64_t N, int64_t L, const int *left, const int *right,
                      const double *ld, const double *rd, float *f);
-int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_tiles, int ntiles, const int64_t *d_desc, const uint8_t *d_xt,
-                          uint32_t *d_cnt);
+// Large alignments of a batch use 64 x 64-column tiles (4 waves, 256-sequence K tiles), the
+// others 32 x 32-column tiles (one wave, 64-sequence K tiles).
+static inline bool rsg_batch_big_tiles(int64_t N, int64_t L) { return L >= 256 && N >= 256; }
+int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_big, int nbig, const int4 *d_small, int nsmall,
+                          const int64_t *d_desc, const uint8_t *d_xt, uint32_t *d_cnt);

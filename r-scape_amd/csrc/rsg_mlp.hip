@@ -34,6 +34,7 @@ struct rsg_mlp {
   float   *d_p = nullptr, *d_m = nullptr, *d_v = nullptr, *d_g = nullptr;
   uint16_t *d_wb = nullptr;                // BF16 copy of the whole parameter vector
   float   *d_part = nullptr;  int64_t part_blocks = 0;
+  float   *d_io = nullptr;    size_t io_cap = 0;     // forward inputs + outputs, grown on demand
   int64_t  step = 0;                       // Adam step counter
   size_t   lds_bytes = 0;
   float   *d_scale = nullptr;                // input mean | input 1/std | output mean | output std
@@ -365,6 +366,7 @@ extern "C" void rsg_mlp_destroy(rsg_ctx *c, rsg_mlp *m)
   if (c) { hipSetDevice(c->device); hipStreamSynchronize(c->stream); }
   hipFree(m->d_p); hipFree(m->d_m); hipFree(m->d_v); hipFree(m->d_g); hipFree(m->d_wb); hipFree(m->d_scale);
   if (m->d_part) hipFree(m->d_part);
+  if (m->d_io) hipFree(m->d_io);
   delete m;
 }
 
@@ -459,10 +461,14 @@ extern "C" int rsg_mlp_forward(rsg_ctx *c, rsg_mlp *m, const float *x, int64_t B
   if (!c || !m || !x || !y || B < 0) return rsg_fail(errbuf, "rsg_mlp_forward: bad arguments");
   if (B == 0) return RSG_OK;
   HIPCHK(hipSetDevice(c->device));
-  float *d_x = nullptr, *d_y = nullptr;
   const size_t xb = sizeof(float) * (size_t)(B * m->dims[0]), yb = sizeof(float) * (size_t)(B * m->dims[m->nl]);
-  HIPCHK(hipMalloc(&d_x, xb));
-  if (hipMalloc(&d_y, yb) != hipSuccess) { hipFree(d_x); return rsg_fail(errbuf, "rsg_mlp_forward: out of device memory"); }
+  if (m->io_cap < xb + yb) {
+    if (m->d_io) hipFree(m->d_io);
+    m->d_io = nullptr; m->io_cap = 0;
+    HIPCHK(hipMalloc(&m->d_io, xb + yb));
+    m->io_cap = xb + yb;
+  }
+  float *d_x = m->d_io, *d_y = m->d_io + (size_t)(B * m->dims[0]);
   MlpArgs a = make_args(m);
   a.x = d_x; a.y = d_y; a.nrows = B; a.first = 0;
   int st = RSG_OK;
@@ -473,7 +479,6 @@ extern "C" int rsg_mlp_forward(rsg_ctx *c, rsg_mlp *m, const float *x, int64_t B
   if (st == RSG_OK && (hipMemcpyAsync(y, d_y, yb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                        hipStreamSynchronize(c->stream) != hipSuccess))
     st = rsg_fail(errbuf, "rsg_mlp_forward: download failed");
-  hipFree(d_x); hipFree(d_y);
   return st;
 }
 

@@ -291,37 +291,53 @@ __global__ void k_suffix(const unsigned long long *__restrict__ obs, int imin, i
   }
 }
 
+static int ensure_eval(rsg_ctx *c, rsg_msa *m, char *errbuf)
+{
+  if (!m->d_eval) HIPCHK(hipMalloc(&m->d_eval, sizeof(double) * (size_t)(m->L * m->L)));
+  (void)c;
+  return RSG_OK;
+}
+
 extern "C" int rsg_cov_Evalues(rsg_ctx *c, rsg_msa *m, const rsg_hist *null, const double *survfit, int64_t Nc,
                                double *Eval, char *errbuf)
 {
   if (!m->have_cov) return rsg_fail(errbuf, "no covariation matrix");
+  HIPCHK(hipSetDevice(c->device));
+  if (ensure_eval(c, m, errbuf) != RSG_OK) return RSG_FAIL;
   const int nb = null->nb;
   const int64_t L = m->L;
   unsigned long long *d_obs = nullptr, *d_S = nullptr;
-  double *d_sf = nullptr, *d_ev = nullptr;
-  HIPCHK(hipMalloc(&d_obs, sizeof(unsigned long long) * nb));
-  HIPCHK(hipMalloc(&d_S, sizeof(unsigned long long) * nb));
-  HIPCHK(hipMalloc(&d_ev, sizeof(double) * (size_t)(L * L)));
-  HIPCHK(hipMemcpyAsync(d_obs, null->obs, sizeof(uint64_t) * nb, hipMemcpyHostToDevice, c->stream));
-  if (survfit) {
-    HIPCHK(hipMalloc(&d_sf, sizeof(double) * 2 * (size_t)nb));
-    HIPCHK(hipMemcpyAsync(d_sf, survfit, sizeof(double) * 2 * (size_t)nb, hipMemcpyHostToDevice, c->stream));
+  double *d_sf = nullptr;
+  int st = RSG_OK;
+  if (hipMalloc(&d_obs, sizeof(unsigned long long) * nb) != hipSuccess || hipMalloc(&d_S, sizeof(unsigned long long) * nb) != hipSuccess ||
+      (survfit && hipMalloc(&d_sf, sizeof(double) * 2 * (size_t)nb) != hipSuccess))
+    st = rsg_fail(errbuf, "rsg_cov_Evalues: out of device memory");
+  if (st == RSG_OK && (hipMemcpyAsync(d_obs, null->obs, sizeof(uint64_t) * nb, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+                       (survfit && hipMemcpyAsync(d_sf, survfit, sizeof(double) * 2 * (size_t)nb, hipMemcpyHostToDevice, c->stream) != hipSuccess) ||
+                       hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream) != hipSuccess))
+    st = rsg_fail(errbuf, "rsg_cov_Evalues: upload failed");
+  if (st == RSG_OK) {
+    hipLaunchKernelGGL(k_suffix, dim3(1), dim3(64), 0, c->stream, d_obs, null->imin, null->imax, d_S);
+    EvArgs A;
+    A.S = d_S; A.sf = d_sf; A.bmin = null->bmin; A.w = null->w; A.phi = null->phi; A.xmax = null->xmax;
+    A.NcNull = (double)null->Nc; A.Nc = (double)Nc; A.nb = nb; A.imin = null->imin; A.imax = null->imax;
+    int t; rsg_tstart(c, "evalues", &t);
+    hipLaunchKernelGGL(k_evalues, dim3((unsigned)((L * L + 255) / 256)), dim3(256), 0, c->stream, m->d_cov, L, A, m->d_eval, c->d_flag);
+    rsg_tstop(c, t);
+    if (hipGetLastError() != hipSuccess) st = rsg_fail(errbuf, "rsg_cov_Evalues: launch failed");
   }
-  HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
-  hipLaunchKernelGGL(k_suffix, dim3(1), dim3(64), 0, c->stream, d_obs, null->imin, null->imax, d_S);
-  EvArgs A;
-  A.S = d_S; A.sf = d_sf; A.bmin = null->bmin; A.w = null->w; A.phi = null->phi; A.xmax = null->xmax;
-  A.NcNull = (double)null->Nc; A.Nc = (double)Nc; A.nb = nb; A.imin = null->imin; A.imax = null->imax;
-  int t; rsg_tstart(c, "evalues", &t);
-  hipLaunchKernelGGL(k_evalues, dim3((unsigned)((L * L + 255) / 256)), dim3(256), 0, c->stream, m->d_cov, L, A, d_ev, c->d_flag);
-  rsg_tstop(c, t);
-  HIPCHK(hipGetLastError());
-  if (Eval) HIPCHK(hipMemcpyAsync(Eval, d_ev, sizeof(double) * (size_t)(L * L), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  hipFree(d_obs); hipFree(d_S); hipFree(d_ev); if (d_sf) hipFree(d_sf);
-  if (*(int *)c->h_pinned) return rsg_fail(errbuf, "cannot find evalue for a covariation score");
-  return RSG_OK;
+  if (st == RSG_OK && Eval &&
+      hipMemcpyAsync(Eval, m->d_eval, sizeof(double) * (size_t)(L * L), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    st = rsg_fail(errbuf, "rsg_cov_Evalues: download failed");
+  if (st == RSG_OK && (hipMemcpyAsync(c->h_pinned, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                       hipStreamSynchronize(c->stream) != hipSuccess))
+    st = rsg_fail(errbuf, "rsg_cov_Evalues failed");
+  if (d_obs) hipFree(d_obs);
+  if (d_S) hipFree(d_S);
+  if (d_sf) hipFree(d_sf);
+  if (st == RSG_OK && *(int *)c->h_pinned) st = rsg_fail(errbuf, "cannot find evalue for a covariation score");
+  m->have_eval = (st == RSG_OK);
+  return st;
 }
 
 __global__ void k_evtail(const double *__restrict__ cov, int64_t L, double m, double lambda, double p0, double Nc,
@@ -338,23 +354,79 @@ __global__ void k_evtail(const double *__restrict__ cov, int64_t L, double m, do
 extern "C" int rsg_cov_EvaluesTail(rsg_ctx *c, rsg_msa *m, double mu, double lambda, double p0, int64_t Nc,
                                    double *Eval, char *errbuf)
 {
-  if (!c || !m || !Eval) return rsg_fail(errbuf, "rsg_cov_EvaluesTail: bad arguments");
+  if (!c || !m) return rsg_fail(errbuf, "rsg_cov_EvaluesTail: bad arguments");
   if (!m->have_cov) return rsg_fail(errbuf, "no covariation matrix");
   if (!(lambda > 0.) || !(p0 > 0.)) return rsg_fail(errbuf, "rsg_cov_EvaluesTail: needs lambda > 0 and p0 > 0");
   HIPCHK(hipSetDevice(c->device));
+  if (ensure_eval(c, m, errbuf) != RSG_OK) return RSG_FAIL;
   const int64_t L = m->L;
-  double *d_ev = nullptr;
-  HIPCHK(hipMalloc(&d_ev, sizeof(double) * (size_t)(L * L)));
   int t; rsg_tstart(c, "evalues_tail", &t);
   hipLaunchKernelGGL(k_evtail, dim3((unsigned)((L * L + 255) / 256)), dim3(256), 0, c->stream, m->d_cov, L, mu, lambda,
-                     p0, (double)Nc, d_ev);
+                     p0, (double)Nc, m->d_eval);
   rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  if (Eval) HIPCHK(hipMemcpyAsync(Eval, m->d_eval, sizeof(double) * (size_t)(L * L), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  m->have_eval = 1;
+  return RSG_OK;
+}
+
+// Pairs i < j with E <= thresh, compacted in the order of an atomic counter (the host sorts).
+__global__ void k_hits(const double *__restrict__ ev, const double *__restrict__ cov, int64_t L, double thresh,
+                       int cap, int *__restrict__ count, int2 *__restrict__ ij, double2 *__restrict__ se)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= L * L) return;
+  const int64_t i = e / L, j = e - i * L;
+  if (i >= j || !(ev[e] <= thresh)) return;
+  const int k = atomicAdd(count, 1);
+  if (k < cap) { ij[k] = make_int2((int)i, (int)j); se[k] = make_double2(cov[e], ev[e]); }
+}
+
+extern "C" int rsg_cov_Hits(rsg_ctx *c, rsg_msa *m, double thresh, int cap, int *nhits, int *hi, int *hj,
+                            double *score, double *evalue, char *errbuf)
+{
+  if (!c || !m || !nhits || cap < 0 || (cap > 0 && (!hi || !hj || !score || !evalue)))
+    return rsg_fail(errbuf, "rsg_cov_Hits: bad arguments");
+  if (!m->have_eval) return rsg_fail(errbuf, "rsg_cov_Hits: no E-values (run rsg_cov_Evalues or rsg_cov_EvaluesTail)");
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t L = m->L;
+  const int dcap = std::max(cap, 1);
+  int2 *d_ij = nullptr; double2 *d_se = nullptr;
+  HIPCHK(hipMalloc(&d_ij, sizeof(int2) * (size_t)dcap));
+  if (hipMalloc(&d_se, sizeof(double2) * (size_t)dcap) != hipSuccess) { hipFree(d_ij); return rsg_fail(errbuf, "rsg_cov_Hits: out of device memory"); }
   int st = RSG_OK;
-  if (hipGetLastError() != hipSuccess ||
-      hipMemcpyAsync(Eval, d_ev, sizeof(double) * (size_t)(L * L), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipStreamSynchronize(c->stream) != hipSuccess) st = rsg_fail(errbuf, "rsg_cov_EvaluesTail failed");
-  hipFree(d_ev);
-  return st;
+  int *d_count = (int *)c->d_flag + 8;
+  std::vector<int2> ij; std::vector<double2> se;
+  if (hipMemsetAsync(d_count, 0, sizeof(int), c->stream) != hipSuccess) st = rsg_fail(errbuf, "rsg_cov_Hits: memset failed");
+  if (st == RSG_OK) {
+    hipLaunchKernelGGL(k_hits, dim3((unsigned)((L * L + 255) / 256)), dim3(256), 0, c->stream, m->d_eval, m->d_cov, L, thresh,
+                       cap, d_count, d_ij, d_se);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(c->h_pinned, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) st = rsg_fail(errbuf, "rsg_cov_Hits failed");
+  }
+  int n = (st == RSG_OK) ? *(int *)c->h_pinned : 0;
+  const int got = std::min(n, cap);
+  if (st == RSG_OK && got > 0) {
+    ij.resize((size_t)got); se.resize((size_t)got);
+    if (hipMemcpy(ij.data(), d_ij, sizeof(int2) * got, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(se.data(), d_se, sizeof(double2) * got, hipMemcpyDeviceToHost) != hipSuccess) st = rsg_fail(errbuf, "rsg_cov_Hits: download failed");
+  }
+  hipFree(d_ij); hipFree(d_se);
+  if (st != RSG_OK) return st;
+  // the atomic order is arbitrary: sort by (i, j)
+  std::vector<int> ord((size_t)got);
+  for (int k = 0; k < got; k++) ord[(size_t)k] = k;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) {
+    return ij[(size_t)a].x != ij[(size_t)b].x ? ij[(size_t)a].x < ij[(size_t)b].x : ij[(size_t)a].y < ij[(size_t)b].y;
+  });
+  for (int k = 0; k < got; k++) {
+    const int q = ord[(size_t)k];
+    hi[k] = ij[(size_t)q].x; hj[k] = ij[(size_t)q].y; score[k] = se[(size_t)q].x; evalue[k] = se[(size_t)q].y;
+  }
+  *nhits = n;
+  return (n > cap) ? rsg_fail(errbuf, "rsg_cov_Hits: %d hits exceed the capacity %d (the first %d returned)", n, cap, cap) : RSG_OK;
 }
 
 // ---------------------------------------------------------------- scalar helpers (host)

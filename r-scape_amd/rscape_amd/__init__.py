@@ -337,15 +337,16 @@ class Msa:
         _check(self.ctx.L.rsg_cov_RankHist(self.ctx.h, self.h, bmin, w, tol, C.byref(p), err), err, "rsg_cov_RankHist")
         return Histogram(p)
 
-    def evalues(self, null, survfit, Nc=None):
-        """E-value of every pair against a fitted null (cov2evalue * Nc)."""
+    def evalues(self, null, survfit, Nc=None, matrix=True):
+        """E-value of every pair against a fitted null (cov2evalue * Nc); with matrix=False the
+        values stay on the device (for null.hits())."""
         L = self.Lc
         Nc = L * (L - 1) // 2 if Nc is None else Nc
-        out = np.zeros(L * L)
+        out = np.zeros(L * L) if matrix else None
         err = _err()
         sfp = None if survfit is None else survfit.ctypes.data_as(C.POINTER(C.c_double))
         _check(self.ctx.L.rsg_cov_Evalues(self.ctx.h, self.h, null.p, sfp, Nc, _p(out), err), err, "rsg_cov_Evalues")
-        return out.reshape(L, L)
+        return None if out is None else out.reshape(L, L)
 
     def calculate(self, covtype="GTp", covclass=CSELECT, counts=COUNTS_INT8, prec=STAT_FP64, allowpair=None,
                   nseqthresh=8, alenthresh=50, tol=1e-6):

@@ -39,6 +39,8 @@ def _declare():
     L.rsg_null_features.argtypes = [vp, vp, vp, vp, vp, vp, vp, cp]
     L.rsg_cov_EvaluesTail.restype = i32
     L.rsg_cov_EvaluesTail.argtypes = [vp, vp, d, d, d, i64, vp, cp]
+    L.rsg_cov_Hits.restype = i32
+    L.rsg_cov_Hits.argtypes = [vp, vp, d, i32, C.POINTER(i32), vp, vp, vp, vp, cp]
     L._null_declared = True
     return L
 
@@ -140,12 +142,25 @@ def features(ctx, msa, tree):
     return f
 
 
-def evalues_tail(ctx, msa, m, lam, p0, Nc=None):
-    """E-values of the resident scores from a parametric tail p0 exp(-lam (x - m))."""
+def evalues_tail(ctx, msa, m, lam, p0, Nc=None, matrix=True):
+    """E-values of the resident scores from a parametric tail p0 exp(-lam (x - m)); with
+    matrix=False they stay on the device (for hits())."""
     L = _declare()
     n = msa.Lc
-    ev = np.zeros((n, n))
+    ev = np.zeros((n, n)) if matrix else None
     err = _err()
     _check(L.rsg_cov_EvaluesTail(ctx.h, msa.h, m, lam, p0, n * (n - 1) // 2 if Nc is None else Nc, _p(ev), err),
            err, "rsg_cov_EvaluesTail")
     return ev
+
+
+def hits(ctx, msa, thresh=0.05, cap=1 << 20):
+    """Significant pairs of the last E-value call: arrays i, j, score, E (sorted by i, j)."""
+    L = _declare()
+    hi = np.zeros(cap, np.int32); hj = np.zeros(cap, np.int32)
+    sc = np.zeros(cap); ev = np.zeros(cap)
+    n = C.c_int(0)
+    err = _err()
+    _check(L.rsg_cov_Hits(ctx.h, msa.h, thresh, cap, C.byref(n), _p(hi), _p(hj), _p(sc), _p(ev), err), err, "rsg_cov_Hits")
+    k = n.value
+    return hi[:k], hj[:k], sc[:k], ev[:k]

@@ -140,7 +140,7 @@ class Surrogate:
 
 
 # ------------------------------------------------------------------ significance runs
-def full_null(ctx, tree, L, pi, gapfrac, K=20, seed=5, covtype="GTp", graph=1, pmass=5e-4):
+def full_null(ctx, tree, L, pi, gapfrac, K=20, seed=5, covtype="GTp", graph=0, pmass=5e-4):
     """The simulated null of one alignment: K replicates on its tree (one batched group).
     Returns (histogram, survfit, fit row)."""
     model = NL.gtr_model(pi)

@@ -136,7 +136,15 @@ def test_surrogate_beats_constant_tail(ctx):
     pm, pl = sur.predict(X[va])
     ok = np.isfinite(pm)
     assert ok.mean() > 0.7
-    rm = np.sqrt(np.mean((pm[ok] * pl[ok] - Y[va, 0][ok]) ** 2)); cm = np.sqrt(np.mean((Y[tr, 0].mean() - Y[va, 0][ok]) ** 2))
-    rl = np.sqrt(np.mean((np.log(pl[ok]) - Y[va, 1][ok]) ** 2)); cl = np.sqrt(np.mean((Y[tr, 1].mean() - Y[va, 1][ok]) ** 2))
-    print("rmse lambda*m %.3f (const %.3f)  rmse ln lambda %.3f (const %.3f)" % (rm, cm, rl, cl))
-    assert rm < 0.7 * cm and rl < 0.7 * cl
+    # log10 tail probability error 5 and 15 e-folds into each held-out tail
+    lt, ut = np.exp(Y[va, 1][ok]), Y[va, 0][ok]             # true lambda, lambda * m
+    ls, us = pl[ok], pm[ok] * pl[ok]
+    lc, uc = np.exp(Y[tr, 1].mean()), Y[tr, 0].mean()        # one tail for every alignment
+    errs, cerrs = [], []
+    for k in (5.0, 15.0):
+        x = (ut + k) / lt                                    # lambda_t (x - m_t) = k
+        errs.append(np.abs(ls * x - us - k) / np.log(10))
+        cerrs.append(np.abs(lc * x - uc - k) / np.log(10))
+    e, ce = np.median(np.concatenate(errs)), np.median(np.concatenate(cerrs))
+    print("median log10 tail error: network %.3f, constant tail %.3f" % (e, ce))
+    assert e < 0.5 * ce
