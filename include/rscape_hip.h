@@ -188,6 +188,14 @@ typedef struct {
   double   gapfrac;          /* i.i.d. gap probability per leaf symbol (0 = none) */
   int32_t  group;            /* rsg_null_batch: replicates of one null share a group id */
 } rsg_null_job;
+/* R-scape's null alignment of a given alignment (null_rscape, src/R-scape.c:1615-1724):
+ * Fitch ancestral states on the tree (src/msatree.c:173-229,1700-1931; gap is a state, other
+ * non-canonical codes enter as one random state), one shuffle of all columns, then per branch
+ * the original (old -> new) substitution counts replayed at random positions of the shuffled
+ * parent holding old (src/msamanip.c:1449-1530,1597-1780).  ax: nseq x alen codes; leaves
+ * receive the nseq x alen shuffled alignment (codes 0..4).  Draws: Philox keyed by seed. */
+int rsg_null_fitch(rsg_ctx *ctx, const uint8_t *ax, int64_t nseq, int64_t alen, const int *left, const int *right,
+                   uint64_t seed, uint8_t *leaves, char *errbuf);
 /* Simulate one alignment; leaves = nleaf x alen codes (the leaf rows, leaf order). */
 int rsg_null_simulate(rsg_ctx *ctx, const rsg_null_job *job, uint8_t *leaves, char *errbuf);
 /* Simulate and score a batch of alignments of any sizes (GT/MI/... + correction, unit

@@ -127,8 +127,23 @@ int ref_evolve_gtr(int64_t nleaf, int64_t L, const int *left, const int *right, 
  * Tree layout as ref_evolve_tree; allmsa rows follow the reference: leaves 0..N-1,
  * internal node v at N+v.  out receives the N shuffled leaves (N x L). */
 #define DIM 6   /* residues 0..3, gap 4, "set" flag 5 */
+int ref_fitch_null_rows(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
+                        uint64_t seed, uint8_t *all_out, uint8_t *sh_out);
+
 int ref_fitch_null(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
                    uint64_t seed, uint8_t *out)
+{
+  uint8_t *sh = malloc((size_t)((2 * N - 1) * L));
+  int st = ref_fitch_null_rows(ax, N, L, left, right, seed, NULL, sh);
+  memcpy(out, sh, (size_t)(N * L));
+  free(sh);
+  return st;
+}
+
+/* As ref_fitch_null, returning every row: all_out = the Fitch reconstruction (leaves 0..N-1,
+ * internal node v at N+v) before the shuffle, sh_out = the shuffled, replayed alignment. */
+int ref_fitch_null_rows(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
+                        uint64_t seed, uint8_t *all_out, uint8_t *sh_out)
 {
   int64_t nn = 2 * N - 1;
   uint8_t *all = malloc((size_t)(nn * L)), *sh = malloc((size_t)(nn * L));
@@ -220,7 +235,8 @@ int ref_fitch_null(const uint8_t *ax, int64_t N, int64_t L, const int *left, con
       }
     }
   }
-  memcpy(out, sh, (size_t)(N * L));
+  if (all_out) memcpy(all_out, all, (size_t)(nn * L));
+  memcpy(sh_out, sh, (size_t)(nn * L));
   free(all); free(sh); free(S); free(parent); free(perm); free(cols); free(cperm);
   return REF_OK;
 }

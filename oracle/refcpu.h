@@ -105,6 +105,8 @@ int ref_evolve_gtr(int64_t nleaf, int64_t L, const int *left, const int *right, 
                    uint8_t *leaves);
 int ref_fitch_null(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
                    uint64_t seed, uint8_t *out);
+int ref_fitch_null_rows(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
+                        uint64_t seed, uint8_t *all_out, uint8_t *sh_out);
 
 /* ---- surrogate MLP reference (FP64 forward / backward) ---- */
 int ref_mlp_forward(int nl, const int *dims, const double *const *W, const double *const *b,

@@ -281,6 +281,23 @@ __device__ T block_reduce(T v, T *sh, Op op)
   return r;
 }
 
+// Inclusive suffix sum over the threads of the block (v of threads tid..T-1), in LDS.
+__device__ long long suffix_scan(long long v, long long *sh)
+{
+  const int T = blockDim.x, tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < T; off <<= 1) {
+    const long long add = (tid + off < T) ? sh[tid + off] : 0;
+    __syncthreads();
+    sh[tid] += add;
+    __syncthreads();
+  }
+  const long long r = sh[tid];
+  __syncthreads();
+  return r;
+}
+
 // One work group.  S[b] = sum_{i >= b} obs[i] and NZ[b] = #nonzero bins >= b over [imin, imax]
 // (suffix sums), so the reference's downward scans become "largest bin satisfying a monotone
 // condition" reductions.
@@ -298,16 +315,10 @@ __device__ void nullfit_body(const unsigned long long *obs, int nb, int imin, in
   const int c0 = imin + tid * chunk, c1 = min(imin + (tid + 1) * chunk, imax + 1);
   unsigned long long csum = 0; long long cnz = 0;
   for (int b = c0; b < c1; b++) { csum += obs[b]; cnz += (obs[b] > 0); }
-  // exclusive suffix of chunk totals: sum over threads with larger index
-  shl[tid] = (long long)csum;
-  __syncthreads();
-  long long above = 0, abovenz = 0;
-  for (int k = tid + 1; k < T; k++) above += shl[k];
-  __syncthreads();
-  shl[tid] = cnz;
-  __syncthreads();
-  for (int k = tid + 1; k < T; k++) abovenz += shl[k];
-  __syncthreads();
+  // exclusive suffix of the chunk totals (sum over threads with a larger index): an
+  // inclusive Hillis-Steele scan from the top, then subtract the thread's own total
+  long long above = suffix_scan(csum, shl), abovenz = suffix_scan(cnz, shl);
+  above -= (long long)csum; abovenz -= cnz;
   unsigned long long run = (unsigned long long)above; long long runnz = abovenz;
   for (int b = c1 - 1; b >= c0; b--) { run += obs[b]; runnz += (obs[b] > 0); S[b] = run; NZ[b] = (int)runnz; }
   __syncthreads();

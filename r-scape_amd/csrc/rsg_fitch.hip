@@ -1,0 +1,261 @@
+// rsg_fitch.hip -- R-scape's own null alignment on the device: Fitch ancestral states,
+// one shuffle of all columns, per-branch substitution replay.
+//
+// Reference: null_rscape (src/R-scape.c:1615-1724) -> Tree_FitchAlgorithmAncenstral
+// (src/msatree.c:173-229, 1700-1931) -> msamanip_ShuffleTreeSubstitutions
+// (src/msamanip.c:1449-1530, 1597-1780).  Symbols 0..4 (gap = 4 is a state); a leaf with
+// another code enters the Fitch pass as one random state.  Draws are Philox4x32-10 keyed by
+// (seed, purpose stream, node, position, tag) in place of Easel's sequential generator, so
+// results match the reference in distribution and the oracle symbol for symbol.
+//
+// Rows: leaf n -> row n, internal node v -> row N + v (the reference's allmsa order).
+// Work: the Fitch passes and the replay run one launch per tree depth (children before
+// parents upward, parents before children downward); a column of a level is one thread in
+// the Fitch passes, a branch is one wave in the replay (ballot compaction of the parent's
+// positions holding each old symbol, then a partial Fisher-Yates on the positions that
+// receive substitutions).
+#include "rsg_internal.h"
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+__device__ __forceinline__ int row_of(int d, int64_t N) { return d > 0 ? (int)(N + d) : -d; }
+
+// leaf state sets: 1 << code for codes 0..4, else one random state
+__global__ void k_fitch_leaves(const uint8_t *__restrict__ ax, int64_t N, int64_t L, uint64_t seed,
+                               uint8_t *__restrict__ mask, uint8_t *__restrict__ all)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N * L) return;
+  const int64_t n = e / L, c = e - n * L;
+  const uint8_t a = ax[e];
+  int s = a;
+  if (a > 4) s = (int)(((uint64_t)rsg_philox(seed, 1u, (uint32_t)n, (uint32_t)c, 1u) * 5u) >> 32);
+  mask[e] = (uint8_t)(1u << s);
+  all[e] = a;
+}
+
+// upward pass over the internal nodes of one depth: S_v = S_l & S_r, or the residues of
+// S_l | S_r when the intersection is empty
+__global__ void k_fitch_up(const int *__restrict__ nodes, int nn, int64_t N, int64_t L, const int *__restrict__ left,
+                           const int *__restrict__ right, uint8_t *__restrict__ mask)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nn * L) return;
+  const int v = nodes[e / L];
+  const int64_t c = e % L;
+  const unsigned sl = mask[(int64_t)row_of(left[v], N) * L + c], sr = mask[(int64_t)row_of(right[v], N) * L + c];
+  const unsigned in = sl & sr;
+  mask[(N + v) * L + c] = (uint8_t)(in ? in : ((sl | sr) & 0xFu));
+}
+
+// downward pass: keep the parent's state when the set holds it, then a uniform member
+__global__ void k_fitch_down(const int *__restrict__ nodes, int nn, int64_t N, int64_t L, const int *__restrict__ parent,
+                             uint64_t seed, const uint8_t *__restrict__ mask, uint8_t *__restrict__ all)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nn * L) return;
+  const int v = nodes[e / L];
+  const int64_t c = e % L;
+  unsigned S = mask[(N + v) * L + c];
+  if (v > 0) {
+    const unsigned pa = all[(N + parent[v]) * L + c];
+    if ((S >> pa) & 1u) S = 1u << pa;
+  }
+  const unsigned m = __popc(S);
+  const uint32_t u = rsg_philox(seed, 2u, (uint32_t)v, (uint32_t)c, 2u);
+  unsigned idx = (unsigned)(((uint64_t)u * m) >> 32), st = 0;
+  for (unsigned k = 0; k < 5; k++)
+    if ((S >> k) & 1u) { if (idx == 0) { st = k; break; } idx--; }
+  all[(N + v) * L + c] = (uint8_t)st;
+}
+
+// substitution counts of every branch b = 2 v + side: nsub[b][old * 5 + new], old != new
+__global__ void __launch_bounds__(64) k_fitch_subs(int64_t N, int64_t L, const int *__restrict__ left,
+                                                   const int *__restrict__ right, const uint8_t *__restrict__ all,
+                                                   int *__restrict__ nsub)
+{
+  const int b = blockIdx.x, v = b >> 1, side = b & 1;
+  const int d = side ? right[v] : left[v];
+  const uint8_t *pa = all + (N + v) * L, *ch = all + (int64_t)row_of(d, N) * L;
+  int cnt[25];
+#pragma unroll
+  for (int k = 0; k < 25; k++) cnt[k] = 0;
+  for (int64_t c = threadIdx.x; c < L; c += 64) {
+    const int a = pa[c], x = ch[c];
+    if (a != x && a <= 4 && x <= 4) {
+#pragma unroll
+      for (int k = 0; k < 25; k++) cnt[k] += (k == a * 5 + x);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 25; k++) {
+    int s = cnt[k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (threadIdx.x == 0) nsub[b * 25 + k] = s;
+  }
+}
+
+__global__ void k_gather_root(const uint8_t *__restrict__ all, const int *__restrict__ perm, int64_t N, int64_t L,
+                              uint8_t *__restrict__ sh)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < L) sh[N * L + c] = all[N * L + perm[c]];
+}
+
+// Orders a wave's global stores before its later loads (lanes exchange data through memory;
+// the waves of a block work on different branches and never wait for each other).
+__device__ __forceinline__ void wave_sync()
+{
+  __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Replay of the branches below the internal nodes of one depth: one wave per branch.
+// cols: scratch of L ints per wave slot of the launch.
+__global__ void __launch_bounds__(256) k_fitch_replay(const int *__restrict__ nodes, int nn, int64_t N, int64_t L,
+                                                      const int *__restrict__ left, const int *__restrict__ right,
+                                                      uint64_t seed, const int *__restrict__ nsub,
+                                                      uint8_t *__restrict__ sh, int *__restrict__ cols_all)
+{
+  const int lane = threadIdx.x & 63;
+  const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);        // branch slot in this level
+  if (slot >= 2 * nn) return;
+  const int v = nodes[slot >> 1], side = slot & 1;
+  const int d = side ? right[v] : left[v];
+  const int chrow = row_of(d, N);
+  const int b = 2 * v + side;
+  const uint8_t *pa = sh + (N + v) * L;
+  uint8_t *ch = sh + (int64_t)chrow * L;
+  int *cols = cols_all + (int64_t)slot * L;
+  for (int64_t c = lane; c < L; c += 64) ch[c] = pa[c];
+  wave_sync();
+  for (int oldc = 0; oldc < 5; oldc++) {
+    int nst = 0;
+    for (int nw = 0; nw < 5; nw++) nst += nsub[b * 25 + oldc * 5 + nw];
+    if (!nst) continue;
+    // positions of the parent holding oldc, ascending (ballot compaction)
+    int ncol = 0;
+    for (int64_t c0 = 0; c0 < L; c0 += 64) {
+      const int64_t c = c0 + lane;
+      const bool hit = c < L && pa[c] == oldc;
+      const unsigned long long bal = __ballot(hit);
+      if (hit) cols[ncol + __popcll(bal & ((1ull << lane) - 1ull))] = (int)c;
+      ncol += __popcll(bal);
+    }
+    wave_sync();                                              // cols visible to lane 0
+    if (lane == 0 && ncol > 0) {
+      // partial Fisher-Yates: iteration n fixes position n - 1; only the last min(nst, ncol)
+      // positions receive substitutions
+      const int take = nst < ncol ? nst : ncol;
+      const int nlow = (ncol - take + 1 > 2) ? ncol - take + 1 : 2;
+      for (int n = ncol; n >= nlow; n--) {
+        const uint32_t u = rsg_philox(seed, 4u, (uint32_t)(chrow * 5 + oldc), (uint32_t)n, 4u);
+        const int k = (int)(((uint64_t)u * (uint64_t)n) >> 32);
+        const int t = cols[n - 1]; cols[n - 1] = cols[k]; cols[k] = t;
+      }
+      int idx = ncol - 1;
+      for (int nw = 0; nw < 5; nw++) {
+        int s = nsub[b * 25 + oldc * 5 + nw];
+        while (s > 0 && idx >= 0) { ch[cols[idx]] = (uint8_t)nw; idx--; s--; }
+      }
+    }
+    wave_sync();
+  }
+}
+
+struct Buf {
+  std::vector<void *> p;
+  ~Buf() { for (void *q : p) hipFree(q); }
+  template <typename T> bool get(T **o, size_t n)
+  {
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return false;
+    p.push_back(q); *o = (T *)q;
+    return true;
+  }
+};
+
+} // namespace
+
+extern "C" int rsg_null_fitch(rsg_ctx *c, const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
+                              uint64_t seed, uint8_t *leaves, char *errbuf)
+{
+  if (!c || !ax || !left || !right || !leaves || N < 2 || L < 1) return rsg_fail(errbuf, "rsg_null_fitch: bad arguments");
+  if (N > (1 << 28) || L > (1 << 30)) return rsg_fail(errbuf, "rsg_null_fitch: alignment too large");
+  HIPCHK(hipSetDevice(c->device));
+  const int m = (int)(N - 1);
+  // depths of the internal nodes (walk from the root; malformed trees rejected), parents
+  std::vector<int> depth((size_t)m, -1), parent((size_t)m, 0), stack(1, 0);
+  depth[0] = 0;
+  int seen = 1, maxd = 0;
+  while (!stack.empty()) {
+    const int v = stack.back(); stack.pop_back();
+    for (int side = 0; side < 2; side++) {
+      const int d = side ? right[v] : left[v];
+      if (d > 0) {
+        if (d >= m || depth[(size_t)d] >= 0) return rsg_fail(errbuf, "rsg_null_fitch: malformed tree at node %d", v);
+        depth[(size_t)d] = depth[(size_t)v] + 1; parent[(size_t)d] = v; maxd = std::max(maxd, depth[(size_t)d]);
+        stack.push_back(d); seen++;
+      } else if (-d >= N) return rsg_fail(errbuf, "rsg_null_fitch: leaf index %d out of range", -d);
+    }
+  }
+  if (seen != m) return rsg_fail(errbuf, "rsg_null_fitch: %d internal nodes unreachable from the root", m - seen);
+  std::vector<int> lev_nodes, lev0((size_t)maxd + 2, 0);
+  for (int v = 0; v < m; v++) lev0[(size_t)depth[(size_t)v] + 1]++;
+  for (int k = 0; k <= maxd; k++) lev0[(size_t)k + 1] += lev0[(size_t)k];
+  lev_nodes.resize((size_t)m);
+  {
+    std::vector<int> fill(lev0.begin(), lev0.end() - 1);
+    for (int v = 0; v < m; v++) lev_nodes[(size_t)fill[(size_t)depth[(size_t)v]]++] = v;
+  }
+  int maxw = 0;
+  for (int k = 0; k <= maxd; k++) maxw = std::max(maxw, lev0[(size_t)k + 1] - lev0[(size_t)k]);
+  // one permutation of all columns (Fisher-Yates as esl_vec_IShuffle)
+  std::vector<int> perm((size_t)L);
+  for (int64_t k = 0; k < L; k++) perm[(size_t)k] = (int)k;
+  for (int64_t n = L; n > 1; n--) {
+    const uint32_t u = rsg_philox(seed, 3u, 0u, (uint32_t)n, 3u);
+    const int64_t k = (int64_t)(((uint64_t)u * (uint64_t)n) >> 32);
+    std::swap(perm[(size_t)(n - 1)], perm[(size_t)k]);
+  }
+  const int64_t nrow = 2 * N - 1;
+  Buf B;
+  uint8_t *d_ax, *d_mask, *d_all, *d_sh; int *d_left, *d_right, *d_parent, *d_nodes, *d_perm, *d_nsub, *d_cols;
+  if (!B.get(&d_ax, (size_t)(N * L)) || !B.get(&d_mask, (size_t)(nrow * L)) || !B.get(&d_all, (size_t)(nrow * L)) ||
+      !B.get(&d_sh, (size_t)(nrow * L)) || !B.get(&d_left, (size_t)m) || !B.get(&d_right, (size_t)m) ||
+      !B.get(&d_parent, (size_t)m) || !B.get(&d_nodes, (size_t)m) || !B.get(&d_perm, (size_t)L) ||
+      !B.get(&d_nsub, (size_t)(2 * m) * 25) || !B.get(&d_cols, (size_t)(2 * maxw) * (size_t)L))
+    return rsg_fail(errbuf, "rsg_null_fitch: out of device memory");
+  HIPCHK(hipMemcpyAsync(d_ax, ax, (size_t)(N * L), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_left, left, sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_right, right, sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_parent, parent.data(), sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_nodes, lev_nodes.data(), sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * L, hipMemcpyHostToDevice, c->stream));
+  int t; rsg_tstart(c, "fitch_null", &t);
+  hipLaunchKernelGGL(k_fitch_leaves, dim3((unsigned)((N * L + 255) / 256)), dim3(256), 0, c->stream, d_ax, N, L, seed, d_mask, d_all);
+  for (int k = maxd; k >= 0; k--) {
+    const int nn = lev0[(size_t)k + 1] - lev0[(size_t)k];
+    hipLaunchKernelGGL(k_fitch_up, dim3((unsigned)(((int64_t)nn * L + 255) / 256)), dim3(256), 0, c->stream,
+                       d_nodes + lev0[(size_t)k], nn, N, L, d_left, d_right, d_mask);
+  }
+  for (int k = 0; k <= maxd; k++) {
+    const int nn = lev0[(size_t)k + 1] - lev0[(size_t)k];
+    hipLaunchKernelGGL(k_fitch_down, dim3((unsigned)(((int64_t)nn * L + 255) / 256)), dim3(256), 0, c->stream,
+                       d_nodes + lev0[(size_t)k], nn, N, L, d_parent, seed, d_mask, d_all);
+  }
+  hipLaunchKernelGGL(k_fitch_subs, dim3((unsigned)(2 * m)), dim3(64), 0, c->stream, N, L, d_left, d_right, d_all, d_nsub);
+  hipLaunchKernelGGL(k_gather_root, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, c->stream, d_all, d_perm, N, L, d_sh);
+  for (int k = 0; k <= maxd; k++) {
+    const int nn = lev0[(size_t)k + 1] - lev0[(size_t)k];
+    hipLaunchKernelGGL(k_fitch_replay, dim3((unsigned)((2 * nn + 3) / 4)), dim3(256), 0, c->stream,
+                       d_nodes + lev0[(size_t)k], nn, N, L, d_left, d_right, seed, d_nsub, d_sh, d_cols);
+  }
+  rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(leaves, d_sh, (size_t)(N * L), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return RSG_OK;
+}

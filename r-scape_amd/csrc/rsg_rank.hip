@@ -283,14 +283,6 @@ __global__ void k_evalues(const double *__restrict__ cov, int64_t L, EvArgs A, d
   ev[idx] = p * A.Nc;
 }
 
-__global__ void k_suffix(const unsigned long long *__restrict__ obs, int imin, int imax, unsigned long long *__restrict__ S)
-{
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    unsigned long long run = 0;
-    for (int b = imax; b >= imin; b--) { run += obs[b]; S[b] = run; }
-  }
-}
-
 static int ensure_eval(rsg_ctx *c, rsg_msa *m, char *errbuf)
 {
   if (!m->d_eval) HIPCHK(hipMalloc(&m->d_eval, sizeof(double) * (size_t)(m->L * m->L)));
@@ -306,18 +298,21 @@ extern "C" int rsg_cov_Evalues(rsg_ctx *c, rsg_msa *m, const rsg_hist *null, con
   if (ensure_eval(c, m, errbuf) != RSG_OK) return RSG_FAIL;
   const int nb = null->nb;
   const int64_t L = m->L;
-  unsigned long long *d_obs = nullptr, *d_S = nullptr;
+  unsigned long long *d_S = nullptr;
   double *d_sf = nullptr;
   int st = RSG_OK;
-  if (hipMalloc(&d_obs, sizeof(unsigned long long) * nb) != hipSuccess || hipMalloc(&d_S, sizeof(unsigned long long) * nb) != hipSuccess ||
+  // tail counts S[b] = sum_{b' >= b} obs[b'] over [imin, imax] (host: the histogram is here)
+  std::vector<unsigned long long> S((size_t)std::max(nb, 1), 0ull);
+  unsigned long long run = 0;
+  for (int b = null->imax; b >= null->imin && b >= 0; b--) { if (b < nb) { run += null->obs[b]; S[(size_t)b] = run; } }
+  if (hipMalloc(&d_S, sizeof(unsigned long long) * S.size()) != hipSuccess ||
       (survfit && hipMalloc(&d_sf, sizeof(double) * 2 * (size_t)nb) != hipSuccess))
     st = rsg_fail(errbuf, "rsg_cov_Evalues: out of device memory");
-  if (st == RSG_OK && (hipMemcpyAsync(d_obs, null->obs, sizeof(uint64_t) * nb, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+  if (st == RSG_OK && (hipMemcpyAsync(d_S, S.data(), sizeof(unsigned long long) * S.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
                        (survfit && hipMemcpyAsync(d_sf, survfit, sizeof(double) * 2 * (size_t)nb, hipMemcpyHostToDevice, c->stream) != hipSuccess) ||
                        hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream) != hipSuccess))
     st = rsg_fail(errbuf, "rsg_cov_Evalues: upload failed");
   if (st == RSG_OK) {
-    hipLaunchKernelGGL(k_suffix, dim3(1), dim3(64), 0, c->stream, d_obs, null->imin, null->imax, d_S);
     EvArgs A;
     A.S = d_S; A.sf = d_sf; A.bmin = null->bmin; A.w = null->w; A.phi = null->phi; A.xmax = null->xmax;
     A.NcNull = (double)null->Nc; A.Nc = (double)Nc; A.nb = nb; A.imin = null->imin; A.imax = null->imax;
@@ -332,7 +327,6 @@ extern "C" int rsg_cov_Evalues(rsg_ctx *c, rsg_msa *m, const rsg_hist *null, con
   if (st == RSG_OK && (hipMemcpyAsync(c->h_pinned, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                        hipStreamSynchronize(c->stream) != hipSuccess))
     st = rsg_fail(errbuf, "rsg_cov_Evalues failed");
-  if (d_obs) hipFree(d_obs);
   if (d_S) hipFree(d_S);
   if (d_sf) hipFree(d_sf);
   if (st == RSG_OK && *(int *)c->h_pinned) st = rsg_fail(errbuf, "cannot find evalue for a covariation score");

@@ -33,6 +33,8 @@ def _declare():
     L.rsg_tree_yule.restype = i32
     L.rsg_tree_yule.argtypes = [i64, d, C.c_uint64, C.c_uint32, vp, vp, vp, vp]
     L.rsg_null_simulate.restype, L.rsg_null_simulate.argtypes = i32, [vp, C.POINTER(NullJob), vp, cp]
+    L.rsg_null_fitch.restype = i32
+    L.rsg_null_fitch.argtypes = [vp, vp, i64, i64, vp, vp, C.c_uint64, vp, cp]
     L.rsg_null_batch.restype = i32
     L.rsg_null_batch.argtypes = [vp, C.POINTER(NullJob), i32, i32, d, d, d, i32, vp, vp, vp, i32, cp]
     L.rsg_null_features.restype = i32
@@ -101,6 +103,19 @@ def simulate(ctx, job):
     s = job.struct()
     err = _err()
     _check(L.rsg_null_simulate(ctx.h, C.byref(s), _p(out), err), err, "rsg_null_simulate")
+    return out
+
+
+def fitch_null(ctx, ax, tree, seed=1):
+    """R-scape's null alignment of ax on its tree (Fitch ancestors, column shuffle,
+    substitution replay): N x L codes 0..4."""
+    L = _declare()
+    ax = np.ascontiguousarray(ax, dtype=np.uint8)
+    left = np.ascontiguousarray(tree["left"], dtype=np.int32); right = np.ascontiguousarray(tree["right"], dtype=np.int32)
+    out = np.zeros_like(ax)
+    err = _err()
+    _check(L.rsg_null_fitch(ctx.h, _p(ax), ax.shape[0], ax.shape[1], _p(left), _p(right), seed, _p(out), err),
+           err, "rsg_null_fitch")
     return out
 
 

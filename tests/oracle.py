@@ -213,3 +213,15 @@ def mlp_forward_backward(W, b, x, dy=None):
     dbp = (C.c_void_p * nl)(*[v.ctypes.data for v in db])
     L.ref_mlp_backward(nl, d, Wp, bp, P(x), C.c_int64(B), P(acts), P(np.ascontiguousarray(dy, dtype=np.float64)), dWp, dbp)
     return y, dW, db
+
+
+def fitch_null_rows(ax, tree, seed):
+    ax = np.ascontiguousarray(ax, dtype=np.uint8)
+    N, L = ax.shape
+    allr = np.zeros((2 * N - 1, L), np.uint8)
+    sh = np.zeros((2 * N - 1, L), np.uint8)
+    left = np.ascontiguousarray(tree["left"], dtype=np.int32); right = np.ascontiguousarray(tree["right"], dtype=np.int32)
+    f = lib().ref_fitch_null_rows
+    f.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
+    f(P(ax), N, L, P(left), P(right), seed, P(allr), P(sh))
+    return allr, sh

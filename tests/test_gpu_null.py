@@ -146,3 +146,17 @@ def test_batch_rejects_bad_groups_and_trees(ctx):
     bad["left"][0] = bad["right"][0] = c                                   # one internal node reached twice
     with pytest.raises(R.RscapeError):
         NL.simulate(ctx, NL.Job(bad, 20, m))
+
+
+@pytest.mark.parametrize("N,L,seed", [(2, 7, 1), (4, 40, 2), (60, 130, 3), (500, 300, 4)])
+def test_fitch_null_matches_oracle(ctx, N, L, seed):
+    """R-scape's null on the device: symbol-identical to the oracle on the same keys."""
+    from rscape_amd import synth
+    rng = np.random.default_rng(seed)
+    tree = synth.yule_tree(N, rng, 0.15)
+    ax = synth.evolve(tree, L, rng, (0.3, 0.2, 0.2, 0.3))[1].copy()
+    ax[rng.random((N, L)) < 0.1] = 4
+    ax[rng.random((N, L)) < 0.02] = 15
+    got = NL.fitch_null(ctx, ax, tree, seed=77 + seed)
+    _, sh = oracle.fitch_null_rows(ax, tree, 77 + seed)
+    assert np.array_equal(got, sh[:N])

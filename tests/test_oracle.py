@@ -422,3 +422,64 @@ def test_mlp_backward_matches_finite_differences():
         bc[l][0] -= 2 * eps
         ym, _, _, _ = fwd(Ws, bc)
         assert np.isclose(db[l][0], (np.sum((yp - t) ** 2) - np.sum((ym - t) ** 2)) / (2 * eps), rtol=1e-5, atol=1e-8)
+
+
+def _branches(tree, N):
+    for v in range(N - 1):
+        for d in (tree["left"][v], tree["right"][v]):
+            yield N + v, (-d if d <= 0 else N + d)
+
+
+def _fitch_score(tree, N, col):
+    """Textbook Fitch parsimony score of one column (leaf states col[0..N-1], no gaps)."""
+    sets = {}
+    score = 0
+    for v in range(N - 2, -1, -1):
+        kids = []
+        for d in (tree["left"][v], tree["right"][v]):
+            kids.append({int(col[-d])} if d <= 0 else sets[d])
+        inter = kids[0] & kids[1]
+        if inter:
+            sets[v] = inter
+        else:
+            sets[v] = kids[0] | kids[1]
+            score += 1
+    return score
+
+
+def test_fitch_null_invariants():
+    """R-scape's null (Fitch ancestors, one column shuffle, per-branch substitution replay):
+    the root's column multiset and every branch's (old -> new) substitution counts survive the
+    shuffle, input leaves are kept, and on gap-free columns the reconstruction is most
+    parsimonious (its change count equals the Fitch score; brute force on 4 leaves)."""
+    from rscape_amd import synth
+    rng = np.random.default_rng(12)
+    N, L = 40, 90
+    tree = synth.yule_tree(N, rng, 0.2)
+    ax = synth.evolve(tree, L, rng, (0.25,) * 4)[1].copy()
+    ax[rng.random((N, L)) < 0.08] = 4
+    ax[rng.random((N, L)) < 0.01] = 15                    # unknown residue: a random state
+    allr, sh = oracle.fitch_null_rows(ax, tree, 99)
+    assert np.array_equal(allr[:N], ax) and allr[N:].max() <= 4 and sh.max() <= 4
+    assert np.array_equal(np.sort(allr[N]), np.sort(sh[N]))
+    for pa, ch in _branches(tree, N):
+        def subs(a, b):
+            m = (a != b) & (a <= 4) & (b <= 4)
+            return np.bincount(a[m].astype(int) * 5 + b[m], minlength=25)
+        assert np.array_equal(subs(allr[pa], allr[ch]), subs(sh[pa], sh[ch])), (pa, ch)
+    for c in range(L):
+        if np.all(ax[:, c] < 4):
+            changes = sum(int(allr[pa, c] != allr[ch, c]) for pa, ch in _branches(tree, N))
+            assert changes == _fitch_score(tree, N, ax[:, c])
+    # brute force: 4 leaves, every internal assignment
+    import itertools
+    t4 = synth.yule_tree(4, rng, 0.2)
+    for trial in range(30):
+        col = rng.integers(0, 4, size=4).astype(np.uint8)
+        a4, _ = oracle.fitch_null_rows(col[:, None], t4, trial)
+        got = sum(int(a4[pa, 0] != a4[ch, 0]) for pa, ch in _branches(t4, 4))
+        best = 10
+        for states in itertools.product(range(4), repeat=3):
+            full = list(col) + list(states)
+            best = min(best, sum(int(full[pa] != full[ch]) for pa, ch in _branches(t4, 4)))
+        assert got == best
