@@ -25,7 +25,7 @@
 #pragma clang fp contract(off)
 
 enum { D_N = 0, D_L, D_NPAD, D_LPAD, D_XT, D_CNT, D_COV, D_PM, D_ROWS, D_HIST, D_LEAF, D_BPACK, D_BCOL,
-       D_BSTAT, D_BEL, D_NB32, D_GROUP, D_BHIST };
+       D_BSTAT, D_BEL, D_NB32, D_GROUP, D_BHIST, D_BPMT, D_PMP, D_BCOL4 };
 #define HIST_EPB 65536       // matrix elements per histogram block
 #define NBMAX 16384          // histogram bins per item (an item needing more is reported as failed)
 #define ELPB 1024            // matrix elements per block in the element-wise kernels
@@ -61,14 +61,32 @@ k_pack_b(const int64_t *__restrict__ desc, int n, const uint8_t *__restrict__ le
   }
 }
 
-__global__ void __launch_bounds__(256)
-k_pm_b(const int64_t *__restrict__ desc, int n, const uint32_t *__restrict__ cnt, double *__restrict__ pm,
-       double tol, int *__restrict__ flag)
+// corr_Marginals per item with the single-alignment kernels' tiles and reduction order
+// (pm_tile / pm_reduce_col), so a replicate's marginals match its rsg_corr_Probs bit for bit
+__global__ void __launch_bounds__(64 * PMW)
+k_pm_tile_b(const int64_t *__restrict__ desc, int n, const uint32_t *__restrict__ cnt, double *__restrict__ part)
 {
-  const int it = find_item(desc, n, D_BCOL, blockIdx.x);
+  const int it = find_item(desc, n, D_BPMT, blockIdx.x);
   const int64_t *d = desc + it * RSG_BDESC;
-  const int64_t L = d[D_L];
-  pm_column<CNT_I8G>(cnt + d[D_CNT], L, blockIdx.x - d[D_BCOL], tol, pm + d[D_PM], pm + d[D_PM] + 4 * L, flag);
+  const int64_t L = d[D_L], nbt = (L + PMT - 1) / PMT, local = blockIdx.x - d[D_BPMT];
+  const int bi = (int)(local / nbt), bj = (int)(local % nbt);
+  if (bi > bj) return;
+  double *pi = part + d[D_PMP];
+  pm_tile<CNT_I8G>(cnt + d[D_CNT], L, (int)nbt, bi, bj, pi, pi + pm_tri(nbt, nbt, nbt) * PMT * 4);
+}
+
+__global__ void __launch_bounds__(256)
+k_pm_reduce_b(const int64_t *__restrict__ desc, int n, const double *__restrict__ part, double *__restrict__ pm,
+              double tol, int *__restrict__ flag)
+{
+  const int it = find_item(desc, n, D_BCOL4, blockIdx.x);
+  const int64_t *d = desc + it * RSG_BDESC;
+  const int64_t L = d[D_L], nbt = (L + PMT - 1) / PMT;
+  const int64_t col = (blockIdx.x - d[D_BCOL4]) * 4 + (threadIdx.x >> 6);
+  if (col >= L) return;
+  const double *pi = part + d[D_PMP];
+  pm_reduce_col(pi, pi + pm_tri(nbt, nbt, nbt) * PMT * 4, (int)nbt, L, col, threadIdx.x & 63, tol, pm + d[D_PM],
+                pm + d[D_PM] + 4 * L, flag);
 }
 
 __global__ void __launch_bounds__(256)
@@ -316,6 +334,7 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   std::vector<int> cls((size_t)nj);
   int64_t o_xt = 0, o_cnt = 0, o_cov = 0, o_pm = 0, o_rows = 0, o_leaf = 0, nodes = 0, seqs = 0;
   int64_t b_pack = 0, b_col = 0, b_stat = 0, b_el = 0, b_sim = 0, b_hist = 0, nmax = 1;
+  int64_t b_pmt = 0, b_col4 = 0, o_pmp = 0;
   std::vector<int4> tiles, btiles;        // 32-column tiles (small alignments), 64-column tiles (large)
   for (int k = 0; k < nj; k++) {
     const rsg_null_job &J = jobs[k];
@@ -328,6 +347,9 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     d[D_BPACK] = b_pack; d[D_BCOL] = b_col; d[D_BSTAT] = b_stat; d[D_BEL] = b_el;
     const int64_t nb32 = (L + 31) / 32;
     d[D_NB32] = nb32; d[D_GROUP] = J.group; d[D_BHIST] = b_hist;
+    const int64_t nbt = (L + PMT - 1) / PMT;
+    d[D_BPMT] = b_pmt; d[D_PMP] = o_pmp; d[D_BCOL4] = b_col4;
+    b_pmt += nbt * nbt; b_col4 += (L + 3) / 4; o_pmp += 2 * pm_tri(nbt, nbt, nbt) * PMT * 4;
     cls[(size_t)k] = (N <= 8 || L <= 50) ? RSG_C2 : RSG_C16;           // CSELECT (correlators.c:335-341)
     SimItem &s = sim[(size_t)k];
     s.N = N; s.L = L; s.tree_off = nodes; s.seq_off = seqs; s.leaf_off = o_leaf; s.seed = J.seed; s.stream = J.stream;
@@ -378,7 +400,7 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   int64_t *d_desc; SimItem *d_sim; int *d_cls; int4 *d_tiles, *d_btiles; int *d_left, *d_right;
   double *d_ld, *d_rd, *d_models; int4 *d_ent; int *d_gfirst; int64_t *d_fitems; unsigned long long *d_fcnt;
   uint32_t *d_cdf; uint8_t *d_seqs, *d_leaves, *d_xt; uint32_t *d_cnt; double *d_cov, *d_pm, *d_rows, *d_lgc;
-  double *d_part_s, *d_part_e, *d_mm; HGeo *d_geo; unsigned long long *d_hist, *d_S; int *d_NZ; double *d_fits;
+  double *d_part_s, *d_part_e, *d_mm, *d_pmp; HGeo *d_geo; unsigned long long *d_hist, *d_S; int *d_NZ; double *d_fits;
   auto carve = [&](Arena &A) {
     d_desc = A.get<int64_t>(desc.size()); d_sim = A.get<SimItem>((size_t)nj); d_cls = A.get<int>((size_t)nj);
     d_tiles = A.get<int4>(tiles.size()); d_btiles = A.get<int4>(btiles.size());
@@ -392,7 +414,7 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     d_cnt = A.get<uint32_t>((size_t)o_cnt); d_cov = A.get<double>((size_t)o_cov); d_pm = A.get<double>((size_t)o_pm);
     d_rows = A.get<double>((size_t)o_rows); d_lgc = A.get<double>(2 * (size_t)(nmax + 1));
     d_part_s = A.get<double>(2 * (size_t)b_stat); d_part_e = A.get<double>(2 * (size_t)b_el);
-    d_mm = A.get<double>(2 * (size_t)nj); d_geo = A.get<HGeo>((size_t)ng);
+    d_mm = A.get<double>(2 * (size_t)nj); d_pmp = A.get<double>((size_t)o_pmp); d_geo = A.get<HGeo>((size_t)ng);
     d_hist = A.get<unsigned long long>((size_t)ng * NBMAX); d_S = A.get<unsigned long long>((size_t)ng * NBMAX);
     d_NZ = A.get<int>((size_t)ng * NBMAX); d_fits = A.get<double>((size_t)ng * 8);
   };
@@ -442,7 +464,8 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
       return RSG_FAIL;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_flag, 16);
     hipLaunchKernelGGL(k_logtab_b, dim3((unsigned)((nmax + 256) / 256)), dim3(256), 0, c->stream, nmax, d_lgc);
-    hipLaunchKernelGGL(k_pm_b, dim3((unsigned)b_col), dim3(256), 0, c->stream, d_desc, nj, d_cnt, d_pm, 1e-6, c->d_flag);
+    hipLaunchKernelGGL(k_pm_tile_b, dim3((unsigned)b_pmt), dim3(64 * PMW), 0, c->stream, d_desc, nj, d_cnt, d_pmp);
+    hipLaunchKernelGGL(k_pm_reduce_b, dim3((unsigned)b_col4), dim3(256), 0, c->stream, d_desc, nj, d_pmp, d_pm, 1e-6, c->d_flag);
     hipLaunchKernelGGL(k_stat_b, dim3((unsigned)b_stat), dim3(256), 0, c->stream, d_desc, nj, A0, d_cnt, d_pm, d_cov, d_cls, base);
     if (corr) {
       hipLaunchKernelGGL(k_rowsum_b, dim3((unsigned)b_col), dim3(256), 0, c->stream, d_desc, nj, d_cov, d_rows);

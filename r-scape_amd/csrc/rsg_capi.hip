@@ -175,12 +175,11 @@ extern "C" int rsg_corr_Probs(rsg_ctx *c, rsg_msa *m, int mode, double tol, char
     if (rsg_gram_counts_bf16(c, m, mode == RSG_COUNTS_BF16X3 ? 3 : 2, (float *)m->d_cnt, errbuf) != RSG_OK) return RSG_FAIL;
   } else return rsg_fail(errbuf, "unknown counts mode %d", mode);
   if (rsg_probs_from_counts(c, m, tol, errbuf) != RSG_OK) return RSG_FAIL;
-  int flag = 0;
-  HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  flag = *(int *)c->h_pinned;
-  if (flag) return rsg_fail(errbuf, "pm validation failed");
+  HIPCHK(hipMemcpyAsync(c->h_pinned + 4, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   m->have_probs = 1;
+  if (c->defer) return RSG_OK;                 // checked by the deferring caller
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (*(int *)(c->h_pinned + 4)) { m->have_probs = 0; return rsg_fail(errbuf, "pm validation failed"); }
   return RSG_OK;
 }
 
@@ -299,11 +298,25 @@ extern "C" int rsg_cov_Calculate(rsg_ctx *c, rsg_msa *m, int covtype, int covcla
                                  double *minCOV, double *maxCOV, char *errbuf)
 {
   int base = covtype - covtype % 3, corr = covtype % 3;
-  if (base != RSG_RAF && base != RSG_RAFS)
-    if (rsg_corr_Probs(c, m, counts_mode, tol, errbuf) != RSG_OK) return RSG_FAIL;
-  if (rsg_corr_CalculateCOV(c, m, base, covclass, allowpair, nseqthresh, alenthresh, prec, errbuf) != RSG_OK) return RSG_FAIL;
-  if (corr == 1 && rsg_corr_CalculateCOVCorrected(c, m, RSG_APC, 0, errbuf) != RSG_OK) return RSG_FAIL;
-  if (corr == 2 && rsg_corr_CalculateCOVCorrected(c, m, RSG_ASC, 0, errbuf) != RSG_OK) return RSG_FAIL;
+  // the whole chain is queued with one synchronisation at the end: the pm validation flag,
+  // the correction's NaN flag and the final min / max arrive together
+  *(int *)(c->h_pinned + 4) = 0; *(int *)(c->h_pinned + 2) = 0;
+  if (rsg_reserve_scratch(c, m->L, errbuf) != RSG_OK) return RSG_FAIL;
+  c->defer = 1;
+  int st = RSG_OK;
+  if (base != RSG_RAF && base != RSG_RAFS) st = rsg_corr_Probs(c, m, counts_mode, tol, errbuf);
+  if (st == RSG_OK) st = rsg_corr_CalculateCOV(c, m, base, covclass, allowpair, nseqthresh, alenthresh, prec, errbuf);
+  if (st == RSG_OK && corr == 1) st = rsg_corr_CalculateCOVCorrected(c, m, RSG_APC, 0, errbuf);
+  if (st == RSG_OK && corr == 2) st = rsg_corr_CalculateCOVCorrected(c, m, RSG_ASC, 0, errbuf);
+  c->defer = 0;
+  if (hipStreamSynchronize(c->stream) != hipSuccess && st == RSG_OK) st = rsg_fail(errbuf, "rsg_cov_Calculate: device failure");
+  if (st != RSG_OK) { m->have_probs = m->have_cov = 0; return st; }
+  if (base != RSG_RAF && base != RSG_RAFS && *(int *)(c->h_pinned + 4)) {
+    m->have_probs = m->have_cov = 0;
+    return rsg_fail(errbuf, "pm validation failed");
+  }
+  if (corr && *(int *)(c->h_pinned + 2)) { m->have_cov = 0; return rsg_fail(errbuf, "bad covariation COVavg %f", c->h_pinned[3]); }
+  m->minCOV = c->h_pinned[0]; m->maxCOV = c->h_pinned[1];
   if (minCOV) *minCOV = m->minCOV;
   if (maxCOV) *maxCOV = m->maxCOV;
   return RSG_OK;

@@ -34,15 +34,15 @@ template <> __device__ __forceinline__ double tlog<double>(double v) { return lo
 template <> __device__ __forceinline__ float tlog<float>(float v) { return logf(v); }
 
 // One pair. c: counts, pmi/pmj: marginals.  Returns the statistic (see the file header).
-template <typename T, bool TAB>
+// TYPE / CLS >= 0 fix the statistic and class at compile time (-1: read them from A).
+template <typename T, bool TAB, int TYPE = -1, int CLS = -1>
 __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int64_t j)
 {
-  T c[16], ns = 0;
-#pragma unroll
-  for (int k = 0; k < 16; k++) { c[k] = (T)cd[k]; ns += c[k]; }
+#pragma clang fp contract(off)   // no FMA: every specialisation and the batched path round alike
+  T ns = 0;
   double nsd = 0.;
 #pragma unroll
-  for (int k = 0; k < 16; k++) nsd += cd[k];
+  for (int k = 0; k < 16; k++) { ns += (T)cd[k]; nsd += cd[k]; }
   T pmi[4], pmj[4], lpi[4], lpj[4];
 #pragma unroll
   for (int x = 0; x < 4; x++) {
@@ -52,13 +52,14 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
   const T den = ns + (T)16e-10;
   T lden;
   if (TAB) lden = (T)A.lgc[A.N + 1 + (int64_t)nsd]; else lden = tlog<T>(den);
-  T pp[16], lpp[16];
-#pragma unroll
-  for (int k = 0; k < 16; k++) {
-    pp[k] = (c[k] + (T)1e-10) / den;
-    if (TAB) lpp[k] = (T)A.lgc[(int64_t)cd[k]] - lden; else lpp[k] = tlog<T>(c[k] + (T)1e-10) - lden;
-  }
-  const int type = A.type;
+  // pp and ln pp of cell k, formed where they are used (no 16-entry arrays kept live)
+  auto PP = [&](int k) -> T { return ((T)cd[k] + (T)1e-10) / den; };
+  auto LPP = [&](int k) -> T {
+    if (TAB) return (T)A.lgc[(int64_t)cd[k]] - lden;
+    return tlog<T>((T)cd[k] + (T)1e-10) - lden;
+  };
+  const int type = TYPE >= 0 ? TYPE : A.type;
+  const int cls = CLS >= 0 ? CLS : A.cls;
   if (type == RSG_CCF) {
     T sa = 0, sb = 0;
 #pragma unroll
@@ -68,14 +69,14 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
     }
     return (double)sqrt((double)(sa * sb));
   }
-  if (A.cls == RSG_C2) {
+  if (cls == RSG_C2) {
     T pw = 0, pn = 0, qw = 0, qn = 0;
 #pragma unroll
     for (int x = 0; x < 4; x++)
 #pragma unroll
       for (int y = 0; y < 4; y++) {
-        if ((A.allow >> (x * 4 + y)) & 1u) { pw += pp[x * 4 + y]; qw += pmi[x] * pmj[y]; }
-        else                               { pn += pp[x * 4 + y]; qn += pmi[x] * pmj[y]; }
+        if ((A.allow >> (x * 4 + y)) & 1u) { pw += PP(x * 4 + y); qw += pmi[x] * pmj[y]; }
+        else                               { pn += PP(x * 4 + y); qn += pmi[x] * pmj[y]; }
       }
     T v = 0, H = 0;
     switch (type) {
@@ -116,16 +117,18 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
       const int k = x * 4 + y;
       const T q = pmi[x] * pmj[y];
       switch (type) {
-      case RSG_CHI:  if (q > 0) s += (pp[k] - q) * (pp[k] - q) / q; break;
-      case RSG_OMES: if (q > 0) s += (pp[k] - q) * (pp[k] - q); break;
+      case RSG_CHI:  if (q > 0) { const T d = PP(k) - q; s += d * d / q; } break;
+      case RSG_OMES: if (q > 0) { const T d = PP(k) - q; s += d * d; } break;
       case RSG_GT:
-        if (A.cls == RSG_CWC && !((A.allow >> k) & 1u)) break;
-        if (q > 0) s += pp[k] * (lpp[k] - lpi[x] - lpj[y]);
+        if (cls == RSG_CWC && !((A.allow >> k) & 1u)) break;
+        if (q > 0) s += PP(k) * (LPP(k) - lpi[x] - lpj[y]);
         break;
-      case RSG_MI: case RSG_MIg: case RSG_MIr:
-        if (pmi[x] > 0 && pmj[y] > 0) s += pp[k] * (lpp[k] - lpi[x] - lpj[y]);
-        if (type == RSG_MIr) H -= pp[k] * lpp[k];
+      case RSG_MI: case RSG_MIg: case RSG_MIr: {
+        const T pp = PP(k), lpp = LPP(k);
+        if (pmi[x] > 0 && pmj[y] > 0) s += pp * (lpp - lpi[x] - lpj[y]);
+        if (type == RSG_MIr) H -= pp * lpp;
         break;
+      }
       }
     }
   switch (type) {
@@ -138,57 +141,98 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
   return 0.;
 }
 
-// corr_Marginals for one column i (one 256-thread block; fixed-order reduction).
-template <int KIND>
-__device__ void pm_column(const void *__restrict__ cnt, int64_t L, int64_t i, double tol,
-                          double *__restrict__ pm, double *__restrict__ lpm, int *__restrict__ flag)
+// ---- corr_Marginals, pm[i][x] = normalize_x(sum over j != i with nseff_ij > 0 of the pp_ij
+// row (i first) or column (i second) sums).  Pass 1 (pm_tile, 64 * PMW threads) reads one
+// 64 x 64 tile of contiguous record rows once and leaves per-tile partial sums for the tile's
+// i-columns (row sums) and j-columns (column sums); pass 2 (pm_reduce_col, one wave per
+// column) adds a column's partials in a fixed order (deterministic) and normalises.  The
+// single-alignment and the batched paths share both, so they round identically.
+#define PMT 64
+#define PMW 16          // waves per marginal tile block
+// partials of upper-triangle tile (bi <= bj) of nbt x nbt tiles, row-major: [tri][64][4] doubles
+__host__ __device__ __forceinline__ int64_t pm_tri(int64_t bi, int64_t bj, int64_t nbt)
 {
-  __shared__ double red[4][256];
-  double acc[4] = {0., 0., 0., 0.};
-  for (int64_t j = threadIdx.x; j < L; j += 256) {
-    if (j == i) continue;
-    int64_t a = (i < j) ? i : j, b = (i < j) ? j : i;
-    double c[16];
-    load_rec<KIND>(cnt, rsg_rec(a, b, L), c);
-    double ns = 0.;
+  return bi * nbt - (bi * (bi - 1)) / 2 + (bj - bi);
+}
+template <int KIND>
+__device__ void pm_tile(const void *__restrict__ cnt, int64_t L, int nbt, int bi, int bj,
+                        double *__restrict__ part_i, double *__restrict__ part_j)
+{
+  __shared__ double si[PMW][PMT][4], sj[PMT][4];
+  const int tid = threadIdx.x, ii = tid & 63, jq = tid >> 6;
+  const int64_t i = (int64_t)bi * PMT + ii;
+  double ai[4] = {0., 0., 0., 0.};
+  for (int jj = jq; jj < PMT; jj += PMW) {
+    const int64_t j = (int64_t)bj * PMT + jj;
+    double cj[4] = {0., 0., 0., 0.};
+    if (i < j && j < L) {
+      double cd[16];
+      load_rec<KIND>(cnt, rsg_rec(i, j, L), cd);
+      double ns = 0.;
 #pragma unroll
-    for (int k = 0; k < 16; k++) ns += c[k];
-    if (ns > 0) {
-      double inv = 1. / (ns + 16e-10);
+      for (int k = 0; k < 16; k++) ns += cd[k];
+      if (ns > 0) {
+        const double inv = 1. / (ns + 16e-10);
 #pragma unroll
-      for (int x = 0; x < 4; x++) {
-        double r = (i < j) ? (c[x * 4] + c[x * 4 + 1] + c[x * 4 + 2] + c[x * 4 + 3])
-                           : (c[x] + c[4 + x] + c[8 + x] + c[12 + x]);
-        acc[x] += (r + 4e-10) * inv;
+        for (int x = 0; x < 4; x++) {
+          ai[x] += (cd[x * 4] + cd[x * 4 + 1] + cd[x * 4 + 2] + cd[x * 4 + 3] + 4e-10) * inv;
+          cj[x] = (cd[x] + cd[4 + x] + cd[8 + x] + cd[12 + x] + 4e-10) * inv;
+        }
       }
     }
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+      for (int o = 32; o > 0; o >>= 1) cj[x] += __shfl_down(cj[x], o, 64);
+    if (ii == 0) for (int x = 0; x < 4; x++) sj[jj][x] = cj[x];
+  }
+  for (int x = 0; x < 4; x++) si[jq][ii][x] = ai[x];
+  __syncthreads();
+  const int64_t t = pm_tri(bi, bj, nbt);
+  if (tid < PMT * 4) {
+    const int c = tid >> 2, x = tid & 3;
+    double a = 0.;
+    for (int w = 0; w < PMW; w++) a += si[w][c][x];
+    part_i[(t * PMT + c) * 4 + x] = a;
+    part_j[(t * PMT + c) * 4 + x] = sj[c][x];
+  }
+}
+
+// one wave (lane = 0..63) per column: lanes take the column's tiles in a fixed assignment,
+// a fixed shuffle tree adds them
+__device__ __forceinline__ void pm_reduce_col(const double *__restrict__ part_i, const double *__restrict__ part_j,
+                                              int nbt, int64_t L, int64_t col, int lane, double tol,
+                                              double *__restrict__ pm, double *__restrict__ lpm, int *__restrict__ flag)
+{
+  const int b = (int)(col / PMT), o = (int)(col % PMT);
+  double v[4] = {0., 0., 0., 0.};
+  for (int k = lane; k < nbt + 1; k += 64) {
+    // k < nbt - b: i-side partial of tile (b, b + k); k >= nbt - b: j-side partial of tile (k - (nbt - b), b)
+    int64_t base;
+    const double *src;
+    if (k < nbt - b) { src = part_i; base = pm_tri(b, b + k, nbt) * PMT + o; }
+    else if (k - (nbt - b) <= b) { src = part_j; base = pm_tri(k - (nbt - b), b, nbt) * PMT + o; }
+    else continue;
+    for (int x = 0; x < 4; x++) v[x] += src[base * 4 + x];
   }
 #pragma unroll
-  for (int x = 0; x < 4; x++) red[x][threadIdx.x] = acc[x];
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      for (int x = 0; x < 4; x++) red[x][threadIdx.x] += red[x][threadIdx.x + s];
-    __syncthreads();
+  for (int x = 0; x < 4; x++)
+    for (int off = 32; off > 0; off >>= 1) v[x] += __shfl_down(v[x], off, 64);
+  if (lane != 0) return;
+  const double sum = v[0] + v[1] + v[2] + v[3];
+  double tot = 0.;
+  for (int x = 0; x < 4; x++) {
+    v[x] = (sum != 0.) ? v[x] / sum : 0.25;
+    if (!(v[x] >= 0. && v[x] <= 1.)) atomicOr(flag, 1);
+    tot += v[x];
+    pm[col * 4 + x] = v[x];
+    lpm[col * 4 + x] = (v[x] > 0.) ? log(v[x]) : -INFINITY;
   }
-  if (threadIdx.x == 0) {
-    double v[4] = {red[0][0], red[1][0], red[2][0], red[3][0]};
-    double s = v[0] + v[1] + v[2] + v[3];
-    double tot = 0.;
-    for (int x = 0; x < 4; x++) {
-      v[x] = (s != 0.) ? v[x] / s : 0.25;
-      if (!(v[x] >= 0. && v[x] <= 1.)) atomicOr(flag, 1);
-      tot += v[x];
-      pm[i * 4 + x] = v[x];
-      lpm[i * 4 + x] = (v[x] > 0.) ? log(v[x]) : -INFINITY;
-    }
-    if (fabs(tot - 1.) > tol) atomicOr(flag, 1);
-  }
+  if (fabs(tot - 1.) > tol) atomicOr(flag, 1);
 }
 
 // 32 x 32 pair tile; COV written to both triangles through an LDS transpose; block min/max
 // to part[2*part_idx].
-template <typename T, int KIND, bool TAB>
+template <typename T, int KIND, bool TAB, int TYPE = -1, int CLS = -1>
 __device__ void stat_tile(const StatArgs &A, int bi, int bj, int64_t part_idx)
 {
   __shared__ double tile[32][33];
@@ -206,7 +250,7 @@ __device__ void stat_tile(const StatArgs &A, int bi, int bj, int64_t part_idx)
       if (i < j && j < A.L) {
         double cd[16];
         load_rec<KIND>(A.cnt, rsg_rec(i, j, A.L), cd);
-        v = pair_stat<T, TAB>(A, cd, i, j);
+        v = pair_stat<T, TAB, TYPE, CLS>(A, cd, i, j);
         lmin = fmin(lmin, v); lmax = fmax(lmax, v);
         A.cov[j * A.L + i] = v;
       } else if (i == j && i < A.L) {

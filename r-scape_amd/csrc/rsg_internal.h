@@ -50,6 +50,7 @@ struct rsg_ctx {
   rsg_timer    timers[RSG_MAX_TIMERS];
   int          gram_shape = 0;          // 0 = automatic
   int          gram_split = 1;          // split the last partial wave of Gram tiles along K
+  int          defer = 0;               // queue status/min/max copies; the caller synchronises once
   int          ncu = 0;                 // compute units of the device
   void        *d_gscr = nullptr;  size_t gscr_cap = 0;   // Gram split partials
   void        *d_arena = nullptr; size_t arena_cap = 0;  // batched-null buffers (grow-only)
@@ -128,6 +129,7 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf);
 int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, int prec, char *errbuf);
 int rsg_raf(rsg_ctx *c, rsg_msa *m, int rafs, const double *allow16, char *errbuf);
 int rsg_correct(rsg_ctx *c, rsg_msa *m, int actype, int shiftnonneg, char *errbuf);
+int rsg_reserve_scratch(rsg_ctx *c, int64_t L, char *errbuf);
 int rsg_minmax(rsg_ctx *c, rsg_msa *m, char *errbuf);
 int rsg_upload_weights_bf16(rsg_ctx *c, rsg_msa *m, int nsplit, char *errbuf);
 int64_t rsg_npad(int64_t N);
@@ -162,7 +164,7 @@ int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t 
 // Batched pipeline: per-item descriptor of RSG_BDESC int64 fields
 // {N, L, Npad, Lpad, off_xt, off_cnt, off_cov, off_pm, off_rows, off_hist, off_leaf, blk0_pack,
 //  blk0_col, blk0_stat, blk0_el, nb32, group, blk0_hist}
-#define RSG_BDESC 18
+#define RSG_BDESC 21
 // Surrogate features (rsg_features.hip): column statistics of items {one-hot panel offset, N,
 // Npad, L, first 4-column block} into 36 uint64 counters per item; the host turns them into
 // 48 floats.

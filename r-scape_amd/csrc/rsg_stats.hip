@@ -15,36 +15,74 @@
 #include <cmath>
 #include <algorithm>
 
-// ---- pm: corr_Marginals.  One block per column i; fixed-order reduction (deterministic).
-template <int KIND>
-__global__ void __launch_bounds__(256) k_pm(const void *__restrict__ cnt, int64_t L, double tol,
-                                            double *__restrict__ pm, double *__restrict__ lpm, int *__restrict__ flag)
+static int ensure_red(rsg_ctx *c, size_t n, char *errbuf);
+
+// Reduction scratch large enough for every stage of one scoring of an L-column alignment
+// (sized before a deferred chain, so no stage reallocates it while earlier work is queued).
+int rsg_reserve_scratch(rsg_ctx *c, int64_t L, char *errbuf)
 {
-  pm_column<KIND>(cnt, L, (int64_t)blockIdx.x, tol, pm, lpm, flag);
+  const int64_t nbt = (L + 63) / 64, nb = (L + 31) / 32;
+  const size_t need = std::max<size_t>({(size_t)(2 * pm_tri(nbt, nbt, nbt) * PMT * 4), (size_t)(2 * nb * nb + 8 + 4 * L),
+                                        (size_t)(2 * L + 8)});
+  return ensure_red(c, need, errbuf);
 }
 
-
-// ---- ps: mutual_naive_psi, one block per column over the transposed raw codes.
-__global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, const double *__restrict__ w,
-                                            int64_t N, int64_t Npad, double *__restrict__ ps)
+// ---- pm: corr_Marginals (pm_tile / pm_reduce_col in rsg_device.h).  Tiles below the
+// diagonal exit at once.
+template <int KIND>
+__global__ void __launch_bounds__(64 * PMW) k_pm_tile(const void *__restrict__ cnt, int64_t L, int nbt,
+                                                      double *__restrict__ part_i, double *__restrict__ part_j)
 {
-  __shared__ double red[5][256];
-  const int64_t i = blockIdx.x;
+  if (blockIdx.y > blockIdx.x) return;
+  pm_tile<KIND>(cnt, L, nbt, (int)blockIdx.y, (int)blockIdx.x, part_i, part_j);
+}
+
+__global__ void __launch_bounds__(256) k_pm_reduce(const double *__restrict__ part_i, const double *__restrict__ part_j,
+                                                   int nbt, int64_t L, double tol, double *__restrict__ pm,
+                                                   double *__restrict__ lpm, int *__restrict__ flag)
+{
+  const int64_t col = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (col < L) pm_reduce_col(part_i, part_j, nbt, L, col, threadIdx.x & 63, tol, pm, lpm, flag);
+}
+
+// corr_NaivePS: ps[i][x] = normalize(1e-5 + sum_s w_s [a_si == x]), x over A, C, G, U, gap.
+// One wave per column (4 per block), 16 codes per load; W: weighted, else unit weights.
+template <bool W>
+__global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, const double *__restrict__ w,
+                                            int64_t N, int64_t Npad, int64_t L, double *__restrict__ ps)
+{
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= L) return;
+  const uint4 *col = (const uint4 *)(axT + i * Npad);
   double acc[5] = {0., 0., 0., 0., 0.};
-  for (int64_t s = threadIdx.x; s < N; s += 256) {
-    uint8_t a = axT[i * Npad + s];
-    if (a < 5) acc[a] += w[s];
+  unsigned cnt[5] = {0u, 0u, 0u, 0u, 0u};                // unit weights: integer counts
+  for (int64_t k = lane; k * 16 < N; k += 64) {
+    const uint4 v = col[k];
+    const unsigned wd[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int b = 0; b < 16; b++) {
+      const int64_t sq = k * 16 + b;
+      const unsigned a = (sq < N) ? (wd[b >> 2] >> (8 * (b & 3))) & 0xFFu : 0xFFu;
+      if (W) {
+        const double ws = (a < 5u) ? w[sq] : 0.0;
+#pragma unroll
+        for (int x = 0; x < 5; x++) acc[x] += (a == (unsigned)x) ? ws : 0.0;
+      } else {
+#pragma unroll
+        for (int x = 0; x < 5; x++) cnt[x] += (a == (unsigned)x);
+      }
+    }
   }
-  for (int x = 0; x < 5; x++) red[x][threadIdx.x] = acc[x];
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) for (int x = 0; x < 5; x++) red[x][threadIdx.x] += red[x][threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    double v[5], s = 0.;
-    for (int x = 0; x < 5; x++) { v[x] = 1e-5 + red[x][0]; s += v[x]; }
-    for (int x = 0; x < 5; x++) ps[i * 5 + x] = v[x] / s;
+  if (!W)
+    for (int x = 0; x < 5; x++) acc[x] = (double)cnt[x];
+#pragma unroll
+  for (int x = 0; x < 5; x++)
+    for (int o = 32; o > 0; o >>= 1) acc[x] += __shfl_down(acc[x], o, 64);
+  if (lane == 0) {
+    double v[5], sum = 0.;
+    for (int x = 0; x < 5; x++) { v[x] = 1e-5 + acc[x]; sum += v[x]; }
+    for (int x = 0; x < 5; x++) ps[i * 5 + x] = v[x] / sum;
   }
 }
 
@@ -61,13 +99,24 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
   if (!m->d_pm) HIPCHK(hipMalloc(&m->d_pm, sizeof(double) * 8 * (size_t)L));
   if (!m->d_ps) HIPCHK(hipMalloc(&m->d_ps, sizeof(double) * 5 * (size_t)L));
   HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+  const int nbt = (int)((L + PMT - 1) / PMT);
+  const size_t npart = (size_t)pm_tri(nbt, nbt, nbt) * PMT * 4;     // nbt (nbt + 1) / 2 tiles
+  if (ensure_red(c, 2 * npart, errbuf) != RSG_OK) return RSG_FAIL;
+  double *part_i = c->d_red, *part_j = c->d_red + npart;
   int t; rsg_tstart(c, "marginals", &t);
-  if (m->cnt_kind == CNT_I8G) hipLaunchKernelGGL(k_pm<CNT_I8G>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, tol, m->d_pm, m->d_pm + 4 * L, c->d_flag);
-  else                            hipLaunchKernelGGL(k_pm<CNT_F32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, tol, m->d_pm, m->d_pm + 4 * L, c->d_flag);
+  if (m->cnt_kind == CNT_I8G)
+    hipLaunchKernelGGL(k_pm_tile<CNT_I8G>, dim3((unsigned)nbt, (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j);
+  else
+    hipLaunchKernelGGL(k_pm_tile<CNT_F32>, dim3((unsigned)nbt, (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j);
+  hipLaunchKernelGGL(k_pm_reduce, dim3((unsigned)((L + 3) / 4)), dim3(256), 0, c->stream, part_i, part_j, nbt, L, tol,
+                     m->d_pm, m->d_pm + 4 * L, c->d_flag);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   rsg_tstart(c, "ps", &t);
-  hipLaunchKernelGGL(k_ps, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_axT, m->d_w, m->N, m->Npad, m->d_ps);
+  if (m->weighted)
+    hipLaunchKernelGGL(k_ps<true>, dim3((unsigned)((L + 3) / 4)), dim3(256), 0, c->stream, m->d_axT, m->d_w, m->N, m->Npad, L, m->d_ps);
+  else
+    hipLaunchKernelGGL(k_ps<false>, dim3((unsigned)((L + 3) / 4)), dim3(256), 0, c->stream, m->d_axT, m->d_w, m->N, m->Npad, L, m->d_ps);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   if (!m->d_lgc) {
@@ -80,10 +129,21 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
 
 // ------------------------------------------------------------------------------------
 // 32 x 32 pair tile per block; COV written to both triangles through an LDS transpose.
-template <typename T, int KIND, bool TAB>
+template <typename T, int KIND, bool TAB, int TYPE = -1, int CLS = -1>
 __global__ void __launch_bounds__(256) k_stat(StatArgs A, int nb)
 {
-  stat_tile<T, KIND, TAB>(A, (int)(blockIdx.x % nb), (int)(blockIdx.x / nb), (int64_t)blockIdx.x);
+  stat_tile<T, KIND, TAB, TYPE, CLS>(A, (int)(blockIdx.x % nb), (int)(blockIdx.x / nb), (int64_t)blockIdx.x);
+}
+
+// FP64 statistics from integer counts with the statistic and class fixed at compile time
+// (the common cases: fewer live registers, no dead branches).
+template <int TYPE>
+static bool launch_stat_fixed(rsg_ctx *c, const StatArgs &A, int cls, dim3 grid, int nb)
+{
+  if (cls == RSG_C16) hipLaunchKernelGGL((k_stat<double, CNT_I8G, true, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
+  else if (cls == RSG_C2) hipLaunchKernelGGL((k_stat<double, CNT_I8G, true, TYPE, RSG_C2>), grid, dim3(256), 0, c->stream, A, nb);
+  else return false;
+  return true;
 }
 
 __global__ void k_reduce_minmax(const double *__restrict__ part, int n, double *__restrict__ out)
@@ -110,11 +170,14 @@ static int ensure_red(rsg_ctx *c, size_t n, char *errbuf)
   return RSG_OK;
 }
 
+// Block partials -> (min, max) into h_pinned[0..1].  In deferred mode (one synchronisation
+// at the end of rsg_cov_Calculate) the copy is only queued.
 static int fetch_minmax(rsg_ctx *c, rsg_msa *m, int nparts, char *errbuf)
 {
   hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(256), 0, c->stream, c->d_red, nparts, c->d_red + 2 * (size_t)nparts);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_red + 2 * (size_t)nparts, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (c->defer) return RSG_OK;
   HIPCHK(hipStreamSynchronize(c->stream));
   m->minCOV = c->h_pinned[0]; m->maxCOV = c->h_pinned[1];
   return RSG_OK;
@@ -170,7 +233,19 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
   int t; rsg_tstart(c, "stat", &t);
   dim3 grid((unsigned)nblk), blk(256);
 #define LAUNCH(T, K, TB) hipLaunchKernelGGL((k_stat<T, K, TB>), grid, blk, 0, c->stream, A, nb)
-  if (prec == RSG_STAT_FP64) {
+  bool fixed = false;
+  if (prec == RSG_STAT_FP64 && m->cnt_kind == CNT_I8G && tab) {
+    switch (base) {
+    case RSG_GT:   fixed = launch_stat_fixed<RSG_GT>(c, A, cls, grid, nb); break;
+    case RSG_MI:   fixed = launch_stat_fixed<RSG_MI>(c, A, cls, grid, nb); break;
+    case RSG_MIr:  fixed = launch_stat_fixed<RSG_MIr>(c, A, cls, grid, nb); break;
+    case RSG_MIg:  fixed = launch_stat_fixed<RSG_MIg>(c, A, cls, grid, nb); break;
+    case RSG_CHI:  fixed = launch_stat_fixed<RSG_CHI>(c, A, cls, grid, nb); break;
+    case RSG_OMES: fixed = launch_stat_fixed<RSG_OMES>(c, A, cls, grid, nb); break;
+    }
+  }
+  if (fixed) {
+  } else if (prec == RSG_STAT_FP64) {
     if (m->cnt_kind == CNT_I8G) { if (tab) LAUNCH(double, CNT_I8G, true); else LAUNCH(double, CNT_I8G, false); }
     else                        LAUNCH(double, CNT_F32, false);
   } else {
@@ -313,29 +388,32 @@ __global__ void __launch_bounds__(256) k_rowmean(int64_t L, double *__restrict__
   if (threadIdx.x == 0) rows[2 * L] = (L > 1) ? red[0] / ((double)L * ((double)L - 1.)) : red[0];
 }
 
+// APC / ASC in place, one block per row i (no per-element index division); block min / max
+// of the corrected row to part[2 i], part[2 i + 1].
 __global__ void __launch_bounds__(256) k_correct(double *__restrict__ cov, int64_t L, const double *__restrict__ rows,
                                                  int actype, int *__restrict__ flag, double *__restrict__ part)
 {
   __shared__ double rmin[256], rmax[256];
   const double avg = rows[2 * L];
+  const int64_t i = blockIdx.x;
+  const double xi = rows[i];
+  double *C = cov + i * L;
   double a = INFINITY, b = -INFINITY;
-  const int64_t n = L * L;
-  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * 256) {
-    int64_t i = idx / L, j = idx - i * L;
-    if (i == j) continue;
-    double v = cov[idx], xi = rows[i], xj = rows[j];
-    double r = (actype == RSG_APC) ? ((avg != 0.0) ? v - xi * xj / avg : 0.0) : v - (xi + xj - avg);
+  for (int64_t j = threadIdx.x; j < L; j += 256) {
+    if (j == i) continue;
+    const double v = C[j], xj = rows[j];
+    const double r = (actype == RSG_APC) ? ((avg != 0.0) ? v - xi * xj / avg : 0.0) : v - (xi + xj - avg);
     if (isnan(r)) atomicOr(flag, 1);
-    cov[idx] = r;
+    C[j] = r;
     a = fmin(a, r); b = fmax(b, r);
   }
   rmin[threadIdx.x] = a; rmax[threadIdx.x] = b;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) { rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + s]); rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + s]); }
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) { rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + st]); rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + st]); }
     __syncthreads();
   }
-  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rmin[0]; part[2 * blockIdx.x + 1] = rmax[0]; }
+  if (threadIdx.x == 0) { part[2 * i] = rmin[0]; part[2 * i + 1] = rmax[0]; }
 }
 
 __global__ void k_shift(double *__restrict__ cov, int64_t L, const double *__restrict__ mm)
@@ -351,13 +429,13 @@ int rsg_correct(rsg_ctx *c, rsg_msa *m, int actype, int shiftnonneg, char *errbu
   const int64_t L = m->L;
   if (!m->have_cov) return rsg_fail(errbuf, "no covariation matrix to correct");
   if (!m->d_rowx) HIPCHK(hipMalloc(&m->d_rowx, sizeof(double) * (2 * (size_t)L + 2)));
-  const int nblk = (int)std::min<int64_t>(1024, (L * L + 255) / 256);
+  const int nblk = (int)L;
   if (ensure_red(c, 2 * (size_t)nblk + 8, errbuf) != RSG_OK) return RSG_FAIL;
-  HIPCHK(hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(c->d_flag + 1, 0, sizeof(int), c->stream));        // NaN flag of the correction
   int t; rsg_tstart(c, "correct", &t);
   hipLaunchKernelGGL(k_rowsum, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cov, L, m->d_rowx);
   hipLaunchKernelGGL(k_rowmean, dim3(1), dim3(256), 0, c->stream, L, m->d_rowx);
-  hipLaunchKernelGGL(k_correct, dim3(nblk), dim3(256), 0, c->stream, m->d_cov, L, m->d_rowx, actype, c->d_flag, c->d_red);
+  hipLaunchKernelGGL(k_correct, dim3(nblk), dim3(256), 0, c->stream, m->d_cov, L, m->d_rowx, actype, c->d_flag + 1, c->d_red);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(256), 0, c->stream, c->d_red, nblk, c->d_red + 2 * (size_t)nblk);
@@ -365,12 +443,12 @@ int rsg_correct(rsg_ctx *c, rsg_msa *m, int actype, int shiftnonneg, char *errbu
     hipLaunchKernelGGL(k_shift, dim3((unsigned)((L * L + 255) / 256)), dim3(256), 0, c->stream, m->d_cov, L, c->d_red + 2 * (size_t)nblk);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_red + 2 * (size_t)nblk, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(c->h_pinned + 2, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_pinned + 2, c->d_flag + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->h_pinned + 3, m->d_rowx + 2 * L, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (c->defer) return RSG_OK;
   HIPCHK(hipStreamSynchronize(c->stream));
   int flag = *(int *)(c->h_pinned + 2);
-  double avg;
-  HIPCHK(hipMemcpy(&avg, m->d_rowx + 2 * L, sizeof(double), hipMemcpyDeviceToHost));
-  if (flag) return rsg_fail(errbuf, "bad covariation COVavg %f", avg);
+  if (flag) return rsg_fail(errbuf, "bad covariation COVavg %f", c->h_pinned[3]);
   m->minCOV = c->h_pinned[0]; m->maxCOV = c->h_pinned[1];
   return RSG_OK;
 }
