@@ -219,11 +219,13 @@ int rsg_null_batch(rsg_ctx *ctx, const rsg_null_job *jobs, int njobs, int covtyp
 int  rsg_null_features(rsg_ctx *ctx, rsg_msa *msa, const int *left, const int *right,
                        const double *ld, const double *rd, float *feat, char *errbuf);
 /* E-values from a parametric tail P(score > x) = p0 exp(-lambda (x - m)) (the surrogate's
- * output): Eval[i][j] = Nc * min(1, p0 exp(-lambda (COV[i][j] - m))) for i != j, +inf on
- * the diagonal (cov2evalue's role, src/covariation.c:2369-2436, for a fitted tail).  Kept on
- * the device for rsg_cov_Hits; Eval may be NULL. */
-int  rsg_cov_EvaluesTail(rsg_ctx *ctx, rsg_msa *msa, double m, double lambda, double p0, int64_t Nc,
-                         double *Eval, char *errbuf);
+ * output): Eval[i][j] = Nc * min(1, p0 exp(-lambda (min(COV[i][j], xcap) - m))) for i != j,
+ * +inf on the diagonal (cov2evalue's role, src/covariation.c:2369-2436, for a fitted tail).
+ * xcap is where cov2evalue stops following the fit: scores in or past the last bin of the
+ * 2 nb survival array (src/covariation.c:2380) all get the E-value of its upper bound,
+ * bmin + 2 nb w; +inf for no cap.  Kept on the device for rsg_cov_Hits; Eval may be NULL. */
+int  rsg_cov_EvaluesTail(rsg_ctx *ctx, rsg_msa *msa, double m, double lambda, double p0, double xcap,
+                         int64_t Nc, double *Eval, char *errbuf);
 typedef struct rsg_mlp rsg_mlp;
 int  rsg_mlp_create(rsg_ctx *ctx, int nlayers, const int *dims, uint64_t seed, rsg_mlp **ret, char *errbuf);
 void rsg_mlp_destroy(rsg_ctx *ctx, rsg_mlp *m);

@@ -334,21 +334,21 @@ extern "C" int rsg_cov_Evalues(rsg_ctx *c, rsg_msa *m, const rsg_hist *null, con
   return st;
 }
 
-__global__ void k_evtail(const double *__restrict__ cov, int64_t L, double m, double lambda, double p0, double Nc,
-                         double *__restrict__ ev)
+__global__ void k_evtail(const double *__restrict__ cov, int64_t L, double m, double lambda, double p0, double xcap,
+                         double Nc, double *__restrict__ ev)
 {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= L * L) return;
   const int64_t i = e / L, j = e - i * L;
   if (i == j) { ev[e] = INFINITY; return; }
-  const double p = p0 * exp(-lambda * (cov[e] - m));
+  const double p = p0 * exp(-lambda * (fmin(cov[e], xcap) - m));
   ev[e] = Nc * (p < 1.0 ? p : 1.0);
 }
 
-extern "C" int rsg_cov_EvaluesTail(rsg_ctx *c, rsg_msa *m, double mu, double lambda, double p0, int64_t Nc,
-                                   double *Eval, char *errbuf)
+extern "C" int rsg_cov_EvaluesTail(rsg_ctx *c, rsg_msa *m, double mu, double lambda, double p0, double xcap,
+                                   int64_t Nc, double *Eval, char *errbuf)
 {
-  if (!c || !m) return rsg_fail(errbuf, "rsg_cov_EvaluesTail: bad arguments");
+  if (!c || !m || std::isnan(xcap)) return rsg_fail(errbuf, "rsg_cov_EvaluesTail: bad arguments");
   if (!m->have_cov) return rsg_fail(errbuf, "no covariation matrix");
   if (!(lambda > 0.) || !(p0 > 0.)) return rsg_fail(errbuf, "rsg_cov_EvaluesTail: needs lambda > 0 and p0 > 0");
   HIPCHK(hipSetDevice(c->device));
@@ -356,7 +356,7 @@ extern "C" int rsg_cov_EvaluesTail(rsg_ctx *c, rsg_msa *m, double mu, double lam
   const int64_t L = m->L;
   int t; rsg_tstart(c, "evalues_tail", &t);
   hipLaunchKernelGGL(k_evtail, dim3((unsigned)((L * L + 255) / 256)), dim3(256), 0, c->stream, m->d_cov, L, mu, lambda,
-                     p0, (double)Nc, m->d_eval);
+                     p0, xcap, (double)Nc, m->d_eval);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   if (Eval) HIPCHK(hipMemcpyAsync(Eval, m->d_eval, sizeof(double) * (size_t)(L * L), hipMemcpyDeviceToHost, c->stream));

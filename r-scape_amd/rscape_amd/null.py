@@ -5,6 +5,7 @@ children > 0 internal, <= 0 leaf -child).  Substitution model: 4-state GTR scale
 expected substitution per unit branch length.  All arithmetic runs in librscape_hip.
 """
 import ctypes as C
+import math
 
 import numpy as np
 
@@ -40,7 +41,7 @@ def _declare():
     L.rsg_null_features.restype = i32
     L.rsg_null_features.argtypes = [vp, vp, vp, vp, vp, vp, vp, cp]
     L.rsg_cov_EvaluesTail.restype = i32
-    L.rsg_cov_EvaluesTail.argtypes = [vp, vp, d, d, d, i64, vp, cp]
+    L.rsg_cov_EvaluesTail.argtypes = [vp, vp, d, d, d, d, i64, vp, cp]
     L.rsg_cov_Hits.restype = i32
     L.rsg_cov_Hits.argtypes = [vp, vp, d, i32, C.POINTER(i32), vp, vp, vp, vp, cp]
     L._null_declared = True
@@ -157,14 +158,14 @@ def features(ctx, msa, tree):
     return f
 
 
-def evalues_tail(ctx, msa, m, lam, p0, Nc=None, matrix=True):
-    """E-values of the resident scores from a parametric tail p0 exp(-lam (x - m)); with
-    matrix=False they stay on the device (for hits())."""
+def evalues_tail(ctx, msa, m, lam, p0, Nc=None, matrix=True, xcap=math.inf):
+    """E-values of the resident scores from a parametric tail p0 exp(-lam (x - m)), scores
+    above xcap held at xcap; with matrix=False they stay on the device (for hits())."""
     L = _declare()
     n = msa.Lc
     ev = np.zeros((n, n)) if matrix else None
     err = _err()
-    _check(L.rsg_cov_EvaluesTail(ctx.h, msa.h, m, lam, p0, n * (n - 1) // 2 if Nc is None else Nc, _p(ev), err),
+    _check(L.rsg_cov_EvaluesTail(ctx.h, msa.h, m, lam, p0, xcap, n * (n - 1) // 2 if Nc is None else Nc, _p(ev), err),
            err, "rsg_cov_EvaluesTail")
     return ev
 

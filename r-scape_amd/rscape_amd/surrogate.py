@@ -9,12 +9,21 @@ trained on tail fits that the batched GPU null simulator produces (rsg_null_batc
 replicate groups and features), and falls back to the full simulated null when an
 alignment's features leave the training range.
 
-Training data: random setups with N log-uniform in [20, 6000], L log-uniform in [50, 3500],
-Yule trees with mean branch length U(0.05, 0.3), random base composition and gap fraction.
-Each setup is simulated as a group of K replicates, K = clamp(ceil(T / pairs), 1, 200) with
-T = 2e5 null pairs by default, so a target fit sees ~100 pairs in a 5e-4 tail (fewer only for
-the smallest alignments); the fit (exponential, pmass 5e-4) is moved to the reference mass P0
-by m = mu + ln(newmass / P0) / lambda.
+Training data: random setups with N log-uniform in [20, 6000], L log-uniform in [50, 3500]
+(a share `big` of them drawn from N in [1000, 6000], L in [300, 3500] instead), Yule trees
+with mean branch length U(0.05, 0.3), random base composition and gap fraction.  Each setup
+is simulated as a group of K replicates, K = clamp(ceil(T / pairs), 1, 200) for T null pairs,
+so a target fit sees T * 5e-4 pairs in its tail (fewer only for the smallest alignments); the
+fit (exponential, pmass 5e-4) is moved to the reference mass P0 by
+m = mu + ln(newmass / P0) / lambda.
+
+R-scape's cov2evalue follows the fitted tail only up to the end of its 2 nb-bin survival array
+(src/covariation.c:2380); every higher score gets the E-value of that last bin.  nb is set by
+the largest null score, so the surrogate also predicts where the cap falls: the largest of n
+null scores, in units of the tail, z = lambda (max - m), is modelled as z = a + b ln(n P0),
+with (a, b) fitted on the training groups' maxima (an exponential tail would give b = 1,
+a = -ln ln 2 for the median); the cap is then bmin + 2 nb w with the histogram geometry of
+calculate_width_histo and cov_GrowRankList.
 """
 import math
 import os
@@ -39,7 +48,9 @@ def replicates_for(L, target_pairs=2e5, cap=200):
     return int(min(cap, max(1, math.ceil(target_pairs / pairs))))
 
 
-def random_setup(rng, Nr=(20, 6000), Lr=(50, 3500)):
+def random_setup(rng, Nr=(20, 6000), Lr=(50, 3500), big=0.0):
+    if big > 0 and rng.random() < big:
+        Nr = (max(Nr[0], min(1000, Nr[1])), Nr[1]); Lr = (max(Lr[0], min(300, Lr[1])), Lr[1])
     N = int(round(math.exp(rng.uniform(math.log(Nr[0]), math.log(Nr[1])))))
     L = int(round(math.exp(rng.uniform(math.log(Lr[0]), math.log(Lr[1])))))
     pi = rng.dirichlet(np.full(4, 8.0))
@@ -52,9 +63,10 @@ def _record_bytes(L, N, K):
 
 
 def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1, log=None, covtype="GTp",
-             target_pairs=2e5):
+             target_pairs=2e5, big=0.0):
     """n training pairs: X (n, 48) features, Y (n, 2) = (lambda m, ln lambda); setups whose fit
-    failed are dropped.  Returns X, Y, list of setups."""
+    failed are dropped.  Returns X, Y, list of setups (each with its replicate count K, null
+    pair count npairs, bin width w and the group's smallest and largest null score)."""
     rng = np.random.default_rng(seed)
     X, Y, S = [], [], []
     pending, used = [], 0.0
@@ -73,6 +85,9 @@ def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1,
             if not (np.isfinite(mu) and np.isfinite(lam) and lam > 0 and newmass > 0):
                 continue
             m, lam = tail_params(mu, lam, newmass)
+            K = replicates_for(st["L"], target_pairs)
+            st = dict(st, K=K, npairs=K * (st["L"] * (st["L"] - 1) // 2), w=float(fits[g][5]),
+                      mn=float(fits[g][6]), mx=float(fits[g][7]))
             X.append(fe[g]); Y.append((lam * m, math.log(lam))); S.append(st)
         if log:
             log("generated %d / %d" % (len(X), n))
@@ -80,7 +95,7 @@ def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1,
 
     made = 0
     while made < n:
-        st = random_setup(rng, Nr, Lr)
+        st = random_setup(rng, Nr, Lr, big)
         tree = NL.yule_tree(st["N"], st["bl"], seed=st["tseed"])
         model = NL.gtr_model(st["pi"])
         b = _record_bytes(st["L"], st["N"], replicates_for(st["L"], target_pairs))
@@ -98,6 +113,32 @@ class Surrogate:
         self.ctx = ctx
         self.net = net or MLP(ctx, DIMS, seed=seed, bf16=True)
         self.xmin = self.xmax = None
+        self.zab = (-math.log(math.log(2.0)), 1.0)     # exponential tail, median maximum
+
+    def fit_cap(self, setups, Y):
+        """(a, b) of z = lambda (max - m) = a + b ln(npairs P0) from the training groups."""
+        lam = np.exp(Y[:, 1].astype(np.float64))
+        m = Y[:, 0] / lam
+        z = lam * (np.array([s["mx"] for s in setups]) - m)
+        u = np.log(np.array([s["npairs"] for s in setups], np.float64) * P0)
+        ok = np.isfinite(z) & np.isfinite(u)
+        b, a = np.polyfit(u[ok], z[ok], 1)
+        self.zab = (float(a), float(b))
+        return self.zab
+
+    def cap(self, m, lam, K, Nc, bmin=-10.0, w0=0.05, hpts=400, tol=1e-6):
+        """Score from which R-scape's E-values stop following the tail, for a K-replicate null
+        of an alignment with Nc pairs: bmin + 2 nb w (src/covariation.c:2380), with w from the
+        first replicate (calculate_width_histo, src/R-scape.c:1353-1360: its largest score
+        against bmin) and nb = (max + 5 w - bmin) / w bins (cov_GrowRankList)."""
+        a, b = self.zab
+        mx1 = m + (a + b * math.log(max(Nc * P0, 1e-300))) / lam
+        mx = m + (a + b * math.log(max(K * Nc * P0, 1e-300))) / lam
+        w = min(w0, (mx1 - bmin) / hpts)
+        if not (w >= tol and mx > bmin):
+            return math.inf
+        nb = int((mx + 5 * w - bmin) / w)
+        return bmin + 2 * nb * w
 
     def fit(self, X, Y, epochs=60, batch=512, lr=1e-3, seed=7):
         xm, xs = X.mean(0), X.std(0)
@@ -124,6 +165,7 @@ class Surrogate:
         xm, xs, ym, ys = self.scaling
         os.makedirs(os.path.dirname(path), exist_ok=True)
         np.savez(path, dims=np.array(DIMS), xm=xm, xs=xs, ym=ym, ys=ys, xmin=self.xmin, xmax=self.xmax,
+                 zab=np.array(self.zab),
                  **{"W%d" % k: w for k, w in enumerate(W)}, **{"b%d" % k: v for k, v in enumerate(b)})
 
     @classmethod
@@ -136,6 +178,8 @@ class Surrogate:
         s.net.set_scaling(*s.scaling)
         s.xmin, s.xmax = z["xmin"], z["xmax"]
         s.net.set_range(s.xmin, s.xmax)
+        if "zab" in z.files:
+            s.zab = (float(z["zab"][0]), float(z["zab"][1]))
         return s
 
 

@@ -1,14 +1,19 @@
 """Train and evaluate the null-tail surrogate on the GPU.
 
-    python -m rscape_amd.surrogate_cli train --n 50000 --out PATH.npz [--epochs E]
+    python -m rscape_amd.surrogate_cli gen --n 100000 --out DATA.npz
+    python -m rscape_amd.surrogate_cli train --data DATA.npz --out PATH.npz [--epochs E]
     python -m rscape_amd.surrogate_cli eval --model PATH.npz --n 200 [--out REPORT.json]
 
-`train` simulates the training nulls with the batched GPU simulator, fits, trains the MLP on
-the device and saves weights, scalers and training range.  `eval` draws held-out structured
-alignments (planted nested pairs, as the configuration-0 generator, at random sizes), scores
-each (GTp), and compares E-values of the surrogate with those of the full simulated null
-(R-scape's replicate count) for pairs the full null calls significant (E < 0.05); a second
-full null with other seeds gives the replicate noise of the full null itself.
+`gen` simulates the training nulls with the batched GPU simulator and fits their tails;
+`train` (which generates when no --data is given) trains the MLP on the device, fits the
+cap model on the groups' null maxima and saves weights, scalers, training range and cap
+model.  `eval` draws held-out structured alignments (planted nested pairs, as the
+configuration-0 generator, at random sizes), scores each (GTp), and compares the surrogate's
+log10 E-values with the mean over R independent full simulated nulls (R-scape's replicate
+count each) for pairs that mean calls significant (E < 0.05); the distance of one full null
+from the mean of the others is the replicate noise a single R-scape run carries.  Pairs are
+reported together and split into the fitted range and the capped range (past the last
+survival bin, where cov2evalue holds E constant).
 """
 import argparse
 import json
@@ -28,18 +33,54 @@ def log(msg):
     print(msg, flush=True)
 
 
+SETUP_KEYS = ("N", "L", "bl", "gap", "K", "npairs", "w", "mn", "mx")
+
+
+def save_data(path, X, Y, setups):
+    np.savez(path, X=X, Y=Y, **{k: np.array([s[k] for s in setups], np.float64) for k in SETUP_KEYS},
+             pi=np.array([s["pi"] for s in setups], np.float64))
+
+
+def load_data(path):
+    with np.load(path, allow_pickle=False) as z:
+        arr = {k: z[k] for k in z.files}                      # each z[k] reads the archive again
+    setups = [dict({k: float(arr[k][i]) for k in SETUP_KEYS}, pi=tuple(arr["pi"][i])) for i in range(len(arr["X"]))]
+    return arr["X"], arr["Y"], setups
+
+
+def cmd_gen(a):
+    ctx = Context(0)
+    t0 = time.time()
+    X, Y, setups = S.generate(ctx, a.n, seed=a.seed, budget=a.budget, log=log, target_pairs=a.target_pairs, big=a.big)
+    log("training pairs: %d (%.1f s to simulate, score and fit)" % (len(X), time.time() - t0))
+    save_data(a.out, X, Y, setups)
+
+
+def tail_error_at(Yt, mp, lp, depth=13.0):
+    """|log10 E| error of a predicted tail at the score where the true tail has
+    lambda (x - m) = depth (E ~ 1e-3 for 1e6 pairs)."""
+    lt = np.exp(Yt[:, 1].astype(np.float64))
+    mt = Yt[:, 0] / lt
+    x = mt + depth / lt
+    return np.abs(lp * (x - mp) - depth) / math.log(10.0)
+
+
 def cmd_train(a):
     ctx = Context(0)
     t0 = time.time()
-    X, Y, setups = S.generate(ctx, a.n, seed=a.seed, budget=a.budget, log=log, target_pairs=a.target_pairs)
+    if a.data:
+        X, Y, setups = load_data(a.data)
+    else:
+        X, Y, setups = S.generate(ctx, a.n, seed=a.seed, budget=a.budget, log=log, target_pairs=a.target_pairs, big=a.big)
     t1 = time.time()
-    log("training pairs: %d (%.1f s to simulate, score and fit)" % (len(X), t1 - t0))
+    log("training pairs: %d (%.1f s to simulate, score and fit or load)" % (len(X), t1 - t0))
     rng = np.random.default_rng(a.seed + 1)
     idx = rng.permutation(len(X))
     nval = max(1, len(X) // 20)
     va, tr = idx[:nval], idx[nval:]
     sur = S.Surrogate(ctx, seed=a.seed)
     loss = sur.fit(X[tr], Y[tr], epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed + 2)
+    zab = sur.fit_cap([setups[i] for i in tr], Y[tr])
     t2 = time.time()
     pm, pl = sur.predict(X[va])
     ok = np.isfinite(pm)
@@ -47,16 +88,29 @@ def cmd_train(a):
     err_l = float(np.sqrt(np.mean((np.log(pl[ok]) - Y[va, 1][ok]) ** 2)))
     const_u = float(np.sqrt(np.mean((Y[tr, 0].mean() - Y[va, 0]) ** 2)))
     const_l = float(np.sqrt(np.mean((Y[tr, 1].mean() - Y[va, 1]) ** 2)))
+    e13 = tail_error_at(Y[va][ok], pm[ok], pl[ok])
+    Nv = np.array([setups[i]["N"] for i in va])[ok]
+    by_n = {}
+    for lo, hi in ((20, 100), (100, 1000), (1000, 3000), (3000, 7000)):
+        s = (Nv >= lo) & (Nv < hi)
+        if s.any():
+            by_n["N%d-%d" % (lo, hi)] = dict(n=int(s.sum()), median=float(np.median(e13[s])),
+                                             p90=float(np.percentile(e13[s], 90)))
     sur.save(a.out)
     rep = dict(pairs=int(len(X)), simulate_s=t1 - t0, train_s=t2 - t1, final_loss=loss,
                val_rmse_lambda_m=err_u, val_rmse_lnlambda=err_l, const_rmse_lambda_m=const_u,
-               const_rmse_lnlambda=const_l, epochs=a.epochs, batch=a.batch, lr=a.lr)
+               const_rmse_lnlambda=const_l, val_log10E_err_at_depth13=by_n, cap_model_ab=list(zab),
+               epochs=a.epochs, batch=a.batch, lr=a.lr)
     log(json.dumps(rep))
     if a.report:
         json.dump(rep, open(a.report, "w"), indent=1)
 
 
-def eval_one(ctx, sur, rng, Nr, Lr, K=None, seed=0, Kref=200):
+def eval_one(ctx, sur, rng, Nr, Lr, K=None, seed=0, R=8):
+    """One held-out structured alignment.  The full null's E-value of a pair depends on the
+    replicates drawn (past the last survival bin it is set by the largest null score), so the
+    reference is the mean log10 E over R independent K-replicate full nulls; one full null's
+    own error is its distance from the mean of the other R - 1."""
     N = int(round(math.exp(rng.uniform(math.log(Nr[0]), math.log(Nr[1])))))
     L = int(round(math.exp(rng.uniform(math.log(Lr[0]), math.log(Lr[1])))))
     d = synth.config(0, seed=int(rng.integers(1 << 31)), N=N, L=L)
@@ -67,43 +121,40 @@ def eval_one(ctx, sur, rng, Nr, Lr, K=None, seed=0, Kref=200):
     pi, gap = S.msa_null_params(ax)
     K = K or S.nshuffle_default(N, L)
     res = dict(N=N, L=L, K=K)
-    evs, fits = [], []
-    for rep, KK in enumerate((K, K, Kref)):
-        h, sf, fit = S.full_null(ctx, tree, L, pi, gap, K=KK, seed=1000 * seed + 17 + rep)
-        evs.append(msa.evalues(h, sf, Nc)); fits.append(fit)
+    iu = np.triu_indices(L, 1)
+    lg = lambda e: np.log10(np.maximum(e, 1e-300))
+    lgs, caps, fits = [], [], []
+    for r in range(R):
+        h, sf, fit = S.full_null(ctx, tree, L, pi, gap, K=K, seed=1000 * seed + 17 + r)
+        lgs.append(lg(msa.evalues(h, sf, Nc)[iu])); caps.append(full_cap(h)); fits.append(fit)
+    lgs = np.array(lgs)
+    ref = lgs.mean(0)
+    sig = ref < math.log10(0.05)
+    res["nsig"] = int(sig.sum())
+    cov = msa.get_cov()[0][iu]
+    res["capped"] = (cov[sig] > np.median(caps)).tolist()
+    loo = np.abs(lgs - (lgs.sum(0)[None] - lgs) / (R - 1))[:, sig]
+    res["noise"] = loo.mean(0).tolist()                       # one full null vs the others, per pair
+    res["tail_full"] = [float(np.mean([f[0] for f in fits])), float(np.mean([f[1] for f in fits]))]
     f = NL.features(ctx, msa, tree)
     m, lam = sur.predict(f)
     res["in_range"] = bool(np.isfinite(m[0]))
-    iu = np.triu_indices(L, 1)
-    ef, ef2, eref = evs[0][iu], evs[1][iu], evs[2][iu]
-    sig = ef < 0.05
-    res["nsig"] = int(sig.sum())
-    lg = lambda e: np.log10(np.maximum(e[sig], 1e-300))
-    lf = lg(ef)
-    res["noise"] = np.abs(lg(ef2) - lf).tolist()             # full null vs an independent full null
-    res["full_vs_ref"] = np.abs(lf - lg(eref)).tolist()      # full null vs the Kref-replicate null
-    res["tail_full"] = [float(fits[0][0]), float(fits[0][1])]
-    res["tail_ref"] = [float(fits[2][0]), float(fits[2][1])]
-    # diagnostic: the same network fed the features of one null replicate instead of the alignment's
-    null_leaves = NL.simulate(ctx, NL.Job(tree, L, NL.gtr_model(pi), seed=4242 + seed, gapfrac=gap))
-    mn_ = ctx.msa(null_leaves)
-    fn = NL.features(ctx, mn_, tree)
-    mn_.close()
-    m2, lam2 = sur.predict(fn)
-    if np.isfinite(m2[0]):
-        es2 = NL.evalues_tail(ctx, msa, float(m2[0]), float(lam2[0]), S.P0, Nc)[iu]
-        res["err_nullfeat_ref"] = np.abs(lg(es2) - lg(eref)).tolist()
-        res["tail_sur_nullfeat"] = [float(m2[0]), float(lam2[0])]
-        res["feat_diff"] = (f - fn).tolist()
     if res["in_range"]:
-        es = NL.evalues_tail(ctx, msa, float(m[0]), float(lam[0]), S.P0, Nc)[iu]
-        res["err"] = np.abs(lg(es) - lf).tolist()
-        res["err_ref"] = np.abs(lg(es) - lg(eref)).tolist()
+        xcap = sur.cap(float(m[0]), float(lam[0]), K, Nc)
+        es = NL.evalues_tail(ctx, msa, float(m[0]), float(lam[0]), S.P0, Nc, xcap=xcap)[iu]
+        res["err"] = np.abs(lg(es[sig]) - ref[sig]).tolist()
         res["agree"] = float(np.mean((es < 0.05) == sig))
         res["tail_sur"] = [float(m[0]), float(lam[0])]
         res["err_median"] = float(np.median(res["err"])) if res["err"] else None
     msa.close()
     return res
+
+
+def full_cap(h):
+    """Score above which a full null's E-values are constant: cov2evalue reads
+    survfit[min(icov + 1, 2 nb - 1)] (src/covariation.c:2380-2381)."""
+    g = h.h
+    return g.bmin + (2 * g.nb - 2) * g.w
 
 
 def cmd_eval(a):
@@ -112,27 +163,38 @@ def cmd_eval(a):
     rng = np.random.default_rng(a.seed)
     rows = []
     for k in range(a.n):
-        r = eval_one(ctx, sur, rng, (a.Nmin, a.Nmax), (a.Lmin, a.Lmax), seed=k)
+        r = eval_one(ctx, sur, rng, (a.Nmin, a.Nmax), (a.Lmin, a.Lmax), seed=k, R=a.nulls)
         rows.append(r)
         if (k + 1) % 10 == 0:
             log("evaluated %d / %d" % (k + 1, a.n))
-    def cat(key):
-        return np.concatenate([np.asarray(r[key]) for r in rows if r.get(key)] or [np.zeros(0)])
+    def cat(key, part=None):
+        out = []
+        for r in rows:
+            if not r.get(key):
+                continue
+            v = np.asarray(r[key])
+            if part is not None:
+                c = np.asarray(r["capped"], bool)
+                v = v[c] if part == "capped" else v[~c]
+            out.append(v)
+        return np.concatenate(out) if out else np.zeros(0)
 
     def stats(v):
         if not len(v):
             return None
         return dict(median=float(np.median(v)), p90=float(np.percentile(v, 90)), frac_within_0p3=float(np.mean(v <= 0.3)))
 
-    out = dict(msas=len(rows), in_range=int(sum(r["in_range"] for r in rows)), sig_pairs=int(len(cat("err"))),
-               surrogate_vs_full=stats(cat("err")), surrogate_vs_ref=stats(cat("err_ref")),
-               surrogate_nullfeatures_vs_ref=stats(cat("err_nullfeat_ref")),
-               full_vs_full=stats(cat("noise")), full_vs_ref=stats(cat("full_vs_ref")),
+    out = dict(model=a.model, msas=len(rows), in_range=int(sum(r["in_range"] for r in rows)), sig_pairs=int(len(cat("err"))),
+               reference="mean log10 E over %d independent full nulls of R-scape's replicate count" % a.nulls,
+               surrogate_vs_full=stats(cat("err")), one_full_null=stats(cat("noise")),
+               fit_range=dict(pairs=int(len(cat("err", "fit"))), surrogate_vs_full=stats(cat("err", "fit")),
+                              one_full_null=stats(cat("noise", "fit"))),
+               capped=dict(pairs=int(len(cat("err", "capped"))), surrogate_vs_full=stats(cat("err", "capped")),
+                           one_full_null=stats(cat("noise", "capped"))),
                call_agreement=float(np.mean([r["agree"] for r in rows if r["in_range"]])) if any(r["in_range"] for r in rows) else None)
     log(json.dumps(out))
     if a.out:
-        json.dump(dict(summary=out, rows=[{k: v for k, v in r.items() if k not in ("err", "noise", "err_ref", "full_vs_ref",
-                                                                                  "err_nullfeat_ref")}
+        json.dump(dict(summary=out, rows=[{k: v for k, v in r.items() if k not in ("err", "noise", "capped")}
                                           for r in rows]),
                   open(a.out, "w"), indent=1)
 
@@ -140,16 +202,25 @@ def cmd_eval(a):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
+    g = sub.add_parser("gen")
+    g.add_argument("--n", type=int, default=100000)
+    g.add_argument("--out", required=True)
+    g.add_argument("--seed", type=int, default=1)
+    g.add_argument("--budget", type=float, default=24e9)
+    g.add_argument("--target-pairs", type=float, default=2e6)
+    g.add_argument("--big", type=float, default=0.5)
     t = sub.add_parser("train")
-    t.add_argument("--n", type=int, default=50000)
+    t.add_argument("--data", default=None)
+    t.add_argument("--n", type=int, default=100000)
+    t.add_argument("--big", type=float, default=0.5)
     t.add_argument("--out", default=S.DATA)
     t.add_argument("--report", default=None)
-    t.add_argument("--epochs", type=int, default=60)
-    t.add_argument("--batch", type=int, default=512)
-    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--epochs", type=int, default=120)
+    t.add_argument("--batch", type=int, default=256)
+    t.add_argument("--lr", type=float, default=3e-4)
     t.add_argument("--seed", type=int, default=1)
     t.add_argument("--budget", type=float, default=24e9)
-    t.add_argument("--target-pairs", type=float, default=2e5)
+    t.add_argument("--target-pairs", type=float, default=2e6)
     e = sub.add_parser("eval")
     e.add_argument("--model", default=S.DATA)
     e.add_argument("--n", type=int, default=100)
@@ -158,9 +229,10 @@ def main(argv=None):
     e.add_argument("--Nmax", type=int, default=1000)
     e.add_argument("--Lmin", type=int, default=50)
     e.add_argument("--Lmax", type=int, default=400)
+    e.add_argument("--nulls", type=int, default=8)
     e.add_argument("--out", default=None)
     a = ap.parse_args(argv)
-    {"train": cmd_train, "eval": cmd_eval}[a.cmd](a)
+    {"gen": cmd_gen, "train": cmd_train, "eval": cmd_eval}[a.cmd](a)
 
 
 if __name__ == "__main__":

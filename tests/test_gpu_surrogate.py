@@ -120,6 +120,13 @@ def test_evalues_tail(ctx):
     off = ~np.eye(60, dtype=bool)
     assert np.allclose(ev[off], ref[off], rtol=1e-12)
     assert np.all(np.isinf(np.diag(ev)))
+    # a cap inside the score range: every score at or above it gets the E-value at the cap
+    xcap = float(np.percentile(cov[off], 99))
+    evc = NL.evalues_tail(ctx, msa, 5.0, 0.4, 1e-3, xcap=xcap)
+    refc = Nc * np.minimum(1.0, 1e-3 * np.exp(-0.4 * (np.minimum(cov, xcap) - 5.0)))
+    assert np.allclose(evc[off], refc[off], rtol=1e-12)
+    top = off & (cov >= xcap)
+    assert top.sum() >= 10 and np.all(evc[top] == evc[top][0])
 
 
 def test_surrogate_beats_constant_tail(ctx):
