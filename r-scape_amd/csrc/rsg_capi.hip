@@ -21,6 +21,8 @@ extern "C" rsg_ctx *rsg_ctx_create(int device, char *errbuf)
   hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->d_flag, 64) != hipSuccess || hipMalloc(&c->d_small, 65536) != hipSuccess ||
+      hipMalloc(&c->d_res, 8 * sizeof(double)) != hipSuccess ||
+      hipMemset(c->d_res, 0, 8 * sizeof(double)) != hipSuccess ||
       hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     rsg_fail(errbuf, "context allocation failed");
     delete c;
@@ -40,6 +42,7 @@ extern "C" void rsg_ctx_destroy(rsg_ctx *c)
   if (c->d_gscr) hipFree(c->d_gscr);
   if (c->d_arena) hipFree(c->d_arena);
   if (c->d_flag) hipFree(c->d_flag);
+  if (c->d_res) hipFree(c->d_res);
   if (c->d_small) hipFree(c->d_small);
   if (c->h_pinned) hipHostFree(c->h_pinned);
   hipStreamDestroy(c->stream);
@@ -175,9 +178,9 @@ extern "C" int rsg_corr_Probs(rsg_ctx *c, rsg_msa *m, int mode, double tol, char
     if (rsg_gram_counts_bf16(c, m, mode == RSG_COUNTS_BF16X3 ? 3 : 2, (float *)m->d_cnt, errbuf) != RSG_OK) return RSG_FAIL;
   } else return rsg_fail(errbuf, "unknown counts mode %d", mode);
   if (rsg_probs_from_counts(c, m, tol, errbuf) != RSG_OK) return RSG_FAIL;
-  HIPCHK(hipMemcpyAsync(c->h_pinned + 4, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   m->have_probs = 1;
   if (c->defer) return RSG_OK;                 // checked by the deferring caller
+  HIPCHK(hipMemcpyAsync(c->h_pinned + 4, c->d_res + 4, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (*(int *)(c->h_pinned + 4)) { m->have_probs = 0; return rsg_fail(errbuf, "pm validation failed"); }
   return RSG_OK;
@@ -298,17 +301,21 @@ extern "C" int rsg_cov_Calculate(rsg_ctx *c, rsg_msa *m, int covtype, int covcla
                                  double *minCOV, double *maxCOV, char *errbuf)
 {
   int base = covtype - covtype % 3, corr = covtype % 3;
-  // the whole chain is queued with one synchronisation at the end: the pm validation flag,
-  // the correction's NaN flag and the final min / max arrive together
-  *(int *)(c->h_pinned + 4) = 0; *(int *)(c->h_pinned + 2) = 0;
+  // the whole chain is queued with one synchronisation at the end: the kernels leave the pm
+  // validation flag, the correction's NaN flag, COVavg and the final min / max in d_res, and
+  // one copy brings them to h_pinned
   if (rsg_reserve_scratch(c, m->L, errbuf) != RSG_OK) return RSG_FAIL;
   c->defer = 1;
+  c->skip_minmax = (corr != 0);
   int st = RSG_OK;
   if (base != RSG_RAF && base != RSG_RAFS) st = rsg_corr_Probs(c, m, counts_mode, tol, errbuf);
   if (st == RSG_OK) st = rsg_corr_CalculateCOV(c, m, base, covclass, allowpair, nseqthresh, alenthresh, prec, errbuf);
+  c->skip_minmax = 0;
   if (st == RSG_OK && corr == 1) st = rsg_corr_CalculateCOVCorrected(c, m, RSG_APC, 0, errbuf);
   if (st == RSG_OK && corr == 2) st = rsg_corr_CalculateCOVCorrected(c, m, RSG_ASC, 0, errbuf);
   c->defer = 0;
+  if (st == RSG_OK && hipMemcpyAsync(c->h_pinned, c->d_res, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    st = rsg_fail(errbuf, "rsg_cov_Calculate: result copy failed");
   if (hipStreamSynchronize(c->stream) != hipSuccess && st == RSG_OK) st = rsg_fail(errbuf, "rsg_cov_Calculate: device failure");
   if (st != RSG_OK) { m->have_probs = m->have_cov = 0; return st; }
   if (base != RSG_RAF && base != RSG_RAFS && *(int *)(c->h_pinned + 4)) {

@@ -51,12 +51,16 @@ struct rsg_ctx {
   int          gram_shape = 0;          // 0 = automatic
   int          gram_split = 1;          // split the last partial wave of Gram tiles along K
   int          defer = 0;               // queue status/min/max copies; the caller synchronises once
+  int          skip_minmax = 0;         // a correction follows: the statistic's min / max are not needed
   int          ncu = 0;                 // compute units of the device
   void        *d_gscr = nullptr;  size_t gscr_cap = 0;   // Gram split partials
   void        *d_arena = nullptr; size_t arena_cap = 0;  // batched-null buffers (grow-only)
   // small reusable device scratch (reductions)
   double      *d_red = nullptr;  size_t red_cap = 0;
   int         *d_flag = nullptr;
+  // results of one scoring call, in the layout of h_pinned[0..4]: min, max, correction NaN flag
+  // (int), COVavg, marginal validation flag (int); one copy brings them to the host
+  double      *d_res = nullptr;
   double      *h_pinned = nullptr;      // 64 doubles of pinned host staging
   void        *d_small = nullptr;       // 64 KB of device scratch for small per-call tables
 };
