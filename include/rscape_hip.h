@@ -36,7 +36,8 @@ enum { RSG_APC = 0, RSG_ASC = 1 };
 
 /* How pair counts are formed. */
 enum {
-  RSG_COUNTS_INT8  = 0,   /* unit weights: int8 one-hot Gram, int32 accumulation, bit-exact */
+  RSG_COUNTS_INT8  = 0,   /* unit weights: int8 one-hot Gram, int32 accumulation, bit-exact;
+                             uint16 counts kept, so at most 65535 sequences */
   RSG_COUNTS_BF16  = 1,   /* sequence weights: split-BF16 (hi+lo) Gram, FP32 accumulation  */
   RSG_COUNTS_BF16X3 = 2   /* sequence weights: 3-term BF16 split, FP32 accumulation        */
 };
@@ -178,7 +179,7 @@ int rsg_gtr_cdf(rsg_ctx *ctx, const double *model, int64_t nt, const double *t, 
 int rsg_tree_yule(int64_t n, double mean_bl, uint64_t seed, uint32_t stream,
                   int *left, int *right, double *ld, double *rd);
 typedef struct {
-  int64_t  nleaf;            /* N sequences */
+  int64_t  nleaf;            /* N sequences (rsg_null_batch: at most 65535) */
   int64_t  alen;             /* L columns */
   const int *left, *right;   /* N-1 internal nodes */
   const double *ld, *rd;     /* branch lengths */

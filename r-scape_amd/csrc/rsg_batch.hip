@@ -64,7 +64,7 @@ k_pack_b(const int64_t *__restrict__ desc, int n, const uint8_t *__restrict__ le
 // corr_Marginals per item with the single-alignment kernels' tiles and reduction order
 // (pm_tile / pm_reduce_col), so a replicate's marginals match its rsg_corr_Probs bit for bit
 __global__ void __launch_bounds__(64 * PMW)
-k_pm_tile_b(const int64_t *__restrict__ desc, int n, const uint32_t *__restrict__ cnt, double *__restrict__ part)
+k_pm_tile_b(const int64_t *__restrict__ desc, int n, const uint16_t *__restrict__ cnt, double *__restrict__ part)
 {
   const int it = find_item(desc, n, D_BPMT, blockIdx.x);
   const int64_t *d = desc + it * RSG_BDESC;
@@ -72,7 +72,7 @@ k_pm_tile_b(const int64_t *__restrict__ desc, int n, const uint32_t *__restrict_
   const int bi = (int)(local / nbt), bj = (int)(local % nbt);
   if (bi > bj) return;
   double *pi = part + d[D_PMP];
-  pm_tile<CNT_I8G>(cnt + d[D_CNT], L, (int)nbt, bi, bj, pi, pi + pm_tri(nbt, nbt, nbt) * PMT * 4);
+  pm_tile<CNT_U16>(cnt + d[D_CNT], L, (int)nbt, bi, bj, pi, pi + pm_tri(nbt, nbt, nbt) * PMT * 4);
 }
 
 __global__ void __launch_bounds__(256)
@@ -90,7 +90,7 @@ k_pm_reduce_b(const int64_t *__restrict__ desc, int n, const double *__restrict_
 }
 
 __global__ void __launch_bounds__(256)
-k_stat_b(const int64_t *__restrict__ desc, int n, StatArgs A0, const uint32_t *__restrict__ cnt,
+k_stat_b(const int64_t *__restrict__ desc, int n, StatArgs A0, const uint16_t *__restrict__ cnt,
          const double *__restrict__ pm, double *__restrict__ cov, const int *__restrict__ cls, int base)
 {
   const int it = find_item(desc, n, D_BSTAT, blockIdx.x);
@@ -99,7 +99,7 @@ k_stat_b(const int64_t *__restrict__ desc, int n, StatArgs A0, const uint32_t *_
   StatArgs A = A0;
   A.cnt = cnt + d[D_CNT]; A.pm = pm + d[D_PM]; A.lpm = pm + d[D_PM] + 4 * L; A.L = L;
   A.wsum = (double)d[D_N]; A.cov = cov + d[D_COV]; A.type = base; A.cls = cls[it];
-  stat_tile<double, CNT_I8G, true>(A, (int)(local % nb), (int)(local / nb), (int64_t)blockIdx.x);
+  stat_tile<double, CNT_U16, true>(A, (int)(local % nb), (int)(local / nb), (int64_t)blockIdx.x);
 }
 
 __global__ void __launch_bounds__(256)
@@ -339,6 +339,7 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   for (int k = 0; k < nj; k++) {
     const rsg_null_job &J = jobs[k];
     if (J.nleaf < 2 || J.alen < 2) return rsg_fail(errbuf, "job %d: needs >= 2 sequences and >= 2 columns", k);
+    if (J.nleaf > RSG_U16_MAXN) return rsg_fail(errbuf, "job %d: more than %d sequences", k, RSG_U16_MAXN);
     const int64_t N = J.nleaf, L = J.alen, Npad = rsg_npad(N), Lpad = ((L + 63) / 64) * 64;
     int64_t *d = &desc[(size_t)k * RSG_BDESC];
     d[D_N] = N; d[D_L] = L; d[D_NPAD] = Npad; d[D_LPAD] = Lpad;
@@ -399,7 +400,7 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   // ---- device buffers: one grow-only arena of the context, carved in two passes
   int64_t *d_desc; SimItem *d_sim; int *d_cls; int4 *d_tiles, *d_btiles; int *d_left, *d_right;
   double *d_ld, *d_rd, *d_models; int4 *d_ent; int *d_gfirst; int64_t *d_fitems; unsigned long long *d_fcnt;
-  uint32_t *d_cdf; uint8_t *d_seqs, *d_leaves, *d_xt; uint32_t *d_cnt; double *d_cov, *d_pm, *d_rows, *d_lgc;
+  uint32_t *d_cdf; uint8_t *d_seqs, *d_leaves, *d_xt; uint16_t *d_cnt; double *d_cov, *d_pm, *d_rows, *d_lgc;
   double *d_part_s, *d_part_e, *d_mm, *d_pmp; HGeo *d_geo; unsigned long long *d_hist, *d_S; int *d_NZ; double *d_fits;
   auto carve = [&](Arena &A) {
     d_desc = A.get<int64_t>(desc.size()); d_sim = A.get<SimItem>((size_t)nj); d_cls = A.get<int>((size_t)nj);
@@ -411,7 +412,7 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     d_fcnt = A.get<unsigned long long>((size_t)ng * 36);
     d_cdf = A.get<uint32_t>((size_t)nodes * 24); d_seqs = A.get<uint8_t>((size_t)seqs);
     d_leaves = A.get<uint8_t>((size_t)o_leaf); d_xt = A.get<uint8_t>((size_t)o_xt);
-    d_cnt = A.get<uint32_t>((size_t)o_cnt); d_cov = A.get<double>((size_t)o_cov); d_pm = A.get<double>((size_t)o_pm);
+    d_cnt = A.get<uint16_t>((size_t)o_cnt); d_cov = A.get<double>((size_t)o_cov); d_pm = A.get<double>((size_t)o_pm);
     d_rows = A.get<double>((size_t)o_rows); d_lgc = A.get<double>(2 * (size_t)(nmax + 1));
     d_part_s = A.get<double>(2 * (size_t)b_stat); d_part_e = A.get<double>(2 * (size_t)b_el);
     d_mm = A.get<double>(2 * (size_t)nj); d_pmp = A.get<double>((size_t)o_pmp); d_geo = A.get<HGeo>((size_t)ng);

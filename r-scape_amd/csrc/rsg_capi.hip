@@ -153,8 +153,7 @@ extern "C" void rsg_msa_destroy(rsg_ctx *c, rsg_msa *m)
 
 static int alloc_counts(rsg_msa *m, void **dst, int kind, char *errbuf)
 {
-  (void)kind;                                       // both kinds use 4-byte cells
-  size_t bytes = 4 * 16 * (size_t)m->L * (size_t)m->L;
+  const size_t bytes = (size_t)rsg_rec_bytes(kind) * (size_t)m->L * (size_t)m->L;
   if (*dst && m->cnt_bytes >= bytes && dst == &m->d_cnt) return RSG_OK;
   if (*dst) hipFree(*dst);
   *dst = nullptr;
@@ -169,9 +168,10 @@ extern "C" int rsg_corr_Probs(rsg_ctx *c, rsg_msa *m, int mode, double tol, char
   m->have_probs = m->have_cov = m->have_eval = 0;
   if (mode == RSG_COUNTS_INT8) {
     if (m->weighted) return rsg_fail(errbuf, "int8 counts need unit weights (alignment has weights)");
-    if (alloc_counts(m, &m->d_cnt, CNT_I8G, errbuf) != RSG_OK) return RSG_FAIL;
-    m->cnt_kind = CNT_I8G;
-    if (rsg_gram_counts_i8(c, m, m->d_cnt, CNT_I8G, errbuf) != RSG_OK) return RSG_FAIL;
+    const int kind = rsg_int_kind(m->N);
+    if (alloc_counts(m, &m->d_cnt, kind, errbuf) != RSG_OK) return RSG_FAIL;
+    m->cnt_kind = kind;
+    if (rsg_gram_counts_i8(c, m, m->d_cnt, kind, errbuf) != RSG_OK) return RSG_FAIL;
   } else if (mode == RSG_COUNTS_BF16 || mode == RSG_COUNTS_BF16X3) {
     if (alloc_counts(m, &m->d_cnt, CNT_F32, errbuf) != RSG_OK) return RSG_FAIL;
     m->cnt_kind = CNT_F32;
@@ -190,10 +190,10 @@ extern "C" int rsg_corr_GetCounts(rsg_ctx *c, rsg_msa *m, double *cnt, char *err
 {
   if (!m->d_cnt) return rsg_fail(errbuf, "no counts");
   const int64_t L = m->L;
-  const size_t n = 16 * (size_t)L * (size_t)L;
-  std::vector<uint8_t> h(n * 4);
+  const size_t bytes = (size_t)rsg_rec_bytes(m->cnt_kind) * (size_t)L * (size_t)L;
+  std::vector<uint8_t> h(bytes);
   HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpy(h.data(), m->d_cnt, n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(h.data(), m->d_cnt, bytes, hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < L; i++)
     for (int64_t j = 0; j < L; j++)
       for (int k = 0; k < 16; k++) {
@@ -201,7 +201,8 @@ extern "C" int rsg_corr_GetCounts(rsg_ctx *c, rsg_msa *m, double *cnt, char *err
         size_t r = (size_t)rsg_rec(i, j, L) * 16 + k;
         double v = 0.;
         if (i < j) {
-          if (m->cnt_kind == CNT_I8G) v = (double)(((uint32_t *)h.data())[r] >> rsg_cell_shift(k));
+          if (m->cnt_kind == CNT_U16) v = (double)((const uint16_t *)h.data())[r];
+          else if (m->cnt_kind == CNT_U32) v = (double)((const uint32_t *)h.data())[r];
           else                        v = ((float *)h.data())[r];
         }
         cnt[o] = v;
@@ -273,9 +274,9 @@ extern "C" int rsg_corr_CalculateRAF(rsg_ctx *c, rsg_msa *m, int rafs, const dou
   const double *allow = allowpair ? allowpair : kAllowWCGU;
   // RAF is defined on unweighted symbols (src/correlators.c:892-921) and, like the reference,
   // does not go through corr_Probs: form the unit-weight counts here.
-  if (!m->d_cnt_u) HIPCHK(hipMalloc(&m->d_cnt_u, 4 * 16 * (size_t)m->L * (size_t)m->L));
-  m->cnt_u_kind = CNT_I8G;
-  if (rsg_gram_counts_i8(c, m, m->d_cnt_u, CNT_I8G, errbuf) != RSG_OK) return RSG_FAIL;
+  m->cnt_u_kind = rsg_int_kind(m->N);
+  if (!m->d_cnt_u) HIPCHK(hipMalloc(&m->d_cnt_u, (size_t)rsg_rec_bytes(m->cnt_u_kind) * (size_t)m->L * (size_t)m->L));
+  if (rsg_gram_counts_i8(c, m, m->d_cnt_u, m->cnt_u_kind, errbuf) != RSG_OK) return RSG_FAIL;
   return rsg_raf(c, m, rafs, allow, errbuf);
 }
 

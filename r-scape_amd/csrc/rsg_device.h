@@ -5,14 +5,19 @@
 
 template <int KIND> __device__ __forceinline__ void load_rec(const void *cnt, int64_t rec, double *c)
 {
-  if (KIND == CNT_I8G) {
+  if (KIND == CNT_U16) {
+    const uint4 *p = (const uint4 *)((const uint16_t *)cnt + rec * 16);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const uint4 v = p[h];                         // cells 8h .. 8h+7, two per word
+      const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int q = 0; q < 4; q++) { c[8 * h + 2 * q] = (double)(w[q] & 0xFFFFu); c[8 * h + 2 * q + 1] = (double)(w[q] >> 16); }
+    }
+  } else if (KIND == CNT_U32) {
     const uint4 *p = (const uint4 *)((const uint32_t *)cnt + rec * 16);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      uint4 v = p[k];                               // row x = k: cells y = 0..3, scale 2^(x+y)
-      c[4 * k] = (double)(v.x >> (k + 0)); c[4 * k + 1] = (double)(v.y >> (k + 1));
-      c[4 * k + 2] = (double)(v.z >> (k + 2)); c[4 * k + 3] = (double)(v.w >> (k + 3));
-    }
+    for (int k = 0; k < 4; k++) { const uint4 v = p[k]; c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w; }
   } else {
     const float4 *p = (const float4 *)((const float *)cnt + rec * 16);
 #pragma unroll

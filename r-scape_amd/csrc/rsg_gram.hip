@@ -26,7 +26,8 @@
 // order in which the hardware pairs K values need not be known.
 //
 // Output: a 16-cell record per column pair i < j at record rsg_rec(i, j) = j*L + i,
-// cell x*4+y (the layout of pp[i][j][IDX(x,y,4)] in the reference).
+// cell x*4+y (the layout of pp[i][j][IDX(x,y,4)] in the reference); unit-weight records hold
+// uint16 counts (N <= 65535), weighted ones FP32.
 #include "rsg_internal.h"
 #include <map>
 #include <tuple>
@@ -104,16 +105,22 @@ struct CodeStage {
     *(uint4 *)p_ = pre[r_];                                                                    \
   }
 
+// Cell (x, y) of an accumulator holds count << (x + y); two cells, scale removed, per word.
+__device__ __forceinline__ unsigned pack_cells(int a, int sa, int b, int sb)
+{
+  return ((unsigned)a >> sa) | (((unsigned)b >> sb) << 16);
+}
+
 // int8 Gram: WI x WJ waves, each 32 i-columns x 32 j-columns, KT sequences per stage.
-// Records are written as the raw accumulators (count << (x+y)), so the accumulators
-// never pass through VALU and stay in AGPRs.
+// The accumulators stay in AGPRs through the K loop; the epilogue shifts the symbol scale
+// out and writes uint16 (OK = CNT_U16) or uint32 (CNT_U32) records.
 // One work-group tile (i0, j0) of one alignment over K-tiles [kt0, kt1); `lds` holds
-// 2 * St::BYTES.  With part == nullptr the records go to `out`; otherwise the tile's partial
-// sums go, unconditionally, to part[(jj * BI + ii) * 16 + cell] (ii, jj: column offsets in
-// the tile) for a later reduction.
-template <int WI, int WJ, int KT>
+// 2 * St::BYTES.  With part == nullptr the records go to `out`; otherwise the tile's raw
+// partial sums go, unconditionally, to part[(jj * BI + ii) * 16 + cell] (ii, jj: column
+// offsets in the tile) for a later reduction.
+template <int WI, int WJ, int KT, int OK>
 __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L,
-                                             uint32_t *__restrict__ out, int64_t i0, int64_t j0, uint8_t *lds,
+                                             void *__restrict__ out, int64_t i0, int64_t j0, uint8_t *lds,
                                              int64_t kt0, int64_t kt1, uint32_t *__restrict__ part)
 {
   constexpr int BI = 32 * WI, BJ = 32 * WJ, NT = 64 * WI * WJ, KS = KT / 32;
@@ -172,21 +179,37 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
   }
 
   // epilogue: lane owns i column `col`; register r holds j = 8*(r/4) + 4*khalf + r%4.
-  // Both destinations are affine in (ii, jj): base + jj * sj + ii * 16 words.
+  // Both destinations are affine in (ii, jj): base + jj * sj + ii * 16 cells.
   const int ii = wi * 32 + col;
   const int64_t i = i0 + ii;
-  const bool all = part != nullptr;
-  uint32_t *const base = all ? part : out + rsg_rec(i0, j0, L) * 16;
-  const int64_t sj = all ? (int64_t)BI * 16 : L * 16;
+  if (part) {
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int jj = wj * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
+      uint4 *p = (uint4 *)(part + ((int64_t)jj * BI + ii) * 16);
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+        p[x] = make_uint4(acc[x][0][r], acc[x][1][r], acc[x][2][r], acc[x][3][r]);
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     const int jj = wj * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
     const int64_t j = j0 + jj;
-    if (all || (i < j && j < L)) {
-      uint4 *p = (uint4 *)(base + jj * sj + ii * 16);
+    if (!(i < j && j < L)) continue;
+    if (OK == CNT_U16) {
+      uint4 *p = (uint4 *)((uint16_t *)out + rsg_rec(i, j, L) * 16);
+      p[0] = make_uint4(pack_cells(acc[0][0][r], 0, acc[0][1][r], 1), pack_cells(acc[0][2][r], 2, acc[0][3][r], 3),
+                        pack_cells(acc[1][0][r], 1, acc[1][1][r], 2), pack_cells(acc[1][2][r], 3, acc[1][3][r], 4));
+      p[1] = make_uint4(pack_cells(acc[2][0][r], 2, acc[2][1][r], 3), pack_cells(acc[2][2][r], 4, acc[2][3][r], 5),
+                        pack_cells(acc[3][0][r], 3, acc[3][1][r], 4), pack_cells(acc[3][2][r], 5, acc[3][3][r], 6));
+    } else {
+      uint4 *p = (uint4 *)((uint32_t *)out + rsg_rec(i, j, L) * 16);
 #pragma unroll
       for (int x = 0; x < 4; x++)
-        p[x] = make_uint4(acc[x][0][r], acc[x][1][r], acc[x][2][r], acc[x][3][r]);
+        p[x] = make_uint4((unsigned)acc[x][0][r] >> x, (unsigned)acc[x][1][r] >> (x + 1),
+                          (unsigned)acc[x][2][r] >> (x + 2), (unsigned)acc[x][3][r] >> (x + 3));
     }
   }
 }
@@ -195,9 +218,9 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
 // groups for each of the remaining ntiles - ndp tiles, each over one K segment, writing
 // partial sums for k_gram_reduce.  The split keeps the last, partial wave of tiles from
 // leaving most CUs idle.
-template <int WI, int WJ, int KT>
+template <int WI, int WJ, int KT, int OK>
 __global__ void __launch_bounds__(64 * WI * WJ)
-k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, uint32_t *__restrict__ out,
+k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, void *__restrict__ out,
           const int2 *__restrict__ tiles, int ndp, int per_xcd, int ntiles, int nsplit,
           uint32_t *__restrict__ part)
 {
@@ -220,14 +243,15 @@ k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, uint32_t *__r
     p = part + (int64_t)u * (32 * WI) * (32 * WJ) * 16;
   }
   const int2 tl = tiles[t];
-  gram_tile_i8<WI, WJ, KT>(xt, Npad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, k0, k1, p);
+  gram_tile_i8<WI, WJ, KT, OK>(xt, Npad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, k0, k1, p);
 }
 
-// Sum the nsplit partial tiles of each split tile into its records (integer sums: exact).
-template <int BI, int BJ>
+// Sum the nsplit partial tiles of each split tile into its records (integer sums: exact);
+// one thread per (pair, symbol row x): four cells, scale 2^(x+y) removed.
+template <int BI, int BJ, int OK>
 __global__ void __launch_bounds__(256)
 k_gram_reduce(const uint32_t *__restrict__ part, const int2 *__restrict__ tiles, int ndp, int nsplit, int64_t L,
-              uint32_t *__restrict__ out)
+              void *__restrict__ out)
 {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;       // (split tile, pair, quarter)
   const int64_t per_tile = (int64_t)BI * BJ * 4;
@@ -244,7 +268,12 @@ k_gram_reduce(const uint32_t *__restrict__ part, const int2 *__restrict__ tiles,
     const uint4 v = p[(int64_t)k * BI * BJ * 4];
     s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
   }
-  ((uint4 *)(out + rsg_rec(i, j, L) * 16))[quarter] = s;
+  const int x = quarter;
+  if (OK == CNT_U16)
+    ((uint2 *)((uint16_t *)out + rsg_rec(i, j, L) * 16))[x] = make_uint2(pack_cells((int)s.x, x, (int)s.y, x + 1),
+                                                                         pack_cells((int)s.z, x + 2, (int)s.w, x + 3));
+  else
+    ((uint4 *)((uint32_t *)out + rsg_rec(i, j, L) * 16))[x] = make_uint4(s.x >> x, s.y >> (x + 1), s.z >> (x + 2), s.w >> (x + 3));
 }
 
 // Batched int8 Gram over many alignments: one work group per (item, tile); tiles are
@@ -253,18 +282,18 @@ k_gram_reduce(const uint32_t *__restrict__ part, const int2 *__restrict__ tiles,
 template <int W, int KT>
 __global__ void __launch_bounds__(64 * W * W)
 k_gram_i8_batch(const int4 *__restrict__ tiles, const int64_t *__restrict__ desc, const uint8_t *__restrict__ xt,
-                uint32_t *__restrict__ cnt)
+                uint16_t *__restrict__ cnt)
 {
   typedef CodeStage<32 * W, 32 * W, KT, 64 * W * W> St;
   __shared__ __attribute__((aligned(16))) uint8_t lds[2 * St::BYTES];
   const int4 tl = tiles[blockIdx.x];                    // (item, bi, bj, -)
   const int64_t *d = desc + (int64_t)tl.x * RSG_BDESC;  // item descriptor (see rsg_batch.hip)
-  gram_tile_i8<W, W, KT>(xt + d[4], d[2], d[1], cnt + d[5], (int64_t)tl.y * 32 * W, (int64_t)tl.z * 32 * W, lds,
+  gram_tile_i8<W, W, KT, CNT_U16>(xt + d[4], d[2], d[1], cnt + d[5], (int64_t)tl.y * 32 * W, (int64_t)tl.z * 32 * W, lds,
                          0, d[2] / KT, nullptr);
 }
 
 int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_big, int nbig, const int4 *d_small, int nsmall,
-                          const int64_t *d_desc, const uint8_t *d_xt, uint32_t *d_cnt)
+                          const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt)
 {
   if (nbig > 0)
     hipLaunchKernelGGL((k_gram_i8_batch<2, 256>), dim3((unsigned)nbig), dim3(256), 0, c->stream, d_big, d_desc, d_xt, d_cnt);
@@ -435,8 +464,8 @@ static const int2 *tile_list(int device, int64_t L, int BI, int BJ, int *ntiles,
   return d;
 }
 
-template <int WI, int WJ, int KT>
-static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, char *errbuf)
+template <int WI, int WJ, int KT, int OK>
+static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, char *errbuf)
 {
   constexpr int BI = 32 * WI, BJ = 32 * WJ;
   if (m->Lpad % BI || m->Lpad % BJ || m->Npad % KT) return rsg_fail(errbuf, "gram: padding does not fit the tile");
@@ -444,7 +473,7 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, char *errbuf)
   const int2 *tiles = tile_list(c->device, m->L, BI, BJ, &ntiles, errbuf);
   if (!tiles) return RSG_FAIL;
   if (ntiles == 0) return RSG_OK;
-  (void)kind;
+  if (OK == CNT_U16 && m->N > RSG_U16_MAXN) return rsg_fail(errbuf, "gram: uint16 records hold up to %d sequences", RSG_U16_MAXN);
   // whole waves of tiles run over the full K range; the last partial wave is split along K
   // into enough segments to occupy every CU, then reduced
   const int ncu = c->ncu > 0 ? c->ncu : 256;
@@ -470,12 +499,12 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, char *errbuf)
   const int per = (ndp + 7) / 8;
   dim3 grid((unsigned)(8 * per + nrem * nsplit)), block(64 * WI * WJ);
   int t; rsg_tstart(c, "gram_i8", &t);
-  hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->L, (uint32_t *)out,
+  hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->L, out,
                      tiles, ndp, per, ntiles, nsplit, part);
   if (nrem > 0) {
     const int64_t nq = (int64_t)nrem * BI * BJ * 4;
-    hipLaunchKernelGGL((k_gram_reduce<BI, BJ>), dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, c->stream, part, tiles,
-                       ndp, nsplit, m->L, (uint32_t *)out);
+    hipLaunchKernelGGL((k_gram_reduce<BI, BJ, OK>), dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, c->stream, part, tiles,
+                       ndp, nsplit, m->L, out);
   }
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
@@ -493,13 +522,16 @@ static int pick_shape(const rsg_ctx *c, const rsg_msa *m)
 
 int rsg_gram_counts_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, char *errbuf)
 {
+  if (kind == CNT_U32)            // more than 65535 sequences: the two default shapes only
+    return (m->L >= 768) ? launch_i8<2, 2, 256, CNT_U32>(c, m, out, errbuf) : launch_i8<1, 1, 128, CNT_U32>(c, m, out, errbuf);
+  if (kind != CNT_U16) return rsg_fail(errbuf, "gram: bad record kind %d", kind);
   switch (pick_shape(c, m)) {
-  case 1: return launch_i8<2, 2, 128>(c, m, out, kind, errbuf);
-  case 2: return launch_i8<2, 2, 64>(c, m, out, kind, errbuf);
-  case 3: return launch_i8<2, 2, 256>(c, m, out, kind, errbuf);
-  case 4: return launch_i8<1, 1, 128>(c, m, out, kind, errbuf);
-  case 5: return launch_i8<2, 1, 128>(c, m, out, kind, errbuf);
-  case 6: return launch_i8<1, 2, 128>(c, m, out, kind, errbuf);
+  case 1: return launch_i8<2, 2, 128, CNT_U16>(c, m, out, errbuf);
+  case 2: return launch_i8<2, 2, 64, CNT_U16>(c, m, out, errbuf);
+  case 3: return launch_i8<2, 2, 256, CNT_U16>(c, m, out, errbuf);
+  case 4: return launch_i8<1, 1, 128, CNT_U16>(c, m, out, errbuf);
+  case 5: return launch_i8<2, 1, 128, CNT_U16>(c, m, out, errbuf);
+  case 6: return launch_i8<1, 2, 128, CNT_U16>(c, m, out, errbuf);
   }
   return rsg_fail(errbuf, "bad gram shape");
 }

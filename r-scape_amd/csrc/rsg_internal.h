@@ -34,11 +34,14 @@ __host__ __device__ static inline uint32_t rsg_philox(uint64_t seed, uint32_t st
 }
 
 // Kinds of stored pair counts.
-//   CNT_I8G: int8-Gram records, cell x*4+y holds count << (x + y) (the MFMA accumulator
-//            as produced; readers shift the scale out), 64 bytes per record.
+//   CNT_U16: int8-Gram records, 16 uint16 counts (cell x*4+y), 32 bytes per record; for
+//            N <= RSG_U16_MAXN sequences.
+//   CNT_U32: int8-Gram records for larger N, 16 uint32 counts, 64 bytes per record.
 //   CNT_F32: split-BF16 weighted records, FP32 cells, 64 bytes per record.
-enum { CNT_I8G = 1, CNT_F32 = 2 };
-__host__ __device__ static inline int rsg_cell_shift(int k) { return (k >> 2) + (k & 3); }
+enum { CNT_U16 = 1, CNT_F32 = 2, CNT_U32 = 3 };
+#define RSG_U16_MAXN 65535
+static inline int rsg_int_kind(int64_t N) { return N <= RSG_U16_MAXN ? CNT_U16 : CNT_U32; }
+__host__ __device__ static inline int rsg_rec_bytes(int kind) { return kind == CNT_U16 ? 32 : 64; }
 
 struct rsg_timer { const char *name; hipEvent_t a, b; };
 
@@ -180,4 +183,4 @@ void rsg_feat_finish(const unsigned long long *cnt, int64_t N, int64_t L, const 
 // others 32 x 32-column tiles (one wave, 64-sequence K tiles).
 static inline bool rsg_batch_big_tiles(int64_t N, int64_t L) { return L >= 256 && N >= 256; }
 int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_big, int nbig, const int4 *d_small, int nsmall,
-                          const int64_t *d_desc, const uint8_t *d_xt, uint32_t *d_cnt);
+                          const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt);

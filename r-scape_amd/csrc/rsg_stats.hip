@@ -105,8 +105,11 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
   if (ensure_red(c, 2 * npart, errbuf) != RSG_OK) return RSG_FAIL;
   double *part_i = c->d_red, *part_j = c->d_red + npart;
   int t; rsg_tstart(c, "marginals", &t);
-  if (m->cnt_kind == CNT_I8G)
-    hipLaunchKernelGGL(k_pm_tile<CNT_I8G>, dim3((unsigned)nbt, (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
+  if (m->cnt_kind == CNT_U16)
+    hipLaunchKernelGGL(k_pm_tile<CNT_U16>, dim3((unsigned)nbt, (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
+                       (int *)(c->d_res + 4));
+  else if (m->cnt_kind == CNT_U32)
+    hipLaunchKernelGGL(k_pm_tile<CNT_U32>, dim3((unsigned)nbt, (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
                        (int *)(c->d_res + 4));
   else
     hipLaunchKernelGGL(k_pm_tile<CNT_F32>, dim3((unsigned)nbt, (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
@@ -144,8 +147,8 @@ __global__ void __launch_bounds__(256) k_stat(StatArgs A, int nb)
 template <int TYPE, bool TAB>
 static bool launch_stat_fixed(rsg_ctx *c, const StatArgs &A, int cls, dim3 grid, int nb)
 {
-  if (cls == RSG_C16) hipLaunchKernelGGL((k_stat<double, CNT_I8G, TAB, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
-  else if (cls == RSG_C2) hipLaunchKernelGGL((k_stat<double, CNT_I8G, TAB, TYPE, RSG_C2>), grid, dim3(256), 0, c->stream, A, nb);
+  if (cls == RSG_C16) hipLaunchKernelGGL((k_stat<double, CNT_U16, TAB, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
+  else if (cls == RSG_C2) hipLaunchKernelGGL((k_stat<double, CNT_U16, TAB, TYPE, RSG_C2>), grid, dim3(256), 0, c->stream, A, nb);
   else return false;
   return true;
 }
@@ -224,7 +227,8 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
   for (int x = 0; x < 4; x++) A.meanp[x] = 0.;
   if (base == RSG_CCF) {
     double *rows = c->d_red + 2 * (size_t)nblk + 8;
-    if (m->cnt_kind == CNT_I8G) hipLaunchKernelGGL(k_ccf_rows<CNT_I8G>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, m->d_pm, rows);
+    if (m->cnt_kind == CNT_U16) hipLaunchKernelGGL(k_ccf_rows<CNT_U16>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, m->d_pm, rows);
+    else if (m->cnt_kind == CNT_U32) hipLaunchKernelGGL(k_ccf_rows<CNT_U32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, m->d_pm, rows);
     else                            hipLaunchKernelGGL(k_ccf_rows<CNT_F32>, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cnt, L, m->d_pm, rows);
     std::vector<double> h(4 * (size_t)L);
     HIPCHK(hipMemcpyAsync(h.data(), rows, sizeof(double) * 4 * (size_t)L, hipMemcpyDeviceToHost, c->stream));
@@ -239,7 +243,7 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
   dim3 grid((unsigned)nblk), blk(256);
 #define LAUNCH(T, K, TB) hipLaunchKernelGGL((k_stat<T, K, TB>), grid, blk, 0, c->stream, A, nb)
   bool fixed = false;
-  if (prec == RSG_STAT_FP64 && m->cnt_kind == CNT_I8G && tab) {
+  if (prec == RSG_STAT_FP64 && m->cnt_kind == CNT_U16 && tab) {
     switch (base) {
     case RSG_GT:   fixed = launch_stat_fixed<RSG_GT, true>(c, A, cls, grid, nb); break;
     case RSG_MI:   fixed = launch_stat_fixed<RSG_MI, true>(c, A, cls, grid, nb); break;
@@ -251,10 +255,12 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
   }
   if (fixed) {
   } else if (prec == RSG_STAT_FP64) {
-    if (m->cnt_kind == CNT_I8G) { if (tab) LAUNCH(double, CNT_I8G, true); else LAUNCH(double, CNT_I8G, false); }
+    if (m->cnt_kind == CNT_U16) { if (tab) LAUNCH(double, CNT_U16, true); else LAUNCH(double, CNT_U16, false); }
+    else if (m->cnt_kind == CNT_U32) LAUNCH(double, CNT_U32, true);
     else                        LAUNCH(double, CNT_F32, false);
   } else {
-    if (m->cnt_kind == CNT_I8G) LAUNCH(float, CNT_I8G, false);
+    if (m->cnt_kind == CNT_U16) LAUNCH(float, CNT_U16, false);
+    else if (m->cnt_kind == CNT_U32) LAUNCH(float, CNT_U32, false);
     else                        LAUNCH(float, CNT_F32, false);
   }
 #undef LAUNCH
@@ -348,7 +354,8 @@ int rsg_raf(rsg_ctx *c, rsg_msa *m, int rafs, const double *allow16, char *errbu
   }
   unsigned nb = (unsigned)((L * L + 255) / 256);
   int t; rsg_tstart(c, "raf", &t);
-  hipLaunchKernelGGL(k_raf<CNT_I8G>, dim3(nb), dim3(256), 0, c->stream, cnt, L, m->N, allow, dst);
+  if (kind == CNT_U16) hipLaunchKernelGGL(k_raf<CNT_U16>, dim3(nb), dim3(256), 0, c->stream, cnt, L, m->N, allow, dst);
+  else                 hipLaunchKernelGGL(k_raf<CNT_U32>, dim3(nb), dim3(256), 0, c->stream, cnt, L, m->N, allow, dst);
   if (rafs) hipLaunchKernelGGL(k_rafs, dim3(nb), dim3(256), 0, c->stream, m->d_tmp, L, m->d_cov);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());

@@ -59,7 +59,7 @@ def random_setup(rng, Nr=(20, 6000), Lr=(50, 3500), big=0.0):
 
 
 def _record_bytes(L, N, K):
-    return K * (80 * L * L + 3 * N * L)
+    return K * (48 * L * L + 3 * N * L)
 
 
 def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1, log=None, covtype="GTp",
