@@ -40,8 +40,9 @@ template <> __device__ __forceinline__ float tlog<float>(float v) { return logf(
 
 // One pair. c: counts, pmi/pmj: marginals.  Returns the statistic (see the file header).
 // TYPE / CLS >= 0 fix the statistic and class at compile time (-1: read them from A).
+// tabc: ln(c + 1e-10) for counts c = 0..N (A.lgc, or a copy of it in LDS) when TAB.
 template <typename T, bool TAB, int TYPE = -1, int CLS = -1>
-__device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int64_t j)
+__device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int64_t j, const double *tabc)
 {
 #pragma clang fp contract(off)   // no FMA: every specialisation and the batched path round alike
   T ns = 0;
@@ -57,10 +58,12 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
   const T den = ns + (T)16e-10;
   T lden;
   if (TAB) lden = (T)A.lgc[A.N + 1 + (int64_t)nsd]; else lden = tlog<T>(den);
-  // pp and ln pp of cell k, formed where they are used (no 16-entry arrays kept live)
-  auto PP = [&](int k) -> T { return ((T)cd[k] + (T)1e-10) / den; };
+  // pp and ln pp of cell k, formed where they are used (no 16-entry arrays kept live); one
+  // reciprocal of the normaliser instead of 16 divisions (pp within 1 ulp of the quotient)
+  const T rden = (T)1 / den;
+  auto PP = [&](int k) -> T { return ((T)cd[k] + (T)1e-10) * rden; };
   auto LPP = [&](int k) -> T {
-    if (TAB) return (T)A.lgc[(int64_t)cd[k]] - lden;
+    if (TAB) return (T)tabc[(int)cd[k]] - lden;
     return tlog<T>((T)cd[k] + (T)1e-10) - lden;
   };
   const int type = TYPE >= 0 ? TYPE : A.type;
@@ -235,16 +238,14 @@ __device__ __forceinline__ void pm_reduce_col(const double *__restrict__ part_i,
   if (fabs(tot - 1.) > tol) atomicOr(flag, 1);
 }
 
-// 32 x 32 pair tile; COV written to both triangles through an LDS transpose; block min/max
-// to part[2*part_idx].
+// 32 x 32 pair tile (bi <= bj); COV written to both triangles through an LDS transpose;
+// the thread's min / max folded into lmin / lmax.
 template <typename T, int KIND, bool TAB, int TYPE = -1, int CLS = -1>
-__device__ void stat_tile(const StatArgs &A, int bi, int bj, int64_t part_idx)
+__device__ __forceinline__ void stat_pairs(const StatArgs &A, int bi, int bj, const double *tabc, double &lmin, double &lmax)
 {
   __shared__ double tile[32][33];
-  __shared__ double rmin[256], rmax[256];
   const int tid = threadIdx.x;
-  double lmin = INFINITY, lmax = -INFINITY;
-  if (bi <= bj) {
+  {
     // phase 1: consecutive threads take consecutive i (consecutive records j*L + i),
     // write row j of COV; phase 2 writes row i from the LDS transpose.
     const int ii = tid & 31;
@@ -255,7 +256,7 @@ __device__ void stat_tile(const StatArgs &A, int bi, int bj, int64_t part_idx)
       if (i < j && j < A.L) {
         double cd[16];
         load_rec<KIND>(A.cnt, rsg_rec(i, j, A.L), cd);
-        v = pair_stat<T, TAB, TYPE, CLS>(A, cd, i, j);
+        v = pair_stat<T, TAB, TYPE, CLS>(A, cd, i, j, tabc);
         lmin = fmin(lmin, v); lmax = fmax(lmax, v);
         A.cov[j * A.L + i] = v;
       } else if (i == j && i < A.L) {
@@ -270,14 +271,31 @@ __device__ void stat_tile(const StatArgs &A, int bi, int bj, int64_t part_idx)
       const int64_t i = (int64_t)bi * 32 + ii2, j = (int64_t)bj * 32 + jj;
       if (i < j && j < A.L) A.cov[i * A.L + j] = tile[jj][ii2];
     }
+    __syncthreads();                      // the tile buffer is reused by the next call
   }
+}
+
+// Block min / max of the 256 threads to part[2 * part_idx], part[2 * part_idx + 1].
+__device__ __forceinline__ void block_minmax(double lmin, double lmax, double *part, int64_t part_idx)
+{
+  __shared__ double rmin[256], rmax[256];
+  const int tid = threadIdx.x;
   rmin[tid] = lmin; rmax[tid] = lmax;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (tid < s) { rmin[tid] = fmin(rmin[tid], rmin[tid + s]); rmax[tid] = fmax(rmax[tid], rmax[tid + s]); }
     __syncthreads();
   }
-  if (tid == 0) { A.part[2 * part_idx] = rmin[0]; A.part[2 * part_idx + 1] = rmax[0]; }
+  if (tid == 0) { part[2 * part_idx] = rmin[0]; part[2 * part_idx + 1] = rmax[0]; }
+}
+
+// One tile per block (tiles below the diagonal contribute +-inf), block min / max to part.
+template <typename T, int KIND, bool TAB, int TYPE = -1, int CLS = -1>
+__device__ void stat_tile(const StatArgs &A, int bi, int bj, int64_t part_idx)
+{
+  double lmin = INFINITY, lmax = -INFINITY;
+  if (bi <= bj) stat_pairs<T, KIND, TAB, TYPE, CLS>(A, bi, bj, A.lgc, lmin, lmax);
+  block_minmax(lmin, lmax, A.part, part_idx);
 }
 
 // ---- FP64 null tail fit (cov_histogram_pmass, SetTailByMass, binned exponential / Gamma ML,

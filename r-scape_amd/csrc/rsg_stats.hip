@@ -142,15 +142,17 @@ __global__ void __launch_bounds__(256) k_stat(StatArgs A, int nb)
 }
 
 // FP64 statistics from integer counts with the statistic and class fixed at compile time
-// (the common cases: fewer live registers, no dead branches).  The log table (TAB) beats
-// inline FP64 log() by 2x here (45 vs 99 us for GT at L = 1500).
-template <int TYPE, bool TAB>
-static bool launch_stat_fixed(rsg_ctx *c, const StatArgs &A, int cls, dim3 grid, int nb)
+// (the common cases: fewer live registers, no dead branches).  The global log table beats
+// inline FP64 log() by 2x (45 vs 99 us for GT at L = 1500) and a per-block LDS copy of it
+// (49 us).  Returns the number of min / max partials written (0: not handled here).
+template <int TYPE>
+static int launch_stat_fixed(rsg_ctx *c, const StatArgs &A, int cls, int nb)
 {
-  if (cls == RSG_C16) hipLaunchKernelGGL((k_stat<double, CNT_U16, TAB, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
-  else if (cls == RSG_C2) hipLaunchKernelGGL((k_stat<double, CNT_U16, TAB, TYPE, RSG_C2>), grid, dim3(256), 0, c->stream, A, nb);
-  else return false;
-  return true;
+  if (cls != RSG_C16 && cls != RSG_C2) return 0;
+  const dim3 grid((unsigned)(nb * nb));
+  if (cls == RSG_C16) hipLaunchKernelGGL((k_stat<double, CNT_U16, true, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
+  else                hipLaunchKernelGGL((k_stat<double, CNT_U16, true, TYPE, RSG_C2>), grid, dim3(256), 0, c->stream, A, nb);
+  return nb * nb;
 }
 
 __global__ void k_reduce_minmax(const double *__restrict__ part, int n, double *__restrict__ out)
@@ -242,18 +244,18 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
   int t; rsg_tstart(c, "stat", &t);
   dim3 grid((unsigned)nblk), blk(256);
 #define LAUNCH(T, K, TB) hipLaunchKernelGGL((k_stat<T, K, TB>), grid, blk, 0, c->stream, A, nb)
-  bool fixed = false;
+  int nparts = 0;
   if (prec == RSG_STAT_FP64 && m->cnt_kind == CNT_U16 && tab) {
     switch (base) {
-    case RSG_GT:   fixed = launch_stat_fixed<RSG_GT, true>(c, A, cls, grid, nb); break;
-    case RSG_MI:   fixed = launch_stat_fixed<RSG_MI, true>(c, A, cls, grid, nb); break;
-    case RSG_MIr:  fixed = launch_stat_fixed<RSG_MIr, true>(c, A, cls, grid, nb); break;
-    case RSG_MIg:  fixed = launch_stat_fixed<RSG_MIg, true>(c, A, cls, grid, nb); break;
-    case RSG_CHI:  fixed = launch_stat_fixed<RSG_CHI, true>(c, A, cls, grid, nb); break;
-    case RSG_OMES: fixed = launch_stat_fixed<RSG_OMES, true>(c, A, cls, grid, nb); break;
+    case RSG_GT:   nparts = launch_stat_fixed<RSG_GT>(c, A, cls, nb); break;
+    case RSG_MI:   nparts = launch_stat_fixed<RSG_MI>(c, A, cls, nb); break;
+    case RSG_MIr:  nparts = launch_stat_fixed<RSG_MIr>(c, A, cls, nb); break;
+    case RSG_MIg:  nparts = launch_stat_fixed<RSG_MIg>(c, A, cls, nb); break;
+    case RSG_CHI:  nparts = launch_stat_fixed<RSG_CHI>(c, A, cls, nb); break;
+    case RSG_OMES: nparts = launch_stat_fixed<RSG_OMES>(c, A, cls, nb); break;
     }
   }
-  if (fixed) {
+  if (nparts > 0) {
   } else if (prec == RSG_STAT_FP64) {
     if (m->cnt_kind == CNT_U16) { if (tab) LAUNCH(double, CNT_U16, true); else LAUNCH(double, CNT_U16, false); }
     else if (m->cnt_kind == CNT_U32) LAUNCH(double, CNT_U32, true);
@@ -266,7 +268,7 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
 #undef LAUNCH
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
-  if (fetch_minmax(c, m, nblk, errbuf) != RSG_OK) return RSG_FAIL;
+  if (fetch_minmax(c, m, nparts > 0 ? nparts : nblk, errbuf) != RSG_OK) return RSG_FAIL;
   m->have_cov = 1;
   return RSG_OK;
 }
