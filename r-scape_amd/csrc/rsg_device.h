@@ -170,7 +170,9 @@ __device__ void pm_tile(const void *__restrict__ cnt, int64_t L, int nbt, int bi
   const int tid = threadIdx.x, ii = tid & 63, jq = tid >> 6;
   const int64_t i = (int64_t)bi * PMT + ii;
   double ai[4] = {0., 0., 0., 0.};
-  for (int jj = jq; jj < PMT; jj += PMW) {
+#pragma unroll
+  for (int q = 0; q < PMT / PMW; q++) {
+    const int jj = jq + q * PMW;
     const int64_t j = (int64_t)bj * PMT + jj;
     double cj[4] = {0., 0., 0., 0.};
     if (i < j && j < L) {

@@ -386,38 +386,34 @@ __global__ void __launch_bounds__(256) k_rowsum(const double *__restrict__ cov, 
   if (threadIdx.x == 0) rows[L + i] = red[0];
 }
 
-__global__ void __launch_bounds__(256) k_rowmean(int64_t L, double *__restrict__ rows, double *__restrict__ avg_out)
-{
-  __shared__ double red[256];
-  double s = 0.;
-  for (int64_t i = threadIdx.x; i < L; i += 256) {
-    double r = rows[L + i];
-    s += r;
-    rows[i] = (L > 1) ? r / ((double)L - 1.) : r;
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) rows[2 * L] = *avg_out = (L > 1) ? red[0] / ((double)L * ((double)L - 1.)) : red[0];
-}
-
 // APC / ASC in place, one block per row i (no per-element index division); block min / max
-// of the corrected row to part[2 i], part[2 i + 1].
+// of the corrected row to part[2 i], part[2 i + 1].  Every block forms COVavg from the raw
+// row sums rows[L..2L) with the loop and tree of the batched k_rowmean_b (same order, same
+// value; block 0 publishes it) and the row means r / (L - 1) on the fly.
 __global__ void __launch_bounds__(256) k_correct(double *__restrict__ cov, int64_t L, const double *__restrict__ rows,
-                                                 int actype, int *__restrict__ flag, double *__restrict__ part)
+                                                 int actype, int *__restrict__ flag, double *__restrict__ part,
+                                                 double *__restrict__ avg_out)
 {
   __shared__ double rmin[256], rmax[256];
-  const double avg = rows[2 * L];
+  double s = 0.;
+  for (int64_t k = threadIdx.x; k < L; k += 256) s += rows[L + k];
+  rmin[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) rmin[threadIdx.x] += rmin[threadIdx.x + st];
+    __syncthreads();
+  }
+  const double avg = (L > 1) ? rmin[0] / ((double)L * ((double)L - 1.)) : rmin[0];
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) *avg_out = avg;
+  const double dl = (L > 1) ? (double)L - 1. : 1.;
   const int64_t i = blockIdx.x;
-  const double xi = rows[i];
+  const double xi = rows[L + i] / dl;
   double *C = cov + i * L;
   double a = INFINITY, b = -INFINITY;
   for (int64_t j = threadIdx.x; j < L; j += 256) {
     if (j == i) continue;
-    const double v = C[j], xj = rows[j];
+    const double v = C[j], xj = rows[L + j] / dl;
     const double r = (actype == RSG_APC) ? ((avg != 0.0) ? v - xi * xj / avg : 0.0) : v - (xi + xj - avg);
     if (isnan(r)) atomicOr(flag, 1);
     C[j] = r;
@@ -450,8 +446,8 @@ int rsg_correct(rsg_ctx *c, rsg_msa *m, int actype, int shiftnonneg, char *errbu
   int *nanflag = (int *)(c->d_res + 2);
   int t; rsg_tstart(c, "correct", &t);
   hipLaunchKernelGGL(k_rowsum, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cov, L, m->d_rowx, nanflag);
-  hipLaunchKernelGGL(k_rowmean, dim3(1), dim3(256), 0, c->stream, L, m->d_rowx, c->d_res + 3);
-  hipLaunchKernelGGL(k_correct, dim3(nblk), dim3(256), 0, c->stream, m->d_cov, L, m->d_rowx, actype, nanflag, c->d_red);
+  hipLaunchKernelGGL(k_correct, dim3(nblk), dim3(256), 0, c->stream, m->d_cov, L, m->d_rowx, actype, nanflag, c->d_red,
+                     c->d_res + 3);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(256), 0, c->stream, c->d_red, nblk, c->d_res);
