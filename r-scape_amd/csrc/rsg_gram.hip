@@ -115,13 +115,13 @@ __device__ __forceinline__ unsigned pack_cells(int a, int sa, int b, int sb)
 // The accumulators stay in AGPRs through the K loop; the epilogue shifts the symbol scale
 // out and writes uint16 (OK = CNT_U16) or uint32 (CNT_U32) records.
 // One work-group tile (i0, j0) of one alignment over K-tiles [kt0, kt1); `lds` holds
-// 2 * St::BYTES.  With part == nullptr the records go to `out`; otherwise the tile's raw
-// partial sums go, unconditionally, to part[(jj * BI + ii) * 16 + cell] (ii, jj: column
-// offsets in the tile) for a later reduction.
+// 2 * St::BYTES.  With part == nullptr the records go to `out`; otherwise the tile's partial
+// counts go, unconditionally and in the record format, to partial record jj * BI + ii (ii, jj:
+// column offsets in the tile) for a later reduction.
 template <int WI, int WJ, int KT, int OK>
 __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L,
                                              void *__restrict__ out, int64_t i0, int64_t j0, uint8_t *lds,
-                                             int64_t kt0, int64_t kt1, uint32_t *__restrict__ part)
+                                             int64_t kt0, int64_t kt1, void *__restrict__ part)
 {
   constexpr int BI = 32 * WI, BJ = 32 * WJ, NT = 64 * WI * WJ, KS = KT / 32;
   typedef CodeStage<BI, BJ, KT, NT> St;
@@ -182,30 +182,21 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
   // Both destinations are affine in (ii, jj): base + jj * sj + ii * 16 cells.
   const int ii = wi * 32 + col;
   const int64_t i = i0 + ii;
-  if (part) {
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const int jj = wj * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
-      uint4 *p = (uint4 *)(part + ((int64_t)jj * BI + ii) * 16);
-#pragma unroll
-      for (int x = 0; x < 4; x++)
-        p[x] = make_uint4(acc[x][0][r], acc[x][1][r], acc[x][2][r], acc[x][3][r]);
-    }
-    return;
-  }
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     const int jj = wj * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
     const int64_t j = j0 + jj;
-    if (!(i < j && j < L)) continue;
+    if (!part && !(i < j && j < L)) continue;
+    const int64_t rr = part ? (int64_t)jj * BI + ii : rsg_rec(i, j, L);
+    void *const dst = part ? part : out;
     if (OK == CNT_U16) {
-      uint4 *p = (uint4 *)((uint16_t *)out + rsg_rec(i, j, L) * 16);
+      uint4 *p = (uint4 *)((uint16_t *)dst + rr * 16);
       p[0] = make_uint4(pack_cells(acc[0][0][r], 0, acc[0][1][r], 1), pack_cells(acc[0][2][r], 2, acc[0][3][r], 3),
                         pack_cells(acc[1][0][r], 1, acc[1][1][r], 2), pack_cells(acc[1][2][r], 3, acc[1][3][r], 4));
       p[1] = make_uint4(pack_cells(acc[2][0][r], 2, acc[2][1][r], 3), pack_cells(acc[2][2][r], 4, acc[2][3][r], 5),
                         pack_cells(acc[3][0][r], 3, acc[3][1][r], 4), pack_cells(acc[3][2][r], 5, acc[3][3][r], 6));
     } else {
-      uint4 *p = (uint4 *)((uint32_t *)out + rsg_rec(i, j, L) * 16);
+      uint4 *p = (uint4 *)((uint32_t *)dst + rr * 16);
 #pragma unroll
       for (int x = 0; x < 4; x++)
         p[x] = make_uint4((unsigned)acc[x][0][r] >> x, (unsigned)acc[x][1][r] >> (x + 1),
@@ -222,7 +213,7 @@ template <int WI, int WJ, int KT, int OK>
 __global__ void __launch_bounds__(64 * WI * WJ)
 k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, void *__restrict__ out,
           const int2 *__restrict__ tiles, int ndp, int per_xcd, int ntiles, int nsplit,
-          uint32_t *__restrict__ part)
+          uint8_t *__restrict__ part)
 {
   typedef CodeStage<32 * WI, 32 * WJ, KT, 64 * WI * WJ> St;
   __shared__ __attribute__((aligned(16))) uint8_t lds[2 * St::BYTES];
@@ -230,7 +221,7 @@ k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, void *__restr
   const int64_t nkt = Npad / KT;
   int t;
   int64_t k0 = 0, k1 = nkt;
-  uint32_t *p = nullptr;
+  uint8_t *p = nullptr;
   if (g < 8 * per_xcd) {
     // XCD-aware mapping: consecutive tiles of the (bi, bj)-sorted list run on one XCD (one L2).
     t = (g & 7) * per_xcd + (g >> 3);
@@ -240,40 +231,37 @@ k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, void *__restr
     t = ndp + u / nsplit;
     if (t >= ntiles) return;
     k0 = nkt * sg / nsplit; k1 = nkt * (sg + 1) / nsplit;
-    p = part + (int64_t)u * (32 * WI) * (32 * WJ) * 16;
+    p = part + (int64_t)u * (32 * WI) * (32 * WJ) * (OK == CNT_U16 ? 32 : 64);
   }
   const int2 tl = tiles[t];
   gram_tile_i8<WI, WJ, KT, OK>(xt, Npad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, k0, k1, p);
 }
 
 // Sum the nsplit partial tiles of each split tile into its records (integer sums: exact);
-// one thread per (pair, symbol row x): four cells, scale 2^(x+y) removed.
+// one thread per 16-byte quarter (uint32 records) or half (uint16 records: two counts per
+// word, added as words since no half exceeds 65535) of a record.
 template <int BI, int BJ, int OK>
 __global__ void __launch_bounds__(256)
-k_gram_reduce(const uint32_t *__restrict__ part, const int2 *__restrict__ tiles, int ndp, int nsplit, int64_t L,
+k_gram_reduce(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, int ndp, int nsplit, int64_t L,
               void *__restrict__ out)
 {
-  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;       // (split tile, pair, quarter)
-  const int64_t per_tile = (int64_t)BI * BJ * 4;
+  constexpr int NQ = (OK == CNT_U16) ? 2 : 4;                       // uint4 per record
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;       // (split tile, pair, part of record)
+  const int64_t per_tile = (int64_t)BI * BJ * NQ;
   const int t = (int)(q / per_tile);
-  const int64_t rem = q % per_tile, pr = rem >> 2;
-  const int quarter = (int)(rem & 3);
+  const int64_t rem = q % per_tile, pr = rem / NQ;
+  const int h = (int)(rem % NQ);
   const int ii = (int)(pr % BI), jj = (int)(pr / BI);
   const int2 tl = tiles[ndp + t];
   const int64_t i = (int64_t)tl.x * BI + ii, j = (int64_t)tl.y * BJ + jj;
   if (!(i < j && j < L)) return;
-  const uint4 *p = (const uint4 *)(part + ((int64_t)t * nsplit * BI * BJ + pr) * 16) + quarter;
+  const uint4 *p = (const uint4 *)part + ((int64_t)t * nsplit * BI * BJ + pr) * NQ + h;
   uint4 s = make_uint4(0, 0, 0, 0);
   for (int k = 0; k < nsplit; k++) {
-    const uint4 v = p[(int64_t)k * BI * BJ * 4];
+    const uint4 v = p[(int64_t)k * BI * BJ * NQ];
     s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
   }
-  const int x = quarter;
-  if (OK == CNT_U16)
-    ((uint2 *)((uint16_t *)out + rsg_rec(i, j, L) * 16))[x] = make_uint2(pack_cells((int)s.x, x, (int)s.y, x + 1),
-                                                                         pack_cells((int)s.z, x + 2, (int)s.w, x + 3));
-  else
-    ((uint4 *)((uint32_t *)out + rsg_rec(i, j, L) * 16))[x] = make_uint4(s.x >> x, s.y >> (x + 1), s.z >> (x + 2), s.w >> (x + 3));
+  ((uint4 *)out)[rsg_rec(i, j, L) * NQ + h] = s;
 }
 
 // Batched int8 Gram over many alignments: one work group per (item, tile); tiles are
@@ -485,16 +473,16 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, char *errbuf)
     nsplit = (int)std::min<int64_t>(nkt, ncu / rem);
   }
   const int nrem = ntiles - ndp;
-  uint32_t *part = nullptr;
+  uint8_t *part = nullptr;
   if (nrem > 0) {
-    const size_t need = (size_t)nrem * nsplit * BI * BJ * 64;
+    const size_t need = (size_t)nrem * nsplit * BI * BJ * rsg_rec_bytes(OK);
     if (c->gscr_cap < need) {
       if (c->d_gscr) hipFree(c->d_gscr);
       c->d_gscr = nullptr; c->gscr_cap = 0;
       HIPCHK(hipMalloc(&c->d_gscr, need));
       c->gscr_cap = need;
     }
-    part = (uint32_t *)c->d_gscr;
+    part = (uint8_t *)c->d_gscr;
   }
   const int per = (ndp + 7) / 8;
   dim3 grid((unsigned)(8 * per + nrem * nsplit)), block(64 * WI * WJ);
@@ -502,7 +490,7 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, char *errbuf)
   hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->L, out,
                      tiles, ndp, per, ntiles, nsplit, part);
   if (nrem > 0) {
-    const int64_t nq = (int64_t)nrem * BI * BJ * 4;
+    const int64_t nq = (int64_t)nrem * BI * BJ * (OK == CNT_U16 ? 2 : 4);
     hipLaunchKernelGGL((k_gram_reduce<BI, BJ, OK>), dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, c->stream, part, tiles,
                        ndp, nsplit, m->L, out);
   }
