@@ -37,7 +37,7 @@ __device__ __forceinline__ int find_item(const int64_t *__restrict__ desc, int n
   return lo;
 }
 
-// leaves (N x L, row-major) -> one-hot nibble codes (Lpad x Npad)
+// leaves (N x L, row-major) -> one-hot codes, K-chunked (pack_xt_chunk)
 __global__ void __launch_bounds__(256)
 k_pack_b(const int64_t *__restrict__ desc, int n, const uint8_t *__restrict__ leaves, uint8_t *__restrict__ xt)
 {
@@ -55,10 +55,7 @@ k_pack_b(const int64_t *__restrict__ desc, int n, const uint8_t *__restrict__ le
     tile[r][tx] = (s < N && c < L) ? ax[s * L + c] : (uint8_t)4;
   }
   __syncthreads();
-  for (int r = ty; r < 64; r += 4) {
-    uint8_t a = tile[tx][r];
-    X[(c0 + r) * Npad + s0 + tx] = (a < 4) ? (uint8_t)(1u << a) : (uint8_t)0;
-  }
+  pack_xt_chunk(tile, s0, c0, Lpad, X);
 }
 
 // corr_Marginals per item with the single-alignment kernels' tiles and reduction order
@@ -389,12 +386,12 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   SimLevels lv;
   if (rsg_sim_levels(sim.data(), nj, left.data(), right.data(), lv, errbuf) != RSG_OK) return RSG_FAIL;
   // surrogate features of each group's first replicate
-  std::vector<int64_t> fitems((size_t)ng * 5);
+  std::vector<int64_t> fitems((size_t)ng * 6);
   int64_t fblocks = 0;
   for (int g = 0; g < ng; g++) {
     const int64_t *d0 = &desc[(size_t)gfirst[(size_t)g] * RSG_BDESC];
-    int64_t *f = &fitems[(size_t)g * 5];
-    f[0] = d0[D_XT]; f[1] = d0[D_N]; f[2] = d0[D_NPAD]; f[3] = d0[D_L]; f[4] = fblocks;
+    int64_t *f = &fitems[(size_t)g * 6];
+    f[0] = d0[D_XT]; f[1] = d0[D_N]; f[2] = d0[D_NPAD]; f[3] = d0[D_L]; f[4] = fblocks; f[5] = d0[D_LPAD];
     fblocks += (d0[D_L] + 3) / 4;
   }
   // ---- device buffers: one grow-only arena of the context, carved in two passes

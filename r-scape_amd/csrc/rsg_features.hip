@@ -20,10 +20,11 @@
 #include <cstring>
 #include <vector>
 
-// Per item descriptor {one-hot panel offset, N, Npad, L, first block}; the panel is the
-// Gram operand (Lpad x Npad bytes, 1 << symbol for canonical residues, 0 otherwise).  One
-// wave per column (4 per block) reads the column's Npad bytes 16 at a time; 36 uint64
-// counters per item: [0..15] entropy bins, [16..31] gap bins, [32..35] residue totals.
+// Per item descriptor {one-hot panel offset, N, Npad, L, first block, Lpad}; the panel is
+// the Gram operand (K-chunked [Npad / 16][Lpad][16] bytes, 1 << symbol for canonical
+// residues, 0 otherwise).  One wave per column (4 per block) reads the column's Npad / 16
+// chunks; 36 uint64 counters per item: [0..15] entropy bins, [16..31] gap bins, [32..35]
+// residue totals.
 __global__ void __launch_bounds__(256)
 k_colfeat(const int64_t *__restrict__ items, int nitems, const uint8_t *__restrict__ base,
           unsigned long long *__restrict__ out)
@@ -31,19 +32,19 @@ k_colfeat(const int64_t *__restrict__ items, int nitems, const uint8_t *__restri
   __shared__ unsigned int eh[16], gh[16];
   __shared__ unsigned long long comp[4];
   int lo = 0, hi = nitems - 1;
-  while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (items[mid * 5 + 4] <= (int64_t)blockIdx.x) lo = mid; else hi = mid - 1; }
-  const int64_t *it = items + lo * 5;
-  const int64_t N = it[1], Npad = it[2], L = it[3];
+  while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (items[mid * 6 + 4] <= (int64_t)blockIdx.x) lo = mid; else hi = mid - 1; }
+  const int64_t *it = items + lo * 6;
+  const int64_t N = it[1], Npad = it[2], L = it[3], Lpad = it[5];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t c = ((int64_t)blockIdx.x - it[4]) * 4 + wave;
   if (threadIdx.x < 16) { eh[threadIdx.x] = 0; gh[threadIdx.x] = 0; }
   if (threadIdx.x < 4) comp[threadIdx.x] = 0;
   __syncthreads();
   if (c < L) {
-    const uint4 *col = (const uint4 *)(base + it[0] + c * Npad);
+    const uint4 *col = (const uint4 *)(base + it[0]) + c;
     unsigned int n[4] = {0, 0, 0, 0};
     for (int64_t k = lane; k < Npad / 16; k += 64) {
-      const uint4 v = col[k];
+      const uint4 v = col[k * Lpad];
 #pragma unroll
       for (int x = 0; x < 4; x++) {
         const unsigned mk = 0x01010101u << x;
@@ -165,8 +166,8 @@ extern "C" int rsg_null_features(rsg_ctx *c, rsg_msa *msa, const int *left, cons
   if (msa->N < 2) return rsg_fail(errbuf, "rsg_null_features: needs >= 2 sequences");
   HIPCHK(hipSetDevice(c->device));
   const int64_t nblk = (msa->L + 3) / 4;
-  const int64_t item[5] = {0, msa->N, msa->Npad, msa->L, 0};
-  int64_t *d_item = (int64_t *)c->d_small;                      // 40 B of context scratch
+  const int64_t item[6] = {0, msa->N, msa->Npad, msa->L, 0, msa->Lpad};
+  int64_t *d_item = (int64_t *)c->d_small;                      // 48 B of context scratch
   unsigned long long *d_out = (unsigned long long *)c->d_small + 8;
   unsigned long long *h = (unsigned long long *)c->h_pinned;   // 36 of the 64 pinned doubles
   unsigned long long cnt[36];

@@ -6,8 +6,10 @@
 // canonical symbols.  Writing X for the N x 4L one-hot matrix, all tables at once are the
 // Gram C = X^T diag(w) X; only upper-triangular column blocks are formed.
 //
-// Staging: one byte per (column, sequence) holding 1 << symbol (0 for gap / N), columns
-// K-contiguous ("xt", Lpad x Npad).  A wave owns 32 i-columns x 32 j-columns and keeps
+// Staging: one byte per (column, sequence) holding 1 << symbol (0 for gap / N), stored
+// K-chunked ("xt": [Npad / 16][Lpad][16 bytes]: the 16 sequences of a chunk for every column,
+// chunk after chunk), so the rows of a stage are contiguous in HBM and in LDS and the int8
+// Gram moves them with LDS-direct loads.  A wave owns 32 i-columns x 32 j-columns and keeps
 // all 16 symbol cells as 16 MFMA accumulators ("symbol-major" tiles): each lane reads
 // the 16 staged bytes of its column once per 32 sequences and derives the four symbol
 // operands in registers, so LDS traffic is one byte per (column, sequence) per wave.
@@ -43,8 +45,9 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 int64_t rsg_npad(int64_t N) { return ((N + NPAD_Q - 1) / NPAD_Q) * NPAD_Q; }
 
 // ------------------------------------------------------------------------------------
-// Pack: ax (N x L, row-major) -> axT (Lpad x Npad raw codes) and xt (Lpad x Npad nibbles).
-__global__ void k_pack(const uint8_t *__restrict__ ax, int64_t N, int64_t L, int64_t Npad,
+// Pack: ax (N x L, row-major) -> axT (Lpad x Npad raw codes) and xt (K-chunked one-hot
+// codes, see above; pack_xt_chunk).  One 64-sequence x 64-column block per work group.
+__global__ void k_pack(const uint8_t *__restrict__ ax, int64_t N, int64_t L, int64_t Npad, int64_t Lpad,
                        uint8_t *__restrict__ axT, uint8_t *__restrict__ xt)
 {
   __shared__ uint8_t tile[64][65];
@@ -57,53 +60,84 @@ __global__ void k_pack(const uint8_t *__restrict__ ax, int64_t N, int64_t L, int
   __syncthreads();
   for (int r = ty; r < 64; r += 4) {
     int64_t c = c0 + r, s = s0 + tx;
-    uint8_t a = tile[tx][r];
-    axT[c * Npad + s] = a;
-    xt[c * Npad + s]  = (a < 4) ? (uint8_t)(1u << a) : (uint8_t)0;
+    axT[c * Npad + s] = tile[tx][r];
   }
+  pack_xt_chunk(tile, s0, c0, Lpad, xt);
 }
 
 int rsg_pack(rsg_ctx *c, rsg_msa *m, char *errbuf)
 {
   dim3 grid((unsigned)(m->Lpad / 64), (unsigned)(m->Npad / 64));
   int t; rsg_tstart(c, "pack", &t);
-  hipLaunchKernelGGL(k_pack, grid, dim3(256), 0, c->stream, m->d_ax, m->N, m->L, m->Npad, m->d_axT, m->d_xt);
+  hipLaunchKernelGGL(k_pack, grid, dim3(256), 0, c->stream, m->d_ax, m->N, m->L, m->Npad, m->Lpad, m->d_axT, m->d_xt);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   return RSG_OK;
 }
 
 // ------------------------------------------------------------------------------------
-// Staging of code bytes for one K-tile: [chunk (16 seqs)][column][16 B], i side after j side.
-// The prefetch registers are a plain local array of the kernel (kept in VGPRs).
+// Staging of code bytes for one K-tile in LDS: the j side [chunk (16 seqs)][BJ columns][16 B]
+// (CSA bytes per chunk row), then the i side [chunk][BI columns][16 B]; no padding (a
+// ds_read_b128 pass of 16 lanes reads 16 consecutive columns: conflict-free).
 template <int BI, int BJ, int KT, int NT>
 struct CodeStage {
   static constexpr int NKC = KT / 16;
-  static constexpr int CSA = BJ * 16 + 64, CSB = BI * 16 + 64;
+  static constexpr int CSA = BJ * 16, CSB = BI * 16;
   static constexpr int BYTES = NKC * (CSA + CSB);
   static constexpr int NCH = (BI + BJ) * NKC;
   static constexpr int NR = (NCH + NT - 1) / NT;
+  static constexpr int NINST = NCH / 64;                    // 64-chunk LDS-direct loads per stage
 };
 
-// Every thread moves exactly NR chunks (NCH is a multiple of the thread count for all
-// shapes), so the prefetch registers are unconditionally assigned and stay in VGPRs
-// while the MFMAs of the current stage run.
-#define STAGE_GLOAD(St, pre, xt, Npad, i0, j0, k0, tid)                                        \
+// VGPR staging (the BF16 Gram): every thread moves exactly NR chunks (NCH is a multiple of
+// the thread count for all shapes); consecutive threads take consecutive columns of a chunk
+// row, so loads and LDS stores are contiguous.
+#define STAGE_GLOAD(St, pre, xt, Lpad, i0, j0, k0, tid)                                        \
   static_assert(St::NCH % NT == 0, "stage chunks must divide evenly among threads");         \
   _Pragma("unroll") for (int r_ = 0; r_ < St::NR; r_++) {                                      \
     const int ch_ = (tid) + r_ * NT;                                                           \
-    const int cc_ = ch_ / St::NKC, kc_ = ch_ % St::NKC;                                        \
+    const int kc_ = ch_ / (BI + BJ), cc_ = ch_ % (BI + BJ);                                    \
     const int64_t col_ = (cc_ < BJ) ? ((j0) + cc_) : ((i0) + cc_ - BJ);                        \
-    pre[r_] = *(const uint4 *)((xt) + col_ * (Npad) + (k0) + kc_ * 16);                        \
+    pre[r_] = *(const uint4 *)((xt) + (((k0) / 16 + kc_) * (Lpad) + col_) * 16);               \
   }
 #define STAGE_SSTORE(St, pre, base, tid)                                                        \
   _Pragma("unroll") for (int r_ = 0; r_ < St::NR; r_++) {                                      \
     const int ch_ = (tid) + r_ * NT;                                                           \
-    const int cc_ = ch_ / St::NKC, kc_ = ch_ % St::NKC;                                        \
+    const int kc_ = ch_ / (BI + BJ), cc_ = ch_ % (BI + BJ);                                    \
     uint8_t *p_ = (cc_ < BJ) ? ((base) + kc_ * St::CSA + cc_ * 16)                             \
                              : ((base) + St::NKC * St::CSA + kc_ * St::CSB + (cc_ - BJ) * 16); \
     *(uint4 *)p_ = pre[r_];                                                                    \
   }
+
+// LDS-direct staging (the int8 Gram): global_load_lds_dwordx4 moves 16 bytes per lane from
+// the lane's own global address to LDS at M0 + 16 * lane, without VGPRs.  The compiler does
+// not model the LDS write of the instruction, so it is issued through inline asm and every
+// wait for it is an explicit vmcnt wait.
+__device__ __forceinline__ void lds_direct16(const uint8_t *g, uint8_t *l)
+{
+  const uint32_t la = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(__attribute__((address_space(3))) uint8_t *)l);
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" :: "s"(la), "v"(g) : "memory", "m0");
+}
+
+// One stage (K-tile `kt`) into LDS image `base`: the stage's NCH chunks are the j-side chunk
+// rows then the i-side chunk rows, 64 consecutive chunks per wave instruction (a j-side
+// image of NKC * BJ chunks is a multiple of 64, so no instruction straddles the sides).
+// Every wave issues exactly NINST / NW instructions.
+template <class St, int BI, int BJ, int NW>
+__device__ __forceinline__ void stage_lds_direct(const uint8_t *__restrict__ xt, int64_t Lpad, int64_t i0, int64_t j0,
+                                                 int64_t kt, uint8_t *base, int wave, int lane)
+{
+  static_assert(St::NINST % NW == 0 && (St::NKC * BJ) % 64 == 0, "LDS-direct stage shape");
+#pragma unroll
+  for (int q = 0; q < St::NINST / NW; q++) {
+    const int c0 = (wave + NW * q) * 64, c = c0 + lane;
+    const bool aside = c < St::NKC * BJ;
+    const int cs = aside ? c : c - St::NKC * BJ, w = aside ? BJ : BI;
+    const int kc = cs / w, cc = cs % w;
+    const int64_t colg = (aside ? j0 : i0) + cc;
+    lds_direct16(xt + ((kt * St::NKC + kc) * Lpad + colg) * 16, base + c0 * 16);
+  }
+}
 
 // Cell (x, y) of an accumulator holds count << (x + y); two cells, scale removed, per word.
 __device__ __forceinline__ unsigned pack_cells(int a, int sa, int b, int sb)
@@ -119,7 +153,7 @@ __device__ __forceinline__ unsigned pack_cells(int a, int sa, int b, int sb)
 // counts go, unconditionally and in the record format, to partial record jj * BI + ii (ii, jj:
 // column offsets in the tile) for a later reduction.
 template <int WI, int WJ, int KT, int OK>
-__device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L,
+__device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int64_t Lpad, int64_t L,
                                              void *__restrict__ out, int64_t i0, int64_t j0, uint8_t *lds,
                                              int64_t kt0, int64_t kt1, void *__restrict__ part)
 {
@@ -139,18 +173,20 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[x][y][r] = 0;
 
-  uint4 pre[St::NR];
-  STAGE_GLOAD(St, pre, xt, Npad, i0, j0, kt0 * KT, tid);
-  STAGE_SSTORE(St, pre, lds, tid);
-  __syncthreads();
+  // three LDS images, loads two stages ahead: stage kt0 + it is read from image it % 3 while
+  // stage kt0 + it + 2 lands in image (it + 2) % 3, which was last read in iteration it - 1
+  // (ended by a barrier).  Stages past kt1 - 1 reload kt1 - 1 into an image nobody reads.
+  constexpr int NW = WI * WJ, QW = St::NINST / NW;     // LDS-direct loads per wave per stage
+  static_assert(QW >= 1 && QW <= 15, "vmcnt range");
   const int nit = (int)(kt1 - kt0);
+  auto stage_of = [&](int it) -> int64_t { return kt0 + ((it < nit) ? it : nit - 1); };
+  stage_lds_direct<St, BI, BJ, NW>(xt, Lpad, i0, j0, stage_of(0), lds, wave, lane);
+  stage_lds_direct<St, BI, BJ, NW>(xt, Lpad, i0, j0, stage_of(1), lds + St::BYTES, wave, lane);
+  __builtin_amdgcn_s_waitcnt(0x0F70 | QW);     // vmcnt(QW): this wave's loads of stage 0 landed
+  __syncthreads();                             // ... and every wave's
   for (int it = 0; it < nit; it++) {
-    const int buf = it & 1;
-    // straight-line prefetch (the last stage reloads itself, in bounds and unused), so the
-    // prefetch registers are never conditionally live
-    const int64_t knext = (kt0 + ((it + 1 < nit) ? it + 1 : it)) * KT;
-    STAGE_GLOAD(St, pre, xt, Npad, i0, j0, knext, tid);
-    const uint8_t *base = lds + buf * St::BYTES;
+    stage_lds_direct<St, BI, BJ, NW>(xt, Lpad, i0, j0, stage_of(it + 2), lds + ((it + 2) % 3) * St::BYTES, wave, lane);
+    const uint8_t *base = lds + (it % 3) * St::BYTES;
     uint4 ra = *(const uint4 *)(base + offA);
     uint4 rb = *(const uint4 *)(base + offB);
 #pragma unroll
@@ -174,9 +210,10 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
           acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[y], b[x], acc[x][y], 0, 0, 0);
       ra = na; rb = nb;
     }
-    STAGE_SSTORE(St, pre, lds + (buf ^ 1) * St::BYTES, tid);
+    __builtin_amdgcn_s_waitcnt(0x0F70 | QW);   // vmcnt(QW): stage it + 1 landed
     __syncthreads();
   }
+  __builtin_amdgcn_s_waitcnt(0x0F70);          // no LDS-direct load outlives the loop
 
   // epilogue: lane owns i column `col`; register r holds j = 8*(r/4) + 4*khalf + r%4.
   // Both destinations are affine in (ii, jj): base + jj * sj + ii * 16 cells.
@@ -211,12 +248,12 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
 // leaving most CUs idle.
 template <int WI, int WJ, int KT, int OK>
 __global__ void __launch_bounds__(64 * WI * WJ)
-k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, void *__restrict__ out,
+k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t Lpad, int64_t L, void *__restrict__ out,
           const int2 *__restrict__ tiles, int ndp, int per_xcd, int ntiles, int nsplit,
           uint8_t *__restrict__ part)
 {
   typedef CodeStage<32 * WI, 32 * WJ, KT, 64 * WI * WJ> St;
-  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * St::BYTES];
+  __shared__ __attribute__((aligned(16))) uint8_t lds[3 * St::BYTES];
   const int g = blockIdx.x;
   const int64_t nkt = Npad / KT;
   int t;
@@ -234,7 +271,7 @@ k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t L, void *__restr
     p = part + (int64_t)u * (32 * WI) * (32 * WJ) * (OK == CNT_U16 ? 32 : 64);
   }
   const int2 tl = tiles[t];
-  gram_tile_i8<WI, WJ, KT, OK>(xt, Npad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, k0, k1, p);
+  gram_tile_i8<WI, WJ, KT, OK>(xt, Lpad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, k0, k1, p);
 }
 
 // Sum the nsplit partial tiles of each split tile into its records (integer sums: exact);
@@ -273,10 +310,10 @@ k_gram_i8_batch(const int4 *__restrict__ tiles, const int64_t *__restrict__ desc
                 uint16_t *__restrict__ cnt)
 {
   typedef CodeStage<32 * W, 32 * W, KT, 64 * W * W> St;
-  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * St::BYTES];
+  __shared__ __attribute__((aligned(16))) uint8_t lds[3 * St::BYTES];
   const int4 tl = tiles[blockIdx.x];                    // (item, bi, bj, -)
   const int64_t *d = desc + (int64_t)tl.x * RSG_BDESC;  // item descriptor (see rsg_batch.hip)
-  gram_tile_i8<W, W, KT, CNT_U16>(xt + d[4], d[2], d[1], cnt + d[5], (int64_t)tl.y * 32 * W, (int64_t)tl.z * 32 * W, lds,
+  gram_tile_i8<W, W, KT, CNT_U16>(xt + d[4], d[3], d[1], cnt + d[5], (int64_t)tl.y * 32 * W, (int64_t)tl.z * 32 * W, lds,
                          0, d[2] / KT, nullptr);
 }
 
@@ -306,7 +343,7 @@ __device__ __forceinline__ void spread(unsigned d, int s, unsigned &lo, unsigned
 
 template <int WI, int WJ, int KT, int NS>
 __global__ void __launch_bounds__(64 * WI * WJ)
-k_gram_bf16(const uint8_t *__restrict__ xt, const uint16_t *__restrict__ wsplit, int64_t Npad, int64_t L,
+k_gram_bf16(const uint8_t *__restrict__ xt, const uint16_t *__restrict__ wsplit, int64_t Npad, int64_t Lpad, int64_t L,
             float *__restrict__ out, const int2 *__restrict__ tiles, int ntiles, int per_xcd)
 {
   constexpr int BI = 32 * WI, BJ = 32 * WJ, NT = 64 * WI * WJ, KS = KT / 32;
@@ -347,7 +384,7 @@ k_gram_bf16(const uint8_t *__restrict__ xt, const uint16_t *__restrict__ wsplit,
     if (tid < NWCH) *(uint4 *)(base + St::BYTES + tid * 16) = wpre;
   };
   const int64_t nkt = Npad / KT;
-  STAGE_GLOAD(St, pre, xt, Npad, i0, j0, 0, tid);
+  STAGE_GLOAD(St, pre, xt, Lpad, i0, j0, 0, tid);
   wload(0);
   STAGE_SSTORE(St, pre, lds, tid);
   wstore(lds);
@@ -355,7 +392,7 @@ k_gram_bf16(const uint8_t *__restrict__ xt, const uint16_t *__restrict__ wsplit,
   for (int64_t kt = 0; kt < nkt; kt++) {
     const int buf = (int)(kt & 1);
     const int64_t knext = ((kt + 1 < nkt) ? kt + 1 : kt) * KT;
-    STAGE_GLOAD(St, pre, xt, Npad, i0, j0, knext, tid);
+    STAGE_GLOAD(St, pre, xt, Lpad, i0, j0, knext, tid);
     wload(knext);
     const uint8_t *base = lds + buf * SB;
     const uint16_t *wl = (const uint16_t *)(base + St::BYTES);
@@ -487,7 +524,7 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, char *errbuf)
   const int per = (ndp + 7) / 8;
   dim3 grid((unsigned)(8 * per + nrem * nsplit)), block(64 * WI * WJ);
   int t; rsg_tstart(c, "gram_i8", &t);
-  hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->L, out,
+  hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->Lpad, m->L, out,
                      tiles, ndp, per, ntiles, nsplit, part);
   if (nrem > 0) {
     const int64_t nq = (int64_t)nrem * BI * BJ * (OK == CNT_U16 ? 2 : 4);
@@ -561,7 +598,7 @@ static int launch_bf16(rsg_ctx *c, rsg_msa *m, float *out, char *errbuf)
   int per = (ntiles + 7) / 8;
   int t; rsg_tstart(c, "gram_bf16", &t);
   hipLaunchKernelGGL((k_gram_bf16<WI, WJ, KT, NS>), dim3((unsigned)(8 * per)), dim3(64 * WI * WJ), 0, c->stream,
-                     m->d_xt, m->d_wsplit, m->Npad, m->L, out, tiles, ntiles, per);
+                     m->d_xt, m->d_wsplit, m->Npad, m->Lpad, m->L, out, tiles, ntiles, per);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   return RSG_OK;

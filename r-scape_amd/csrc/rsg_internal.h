@@ -43,6 +43,23 @@ enum { CNT_U16 = 1, CNT_F32 = 2, CNT_U32 = 3 };
 static inline int rsg_int_kind(int64_t N) { return N <= RSG_U16_MAXN ? CNT_U16 : CNT_U32; }
 __host__ __device__ static inline int rsg_rec_bytes(int kind) { return kind == CNT_U16 ? 32 : 64; }
 
+// One-hot Gram operand ("xt"), K-chunked: [Npad / 16][Lpad][16 bytes], byte = 1 << symbol for
+// canonical residues, 0 otherwise.  Writes the four 16-sequence chunk rows of a 64 x 64
+// (sequence, column) block held in tile[sequence][column] (256 threads: one 16-byte store
+// each, rows contiguous).
+__device__ __forceinline__ void pack_xt_chunk(const uint8_t (*tile)[65], int64_t s0, int64_t c0, int64_t Lpad,
+                                              uint8_t *__restrict__ xt)
+{
+  const int q = threadIdx.x >> 6, cc = threadIdx.x & 63;
+  unsigned w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    const unsigned a = tile[q * 16 + e][cc];
+    w[e >> 2] |= ((a < 4u) ? (1u << a) : 0u) << (8 * (e & 3));
+  }
+  *(uint4 *)(xt + ((s0 / 16 + q) * Lpad + c0 + cc) * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 struct rsg_timer { const char *name; hipEvent_t a, b; };
 
 struct rsg_ctx {
