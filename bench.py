@@ -40,15 +40,22 @@ def kernel_avgs(timings):
     return {k: (float(np.mean(v)), len(v)) for k, v in acc.items()}
 
 
-def run_steps(ctx, m, k, **kw):
-    ctx.set_timing(True)
+def run_steps(ctx, m, k, breakdown=5, **kw):
+    """k timed scorings with HIP events around the Gram launches only (its live duration for
+    the roofline), then `breakdown` more with events around every stage (kernels_ms)."""
+    ctx.set_timing("gram")
     t0 = time.perf_counter()
     for _ in range(k):
         m.calculate(**kw)
     t1 = time.perf_counter()
-    tim = ctx.timings()
+    gram = kernel_avgs(ctx.timings())
+    ctx.set_timing(True)
+    for _ in range(breakdown):
+        m.calculate(**kw)
+    allk = kernel_avgs(ctx.timings())
     ctx.set_timing(False)
-    return t1 - t0, kernel_avgs(tim)
+    allk.update(gram)
+    return t1 - t0, allk
 
 
 def cpu_baseline(ax, Lwin):

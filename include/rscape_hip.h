@@ -50,9 +50,9 @@ typedef struct rsg_msa rsg_msa;   /* one alignment resident in HBM + its pair ta
 /* ---- context ---- */
 rsg_ctx *rsg_ctx_create(int device, char *errbuf);
 void     rsg_ctx_destroy(rsg_ctx *ctx);
-/* Per-kernel timing of the next calls (HIP events on the context stream).  The
- * durations of the most recent launches, in ms, are returned by rsg_ctx_timings:
- * names[k] / ms[k] for k < *n. */
+/* Per-kernel timing of the next calls (HIP events on the context stream): on = 1 every
+ * stage, 2 the Gram launches only, 0 off.  The durations of the most recent launches, in
+ * ms, are returned by rsg_ctx_timings: names[k] / ms[k] for k < *n. */
 int      rsg_ctx_set_timing(rsg_ctx *ctx, int on);
 int      rsg_ctx_timings(rsg_ctx *ctx, const char **names, double *ms, int *n, int cap);
 /* Gram tile shape: 0 = autotuned / default, else an index into the shape table

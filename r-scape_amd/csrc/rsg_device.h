@@ -3,6 +3,33 @@
 #include "rsg_internal.h"
 #include <cmath>
 
+// A record as raw words (16-byte loads issued ahead of their use) and its decoding.
+template <int KIND> struct RawRec { uint4 q[KIND == CNT_U16 ? 2 : 4]; };
+template <int KIND> __device__ __forceinline__ void load_raw(const void *cnt, int64_t rec, RawRec<KIND> &r)
+{
+  const uint4 *p = (const uint4 *)((const uint8_t *)cnt + rec * (KIND == CNT_U16 ? 32 : 64));
+#pragma unroll
+  for (int h = 0; h < (KIND == CNT_U16 ? 2 : 4); h++) r.q[h] = p[h];
+}
+template <int KIND> __device__ __forceinline__ void decode_rec(const RawRec<KIND> &r, double *c)
+{
+  if (KIND == CNT_U16) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const unsigned w[4] = {r.q[h].x, r.q[h].y, r.q[h].z, r.q[h].w};
+#pragma unroll
+      for (int q = 0; q < 4; q++) { c[8 * h + 2 * q] = (double)(w[q] & 0xFFFFu); c[8 * h + 2 * q + 1] = (double)(w[q] >> 16); }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint4 v = r.q[k];
+      if (KIND == CNT_U32) { c[4 * k] = v.x; c[4 * k + 1] = v.y; c[4 * k + 2] = v.z; c[4 * k + 3] = v.w; }
+      else { c[4 * k] = __uint_as_float(v.x); c[4 * k + 1] = __uint_as_float(v.y); c[4 * k + 2] = __uint_as_float(v.z); c[4 * k + 3] = __uint_as_float(v.w); }
+    }
+  }
+}
+
 template <int KIND> __device__ __forceinline__ void load_rec(const void *cnt, int64_t rec, double *c)
 {
   if (KIND == CNT_U16) {
@@ -249,22 +276,36 @@ __device__ __forceinline__ void stat_pairs(const StatArgs &A, int bi, int bj, co
   const int tid = threadIdx.x;
   {
     // phase 1: consecutive threads take consecutive i (consecutive records j*L + i),
-    // write row j of COV; phase 2 writes row i from the LDS transpose.
+    // write row j of COV; phase 2 writes row i from the LDS transpose.  The four records of
+    // a thread are loaded before any statistic is formed and every COV store waits until all
+    // four are done, so the loads (and the table reads they feed) of the four pairs overlap.
     const int ii = tid & 31;
+    const int64_t i = (int64_t)bi * 32 + ii;
+    RawRec<KIND> raw[4];
+#pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int jj = (tid >> 5) + 8 * k;
-      const int64_t i = (int64_t)bi * 32 + ii, j = (int64_t)bj * 32 + jj;
-      double v = -INFINITY;
+      const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
+      if (i < j && j < A.L) load_raw<KIND>(A.cnt, rsg_rec(i, j, A.L), raw[k]);
+    }
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
+      v[k] = -INFINITY;
       if (i < j && j < A.L) {
         double cd[16];
-        load_rec<KIND>(A.cnt, rsg_rec(i, j, A.L), cd);
-        v = pair_stat<T, TAB, TYPE, CLS>(A, cd, i, j, tabc);
-        lmin = fmin(lmin, v); lmax = fmax(lmax, v);
-        A.cov[j * A.L + i] = v;
-      } else if (i == j && i < A.L) {
-        A.cov[i * A.L + j] = -INFINITY;
+        decode_rec<KIND>(raw[k], cd);
+        v[k] = pair_stat<T, TAB, TYPE, CLS>(A, cd, i, j, tabc);
+        lmin = fmin(lmin, v[k]); lmax = fmax(lmax, v[k]);
       }
-      tile[jj][ii] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int jj = (tid >> 5) + 8 * k;
+      const int64_t j = (int64_t)bj * 32 + jj;
+      if (i < j && j < A.L) A.cov[j * A.L + i] = v[k];
+      else if (i == j && i < A.L) A.cov[i * A.L + j] = -INFINITY;
+      tile[jj][ii] = v[k];
     }
     __syncthreads();
     const int jj = tid & 31;

@@ -226,7 +226,8 @@ class Context:
             pass
 
     def set_timing(self, on=True):
-        self.L.rsg_ctx_set_timing(self.h, 1 if on else 0)
+        """True: time every stage; "gram": the Gram launches only (two events per scoring)."""
+        self.L.rsg_ctx_set_timing(self.h, 2 if on == "gram" else (1 if on else 0))
 
     def timings(self):
         names = (C.c_char_p * 64)()
