@@ -66,10 +66,10 @@ k_pm_tile_b(const int64_t *__restrict__ desc, int n, const uint16_t *__restrict_
   const int it = find_item(desc, n, D_BPMT, blockIdx.x);
   const int64_t *d = desc + it * RSG_BDESC;
   const int64_t L = d[D_L], nbt = (L + PMT - 1) / PMT, local = blockIdx.x - d[D_BPMT];
-  const int bi = (int)(local / nbt), bj = (int)(local % nbt);
+  const int bi = (int)(local / (nbt * PMS)), bj = (int)(local % (nbt * PMS)) / PMS, sub = (int)(local % PMS);
   if (bi > bj) return;
   double *pi = part + d[D_PMP];
-  pm_tile<CNT_U16>(cnt + d[D_CNT], L, (int)nbt, bi, bj, pi, pi + pm_tri(nbt, nbt, nbt) * PMT * 4);
+  pm_tile<CNT_U16>(cnt + d[D_CNT], L, (int)nbt, bi, bj, sub, pi, pi + pm_npi(nbt));
 }
 
 __global__ void __launch_bounds__(256)
@@ -82,7 +82,7 @@ k_pm_reduce_b(const int64_t *__restrict__ desc, int n, const double *__restrict_
   const int64_t col = (blockIdx.x - d[D_BCOL4]) * 4 + (threadIdx.x >> 6);
   if (col >= L) return;
   const double *pi = part + d[D_PMP];
-  pm_reduce_col(pi, pi + pm_tri(nbt, nbt, nbt) * PMT * 4, (int)nbt, L, col, threadIdx.x & 63, tol, pm + d[D_PM],
+  pm_reduce_col(pi, pi + pm_npi(nbt), (int)nbt, L, col, threadIdx.x & 63, tol, pm + d[D_PM],
                 pm + d[D_PM] + 4 * L, flag);
 }
 
@@ -347,7 +347,7 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     d[D_NB32] = nb32; d[D_GROUP] = J.group; d[D_BHIST] = b_hist;
     const int64_t nbt = (L + PMT - 1) / PMT;
     d[D_BPMT] = b_pmt; d[D_PMP] = o_pmp; d[D_BCOL4] = b_col4;
-    b_pmt += nbt * nbt; b_col4 += (L + 3) / 4; o_pmp += 2 * pm_tri(nbt, nbt, nbt) * PMT * 4;
+    b_pmt += nbt * nbt * PMS; b_col4 += (L + 3) / 4; o_pmp += pm_npi(nbt) + pm_npj(nbt);
     cls[(size_t)k] = (N <= 8 || L <= 50) ? RSG_C2 : RSG_C16;           // CSELECT (correlators.c:335-341)
     SimItem &s = sim[(size_t)k];
     s.N = N; s.L = L; s.tree_off = nodes; s.seq_off = seqs; s.leaf_off = o_leaf; s.seed = J.seed; s.stream = J.stream;

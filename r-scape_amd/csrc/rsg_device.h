@@ -177,29 +177,35 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
 }
 
 // ---- corr_Marginals, pm[i][x] = normalize_x(sum over j != i with nseff_ij > 0 of the pp_ij
-// row (i first) or column (i second) sums).  Pass 1 (pm_tile, 64 * PMW threads) reads one
-// 64 x 64 tile of contiguous record rows once and leaves per-tile partial sums for the tile's
-// i-columns (row sums) and j-columns (column sums); pass 2 (pm_reduce_col, one wave per
-// column) adds a column's partials in a fixed order (deterministic) and normalises.  The
-// single-alignment and the batched paths share both, so they round identically.
+// row (i first) or column (i second) sums).  Pass 1 (pm_tile, 64 * PMW threads) reads the
+// records of 64 i-columns x PMJ j-columns (sub-block `sub` of a 64 x 64 tile) once and
+// leaves per-block partial sums for the 64 i-columns (row sums) and per-tile sums for the PMJ
+// j-columns (column sums); pass 2 (pm_reduce_col, one wave per column) adds a column's
+// partials in a fixed order (deterministic) and normalises.  The single-alignment and the
+// batched paths share both, so they round identically.
 #define PMT 64
-#define PMW 16          // waves per marginal tile block
-// partials of upper-triangle tile (bi <= bj) of nbt x nbt tiles, row-major: [tri][64][4] doubles
+#define PMS 4           // j sub-blocks per 64 x 64 tile (one work group each)
+#define PMJ (PMT / PMS) // j-columns per work group
+#define PMW 4           // waves per work group: PMJ / PMW = 4 j-columns per thread
+// partials of upper-triangle tile (bi <= bj) of nbt x nbt tiles, row-major
 __host__ __device__ __forceinline__ int64_t pm_tri(int64_t bi, int64_t bj, int64_t nbt)
 {
   return bi * nbt - (bi * (bi - 1)) / 2 + (bj - bi);
 }
+// doubles of the i-side partials [tri][PMS][64][4] and of the j-side partials [tri][64][4]
+__host__ __device__ __forceinline__ int64_t pm_npi(int64_t nbt) { return pm_tri(nbt, nbt, nbt) * PMS * PMT * 4; }
+__host__ __device__ __forceinline__ int64_t pm_npj(int64_t nbt) { return pm_tri(nbt, nbt, nbt) * PMT * 4; }
 template <int KIND>
-__device__ void pm_tile(const void *__restrict__ cnt, int64_t L, int nbt, int bi, int bj,
+__device__ void pm_tile(const void *__restrict__ cnt, int64_t L, int nbt, int bi, int bj, int sub,
                         double *__restrict__ part_i, double *__restrict__ part_j)
 {
-  __shared__ double si[PMW][PMT][4], sj[PMT][4];
+  __shared__ double si[PMW][PMT][4], sj[PMJ][4];
   const int tid = threadIdx.x, ii = tid & 63, jq = tid >> 6;
   const int64_t i = (int64_t)bi * PMT + ii;
   double ai[4] = {0., 0., 0., 0.};
 #pragma unroll
-  for (int q = 0; q < PMT / PMW; q++) {
-    const int jj = jq + q * PMW;
+  for (int q = 0; q < PMJ / PMW; q++) {
+    const int js = jq + q * PMW, jj = sub * PMJ + js;       // js: column within the sub-block
     const int64_t j = (int64_t)bj * PMT + jj;
     double cj[4] = {0., 0., 0., 0.};
     if (i < j && j < L) {
@@ -220,18 +226,16 @@ __device__ void pm_tile(const void *__restrict__ cnt, int64_t L, int nbt, int bi
 #pragma unroll
     for (int x = 0; x < 4; x++)
       for (int o = 32; o > 0; o >>= 1) cj[x] += __shfl_down(cj[x], o, 64);
-    if (ii == 0) for (int x = 0; x < 4; x++) sj[jj][x] = cj[x];
+    if (ii == 0) for (int x = 0; x < 4; x++) sj[js][x] = cj[x];
   }
   for (int x = 0; x < 4; x++) si[jq][ii][x] = ai[x];
   __syncthreads();
   const int64_t t = pm_tri(bi, bj, nbt);
-  if (tid < PMT * 4) {
-    const int c = tid >> 2, x = tid & 3;
-    double a = 0.;
-    for (int w = 0; w < PMW; w++) a += si[w][c][x];
-    part_i[(t * PMT + c) * 4 + x] = a;
-    part_j[(t * PMT + c) * 4 + x] = sj[c][x];
-  }
+  const int c = tid >> 2, x = tid & 3;                       // 256 threads: 64 columns x 4
+  double a = 0.;
+  for (int w = 0; w < PMW; w++) a += si[w][c][x];
+  part_i[((t * PMS + sub) * PMT + c) * 4 + x] = a;
+  if (c < PMJ) part_j[(t * PMT + sub * PMJ + c) * 4 + x] = sj[c][x];
 }
 
 // one wave (lane = 0..63) per column: lanes take the column's tiles in a fixed assignment,
@@ -241,14 +245,15 @@ __device__ __forceinline__ void pm_reduce_col(const double *__restrict__ part_i,
                                               double *__restrict__ pm, double *__restrict__ lpm, int *__restrict__ flag)
 {
   const int b = (int)(col / PMT), o = (int)(col % PMT);
+  const int ni = PMS * (nbt - b);                             // i-side entries: tiles (b, b..nbt-1) x sub-blocks
   double v[4] = {0., 0., 0., 0.};
-  for (int k = lane; k < nbt + 1; k += 64) {
-    // k < nbt - b: i-side partial of tile (b, b + k); k >= nbt - b: j-side partial of tile (k - (nbt - b), b)
+  for (int k = lane; k < ni + b + 1; k += 64) {
+    // k < ni: i-side partial of tile (b, b + k / PMS), sub-block k % PMS; else j-side partial of
+    // tile (k - ni, b)
     int64_t base;
     const double *src;
-    if (k < nbt - b) { src = part_i; base = pm_tri(b, b + k, nbt) * PMT + o; }
-    else if (k - (nbt - b) <= b) { src = part_j; base = pm_tri(k - (nbt - b), b, nbt) * PMT + o; }
-    else continue;
+    if (k < ni) { src = part_i; base = (pm_tri(b, b + k / PMS, nbt) * PMS + k % PMS) * PMT + o; }
+    else { src = part_j; base = pm_tri(k - ni, b, nbt) * PMT + o; }
     for (int x = 0; x < 4; x++) v[x] += src[base * 4 + x];
   }
 #pragma unroll

@@ -22,7 +22,7 @@ static int ensure_red(rsg_ctx *c, size_t n, char *errbuf);
 int rsg_reserve_scratch(rsg_ctx *c, int64_t L, char *errbuf)
 {
   const int64_t nbt = (L + 63) / 64, nb = (L + 31) / 32;
-  const size_t need = std::max<size_t>({(size_t)(2 * pm_tri(nbt, nbt, nbt) * PMT * 4), (size_t)(2 * nb * nb + 8 + 4 * L),
+  const size_t need = std::max<size_t>({(size_t)(pm_npi(nbt) + pm_npj(nbt)), (size_t)(2 * nb * nb + 8 + 4 * L),
                                         (size_t)(2 * L + 8)});
   return ensure_red(c, need, errbuf);
 }
@@ -35,8 +35,9 @@ __global__ void __launch_bounds__(64 * PMW) k_pm_tile(const void *__restrict__ c
                                                       int *__restrict__ flag)
 {
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *flag = 0;   // k_pm_reduce sets it
-  if (blockIdx.y > blockIdx.x) return;
-  pm_tile<KIND>(cnt, L, nbt, (int)blockIdx.y, (int)blockIdx.x, part_i, part_j);
+  const int bj = (int)blockIdx.x / PMS, sub = (int)blockIdx.x % PMS, bi = (int)blockIdx.y;
+  if (bi > bj) return;
+  pm_tile<KIND>(cnt, L, nbt, bi, bj, sub, part_i, part_j);
 }
 
 __global__ void __launch_bounds__(256) k_pm_reduce(const double *__restrict__ part_i, const double *__restrict__ part_j,
@@ -101,18 +102,17 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
   if (!m->d_pm) HIPCHK(hipMalloc(&m->d_pm, sizeof(double) * 8 * (size_t)L));
   if (!m->d_ps) HIPCHK(hipMalloc(&m->d_ps, sizeof(double) * 5 * (size_t)L));
   const int nbt = (int)((L + PMT - 1) / PMT);
-  const size_t npart = (size_t)pm_tri(nbt, nbt, nbt) * PMT * 4;     // nbt (nbt + 1) / 2 tiles
-  if (ensure_red(c, 2 * npart, errbuf) != RSG_OK) return RSG_FAIL;
-  double *part_i = c->d_red, *part_j = c->d_red + npart;
+  if (ensure_red(c, (size_t)(pm_npi(nbt) + pm_npj(nbt)), errbuf) != RSG_OK) return RSG_FAIL;
+  double *part_i = c->d_red, *part_j = c->d_red + pm_npi(nbt);
   int t; rsg_tstart(c, "marginals", &t);
   if (m->cnt_kind == CNT_U16)
-    hipLaunchKernelGGL(k_pm_tile<CNT_U16>, dim3((unsigned)nbt, (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
+    hipLaunchKernelGGL(k_pm_tile<CNT_U16>, dim3((unsigned)(nbt * PMS), (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
                        (int *)(c->d_res + 4));
   else if (m->cnt_kind == CNT_U32)
-    hipLaunchKernelGGL(k_pm_tile<CNT_U32>, dim3((unsigned)nbt, (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
+    hipLaunchKernelGGL(k_pm_tile<CNT_U32>, dim3((unsigned)(nbt * PMS), (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
                        (int *)(c->d_res + 4));
   else
-    hipLaunchKernelGGL(k_pm_tile<CNT_F32>, dim3((unsigned)nbt, (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
+    hipLaunchKernelGGL(k_pm_tile<CNT_F32>, dim3((unsigned)(nbt * PMS), (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
                        (int *)(c->d_res + 4));
   hipLaunchKernelGGL(k_pm_reduce, dim3((unsigned)((L + 3) / 4)), dim3(256), 0, c->stream, part_i, part_j, nbt, L, tol,
                      m->d_pm, m->d_pm + 4 * L, (int *)(c->d_res + 4));
