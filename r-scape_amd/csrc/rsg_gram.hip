@@ -526,12 +526,14 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, char *errbuf)
   int t; rsg_tstart(c, "gram_i8", &t);
   hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->Lpad, m->L, out,
                      tiles, ndp, per, ntiles, nsplit, part);
+  rsg_tstop(c, t);
   if (nrem > 0) {
+    rsg_tstart(c, "gram_reduce", &t);
     const int64_t nq = (int64_t)nrem * BI * BJ * (OK == CNT_U16 ? 2 : 4);
     hipLaunchKernelGGL((k_gram_reduce<BI, BJ, OK>), dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, c->stream, part, tiles,
                        ndp, nsplit, m->L, out);
+    rsg_tstop(c, t);
   }
-  rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   return RSG_OK;
 }
