@@ -145,9 +145,12 @@ __device__ __forceinline__ unsigned pack_cells(int a, int sa, int b, int sb)
   return ((unsigned)a >> sa) | (((unsigned)b >> sb) << 16);
 }
 
-// int8 Gram: WI x WJ waves, each 32 i-columns x 32 j-columns, KT sequences per stage.
-// The accumulators stay in AGPRs through the K loop; the epilogue shifts the symbol scale
-// out and writes uint16 (OK = CNT_U16) or uint32 (CNT_U32) records.
+// int8 Gram: 2 x WI x WJ waves; wave w owns the 32 i-columns x 32 j-columns block w % (WI WJ)
+// and the symbol rows x of its half (w / (WI WJ)): x in {0, 1} or {2, 3}, 8 accumulators of
+// 16 int32 per lane.  Two waves share every SIMD (at most 256 registers each), so one wave's
+// LDS and barrier waits are covered by the other's MFMAs.  KT sequences per stage.  The
+// epilogue shifts the symbol scale out and writes uint16 (OK = CNT_U16) or uint32 (CNT_U32)
+// records; each half writes the cells of its own symbol rows.
 // One work-group tile (i0, j0) of one alignment over K-tiles [kt0, kt1); `lds` holds
 // 2 * St::BYTES.  With part == nullptr the records go to `out`; otherwise the tile's partial
 // counts go, unconditionally and in the record format, to partial record jj * BI + ii (ii, jj:
@@ -157,17 +160,18 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
                                              void *__restrict__ out, int64_t i0, int64_t j0, uint8_t *lds,
                                              int64_t kt0, int64_t kt1, void *__restrict__ part)
 {
-  constexpr int BI = 32 * WI, BJ = 32 * WJ, NT = 64 * WI * WJ, KS = KT / 32;
+  constexpr int BI = 32 * WI, BJ = 32 * WJ, NT = 128 * WI * WJ, KS = KT / 32;
   typedef CodeStage<BI, BJ, KT, NT> St;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wi = wave % WI, wj = wave / WI;
+  const int blk = wave % (WI * WJ), half = wave / (WI * WJ), x0 = 2 * half;
+  const int wi = blk % WI, wj = blk / WI;
   const int col = lane & 31, khalf = lane >> 5;
   const int offA = (wj * 32 + col) * 16 + khalf * St::CSA;
   const int offB = St::NKC * St::CSA + (wi * 32 + col) * 16 + khalf * St::CSB;
 
-  v16i acc[4][4];
+  v16i acc[2][4];                                  // [x - x0][y]
 #pragma unroll
-  for (int x = 0; x < 4; x++)
+  for (int x = 0; x < 2; x++)
 #pragma unroll
     for (int y = 0; y < 4; y++)
 #pragma unroll
@@ -176,7 +180,7 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
   // three LDS images, loads two stages ahead: stage kt0 + it is read from image it % 3 while
   // stage kt0 + it + 2 lands in image (it + 2) % 3, which was last read in iteration it - 1
   // (ended by a barrier).  Stages past kt1 - 1 reload kt1 - 1 into an image nobody reads.
-  constexpr int NW = WI * WJ, QW = St::NINST / NW;     // LDS-direct loads per wave per stage
+  constexpr int NW = 2 * WI * WJ, QW = St::NINST / NW; // LDS-direct loads per wave per stage
   static_assert(QW >= 1 && QW <= 15, "vmcnt range");
   const int nit = (int)(kt1 - kt0);
   auto stage_of = [&](int it) -> int64_t { return kt0 + ((it < nit) ? it : nit - 1); };
@@ -196,15 +200,19 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
         na = *(const uint4 *)(base + offA + (ks + 1) * 2 * St::CSA);
         nb = *(const uint4 *)(base + offB + (ks + 1) * 2 * St::CSB);
       }
-      v4i a[4], b[4];
+      v4i a[4], b[2];
 #pragma unroll
       for (int s = 0; s < 4; s++) {
         const unsigned mk = 0x01010101u << s;
         a[s] = (v4i){(int)(ra.x & mk), (int)(ra.y & mk), (int)(ra.z & mk), (int)(ra.w & mk)};
-        b[s] = (v4i){(int)(rb.x & mk), (int)(rb.y & mk), (int)(rb.z & mk), (int)(rb.w & mk)};
       }
 #pragma unroll
-      for (int x = 0; x < 4; x++)
+      for (int x = 0; x < 2; x++) {
+        const unsigned mk = 0x01010101u << (x0 + x);
+        b[x] = (v4i){(int)(rb.x & mk), (int)(rb.y & mk), (int)(rb.z & mk), (int)(rb.w & mk)};
+      }
+#pragma unroll
+      for (int x = 0; x < 2; x++)
 #pragma unroll
         for (int y = 0; y < 4; y++)
           acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[y], b[x], acc[x][y], 0, 0, 0);
@@ -226,18 +234,16 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
     if (!part && !(i < j && j < L)) continue;
     const int64_t rr = part ? (int64_t)jj * BI + ii : rsg_rec(i, j, L);
     void *const dst = part ? part : out;
-    if (OK == CNT_U16) {
-      uint4 *p = (uint4 *)((uint16_t *)dst + rr * 16);
-      p[0] = make_uint4(pack_cells(acc[0][0][r], 0, acc[0][1][r], 1), pack_cells(acc[0][2][r], 2, acc[0][3][r], 3),
-                        pack_cells(acc[1][0][r], 1, acc[1][1][r], 2), pack_cells(acc[1][2][r], 3, acc[1][3][r], 4));
-      p[1] = make_uint4(pack_cells(acc[2][0][r], 2, acc[2][1][r], 3), pack_cells(acc[2][2][r], 4, acc[2][3][r], 5),
-                        pack_cells(acc[3][0][r], 3, acc[3][1][r], 4), pack_cells(acc[3][2][r], 5, acc[3][3][r], 6));
+    if (OK == CNT_U16) {                        // cells 8 half .. 8 half + 7: the 16 bytes of this half
+      uint4 *p = (uint4 *)((uint16_t *)dst + rr * 16) + half;
+      *p = make_uint4(pack_cells(acc[0][0][r], x0, acc[0][1][r], x0 + 1), pack_cells(acc[0][2][r], x0 + 2, acc[0][3][r], x0 + 3),
+                      pack_cells(acc[1][0][r], x0 + 1, acc[1][1][r], x0 + 2), pack_cells(acc[1][2][r], x0 + 3, acc[1][3][r], x0 + 4));
     } else {
-      uint4 *p = (uint4 *)((uint32_t *)dst + rr * 16);
+      uint4 *p = (uint4 *)((uint32_t *)dst + rr * 16) + x0;
 #pragma unroll
-      for (int x = 0; x < 4; x++)
-        p[x] = make_uint4((unsigned)acc[x][0][r] >> x, (unsigned)acc[x][1][r] >> (x + 1),
-                          (unsigned)acc[x][2][r] >> (x + 2), (unsigned)acc[x][3][r] >> (x + 3));
+      for (int x = 0; x < 2; x++)
+        p[x] = make_uint4((unsigned)acc[x][0][r] >> (x0 + x), (unsigned)acc[x][1][r] >> (x0 + x + 1),
+                          (unsigned)acc[x][2][r] >> (x0 + x + 2), (unsigned)acc[x][3][r] >> (x0 + x + 3));
     }
   }
 }
@@ -247,12 +253,12 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
 // partial sums for k_gram_reduce.  The split keeps the last, partial wave of tiles from
 // leaving most CUs idle.
 template <int WI, int WJ, int KT, int OK>
-__global__ void __launch_bounds__(64 * WI * WJ)
+__global__ void __launch_bounds__(128 * WI * WJ) __attribute__((amdgpu_waves_per_eu(2)))
 k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t Lpad, int64_t L, void *__restrict__ out,
           const int2 *__restrict__ tiles, int ndp, int per_xcd, int ntiles, int nsplit,
           uint8_t *__restrict__ part)
 {
-  typedef CodeStage<32 * WI, 32 * WJ, KT, 64 * WI * WJ> St;
+  typedef CodeStage<32 * WI, 32 * WJ, KT, 128 * WI * WJ> St;
   __shared__ __attribute__((aligned(16))) uint8_t lds[3 * St::BYTES];
   const int g = blockIdx.x;
   const int64_t nkt = Npad / KT;
@@ -303,13 +309,14 @@ k_gram_reduce(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, 
 
 // Batched int8 Gram over many alignments: one work group per (item, tile); tiles are
 // pre-sorted by decreasing sequence count (cost).  Large alignments: 64 x 64 columns, 4
-// waves, 256-sequence K tiles; small ones: 32 x 32 columns, one wave, 64-sequence K tiles.
+// waves per half, 256-sequence K tiles; small ones: 32 x 32 columns, one wave per half,
+// 64-sequence K tiles.
 template <int W, int KT>
-__global__ void __launch_bounds__(64 * W * W)
+__global__ void __launch_bounds__(128 * W * W) __attribute__((amdgpu_waves_per_eu(2)))
 k_gram_i8_batch(const int4 *__restrict__ tiles, const int64_t *__restrict__ desc, const uint8_t *__restrict__ xt,
                 uint16_t *__restrict__ cnt)
 {
-  typedef CodeStage<32 * W, 32 * W, KT, 64 * W * W> St;
+  typedef CodeStage<32 * W, 32 * W, KT, 128 * W * W> St;
   __shared__ __attribute__((aligned(16))) uint8_t lds[3 * St::BYTES];
   const int4 tl = tiles[blockIdx.x];                    // (item, bi, bj, -)
   const int64_t *d = desc + (int64_t)tl.x * RSG_BDESC;  // item descriptor (see rsg_batch.hip)
@@ -321,9 +328,9 @@ int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_big, int nbig, const int4 *d
                           const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt)
 {
   if (nbig > 0)
-    hipLaunchKernelGGL((k_gram_i8_batch<2, 256>), dim3((unsigned)nbig), dim3(256), 0, c->stream, d_big, d_desc, d_xt, d_cnt);
+    hipLaunchKernelGGL((k_gram_i8_batch<2, 256>), dim3((unsigned)nbig), dim3(512), 0, c->stream, d_big, d_desc, d_xt, d_cnt);
   if (nsmall > 0)
-    hipLaunchKernelGGL((k_gram_i8_batch<1, 64>), dim3((unsigned)nsmall), dim3(64), 0, c->stream, d_small, d_desc, d_xt, d_cnt);
+    hipLaunchKernelGGL((k_gram_i8_batch<1, 64>), dim3((unsigned)nsmall), dim3(128), 0, c->stream, d_small, d_desc, d_xt, d_cnt);
   return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
 }
 
@@ -455,12 +462,12 @@ k_gram_bf16(const uint8_t *__restrict__ xt, const uint16_t *__restrict__ wsplit,
 // Tile shapes (the autotuner sweeps these): waves per work group and K-tile.
 struct GramShape { int WI, WJ, KT; const char *name; };
 static const GramShape kShapes[] = {
-  {2, 2, 128, "64x64c/4w/kt128"},
-  {2, 2, 64,  "64x64c/4w/kt64"},
-  {2, 2, 256, "64x64c/4w/kt256"},
-  {1, 1, 128, "32x32c/1w/kt128"},
-  {2, 1, 128, "64x32c/2w/kt128"},
-  {1, 2, 128, "32x64c/2w/kt128"},
+  {2, 2, 128, "64x64c/8w/kt128"},
+  {2, 2, 64,  "64x64c/8w/kt64"},
+  {2, 2, 256, "64x64c/8w/kt256"},
+  {1, 1, 128, "32x32c/2w/kt128"},
+  {2, 1, 128, "64x32c/4w/kt128"},
+  {1, 2, 128, "32x64c/4w/kt128"},
 };
 static const int kNShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
 
@@ -522,7 +529,7 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, char *errbuf)
     part = (uint8_t *)c->d_gscr;
   }
   const int per = (ndp + 7) / 8;
-  dim3 grid((unsigned)(8 * per + nrem * nsplit)), block(64 * WI * WJ);
+  dim3 grid((unsigned)(8 * per + nrem * nsplit)), block(128 * WI * WJ);
   int t; rsg_tstart(c, "gram_i8", &t);
   hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->Lpad, m->L, out,
                      tiles, ndp, per, ntiles, nsplit, part);
