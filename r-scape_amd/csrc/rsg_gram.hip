@@ -248,33 +248,53 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
   }
 }
 
-// Grid: 8 * per_xcd work groups for the first ndp tiles (whole K range), then nsplit work
-// groups for each of the remaining ntiles - ndp tiles, each over one K segment, writing
-// partial sums for k_gram_reduce.  The split keeps the last, partial wave of tiles from
-// leaving most CUs idle.
+// K segments of the split tiles: nsu units over the nrem = ntiles - ndp last tiles, tile t
+// taking sbase + 1 segments if t < sextra, else sbase (so nsu = nrem * sbase + sextra can match
+// the CU count); tile t's units are split_first(t) .. split_first(t) + segments - 1.
+__host__ __device__ __forceinline__ int split_first(int t, int sbase, int sextra)
+{
+  return t * sbase + (t < sextra ? t : sextra);
+}
+__device__ __forceinline__ void split_unit(int u, int sbase, int sextra, int &t, int &sg, int &ns)
+{
+  const int big = sextra * (sbase + 1);
+  if (u < big) { t = u / (sbase + 1); sg = u % (sbase + 1); ns = sbase + 1; }
+  else { t = sextra + (u - big) / sbase; sg = (u - big) % sbase; ns = sbase; }
+}
+
+// Grid: first the nsu split units, each over one K segment of one of the last tiles, writing
+// partial records for k_gram_reduce (nsu8 groups: the count rounded up to a multiple of 8),
+// then 8 * per_xcd work groups for the first ndp tiles (whole K range).  The
+// split keeps the last, partial wave of tiles from leaving most CUs idle; running it first
+// lets its partial-record stores drain while the full tiles compute, and leaves the full
+// tiles' record stores as the only burst at the end.
 template <int WI, int WJ, int KT, int OK>
 __global__ void __launch_bounds__(128 * WI * WJ) __attribute__((amdgpu_waves_per_eu(2)))
 k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t Lpad, int64_t L, void *__restrict__ out,
-          const int2 *__restrict__ tiles, int ndp, int per_xcd, int ntiles, int nsplit,
+          const int2 *__restrict__ tiles, int ndp, int per_xcd, int ntiles, int sbase, int sextra,
           uint8_t *__restrict__ part)
 {
   typedef CodeStage<32 * WI, 32 * WJ, KT, 128 * WI * WJ> St;
   __shared__ __attribute__((aligned(16))) uint8_t lds[3 * St::BYTES];
-  const int g = blockIdx.x;
   const int64_t nkt = Npad / KT;
+  const int nsu = (ntiles - ndp) * sbase + sextra, nsu8 = (nsu + 7) & ~7;
   int t;
   int64_t k0 = 0, k1 = nkt;
   uint8_t *p = nullptr;
-  if (g < 8 * per_xcd) {
-    // XCD-aware mapping: consecutive tiles of the (bi, bj)-sorted list run on one XCD (one L2).
+  if ((int)blockIdx.x < nsu8) {
+    const int u = blockIdx.x;
+    if (u >= nsu) return;
+    int tr, sg, ns;
+    split_unit(u, sbase, sextra, tr, sg, ns);
+    t = ndp + tr;
+    k0 = nkt * sg / ns; k1 = nkt * (sg + 1) / ns;
+    p = part + (int64_t)u * (32 * WI) * (32 * WJ) * (OK == CNT_U16 ? 32 : 64);
+  } else {
+    // XCD-aware mapping (nsu8 is a multiple of 8, so g & 7 is still the XCD): consecutive
+    // tiles of the (bi, bj)-sorted list run on one XCD (one L2).
+    const int g = blockIdx.x - nsu8;
     t = (g & 7) * per_xcd + (g >> 3);
     if (t >= ndp) return;
-  } else {
-    const int u = g - 8 * per_xcd, sg = u % nsplit;
-    t = ndp + u / nsplit;
-    if (t >= ntiles) return;
-    k0 = nkt * sg / nsplit; k1 = nkt * (sg + 1) / nsplit;
-    p = part + (int64_t)u * (32 * WI) * (32 * WJ) * (OK == CNT_U16 ? 32 : 64);
   }
   const int2 tl = tiles[t];
   gram_tile_i8<WI, WJ, KT, OK>(xt, Lpad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, k0, k1, p);
@@ -285,7 +305,7 @@ k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t Lpad, int64_t L,
 // word, added as words since no half exceeds 65535) of a record.
 template <int BI, int BJ, int OK>
 __global__ void __launch_bounds__(256)
-k_gram_reduce(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, int ndp, int nsplit, int64_t L,
+k_gram_reduce(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, int ndp, int sbase, int sextra, int64_t L,
               void *__restrict__ out)
 {
   constexpr int NQ = (OK == CNT_U16) ? 2 : 4;                       // uint4 per record
@@ -298,9 +318,10 @@ k_gram_reduce(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, 
   const int2 tl = tiles[ndp + t];
   const int64_t i = (int64_t)tl.x * BI + ii, j = (int64_t)tl.y * BJ + jj;
   if (!(i < j && j < L)) return;
-  const uint4 *p = (const uint4 *)part + ((int64_t)t * nsplit * BI * BJ + pr) * NQ + h;
+  const int ns = sbase + (t < sextra ? 1 : 0);
+  const uint4 *p = (const uint4 *)part + ((int64_t)split_first(t, sbase, sextra) * BI * BJ + pr) * NQ + h;
   uint4 s = make_uint4(0, 0, 0, 0);
-  for (int k = 0; k < nsplit; k++) {
+  for (int k = 0; k < ns; k++) {
     const uint4 v = p[(int64_t)k * BI * BJ * NQ];
     s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
   }
@@ -510,16 +531,18 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, char *errbuf)
   // into enough segments to occupy every CU, then reduced
   const int ncu = c->ncu > 0 ? c->ncu : 256;
   const int64_t nkt = m->Npad / KT;
-  int ndp = ntiles, nsplit = 1;
+  int ndp = ntiles, sbase = 1, sextra = 0;
   const int rem = ntiles % ncu;
   if (c->gram_split && rem > 0 && rem <= ncu / 2 && nkt >= 2) {
+    // every CU takes one K segment of the last tiles before the full tiles start
     ndp = ntiles - rem;
-    nsplit = (int)std::min<int64_t>(nkt, ncu / rem);
+    const int nsu = (int)std::min<int64_t>((int64_t)ncu, (int64_t)rem * nkt);
+    sbase = nsu / rem; sextra = nsu % rem;
   }
-  const int nrem = ntiles - ndp;
+  const int nrem = ntiles - ndp, nsu = nrem * sbase + sextra;
   uint8_t *part = nullptr;
   if (nrem > 0) {
-    const size_t need = (size_t)nrem * nsplit * BI * BJ * rsg_rec_bytes(OK);
+    const size_t need = (size_t)nsu * BI * BJ * rsg_rec_bytes(OK);
     if (c->gscr_cap < need) {
       if (c->d_gscr) hipFree(c->d_gscr);
       c->d_gscr = nullptr; c->gscr_cap = 0;
@@ -529,16 +552,16 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, char *errbuf)
     part = (uint8_t *)c->d_gscr;
   }
   const int per = (ndp + 7) / 8;
-  dim3 grid((unsigned)(8 * per + nrem * nsplit)), block(128 * WI * WJ);
+  dim3 grid((unsigned)(8 * per + ((nsu + 7) & ~7))), block(128 * WI * WJ);
   int t; rsg_tstart(c, "gram_i8", &t);
   hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->Lpad, m->L, out,
-                     tiles, ndp, per, ntiles, nsplit, part);
+                     tiles, ndp, per, ntiles, sbase, sextra, part);
   rsg_tstop(c, t);
   if (nrem > 0) {
     rsg_tstart(c, "gram_reduce", &t);
     const int64_t nq = (int64_t)nrem * BI * BJ * (OK == CNT_U16 ? 2 : 4);
     hipLaunchKernelGGL((k_gram_reduce<BI, BJ, OK>), dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, c->stream, part, tiles,
-                       ndp, nsplit, m->L, out);
+                       ndp, sbase, sextra, m->L, out);
     rsg_tstop(c, t);
   }
   HIPCHK(hipGetLastError());
