@@ -25,7 +25,7 @@
 #pragma clang fp contract(off)
 
 enum { D_N = 0, D_L, D_NPAD, D_LPAD, D_XT, D_CNT, D_COV, D_PM, D_ROWS, D_HIST, D_LEAF, D_BPACK, D_BCOL,
-       D_BSTAT, D_BEL, D_NB32, D_GROUP, D_BHIST, D_BPMT, D_PMP, D_BCOL4 };
+       D_BSTAT, D_BEL, D_NB32, D_GROUP, D_BHIST, D_PMP, D_BCOL16 };
 #define HIST_EPB 65536       // matrix elements per histogram block
 #define NBMAX 16384          // histogram bins per item (an item needing more is reported as failed)
 #define ELPB 1024            // matrix elements per block in the element-wise kernels
@@ -58,32 +58,20 @@ k_pack_b(const int64_t *__restrict__ desc, int n, const uint8_t *__restrict__ le
   pack_xt_chunk(tile, s0, c0, Lpad, X);
 }
 
-// corr_Marginals per item with the single-alignment kernels' tiles and reduction order
-// (pm_tile / pm_reduce_col), so a replicate's marginals match its rsg_corr_Probs bit for bit
-__global__ void __launch_bounds__(64 * PMW)
-k_pm_tile_b(const int64_t *__restrict__ desc, int n, const uint16_t *__restrict__ cnt, double *__restrict__ part)
-{
-  const int it = find_item(desc, n, D_BPMT, blockIdx.x);
-  const int64_t *d = desc + it * RSG_BDESC;
-  const int64_t L = d[D_L], nbt = (L + PMT - 1) / PMT, local = blockIdx.x - d[D_BPMT];
-  const int bi = (int)(local / (nbt * PMS)), bj = (int)(local % (nbt * PMS)) / PMS, sub = (int)(local % PMS);
-  if (bi > bj) return;
-  double *pi = part + d[D_PMP];
-  pm_tile<CNT_U16>(cnt + d[D_CNT], L, (int)nbt, bi, bj, sub, pi, pi + pm_npi(nbt));
-}
-
+// corr_Marginals per item from the 32 x 32 block partials the batched Gram leaves (the
+// single-alignment path's layout, order and reduction: a replicate's marginals match its
+// rsg_corr_Probs bit for bit)
 __global__ void __launch_bounds__(256)
 k_pm_reduce_b(const int64_t *__restrict__ desc, int n, const double *__restrict__ part, double *__restrict__ pm,
               double tol, int *__restrict__ flag)
 {
-  const int it = find_item(desc, n, D_BCOL4, blockIdx.x);
+  const int it = find_item(desc, n, D_BCOL16, blockIdx.x);
   const int64_t *d = desc + it * RSG_BDESC;
-  const int64_t L = d[D_L], nbt = (L + PMT - 1) / PMT;
-  const int64_t col = (blockIdx.x - d[D_BCOL4]) * 4 + (threadIdx.x >> 6);
+  const int64_t L = d[D_L], nb32 = d[D_NB32];
+  const int64_t col = (blockIdx.x - d[D_BCOL16]) * 16 + (threadIdx.x >> 4);
   if (col >= L) return;
   const double *pi = part + d[D_PMP];
-  pm_reduce_col(pi, pi + pm_npi(nbt), (int)nbt, L, col, threadIdx.x & 63, tol, pm + d[D_PM],
-                pm + d[D_PM] + 4 * L, flag);
+  pm32_reduce_col(pi, pi + pm32_n(nb32), (int)nb32, L, col, threadIdx.x & 15, tol, pm + d[D_PM], pm + d[D_PM] + 4 * L, flag);
 }
 
 __global__ void __launch_bounds__(256)
@@ -331,7 +319,7 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   std::vector<int> cls((size_t)nj);
   int64_t o_xt = 0, o_cnt = 0, o_cov = 0, o_pm = 0, o_rows = 0, o_leaf = 0, nodes = 0, seqs = 0;
   int64_t b_pack = 0, b_col = 0, b_stat = 0, b_el = 0, b_sim = 0, b_hist = 0, nmax = 1;
-  int64_t b_pmt = 0, b_col4 = 0, o_pmp = 0;
+  int64_t b_col16 = 0, o_pmp = 0;
   std::vector<int4> tiles, btiles;        // 32-column tiles (small alignments), 64-column tiles (large)
   for (int k = 0; k < nj; k++) {
     const rsg_null_job &J = jobs[k];
@@ -345,9 +333,8 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     d[D_BPACK] = b_pack; d[D_BCOL] = b_col; d[D_BSTAT] = b_stat; d[D_BEL] = b_el;
     const int64_t nb32 = (L + 31) / 32;
     d[D_NB32] = nb32; d[D_GROUP] = J.group; d[D_BHIST] = b_hist;
-    const int64_t nbt = (L + PMT - 1) / PMT;
-    d[D_BPMT] = b_pmt; d[D_PMP] = o_pmp; d[D_BCOL4] = b_col4;
-    b_pmt += nbt * nbt * PMS; b_col4 += (L + 3) / 4; o_pmp += pm_npi(nbt) + pm_npj(nbt);
+    d[D_PMP] = o_pmp; d[D_BCOL16] = b_col16;
+    b_col16 += (L + 15) / 16; o_pmp += 2 * pm32_n(nb32);
     cls[(size_t)k] = (N <= 8 || L <= 50) ? RSG_C2 : RSG_C16;           // CSELECT (correlators.c:335-341)
     SimItem &s = sim[(size_t)k];
     s.N = N; s.L = L; s.tree_off = nodes; s.seq_off = seqs; s.leaf_off = o_leaf; s.seed = J.seed; s.stream = J.stream;
@@ -458,12 +445,12 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
                        d_leaves) != RSG_OK) return RSG_FAIL;
     hipLaunchKernelGGL(k_pack_b, dim3((unsigned)b_pack), dim3(256), 0, c->stream, d_desc, nj, d_leaves, d_xt);
     if (feats && rsg_feat_launch(c, d_fitems, ng, fblocks, d_xt, d_fcnt) != RSG_OK) return RSG_FAIL;
-    if (rsg_gram_batch_launch(c, d_btiles, (int)btiles.size(), d_tiles, (int)tiles.size(), d_desc, d_xt, d_cnt) != RSG_OK)
+    if (rsg_gram_batch_launch(c, d_btiles, (int)btiles.size(), d_tiles, (int)tiles.size(), d_desc, d_xt, d_cnt, d_pmp,
+                              (int)D_PMP) != RSG_OK)
       return RSG_FAIL;
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->d_flag, 16);
     hipLaunchKernelGGL(k_logtab_b, dim3((unsigned)((nmax + 256) / 256)), dim3(256), 0, c->stream, nmax, d_lgc);
-    hipLaunchKernelGGL(k_pm_tile_b, dim3((unsigned)b_pmt), dim3(64 * PMW), 0, c->stream, d_desc, nj, d_cnt, d_pmp);
-    hipLaunchKernelGGL(k_pm_reduce_b, dim3((unsigned)b_col4), dim3(256), 0, c->stream, d_desc, nj, d_pmp, d_pm, 1e-6, c->d_flag);
+    hipLaunchKernelGGL(k_pm_reduce_b, dim3((unsigned)b_col16), dim3(256), 0, c->stream, d_desc, nj, d_pmp, d_pm, 1e-6, c->d_flag);
     hipLaunchKernelGGL(k_stat_b, dim3((unsigned)b_stat), dim3(256), 0, c->stream, d_desc, nj, A0, d_cnt, d_pm, d_cov, d_cls, base);
     if (corr) {
       hipLaunchKernelGGL(k_rowsum_b, dim3((unsigned)b_col), dim3(256), 0, c->stream, d_desc, nj, d_cov, d_rows);

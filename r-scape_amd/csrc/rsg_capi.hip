@@ -171,8 +171,13 @@ extern "C" int rsg_corr_Probs(rsg_ctx *c, rsg_msa *m, int mode, double tol, char
     const int kind = rsg_int_kind(m->N);
     if (alloc_counts(m, &m->d_cnt, kind, errbuf) != RSG_OK) return RSG_FAIL;
     m->cnt_kind = kind;
-    if (rsg_gram_counts_i8(c, m, m->d_cnt, kind, errbuf) != RSG_OK) return RSG_FAIL;
+    double *pm_part = nullptr;                    // uint16 records: marginal partials from the Gram epilogue
+    if (kind == CNT_U16 && m->L >= 2 && !(pm_part = rsg_pm32_scratch(c, m->L, errbuf))) return RSG_FAIL;
+    m->pm_fused = 0;
+    if (rsg_gram_counts_i8(c, m, m->d_cnt, kind, pm_part, (int *)(c->d_res + 4), errbuf) != RSG_OK) return RSG_FAIL;
+    m->pm_fused = (pm_part != nullptr);
   } else if (mode == RSG_COUNTS_BF16 || mode == RSG_COUNTS_BF16X3) {
+    m->pm_fused = 0;
     if (alloc_counts(m, &m->d_cnt, CNT_F32, errbuf) != RSG_OK) return RSG_FAIL;
     m->cnt_kind = CNT_F32;
     if (rsg_gram_counts_bf16(c, m, mode == RSG_COUNTS_BF16X3 ? 3 : 2, (float *)m->d_cnt, errbuf) != RSG_OK) return RSG_FAIL;
@@ -276,7 +281,7 @@ extern "C" int rsg_corr_CalculateRAF(rsg_ctx *c, rsg_msa *m, int rafs, const dou
   // does not go through corr_Probs: form the unit-weight counts here.
   m->cnt_u_kind = rsg_int_kind(m->N);
   if (!m->d_cnt_u) HIPCHK(hipMalloc(&m->d_cnt_u, (size_t)rsg_rec_bytes(m->cnt_u_kind) * (size_t)m->L * (size_t)m->L));
-  if (rsg_gram_counts_i8(c, m, m->d_cnt_u, m->cnt_u_kind, errbuf) != RSG_OK) return RSG_FAIL;
+  if (rsg_gram_counts_i8(c, m, m->d_cnt_u, m->cnt_u_kind, nullptr, nullptr, errbuf) != RSG_OK) return RSG_FAIL;
   return rsg_raf(c, m, rafs, allow, errbuf);
 }
 

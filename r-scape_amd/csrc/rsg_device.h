@@ -238,6 +238,97 @@ __device__ void pm_tile(const void *__restrict__ cnt, int64_t L, int nbt, int bi
   if (c < PMJ) part_j[(t * PMT + sub * PMJ + c) * 4 + x] = sj[c][x];
 }
 
+// ---- corr_Marginals from unit-weight counts, fused into the int8 Gram.  The partial sums are
+// formed per 32 x 32 column block (bi <= bj, in 32-column units) in the MFMA result layout:
+// lane (col = lane & 31, khalf = lane >> 5) holds i = 32 bi + col and, for r = 0..15,
+// j = 32 bj + 8 (r / 4) + 4 khalf + r % 4.  Per pair: inv = 1 / (ns + 16e-10), row term
+// (rowsum_x + 4e-10) inv, column term (colsum_y + 4e-10) inv, with ns, rowsum and colsum exact
+// integers; pairs with i >= j, j >= L or ns = 0 have inv = 0 and add +0.
+//   i side: a lane adds its row terms over r = 0..7 and over r = 8..15, each in order, then the
+//   two sums, then the two khalf lanes of a column (pm_khalf_sum).
+//   j side: the 32 column terms of r = 8 h .. 8 h + 7 (index 4 (r - 8 h) + y) are summed over the
+//   32 lanes of a khalf by pm_rscatter32, after which lane col holds index col.
+// The Gram epilogue, the split-K fix-up and the batched Gram all follow this, so the partials do
+// not depend on tile shape or K splitting.  Layout: part_i[blk][32][4] and part_j[blk][32][4],
+// blk = pm_tri(bi, bj, nb32).
+__host__ __device__ __forceinline__ int64_t pm32_n(int64_t nb32) { return pm_tri(nb32, nb32, nb32) * 32 * 4; }
+
+__device__ __forceinline__ double pm_inv(bool valid, int ns)
+{
+#pragma clang fp contract(off)
+  return (valid && ns > 0) ? 1. / ((double)ns + 16e-10) : 0.;
+}
+__device__ __forceinline__ double pm_term(int sum, double inv)
+{
+#pragma clang fp contract(off)
+  return ((double)sum + 4e-10) * inv;
+}
+// i side: the khalf halves of a lane pair combined (a + b is exact-commutative, so both lanes agree)
+__device__ __forceinline__ double pm_khalf_sum(double a) { return a + __shfl_xor(a, 32, 64); }
+// j side: reduce-scatter over the 32 lanes of a khalf.  Butterfly over lane bits 16, 8, 4, 2, 1:
+// at each step a lane keeps the lower half of its values if its bit is clear, the upper half
+// otherwise, and adds the partner's copy of the half it keeps.  Lane col ends with the sum over
+// the 32 lanes of value index col (31 exchanges instead of 32 x 5).
+// (The stage width is a template argument so that every index is a constant and v stays in
+// registers.)
+template <int M>
+__device__ __forceinline__ void pm_rscatter_step(double (&v)[32], int col)
+{
+  const bool up = (col & M) != 0;
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const double send = up ? v[k] : v[k + M];
+    const double keep = up ? v[k + M] : v[k];
+    v[k] = keep + __shfl_xor(send, M, 64);
+  }
+}
+__device__ __forceinline__ double pm_rscatter32(double (&v)[32], int col)
+{
+  pm_rscatter_step<16>(v, col);
+  pm_rscatter_step<8>(v, col);
+  pm_rscatter_step<4>(v, col);
+  pm_rscatter_step<2>(v, col);
+  pm_rscatter_step<1>(v, col);
+  return v[0];
+}
+
+// 16 lanes (t = 0..15) per column: lane t adds blocks k = t, t + 16, ... of the column's list
+// (blocks (b, b..nb32-1) on the i side, then (0..b, b) on the j side) in order, an xor tree over
+// the 16 lanes combines them (every lane ends with the same totals) and lane x < 4 writes
+// pm[col][x] and its log.  Every block holds a pair i < j < L, so its tile is computed, except a
+// last diagonal block with a single column (32 b + 1 >= L), which no producer writes and which
+// is skipped here.
+__device__ __forceinline__ void pm32_reduce_col(const double *__restrict__ part_i, const double *__restrict__ part_j,
+                                                int nb32, int64_t L, int64_t col, int t, double tol,
+                                                double *__restrict__ pm, double *__restrict__ lpm, int *__restrict__ flag)
+{
+  const int b = (int)(col / 32), o = (int)(col % 32);
+  const bool diag = 32 * (int64_t)b + 1 < L;
+  double v[4] = {0., 0., 0., 0.};
+  for (int k = t; k < nb32 + 1; k += 16) {
+    int64_t base;
+    const double *src;
+    if (k < nb32 - b) { if (k == 0 && !diag) continue; src = part_i; base = pm_tri(b, b + k, nb32) * 32 + o; }
+    else if (k - (nb32 - b) <= b) { if (k - (nb32 - b) == b && !diag) continue; src = part_j; base = pm_tri(k - (nb32 - b), b, nb32) * 32 + o; }
+    else continue;
+    const double4 q = *(const double4 *)(src + base * 4);
+    v[0] += q.x; v[1] += q.y; v[2] += q.z; v[3] += q.w;
+  }
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1)
+#pragma unroll
+    for (int x = 0; x < 4; x++) v[x] += __shfl_xor(v[x], off, 16);
+  if (t >= 4) return;
+  const double sum = v[0] + v[1] + v[2] + v[3];
+  double p[4], tot = 0.;
+#pragma unroll
+  for (int x = 0; x < 4; x++) { p[x] = (sum != 0.) ? v[x] / sum : 0.25; tot += p[x]; }
+  const double px = (t == 0) ? p[0] : (t == 1) ? p[1] : (t == 2) ? p[2] : p[3];
+  if (!(px >= 0. && px <= 1.) || (t == 0 && fabs(tot - 1.) > tol)) atomicOr(flag, 1);
+  pm[col * 4 + t] = px;
+  lpm[col * 4 + t] = (px > 0.) ? log(px) : -INFINITY;
+}
+
 // one wave (lane = 0..63) per column: lanes take the column's tiles in a fixed assignment,
 // a fixed shuffle tree adds them
 __device__ __forceinline__ void pm_reduce_col(const double *__restrict__ part_i, const double *__restrict__ part_j,

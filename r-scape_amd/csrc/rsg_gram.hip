@@ -30,7 +30,7 @@
 // Output: a 16-cell record per column pair i < j at record rsg_rec(i, j) = j*L + i,
 // cell x*4+y (the layout of pp[i][j][IDX(x,y,4)] in the reference); unit-weight records hold
 // uint16 counts (N <= 65535), weighted ones FP32.
-#include "rsg_internal.h"
+#include "rsg_device.h"
 #include <map>
 #include <tuple>
 #include <algorithm>
@@ -87,6 +87,10 @@ struct CodeStage {
   static constexpr int NCH = (BI + BJ) * NKC;
   static constexpr int NR = (NCH + NT - 1) / NT;
   static constexpr int NINST = NCH / 64;                    // 64-chunk LDS-direct loads per stage
+  // LDS of the int8 Gram: three stage images; with fused marginals at least the epilogue's
+  // exchange (8 uint4 per lane and wave, NT * 128 bytes) and row sums (4 doubles per lane of
+  // half the waves, NT * 16 bytes)
+  static constexpr int lds_bytes(bool pm) { return (pm && NT * 144 > 3 * BYTES) ? NT * 144 : 3 * BYTES; }
 };
 
 // VGPR staging (the BF16 Gram): every thread moves exactly NR chunks (NCH is a multiple of
@@ -152,14 +156,19 @@ __device__ __forceinline__ unsigned pack_cells(int a, int sa, int b, int sb)
 // epilogue shifts the symbol scale out and writes uint16 (OK = CNT_U16) or uint32 (CNT_U32)
 // records; each half writes the cells of its own symbol rows.
 // One work-group tile (i0, j0) of one alignment over K-tiles [kt0, kt1); `lds` holds
-// 2 * St::BYTES.  With part == nullptr the records go to `out`; otherwise the tile's partial
+// St::lds_bytes(pm_i != nullptr).  With part == nullptr the records go to `out`; otherwise the tile's partial
 // counts go, unconditionally and in the record format, to partial record jj * BI + ii (ii, jj:
 // column offsets in the tile) for a later reduction.
+// With pm_i != nullptr (uint16 records over the whole K range only) the epilogue also leaves the
+// marginal partials of the tile's 32 x 32 blocks (pm_i / pm_j, layout and order in rsg_device.h).
 template <int WI, int WJ, int KT, int OK>
 __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int64_t Lpad, int64_t L,
                                              void *__restrict__ out, int64_t i0, int64_t j0, uint8_t *lds,
-                                             int64_t kt0, int64_t kt1, void *__restrict__ part)
+                                             int64_t kt0, int64_t kt1, void *__restrict__ part,
+                                             double *__restrict__ pm_i = nullptr, double *__restrict__ pm_j = nullptr,
+                                             int64_t nb32 = 0)
 {
+#pragma clang fp contract(off)
   constexpr int BI = 32 * WI, BJ = 32 * WJ, NT = 128 * WI * WJ, KS = KT / 32;
   typedef CodeStage<BI, BJ, KT, NT> St;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -246,6 +255,73 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
                           (unsigned)acc[x][2][r] >> (x0 + x + 2), (unsigned)acc[x][3][r] >> (x0 + x + 3));
     }
   }
+  if (OK != CNT_U16 || pm_i == nullptr) return;
+
+  // marginal partials.  The two halves of a block split its 16 r rows: half h takes
+  // r = 8 h .. 8 h + 7 and receives the sibling's cells of those rows (8 x uint16 per pair, 64 KiB
+  // for 8 waves) through the LDS images, which every wave has finished with.  No FMA contraction
+  // anywhere in this function, so every producer rounds alike.
+  __syncthreads();
+  uint4 *xch = (uint4 *)lds;
+  uint4 own[8];                                   // this half's cells of its own rows, packed alike
+#pragma unroll
+  for (int rl = 0; rl < 8; rl++) {
+    unsigned cs[2][4], co[2][4];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        const unsigned lo = (unsigned)acc[x][y][rl] >> (x0 + x + y), hi = (unsigned)acc[x][y][8 + rl] >> (x0 + x + y);
+        cs[x][y] = half ? lo : hi;
+        co[x][y] = half ? hi : lo;
+      }
+    xch[(wave * 8 + rl) * 64 + lane] = make_uint4(cs[0][0] | (cs[0][1] << 16), cs[0][2] | (cs[0][3] << 16),
+                                                  cs[1][0] | (cs[1][1] << 16), cs[1][2] | (cs[1][3] << 16));
+    own[rl] = make_uint4(co[0][0] | (co[0][1] << 16), co[0][2] | (co[0][3] << 16), co[1][0] | (co[1][1] << 16),
+                         co[1][2] | (co[1][3] << 16));
+  }
+  __syncthreads();
+  const int sib = blk + (1 - half) * (WI * WJ);
+  const int bi32 = (int)(i0 / 32) + wi, bj32 = (int)(j0 / 32) + wj, Li = (int)L, ic = bi32 * 32 + col;
+  const bool blkok = bi32 <= bj32 && bj32 < (int)nb32;      // blocks of the pm32 layout only
+  double ai[4] = {0., 0., 0., 0.}, cv[32];
+#pragma unroll
+  for (int rl = 0; rl < 8; rl++) {
+    const int r = 8 * half + rl, j = bj32 * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
+    const uint4 sv = xch[(sib * 8 + rl) * 64 + lane];
+    const unsigned sw[4] = {sv.x, sv.y, sv.z, sv.w}, ow[4] = {own[rl].x, own[rl].y, own[rl].z, own[rl].w};
+    int colf[4], ro[2] = {0, 0}, rsb[2] = {0, 0}, ns = 0;
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const int o0 = (int)((ow[y >> 1] >> (16 * (y & 1))) & 0xFFFFu), o1 = (int)((ow[2 + (y >> 1)] >> (16 * (y & 1))) & 0xFFFFu);
+      const int s0 = (int)((sw[y >> 1] >> (16 * (y & 1))) & 0xFFFFu), s1 = (int)((sw[2 + (y >> 1)] >> (16 * (y & 1))) & 0xFFFFu);
+      ro[0] += o0; ro[1] += o1; rsb[0] += s0; rsb[1] += s1;
+      colf[y] = o0 + o1 + s0 + s1;
+      ns += colf[y];
+    }
+    const int rs[4] = {half ? rsb[0] : ro[0], half ? rsb[1] : ro[1], half ? ro[0] : rsb[0], half ? ro[1] : rsb[1]};
+    const double inv = pm_inv(blkok && ic < j && j < Li, ns);
+#pragma unroll
+    for (int x = 0; x < 4; x++) ai[x] += pm_term(rs[x], inv);
+#pragma unroll
+    for (int y = 0; y < 4; y++) cv[4 * rl + y] = pm_term(colf[y], inv);
+  }
+  const double sj = pm_rscatter32(cv, col);                  // index col: r = 8 half + col / 4, y = col % 4
+  const int64_t bq = blkok ? pm_tri(bi32, bj32, nb32) : 0;
+  const int rj = 8 * half + (col >> 2), jbo = 8 * (rj >> 2) + 4 * khalf + (rj & 3);
+  if (blkok) pm_j[(bq * 32 + jbo) * 4 + (col & 3)] = sj;
+  // i side: half 1 hands its sums over r = 8..15 to half 0, which adds them to its own (r = 0..7)
+  double *aix = (double *)(lds + NT * 128);
+  if (half == 1)
+#pragma unroll
+    for (int x = 0; x < 4; x++) aix[(blk * 4 + x) * 64 + lane] = ai[x];
+  __syncthreads();
+  if (half == 0)
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      const double si = pm_khalf_sum(ai[x] + aix[(blk * 4 + x) * 64 + lane]);
+      if (blkok && khalf == 0) pm_i[(bq * 32 + col) * 4 + x] = si;
+    }
 }
 
 // K segments of the split tiles: nsu units over the nrem = ntiles - ndp last tiles, tile t
@@ -272,10 +348,10 @@ template <int WI, int WJ, int KT, int OK>
 __global__ void __launch_bounds__(128 * WI * WJ) __attribute__((amdgpu_waves_per_eu(2)))
 k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t Lpad, int64_t L, void *__restrict__ out,
           const int2 *__restrict__ tiles, int ndp, int per_xcd, int ntiles, int sbase, int sextra,
-          uint8_t *__restrict__ part)
+          uint8_t *__restrict__ part, double *__restrict__ pm_part, int *__restrict__ pmflag)
 {
   typedef CodeStage<32 * WI, 32 * WJ, KT, 128 * WI * WJ> St;
-  __shared__ __attribute__((aligned(16))) uint8_t lds[3 * St::BYTES];
+  __shared__ __attribute__((aligned(16))) uint8_t lds[St::lds_bytes(OK == CNT_U16)];
   const int64_t nkt = Npad / KT;
   const int nsu = (ntiles - ndp) * sbase + sextra, nsu8 = (nsu + 7) & ~7;
   int t;
@@ -296,36 +372,105 @@ k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t Lpad, int64_t L,
     t = (g & 7) * per_xcd + (g >> 3);
     if (t >= ndp) return;
   }
+  if (pmflag && blockIdx.x == 0 && threadIdx.x == 0) *pmflag = 0;    // set by the marginal reduction
   const int2 tl = tiles[t];
-  gram_tile_i8<WI, WJ, KT, OK>(xt, Lpad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, k0, k1, p);
+  const int64_t nb32 = (L + 31) / 32;
+  double *pmi = (pm_part && !p) ? pm_part : nullptr;               // split segments: in k_gram_fix
+  gram_tile_i8<WI, WJ, KT, OK>(xt, Lpad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, k0, k1, p,
+                               pmi, pmi ? pmi + pm32_n(nb32) : nullptr, nb32);
 }
 
-// Sum the nsplit partial tiles of each split tile into its records (integer sums: exact);
-// one thread per 16-byte quarter (uint32 records) or half (uint16 records: two counts per
-// word, added as words since no half exceeds 65535) of a record.
+// Split tiles: one work group (4 waves) per 32 x 32 block sums the block's partial records over
+// the tile's K segments (integer sums: exact; uint16 records add as words, no half exceeding
+// 65535) and writes the records; wave w takes r = 4w .. 4w + 3 of the MFMA result layout.  For
+// uint16 records with pm_part the block's marginal partials follow with the arithmetic and order
+// of the Gram epilogue: the row and column terms go through LDS, wave 0 adds the row terms over
+// r = 0..7 and r = 8..15 and then the two sums, waves 2 and 3 reduce-scatter the column terms of
+// r = 0..7 and r = 8..15 (index 4 (r % 8) + y).
 template <int BI, int BJ, int OK>
 __global__ void __launch_bounds__(256)
-k_gram_reduce(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, int ndp, int sbase, int sextra, int64_t L,
-              void *__restrict__ out)
+k_gram_fix(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, int ndp, int sbase, int sextra, int64_t L,
+           void *__restrict__ out, double *__restrict__ pm_part)
 {
+#pragma clang fp contract(off)
   constexpr int NQ = (OK == CNT_U16) ? 2 : 4;                       // uint4 per record
-  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;       // (split tile, pair, part of record)
-  const int64_t per_tile = (int64_t)BI * BJ * NQ;
-  const int t = (int)(q / per_tile);
-  const int64_t rem = q % per_tile, pr = rem / NQ;
-  const int h = (int)(rem % NQ);
-  const int ii = (int)(pr % BI), jj = (int)(pr / BI);
-  const int2 tl = tiles[ndp + t];
-  const int64_t i = (int64_t)tl.x * BI + ii, j = (int64_t)tl.y * BJ + jj;
-  if (!(i < j && j < L)) return;
-  const int ns = sbase + (t < sextra ? 1 : 0);
-  const uint4 *p = (const uint4 *)part + ((int64_t)split_first(t, sbase, sextra) * BI * BJ + pr) * NQ + h;
-  uint4 s = make_uint4(0, 0, 0, 0);
-  for (int k = 0; k < ns; k++) {
-    const uint4 v = p[(int64_t)k * BI * BJ * NQ];
-    s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+  constexpr int NBI = BI / 32, NB = NBI * (BJ / 32);
+  __shared__ double rowt[16][4][64], colt[16][4][64];
+  const int tr = blockIdx.x / NB, sb = blockIdx.x % NB, wi = sb % NBI, wj = sb / NBI;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31, khalf = lane >> 5;
+  const int2 tl = tiles[ndp + tr];
+  const int bi32 = tl.x * NBI + wi, bj32 = tl.y * (BJ / 32) + wj, Li = (int)L, ic = bi32 * 32 + col;
+  const int ns = sbase + (tr < sextra ? 1 : 0);
+  const uint4 *p0 = (const uint4 *)part + (int64_t)split_first(tr, sbase, sextra) * BI * BJ * NQ;
+  uint4 s[4][NQ];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int h = 0; h < NQ; h++) s[q][h] = make_uint4(0, 0, 0, 0);
+  for (int k = 0; k < ns; k++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int r = 4 * wave + q, jb = 8 * (r >> 2) + 4 * khalf + (r & 3);
+      const int64_t pr = (int64_t)(wj * 32 + jb) * BI + wi * 32 + col;   // partial record in the tile
+#pragma unroll
+      for (int h = 0; h < NQ; h++) {
+        const uint4 v = p0[((int64_t)k * BI * BJ + pr) * NQ + h];
+        s[q][h].x += v.x; s[q][h].y += v.y; s[q][h].z += v.z; s[q][h].w += v.w;
+      }
+    }
+  const int nb32 = (Li + 31) / 32;
+  const bool blkok = bi32 <= bj32 && bj32 < nb32;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int r = 4 * wave + q, j = bj32 * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
+    const bool valid = ic < j && j < Li;
+    if (valid)
+#pragma unroll
+      for (int h = 0; h < NQ; h++) ((uint4 *)out)[rsg_rec(ic, j, L) * NQ + h] = s[q][h];
+    if (OK != CNT_U16 || pm_part == nullptr) continue;
+    int c[16];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const unsigned w[4] = {s[q][h].x, s[q][h].y, s[q][h].z, s[q][h].w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) { c[8 * h + 2 * e] = (int)(w[e] & 0xFFFFu); c[8 * h + 2 * e + 1] = (int)(w[e] >> 16); }
+    }
+    int colf[4] = {0, 0, 0, 0}, rs[4] = {0, 0, 0, 0}, nsp = 0;
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) { colf[y] += c[x * 4 + y]; rs[x] += c[x * 4 + y]; }
+#pragma unroll
+    for (int y = 0; y < 4; y++) nsp += colf[y];
+    const double inv = pm_inv(blkok && valid, nsp);
+#pragma unroll
+    for (int x = 0; x < 4; x++) { rowt[r][x][lane] = pm_term(rs[x], inv); colt[r][x][lane] = pm_term(colf[x], inv); }
   }
-  ((uint4 *)out)[rsg_rec(i, j, L) * NQ + h] = s;
+  if (OK != CNT_U16 || pm_part == nullptr) return;
+  __syncthreads();
+  if (!blkok || wave == 1) return;
+  double *pm_i = pm_part, *pm_j = pm_part + pm32_n(nb32);
+  const int64_t bq = pm_tri(bi32, bj32, nb32);
+  if (wave == 0) {
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      double a0 = 0., a1 = 0.;
+#pragma unroll
+      for (int rl = 0; rl < 8; rl++) { a0 += rowt[rl][x][lane]; a1 += rowt[8 + rl][x][lane]; }
+      const double si = pm_khalf_sum(a0 + a1);
+      if (khalf == 0) pm_i[(bq * 32 + col) * 4 + x] = si;
+    }
+  } else {
+    const int h = wave - 2;
+    double cv[32];
+#pragma unroll
+    for (int rl = 0; rl < 8; rl++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) cv[4 * rl + y] = colt[8 * h + rl][y][lane];
+    const double sj = pm_rscatter32(cv, col);
+    const int rj = 8 * h + (col >> 2), jbo = 8 * (rj >> 2) + 4 * khalf + (rj & 3);
+    pm_j[(bq * 32 + jbo) * 4 + (col & 3)] = sj;
+  }
 }
 
 // Batched int8 Gram over many alignments: one work group per (item, tile); tiles are
@@ -335,23 +480,27 @@ k_gram_reduce(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, 
 template <int W, int KT>
 __global__ void __launch_bounds__(128 * W * W) __attribute__((amdgpu_waves_per_eu(2)))
 k_gram_i8_batch(const int4 *__restrict__ tiles, const int64_t *__restrict__ desc, const uint8_t *__restrict__ xt,
-                uint16_t *__restrict__ cnt)
+                uint16_t *__restrict__ cnt, double *__restrict__ pm_part, int pm_field)
 {
   typedef CodeStage<32 * W, 32 * W, KT, 128 * W * W> St;
-  __shared__ __attribute__((aligned(16))) uint8_t lds[3 * St::BYTES];
+  __shared__ __attribute__((aligned(16))) uint8_t lds[St::lds_bytes(true)];
   const int4 tl = tiles[blockIdx.x];                    // (item, bi, bj, -)
   const int64_t *d = desc + (int64_t)tl.x * RSG_BDESC;  // item descriptor (see rsg_batch.hip)
+  const int64_t nb32 = (d[1] + 31) / 32;
+  double *pmi = pm_part + d[pm_field];
   gram_tile_i8<W, W, KT, CNT_U16>(xt + d[4], d[3], d[1], cnt + d[5], (int64_t)tl.y * 32 * W, (int64_t)tl.z * 32 * W, lds,
-                         0, d[2] / KT, nullptr);
+                         0, d[2] / KT, nullptr, pmi, pmi + pm32_n(nb32), nb32);
 }
 
 int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_big, int nbig, const int4 *d_small, int nsmall,
-                          const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt)
+                          const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt, double *d_pm_part, int pm_field)
 {
   if (nbig > 0)
-    hipLaunchKernelGGL((k_gram_i8_batch<2, 256>), dim3((unsigned)nbig), dim3(512), 0, c->stream, d_big, d_desc, d_xt, d_cnt);
+    hipLaunchKernelGGL((k_gram_i8_batch<2, 256>), dim3((unsigned)nbig), dim3(512), 0, c->stream, d_big, d_desc, d_xt, d_cnt,
+                       d_pm_part, pm_field);
   if (nsmall > 0)
-    hipLaunchKernelGGL((k_gram_i8_batch<1, 64>), dim3((unsigned)nsmall), dim3(128), 0, c->stream, d_small, d_desc, d_xt, d_cnt);
+    hipLaunchKernelGGL((k_gram_i8_batch<1, 64>), dim3((unsigned)nsmall), dim3(128), 0, c->stream, d_small, d_desc, d_xt, d_cnt,
+                       d_pm_part, pm_field);
   return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
 }
 
@@ -518,7 +667,7 @@ static const int2 *tile_list(int device, int64_t L, int BI, int BJ, int *ntiles,
 }
 
 template <int WI, int WJ, int KT, int OK>
-static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, char *errbuf)
+static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, double *pm_part, int *pmflag, char *errbuf)
 {
   constexpr int BI = 32 * WI, BJ = 32 * WJ;
   if (m->Lpad % BI || m->Lpad % BJ || m->Npad % KT) return rsg_fail(errbuf, "gram: padding does not fit the tile");
@@ -555,13 +704,12 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, char *errbuf)
   dim3 grid((unsigned)(8 * per + ((nsu + 7) & ~7))), block(128 * WI * WJ);
   int t; rsg_tstart(c, "gram_i8", &t);
   hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->Lpad, m->L, out,
-                     tiles, ndp, per, ntiles, sbase, sextra, part);
+                     tiles, ndp, per, ntiles, sbase, sextra, part, pm_part, pmflag);
   rsg_tstop(c, t);
   if (nrem > 0) {
     rsg_tstart(c, "gram_reduce", &t);
-    const int64_t nq = (int64_t)nrem * BI * BJ * (OK == CNT_U16 ? 2 : 4);
-    hipLaunchKernelGGL((k_gram_reduce<BI, BJ, OK>), dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, c->stream, part, tiles,
-                       ndp, sbase, sextra, m->L, out);
+    hipLaunchKernelGGL((k_gram_fix<BI, BJ, OK>), dim3((unsigned)(nrem * (BI / 32) * (BJ / 32))), dim3(256), 0, c->stream,
+                       part, tiles, ndp, sbase, sextra, m->L, out, pm_part);
     rsg_tstop(c, t);
   }
   HIPCHK(hipGetLastError());
@@ -577,18 +725,19 @@ static int pick_shape(const rsg_ctx *c, const rsg_msa *m)
   return (m->L >= 768) ? 3 : 4;
 }
 
-int rsg_gram_counts_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, char *errbuf)
+int rsg_gram_counts_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, double *pm_part, int *pmflag, char *errbuf)
 {
-  if (kind == CNT_U32)            // more than 65535 sequences: the two default shapes only
-    return (m->L >= 768) ? launch_i8<2, 2, 256, CNT_U32>(c, m, out, errbuf) : launch_i8<1, 1, 128, CNT_U32>(c, m, out, errbuf);
+  if (kind == CNT_U32)            // more than 65535 sequences: the two default shapes only, no fused marginals
+    return (m->L >= 768) ? launch_i8<2, 2, 256, CNT_U32>(c, m, out, nullptr, nullptr, errbuf)
+                         : launch_i8<1, 1, 128, CNT_U32>(c, m, out, nullptr, nullptr, errbuf);
   if (kind != CNT_U16) return rsg_fail(errbuf, "gram: bad record kind %d", kind);
   switch (pick_shape(c, m)) {
-  case 1: return launch_i8<2, 2, 128, CNT_U16>(c, m, out, errbuf);
-  case 2: return launch_i8<2, 2, 64, CNT_U16>(c, m, out, errbuf);
-  case 3: return launch_i8<2, 2, 256, CNT_U16>(c, m, out, errbuf);
-  case 4: return launch_i8<1, 1, 128, CNT_U16>(c, m, out, errbuf);
-  case 5: return launch_i8<2, 1, 128, CNT_U16>(c, m, out, errbuf);
-  case 6: return launch_i8<1, 2, 128, CNT_U16>(c, m, out, errbuf);
+  case 1: return launch_i8<2, 2, 128, CNT_U16>(c, m, out, pm_part, pmflag, errbuf);
+  case 2: return launch_i8<2, 2, 64, CNT_U16>(c, m, out, pm_part, pmflag, errbuf);
+  case 3: return launch_i8<2, 2, 256, CNT_U16>(c, m, out, pm_part, pmflag, errbuf);
+  case 4: return launch_i8<1, 1, 128, CNT_U16>(c, m, out, pm_part, pmflag, errbuf);
+  case 5: return launch_i8<2, 1, 128, CNT_U16>(c, m, out, pm_part, pmflag, errbuf);
+  case 6: return launch_i8<1, 2, 128, CNT_U16>(c, m, out, pm_part, pmflag, errbuf);
   }
   return rsg_fail(errbuf, "bad gram shape");
 }

@@ -101,6 +101,7 @@ struct rsg_msa {
   // pair tables
   void    *d_cnt = nullptr;     // L x L x 16 (i<j records), kind below
   int      cnt_kind = -1;
+  int      pm_fused = 0;          // the last int8 Gram left the marginal partials in d_red
   size_t   cnt_bytes = 0;
   void    *d_cnt_u = nullptr;   // unit-weight counts kept for RAF when the main counts are weighted
   int      cnt_u_kind = -1;
@@ -149,9 +150,12 @@ static inline void rsg_tstop(rsg_ctx *c, int slot)
 
 // ---- kernel launchers implemented in the .hip files ----
 int rsg_pack(rsg_ctx *c, rsg_msa *m, char *errbuf);
-int rsg_gram_counts_i8(rsg_ctx *c, rsg_msa *m, void *d_out, int kind, char *errbuf);
+// pm_part (uint16 records only; nullptr = none): marginal partials of every 32 x 32 block
+// (pm32 layout, rsg_device.h); pmflag: cleared for the marginal reduction that follows
+int rsg_gram_counts_i8(rsg_ctx *c, rsg_msa *m, void *d_out, int kind, double *pm_part, int *pmflag, char *errbuf);
 int rsg_gram_counts_bf16(rsg_ctx *c, rsg_msa *m, int nsplit, float *d_out, char *errbuf);
 int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf);
+double *rsg_pm32_scratch(rsg_ctx *c, int64_t L, char *errbuf);
 int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, int prec, char *errbuf);
 int rsg_raf(rsg_ctx *c, rsg_msa *m, int rafs, const double *allow16, char *errbuf);
 int rsg_correct(rsg_ctx *c, rsg_msa *m, int actype, int shiftnonneg, char *errbuf);
@@ -189,8 +193,9 @@ int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t 
 
 // Batched pipeline: per-item descriptor of RSG_BDESC int64 fields
 // {N, L, Npad, Lpad, off_xt, off_cnt, off_cov, off_pm, off_rows, off_hist, off_leaf, blk0_pack,
-//  blk0_col, blk0_stat, blk0_el, nb32, group, blk0_hist}
-#define RSG_BDESC 21
+//  blk0_col, blk0_stat, blk0_el, nb32, group, blk0_hist, off_pm32 (marginal partials),
+//  blk0_col16 (16-column blocks)}
+#define RSG_BDESC 20
 // Surrogate features (rsg_features.hip): column statistics of items {one-hot panel offset, N,
 // Npad, L, first 4-column block} into 36 uint64 counters per item; the host turns them into
 // 48 floats.
@@ -202,4 +207,4 @@ void rsg_feat_finish(const unsigned long long *cnt, int64_t N, int64_t L, const 
 // others 32 x 32-column tiles (one wave, 64-sequence K tiles).
 static inline bool rsg_batch_big_tiles(int64_t N, int64_t L) { return L >= 256 && N >= 256; }
 int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_big, int nbig, const int4 *d_small, int nsmall,
-                          const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt);
+                          const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt, double *d_pm_part, int pm_field);

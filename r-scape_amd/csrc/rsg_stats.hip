@@ -22,8 +22,8 @@ static int ensure_red(rsg_ctx *c, size_t n, char *errbuf);
 int rsg_reserve_scratch(rsg_ctx *c, int64_t L, char *errbuf)
 {
   const int64_t nbt = (L + 63) / 64, nb = (L + 31) / 32;
-  const size_t need = std::max<size_t>({(size_t)(pm_npi(nbt) + pm_npj(nbt)), (size_t)(2 * nb * nb + 8 + 4 * L),
-                                        (size_t)(2 * L + 8)});
+  const size_t need = std::max<size_t>({(size_t)(pm_npi(nbt) + pm_npj(nbt)), (size_t)(2 * pm32_n(nb)),
+                                        (size_t)(2 * nb * nb + 8 + 4 * L), (size_t)(2 * L + 8)});
   return ensure_red(c, need, errbuf);
 }
 
@@ -38,6 +38,21 @@ __global__ void __launch_bounds__(64 * PMW) k_pm_tile(const void *__restrict__ c
   const int bj = (int)blockIdx.x / PMS, sub = (int)blockIdx.x % PMS, bi = (int)blockIdx.y;
   if (bi > bj) return;
   pm_tile<KIND>(cnt, L, nbt, bi, bj, sub, part_i, part_j);
+}
+
+// marginals from the 32 x 32 block partials left by the int8 Gram (16 lanes per column)
+__global__ void __launch_bounds__(256) k_pm32_reduce(const double *__restrict__ part, int nb32, int64_t L, double tol,
+                                                     double *__restrict__ pm, double *__restrict__ lpm, int *__restrict__ flag)
+{
+  const int64_t col = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+  if (col < L) pm32_reduce_col(part, part + pm32_n(nb32), nb32, L, col, threadIdx.x & 15, tol, pm, lpm, flag);
+}
+
+// Scratch for the marginal partials the int8 Gram leaves (2 pm32_n doubles), in d_red.
+double *rsg_pm32_scratch(rsg_ctx *c, int64_t L, char *errbuf)
+{
+  if (ensure_red(c, 2 * (size_t)pm32_n((L + 31) / 32), errbuf) != RSG_OK) return nullptr;
+  return c->d_red;
 }
 
 __global__ void __launch_bounds__(256) k_pm_reduce(const double *__restrict__ part_i, const double *__restrict__ part_j,
@@ -101,10 +116,21 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
   const int64_t L = m->L;
   if (!m->d_pm) HIPCHK(hipMalloc(&m->d_pm, sizeof(double) * 8 * (size_t)L));
   if (!m->d_ps) HIPCHK(hipMalloc(&m->d_ps, sizeof(double) * 5 * (size_t)L));
+  int t;
+  if (m->pm_fused) {
+    // the Gram left 32 x 32 block partials in d_red (rsg_pm32_scratch) and cleared the flag
+    m->pm_fused = 0;
+    const int nb32 = (int)((L + 31) / 32);
+    rsg_tstart(c, "marginals", &t);
+    hipLaunchKernelGGL(k_pm32_reduce, dim3((unsigned)((L + 15) / 16)), dim3(256), 0, c->stream, c->d_red, nb32, L, tol,
+                       m->d_pm, m->d_pm + 4 * L, (int *)(c->d_res + 4));
+    rsg_tstop(c, t);
+    HIPCHK(hipGetLastError());
+  } else {
   const int nbt = (int)((L + PMT - 1) / PMT);
   if (ensure_red(c, (size_t)(pm_npi(nbt) + pm_npj(nbt)), errbuf) != RSG_OK) return RSG_FAIL;
   double *part_i = c->d_red, *part_j = c->d_red + pm_npi(nbt);
-  int t; rsg_tstart(c, "marginals", &t);
+  rsg_tstart(c, "marginals", &t);
   if (m->cnt_kind == CNT_U16)
     hipLaunchKernelGGL(k_pm_tile<CNT_U16>, dim3((unsigned)(nbt * PMS), (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
                        (int *)(c->d_res + 4));
@@ -118,6 +144,7 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
                      m->d_pm, m->d_pm + 4 * L, (int *)(c->d_res + 4));
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
+  }
   rsg_tstart(c, "ps", &t);
   if (m->weighted)
     hipLaunchKernelGGL(k_ps<true>, dim3((unsigned)((L + 3) / 4)), dim3(256), 0, c->stream, m->d_axT, m->d_w, m->N, m->Npad, L, m->d_ps);

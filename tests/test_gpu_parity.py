@@ -57,11 +57,17 @@ def test_gram_shapes_bit_exact(ctx, shape):
         m = ctx.msa(ax)
         m.probs(R.COUNTS_INT8)
         got = m.counts().reshape(L, L, 16)
+        pm = m.get_probs()["pm"]
     finally:
         ctx.set_gram_shape(0)
     ref = oracle.pair_counts(ax).reshape(L, L, 16)
     iu = np.triu_indices(L, 1)
     assert np.array_equal(got[iu], ref[iu].astype(float))
+    # the marginals are summed per 32 x 32 column block in a fixed order whatever the tile shape
+    m0 = ctx.msa(ax)
+    m0.probs(R.COUNTS_INT8)
+    assert np.array_equal(pm, m0.get_probs()["pm"])
+    assert np.allclose(pm, oracle.probs(ax)["pm"], rtol=1e-12, atol=1e-15)
 
 
 @pytest.mark.parametrize("N,L", [(600, 1500), (300, 203), (5000, 90)])
@@ -71,14 +77,16 @@ def test_gram_split_matches_unsplit(ctx, N, L):
     ax = _msa(N, L, 31 + L)
     m = ctx.msa(ax)
     m.probs(R.COUNTS_INT8)
-    a = m.counts()
+    a, pa = m.counts(), m.get_probs()["pm"]
     ctx.set_gram_split(False)
     try:
         m.probs(R.COUNTS_INT8)
-        b = m.counts()
+        b, pb = m.counts(), m.get_probs()["pm"]
     finally:
         ctx.set_gram_split(True)
     assert np.array_equal(a, b)
+    # split tiles get their marginal partials from the fix-up pass, in the epilogue's order
+    assert np.array_equal(pa, pb)
     ref = oracle.pair_counts(ax).reshape(L, L, 16)
     iu = np.triu_indices(L, 1)
     assert np.array_equal(a.reshape(L, L, 16)[iu], ref[iu].astype(float))
