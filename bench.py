@@ -271,8 +271,8 @@ def main():
     dt, kav = run_steps(ctx, m, args.steps, **kw)
     value = upd * args.steps / dt
     # the int8 Gram is k_gram_i8 plus, when the last partial wave of tiles is split along K,
-    # k_gram_reduce: both launches timed with HIP events on the library stream
-    g_ms = kav.get("gram_i8", (float("nan"), 0))[0] + kav.get("gram_reduce", (0.0, 0))[0]
+    # k_gram_fix: one pair of HIP events on the library stream brackets both launches
+    g_ms = kav.get("gram_i8", (float("nan"), 0))[0]
     gram_ops = 2.0 * 16.0 * upd                      # 16 int8 MACs per update (4x4 canonical table)
     achieved = gram_ops / (g_ms * 1e-3) / 1e12
     out = {
@@ -294,7 +294,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
                      "frac": achieved / PEAK_INT8_TOPS,
                      "traffic": load_traffic("gram_i8_n5000_l1500_traffic.txt"),
-                     "kernel": "k_gram_i8 + k_gram_reduce", "kernel_ms": g_ms,
+                     "kernel": "k_gram_i8 + k_gram_fix", "kernel_ms": g_ms,
                      "algorithmic_ops_per_launch": gram_ops},
         "kernels_ms": {k: v[0] for k, v in kav.items()},
     }

@@ -23,7 +23,8 @@ extern "C" rsg_ctx *rsg_ctx_create(int device, char *errbuf)
       hipMalloc(&c->d_flag, 64) != hipSuccess || hipMalloc(&c->d_small, 65536) != hipSuccess ||
       hipMalloc(&c->d_res, 8 * sizeof(double)) != hipSuccess ||
       hipMemset(c->d_res, 0, 8 * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->d_hres, c->h_pinned, 0) != hipSuccess) {
     rsg_fail(errbuf, "context allocation failed");
     delete c;
     return nullptr;
@@ -320,8 +321,10 @@ extern "C" int rsg_cov_Calculate(rsg_ctx *c, rsg_msa *m, int covtype, int covcla
   if (st == RSG_OK && corr == 1) st = rsg_corr_CalculateCOVCorrected(c, m, RSG_APC, 0, errbuf);
   if (st == RSG_OK && corr == 2) st = rsg_corr_CalculateCOVCorrected(c, m, RSG_ASC, 0, errbuf);
   c->defer = 0;
-  if (st == RSG_OK && hipMemcpyAsync(c->h_pinned, c->d_res, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+  if (st == RSG_OK && !c->res_on_host &&
+      hipMemcpyAsync(c->h_pinned, c->d_res, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
     st = rsg_fail(errbuf, "rsg_cov_Calculate: result copy failed");
+  c->res_on_host = 0;
   if (hipStreamSynchronize(c->stream) != hipSuccess && st == RSG_OK) st = rsg_fail(errbuf, "rsg_cov_Calculate: device failure");
   if (st != RSG_OK) { m->have_probs = m->have_cov = 0; return st; }
   if (base != RSG_RAF && base != RSG_RAFS && *(int *)(c->h_pinned + 4)) {

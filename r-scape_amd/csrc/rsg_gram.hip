@@ -339,7 +339,7 @@ __device__ __forceinline__ void split_unit(int u, int sbase, int sextra, int &t,
 }
 
 // Grid: first the nsu split units, each over one K segment of one of the last tiles, writing
-// partial records for k_gram_reduce (nsu8 groups: the count rounded up to a multiple of 8),
+// partial records for k_gram_fix (nsu8 groups: the count rounded up to a multiple of 8),
 // then 8 * per_xcd work groups for the first ndp tiles (whole K range).  The
 // split keeps the last, partial wave of tiles from leaving most CUs idle; running it first
 // lets its partial-record stores drain while the full tiles compute, and leaves the full
@@ -702,16 +702,14 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, double *pm_part, int *pm
   }
   const int per = (ndp + 7) / 8;
   dim3 grid((unsigned)(8 * per + ((nsu + 7) & ~7))), block(128 * WI * WJ);
+  // one timer over the Gram and its split-K fix-up
   int t; rsg_tstart(c, "gram_i8", &t);
   hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->Lpad, m->L, out,
                      tiles, ndp, per, ntiles, sbase, sextra, part, pm_part, pmflag);
-  rsg_tstop(c, t);
-  if (nrem > 0) {
-    rsg_tstart(c, "gram_reduce", &t);
+  if (nrem > 0)
     hipLaunchKernelGGL((k_gram_fix<BI, BJ, OK>), dim3((unsigned)(nrem * (BI / 32) * (BJ / 32))), dim3(256), 0, c->stream,
                        part, tiles, ndp, sbase, sextra, m->L, out, pm_part);
-    rsg_tstop(c, t);
-  }
+  rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   return RSG_OK;
 }

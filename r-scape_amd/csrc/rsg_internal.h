@@ -80,9 +80,12 @@ struct rsg_ctx {
   double      *d_red = nullptr;  size_t red_cap = 0;
   int         *d_flag = nullptr;
   // results of one scoring call, in the layout of h_pinned[0..4]: min, max, correction NaN flag
-  // (int), COVavg, marginal validation flag (int); one copy brings them to the host
+  // (int), COVavg, marginal validation flag (int); one copy brings them to the host, or the
+  // chain's last kernel (the min / max reduction) writes them there itself (res_on_host).
   double      *d_res = nullptr;
   double      *h_pinned = nullptr;      // 64 doubles of pinned host staging
+  double      *d_hres = nullptr;        // h_pinned as the device addresses it
+  int          res_on_host = 0;
   void        *d_small = nullptr;       // 64 KB of device scratch for small per-call tables
 };
 

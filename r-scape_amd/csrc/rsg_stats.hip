@@ -63,8 +63,18 @@ __global__ void __launch_bounds__(256) k_pm_reduce(const double *__restrict__ pa
   if (col < L) pm_reduce_col(part_i, part_j, nbt, L, col, threadIdx.x & 63, tol, pm, lpm, flag);
 }
 
+// bytes of w equal to the byte pattern pat (0x01010101 x): a byte of w ^ pat is zero exactly
+// there, and ((b & 0x7F) + 0x7F) | b has its high bit set exactly when b != 0
+__device__ __forceinline__ unsigned bytes_eq(unsigned w, unsigned pat)
+{
+  const unsigned y = w ^ pat;
+  return 4u - (unsigned)__popc((((y & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | y) & 0x80808080u);
+}
+
 // corr_NaivePS: ps[i][x] = normalize(1e-5 + sum_s w_s [a_si == x]), x over A, C, G, U, gap.
-// One wave per column (4 per block), 16 codes per load; W: weighted, else unit weights.
+// One wave per column (4 per block), 16 codes per load; W: weighted, else unit weights (exact
+// integer counts: four loads in flight per lane, the padding codes (4) of the last load taken
+// off the gap count).
 template <bool W>
 __global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, const double *__restrict__ w,
                                             int64_t N, int64_t Npad, int64_t L, double *__restrict__ ps)
@@ -73,8 +83,34 @@ __global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, con
   const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= L) return;
   const uint4 *col = (const uint4 *)(axT + i * Npad);
+  if (!W) {
+    const int64_t nch = (N + 15) / 16;
+    unsigned n[5] = {0u, 0u, 0u, 0u, 0u};
+    auto count = [&](const uint4 v) {
+#pragma unroll
+      for (int x = 0; x < 5; x++)
+        n[x] += bytes_eq(v.x, 0x01010101u * x) + bytes_eq(v.y, 0x01010101u * x) + bytes_eq(v.z, 0x01010101u * x) +
+                bytes_eq(v.w, 0x01010101u * x);
+    };
+    int64_t k = lane;
+    for (; k + 192 < nch; k += 256) {
+      const uint4 v0 = col[k], v1 = col[k + 64], v2 = col[k + 128], v3 = col[k + 192];
+      count(v0); count(v1); count(v2); count(v3);
+    }
+    for (; k < nch; k += 64) count(col[k]);
+#pragma unroll
+    for (int x = 0; x < 5; x++)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) n[x] += __shfl_xor(n[x], o, 64);
+    if (lane == 0) {
+      n[4] -= (unsigned)(16 * nch - N);
+      double v[5], sum = 0.;
+      for (int x = 0; x < 5; x++) { v[x] = 1e-5 + (double)n[x]; sum += v[x]; }
+      for (int x = 0; x < 5; x++) ps[i * 5 + x] = v[x] / sum;
+    }
+    return;
+  }
   double acc[5] = {0., 0., 0., 0., 0.};
-  unsigned cnt[5] = {0u, 0u, 0u, 0u, 0u};                // unit weights: integer counts
   for (int64_t k = lane; k * 16 < N; k += 64) {
     const uint4 v = col[k];
     const unsigned wd[4] = {v.x, v.y, v.z, v.w};
@@ -82,18 +118,11 @@ __global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, con
     for (int b = 0; b < 16; b++) {
       const int64_t sq = k * 16 + b;
       const unsigned a = (sq < N) ? (wd[b >> 2] >> (8 * (b & 3))) & 0xFFu : 0xFFu;
-      if (W) {
-        const double ws = (a < 5u) ? w[sq] : 0.0;
+      const double ws = (a < 5u) ? w[sq] : 0.0;
 #pragma unroll
-        for (int x = 0; x < 5; x++) acc[x] += (a == (unsigned)x) ? ws : 0.0;
-      } else {
-#pragma unroll
-        for (int x = 0; x < 5; x++) cnt[x] += (a == (unsigned)x);
-      }
+      for (int x = 0; x < 5; x++) acc[x] += (a == (unsigned)x) ? ws : 0.0;
     }
   }
-  if (!W)
-    for (int x = 0; x < 5; x++) acc[x] = (double)cnt[x];
 #pragma unroll
   for (int x = 0; x < 5; x++)
     for (int o = 32; o > 0; o >>= 1) acc[x] += __shfl_down(acc[x], o, 64);
@@ -182,18 +211,38 @@ static int launch_stat_fixed(rsg_ctx *c, const StatArgs &A, int cls, int nb)
   return nb * nb;
 }
 
-__global__ void k_reduce_minmax(const double *__restrict__ part, int n, double *__restrict__ out)
+// (min, max) of n block partials into out[0..1]; one block of 1024 threads, four partials per
+// thread loaded at once.  With hout != nullptr out[0..4] (min, max and the flags and COVavg the
+// chain left in out[2..4]) also go to hout, host memory, which the caller reads after its
+// stream synchronisation.
+__global__ void __launch_bounds__(1024) k_reduce_minmax(const double *__restrict__ part, int n, double *__restrict__ out,
+                                                        double *__restrict__ hout)
 {
-  __shared__ double rmin[256], rmax[256];
+  __shared__ double rmin[1024], rmax[1024];
   double a = INFINITY, b = -INFINITY;
-  for (int k = threadIdx.x; k < n; k += 256) { a = fmin(a, part[2 * k]); b = fmax(b, part[2 * k + 1]); }
+  for (int k0 = threadIdx.x; k0 < n; k0 += 4096) {
+    double2 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int k = k0 + 1024 * q;
+      v[q] = (k < n) ? *(const double2 *)(part + 2 * k) : make_double2(INFINITY, -INFINITY);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) { a = fmin(a, v[q].x); b = fmax(b, v[q].y); }
+  }
   rmin[threadIdx.x] = a; rmax[threadIdx.x] = b;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = 512; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) { rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + s]); rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + s]); }
     __syncthreads();
   }
-  if (threadIdx.x == 0) { out[0] = rmin[0]; out[1] = rmax[0]; }
+  if (threadIdx.x == 0) {
+    out[0] = rmin[0]; out[1] = rmax[0];
+    if (hout) {
+      hout[0] = rmin[0]; hout[1] = rmax[0];
+      hout[2] = out[2]; hout[3] = out[3]; hout[4] = out[4];
+    }
+  }
 }
 
 static int ensure_red(rsg_ctx *c, size_t n, char *errbuf)
@@ -211,9 +260,11 @@ static int ensure_red(rsg_ctx *c, size_t n, char *errbuf)
 static int fetch_minmax(rsg_ctx *c, rsg_msa *m, int nparts, char *errbuf)
 {
   if (c->skip_minmax) return RSG_OK;            // a correction recomputes them
-  hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(256), 0, c->stream, c->d_red, nparts, c->d_res);
+  // a deferred chain ends here: the reduction also writes the result block to host memory
+  hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(1024), 0, c->stream, c->d_red, nparts, c->d_res,
+                     c->defer ? c->d_hres : nullptr);
   HIPCHK(hipGetLastError());
-  if (c->defer) return RSG_OK;
+  if (c->defer) { c->res_on_host = 1; return RSG_OK; }
   HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_res, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   m->minCOV = c->h_pinned[0]; m->maxCOV = c->h_pinned[1];
@@ -471,13 +522,17 @@ int rsg_correct(rsg_ctx *c, rsg_msa *m, int actype, int shiftnonneg, char *errbu
   const int nblk = (int)L;
   if (ensure_red(c, 2 * (size_t)nblk + 8, errbuf) != RSG_OK) return RSG_FAIL;
   int *nanflag = (int *)(c->d_res + 2);
+  // deferred chain without a shift: the min / max reduction also writes the result block to host
+  // memory, so no copy follows
+  const bool direct = c->defer && !shiftnonneg;
   int t; rsg_tstart(c, "correct", &t);
   hipLaunchKernelGGL(k_rowsum, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_cov, L, m->d_rowx, nanflag);
   hipLaunchKernelGGL(k_correct, dim3(nblk), dim3(256), 0, c->stream, m->d_cov, L, m->d_rowx, actype, nanflag, c->d_red,
                      c->d_res + 3);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(256), 0, c->stream, c->d_red, nblk, c->d_res);
+  hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(1024), 0, c->stream, c->d_red, nblk, c->d_res, direct ? c->d_hres : nullptr);
+  if (direct) { c->res_on_host = 1; return RSG_OK; }
   if (shiftnonneg)
     hipLaunchKernelGGL(k_shift, dim3((unsigned)((L * L + 255) / 256)), dim3(256), 0, c->stream, m->d_cov, L, c->d_res);
   HIPCHK(hipGetLastError());
