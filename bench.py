@@ -118,6 +118,30 @@ def evalue_runs(ctx, d, K=20, reps=3, acc_nulls=16):
         ev = m.evalues(h, sf, Nc, matrix=matrix)
         return ev if matrix else NL.hits(ctx, m, 0.05)
 
+    seeds = [7 + r for r in range(K)]
+
+    def plain_fitch(matrix=False):
+        # R-scape's own null (null_rscape): Fitch ancestors + column shuffle per replicate
+        m.calculate("GTp")
+        cum, w = None, None
+        for sd in seeds:
+            mr = ctx.msa(NL.fitch_null(ctx, ax, tree, seed=sd))
+            a, b = mr.calculate("GTp")
+            if w is None:
+                w = R.width(b, a)
+            cum = R.Histogram.null_add(cum, mr.rank_hist(-10.0, w))
+            mr.close()
+        _, sf = cum.fit(ctx, 5e-4, 1.0, True)
+        ev = m.evalues(cum, sf, Nc, matrix=matrix)
+        return ev if matrix else NL.hits(ctx, m, 0.05)
+
+    def batched_fitch(matrix=False, graph=0):
+        m.calculate("GTp")
+        _, h = NL.batch_fitch(ctx, ax, tree, seeds, "GTp", pmass=5e-4, graph=graph)
+        _, sf = h.fit(ctx, 5e-4, 1.0, True)
+        ev = m.evalues(h, sf, Nc, matrix=matrix)
+        return ev if matrix else NL.hits(ctx, m, 0.05)
+
     def surrogate(matrix=False):
         m.calculate("GTp")
         mu, lam = sur.predict(NL.features(ctx, m, tree))
@@ -146,6 +170,15 @@ def evalue_runs(ctx, d, K=20, reps=3, acc_nulls=16):
     out["batched_null"] = 1.0 / tb
     out["batched_null_graph"] = 1.0 / tg
     out["speedup_batched_vs_plain"] = tp / tb
+    tpf = timed(plain_fitch)
+    tbf = timed(lambda: batched_fitch(graph=1))
+    out["rscape_null_plain_per_replicate"] = 1.0 / tpf
+    out["rscape_null_batched_graph"] = 1.0 / tbf
+    out["speedup_rscape_null_batched_vs_plain"] = tpf / tbf
+    ef, eg = plain_fitch(matrix=True), batched_fitch(matrix=True, graph=1)
+    ok = np.isfinite(ef) & np.isfinite(eg)
+    out["rscape_null_plain_vs_batched_max_abs_log10E"] = float(np.max(np.abs(
+        np.log10(np.maximum(ef[ok], 1e-300)) - np.log10(np.maximum(eg[ok], 1e-300)))))
     iu = np.triu_indices(L, 1)
     eb = batched(matrix=True)[iu]
     ep = plain(matrix=True)[iu]
@@ -187,6 +220,8 @@ def evalue_runs(ctx, d, K=20, reps=3, acc_nulls=16):
         out["surrogate"] = 1.0 / ts
         out["surrogate_fallback"] = fallback[0]
         out["speedup_surrogate_vs_plain"] = tp / ts
+        out["speedup_surrogate_vs_rscape_null_plain"] = tpf / ts
+        out["speedup_surrogate_vs_rscape_null_batched"] = tbf / ts
         d = np.abs(lg(es) - ref)
         out["surrogate_log10E"] = {"all": errs(d, sig), "fit_range": errs(d, fitr), "capped": errs(d, capped)}
         out["call_agreement"] = float(np.mean((es < 0.05) == sig))

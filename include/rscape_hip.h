@@ -212,6 +212,16 @@ int rsg_null_simulate(rsg_ctx *ctx, const rsg_null_job *job, uint8_t *leaves, ch
 int rsg_null_batch(rsg_ctx *ctx, const rsg_null_job *jobs, int njobs, int covtype,
                    double bmin, double pmass, double fracfit, int doexpfit,
                    rsg_hist **hists, double *fits, float *feats, int use_graph, char *errbuf);
+/* R-scape's null of one alignment in one batch: null_rscape's nshuffle loop
+ * (src/R-scape.c:1615-1724) with replicate r = rsg_null_fitch(ax, tree, seeds[r]) (Fitch
+ * ancestors, column shuffle, substitution replay), every replicate scored (covtype, unit
+ * weights, CSELECT), the histogram width from replicate 0 (calculate_width_histo), all pairs
+ * merged into one histogram (null_add2cumranklist) and one FP64 tail fit: fit[0..7] as in
+ * rsg_null_batch, *hist (may be NULL) the merged histogram.  Each Fitch kernel covers every
+ * replicate in one launch per tree depth; use_graph as in rsg_null_batch.  nseq <= 65535. */
+int rsg_null_batch_fitch(rsg_ctx *ctx, const uint8_t *ax, int64_t nseq, int64_t alen, const int *left, const int *right,
+                         const uint64_t *seeds, int nrep, int covtype, double bmin, double pmass, double fracfit,
+                         int doexpfit, rsg_hist **hist, double *fit, int use_graph, char *errbuf);
 
 /* ---- learned null-tail surrogate (no reference counterpart) ---- */
 /* The surrogate's 48 inputs of an alignment and its tree (column statistics on the device,

@@ -294,9 +294,12 @@ struct Arena {
 };
 }
 
-extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int covtype, double bmin, double pmass,
-                              double fracfit, int doexpfit, rsg_hist **hists, double *fits, float *feats,
-                              int use_graph, char *errbuf)
+// The batch; with fp != nullptr the null alignments are R-scape's nulls of one alignment fax
+// (Fitch + shuffle, rsg_fitch.hip), one per job (all jobs of that alignment's size, seeds from the
+// jobs), in place of the simulation.
+static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int covtype, double bmin, double pmass,
+                           double fracfit, int doexpfit, rsg_hist **hists, double *fits, float *feats, int use_graph,
+                           const uint8_t *fax, const FitchPlan *fp, char *errbuf)
 {
   HIPCHK(hipSetDevice(c->device));
   if (nj < 1) return rsg_fail(errbuf, "empty batch");
@@ -360,18 +363,36 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   };
   std::stable_sort(tiles.begin(), tiles.end(), by_k);
   std::stable_sort(btiles.begin(), btiles.end(), by_k);
-  std::vector<int> left((size_t)nodes), right((size_t)nodes);
-  std::vector<double> ld((size_t)nodes), rd((size_t)nodes), models((size_t)nj * 40);
-  for (int k = 0; k < nj; k++) {
-    const int64_t off = sim[(size_t)k].tree_off, m1 = jobs[k].nleaf - 1;
-    memcpy(&left[(size_t)off], jobs[k].left, sizeof(int) * m1);
-    memcpy(&right[(size_t)off], jobs[k].right, sizeof(int) * m1);
-    memcpy(&ld[(size_t)off], jobs[k].ld, sizeof(double) * m1);
-    memcpy(&rd[(size_t)off], jobs[k].rd, sizeof(double) * m1);
-    memcpy(&models[(size_t)k * 40], jobs[k].model, sizeof(double) * 40);
-  }
+  std::vector<int> left, right;
+  std::vector<double> ld, rd, models;
   SimLevels lv;
-  if (rsg_sim_levels(sim.data(), nj, left.data(), right.data(), lv, errbuf) != RSG_OK) return RSG_FAIL;
+  std::vector<int> fperm, ftree;
+  std::vector<uint64_t> fseed;
+  if (!fp) {
+    left.resize((size_t)nodes); right.resize((size_t)nodes);
+    ld.resize((size_t)nodes); rd.resize((size_t)nodes); models.resize((size_t)nj * 40);
+    for (int k = 0; k < nj; k++) {
+      const int64_t off = sim[(size_t)k].tree_off, m1 = jobs[k].nleaf - 1;
+      memcpy(&left[(size_t)off], jobs[k].left, sizeof(int) * m1);
+      memcpy(&right[(size_t)off], jobs[k].right, sizeof(int) * m1);
+      memcpy(&ld[(size_t)off], jobs[k].ld, sizeof(double) * m1);
+      memcpy(&rd[(size_t)off], jobs[k].rd, sizeof(double) * m1);
+      memcpy(&models[(size_t)k * 40], jobs[k].model, sizeof(double) * 40);
+    }
+    if (rsg_sim_levels(sim.data(), nj, left.data(), right.data(), lv, errbuf) != RSG_OK) return RSG_FAIL;
+  } else {
+    // one column permutation per replicate; the tree as {left, right, parent, level-ordered nodes}
+    for (int k = 0; k < nj; k++)
+      if (jobs[k].nleaf != fp->N || jobs[k].alen != fp->L) return rsg_fail(errbuf, "Fitch batch: job %d has another size", k);
+    fperm.resize((size_t)nj * fp->L);
+    fseed.resize((size_t)nj);
+    for (int k = 0; k < nj; k++) { fseed[(size_t)k] = jobs[k].seed; rsg_fitch_perm(jobs[k].seed, fp->L, &fperm[(size_t)k * fp->L]); }
+    ftree.resize(4 * (size_t)fp->m);
+    memcpy(&ftree[0], jobs[0].left, sizeof(int) * fp->m);
+    memcpy(&ftree[(size_t)fp->m], jobs[0].right, sizeof(int) * fp->m);
+    memcpy(&ftree[2 * (size_t)fp->m], fp->parent.data(), sizeof(int) * fp->m);
+    memcpy(&ftree[3 * (size_t)fp->m], fp->nodes.data(), sizeof(int) * fp->m);
+  }
   // surrogate features of each group's first replicate
   std::vector<int64_t> fitems((size_t)ng * 6);
   int64_t fblocks = 0;
@@ -386,6 +407,8 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   double *d_ld, *d_rd, *d_models; int4 *d_ent; int *d_gfirst; int64_t *d_fitems; unsigned long long *d_fcnt;
   uint32_t *d_cdf; uint8_t *d_seqs, *d_leaves, *d_xt; uint16_t *d_cnt; double *d_cov, *d_pm, *d_rows, *d_lgc;
   double *d_part_s, *d_part_e, *d_mm, *d_pmp; HGeo *d_geo; unsigned long long *d_hist, *d_S; int *d_NZ; double *d_fits;
+  uint8_t *f_ax = nullptr, *f_mask = nullptr, *f_all = nullptr; int *f_tree = nullptr, *f_perm = nullptr, *f_nsub = nullptr;
+  int *f_cols = nullptr; uint64_t *f_seeds = nullptr;
   auto carve = [&](Arena &A) {
     d_desc = A.get<int64_t>(desc.size()); d_sim = A.get<SimItem>((size_t)nj); d_cls = A.get<int>((size_t)nj);
     d_tiles = A.get<int4>(tiles.size()); d_btiles = A.get<int4>(btiles.size());
@@ -402,6 +425,12 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     d_mm = A.get<double>(2 * (size_t)nj); d_pmp = A.get<double>((size_t)o_pmp); d_geo = A.get<HGeo>((size_t)ng);
     d_hist = A.get<unsigned long long>((size_t)ng * NBMAX); d_S = A.get<unsigned long long>((size_t)ng * NBMAX);
     d_NZ = A.get<int>((size_t)ng * NBMAX); d_fits = A.get<double>((size_t)ng * 8);
+    if (fp) {
+      const size_t N = (size_t)fp->N, L = (size_t)fp->L, R = (size_t)nj;
+      f_ax = A.get<uint8_t>(N * L); f_tree = A.get<int>(4 * (size_t)fp->m); f_perm = A.get<int>(R * L);
+      f_seeds = A.get<uint64_t>(R); f_mask = A.get<uint8_t>(R * (2 * N - 1) * L); f_all = A.get<uint8_t>(R * (2 * N - 1) * L);
+      f_nsub = A.get<int>(R * 2 * (size_t)fp->m * 25); f_cols = A.get<int>(R * 2 * (size_t)fp->maxw * L);
+    }
   };
   {
     Arena plan;
@@ -421,12 +450,19 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   HIPCHK(hipMemcpyAsync(d_cls, cls.data(), sizeof(int) * nj, hipMemcpyHostToDevice, c->stream));
   if (!tiles.empty()) HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice, c->stream));
   if (!btiles.empty()) HIPCHK(hipMemcpyAsync(d_btiles, btiles.data(), sizeof(int4) * btiles.size(), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_left, left.data(), sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_right, right.data(), sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_ld, ld.data(), sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_rd, rd.data(), sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_models, models.data(), sizeof(double) * models.size(), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_ent, lv.ent.data(), sizeof(int4) * lv.ent.size(), hipMemcpyHostToDevice, c->stream));
+  if (!fp) {
+    HIPCHK(hipMemcpyAsync(d_left, left.data(), sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_right, right.data(), sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_ld, ld.data(), sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_rd, rd.data(), sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_models, models.data(), sizeof(double) * models.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_ent, lv.ent.data(), sizeof(int4) * lv.ent.size(), hipMemcpyHostToDevice, c->stream));
+  } else {
+    HIPCHK(hipMemcpyAsync(f_ax, fax, (size_t)(fp->N * fp->L), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(f_tree, ftree.data(), sizeof(int) * ftree.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(f_perm, fperm.data(), sizeof(int) * fperm.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(f_seeds, fseed.data(), sizeof(uint64_t) * fseed.size(), hipMemcpyHostToDevice, c->stream));
+  }
   HIPCHK(hipMemcpyAsync(d_gfirst, gfirst.data(), sizeof(int) * gfirst.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_fitems, fitems.data(), sizeof(int64_t) * fitems.size(), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -441,8 +477,13 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
   }
 
   auto launch_all = [&]() -> int {
-    if (rsg_sim_launch(c, d_sim, nj, nodes, b_sim, lv, d_ent, d_left, d_right, d_ld, d_rd, d_models, d_cdf, d_seqs,
-                       d_leaves) != RSG_OK) return RSG_FAIL;
+    if (fp) {
+      const int m = fp->m;
+      const FitchDev D{f_ax, f_tree, f_tree + m, f_tree + 2 * m, f_tree + 3 * m, f_perm, f_seeds, f_mask, f_all, d_seqs,
+                       d_leaves, f_nsub, f_cols};
+      if (rsg_fitch_launch(c, *fp, D, nj) != RSG_OK) return RSG_FAIL;
+    } else if (rsg_sim_launch(c, d_sim, nj, nodes, b_sim, lv, d_ent, d_left, d_right, d_ld, d_rd, d_models, d_cdf, d_seqs,
+                              d_leaves) != RSG_OK) return RSG_FAIL;
     hipLaunchKernelGGL(k_pack_b, dim3((unsigned)b_pack), dim3(256), 0, c->stream, d_desc, nj, d_leaves, d_xt);
     if (feats && rsg_feat_launch(c, d_fitems, ng, fblocks, d_xt, d_fcnt) != RSG_OK) return RSG_FAIL;
     if (rsg_gram_batch_launch(c, d_btiles, (int)btiles.size(), d_tiles, (int)tiles.size(), d_desc, d_xt, d_cnt, d_pmp,
@@ -523,4 +564,31 @@ extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int 
     }
   }
   return RSG_OK;
+}
+
+extern "C" int rsg_null_batch(rsg_ctx *c, const rsg_null_job *jobs, int nj, int covtype, double bmin, double pmass,
+                              double fracfit, int doexpfit, rsg_hist **hists, double *fits, float *feats,
+                              int use_graph, char *errbuf)
+{
+  if (!c || !jobs || !fits) return rsg_fail(errbuf, "rsg_null_batch: bad arguments");
+  return null_batch_impl(c, jobs, nj, covtype, bmin, pmass, fracfit, doexpfit, hists, fits, feats, use_graph, nullptr,
+                         nullptr, errbuf);
+}
+
+extern "C" int rsg_null_batch_fitch(rsg_ctx *c, const uint8_t *ax, int64_t nseq, int64_t alen, const int *left,
+                                    const int *right, const uint64_t *seeds, int nrep, int covtype, double bmin,
+                                    double pmass, double fracfit, int doexpfit, rsg_hist **hist, double *fit,
+                                    int use_graph, char *errbuf)
+{
+  if (!c || !ax || !left || !right || !seeds || nrep < 1 || !fit) return rsg_fail(errbuf, "rsg_null_batch_fitch: bad arguments");
+  FitchPlan P;
+  if (rsg_fitch_plan(nseq, alen, left, right, P, errbuf) != RSG_OK) return RSG_FAIL;
+  std::vector<rsg_null_job> jobs((size_t)nrep);
+  for (int r = 0; r < nrep; r++) {
+    rsg_null_job &J = jobs[(size_t)r];
+    J.nleaf = nseq; J.alen = alen; J.left = left; J.right = right; J.ld = nullptr; J.rd = nullptr; J.model = nullptr;
+    J.seed = seeds[r]; J.stream = (uint32_t)r; J.gapfrac = 0.; J.group = 0;
+  }
+  return null_batch_impl(c, jobs.data(), nrep, covtype, bmin, pmass, fracfit, doexpfit, hist, fit, nullptr, use_graph, ax,
+                         &P, errbuf);
 }

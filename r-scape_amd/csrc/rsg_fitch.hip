@@ -22,12 +22,18 @@ namespace {
 
 __device__ __forceinline__ int row_of(int d, int64_t N) { return d > 0 ? (int)(N + d) : -d; }
 
+// Replicate r = blockIdx.y of every kernel: its seed is seeds[r]; its state sets and states
+// (mask, all: 2N - 1 rows) start at r * rs, its shuffled leaf rows at r * N * L of `leaves` and
+// its shuffled internal rows at r * (N - 1) * L of `inner`.
+
 // leaf state sets: 1 << code for codes 0..4, else one random state
-__global__ void k_fitch_leaves(const uint8_t *__restrict__ ax, int64_t N, int64_t L, uint64_t seed,
-                               uint8_t *__restrict__ mask, uint8_t *__restrict__ all)
+__global__ void k_fitch_leaves(const uint8_t *__restrict__ ax, int64_t N, int64_t L, const uint64_t *__restrict__ seeds,
+                               uint8_t *__restrict__ mask, uint8_t *__restrict__ all, int64_t rs)
 {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= N * L) return;
+  const uint64_t seed = seeds[blockIdx.y];
+  mask += blockIdx.y * rs; all += blockIdx.y * rs;
   const int64_t n = e / L, c = e - n * L;
   const uint8_t a = ax[e];
   int s = a;
@@ -39,10 +45,11 @@ __global__ void k_fitch_leaves(const uint8_t *__restrict__ ax, int64_t N, int64_
 // upward pass over the internal nodes of one depth: S_v = S_l & S_r, or the residues of
 // S_l | S_r when the intersection is empty
 __global__ void k_fitch_up(const int *__restrict__ nodes, int nn, int64_t N, int64_t L, const int *__restrict__ left,
-                           const int *__restrict__ right, uint8_t *__restrict__ mask)
+                           const int *__restrict__ right, uint8_t *__restrict__ mask, int64_t rs)
 {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)nn * L) return;
+  mask += blockIdx.y * rs;
   const int v = nodes[e / L];
   const int64_t c = e % L;
   const unsigned sl = mask[(int64_t)row_of(left[v], N) * L + c], sr = mask[(int64_t)row_of(right[v], N) * L + c];
@@ -52,10 +59,13 @@ __global__ void k_fitch_up(const int *__restrict__ nodes, int nn, int64_t N, int
 
 // downward pass: keep the parent's state when the set holds it, then a uniform member
 __global__ void k_fitch_down(const int *__restrict__ nodes, int nn, int64_t N, int64_t L, const int *__restrict__ parent,
-                             uint64_t seed, const uint8_t *__restrict__ mask, uint8_t *__restrict__ all)
+                             const uint64_t *__restrict__ seeds, const uint8_t *__restrict__ mask,
+                             uint8_t *__restrict__ all, int64_t rs)
 {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)nn * L) return;
+  const uint64_t seed = seeds[blockIdx.y];
+  mask += blockIdx.y * rs; all += blockIdx.y * rs;
   const int v = nodes[e / L];
   const int64_t c = e % L;
   unsigned S = mask[(N + v) * L + c];
@@ -74,9 +84,11 @@ __global__ void k_fitch_down(const int *__restrict__ nodes, int nn, int64_t N, i
 // substitution counts of every branch b = 2 v + side: nsub[b][old * 5 + new], old != new
 __global__ void __launch_bounds__(64) k_fitch_subs(int64_t N, int64_t L, const int *__restrict__ left,
                                                    const int *__restrict__ right, const uint8_t *__restrict__ all,
-                                                   int *__restrict__ nsub)
+                                                   int *__restrict__ nsub, int64_t rs)
 {
   const int b = blockIdx.x, v = b >> 1, side = b & 1;
+  all += blockIdx.y * rs;
+  nsub += (int64_t)blockIdx.y * gridDim.x * 25;
   const int d = side ? right[v] : left[v];
   const uint8_t *pa = all + (N + v) * L, *ch = all + (int64_t)row_of(d, N) * L;
   int cnt[25];
@@ -97,11 +109,16 @@ __global__ void __launch_bounds__(64) k_fitch_subs(int64_t N, int64_t L, const i
   }
 }
 
+// the shuffled root: column c of the root takes column perm[c] of its ancestral states
 __global__ void k_gather_root(const uint8_t *__restrict__ all, const int *__restrict__ perm, int64_t N, int64_t L,
-                              uint8_t *__restrict__ sh)
+                              uint8_t *__restrict__ inner, int64_t rs)
 {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < L) sh[N * L + c] = all[N * L + perm[c]];
+  if (c >= L) return;
+  all += blockIdx.y * rs;
+  perm += blockIdx.y * L;
+  inner += blockIdx.y * (N - 1) * L;
+  inner[c] = all[N * L + perm[c]];
 }
 
 // Orders a wave's global stores before its later loads (lanes exchange data through memory;
@@ -113,22 +130,28 @@ __device__ __forceinline__ void wave_sync()
 }
 
 // Replay of the branches below the internal nodes of one depth: one wave per branch.
-// cols: scratch of L ints per wave slot of the launch.
+// cols: scratch of L ints per wave slot (2 maxw slots per replicate).
 __global__ void __launch_bounds__(256) k_fitch_replay(const int *__restrict__ nodes, int nn, int64_t N, int64_t L,
                                                       const int *__restrict__ left, const int *__restrict__ right,
-                                                      uint64_t seed, const int *__restrict__ nsub,
-                                                      uint8_t *__restrict__ sh, int *__restrict__ cols_all)
+                                                      const uint64_t *__restrict__ seeds, const int *__restrict__ nsub,
+                                                      uint8_t *__restrict__ leaves, uint8_t *__restrict__ inner,
+                                                      int *__restrict__ cols_all, int maxw)
 {
   const int lane = threadIdx.x & 63;
   const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);        // branch slot in this level
   if (slot >= 2 * nn) return;
+  const int64_t r = blockIdx.y;
+  const uint64_t seed = seeds[r];
+  nsub += r * 2 * (N - 1) * 25;
+  leaves += r * N * L;
+  inner += r * (N - 1) * L;
   const int v = nodes[slot >> 1], side = slot & 1;
   const int d = side ? right[v] : left[v];
   const int chrow = row_of(d, N);
   const int b = 2 * v + side;
-  const uint8_t *pa = sh + (N + v) * L;
-  uint8_t *ch = sh + (int64_t)chrow * L;
-  int *cols = cols_all + (int64_t)slot * L;
+  const uint8_t *pa = inner + (int64_t)v * L;
+  uint8_t *ch = (d > 0) ? inner + (int64_t)d * L : leaves + (int64_t)(-d) * L;
+  int *cols = cols_all + (r * 2 * maxw + slot) * L;
   for (int64_t c = lane; c < L; c += 64) ch[c] = pa[c];
   wave_sync();
   for (int oldc = 0; oldc < 5; oldc++) {
@@ -179,15 +202,14 @@ struct Buf {
 
 } // namespace
 
-extern "C" int rsg_null_fitch(rsg_ctx *c, const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
-                              uint64_t seed, uint8_t *leaves, char *errbuf)
+int rsg_fitch_plan(int64_t N, int64_t L, const int *left, const int *right, FitchPlan &P, char *errbuf)
 {
-  if (!c || !ax || !left || !right || !leaves || N < 2 || L < 1) return rsg_fail(errbuf, "rsg_null_fitch: bad arguments");
-  if (N > (1 << 28) || L > (1 << 30)) return rsg_fail(errbuf, "rsg_null_fitch: alignment too large");
-  HIPCHK(hipSetDevice(c->device));
+  if (!left || !right || N < 2 || L < 1) return rsg_fail(errbuf, "Fitch null: bad arguments");
+  if (N > (1 << 28) || L > (1 << 30)) return rsg_fail(errbuf, "Fitch null: alignment too large");
   const int m = (int)(N - 1);
   // depths of the internal nodes (walk from the root; malformed trees rejected), parents
-  std::vector<int> depth((size_t)m, -1), parent((size_t)m, 0), stack(1, 0);
+  std::vector<int> depth((size_t)m, -1), stack(1, 0);
+  P.parent.assign((size_t)m, 0);
   depth[0] = 0;
   int seen = 1, maxd = 0;
   while (!stack.empty()) {
@@ -195,67 +217,99 @@ extern "C" int rsg_null_fitch(rsg_ctx *c, const uint8_t *ax, int64_t N, int64_t 
     for (int side = 0; side < 2; side++) {
       const int d = side ? right[v] : left[v];
       if (d > 0) {
-        if (d >= m || depth[(size_t)d] >= 0) return rsg_fail(errbuf, "rsg_null_fitch: malformed tree at node %d", v);
-        depth[(size_t)d] = depth[(size_t)v] + 1; parent[(size_t)d] = v; maxd = std::max(maxd, depth[(size_t)d]);
+        if (d >= m || depth[(size_t)d] >= 0) return rsg_fail(errbuf, "Fitch null: malformed tree at node %d", v);
+        depth[(size_t)d] = depth[(size_t)v] + 1; P.parent[(size_t)d] = v; maxd = std::max(maxd, depth[(size_t)d]);
         stack.push_back(d); seen++;
-      } else if (-d >= N) return rsg_fail(errbuf, "rsg_null_fitch: leaf index %d out of range", -d);
+      } else if (-d >= N) return rsg_fail(errbuf, "Fitch null: leaf index %d out of range", -d);
     }
   }
-  if (seen != m) return rsg_fail(errbuf, "rsg_null_fitch: %d internal nodes unreachable from the root", m - seen);
-  std::vector<int> lev_nodes, lev0((size_t)maxd + 2, 0);
-  for (int v = 0; v < m; v++) lev0[(size_t)depth[(size_t)v] + 1]++;
-  for (int k = 0; k <= maxd; k++) lev0[(size_t)k + 1] += lev0[(size_t)k];
-  lev_nodes.resize((size_t)m);
+  if (seen != m) return rsg_fail(errbuf, "Fitch null: %d internal nodes unreachable from the root", m - seen);
+  P.lev0.assign((size_t)maxd + 2, 0);
+  for (int v = 0; v < m; v++) P.lev0[(size_t)depth[(size_t)v] + 1]++;
+  for (int k = 0; k <= maxd; k++) P.lev0[(size_t)k + 1] += P.lev0[(size_t)k];
+  P.nodes.resize((size_t)m);
   {
-    std::vector<int> fill(lev0.begin(), lev0.end() - 1);
-    for (int v = 0; v < m; v++) lev_nodes[(size_t)fill[(size_t)depth[(size_t)v]]++] = v;
+    std::vector<int> fill(P.lev0.begin(), P.lev0.end() - 1);
+    for (int v = 0; v < m; v++) P.nodes[(size_t)fill[(size_t)depth[(size_t)v]]++] = v;
   }
-  int maxw = 0;
-  for (int k = 0; k <= maxd; k++) maxw = std::max(maxw, lev0[(size_t)k + 1] - lev0[(size_t)k]);
-  // one permutation of all columns (Fisher-Yates as esl_vec_IShuffle)
-  std::vector<int> perm((size_t)L);
-  for (int64_t k = 0; k < L; k++) perm[(size_t)k] = (int)k;
+  P.maxw = 0;
+  for (int k = 0; k <= maxd; k++) P.maxw = std::max(P.maxw, P.lev0[(size_t)k + 1] - P.lev0[(size_t)k]);
+  P.N = N; P.L = L; P.m = m; P.maxd = maxd;
+  return RSG_OK;
+}
+
+// one permutation of all columns (Fisher-Yates as esl_vec_IShuffle)
+void rsg_fitch_perm(uint64_t seed, int64_t L, int *perm)
+{
+  for (int64_t k = 0; k < L; k++) perm[k] = (int)k;
   for (int64_t n = L; n > 1; n--) {
     const uint32_t u = rsg_philox(seed, 3u, 0u, (uint32_t)n, 3u);
     const int64_t k = (int64_t)(((uint64_t)u * (uint64_t)n) >> 32);
-    std::swap(perm[(size_t)(n - 1)], perm[(size_t)k]);
+    std::swap(perm[n - 1], perm[k]);
   }
-  const int64_t nrow = 2 * N - 1;
+}
+
+int rsg_fitch_launch(rsg_ctx *c, const FitchPlan &P, const FitchDev &D, int R)
+{
+  const int64_t N = P.N, L = P.L, rs = (2 * N - 1) * L;
+  const unsigned ry = (unsigned)R;
+  hipLaunchKernelGGL(k_fitch_leaves, dim3((unsigned)((N * L + 255) / 256), ry), dim3(256), 0, c->stream, D.ax, N, L,
+                     D.seeds, D.mask, D.all, rs);
+  for (int k = P.maxd; k >= 0; k--) {
+    const int nn = P.lev0[(size_t)k + 1] - P.lev0[(size_t)k];
+    hipLaunchKernelGGL(k_fitch_up, dim3((unsigned)(((int64_t)nn * L + 255) / 256), ry), dim3(256), 0, c->stream,
+                       D.nodes + P.lev0[(size_t)k], nn, N, L, D.left, D.right, D.mask, rs);
+  }
+  for (int k = 0; k <= P.maxd; k++) {
+    const int nn = P.lev0[(size_t)k + 1] - P.lev0[(size_t)k];
+    hipLaunchKernelGGL(k_fitch_down, dim3((unsigned)(((int64_t)nn * L + 255) / 256), ry), dim3(256), 0, c->stream,
+                       D.nodes + P.lev0[(size_t)k], nn, N, L, D.parent, D.seeds, D.mask, D.all, rs);
+  }
+  hipLaunchKernelGGL(k_fitch_subs, dim3((unsigned)(2 * P.m), ry), dim3(64), 0, c->stream, N, L, D.left, D.right, D.all,
+                     D.nsub, rs);
+  hipLaunchKernelGGL(k_gather_root, dim3((unsigned)((L + 255) / 256), ry), dim3(256), 0, c->stream, D.all, D.perm, N, L,
+                     D.inner, rs);
+  for (int k = 0; k <= P.maxd; k++) {
+    const int nn = P.lev0[(size_t)k + 1] - P.lev0[(size_t)k];
+    hipLaunchKernelGGL(k_fitch_replay, dim3((unsigned)((2 * nn + 3) / 4), ry), dim3(256), 0, c->stream,
+                       D.nodes + P.lev0[(size_t)k], nn, N, L, D.left, D.right, D.seeds, D.nsub, D.leaves, D.inner,
+                       D.cols, P.maxw);
+  }
+  return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
+}
+
+extern "C" int rsg_null_fitch(rsg_ctx *c, const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
+                              uint64_t seed, uint8_t *leaves, char *errbuf)
+{
+  if (!c || !ax || !left || !right || !leaves) return rsg_fail(errbuf, "rsg_null_fitch: bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  FitchPlan P;
+  if (rsg_fitch_plan(N, L, left, right, P, errbuf) != RSG_OK) return RSG_FAIL;
+  std::vector<int> perm((size_t)L);
+  rsg_fitch_perm(seed, L, perm.data());
+  const int64_t nrow = 2 * N - 1, m = P.m;
   Buf B;
-  uint8_t *d_ax, *d_mask, *d_all, *d_sh; int *d_left, *d_right, *d_parent, *d_nodes, *d_perm, *d_nsub, *d_cols;
+  uint8_t *d_ax, *d_mask, *d_all, *d_inner, *d_leaves; int *d_left, *d_right, *d_parent, *d_nodes, *d_perm, *d_nsub, *d_cols;
+  uint64_t *d_seed;
   if (!B.get(&d_ax, (size_t)(N * L)) || !B.get(&d_mask, (size_t)(nrow * L)) || !B.get(&d_all, (size_t)(nrow * L)) ||
-      !B.get(&d_sh, (size_t)(nrow * L)) || !B.get(&d_left, (size_t)m) || !B.get(&d_right, (size_t)m) ||
-      !B.get(&d_parent, (size_t)m) || !B.get(&d_nodes, (size_t)m) || !B.get(&d_perm, (size_t)L) ||
-      !B.get(&d_nsub, (size_t)(2 * m) * 25) || !B.get(&d_cols, (size_t)(2 * maxw) * (size_t)L))
+      !B.get(&d_inner, (size_t)((N - 1) * L)) || !B.get(&d_leaves, (size_t)(N * L)) || !B.get(&d_left, (size_t)m) ||
+      !B.get(&d_right, (size_t)m) || !B.get(&d_parent, (size_t)m) || !B.get(&d_nodes, (size_t)m) ||
+      !B.get(&d_perm, (size_t)L) || !B.get(&d_nsub, (size_t)(2 * m) * 25) || !B.get(&d_cols, (size_t)(2 * P.maxw) * (size_t)L) ||
+      !B.get(&d_seed, 1))
     return rsg_fail(errbuf, "rsg_null_fitch: out of device memory");
   HIPCHK(hipMemcpyAsync(d_ax, ax, (size_t)(N * L), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_left, left, sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_right, right, sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_parent, parent.data(), sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_nodes, lev_nodes.data(), sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_parent, P.parent.data(), sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_nodes, P.nodes.data(), sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * L, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_seed, &seed, sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+  FitchDev D{d_ax, d_left, d_right, d_parent, d_nodes, d_perm, d_seed, d_mask, d_all, d_inner, d_leaves, d_nsub, d_cols};
   int t; rsg_tstart(c, "fitch_null", &t);
-  hipLaunchKernelGGL(k_fitch_leaves, dim3((unsigned)((N * L + 255) / 256)), dim3(256), 0, c->stream, d_ax, N, L, seed, d_mask, d_all);
-  for (int k = maxd; k >= 0; k--) {
-    const int nn = lev0[(size_t)k + 1] - lev0[(size_t)k];
-    hipLaunchKernelGGL(k_fitch_up, dim3((unsigned)(((int64_t)nn * L + 255) / 256)), dim3(256), 0, c->stream,
-                       d_nodes + lev0[(size_t)k], nn, N, L, d_left, d_right, d_mask);
-  }
-  for (int k = 0; k <= maxd; k++) {
-    const int nn = lev0[(size_t)k + 1] - lev0[(size_t)k];
-    hipLaunchKernelGGL(k_fitch_down, dim3((unsigned)(((int64_t)nn * L + 255) / 256)), dim3(256), 0, c->stream,
-                       d_nodes + lev0[(size_t)k], nn, N, L, d_parent, seed, d_mask, d_all);
-  }
-  hipLaunchKernelGGL(k_fitch_subs, dim3((unsigned)(2 * m)), dim3(64), 0, c->stream, N, L, d_left, d_right, d_all, d_nsub);
-  hipLaunchKernelGGL(k_gather_root, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, c->stream, d_all, d_perm, N, L, d_sh);
-  for (int k = 0; k <= maxd; k++) {
-    const int nn = lev0[(size_t)k + 1] - lev0[(size_t)k];
-    hipLaunchKernelGGL(k_fitch_replay, dim3((unsigned)((2 * nn + 3) / 4)), dim3(256), 0, c->stream,
-                       d_nodes + lev0[(size_t)k], nn, N, L, d_left, d_right, seed, d_nsub, d_sh, d_cols);
-  }
+  const int st = rsg_fitch_launch(c, P, D, 1);
   rsg_tstop(c, t);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(leaves, d_sh, (size_t)(N * L), hipMemcpyDeviceToHost, c->stream));
+  if (st != RSG_OK) return rsg_fail(errbuf, "rsg_null_fitch: launch failed");
+  HIPCHK(hipMemcpyAsync(leaves, d_leaves, (size_t)(N * L), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return RSG_OK;
 }

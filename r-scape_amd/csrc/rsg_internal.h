@@ -194,6 +194,29 @@ int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t 
                         const double *models, const uint64_t *seed, const uint32_t *stream, const double *gapfrac,
                         uint8_t *d_leaves, const int64_t *leaf_off, char *errbuf);
 
+// R-scape's null of one alignment for R replicates at once (rsg_fitch.hip): the tree's levels
+// (internal nodes grouped by depth, nodes[lev0[k] .. lev0[k+1]) at depth k), parents and the
+// widest level, then one launch per kernel and depth covering every replicate.
+struct FitchPlan {
+  int64_t N = 0, L = 0;
+  int m = 0, maxd = 0, maxw = 0;
+  std::vector<int> parent, nodes, lev0;
+};
+// Device buffers of R replicates: ax (N x L codes), the tree (left, right, parent, nodes: m
+// each), perm (R x L), seeds (R), mask and all (R x (2N - 1) x L), inner (R x (N - 1) x L:
+// the shuffled internal rows), leaves (R x N x L: the null alignments), nsub (R x 2m x 25),
+// cols (R x 2 maxw x L ints).
+struct FitchDev {
+  const uint8_t *ax;
+  const int *left, *right, *parent, *nodes, *perm;
+  const uint64_t *seeds;
+  uint8_t *mask, *all, *inner, *leaves;
+  int *nsub, *cols;
+};
+int rsg_fitch_plan(int64_t N, int64_t L, const int *left, const int *right, FitchPlan &P, char *errbuf);
+void rsg_fitch_perm(uint64_t seed, int64_t L, int *perm);
+int rsg_fitch_launch(rsg_ctx *c, const FitchPlan &P, const FitchDev &D, int R);
+
 // Batched pipeline: per-item descriptor of RSG_BDESC int64 fields
 // {N, L, Npad, Lpad, off_xt, off_cnt, off_cov, off_pm, off_rows, off_hist, off_leaf, blk0_pack,
 //  blk0_col, blk0_stat, blk0_el, nb32, group, blk0_hist, off_pm32 (marginal partials),

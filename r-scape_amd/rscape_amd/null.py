@@ -38,6 +38,8 @@ def _declare():
     L.rsg_null_fitch.argtypes = [vp, vp, i64, i64, vp, vp, C.c_uint64, vp, cp]
     L.rsg_null_batch.restype = i32
     L.rsg_null_batch.argtypes = [vp, C.POINTER(NullJob), i32, i32, d, d, d, i32, vp, vp, vp, i32, cp]
+    L.rsg_null_batch_fitch.restype = i32
+    L.rsg_null_batch_fitch.argtypes = [vp, vp, i64, i64, vp, vp, vp, i32, i32, d, d, d, i32, vp, vp, i32, cp]
     L.rsg_null_features.restype = i32
     L.rsg_null_features.argtypes = [vp, vp, vp, vp, vp, vp, vp, cp]
     L.rsg_cov_EvaluesTail.restype = i32
@@ -118,6 +120,25 @@ def fitch_null(ctx, ax, tree, seed=1):
     _check(L.rsg_null_fitch(ctx.h, _p(ax), ax.shape[0], ax.shape[1], _p(left), _p(right), seed, _p(out), err),
            err, "rsg_null_fitch")
     return out
+
+
+def batch_fitch(ctx, ax, tree, seeds, covtype="GTp", bmin=-10.0, pmass=5e-4, fracfit=1.0, expo=True, graph=0):
+    """R-scape's null of one alignment (null_rscape's nshuffle loop) in one device batch: one
+    Fitch + shuffle replicate per seed, each scored, all merged into one histogram and one
+    tail fit.  Returns (fit (8,), histogram)."""
+    from . import Hist, Histogram
+    L = _declare()
+    ax = np.ascontiguousarray(ax, dtype=np.uint8)
+    left = np.ascontiguousarray(tree["left"], dtype=np.int32); right = np.ascontiguousarray(tree["right"], dtype=np.int32)
+    sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+    fit = np.zeros(8)
+    hp = (C.POINTER(Hist) * 1)()
+    err = _err()
+    t = COV[covtype] if isinstance(covtype, str) else covtype
+    _check(L.rsg_null_batch_fitch(ctx.h, _p(ax), ax.shape[0], ax.shape[1], _p(left), _p(right), _p(sd), len(sd), t,
+                                  bmin, pmass, fracfit, 1 if expo else 0, C.cast(hp, C.c_void_p), _p(fit), int(graph),
+                                  err), err, "rsg_null_batch_fitch")
+    return fit, (Histogram(hp[0]) if hp[0] else None)
 
 
 def batch(ctx, jobs, covtype="GTp", bmin=-10.0, pmass=5e-4, fracfit=1.0, expo=True, graph=0, hists=False,

@@ -160,3 +160,38 @@ def test_fitch_null_matches_oracle(ctx, N, L, seed):
     got = NL.fitch_null(ctx, ax, tree, seed=77 + seed)
     _, sh = oracle.fitch_null_rows(ax, tree, 77 + seed)
     assert np.array_equal(got, sh[:N])
+
+
+@pytest.mark.parametrize("N,L,nrep", [(60, 130, 4), (300, 211, 3)])
+def test_batch_fitch_matches_cumulative_null(ctx, N, L, nrep):
+    """R-scape's own null loop in one batch (every Fitch kernel covers all replicates) = the
+    per-replicate path: rsg_null_fitch per seed, score, width from the first replicate,
+    histograms merged, one fit; eager and graph replays agree."""
+    from rscape_amd import synth
+    print("batch fitch N=%d L=%d nrep=%d" % (N, L, nrep), flush=True)
+    rng = np.random.default_rng(N + L)
+    tree = synth.yule_tree(N, rng, 0.15)
+    ax = synth.evolve(tree, L, rng, (0.3, 0.2, 0.2, 0.3))[1].copy()
+    ax[rng.random((N, L)) < 0.05] = 4
+    ax[rng.random((N, L)) < 0.01] = 15
+    seeds = [1000 + 17 * r for r in range(nrep)]
+    fit_b, h_b = NL.batch_fitch(ctx, ax, tree, seeds, "GTp", pmass=0.01)
+    cum, w = None, None
+    mn, mx = np.inf, -np.inf
+    for sd in seeds:
+        msa = ctx.msa(NL.fitch_null(ctx, ax, tree, seed=sd))
+        a, b = msa.calculate("GTp")
+        mn, mx = min(mn, a), max(mx, b)
+        if w is None:
+            w = R.width(b, a)
+        cum = R.Histogram.null_add(cum, msa.rank_hist(-10.0, w))
+    fit, _ = cum.fit(ctx, 0.01, 1.0, True)
+    assert fit_b[5] == w and fit_b[6] == mn and fit_b[7] == mx
+    got = h_b.h
+    assert (got.n, got.Nc, got.imin, got.imax) == (cum.h.n, cum.h.Nc, cum.h.imin, cum.h.imax)
+    assert np.array_equal(got.counts()[got.imin:got.imax + 1], cum.h.counts()[cum.h.imin:cum.h.imax + 1])
+    for i, key in enumerate(["mu", "lambda", "tau", "newmass", "pmass"]):
+        assert np.isclose(fit_b[i], fit[key], rtol=1e-9, atol=1e-12), key
+    fit_g, h_g = NL.batch_fitch(ctx, ax, tree, seeds, "GTp", pmass=0.01, graph=2)
+    assert np.array_equal(fit_g, fit_b)
+    assert np.array_equal(h_g.h.counts(), h_b.h.counts())
