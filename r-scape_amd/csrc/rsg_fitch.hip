@@ -82,31 +82,25 @@ __global__ void k_fitch_down(const int *__restrict__ nodes, int nn, int64_t N, i
 }
 
 // substitution counts of every branch b = 2 v + side: nsub[b][old * 5 + new], old != new
+// (one wave per branch; a 25-bin LDS histogram, exact integer counts)
 __global__ void __launch_bounds__(64) k_fitch_subs(int64_t N, int64_t L, const int *__restrict__ left,
                                                    const int *__restrict__ right, const uint8_t *__restrict__ all,
                                                    int *__restrict__ nsub, int64_t rs)
 {
+  __shared__ int hist[25];
   const int b = blockIdx.x, v = b >> 1, side = b & 1;
   all += blockIdx.y * rs;
   nsub += (int64_t)blockIdx.y * gridDim.x * 25;
+  if (threadIdx.x < 25) hist[threadIdx.x] = 0;
+  __syncthreads();
   const int d = side ? right[v] : left[v];
   const uint8_t *pa = all + (N + v) * L, *ch = all + (int64_t)row_of(d, N) * L;
-  int cnt[25];
-#pragma unroll
-  for (int k = 0; k < 25; k++) cnt[k] = 0;
   for (int64_t c = threadIdx.x; c < L; c += 64) {
     const int a = pa[c], x = ch[c];
-    if (a != x && a <= 4 && x <= 4) {
-#pragma unroll
-      for (int k = 0; k < 25; k++) cnt[k] += (k == a * 5 + x);
-    }
+    if (a != x && a <= 4 && x <= 4) atomicAdd(&hist[a * 5 + x], 1);
   }
-#pragma unroll
-  for (int k = 0; k < 25; k++) {
-    int s = cnt[k];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-    if (threadIdx.x == 0) nsub[b * 25 + k] = s;
-  }
+  __syncthreads();
+  if (threadIdx.x < 25) nsub[b * 25 + threadIdx.x] = hist[threadIdx.x];
 }
 
 // the shuffled root: column c of the root takes column perm[c] of its ancestral states
@@ -129,16 +123,24 @@ __device__ __forceinline__ void wave_sync()
   __builtin_amdgcn_wave_barrier();
 }
 
-// Replay of the branches below the internal nodes of one depth: one wave per branch.
-// cols: scratch of L ints per wave slot (2 maxw slots per replicate).
+// Replay of the branches below the internal nodes of one depth: one wave per branch.  For each
+// old symbol: the parent's positions holding it (ascending, ballot compaction), a partial
+// Fisher-Yates whose swap indices the 64 lanes draw ahead (a draw depends only on its step, not
+// on the array) and lane 0 applies in order, then the positions that receive substitutions
+// written in parallel.  LCOLS: the position list in LDS (L <= FITCH_LDS_L), else in cols_all
+// (L ints per wave slot, 2 maxw slots per replicate).
+#define FITCH_LDS_L 4000
+template <bool LCOLS>
 __global__ void __launch_bounds__(256) k_fitch_replay(const int *__restrict__ nodes, int nn, int64_t N, int64_t L,
                                                       const int *__restrict__ left, const int *__restrict__ right,
                                                       const uint64_t *__restrict__ seeds, const int *__restrict__ nsub,
                                                       uint8_t *__restrict__ leaves, uint8_t *__restrict__ inner,
                                                       int *__restrict__ cols_all, int maxw)
 {
-  const int lane = threadIdx.x & 63;
-  const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);        // branch slot in this level
+  extern __shared__ int lds_cols[];                           // LCOLS: 4 x L ints
+  __shared__ int kbuf[4][64];                                 // drawn swap indices, per wave
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int slot = blockIdx.x * 4 + wv;                       // branch slot in this level
   if (slot >= 2 * nn) return;
   const int64_t r = blockIdx.y;
   const uint64_t seed = seeds[r];
@@ -151,12 +153,12 @@ __global__ void __launch_bounds__(256) k_fitch_replay(const int *__restrict__ no
   const int b = 2 * v + side;
   const uint8_t *pa = inner + (int64_t)v * L;
   uint8_t *ch = (d > 0) ? inner + (int64_t)d * L : leaves + (int64_t)(-d) * L;
-  int *cols = cols_all + (r * 2 * maxw + slot) * L;
+  int *cols = LCOLS ? lds_cols + wv * L : cols_all + (r * 2 * maxw + slot) * L;
   for (int64_t c = lane; c < L; c += 64) ch[c] = pa[c];
   wave_sync();
   for (int oldc = 0; oldc < 5; oldc++) {
-    int nst = 0;
-    for (int nw = 0; nw < 5; nw++) nst += nsub[b * 25 + oldc * 5 + nw];
+    int cnt[5], nst = 0;
+    for (int nw = 0; nw < 5; nw++) { cnt[nw] = nsub[b * 25 + oldc * 5 + nw]; nst += cnt[nw]; }
     if (!nst) continue;
     // positions of the parent holding oldc, ascending (ballot compaction)
     int ncol = 0;
@@ -168,21 +170,32 @@ __global__ void __launch_bounds__(256) k_fitch_replay(const int *__restrict__ no
       ncol += __popcll(bal);
     }
     wave_sync();                                              // cols visible to lane 0
-    if (lane == 0 && ncol > 0) {
-      // partial Fisher-Yates: iteration n fixes position n - 1; only the last min(nst, ncol)
-      // positions receive substitutions
-      const int take = nst < ncol ? nst : ncol;
-      const int nlow = (ncol - take + 1 > 2) ? ncol - take + 1 : 2;
-      for (int n = ncol; n >= nlow; n--) {
+    if (ncol == 0) continue;
+    // partial Fisher-Yates: step n (ncol down to nlow) swaps positions n - 1 and k_n; only the
+    // last take = min(nst, ncol) positions receive substitutions
+    const int take = nst < ncol ? nst : ncol;
+    const int nlow = (ncol - take + 1 > 2) ? ncol - take + 1 : 2;
+    for (int n0 = ncol; n0 >= nlow; n0 -= 64) {
+      const int n = n0 - lane;
+      if (n >= nlow) {
         const uint32_t u = rsg_philox(seed, 4u, (uint32_t)(chrow * 5 + oldc), (uint32_t)n, 4u);
-        const int k = (int)(((uint64_t)u * (uint64_t)n) >> 32);
-        const int t = cols[n - 1]; cols[n - 1] = cols[k]; cols[k] = t;
+        kbuf[wv][lane] = (int)(((uint64_t)u * (uint64_t)n) >> 32);
       }
-      int idx = ncol - 1;
-      for (int nw = 0; nw < 5; nw++) {
-        int s = nsub[b * 25 + oldc * 5 + nw];
-        while (s > 0 && idx >= 0) { ch[cols[idx]] = (uint8_t)nw; idx--; s--; }
+      wave_sync();
+      if (lane == 0) {
+        const int steps = (n0 - nlow + 1 < 64) ? n0 - nlow + 1 : 64;
+        for (int t = 0; t < steps; t++) {
+          const int nn1 = n0 - t - 1, k = kbuf[wv][t];
+          const int x = cols[nn1]; cols[nn1] = cols[k]; cols[k] = x;
+        }
       }
+      wave_sync();
+    }
+    // position ncol - 1 - t (t < take) gets the new symbol whose cumulative count first exceeds t
+    for (int t = lane; t < take; t += 64) {
+      int acc = 0, sym = 4;
+      for (int nw = 0; nw < 5; nw++) { if (t < acc + cnt[nw]) { sym = nw; break; } acc += cnt[nw]; }
+      ch[cols[ncol - 1 - t]] = (uint8_t)sym;
     }
     wave_sync();
   }
@@ -269,11 +282,18 @@ int rsg_fitch_launch(rsg_ctx *c, const FitchPlan &P, const FitchDev &D, int R)
                      D.nsub, rs);
   hipLaunchKernelGGL(k_gather_root, dim3((unsigned)((L + 255) / 256), ry), dim3(256), 0, c->stream, D.all, D.perm, N, L,
                      D.inner, rs);
+  const bool lcols = L <= FITCH_LDS_L;
   for (int k = 0; k <= P.maxd; k++) {
     const int nn = P.lev0[(size_t)k + 1] - P.lev0[(size_t)k];
-    hipLaunchKernelGGL(k_fitch_replay, dim3((unsigned)((2 * nn + 3) / 4), ry), dim3(256), 0, c->stream,
-                       D.nodes + P.lev0[(size_t)k], nn, N, L, D.left, D.right, D.seeds, D.nsub, D.leaves, D.inner,
-                       D.cols, P.maxw);
+    const dim3 grid((unsigned)((2 * nn + 3) / 4), ry);
+    if (lcols)
+      hipLaunchKernelGGL(k_fitch_replay<true>, grid, dim3(256), (size_t)(4 * L) * sizeof(int), c->stream,
+                         D.nodes + P.lev0[(size_t)k], nn, N, L, D.left, D.right, D.seeds, D.nsub, D.leaves, D.inner,
+                         D.cols, P.maxw);
+    else
+      hipLaunchKernelGGL(k_fitch_replay<false>, grid, dim3(256), 0, c->stream,
+                         D.nodes + P.lev0[(size_t)k], nn, N, L, D.left, D.right, D.seeds, D.nsub, D.leaves, D.inner,
+                         D.cols, P.maxw);
   }
   return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
 }

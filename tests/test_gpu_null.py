@@ -148,7 +148,7 @@ def test_batch_rejects_bad_groups_and_trees(ctx):
         NL.simulate(ctx, NL.Job(bad, 20, m))
 
 
-@pytest.mark.parametrize("N,L,seed", [(2, 7, 1), (4, 40, 2), (60, 130, 3), (500, 300, 4)])
+@pytest.mark.parametrize("N,L,seed", [(2, 7, 1), (4, 40, 2), (60, 130, 3), (500, 300, 4), (24, 4100, 5)])
 def test_fitch_null_matches_oracle(ctx, N, L, seed):
     """R-scape's null on the device: symbol-identical to the oracle on the same keys."""
     from rscape_amd import synth
@@ -162,7 +162,7 @@ def test_fitch_null_matches_oracle(ctx, N, L, seed):
     assert np.array_equal(got, sh[:N])
 
 
-@pytest.mark.parametrize("N,L,nrep", [(60, 130, 4), (300, 211, 3)])
+@pytest.mark.parametrize("N,L,nrep", [(60, 130, 4), (300, 211, 3), (5, 80, 1), (9, 64, 2)])
 def test_batch_fitch_matches_cumulative_null(ctx, N, L, nrep):
     """R-scape's own null loop in one batch (every Fitch kernel covers all replicates) = the
     per-replicate path: rsg_null_fitch per seed, score, width from the first replicate,
