@@ -380,15 +380,17 @@ k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t Lpad, int64_t L,
                                pmi, pmi ? pmi + pm32_n(nb32) : nullptr, nb32);
 }
 
-// Split tiles: one work group (4 waves) per 32 x 32 block sums the block's partial records over
+// Split tiles: one work group (16 waves) per 32 x 32 block sums the block's partial records over
 // the tile's K segments (integer sums: exact; uint16 records add as words, no half exceeding
-// 65535) and writes the records; wave w takes r = 4w .. 4w + 3 of the MFMA result layout.  For
+// 65535) and writes the records; wave w takes row r = w of the MFMA result layout, all its
+// segments' loads in flight together.  For
 // uint16 records with pm_part the block's marginal partials follow with the arithmetic and order
 // of the Gram epilogue: the row and column terms go through LDS, wave 0 adds the row terms over
 // r = 0..7 and r = 8..15 and then the two sums, waves 2 and 3 reduce-scatter the column terms of
 // r = 0..7 and r = 8..15 (index 4 (r % 8) + y).
+#define FIX_WAVES 16
 template <int BI, int BJ, int OK>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * FIX_WAVES)
 k_gram_fix(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, int ndp, int sbase, int sextra, int64_t L,
            void *__restrict__ out, double *__restrict__ pm_part)
 {
@@ -402,15 +404,17 @@ k_gram_fix(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, int
   const int bi32 = tl.x * NBI + wi, bj32 = tl.y * (BJ / 32) + wj, Li = (int)L, ic = bi32 * 32 + col;
   const int ns = sbase + (tr < sextra ? 1 : 0);
   const uint4 *p0 = (const uint4 *)part + (int64_t)split_first(tr, sbase, sextra) * BI * BJ * NQ;
-  uint4 s[4][NQ];
+  constexpr int RPW = 16 / FIX_WAVES;                               // rows per wave
+  uint4 s[RPW][NQ];
 #pragma unroll
-  for (int q = 0; q < 4; q++)
+  for (int q = 0; q < RPW; q++)
 #pragma unroll
     for (int h = 0; h < NQ; h++) s[q][h] = make_uint4(0, 0, 0, 0);
+#pragma unroll 4
   for (int k = 0; k < ns; k++)
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int r = 4 * wave + q, jb = 8 * (r >> 2) + 4 * khalf + (r & 3);
+    for (int q = 0; q < RPW; q++) {
+      const int r = RPW * wave + q, jb = 8 * (r >> 2) + 4 * khalf + (r & 3);
       const int64_t pr = (int64_t)(wj * 32 + jb) * BI + wi * 32 + col;   // partial record in the tile
 #pragma unroll
       for (int h = 0; h < NQ; h++) {
@@ -421,8 +425,8 @@ k_gram_fix(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, int
   const int nb32 = (Li + 31) / 32;
   const bool blkok = bi32 <= bj32 && bj32 < nb32;
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int r = 4 * wave + q, j = bj32 * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
+  for (int q = 0; q < RPW; q++) {
+    const int r = RPW * wave + q, j = bj32 * 32 + 8 * (r >> 2) + 4 * khalf + (r & 3);
     const bool valid = ic < j && j < Li;
     if (valid)
 #pragma unroll
@@ -448,7 +452,7 @@ k_gram_fix(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, int
   }
   if (OK != CNT_U16 || pm_part == nullptr) return;
   __syncthreads();
-  if (!blkok || wave == 1) return;
+  if (!blkok || wave == 1 || wave > 3) return;
   double *pm_i = pm_part, *pm_j = pm_part + pm32_n(nb32);
   const int64_t bq = pm_tri(bi32, bj32, nb32);
   if (wave == 0) {
@@ -707,7 +711,7 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, double *pm_part, int *pm
   hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->Lpad, m->L, out,
                      tiles, ndp, per, ntiles, sbase, sextra, part, pm_part, pmflag);
   if (nrem > 0)
-    hipLaunchKernelGGL((k_gram_fix<BI, BJ, OK>), dim3((unsigned)(nrem * (BI / 32) * (BJ / 32))), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL((k_gram_fix<BI, BJ, OK>), dim3((unsigned)(nrem * (BI / 32) * (BJ / 32))), dim3(64 * FIX_WAVES), 0, c->stream,
                        part, tiles, ndp, sbase, sextra, m->L, out, pm_part);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
