@@ -199,6 +199,19 @@ int rsg_null_fitch(rsg_ctx *ctx, const uint8_t *ax, int64_t nseq, int64_t alen, 
                    uint64_t seed, uint8_t *leaves, char *errbuf);
 /* Simulate one alignment; leaves = nleaf x alen codes (the leaf rows, leaf order). */
 int rsg_null_simulate(rsg_ctx *ctx, const rsg_null_job *job, uint8_t *leaves, char *errbuf);
+/* 16-state pair model for structured simulation: Q16 = the ribosum pair rate matrix bprsQ
+ * (16 x 16 row-major over pair states a*4 + b, 5' residue a, 3' residue b; rows sum to 0) and
+ * M16 = its stationary pair frequencies bprsM (src/cov_simulate.c:393-394).  Q16 must be
+ * reversible with respect to M16.  model16 receives 544 doubles (eigendecomposition for
+ * P(t) = exp(t Q16), ratematrix_ConditionalsFromRate src/ratematrix.c:185-233).  RSG_FAIL for
+ * an invalid or non-reversible model.  Host only. */
+int rsg_pair_model(const double *Q16, const double *M16, double *model16);
+/* cov_GenerateAlignmentUngapped with a structure (src/cov_simulate.c:88-159, 288-451, 724-798;
+ * noss off, no indels): as rsg_null_simulate, but every base-paired column pair evolves as one
+ * 16-state character under model16 (root pair from M16, cov_addpair).  ct[c] = partner column
+ * of c (0-based) or -1 (the reference's 1-based ct with 0 = unpaired, shifted). */
+int rsg_null_simulate_pairs(rsg_ctx *ctx, const rsg_null_job *job, const int *ct, const double *model16,
+                            uint8_t *leaves, char *errbuf);
 /* Simulate and score a batch of alignments of any sizes (GT/MI/... + correction, unit
  * weights, CSELECT) entirely on the device, then build one null per group of jobs: jobs that
  * share a group id (ids contiguous, ascending from 0) are the replicates of one null, as

@@ -119,6 +119,90 @@ int ref_evolve_gtr(int64_t nleaf, int64_t L, const int *left, const int *right, 
   return REF_OK;
 }
 
+/* Structured null (cov_simulate.c:393-412, 774-798 with a structure, noss off): a base-paired
+ * column pair (i, ct[i]) evolves as one 16-state character, pair state a * 4 + b; the root pair
+ * is drawn from the stationary pair frequencies, a child pair from its parent pair's P(t) row.
+ * m16: [0..255] Pi^-1/2 V, [256..511] V^T Pi^1/2, [512..527] eigenvalues, [528..543] pair
+ * frequencies.  Keys: (row, 5' column, tag 2); leaf gaps reuse the tag-1 draws.  Unpaired
+ * columns (and, before they are overwritten, paired ones) evolve as in ref_evolve_gtr.
+ * ct[c] = partner column or -1. */
+static void row_cdf16(const double *m, double t, int a, uint32_t *out)
+{
+  double e[16], cum = 0.;
+  for (int k = 0; k < 16; k++) e[k] = exp(m[512 + k] * t);
+  for (int b = 0; b < 15; b++) {
+    double p = 0.;
+    for (int k = 0; k < 16; k++) p += m[a * 16 + k] * e[k] * m[256 + k * 16 + b];
+    cum += (p > 0.) ? p : 0.;
+    double v = cum * 4294967296.0;
+    out[b] = (v >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)v;
+  }
+}
+
+int ref_evolve_pairs(int64_t nleaf, int64_t L, const int *left, const int *right, const double *ld,
+                     const double *rd, const double *model, const double *m16, const int *ct, uint64_t seed,
+                     uint32_t stream, uint32_t gapthr, uint8_t *leaves)
+{
+  int64_t nrow = 2 * nleaf - 1;
+  uint8_t *S = malloc((size_t)(nrow * L));
+  uint32_t pic[3], pth[15], tab[12], tab16[16 * 15];
+  double cum = 0.;
+  for (int b = 0; b < 3; b++) {
+    cum += model[36 + b];
+    double v = cum * 4294967296.0;
+    pic[b] = (v >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)v;
+  }
+  cum = 0.;
+  for (int b = 0; b < 15; b++) {
+    cum += m16[528 + b];
+    double v = cum * 4294967296.0;
+    pth[b] = (v >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)v;
+  }
+  for (int64_t pos = 0; pos < L; pos++) {
+    uint32_t u = ref_rng_u32(seed, stream, (uint32_t)nrow, (uint32_t)pos);
+    int a = 3;
+    for (int b = 0; b < 3; b++) if (u < pic[b]) { a = b; break; }
+    S[pos] = (uint8_t)a;
+  }
+  for (int64_t pos = 0; pos < L; pos++) {
+    if (ct[pos] <= pos) continue;
+    int st = icdf(rng_tag(seed, stream, (uint32_t)nrow, (uint32_t)pos, 2u), pth, 16);
+    S[pos] = (uint8_t)(st >> 2); S[ct[pos]] = (uint8_t)(st & 3);
+  }
+  for (int64_t v = 0; v < nleaf - 1; v++)
+    for (int side = 0; side < 2; side++) {
+      int d = side ? right[v] : left[v];
+      int64_t row = (d > 0) ? d : (nleaf - 1 - d);
+      double t = side ? rd[v] : ld[v];
+      for (int a = 0; a < 4; a++) row_cdf(model, t, a, tab + a * 3);
+      for (int a = 0; a < 16; a++) row_cdf16(m16, t, a, tab16 + a * 15);
+      for (int64_t pos = 0; pos < L; pos++) {
+        uint32_t u = ref_rng_u32(seed, stream, (uint32_t)row, (uint32_t)pos);
+        int sym = icdf(u, tab + S[v * L + pos] * 3, 4);
+        S[row * L + pos] = (uint8_t)sym;
+        if (d <= 0) {
+          if (gapthr && rng_tag(seed, stream, (uint32_t)row, (uint32_t)pos, 1u) < gapthr) sym = 4;
+          leaves[(-d) * L + pos] = (uint8_t)sym;
+        }
+      }
+      for (int64_t pos = 0; pos < L; pos++) {
+        int64_t q = ct[pos];
+        if (q <= pos) continue;
+        int a = 4 * S[v * L + pos] + S[v * L + q];
+        int st = icdf(rng_tag(seed, stream, (uint32_t)row, (uint32_t)pos, 2u), tab16 + a * 15, 16);
+        int x = st >> 2, y = st & 3;
+        S[row * L + pos] = (uint8_t)x; S[row * L + q] = (uint8_t)y;
+        if (d <= 0) {
+          if (gapthr && rng_tag(seed, stream, (uint32_t)row, (uint32_t)pos, 1u) < gapthr) x = 4;
+          if (gapthr && rng_tag(seed, stream, (uint32_t)row, (uint32_t)q, 1u) < gapthr) y = 4;
+          leaves[(-d) * L + pos] = (uint8_t)x; leaves[(-d) * L + q] = (uint8_t)y;
+        }
+      }
+    }
+  free(S);
+  return REF_OK;
+}
+
 /* R-scape's null (R-scape.c:1650-1697): Fitch ancestral reconstruction per column
  * (msatree.c:173-229, 1700-1931), one shuffle of all columns of leaves+ancestors
  * (msamanip.c:1461 -> 1164), then for every branch parent -> child replay the

@@ -103,6 +103,9 @@ uint32_t ref_rng_u32(uint64_t seed, uint32_t stream, uint32_t node, uint32_t pos
 int ref_evolve_gtr(int64_t nleaf, int64_t L, const int *left, const int *right, const double *ld,
                    const double *rd, const double *model, uint64_t seed, uint32_t stream, uint32_t gapthr,
                    uint8_t *leaves);
+int ref_evolve_pairs(int64_t nleaf, int64_t L, const int *left, const int *right, const double *ld,
+                     const double *rd, const double *model, const double *m16, const int *ct, uint64_t seed,
+                     uint32_t stream, uint32_t gapthr, uint8_t *leaves);
 int ref_fitch_null(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
                    uint64_t seed, uint8_t *out);
 int ref_fitch_null_rows(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,

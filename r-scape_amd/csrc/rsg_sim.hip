@@ -389,3 +389,227 @@ extern "C" int rsg_null_simulate(rsg_ctx *c, const rsg_null_job *job, uint8_t *l
   hipFree(d_leaves);
   return rc;
 }
+
+// ---------------------------------------------------------------- structured nulls: pair model
+// cov_GenerateAlignmentUngapped with a structure (noss off): a base-paired column pair (i, ct[i])
+// evolves as one 16-state character under the ribosum pair rate matrix (src/cov_simulate.c:393-394,
+// 406-412: P(t) = ratematrix_ConditionalsFromRate(t, bprsQ)); the root pair is drawn from the
+// stationary pair frequencies bprsM and a child pair from its parent pair's P(t) row
+// (cov_addpair, :774-798).  Unpaired columns evolve under the 4-state GTR model as before.
+// Device order: the 4-state level kernel draws every column of a level's children, then the
+// pair kernel overwrites both columns of every pair; draws are Philox keyed by (seed, stream,
+// child row, 5' column, tag 2), leaf gaps reuse the tag-1 draws of the 4-state path.
+
+// Eigendecomposition of a symmetric n x n (cyclic Jacobi; A, V row-major n x n).
+static void jacobi_n(int n, double *A, double *V, double *lam)
+{
+  for (int i = 0; i < n; i++) for (int j = 0; j < n; j++) V[i * n + j] = (i == j);
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.;
+    for (int p = 0; p < n; p++) for (int q = p + 1; q < n; q++) off += A[p * n + q] * A[p * n + q];
+    if (off < 1e-30) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        if (fabs(A[p * n + q]) < 1e-300) continue;
+        double th = (A[q * n + q] - A[p * n + p]) / (2 * A[p * n + q]);
+        double t = (th >= 0 ? 1. : -1.) / (fabs(th) + sqrt(th * th + 1.));
+        double cs = 1. / sqrt(t * t + 1.), sn = t * cs;
+        for (int k = 0; k < n; k++) {
+          double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = cs * akp - sn * akq; A[k * n + q] = sn * akp + cs * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = cs * apk - sn * aqk; A[q * n + k] = sn * apk + cs * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          double vkp = V[k * n + p], vkq = V[k * n + q];
+          V[k * n + p] = cs * vkp - sn * vkq; V[k * n + q] = sn * vkp + cs * vkq;
+        }
+      }
+  }
+  for (int i = 0; i < n; i++) lam[i] = A[i * n + i];
+}
+
+// Packed: [0..255] Pi^-1/2 V, [256..511] V^T Pi^1/2, [512..527] eigenvalues, [528..543] M16.
+// P(t)[a][b] = sum_k model[a * 16 + k] exp(lambda_k t) model[256 + k * 16 + b].
+extern "C" int rsg_pair_model(const double *Q16, const double *M16, double *model)
+{
+  if (!Q16 || !M16 || !model) return RSG_FAIL;
+  double msum = 0., qmax = 0.;
+  for (int a = 0; a < 16; a++) {
+    if (!(M16[a] > 0.)) return RSG_FAIL;                       // every pair state reachable
+    msum += M16[a];
+    double rs = 0.;
+    for (int b = 0; b < 16; b++) {
+      if (a != b && !(Q16[a * 16 + b] >= 0.)) return RSG_FAIL;
+      rs += Q16[a * 16 + b]; qmax = std::max(qmax, fabs(Q16[a * 16 + b]));
+    }
+    if (fabs(rs) > 1e-9 * std::max(1., qmax)) return RSG_FAIL;   // rows of a rate matrix sum to 0
+  }
+  if (fabs(msum - 1.) > 1e-9) return RSG_FAIL;
+  std::vector<double> S(256), V(256), lam(16);
+  for (int a = 0; a < 16; a++)
+    for (int b = 0; b < 16; b++) S[a * 16 + b] = sqrt(M16[a]) * Q16[a * 16 + b] / sqrt(M16[b]);
+  for (int a = 0; a < 16; a++)                                  // reversible: M16[a] Q[a][b] = M16[b] Q[b][a]
+    for (int b = a + 1; b < 16; b++) {
+      if (fabs(S[a * 16 + b] - S[b * 16 + a]) > 1e-9 * std::max(1., qmax)) return RSG_FAIL;
+      const double s = 0.5 * (S[a * 16 + b] + S[b * 16 + a]);
+      S[a * 16 + b] = S[b * 16 + a] = s;
+    }
+  jacobi_n(16, S.data(), V.data(), lam.data());
+  for (int a = 0; a < 16; a++)
+    for (int k = 0; k < 16; k++) {
+      model[a * 16 + k] = V[a * 16 + k] / sqrt(M16[a]);
+      model[256 + k * 16 + a] = V[a * 16 + k] * sqrt(M16[a]);
+    }
+  for (int k = 0; k < 16; k++) { model[512 + k] = lam[(size_t)k]; model[528 + k] = M16[k]; }
+  return RSG_OK;
+}
+
+// CDF thresholds of one 16-state P(t) row: c_k = floor(2^32 sum_{b<=k} P[a][b]), k = 0..14.
+__device__ void row_cdf16(const double *m, double t, int a, uint32_t *out)
+{
+  double e[16];
+  for (int k = 0; k < 16; k++) e[k] = exp(m[512 + k] * t);
+  double cum = 0.;
+  for (int b = 0; b < 15; b++) {
+    double p = 0.;
+    for (int k = 0; k < 16; k++) p += m[a * 16 + k] * e[k] * m[256 + k * 16 + b];
+    cum += (p > 0.) ? p : 0.;
+    const double v = cum * 4294967296.0;
+    out[b] = (v >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)v;
+  }
+}
+
+__device__ __forceinline__ int icdf15(uint32_t u, const uint32_t *c)
+{
+  int k = 0;
+#pragma unroll
+  for (int b = 0; b < 15; b++) k += (u >= c[b]);
+  return k;
+}
+
+// Pair tables [node][side][16 parent states][15]
+__global__ void k_pair_tables(const double *__restrict__ m16, const double *__restrict__ ld, const double *__restrict__ rd,
+                              int64_t nodes, uint32_t *__restrict__ tab)
+{
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;     // (node, side, parent state)
+  if (idx >= nodes * 32) return;
+  const int64_t node = idx / 32; const int side = (int)((idx / 16) & 1), a = (int)(idx & 15);
+  row_cdf16(m16, side ? rd[node] : ld[node], a, tab + (node * 2 + side) * 240 + a * 15);
+}
+
+// Root pairs from the stationary pair frequencies (key of row 2N - 1, 5' column, tag 2).
+__global__ void __launch_bounds__(64)
+k_pair_root(const double *__restrict__ m16, const int2 *__restrict__ pairs, int npairs, int64_t N, uint64_t seed,
+            uint32_t stream, uint8_t *__restrict__ root)
+{
+  const int p = blockIdx.x * 64 + threadIdx.x;
+  if (p >= npairs) return;
+  const int2 pr = pairs[p];
+  uint32_t th[15];
+  double cum = 0.;
+  for (int b = 0; b < 15; b++) {
+    cum += m16[528 + b];
+    const double v = cum * 4294967296.0;
+    th[b] = (v >= 4294967295.0) ? 0xFFFFFFFFu : (uint32_t)v;
+  }
+  const int st = icdf15(rsg_philox(seed, stream, (uint32_t)(2 * N - 1), (uint32_t)pr.x, 2u), th);
+  root[pr.x] = (uint8_t)(st >> 2);
+  root[pr.y] = (uint8_t)(st & 3);
+}
+
+// One tree level: the pairs of both children of every node at this depth (blockIdx.y).
+__global__ void __launch_bounds__(64)
+k_pair_level(const int *__restrict__ nodes, const int2 *__restrict__ pairs, int npairs, const int *__restrict__ left,
+             const int *__restrict__ right, const uint32_t *__restrict__ tab, int64_t N, int64_t L, uint64_t seed,
+             uint32_t stream, uint32_t gapthr, uint8_t *__restrict__ S, uint8_t *__restrict__ leaves)
+{
+  const int p = blockIdx.x * 64 + threadIdx.x;
+  if (p >= npairs) return;
+  const int v = nodes[blockIdx.y];
+  const int2 pr = pairs[p];
+  const int a = 4 * S[(int64_t)v * L + pr.x] + S[(int64_t)v * L + pr.y];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const int d = side ? right[v] : left[v];
+    const int64_t row = (d > 0) ? d : (N - 1 - d);
+    const int st = icdf15(rsg_philox(seed, stream, (uint32_t)row, (uint32_t)pr.x, 2u), tab + ((int64_t)v * 2 + side) * 240 + a * 15);
+    int x = st >> 2, y = st & 3;
+    if (d > 0) { S[(int64_t)d * L + pr.x] = (uint8_t)x; S[(int64_t)d * L + pr.y] = (uint8_t)y; }
+    else {
+      if (gapthr && rsg_philox(seed, stream, (uint32_t)row, (uint32_t)pr.x, 1u) < gapthr) x = 4;
+      if (gapthr && rsg_philox(seed, stream, (uint32_t)row, (uint32_t)pr.y, 1u) < gapthr) y = 4;
+      leaves[(int64_t)(-d) * L + pr.x] = (uint8_t)x;
+      leaves[(int64_t)(-d) * L + pr.y] = (uint8_t)y;
+    }
+  }
+}
+
+extern "C" int rsg_null_simulate_pairs(rsg_ctx *c, const rsg_null_job *job, const int *ct, const double *model16,
+                                       uint8_t *leaves, char *errbuf)
+{
+  if (!c || !job || !ct || !model16 || !leaves) return rsg_fail(errbuf, "rsg_null_simulate_pairs: bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t N = job->nleaf, L = job->alen;
+  if (N < 2 || L < 1) return rsg_fail(errbuf, "simulation needs >= 2 leaves and >= 1 column");
+  if (L > INT_MAX) return rsg_fail(errbuf, "alignment too long");
+  std::vector<int2> pairs;
+  for (int64_t i = 0; i < L; i++) {
+    const int j = ct[i];
+    if (j < 0) continue;
+    if (j >= L || j == i || ct[j] != (int)i) return rsg_fail(errbuf, "rsg_null_simulate_pairs: ct is not a pairing at column %lld", (long long)i);
+    if (j > i) pairs.push_back(make_int2((int)i, j));
+  }
+  SimItem it{};
+  it.N = N; it.L = L; it.tree_off = 0; it.seq_off = 0; it.leaf_off = 0; it.seed = job->seed; it.stream = job->stream;
+  it.gapthr = (job->gapfrac <= 0.) ? 0u : (job->gapfrac >= 1. ? 0xFFFFFFFFu : (uint32_t)(job->gapfrac * 4294967296.0));
+  it.col_blk0 = 0; it.model_off = 0;
+  SimLevels lv;
+  if (rsg_sim_levels(&it, 1, job->left, job->right, lv, errbuf) != RSG_OK) return RSG_FAIL;
+  const int64_t nodes = N - 1, blocks = (L + 63) / 64, np = (int64_t)pairs.size();
+  std::vector<int> lev_nodes(lv.ent.size());
+  for (size_t k = 0; k < lv.ent.size(); k++) lev_nodes[k] = lv.ent[k].y;
+  SimBuffers b;
+  int *d_left, *d_right, *d_nodes; double *d_ld, *d_rd, *d_models, *d_m16; uint32_t *d_cdf, *d_tab; uint8_t *d_seqs, *d_leaves;
+  SimItem *d_items; int4 *d_ent; int2 *d_pairs;
+  if (!b.get(&d_left, (size_t)nodes) || !b.get(&d_right, (size_t)nodes) || !b.get(&d_ld, (size_t)nodes) ||
+      !b.get(&d_rd, (size_t)nodes) || !b.get(&d_cdf, 24 * (size_t)nodes) || !b.get(&d_models, 40) || !b.get(&d_m16, 544) ||
+      !b.get(&d_tab, 480 * (size_t)nodes) || !b.get(&d_seqs, (size_t)(nodes * L)) || !b.get(&d_leaves, (size_t)(N * L)) ||
+      !b.get(&d_items, 1) || !b.get(&d_ent, lv.ent.size()) || !b.get(&d_pairs, (size_t)np) || !b.get(&d_nodes, lev_nodes.size()))
+    return rsg_fail(errbuf, "simulation buffers: out of device memory");
+  HIPCHK(hipMemcpyAsync(d_left, job->left, sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_right, job->right, sizeof(int) * nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_ld, job->ld, sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_rd, job->rd, sizeof(double) * nodes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_models, job->model, sizeof(double) * 40, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_m16, model16, sizeof(double) * 544, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_items, &it, sizeof(SimItem), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_ent, lv.ent.data(), sizeof(int4) * lv.ent.size(), hipMemcpyHostToDevice, c->stream));
+  if (np) HIPCHK(hipMemcpyAsync(d_pairs, pairs.data(), sizeof(int2) * np, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_nodes, lev_nodes.data(), sizeof(int) * lev_nodes.size(), hipMemcpyHostToDevice, c->stream));
+  int t; rsg_tstart(c, "simulate_pairs", &t);
+  hipLaunchKernelGGL(k_branch_tables, dim3((unsigned)((nodes * 8 + 255) / 256)), dim3(256), 0, c->stream,
+                     d_items, 1, d_ld, d_rd, d_models, nodes, d_cdf);
+  hipLaunchKernelGGL(k_pair_tables, dim3((unsigned)((nodes * 32 + 255) / 256)), dim3(256), 0, c->stream, d_m16, d_ld, d_rd,
+                     nodes, d_tab);
+  hipLaunchKernelGGL(k_root, dim3((unsigned)blocks), dim3(64), 0, c->stream, d_items, 1, d_models, d_seqs);
+  if (np) hipLaunchKernelGGL(k_pair_root, dim3((unsigned)((np + 63) / 64)), dim3(64), 0, c->stream, d_m16, d_pairs, (int)np, N,
+                             it.seed, it.stream, d_seqs);
+  for (size_t l = 0; l < lv.lev_blocks.size(); l++) {
+    const int64_t e0 = lv.lev_ent0[l], ne = lv.lev_ent0[l + 1] - e0;
+    if (lv.lev_blocks[l] > 0)
+      hipLaunchKernelGGL(k_evolve_level, dim3((unsigned)lv.lev_blocks[l]), dim3(64), 0, c->stream, d_items,
+                         d_ent + e0, (int)ne, d_left, d_right, d_cdf, d_seqs, d_leaves);
+    if (np && ne > 0)
+      hipLaunchKernelGGL(k_pair_level, dim3((unsigned)((np + 63) / 64), (unsigned)ne), dim3(64), 0, c->stream,
+                         d_nodes + e0, d_pairs, (int)np, d_left, d_right, d_tab, N, L, it.seed, it.stream, it.gapthr,
+                         d_seqs, d_leaves);
+  }
+  rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(leaves, d_leaves, (size_t)(N * L), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return RSG_OK;
+}

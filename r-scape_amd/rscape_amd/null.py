@@ -34,6 +34,9 @@ def _declare():
     L.rsg_tree_yule.restype = i32
     L.rsg_tree_yule.argtypes = [i64, d, C.c_uint64, C.c_uint32, vp, vp, vp, vp]
     L.rsg_null_simulate.restype, L.rsg_null_simulate.argtypes = i32, [vp, C.POINTER(NullJob), vp, cp]
+    L.rsg_pair_model.restype, L.rsg_pair_model.argtypes = i32, [vp, vp, vp]
+    L.rsg_null_simulate_pairs.restype = i32
+    L.rsg_null_simulate_pairs.argtypes = [vp, C.POINTER(NullJob), vp, vp, vp, cp]
     L.rsg_null_fitch.restype = i32
     L.rsg_null_fitch.argtypes = [vp, vp, i64, i64, vp, vp, C.c_uint64, vp, cp]
     L.rsg_null_batch.restype = i32
@@ -106,6 +109,30 @@ def simulate(ctx, job):
     s = job.struct()
     err = _err()
     _check(L.rsg_null_simulate(ctx.h, C.byref(s), _p(out), err), err, "rsg_null_simulate")
+    return out
+
+
+def pair_model(Q16, M16):
+    """Eigendecomposed 16-state pair model (544 doubles) from a reversible pair rate matrix and
+    its stationary pair frequencies (states a*4 + b)."""
+    L = _declare()
+    q = np.ascontiguousarray(Q16, dtype=np.float64).reshape(256)
+    m = np.ascontiguousarray(M16, dtype=np.float64).reshape(16)
+    out = np.zeros(544)
+    if L.rsg_pair_model(_p(q), _p(m), _p(out)) != 0:
+        raise ValueError("bad or non-reversible pair model")
+    return out
+
+
+def simulate_pairs(ctx, job, ct, model16):
+    """Structured simulation: columns paired by ct (partner or -1) evolve under model16."""
+    L = _declare()
+    s = job.struct(0)
+    out = np.zeros((job.tree["n"], job.L), dtype=np.uint8)
+    ctv = np.ascontiguousarray(ct, dtype=np.int32)
+    m16 = np.ascontiguousarray(model16, dtype=np.float64)
+    err = _err()
+    _check(L.rsg_null_simulate_pairs(ctx.h, C.byref(s), _p(ctv), _p(m16), _p(out), err), err, "rsg_null_simulate_pairs")
     return out
 
 

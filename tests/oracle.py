@@ -189,6 +189,21 @@ def evolve_gtr(tree, L, model, seed, stream, gapthr=0):
     return out
 
 
+def evolve_pairs(tree, L, model, m16, ct, seed, stream, gapthr=0):
+    """Structured null: paired columns (ct[c] = partner or -1) evolve under the 16-state model."""
+    n = tree["n"]
+    out = np.zeros((n, L), dtype=np.uint8)
+    f = lib().ref_evolve_pairs
+    f.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p]
+    left = np.ascontiguousarray(tree["left"], dtype=np.int32); right = np.ascontiguousarray(tree["right"], dtype=np.int32)
+    ld = np.ascontiguousarray(tree["ld"], dtype=np.float64); rd = np.ascontiguousarray(tree["rd"], dtype=np.float64)
+    ctv = np.ascontiguousarray(ct, dtype=np.int32)
+    f(n, L, P(left), P(right), P(ld), P(rd), P(np.ascontiguousarray(model, dtype=np.float64)),
+      P(np.ascontiguousarray(m16, dtype=np.float64)), P(ctv), seed, stream, gapthr, P(out))
+    return out
+
+
 def mlp_forward_backward(W, b, x, dy=None):
     """FP64 oracle forward (and backward for dy) of the surrogate MLP."""
     nl = len(W)

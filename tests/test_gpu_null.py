@@ -195,3 +195,52 @@ def test_batch_fitch_matches_cumulative_null(ctx, N, L, nrep):
     fit_g, h_g = NL.batch_fitch(ctx, ax, tree, seeds, "GTp", pmass=0.01, graph=2)
     assert np.array_equal(fit_g, fit_b)
     assert np.array_equal(h_g.h.counts(), h_b.h.counts())
+
+
+def _pairing(L, npairs, seed):
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(L)[:2 * npairs]
+    ct = -np.ones(L, np.int32)
+    for k in range(npairs):
+        i, j = int(perm[2 * k]), int(perm[2 * k + 1])
+        ct[i], ct[j] = j, i
+    return ct
+
+
+@pytest.mark.parametrize("N,L,npairs,gap", [(2, 9, 3, 0.0), (37, 64, 20, 0.0), (300, 131, 40, 0.1)])
+def test_pair_simulation_matches_oracle(ctx, N, L, npairs, gap):
+    """Structured simulation (16-state pairs): symbol-identical to the oracle on the same keys."""
+    from test_pair_model import pair_params
+    print("pair sim N=%d L=%d pairs=%d" % (N, L, npairs), flush=True)
+    m = NL.gtr_model(PI)
+    Q, M = pair_params(N)
+    m16 = NL.pair_model(Q, M)
+    tree = NL.yule_tree(N, 0.15, seed=N)
+    ct = _pairing(L, npairs, N)
+    job = NL.Job(tree, L, m, seed=99 + N, stream=3, gapfrac=gap)
+    got = NL.simulate_pairs(ctx, job, ct, m16)
+    gapthr = 0 if gap <= 0 else int(gap * 2 ** 32)
+    ref = oracle.evolve_pairs(tree, L, m, m16, ct, 99 + N, 3, gapthr)
+    assert np.mean(got != ref) < 1e-5
+    assert got.max() <= 4
+
+
+def test_pair_simulation_distribution(ctx):
+    """Long branches: leaf pair states of paired columns follow the stationary pair frequencies
+    (so base pairs covary: WC/GU fraction = its weight in M16), unpaired columns follow pi."""
+    from test_pair_model import pair_params, WC
+    m = NL.gtr_model(PI)
+    Q, M = pair_params(7, wc=0.85)
+    m16 = NL.pair_model(Q, M)
+    tree = NL.yule_tree(300, 1.0, seed=4)
+    L, npairs = 3000, 1000
+    ct = _pairing(L, npairs, 5)
+    leaves = NL.simulate_pairs(ctx, NL.Job(tree, L, m, seed=21), ct, m16)
+    i5 = np.array([i for i in range(L) if ct[i] > i]); i3 = ct[i5]
+    st = 4 * leaves[:, i5].astype(int) + leaves[:, i3]
+    freq = np.bincount(st.ravel(), minlength=16) / st.size
+    assert np.allclose(freq, M, atol=0.01)
+    wc = sum(M[a * 4 + b] for a, b in WC)
+    assert abs(np.isin(st, [a * 4 + b for a, b in WC]).mean() - wc) < 0.01
+    un = leaves[:, ct < 0]
+    assert np.allclose(np.bincount(un.ravel(), minlength=4)[:4] / un.size, PI, atol=0.01)
