@@ -64,7 +64,7 @@ def cpu_baseline(ax, Lwin):
     try:
         import oracle
         oracle.lib()
-    except Exception as e:  # the oracle library is optional on the GPU box
+    except Exception as e:  # the oracle library is optional where the benchmark runs
         return {"value": None, "error": str(e)}
     sub = np.ascontiguousarray(ax[:, :Lwin])
     N = sub.shape[0]
@@ -289,8 +289,8 @@ def main():
     ap.add_argument("--evalue-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the design for sharding is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(a sharded design is described with the library). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     d = synth.config(1, seed=42, N=args.N, L=args.L)
