@@ -258,3 +258,26 @@ def test_tutorial_golden_gtp(ctx):
     pos = {c + 1: k for k, c in enumerate(cols)}
     for i, j, sc in gold["pairs"]:
         assert abs(got[pos[i], pos[j]] - sc) < 0.05, (i, j, got[pos[i], pos[j]], sc)
+
+
+@pytest.mark.parametrize("N,L", [(40, 2), (40, 33), (150, 65), (64, 97), (300, 161)])
+def test_marginals_and_gtp_edge_columns(ctx, N, L):
+    """Marginals from the Gram's 32 x 32 block partials at sizes whose last block holds one
+    column (L = 32 b + 1) or two, with an all-gap column and an all-non-canonical column (their
+    pair sums are 0: pm falls back to 1/4 as esl_vec_DNorm does), then the whole GTp scoring."""
+    print("edge columns N=%d L=%d" % (N, L), flush=True)
+    ax = _msa(N, L, 100 + L)
+    ax[:, 0] = 4                                  # all gaps
+    ax[:, L - 1] = 15                             # all non-canonical
+    m = ctx.msa(ax)
+    m.probs(R.COUNTS_INT8)
+    g = m.get_probs()
+    pr = oracle.probs(ax)
+    assert np.allclose(g["pm"], pr["pm"], rtol=1e-12, atol=1e-15)
+    assert np.allclose(g["pm"].reshape(L, 4)[0], 0.25) and np.allclose(g["pm"].reshape(L, 4)[L - 1], 0.25)
+    mn, mx = m.calculate("GTp")
+    got, _, _ = m.get_cov()
+    ref, _ = oracle.cov(pr, oracle.GT, R.CSELECT)
+    ref2, mm = oracle.corrected(ref, L, R.APC)
+    assert _relerr(_upper(got, L), _upper(ref2, L)) < 1e-10
+    assert np.allclose([mn, mx], mm, rtol=1e-10, atol=1e-12)
