@@ -44,7 +44,7 @@ k_pack_b(const int64_t *__restrict__ desc, int n, const uint8_t *__restrict__ le
   __shared__ uint8_t tile[64][65];
   const int it = find_item(desc, n, D_BPACK, blockIdx.x);
   const int64_t *d = desc + it * RSG_BDESC;
-  const int64_t N = d[D_N], L = d[D_L], Npad = d[D_NPAD], Lpad = d[D_LPAD];
+  const int64_t N = d[D_N], L = d[D_L], Lpad = d[D_LPAD];
   const int64_t local = blockIdx.x - d[D_BPACK], nbx = Lpad / 64;
   const int64_t c0 = (local % nbx) * 64, s0 = (local / nbx) * 64;
   const uint8_t *ax = leaves + d[D_LEAF];
