@@ -197,17 +197,28 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
   stage_lds_direct<St, BI, BJ, NW>(xt, Lpad, i0, j0, stage_of(1), lds + St::BYTES, wave, lane);
   __builtin_amdgcn_s_waitcnt(0x0F70 | QW);     // vmcnt(QW): this wave's loads of stage 0 landed
   __syncthreads();                             // ... and every wave's
+  // The wait for stage it + 1 and the barrier sit before the last k-step of stage it: every read
+  // of image it % 3 has completed there (the barrier waits for LDS reads), so the loads issued at
+  // the next iteration's start may overwrite it, and the first fragments of stage it + 1 are read
+  // right after the barrier while the last MFMAs of stage it run.
+  static_assert(KS >= 2, "the barrier precedes the last k-step");
+  uint4 ra = *(const uint4 *)(lds + offA);
+  uint4 rb = *(const uint4 *)(lds + offB);
   for (int it = 0; it < nit; it++) {
     stage_lds_direct<St, BI, BJ, NW>(xt, Lpad, i0, j0, stage_of(it + 2), lds + ((it + 2) % 3) * St::BYTES, wave, lane);
     const uint8_t *base = lds + (it % 3) * St::BYTES;
-    uint4 ra = *(const uint4 *)(base + offA);
-    uint4 rb = *(const uint4 *)(base + offB);
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) {
       uint4 na = ra, nb = rb;
       if (ks + 1 < KS) {
         na = *(const uint4 *)(base + offA + (ks + 1) * 2 * St::CSA);
         nb = *(const uint4 *)(base + offB + (ks + 1) * 2 * St::CSB);
+      } else {
+        __builtin_amdgcn_s_waitcnt(0x0F70 | QW); // vmcnt(QW): stage it + 1 landed
+        __syncthreads();
+        const uint8_t *nbase = lds + ((it + 1) % 3) * St::BYTES;
+        na = *(const uint4 *)(nbase + offA);
+        nb = *(const uint4 *)(nbase + offB);
       }
       v4i a[4], b[2];
 #pragma unroll
@@ -227,8 +238,6 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
           acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[y], b[x], acc[x][y], 0, 0, 0);
       ra = na; rb = nb;
     }
-    __builtin_amdgcn_s_waitcnt(0x0F70 | QW);   // vmcnt(QW): stage it + 1 landed
-    __syncthreads();
   }
   __builtin_amdgcn_s_waitcnt(0x0F70);          // no LDS-direct load outlives the loop
 
