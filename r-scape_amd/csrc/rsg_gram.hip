@@ -731,9 +731,10 @@ static int pick_shape(const rsg_ctx *c, const rsg_msa *m)
 {
   int s = c->gram_shape;
   if (s >= 1 && s <= kNShapes) return s;
-  // autotuned on MI355X (N=5000, L=1500: 64x64 columns, K-tile 256); small alignments take
-  // one-wave 32x32 tiles so the grid still fills the chip
-  return (m->L >= 768) ? 3 : 4;
+  // autotuned on MI355X: 64 x 32-column tiles of four waves with 128-sequence K tiles win or
+  // tie from N = 200 to 20000 and L = 120 to 5000 (two work groups per CU, so one's epilogue runs
+  // under the other's K loop)
+  return 5;
 }
 
 int rsg_gram_counts_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, double *pm_part, int *pmflag, char *errbuf)
