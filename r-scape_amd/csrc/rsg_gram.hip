@@ -689,13 +689,14 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, double *pm_part, int *pm
   if (!tiles) return RSG_FAIL;
   if (ntiles == 0) return RSG_OK;
   if (OK == CNT_U16 && m->N > RSG_U16_MAXN) return rsg_fail(errbuf, "gram: uint16 records hold up to %d sequences", RSG_U16_MAXN);
-  // whole waves of tiles run over the full K range; the last partial wave is split along K
-  // into enough segments to occupy every CU, then reduced
+  // whole waves of tiles run over the full K range; when there are few waves (under four
+  // tiles per CU) the last partial wave is split along K into enough segments to occupy every
+  // CU, then reduced (with many waves the tail is small and the partial traffic costs more)
   const int ncu = c->ncu > 0 ? c->ncu : 256;
   const int64_t nkt = m->Npad / KT;
   int ndp = ntiles, sbase = 1, sextra = 0;
   const int rem = ntiles % ncu;
-  if (c->gram_split && rem > 0 && rem <= ncu / 2 && nkt >= 2) {
+  if (c->gram_split && ntiles < 4 * ncu && rem > 0 && rem <= ncu / 2 && nkt >= 2) {
     // every CU takes one K segment of the last tiles before the full tiles start
     ndp = ntiles - rem;
     const int nsu = (int)std::min<int64_t>((int64_t)ncu, (int64_t)rem * nkt);
