@@ -651,6 +651,9 @@ static const GramShape kShapes[] = {
   {1, 1, 128, "32x32c/2w/kt128"},
   {2, 1, 128, "64x32c/4w/kt128"},
   {1, 2, 128, "32x64c/4w/kt128"},
+  {2, 1, 256, "64x32c/4w/kt256"},
+  {1, 2, 256, "32x64c/4w/kt256"},
+  {1, 1, 256, "32x32c/2w/kt256"},
 };
 static const int kNShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
 
@@ -732,10 +735,11 @@ static int pick_shape(const rsg_ctx *c, const rsg_msa *m)
 {
   int s = c->gram_shape;
   if (s >= 1 && s <= kNShapes) return s;
-  // autotuned on MI355X: 64 x 32-column tiles of four waves with 128-sequence K tiles win or
-  // tie from N = 200 to 20000 and L = 120 to 5000 (two work groups per CU, so one's epilogue runs
-  // under the other's K loop)
-  return 5;
+  // autotuned on MI355X: 64 x 32-column tiles of four waves with 256-sequence K tiles win or tie
+  // from N = 2000 to 20000 and L = 500 to 5000 (two work groups per CU, so one's epilogue runs
+  // under the other's K loop); below ~1000 sequences (two or three K tiles) 32 x 32 tiles of two
+  // waves are as fast or faster
+  return (m->N < 1024) ? 4 : 7;
 }
 
 int rsg_gram_counts_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, double *pm_part, int *pmflag, char *errbuf)
@@ -751,6 +755,9 @@ int rsg_gram_counts_i8(rsg_ctx *c, rsg_msa *m, void *out, int kind, double *pm_p
   case 4: return launch_i8<1, 1, 128, CNT_U16>(c, m, out, pm_part, pmflag, errbuf);
   case 5: return launch_i8<2, 1, 128, CNT_U16>(c, m, out, pm_part, pmflag, errbuf);
   case 6: return launch_i8<1, 2, 128, CNT_U16>(c, m, out, pm_part, pmflag, errbuf);
+  case 7: return launch_i8<2, 1, 256, CNT_U16>(c, m, out, pm_part, pmflag, errbuf);
+  case 8: return launch_i8<1, 2, 256, CNT_U16>(c, m, out, pm_part, pmflag, errbuf);
+  case 9: return launch_i8<1, 1, 256, CNT_U16>(c, m, out, pm_part, pmflag, errbuf);
   }
   return rsg_fail(errbuf, "bad gram shape");
 }

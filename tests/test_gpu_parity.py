@@ -47,7 +47,7 @@ def test_int8_counts_bit_exact(ctx, N, L):
     assert np.array_equal(got[iu], ref[iu].astype(float))
 
 
-@pytest.mark.parametrize("shape", range(1, 7))
+@pytest.mark.parametrize("shape", range(1, 10))
 def test_gram_shapes_bit_exact(ctx, shape):
     print("gram shape %d (%s)" % (shape, R.lib().rsg_gram_shape_name(shape).decode()), flush=True)
     N, L = 300, 203
