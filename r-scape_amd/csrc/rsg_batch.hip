@@ -323,7 +323,7 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
   int64_t o_xt = 0, o_cnt = 0, o_cov = 0, o_pm = 0, o_rows = 0, o_leaf = 0, nodes = 0, seqs = 0;
   int64_t b_pack = 0, b_col = 0, b_stat = 0, b_el = 0, b_sim = 0, b_hist = 0, nmax = 1;
   int64_t b_col16 = 0, o_pmp = 0;
-  std::vector<int4> tiles, btiles;        // 32-column tiles (small alignments), 64-column tiles (large)
+  std::vector<int4> tiles, btiles;        // 32 x 32-column tiles (small alignments), 64 x 32 (large)
   for (int k = 0; k < nj; k++) {
     const rsg_null_job &J = jobs[k];
     if (J.nleaf < 2 || J.alen < 2) return rsg_fail(errbuf, "job %d: needs >= 2 sequences and >= 2 columns", k);
@@ -344,9 +344,10 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
     s.gapthr = (J.gapfrac <= 0.) ? 0u : (J.gapfrac >= 1. ? 0xFFFFFFFFu : (uint32_t)(J.gapfrac * 4294967296.0));
     s.col_blk0 = b_sim; s.model_off = 40 * (int64_t)k;
     if (rsg_batch_big_tiles(N, L)) {
-      const int64_t nb64 = Lpad / 64;
-      for (int64_t bi = 0; bi < nb64; bi++)
-        for (int64_t bj = bi; bj < nb64; bj++) btiles.push_back(make_int4(k, (int)bi, (int)bj, 0));
+      // 64 (i) x 32 (j) columns: every tile holding a pair i < j < L
+      for (int64_t bi = 0; bi < Lpad / 64; bi++)
+        for (int64_t bj = 0; bj < nb32; bj++)
+          if (bi * 64 < std::min((bj + 1) * 32, L) - 1) btiles.push_back(make_int4(k, (int)bi, (int)bj, 0));
     } else {
       for (int64_t bi = 0; bi < nb32; bi++)
         for (int64_t bj = bi; bj < nb32; bj++) tiles.push_back(make_int4(k, (int)bi, (int)bj, 0));

@@ -490,29 +490,29 @@ k_gram_fix(const uint8_t *__restrict__ part, const int2 *__restrict__ tiles, int
 // pre-sorted by decreasing sequence count (cost).  Large alignments: 64 x 64 columns, 4
 // waves per half, 256-sequence K tiles; small ones: 32 x 32 columns, one wave per half,
 // 64-sequence K tiles.
-template <int W, int KT>
-__global__ void __launch_bounds__(128 * W * W) __attribute__((amdgpu_waves_per_eu(2)))
+template <int WI, int WJ, int KT>
+__global__ void __launch_bounds__(128 * WI * WJ) __attribute__((amdgpu_waves_per_eu(2)))
 k_gram_i8_batch(const int4 *__restrict__ tiles, const int64_t *__restrict__ desc, const uint8_t *__restrict__ xt,
                 uint16_t *__restrict__ cnt, double *__restrict__ pm_part, int pm_field)
 {
-  typedef CodeStage<32 * W, 32 * W, KT, 128 * W * W> St;
+  typedef CodeStage<32 * WI, 32 * WJ, KT, 128 * WI * WJ> St;
   __shared__ __attribute__((aligned(16))) uint8_t lds[St::lds_bytes(true)];
   const int4 tl = tiles[blockIdx.x];                    // (item, bi, bj, -)
   const int64_t *d = desc + (int64_t)tl.x * RSG_BDESC;  // item descriptor (see rsg_batch.hip)
   const int64_t nb32 = (d[1] + 31) / 32;
   double *pmi = pm_part + d[pm_field];
-  gram_tile_i8<W, W, KT, CNT_U16>(xt + d[4], d[3], d[1], cnt + d[5], (int64_t)tl.y * 32 * W, (int64_t)tl.z * 32 * W, lds,
-                         0, d[2] / KT, nullptr, pmi, pmi + pm32_n(nb32), nb32);
+  gram_tile_i8<WI, WJ, KT, CNT_U16>(xt + d[4], d[3], d[1], cnt + d[5], (int64_t)tl.y * 32 * WI, (int64_t)tl.z * 32 * WJ,
+                                    lds, 0, d[2] / KT, nullptr, pmi, pmi + pm32_n(nb32), nb32);
 }
 
 int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_big, int nbig, const int4 *d_small, int nsmall,
                           const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt, double *d_pm_part, int pm_field)
 {
   if (nbig > 0)
-    hipLaunchKernelGGL((k_gram_i8_batch<2, 256>), dim3((unsigned)nbig), dim3(512), 0, c->stream, d_big, d_desc, d_xt, d_cnt,
+    hipLaunchKernelGGL((k_gram_i8_batch<2, 1, 256>), dim3((unsigned)nbig), dim3(256), 0, c->stream, d_big, d_desc, d_xt, d_cnt,
                        d_pm_part, pm_field);
   if (nsmall > 0)
-    hipLaunchKernelGGL((k_gram_i8_batch<1, 64>), dim3((unsigned)nsmall), dim3(128), 0, c->stream, d_small, d_desc, d_xt, d_cnt,
+    hipLaunchKernelGGL((k_gram_i8_batch<1, 1, 64>), dim3((unsigned)nsmall), dim3(128), 0, c->stream, d_small, d_desc, d_xt, d_cnt,
                        d_pm_part, pm_field);
   return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
 }

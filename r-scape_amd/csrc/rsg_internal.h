@@ -229,8 +229,8 @@ int rsg_feat_launch(rsg_ctx *c, const int64_t *d_items, int nitems, int64_t nblo
                     unsigned long long *d_out);
 void rsg_feat_finish(const unsigned long long *cnt, int64_t N, int64_t L, const int *left, const int *right,
                      const double *ld, const double *rd, float *f);
-// Large alignments of a batch use 64 x 64-column tiles (4 waves, 256-sequence K tiles), the
-// others 32 x 32-column tiles (one wave, 64-sequence K tiles).
+// Large alignments of a batch use 64 x 32-column tiles (4 waves, 256-sequence K tiles), the
+// others 32 x 32-column tiles (2 waves, 64-sequence K tiles).
 static inline bool rsg_batch_big_tiles(int64_t N, int64_t L) { return L >= 256 && N >= 256; }
 int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_big, int nbig, const int4 *d_small, int nsmall,
                           const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt, double *d_pm_part, int pm_field);
