@@ -144,6 +144,28 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
     return (double)v;
   }
   // C16 / CWC
+  if (TAB && type == RSG_GT && cls == RSG_C16) {
+    // GT over all 16 cells, regrouped: pm_i[x] pm_j[y] > 0 for every cell (the marginals carry
+    // the 1e-10 pseudo-counts of every pair, or are uniform), so
+    //   sum_xy pp (ln pp - ln pm_i[x] - ln pm_j[y]) = rden sum_xy (c + 1e-10) ln(c + 1e-10)
+    //     - ln(den) rden (ns + 16e-10) - rden sum_x ln pm_i[x] (r_x + 4e-10)
+    //     - rden sum_y ln pm_j[y] (s_y + 4e-10)
+    // with r_x, s_y the row and column sums of the counts
+    T a = 0, r[4] = {0, 0, 0, 0}, cs[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        const int k = x * 4 + y;
+        a += ((T)cd[k] + (T)1e-10) * (T)tabc[(int)cd[k]];
+        r[x] += (T)cd[k]; cs[y] += (T)cd[k];
+      }
+    T b = 0;
+#pragma unroll
+    for (int x = 0; x < 4; x++) b += lpi[x] * (r[x] + (T)4e-10) + lpj[x] * (cs[x] + (T)4e-10);
+    const T s = rden * (a - b) - lden * (rden * den);
+    return (ns > 0) ? 2. * (double)ns * (double)s : 0.;
+  }
   T s = 0, H = 0;
 #pragma unroll
   for (int x = 0; x < 4; x++)
