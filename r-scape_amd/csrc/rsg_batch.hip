@@ -272,10 +272,14 @@ k_fit_b(const HGeo *__restrict__ geo, const unsigned long long *__restrict__ his
   if (threadIdx.x == 0) { o[5] = G.w; o[6] = G.mn; o[7] = G.mx; }
 }
 
+// the single-alignment table (k_logtab) for counts up to NM
 __global__ void k_logtab_b(int64_t NM, double *__restrict__ t)
 {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k <= NM) { t[k] = log((double)k + 1e-10); t[NM + 1 + k] = log((double)k + 16e-10); }
+  if (k <= NM) {
+    const double l = log((double)k + 1e-10);
+    t[k] = l; t[NM + 1 + k] = log((double)k + 16e-10); t[2 * (NM + 1) + k] = ((double)k + 1e-10) * l;
+  }
 }
 
 // ------------------------------------------------------------------------------- host side
@@ -421,7 +425,7 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
     d_cdf = A.get<uint32_t>((size_t)nodes * 24); d_seqs = A.get<uint8_t>((size_t)seqs);
     d_leaves = A.get<uint8_t>((size_t)o_leaf); d_xt = A.get<uint8_t>((size_t)o_xt);
     d_cnt = A.get<uint16_t>((size_t)o_cnt); d_cov = A.get<double>((size_t)o_cov); d_pm = A.get<double>((size_t)o_pm);
-    d_rows = A.get<double>((size_t)o_rows); d_lgc = A.get<double>(2 * (size_t)(nmax + 1));
+    d_rows = A.get<double>((size_t)o_rows); d_lgc = A.get<double>(3 * (size_t)(nmax + 1));
     d_part_s = A.get<double>(2 * (size_t)b_stat); d_part_e = A.get<double>(2 * (size_t)b_el);
     d_mm = A.get<double>(2 * (size_t)nj); d_pmp = A.get<double>((size_t)o_pmp); d_geo = A.get<HGeo>((size_t)ng);
     d_hist = A.get<unsigned long long>((size_t)ng * NBMAX); d_S = A.get<unsigned long long>((size_t)ng * NBMAX);

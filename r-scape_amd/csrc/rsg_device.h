@@ -67,7 +67,8 @@ template <> __device__ __forceinline__ float tlog<float>(float v) { return logf(
 
 // One pair. c: counts, pmi/pmj: marginals.  Returns the statistic (see the file header).
 // TYPE / CLS >= 0 fix the statistic and class at compile time (-1: read them from A).
-// tabc: ln(c + 1e-10) for counts c = 0..N (A.lgc, or a copy of it in LDS) when TAB.
+// tabc: ln(c + 1e-10) for counts c = 0..N (A.lgc, or a copy of it in LDS) when TAB; A.lgc holds
+// ln(c + 1e-10), ln(c + 16e-10) and (c + 1e-10) ln(c + 1e-10) for c = 0..N in three runs.
 template <typename T, bool TAB, int TYPE = -1, int CLS = -1>
 __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int64_t j, const double *tabc)
 {
@@ -157,7 +158,7 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
 #pragma unroll
       for (int y = 0; y < 4; y++) {
         const int k = x * 4 + y;
-        a += ((T)cd[k] + (T)1e-10) * (T)tabc[(int)cd[k]];
+        a += (T)A.lgc[2 * (A.N + 1) + (int64_t)cd[k]];      // (c + 1e-10) ln(c + 1e-10)
         r[x] += (T)cd[k]; cs[y] += (T)cd[k];
       }
     T b = 0;

@@ -133,11 +133,14 @@ __global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, con
   }
 }
 
-// ln(k + 1e-10) and ln(k + 16e-10) for integer k in [0, N]
+// ln(k + 1e-10), ln(k + 16e-10) and (k + 1e-10) ln(k + 1e-10) for integer k in [0, N]
 __global__ void k_logtab(int64_t N, double *__restrict__ t)
 {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k <= N) { t[k] = log((double)k + 1e-10); t[N + 1 + k] = log((double)k + 16e-10); }
+  if (k <= N) {
+    const double l = log((double)k + 1e-10);
+    t[k] = l; t[N + 1 + k] = log((double)k + 16e-10); t[2 * (N + 1) + k] = ((double)k + 1e-10) * l;
+  }
 }
 
 int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
@@ -182,7 +185,7 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   if (!m->d_lgc) {
-    HIPCHK(hipMalloc(&m->d_lgc, sizeof(double) * 2 * (size_t)(m->N + 1)));
+    HIPCHK(hipMalloc(&m->d_lgc, sizeof(double) * 3 * (size_t)(m->N + 1)));
     hipLaunchKernelGGL(k_logtab, dim3((unsigned)((m->N + 256) / 256)), dim3(256), 0, c->stream, m->N, m->d_lgc);
     HIPCHK(hipGetLastError());
   }
