@@ -145,9 +145,9 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
     return (double)v;
   }
   // C16 / CWC
-  if (TAB && type == RSG_GT && cls == RSG_C16) {
-    // GT over all 16 cells, regrouped: pm_i[x] pm_j[y] > 0 for every cell (the marginals carry
-    // the 1e-10 pseudo-counts of every pair, or are uniform), so
+  if (TAB && cls == RSG_C16 && (type == RSG_GT || type == RSG_MI || type == RSG_MIr || type == RSG_MIg)) {
+    // GT and the MI family over all 16 cells, regrouped: pm_i[x] pm_j[y] > 0 for every cell (the
+    // marginals carry the 1e-10 pseudo-counts of every pair, or are uniform), so
     //   sum_xy pp (ln pp - ln pm_i[x] - ln pm_j[y]) = rden sum_xy (c + 1e-10) ln(c + 1e-10)
     //     - ln(den) rden (ns + 16e-10) - rden sum_x ln pm_i[x] (r_x + 4e-10)
     //     - rden sum_y ln pm_j[y] (s_y + 4e-10)
@@ -164,8 +164,16 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
     T b = 0;
 #pragma unroll
     for (int x = 0; x < 4; x++) b += lpi[x] * (r[x] + (T)4e-10) + lpj[x] * (cs[x] + (T)4e-10);
-    const T s = rden * (a - b) - lden * (rden * den);
-    return (ns > 0) ? 2. * (double)ns * (double)s : 0.;
+    const T s = rden * (a - b) - lden * (rden * den);               // MI
+    switch (type) {
+    case RSG_GT:  return (ns > 0) ? 2. * (double)ns * (double)s : 0.;
+    case RSG_MI:  return (double)s;
+    case RSG_MIg: return (double)s - ((nsd > 0) ? (A.wsum - nsd) / nsd : 0.);
+    default: {                                                       // MIr: MI / H, H = -sum pp ln pp
+      const T H = -(rden * a - lden * (rden * den));
+      return (H > (T)1e-2) ? (double)(s / H) : 0.;
+    }
+    }
   }
   T s = 0, H = 0;
 #pragma unroll
