@@ -244,3 +244,47 @@ def test_pair_simulation_distribution(ctx):
     assert abs(np.isin(st, [a * 4 + b for a, b in WC]).mean() - wc) < 0.01
     un = leaves[:, ct < 0]
     assert np.allclose(np.bincount(un.ravel(), minlength=4)[:4] / un.size, PI, atol=0.01)
+
+
+def _device_gt(ctx, ax):
+    msa = ctx.msa(ax)
+    try:
+        msa.probs(R.COUNTS_INT8)
+        msa.cov("GT", R.C16)
+        return msa.get_cov()[0]
+    finally:
+        msa.close()
+
+
+@pytest.mark.parametrize("structured", [False, True])
+def test_simulated_null_scores_ks(ctx, structured):
+    """Device-simulated nulls validated by distribution: GT scores of the disjoint column pairs
+    (0,1), (2,3), ... of 100 device-simulated alignments on one tree (independent draws: columns
+    evolve independently without gaps), scored on the device, against the textbook null
+    (scipy expm, numpy's generator) scored by the oracle -- two-sample KS; the same test
+    rejects a textbook null with every branch 1.25x too long."""
+    from scipy.stats import ks_2samp
+    from rscape_amd import synth
+    from test_pair_model import pair_params
+    import textbook_null as TB
+    N, L, Rn = 40, 64, 100
+    print("KS null structured=%d" % structured, flush=True)
+    tree = synth.yule_tree(N, np.random.default_rng(8), mean_bl=0.1)
+    m = NL.gtr_model(PI)
+    Q, M = pair_params(4, wc=0.8)
+    m16, ct = NL.pair_model(Q, M), TB.pairing(L)
+    got = []
+    for r in range(Rn):
+        job = NL.Job(tree, L, m, seed=505, stream=r)
+        ax = NL.simulate_pairs(ctx, job, ct, m16) if structured else NL.simulate(ctx, job)
+        got.append(TB.disjoint_pair_gt(_device_gt(ctx, ax), L))
+    got = np.concatenate(got)
+
+    def textbook(seed, scale=1.0):
+        rng = np.random.default_rng(seed)
+        if structured:
+            return np.concatenate([TB.oracle_gt(TB.evolve_pairs16(tree, L // 2, Q, M, rng, scale)) for _ in range(Rn)])
+        return np.concatenate([TB.oracle_gt(TB.evolve_gtr(tree, L, PI, rng, scale)) for _ in range(Rn)])
+
+    assert ks_2samp(got, textbook(41)).pvalue > 1e-3
+    assert ks_2samp(got, textbook(42, scale=1.25)).pvalue < 1e-3
