@@ -29,6 +29,7 @@ extern "C" rsg_ctx *rsg_ctx_create(int device, char *errbuf)
     delete c;
     return nullptr;
   }
+  memset(c->h_pinned, 0, 64 * sizeof(double));          // completion stamps start from 1
   return c;
 }
 
@@ -303,6 +304,21 @@ extern "C" int rsg_corr_GetCOV(rsg_ctx *c, rsg_msa *m, double *COV, double *minC
   return RSG_OK;
 }
 
+// Wait for the completion stamp of the chain's last kernel in pinned memory (the result block is
+// complete once it shows); true when it was seen.  The stream is polled between spins, so a chain
+// that ends without writing the stamp (a failed launch) falls through to the caller's stream
+// synchronisation, which reports the failure.
+static bool wait_stamp(rsg_ctx *c)
+{
+  const unsigned long long want = c->hstamp;
+  unsigned long long *slot = (unsigned long long *)(c->h_pinned + 63);
+  for (unsigned spin = 1;; spin++) {
+    if (__atomic_load_n(slot, __ATOMIC_ACQUIRE) == want) return true;
+    if ((spin & 255) == 0 && hipStreamQuery(c->stream) != hipErrorNotReady)
+      return __atomic_load_n(slot, __ATOMIC_ACQUIRE) == want;
+  }
+}
+
 extern "C" int rsg_cov_Calculate(rsg_ctx *c, rsg_msa *m, int covtype, int covclass, int counts_mode, int prec,
                                  const double *allowpair, int nseqthresh, int alenthresh, double tol,
                                  double *minCOV, double *maxCOV, char *errbuf)
@@ -324,8 +340,10 @@ extern "C" int rsg_cov_Calculate(rsg_ctx *c, rsg_msa *m, int covtype, int covcla
   if (st == RSG_OK && !c->res_on_host &&
       hipMemcpyAsync(c->h_pinned, c->d_res, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
     st = rsg_fail(errbuf, "rsg_cov_Calculate: result copy failed");
+  const bool stamped = (st == RSG_OK && c->res_on_host);
   c->res_on_host = 0;
-  if (hipStreamSynchronize(c->stream) != hipSuccess && st == RSG_OK) st = rsg_fail(errbuf, "rsg_cov_Calculate: device failure");
+  if (!(stamped && wait_stamp(c)) && hipStreamSynchronize(c->stream) != hipSuccess && st == RSG_OK)
+    st = rsg_fail(errbuf, "rsg_cov_Calculate: device failure");
   if (st != RSG_OK) { m->have_probs = m->have_cov = 0; return st; }
   if (base != RSG_RAF && base != RSG_RAFS && *(int *)(c->h_pinned + 4)) {
     m->have_probs = m->have_cov = 0;

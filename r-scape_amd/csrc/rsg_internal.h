@@ -86,6 +86,9 @@ struct rsg_ctx {
   double      *h_pinned = nullptr;      // 64 doubles of pinned host staging
   double      *d_hres = nullptr;        // h_pinned as the device addresses it
   int          res_on_host = 0;
+  // completion stamp: the min / max reduction writes hstamp to h_pinned[63] after the result
+  // block (system-scope release), so the caller can wait for it without a stream synchronisation
+  unsigned long long hstamp = 0;
   void        *d_small = nullptr;       // 64 KB of device scratch for small per-call tables
 };
 

@@ -216,10 +216,10 @@ static int launch_stat_fixed(rsg_ctx *c, const StatArgs &A, int cls, int nb)
 
 // (min, max) of n block partials into out[0..1]; one block of 1024 threads, four partials per
 // thread loaded at once.  With hout != nullptr out[0..4] (min, max and the flags and COVavg the
-// chain left in out[2..4]) also go to hout, host memory, which the caller reads after its
-// stream synchronisation.
+// chain left in out[2..4]) also go to hout, host memory, followed (after a system-scope
+// release) by the completion stamp in hout[63], which the caller waits for.
 __global__ void __launch_bounds__(1024) k_reduce_minmax(const double *__restrict__ part, int n, double *__restrict__ out,
-                                                        double *__restrict__ hout)
+                                                        double *__restrict__ hout, unsigned long long stamp)
 {
   __shared__ double rmin[1024], rmax[1024];
   double a = INFINITY, b = -INFINITY;
@@ -244,6 +244,8 @@ __global__ void __launch_bounds__(1024) k_reduce_minmax(const double *__restrict
     if (hout) {
       hout[0] = rmin[0]; hout[1] = rmax[0];
       hout[2] = out[2]; hout[3] = out[3]; hout[4] = out[4];
+      __threadfence_system();
+      *(volatile unsigned long long *)(hout + 63) = stamp;
     }
   }
 }
@@ -265,7 +267,7 @@ static int fetch_minmax(rsg_ctx *c, rsg_msa *m, int nparts, char *errbuf)
   if (c->skip_minmax) return RSG_OK;            // a correction recomputes them
   // a deferred chain ends here: the reduction also writes the result block to host memory
   hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(1024), 0, c->stream, c->d_red, nparts, c->d_res,
-                     c->defer ? c->d_hres : nullptr);
+                     c->defer ? c->d_hres : nullptr, ++c->hstamp);
   HIPCHK(hipGetLastError());
   if (c->defer) { c->res_on_host = 1; return RSG_OK; }
   HIPCHK(hipMemcpyAsync(c->h_pinned, c->d_res, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -534,7 +536,8 @@ int rsg_correct(rsg_ctx *c, rsg_msa *m, int actype, int shiftnonneg, char *errbu
                      c->d_res + 3);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(1024), 0, c->stream, c->d_red, nblk, c->d_res, direct ? c->d_hres : nullptr);
+  hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(1024), 0, c->stream, c->d_red, nblk, c->d_res, direct ? c->d_hres : nullptr,
+                     ++c->hstamp);
   if (direct) { c->res_on_host = 1; return RSG_OK; }
   if (shiftnonneg)
     hipLaunchKernelGGL(k_shift, dim3((unsigned)((L * L + 255) / 256)), dim3(256), 0, c->stream, m->d_cov, L, c->d_res);
