@@ -40,14 +40,6 @@ __global__ void __launch_bounds__(64 * PMW) k_pm_tile(const void *__restrict__ c
   pm_tile<KIND>(cnt, L, nbt, bi, bj, sub, part_i, part_j);
 }
 
-// marginals from the 32 x 32 block partials left by the int8 Gram (16 lanes per column)
-__global__ void __launch_bounds__(256) k_pm32_reduce(const double *__restrict__ part, int nb32, int64_t L, double tol,
-                                                     double *__restrict__ pm, double *__restrict__ lpm, int *__restrict__ flag)
-{
-  const int64_t col = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
-  if (col < L) pm32_reduce_col(part, part + pm32_n(nb32), nb32, L, col, threadIdx.x & 15, tol, pm, lpm, flag);
-}
-
 // Scratch for the marginal partials the int8 Gram leaves (2 pm32_n doubles), in d_red.
 double *rsg_pm32_scratch(rsg_ctx *c, int64_t L, char *errbuf)
 {
@@ -76,11 +68,11 @@ __device__ __forceinline__ unsigned bytes_eq(unsigned w, unsigned pat)
 // integer counts: four loads in flight per lane, the padding codes (4) of the last load taken
 // off the gap count).
 template <bool W>
-__global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, const double *__restrict__ w,
-                                            int64_t N, int64_t Npad, int64_t L, double *__restrict__ ps)
+__device__ __forceinline__ void ps_cols(const uint8_t *__restrict__ axT, const double *__restrict__ w, int64_t N,
+                                        int64_t Npad, int64_t L, double *__restrict__ ps, int64_t blk)
 {
   const int lane = threadIdx.x & 63;
-  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t i = blk * 4 + (threadIdx.x >> 6);
   if (i >= L) return;
   const uint4 *col = (const uint4 *)(axT + i * Npad);
   if (!W) {
@@ -133,6 +125,29 @@ __global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, con
   }
 }
 
+template <bool W>
+__global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, const double *__restrict__ w,
+                                            int64_t N, int64_t Npad, int64_t L, double *__restrict__ ps)
+{
+  ps_cols<W>(axT, w, N, Npad, L, ps, blockIdx.x);
+}
+
+// Unit weights after the int8 Gram, in one launch: the marginals from the Gram's 32 x 32 block
+// partials (blocks [0, nbm), 16 lanes per column) and the single-residue frequencies (the rest,
+// as k_ps<false>).
+__global__ void __launch_bounds__(256) k_pm32_ps(const double *__restrict__ part, int nb32, int64_t L, double tol,
+                                                 double *__restrict__ pm, double *__restrict__ lpm, int *__restrict__ flag,
+                                                 const uint8_t *__restrict__ axT, int64_t N, int64_t Npad,
+                                                 double *__restrict__ ps, int nbm)
+{
+  if ((int)blockIdx.x < nbm) {
+    const int64_t col = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    if (col < L) pm32_reduce_col(part, part + pm32_n(nb32), nb32, L, col, threadIdx.x & 15, tol, pm, lpm, flag);
+    return;
+  }
+  ps_cols<false>(axT, nullptr, N, Npad, L, ps, (int64_t)blockIdx.x - nbm);
+}
+
 // ln(k + 1e-10), ln(k + 16e-10) and (k + 1e-10) ln(k + 1e-10) for integer k in [0, N]
 __global__ void k_logtab(int64_t N, double *__restrict__ t)
 {
@@ -149,14 +164,17 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
   if (!m->d_pm) HIPCHK(hipMalloc(&m->d_pm, sizeof(double) * 8 * (size_t)L));
   if (!m->d_ps) HIPCHK(hipMalloc(&m->d_ps, sizeof(double) * 5 * (size_t)L));
   int t;
-  if (m->pm_fused) {
+  bool ps_done = false;
+  if (m->pm_fused) {                 // unit weights (int8 counts)
     // the Gram left 32 x 32 block partials in d_red (rsg_pm32_scratch) and cleared the flag
     m->pm_fused = 0;
     const int nb32 = (int)((L + 31) / 32);
-    rsg_tstart(c, "marginals", &t);
-    hipLaunchKernelGGL(k_pm32_reduce, dim3((unsigned)((L + 15) / 16)), dim3(256), 0, c->stream, c->d_red, nb32, L, tol,
-                       m->d_pm, m->d_pm + 4 * L, (int *)(c->d_res + 4));
+    const int nbm = (int)((L + 15) / 16), nbp = (int)((L + 3) / 4);
+    rsg_tstart(c, "marginals+ps", &t);
+    hipLaunchKernelGGL(k_pm32_ps, dim3((unsigned)(nbm + nbp)), dim3(256), 0, c->stream, c->d_red, nb32, L, tol,
+                       m->d_pm, m->d_pm + 4 * L, (int *)(c->d_res + 4), m->d_axT, m->N, m->Npad, m->d_ps, nbm);
     rsg_tstop(c, t);
+    ps_done = true;
     HIPCHK(hipGetLastError());
   } else {
   const int nbt = (int)((L + PMT - 1) / PMT);
@@ -177,6 +195,7 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   }
+  if (!ps_done) {
   rsg_tstart(c, "ps", &t);
   if (m->weighted)
     hipLaunchKernelGGL(k_ps<true>, dim3((unsigned)((L + 3) / 4)), dim3(256), 0, c->stream, m->d_axT, m->d_w, m->N, m->Npad, L, m->d_ps);
@@ -184,6 +203,7 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
     hipLaunchKernelGGL(k_ps<false>, dim3((unsigned)((L + 3) / 4)), dim3(256), 0, c->stream, m->d_axT, m->d_w, m->N, m->Npad, L, m->d_ps);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
+  }
   if (!m->d_lgc) {
     HIPCHK(hipMalloc(&m->d_lgc, sizeof(double) * 3 * (size_t)(m->N + 1)));
     hipLaunchKernelGGL(k_logtab, dim3((unsigned)((m->N + 256) / 256)), dim3(256), 0, c->stream, m->N, m->d_lgc);
