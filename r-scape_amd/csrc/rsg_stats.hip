@@ -213,11 +213,24 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
 }
 
 // ------------------------------------------------------------------------------------
-// 32 x 32 pair tile per block; COV written to both triangles through an LDS transpose.
+// Block t of a triangular grid -> pair tile (bi, bj), bi <= bj, column-block major:
+// t = bj (bj + 1) / 2 + bi.
+__device__ __forceinline__ void tri_tile(int64_t t, int &bi, int &bj)
+{
+  int64_t b = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((b + 1) * (b + 2) / 2 <= t) b++;
+  while (b * (b + 1) / 2 > t) b--;
+  bj = (int)b; bi = (int)(t - b * (b + 1) / 2);
+}
+
+// One 32 x 32 pair tile on or above the diagonal per block (nb (nb + 1) / 2 blocks); COV
+// written to both triangles through an LDS transpose.
 template <typename T, int KIND, bool TAB, int TYPE = -1, int CLS = -1>
 __global__ void __launch_bounds__(256) k_stat(StatArgs A, int nb)
 {
-  stat_tile<T, KIND, TAB, TYPE, CLS>(A, (int)(blockIdx.x % nb), (int)(blockIdx.x / nb), (int64_t)blockIdx.x);
+  int bi, bj;
+  tri_tile(blockIdx.x, bi, bj);
+  stat_tile<T, KIND, TAB, TYPE, CLS>(A, bi, bj, (int64_t)blockIdx.x);
 }
 
 // FP64 statistics from integer counts with the statistic and class fixed at compile time
@@ -228,10 +241,11 @@ template <int TYPE>
 static int launch_stat_fixed(rsg_ctx *c, const StatArgs &A, int cls, int nb)
 {
   if (cls != RSG_C16 && cls != RSG_C2) return 0;
-  const dim3 grid((unsigned)(nb * nb));
+  const int ntri = nb * (nb + 1) / 2;
+  const dim3 grid((unsigned)ntri);
   if (cls == RSG_C16) hipLaunchKernelGGL((k_stat<double, CNT_U16, true, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
   else                hipLaunchKernelGGL((k_stat<double, CNT_U16, true, TYPE, RSG_C2>), grid, dim3(256), 0, c->stream, A, nb);
-  return nb * nb;
+  return ntri;
 }
 
 // (min, max) of n block partials into out[0..1]; one block of 1024 threads, four partials per
@@ -321,7 +335,7 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
   const int64_t L = m->L;
   if (!m->d_cov) HIPCHK(hipMalloc(&m->d_cov, sizeof(double) * (size_t)(L * L)));
   const int nb = (int)((L + 31) / 32);
-  const int nblk = nb * nb;
+  const int nblk = nb * (nb + 1) / 2;             // pair tiles on or above the diagonal
   if (ensure_red(c, 2 * (size_t)nblk + 8 + 4 * (size_t)L, errbuf) != RSG_OK) return RSG_FAIL;
   StatArgs A;
   A.cnt = m->d_cnt; A.pm = m->d_pm; A.lpm = m->d_pm + 4 * L; A.lgc = m->d_lgc;
