@@ -80,11 +80,13 @@ k_stat_b(const int64_t *__restrict__ desc, int n, StatArgs A0, const uint16_t *_
 {
   const int it = find_item(desc, n, D_BSTAT, blockIdx.x);
   const int64_t *d = desc + it * RSG_BDESC;
-  const int64_t L = d[D_L], local = blockIdx.x - d[D_BSTAT], nb = d[D_NB32];
+  const int64_t L = d[D_L], local = blockIdx.x - d[D_BSTAT];
   StatArgs A = A0;
   A.cnt = cnt + d[D_CNT]; A.pm = pm + d[D_PM]; A.lpm = pm + d[D_PM] + 4 * L; A.L = L;
   A.wsum = (double)d[D_N]; A.cov = cov + d[D_COV]; A.type = base; A.cls = cls[it];
-  stat_tile<double, CNT_U16, true>(A, (int)(local % nb), (int)(local / nb), (int64_t)blockIdx.x);
+  int bi, bj;
+  tri_tile(local, bi, bj);                      // tiles on or above the diagonal only
+  stat_tile<double, CNT_U16, true>(A, bi, bj, (int64_t)blockIdx.x);
 }
 
 __global__ void __launch_bounds__(256)
@@ -358,7 +360,7 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
     }
     o_xt += Lpad * Npad; o_cnt += 16 * L * L; o_cov += L * L; o_pm += 8 * L; o_rows += 2 * L + 2; o_leaf += N * L;
     nodes += N - 1; seqs += (N - 1) * L;
-    b_pack += (Lpad / 64) * (Npad / 64); b_col += L; b_stat += nb32 * nb32; b_el += (L * L + ELPB - 1) / ELPB;
+    b_pack += (Lpad / 64) * (Npad / 64); b_col += L; b_stat += nb32 * (nb32 + 1) / 2; b_el += (L * L + ELPB - 1) / ELPB;
     b_sim += (L + 63) / 64; b_hist += (L * L + HIST_EPB - 1) / HIST_EPB;
     nmax = std::max(nmax, N);
   }

@@ -459,6 +459,16 @@ __device__ __forceinline__ void block_minmax(double lmin, double lmax, double *p
   if (tid == 0) { part[2 * part_idx] = rmin[0]; part[2 * part_idx + 1] = rmax[0]; }
 }
 
+// Block t of a triangular grid -> pair tile (bi, bj), bi <= bj, column-block major:
+// t = bj (bj + 1) / 2 + bi.
+__device__ __forceinline__ void tri_tile(int64_t t, int &bi, int &bj)
+{
+  int64_t b = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((b + 1) * (b + 2) / 2 <= t) b++;
+  while (b * (b + 1) / 2 > t) b--;
+  bj = (int)b; bi = (int)(t - b * (b + 1) / 2);
+}
+
 // One tile per block (tiles below the diagonal contribute +-inf), block min / max to part.
 template <typename T, int KIND, bool TAB, int TYPE = -1, int CLS = -1>
 __device__ void stat_tile(const StatArgs &A, int bi, int bj, int64_t part_idx)

@@ -213,16 +213,6 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
 }
 
 // ------------------------------------------------------------------------------------
-// Block t of a triangular grid -> pair tile (bi, bj), bi <= bj, column-block major:
-// t = bj (bj + 1) / 2 + bi.
-__device__ __forceinline__ void tri_tile(int64_t t, int &bi, int &bj)
-{
-  int64_t b = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while ((b + 1) * (b + 2) / 2 <= t) b++;
-  while (b * (b + 1) / 2 > t) b--;
-  bj = (int)b; bi = (int)(t - b * (b + 1) / 2);
-}
-
 // One 32 x 32 pair tile on or above the diagonal per block (nb (nb + 1) / 2 blocks); COV
 // written to both triangles through an LDS transpose.
 template <typename T, int KIND, bool TAB, int TYPE = -1, int CLS = -1>
