@@ -445,18 +445,49 @@ __device__ __forceinline__ void stat_pairs(const StatArgs &A, int bi, int bj, co
   }
 }
 
+// Sum of v over a 256-thread block with the pairing of the LDS tree "for st = 128 .. 1:
+// red[t] += red[t + st]": the first two levels in LDS, the last six by wave
+// shuffles in wave 0 -- the same additions in the same order, so the same value, with three
+// barriers instead of eight.  Every thread gets the sum (sh: 257 doubles, sh[256] the result).
+__device__ __forceinline__ double block_sum_tree256(double v, double *sh)
+{
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  if (t < 128) sh[t] += sh[t + 128];
+  __syncthreads();
+  if (t < 64) {
+    double x = sh[t] + sh[t + 64];
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1) x += __shfl_down(x, st, 64);
+    if (t == 0) sh[256] = x;
+  }
+  __syncthreads();
+  return sh[256];
+}
+
+// Min and max over a 256-thread block, the same way (sa, sb: 256 doubles each); the result is
+// valid in thread 0.
+__device__ __forceinline__ void block_minmax256(double &a, double &b, double *sa, double *sb)
+{
+  const int t = threadIdx.x;
+  sa[t] = a; sb[t] = b;
+  __syncthreads();
+  if (t < 128) { sa[t] = fmin(sa[t], sa[t + 128]); sb[t] = fmax(sb[t], sb[t + 128]); }
+  __syncthreads();
+  if (t < 64) {
+    a = fmin(sa[t], sa[t + 64]); b = fmax(sb[t], sb[t + 64]);
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1) { a = fmin(a, __shfl_down(a, st, 64)); b = fmax(b, __shfl_down(b, st, 64)); }
+  }
+}
+
 // Block min / max of the 256 threads to part[2 * part_idx], part[2 * part_idx + 1].
 __device__ __forceinline__ void block_minmax(double lmin, double lmax, double *part, int64_t part_idx)
 {
   __shared__ double rmin[256], rmax[256];
-  const int tid = threadIdx.x;
-  rmin[tid] = lmin; rmax[tid] = lmax;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s) { rmin[tid] = fmin(rmin[tid], rmin[tid + s]); rmax[tid] = fmax(rmax[tid], rmax[tid + s]); }
-    __syncthreads();
-  }
-  if (tid == 0) { part[2 * part_idx] = rmin[0]; part[2 * part_idx + 1] = rmax[0]; }
+  block_minmax256(lmin, lmax, rmin, rmax);
+  if (threadIdx.x == 0) { part[2 * part_idx] = lmin; part[2 * part_idx + 1] = lmax; }
 }
 
 // Block t of a triangular grid -> pair tile (bi, bj), bi <= bj, column-block major:

@@ -479,18 +479,13 @@ int rsg_raf(rsg_ctx *c, rsg_msa *m, int rafs, const double *allow16, char *errbu
 __global__ void __launch_bounds__(256) k_rowsum(const double *__restrict__ cov, int64_t L, double *__restrict__ rows,
                                                 int *__restrict__ flag)
 {
-  __shared__ double red[256];
+  __shared__ double red[257];
   const int64_t i = blockIdx.x;
   if (i == 0 && threadIdx.x == 0) *flag = 0;       // NaN flag of k_correct (runs after this kernel)
   double s = 0.;
   for (int64_t j = threadIdx.x; j < L; j += 256) if (j != i) s += cov[i * L + j];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) rows[L + i] = red[0];
+  s = block_sum_tree256(s, red);
+  if (threadIdx.x == 0) rows[L + i] = s;
 }
 
 // APC / ASC in place, one block per row i (no per-element index division); block min / max
@@ -501,17 +496,12 @@ __global__ void __launch_bounds__(256) k_correct(double *__restrict__ cov, int64
                                                  int actype, int *__restrict__ flag, double *__restrict__ part,
                                                  double *__restrict__ avg_out)
 {
-  __shared__ double rmin[256], rmax[256];
+  __shared__ double rmin[257], rmax[256];
   double s = 0.;
   for (int64_t k = threadIdx.x; k < L; k += 256) s += rows[L + k];
-  rmin[threadIdx.x] = s;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) rmin[threadIdx.x] += rmin[threadIdx.x + st];
-    __syncthreads();
-  }
-  const double avg = (L > 1) ? rmin[0] / ((double)L * ((double)L - 1.)) : rmin[0];
-  __syncthreads();
+  s = block_sum_tree256(s, rmin);
+  const double avg = (L > 1) ? s / ((double)L * ((double)L - 1.)) : s;
+  __syncthreads();                                   // rmin is reused below
   if (blockIdx.x == 0 && threadIdx.x == 0) *avg_out = avg;
   const double dl = (L > 1) ? (double)L - 1. : 1.;
   const int64_t i = blockIdx.x;
@@ -526,13 +516,8 @@ __global__ void __launch_bounds__(256) k_correct(double *__restrict__ cov, int64
     C[j] = r;
     a = fmin(a, r); b = fmax(b, r);
   }
-  rmin[threadIdx.x] = a; rmax[threadIdx.x] = b;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) { rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + st]); rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + st]); }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) { part[2 * i] = rmin[0]; part[2 * i + 1] = rmax[0]; }
+  block_minmax256(a, b, rmin, rmax);
+  if (threadIdx.x == 0) { part[2 * i] = a; part[2 * i + 1] = b; }
 }
 
 __global__ void k_shift(double *__restrict__ cov, int64_t L, const double *__restrict__ mm)
