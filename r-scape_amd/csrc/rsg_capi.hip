@@ -1,6 +1,7 @@
 // rsg_capi.hip -- extern "C" entry points of librscape_hip (declared in include/rscape_hip.h).
 #include "rsg_internal.h"
 #include <cmath>
+#include <chrono>
 #include <cstring>
 #include <new>
 
@@ -307,15 +308,19 @@ extern "C" int rsg_corr_GetCOV(rsg_ctx *c, rsg_msa *m, double *COV, double *minC
 // Wait for the completion stamp of the chain's last kernel in pinned memory (the result block is
 // complete once it shows); true when it was seen.  The stream is polled between spins, so a chain
 // that ends without writing the stamp (a failed launch) falls through to the caller's stream
-// synchronisation, which reports the failure.
+// synchronisation, which reports the failure; so does a chain still running after 2 ms (large
+// alignments), which then waits there instead of keeping a core spinning.
 static bool wait_stamp(rsg_ctx *c)
 {
   const unsigned long long want = c->hstamp;
   unsigned long long *slot = (unsigned long long *)(c->h_pinned + 63);
+  const auto t0 = std::chrono::steady_clock::now();
   for (unsigned spin = 1;; spin++) {
     if (__atomic_load_n(slot, __ATOMIC_ACQUIRE) == want) return true;
-    if ((spin & 255) == 0 && hipStreamQuery(c->stream) != hipErrorNotReady)
-      return __atomic_load_n(slot, __ATOMIC_ACQUIRE) == want;
+    if ((spin & 255) == 0) {
+      if (hipStreamQuery(c->stream) != hipErrorNotReady) return __atomic_load_n(slot, __ATOMIC_ACQUIRE) == want;
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) return false;
+    }
   }
 }
 
