@@ -8,7 +8,9 @@
  *
  * Threading: one context = one HIP stream; a context is not re-entrant.  Calls are
  * synchronous at this boundary (work is queued on the context stream, and the call
- * returns after it finished), except where marked "async".
+ * returns after it finished), except where marked "async".  rsg_cov_Calculate returns as
+ * soon as its last kernel has written the result block to pinned host memory (every
+ * device output is complete by then); later calls on the context follow on its stream.
  *
  * Each entry point names the reference interface it replaces (R-scape v2.6.11).
  * Symbol conventions follow the reference: digital RNA codes 0..3 = A,C,G,U,
