@@ -44,6 +44,8 @@ extern "C" void rsg_ctx_destroy(rsg_ctx *c)
   if (c->d_red) hipFree(c->d_red);
   if (c->d_gscr) hipFree(c->d_gscr);
   if (c->d_arena) hipFree(c->d_arena);
+  if (c->d_hits) hipFree(c->d_hits);
+  if (c->h_hits) hipHostFree(c->h_hits);
   if (c->d_flag) hipFree(c->d_flag);
   if (c->d_res) hipFree(c->d_res);
   if (c->d_small) hipFree(c->d_small);

@@ -76,6 +76,8 @@ struct rsg_ctx {
   int          ncu = 0;                 // compute units of the device
   void        *d_gscr = nullptr;  size_t gscr_cap = 0;   // Gram split partials
   void        *d_arena = nullptr; size_t arena_cap = 0;  // batched-null buffers (grow-only)
+  void        *d_hits = nullptr;  size_t hits_cap = 0;   // hit-list compaction (grow-only, entries)
+  void        *h_hits = nullptr;                          // its pinned host mirror, same capacity
   // small reusable device scratch (reductions)
   double      *d_red = nullptr;  size_t red_cap = 0;
   int         *d_flag = nullptr;

@@ -35,6 +35,7 @@ struct rsg_mlp {
   uint16_t *d_wb = nullptr;                // BF16 copy of the whole parameter vector
   float   *d_part = nullptr;  int64_t part_blocks = 0;
   float   *d_io = nullptr;    size_t io_cap = 0;     // forward inputs + outputs, grown on demand
+  float   *h_io = nullptr;                           // pinned host staging of d_io, same capacity
   int64_t  step = 0;                       // Adam step counter
   size_t   lds_bytes = 0;
   float   *d_scale = nullptr;                // input mean | input 1/std | output mean | output std
@@ -367,6 +368,7 @@ extern "C" void rsg_mlp_destroy(rsg_ctx *c, rsg_mlp *m)
   hipFree(m->d_p); hipFree(m->d_m); hipFree(m->d_v); hipFree(m->d_g); hipFree(m->d_wb); hipFree(m->d_scale);
   if (m->d_part) hipFree(m->d_part);
   if (m->d_io) hipFree(m->d_io);
+  if (m->h_io) hipHostFree(m->h_io);
   delete m;
 }
 
@@ -464,21 +466,27 @@ extern "C" int rsg_mlp_forward(rsg_ctx *c, rsg_mlp *m, const float *x, int64_t B
   const size_t xb = sizeof(float) * (size_t)(B * m->dims[0]), yb = sizeof(float) * (size_t)(B * m->dims[m->nl]);
   if (m->io_cap < xb + yb) {
     if (m->d_io) hipFree(m->d_io);
-    m->d_io = nullptr; m->io_cap = 0;
+    if (m->h_io) hipHostFree(m->h_io);
+    m->d_io = m->h_io = nullptr; m->io_cap = 0;
     HIPCHK(hipMalloc(&m->d_io, xb + yb));
+    HIPCHK(hipHostMalloc((void **)&m->h_io, xb + yb, hipHostMallocDefault));
     m->io_cap = xb + yb;
   }
+  // inputs and outputs go through pinned staging: small pageable copies cost tens of us each
   float *d_x = m->d_io, *d_y = m->d_io + (size_t)(B * m->dims[0]);
+  float *h_x = m->h_io, *h_y = m->h_io + (size_t)(B * m->dims[0]);
+  memcpy(h_x, x, xb);
   MlpArgs a = make_args(m);
   a.x = d_x; a.y = d_y; a.nrows = B; a.first = 0;
   int st = RSG_OK;
-  if (hipMemcpyAsync(d_x, x, xb, hipMemcpyHostToDevice, c->stream) != hipSuccess) st = rsg_fail(errbuf, "rsg_mlp_forward: upload failed");
+  if (hipMemcpyAsync(d_x, h_x, xb, hipMemcpyHostToDevice, c->stream) != hipSuccess) st = rsg_fail(errbuf, "rsg_mlp_forward: upload failed");
   int t; rsg_tstart(c, "mlp_forward", &t);
   if (st == RSG_OK) st = launch_mlp(c, m, a, false, errbuf);
   rsg_tstop(c, t);
-  if (st == RSG_OK && (hipMemcpyAsync(y, d_y, yb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+  if (st == RSG_OK && (hipMemcpyAsync(h_y, d_y, yb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                        hipStreamSynchronize(c->stream) != hipSuccess))
     st = rsg_fail(errbuf, "rsg_mlp_forward: download failed");
+  if (st == RSG_OK) memcpy(y, h_y, yb);
   return st;
 }
 

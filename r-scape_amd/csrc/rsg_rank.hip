@@ -385,29 +385,39 @@ extern "C" int rsg_cov_Hits(rsg_ctx *c, rsg_msa *m, double thresh, int cap, int 
   if (!m->have_eval) return rsg_fail(errbuf, "rsg_cov_Hits: no E-values (run rsg_cov_Evalues or rsg_cov_EvaluesTail)");
   HIPCHK(hipSetDevice(c->device));
   const int64_t L = m->L;
-  const int dcap = std::max(cap, 1);
-  int2 *d_ij = nullptr; double2 *d_se = nullptr;
-  HIPCHK(hipMalloc(&d_ij, sizeof(int2) * (size_t)dcap));
-  if (hipMalloc(&d_se, sizeof(double2) * (size_t)dcap) != hipSuccess) { hipFree(d_ij); return rsg_fail(errbuf, "rsg_cov_Hits: out of device memory"); }
+  const size_t dcap = (size_t)std::max(cap, 1);
+  if (c->hits_cap < dcap) {                      // grow-only: repeated calls allocate nothing
+    if (c->d_hits) hipFree(c->d_hits);
+    if (c->h_hits) hipHostFree(c->h_hits);
+    c->d_hits = c->h_hits = nullptr; c->hits_cap = 0;
+    HIPCHK(hipMalloc(&c->d_hits, (sizeof(double2) + sizeof(int2)) * dcap));
+    HIPCHK(hipHostMalloc(&c->h_hits, (sizeof(double2) + sizeof(int2)) * dcap, hipHostMallocDefault));
+    c->hits_cap = dcap;
+  }
+  double2 *d_se = (double2 *)c->d_hits, *se = (double2 *)c->h_hits;
+  int2 *d_ij = (int2 *)(d_se + c->hits_cap), *ij = (int2 *)(se + c->hits_cap);
   int st = RSG_OK;
   int *d_count = (int *)c->d_flag + 8;
-  std::vector<int2> ij; std::vector<double2> se;
+  // the count and the first `pre` entries come back with one synchronisation; more hits than
+  // that take a second copy
+  const int pre = std::min(cap, 4096);
   if (hipMemsetAsync(d_count, 0, sizeof(int), c->stream) != hipSuccess) st = rsg_fail(errbuf, "rsg_cov_Hits: memset failed");
   if (st == RSG_OK) {
     hipLaunchKernelGGL(k_hits, dim3((unsigned)((L * L + 255) / 256)), dim3(256), 0, c->stream, m->d_eval, m->d_cov, L, thresh,
                        cap, d_count, d_ij, d_se);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(c->h_pinned, d_count, sizeof(int), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        (pre > 0 && (hipMemcpyAsync(ij, d_ij, sizeof(int2) * pre, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                     hipMemcpyAsync(se, d_se, sizeof(double2) * pre, hipMemcpyDeviceToHost, c->stream) != hipSuccess)) ||
         hipStreamSynchronize(c->stream) != hipSuccess) st = rsg_fail(errbuf, "rsg_cov_Hits failed");
   }
   int n = (st == RSG_OK) ? *(int *)c->h_pinned : 0;
   const int got = std::min(n, cap);
-  if (st == RSG_OK && got > 0) {
-    ij.resize((size_t)got); se.resize((size_t)got);
-    if (hipMemcpy(ij.data(), d_ij, sizeof(int2) * got, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(se.data(), d_se, sizeof(double2) * got, hipMemcpyDeviceToHost) != hipSuccess) st = rsg_fail(errbuf, "rsg_cov_Hits: download failed");
+  if (st == RSG_OK && got > pre) {
+    if (hipMemcpyAsync(ij + pre, d_ij + pre, sizeof(int2) * (got - pre), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(se + pre, d_se + pre, sizeof(double2) * (got - pre), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) st = rsg_fail(errbuf, "rsg_cov_Hits: download failed");
   }
-  hipFree(d_ij); hipFree(d_se);
   if (st != RSG_OK) return st;
   // the atomic order is arbitrary: sort by (i, j)
   std::vector<int> ord((size_t)got);

@@ -218,13 +218,21 @@ def evalues_tail(ctx, msa, m, lam, p0, Nc=None, matrix=True, xcap=math.inf):
     return ev
 
 
-def hits(ctx, msa, thresh=0.05, cap=1 << 20):
-    """Significant pairs of the last E-value call: arrays i, j, score, E (sorted by i, j)."""
+def hits(ctx, msa, thresh=0.05, cap=None):
+    """Significant pairs of the last E-value call: arrays i, j, score, E (sorted by i, j).
+    cap=None sizes the arrays to the hits (4096 first; the library reports the total when
+    there are more, and the call is made again at that size); an explicit cap is a hard limit."""
     L = _declare()
-    hi = np.zeros(cap, np.int32); hj = np.zeros(cap, np.int32)
-    sc = np.zeros(cap); ev = np.zeros(cap)
-    n = C.c_int(0)
-    err = _err()
-    _check(L.rsg_cov_Hits(ctx.h, msa.h, thresh, cap, C.byref(n), _p(hi), _p(hj), _p(sc), _p(ev), err), err, "rsg_cov_Hits")
-    k = n.value
-    return hi[:k], hj[:k], sc[:k], ev[:k]
+    size = 4096 if cap is None else cap
+    while True:
+        hi = np.empty(size, np.int32); hj = np.empty(size, np.int32)
+        sc = np.empty(size); ev = np.empty(size)
+        n = C.c_int(0)
+        err = _err()
+        rc = L.rsg_cov_Hits(ctx.h, msa.h, thresh, size, C.byref(n), _p(hi), _p(hj), _p(sc), _p(ev), err)
+        if rc != 0 and cap is None and n.value > size:
+            size = n.value
+            continue
+        _check(rc, err, "rsg_cov_Hits")
+        k = n.value
+        return hi[:k], hj[:k], sc[:k], ev[:k]

@@ -129,6 +129,30 @@ def test_evalues_tail(ctx):
     assert top.sum() >= 10 and np.all(evc[top] == evc[top][0])
 
 
+
+@pytest.mark.parametrize("thresh", [1e-3, 1e300])
+def test_hit_list_matches_evalue_matrix(ctx, thresh):
+    """rsg_cov_Hits: the pairs i < j with E <= thresh, sorted by (i, j), with their scores and
+    E-values, as read off the full matrices on the host -- a few hits, and every pair (more than
+    the 4096 fetched with the count, so the remainder takes a second copy); called twice in a
+    row (the compaction buffers are reused)."""
+    L = 120
+    d = synth.config(0, seed=6, N=150, L=L)
+    msa = ctx.msa(d["ax"])
+    msa.calculate("GTp")
+    cov, _, _ = msa.get_cov()
+    ev = NL.evalues_tail(ctx, msa, 5.0, 0.4, 1e-3)
+    iu = np.triu_indices(L, 1)
+    sel = ev[iu] <= thresh
+    ri, rj = iu[0][sel], iu[1][sel]
+    assert 0 < len(ri) and (thresh < 1 or len(ri) == L * (L - 1) // 2 > 4096)
+    for _ in range(2):
+        NL.evalues_tail(ctx, msa, 5.0, 0.4, 1e-3, matrix=False)
+        hi, hj, sc, he = NL.hits(ctx, msa, thresh)
+        assert np.array_equal(hi, ri) and np.array_equal(hj, rj)
+        assert np.array_equal(sc, cov[ri, rj]) and np.array_equal(he, ev[ri, rj])
+    msa.close()
+
 def test_surrogate_beats_constant_tail(ctx):
     """Trained on batched-null fits of small random setups, the network predicts held-out
     tails better than the method without it (one tail for every alignment)."""
