@@ -19,6 +19,7 @@
  */
 #ifndef RSCAPE_HIP_H
 #define RSCAPE_HIP_H
+#include <stddef.h>
 #include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
@@ -66,6 +67,10 @@ int         rsg_ctx_set_gram_shape(rsg_ctx *ctx, int shape);
  * are split along the sequence axis over the idle CUs and their partial counts summed
  * (integers, so the result is identical); 0: one work group per tile. */
 int         rsg_ctx_set_gram_split(rsg_ctx *ctx, int on);
+/* How long rsg_cov_Calculate spins on its completion stamp in pinned host memory before it
+ * waits in a stream synchronisation instead (default 2000 microseconds; 0 = always
+ * synchronise).  Results are the same either way (no reference counterpart). */
+int         rsg_ctx_set_stamp_wait(rsg_ctx *ctx, int microseconds);
 /* Counter calibration (no reference counterpart): runs the plain copy kernel
  * k_copy_probe over `bytes` (multiple of 16) three times, reading and writing `bytes` per
  * launch, so HBM counters of other kernels can be converted to bytes. */

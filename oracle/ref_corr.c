@@ -17,6 +17,11 @@
 
 static int canonical(uint8_t a) { return a < K; }
 
+/* Threads of the count loops (OpenMP over rows i; results do not depend on it). */
+static int ref_threads = 1;
+void ref_set_threads(int n) { ref_threads = n < 1 ? 1 : n; }
+int  ref_get_threads(void) { return ref_threads; }
+
 /* Compensated sum; restatement of Easel esl_vec_DSum (un-vendored). */
 static double dsum(const double *v, int n)
 {
@@ -50,27 +55,38 @@ static int dvalidate(const double *v, int n, double tol)
 int ref_naive_pp(const uint8_t *ax, const double *wgt, int64_t N, int64_t L,
                  double *pp, double *nseff, double *ngap)
 {
+  /* the columns are read from a column-major copy (contiguous over s); each pair still sums
+     over s = 0..N-1 in the reference's order.  Rows i are independent, so the threaded loop
+     (ref_set_threads; 1 thread by default) gives bit-identical results */
+  uint8_t *axT = malloc((size_t)(N * L));
+  if (!axT) return REF_FAIL;
+  for (int64_t s = 0; s < N; s++)
+    for (int64_t i = 0; i < L; i++) axT[i * N + s] = ax[s * L + i];
+#pragma omp parallel for schedule(dynamic, 1) num_threads(ref_threads)
   for (int64_t i = 0; i < L - 1; i++)
     for (int64_t j = i + 1; j < L; j++) {
+      const uint8_t *ci = axT + i * N, *cj = axT + j * N;
       double *p = pp + (i * L + j) * K2;
       for (int c = 0; c < K2; c++) p[c] = 1e-10;             /* :1713 */
-      nseff[i * L + j] = 0.;
-      ngap[i * L + j]  = 0.;                                   /* zeroed by corr_Reuse :1221-1251 */
+      double ns = 0., ng = 0.;                                 /* zeroed by corr_Reuse :1221-1251 */
       for (int64_t s = 0; s < N; s++) {
-        uint8_t ri = ax[s * L + i], rj = ax[s * L + j];
+        uint8_t ri = ci[s], rj = cj[s];
         if (canonical(ri) && canonical(rj)) {
-          nseff[i * L + j] += wgt[s];
-          p[IDX(ri, rj)]   += wgt[s];
+          ns += wgt[s];
+          p[IDX(ri, rj)] += wgt[s];
         } else {
-          ngap[i * L + j]  += wgt[s];                          /* :1752 */
+          ng += wgt[s];                                        /* :1752 */
         }
       }
+      nseff[i * L + j] = ns;
+      ngap[i * L + j]  = ng;
       dnorm(p, K2);                                            /* :1758 */
       double *q = pp + (j * L + i) * K2;
       for (int x = 0; x < K; x++)
         for (int y = 0; y < K; y++) q[IDX(y, x)] = p[IDX(x, y)];
       nseff[j * L + i] = nseff[i * L + j];
     }
+  free(axT);
   return REF_OK;
 }
 
@@ -78,14 +94,21 @@ int ref_naive_pp(const uint8_t *ax, const double *wgt, int64_t N, int64_t L,
 int ref_pair_counts(const uint8_t *ax, int64_t N, int64_t L, int32_t *cnt)
 {
   memset(cnt, 0, sizeof(int32_t) * (size_t)(L * L * K2));
+  uint8_t *axT = malloc((size_t)(N * L));
+  if (!axT) return REF_FAIL;
+  for (int64_t s = 0; s < N; s++)
+    for (int64_t i = 0; i < L; i++) axT[i * N + s] = ax[s * L + i];
+#pragma omp parallel for schedule(dynamic, 1) num_threads(ref_threads)
   for (int64_t i = 0; i < L - 1; i++)
     for (int64_t j = i + 1; j < L; j++) {
+      const uint8_t *ci = axT + i * N, *cj = axT + j * N;
       int32_t *c = cnt + (i * L + j) * K2;
       for (int64_t s = 0; s < N; s++) {
-        uint8_t ri = ax[s * L + i], rj = ax[s * L + j];
+        uint8_t ri = ci[s], rj = cj[s];
         if (canonical(ri) && canonical(rj)) c[IDX(ri, rj)]++;
       }
     }
+  free(axT);
   return REF_OK;
 }
 
@@ -312,6 +335,47 @@ int ref_raf(const uint8_t *ax, int64_t N, int64_t L, const double *allowpair, do
       cij /= (N > 1) ? (double)N * ((double)N - 1.0) : 1.0;
       cij *= 2.0;
       put(L, COV, minmax, i, j, cij - qij);
+    }
+  return REF_OK;
+}
+
+/* RAF of correlators.c:876-933 from the unweighted 4x4 pair counts instead of the O(N^2)
+ * sequence-pair loop (for alignments where that loop cannot finish).  With n_xy the count of
+ * allowed pairs (x, y) at (i, j), M = sum n_xy, m_x / m'_y its row / column sums:
+ *   sum_{s1<s2} A_s1 A_s2 [a_s1 != a_s2 at i] = (M^2 - sum_x m_x^2) / 2, same at j, and a pair of
+ *   sequences scores [differ at i] + [differ at j] (2 when both, 1 when one), so
+ *   cij = 2 / (N (N - 1)) ((M^2 - sum m_x^2) / 2 + (M^2 - sum m'_y^2) / 2), qij = (N - M) / N.
+ * Pinned against ref_raf (the literal loop) on small alignments. */
+int ref_raf_counts(const int32_t *cnt, int64_t N, int64_t L, const double *allowpair, double *COV, double *minmax)
+{
+  reset_cov(L, COV, minmax);
+  for (int64_t i = 0; i < L - 1; i++)
+    for (int64_t j = i + 1; j < L; j++) {
+      const int32_t *c = cnt + (i * L + j) * K2;
+      double M = 0., mx[K] = {0., 0., 0., 0.}, my[K] = {0., 0., 0., 0.};
+      for (int x = 0; x < K; x++)
+        for (int y = 0; y < K; y++)
+          if (allowed(x, y, allowpair)) { double n = c[IDX(x, y)]; M += n; mx[x] += n; my[y] += n; }
+      double sx = 0., sy = 0.;
+      for (int k = 0; k < K; k++) { sx += mx[k] * mx[k]; sy += my[k] * my[k]; }
+      double cij = 0.5 * (M * M - sx) + 0.5 * (M * M - sy);
+      cij /= (N > 1) ? (double)N * ((double)N - 1.0) : 1.0;
+      cij *= 2.0;
+      put(L, COV, minmax, i, j, cij - ((double)N - M) / (double)N);
+    }
+  return REF_OK;
+}
+
+/* The RAFS stencil (correlators.c:954-960) applied to a RAF matrix B (both triangles). */
+int ref_rafs_from(const double *B, int64_t L, double *COV, double *minmax)
+{
+  reset_cov(L, COV, minmax);
+  for (int64_t i = 0; i < L - 1; i++)
+    for (int64_t j = i + 1; j < L; j++) {
+      double v = 2.0 * B[i * L + j];
+      if (i > 0 && j < L - 1)               v += B[(i - 1) * L + (j + 1)];
+      if (j > 0 && i < L - 1 && i < j - 2)  v += B[(i + 1) * L + (j - 1)];
+      put(L, COV, minmax, i, j, 0.25 * v);
     }
   return REF_OK;
 }

@@ -43,6 +43,10 @@ int ref_marginals(int64_t L, const double *pp, const double *nseff, double tol, 
 int ref_validate_probs(int64_t L, const double *pp, const double *pm, const double *ps, double tol, char *errbuf);
 int ref_probs(const uint8_t *ax, const double *wgt, int64_t N, int64_t L, double tol,
               double *pp, double *nseff, double *ngap, double *pm, double *ps, char *errbuf);
+/* threads of the pair-count loops (default 1, the reference's serial run); results are
+   bit-identical for any count */
+void ref_set_threads(int n);
+int  ref_get_threads(void);
 /* integer 4x4 pair counts (unit weights), cnt[(i*L+j)*16+x*4+y] for i<j */
 int ref_pair_counts(const uint8_t *ax, int64_t N, int64_t L, int32_t *cnt);
 
@@ -52,6 +56,9 @@ int ref_cov_stat(int covtype, int covclass, int64_t nseq, int64_t L, int nseqthr
                  const double *nseff, const double *ngap, double *COV, double *minmax, char *errbuf);
 int ref_raf(const uint8_t *ax, int64_t N, int64_t L, const double *allowpair, double *COV, double *minmax);
 int ref_rafs(const uint8_t *ax, int64_t N, int64_t L, const double *allowpair, double *COV, double *minmax);
+/* RAF from unweighted pair counts (closed form of the literal loop); RAFS stencil over a RAF matrix */
+int ref_raf_counts(const int32_t *cnt, int64_t N, int64_t L, const double *allowpair, double *COV, double *minmax);
+int ref_rafs_from(const double *B, int64_t L, double *COV, double *minmax);
 int ref_cov_corrected(int actype, int64_t L, double *COV, double *minmax, int shiftnonneg, char *errbuf);
 
 /* ---- histogram with Easel semantics (esl_histogram, restated) ---- */

@@ -27,7 +27,8 @@
 enum { D_N = 0, D_L, D_NPAD, D_LPAD, D_XT, D_CNT, D_COV, D_PM, D_ROWS, D_HIST, D_LEAF, D_BPACK, D_BCOL,
        D_BSTAT, D_BEL, D_NB32, D_GROUP, D_BHIST, D_PMP, D_BCOL16 };
 #define HIST_EPB 65536       // matrix elements per histogram block
-#define NBMAX 16384          // histogram bins per item (an item needing more is reported as failed)
+#define NBLDS 16384          // histograms up to this many bins are privatised in LDS per block
+#define NBPOOL 16384         // bins of the shared bin pool per group (a group may take more)
 #define ELPB 1024            // matrix elements per block in the element-wise kernels
 
 __device__ __forceinline__ int find_item(const int64_t *__restrict__ desc, int n, int field, int64_t blk)
@@ -175,7 +176,7 @@ k_item_minmax(const int64_t *__restrict__ desc, int field, int64_t nblk_total, i
 // replicate (calculate_width_histo runs on the first null replicate, src/R-scape.c:1353);
 // the bounds cover every replicate, as the cumulative histogram grows to do
 // (null_add2cumranklist, src/R-scape.c:1564-1612).
-struct HGeo { double w, bmax, xmin, xmax, npairs, mn, mx; int nb, imin, imax, ok; };
+struct HGeo { double w, bmax, xmin, xmax, npairs, mn, mx; int64_t off; int nb, imin, imax, ok; };
 
 __global__ void k_width_b(int ng, const int *__restrict__ gfirst, const int64_t *__restrict__ desc,
                           const double *__restrict__ mm, double bmin, double w0, int hpts, double tol,
@@ -194,13 +195,13 @@ __global__ void k_width_b(int ng, const int *__restrict__ gfirst, const int64_t 
     const int64_t L = desc[j * RSG_BDESC + D_L];
     np += (double)(L * (L - 1) / 2);
   }
-  G.w = w; G.bmax = 0.; G.xmin = G.xmax = 0.; G.nb = 0; G.imin = 0; G.imax = -1;
+  G.w = w; G.bmax = 0.; G.xmin = G.xmax = 0.; G.nb = 0; G.imin = 0; G.imax = -1; G.off = 0;
   G.npairs = np; G.mn = mn; G.mx = mx;
   if (w > 0. && mx > bmin) {
     double bmax = mx + 5 * w;
     while (fabs(bmax - bmin) < tol) bmax += w;
     const double nbd = (bmax - bmin) / w;
-    if (nbd < (double)NBMAX) {
+    if (nbd < (double)INT_MAX) {
       G.nb = (int)nbd; G.bmax = bmax;
       G.xmin = fmax(mn, bmin + w); G.xmax = fmax(mx, bmin + w);
       G.imin = (int)ceil((G.xmin - bmin) / w - 1.); G.imax = (int)ceil((G.xmax - bmin) / w - 1.);
@@ -210,10 +211,41 @@ __global__ void k_width_b(int ng, const int *__restrict__ gfirst, const int64_t 
   geo[g] = G;
 }
 
-__global__ void k_fill_u64(unsigned long long *__restrict__ p, int64_t n)
+// Bins of every group in one pool: offsets by an exclusive scan over the groups (one block,
+// fixed order); a group that does not fit in the pool's cap bins is marked failed (its fit is
+// NaN, as for a group without a histogram).
+__global__ void __launch_bounds__(1024) k_hist_layout(int ng, HGeo *__restrict__ geo, int64_t cap)
 {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) p[k] = 0ull;
+  __shared__ int64_t sh[1024];
+  const int T = 1024, tid = threadIdx.x;
+  const int chunk = (ng + T - 1) / T, g0 = tid * chunk, g1 = min(g0 + chunk, ng);
+  int64_t own = 0;
+  for (int g = g0; g < g1; g++) own += geo[g].ok ? (int64_t)geo[g].nb : 0;
+  sh[tid] = own;
+  __syncthreads();
+  for (int o = 1; o < T; o <<= 1) {                 // inclusive Hillis-Steele scan
+    const int64_t add = (tid >= o) ? sh[tid - o] : 0;
+    __syncthreads();
+    sh[tid] += add;
+    __syncthreads();
+  }
+  int64_t off = sh[tid] - own;
+  for (int g = g0; g < g1; g++) {
+    if (!geo[g].ok) continue;
+    geo[g].off = off;
+    off += geo[g].nb;
+    if (off > cap) geo[g].ok = 0;
+  }
+}
+
+// clear the bins of every group (blocks take groups round-robin)
+__global__ void __launch_bounds__(256) k_clear_bins(int ng, const HGeo *__restrict__ geo, unsigned long long *__restrict__ hist)
+{
+  for (int g = blockIdx.x; g < ng; g += gridDim.x) {
+    const HGeo G = geo[g];
+    if (!G.ok) continue;
+    for (int64_t b = threadIdx.x; b < G.nb; b += 256) hist[G.off + b] = 0ull;
+  }
 }
 
 __global__ void k_fill_int(int *__restrict__ p, int n)
@@ -229,12 +261,14 @@ __global__ void __launch_bounds__(256)
 k_hist_b(const int64_t *__restrict__ desc, int n, const double *__restrict__ cov, const HGeo *__restrict__ geo,
          double bmin, unsigned long long *__restrict__ hist)
 {
-  __shared__ unsigned int lh[NBMAX];
+  __shared__ unsigned int lh[NBLDS];
   const int it = find_item(desc, n, D_BHIST, blockIdx.x);
   const int64_t *d = desc + it * RSG_BDESC;
   const HGeo g = geo[d[D_GROUP]];
   if (!g.ok) return;
-  for (int b = threadIdx.x; b < g.nb; b += 256) lh[b] = 0u;
+  const bool lds = (g.nb <= NBLDS);                 // else straight to the group's bins
+  unsigned long long *H = hist + g.off;
+  if (lds) for (int b = threadIdx.x; b < g.nb; b += 256) lh[b] = 0u;
   __syncthreads();
   const int64_t L = d[D_L], e0 = (blockIdx.x - d[D_BHIST]) * (int64_t)HIST_EPB;
   const int64_t e1 = min(e0 + (int64_t)HIST_EPB, L * L);
@@ -247,12 +281,13 @@ k_hist_b(const int64_t *__restrict__ desc, int n, const double *__restrict__ cov
     const double add = (x > fl) ? x : fl;
     int b = (int)ceil(((add - bmin) / g.w) - 1.);
     b = (b < 0) ? 0 : ((b >= g.nb) ? g.nb - 1 : b);
-    atomicAdd(&lh[b], 1u);
+    if (lds) atomicAdd(&lh[b], 1u);
+    else     atomicAdd(&H[b], 1ull);
   }
   __syncthreads();
-  unsigned long long *H = hist + d[D_GROUP] * NBMAX;
-  for (int b = threadIdx.x; b < g.nb; b += 256)
-    if (lh[b]) atomicAdd(&H[b], (unsigned long long)lh[b]);
+  if (lds)
+    for (int b = threadIdx.x; b < g.nb; b += 256)
+      if (lh[b]) atomicAdd(&H[b], (unsigned long long)lh[b]);
 }
 
 // tail fit per null (one 1024-thread block per group): see rsg_rank.hip for the method
@@ -269,8 +304,8 @@ k_fit_b(const HGeo *__restrict__ geo, const unsigned long long *__restrict__ his
     if (threadIdx.x == 0) { o[0] = NAN; o[1] = INFINITY; o[2] = 0.; o[3] = 0.; o[4] = NAN; o[5] = G.w; }
     return;
   }
-  nullfit_body(hist + (int64_t)g * NBMAX, G.nb, G.imin, G.imax, G.npairs, G.npairs, bmin, G.w, pmass, fracfit, doexp,
-               S + (int64_t)g * NBMAX, NZ + (int64_t)g * NBMAX, nullptr, o);
+  nullfit_body(hist + G.off, G.nb, G.imin, G.imax, G.npairs, G.npairs, bmin, G.w, pmass, fracfit, doexp,
+               S + G.off, NZ + G.off, nullptr, o);
   if (threadIdx.x == 0) { o[5] = G.w; o[6] = G.mn; o[7] = G.mx; }
 }
 
@@ -338,7 +373,7 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
     int64_t *d = &desc[(size_t)k * RSG_BDESC];
     d[D_N] = N; d[D_L] = L; d[D_NPAD] = Npad; d[D_LPAD] = Lpad;
     d[D_XT] = o_xt; d[D_CNT] = o_cnt; d[D_COV] = o_cov; d[D_PM] = o_pm; d[D_ROWS] = o_rows;
-    d[D_HIST] = (int64_t)k * NBMAX; d[D_LEAF] = o_leaf;
+    d[D_HIST] = 0; d[D_LEAF] = o_leaf;
     d[D_BPACK] = b_pack; d[D_BCOL] = b_col; d[D_BSTAT] = b_stat; d[D_BEL] = b_el;
     const int64_t nb32 = (L + 31) / 32;
     d[D_NB32] = nb32; d[D_GROUP] = J.group; d[D_BHIST] = b_hist;
@@ -409,6 +444,9 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
     f[0] = d0[D_XT]; f[1] = d0[D_N]; f[2] = d0[D_NPAD]; f[3] = d0[D_L]; f[4] = fblocks; f[5] = d0[D_LPAD];
     fblocks += (d0[D_L] + 3) / 4;
   }
+  // histogram bins of all groups share one pool (a group's share is set on the device once its
+  // score range is known)
+  const int64_t pool = std::max<int64_t>((int64_t)ng * NBPOOL, (int64_t)1 << 21);
   // ---- device buffers: one grow-only arena of the context, carved in two passes
   int64_t *d_desc; SimItem *d_sim; int *d_cls; int4 *d_tiles, *d_btiles; int *d_left, *d_right;
   double *d_ld, *d_rd, *d_models; int4 *d_ent; int *d_gfirst; int64_t *d_fitems; unsigned long long *d_fcnt;
@@ -430,8 +468,8 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
     d_rows = A.get<double>((size_t)o_rows); d_lgc = A.get<double>(3 * (size_t)(nmax + 1));
     d_part_s = A.get<double>(2 * (size_t)b_stat); d_part_e = A.get<double>(2 * (size_t)b_el);
     d_mm = A.get<double>(2 * (size_t)nj); d_pmp = A.get<double>((size_t)o_pmp); d_geo = A.get<HGeo>((size_t)ng);
-    d_hist = A.get<unsigned long long>((size_t)ng * NBMAX); d_S = A.get<unsigned long long>((size_t)ng * NBMAX);
-    d_NZ = A.get<int>((size_t)ng * NBMAX); d_fits = A.get<double>((size_t)ng * 8);
+    d_hist = A.get<unsigned long long>((size_t)pool); d_S = A.get<unsigned long long>((size_t)pool);
+    d_NZ = A.get<int>((size_t)pool); d_fits = A.get<double>((size_t)ng * 8);
     if (fp) {
       const size_t N = (size_t)fp->N, L = (size_t)fp->L, R = (size_t)nj;
       f_ax = A.get<uint8_t>(N * L); f_tree = A.get<int>(4 * (size_t)fp->m); f_perm = A.get<int>(R * L);
@@ -511,7 +549,8 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
     }
     hipLaunchKernelGGL(k_width_b, dim3((unsigned)((ng + 63) / 64)), dim3(64), 0, c->stream, ng, d_gfirst, d_desc, d_mm,
                        bmin, 0.05, 400, 1e-6, d_geo);
-    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)(((int64_t)ng * NBMAX + 255) / 256)), dim3(256), 0, c->stream, d_hist, (int64_t)ng * NBMAX);
+    hipLaunchKernelGGL(k_hist_layout, dim3(1), dim3(1024), 0, c->stream, ng, d_geo, pool);
+    hipLaunchKernelGGL(k_clear_bins, dim3((unsigned)std::min(ng, 4096)), dim3(256), 0, c->stream, ng, d_geo, d_hist);
     hipLaunchKernelGGL(k_hist_b, dim3((unsigned)b_hist), dim3(256), 0, c->stream, d_desc, nj, d_cov, d_geo, bmin, d_hist);
     hipLaunchKernelGGL(k_fit_b, dim3((unsigned)ng), dim3(1024), 0, c->stream, d_geo, d_hist, d_S, d_NZ,
                        bmin, pmass, fracfit, doexpfit, d_fits);
@@ -557,13 +596,15 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
   if (hists) {
     std::vector<HGeo> geo((size_t)ng);
     HIPCHK(hipMemcpy(geo.data(), d_geo, sizeof(HGeo) * ng, hipMemcpyDeviceToHost));
-    std::vector<unsigned long long> h((size_t)NBMAX);
+    std::vector<unsigned long long> h;
     for (int k = 0; k < ng; k++) {
       hists[k] = nullptr;
       const HGeo &g = geo[(size_t)k];
       if (!g.ok) continue;
       rsg_hist *H = rsg_hist_create(bmin, g.bmax, g.w);
-      HIPCHK(hipMemcpy(h.data(), d_hist + (size_t)k * NBMAX, sizeof(unsigned long long) * g.nb, hipMemcpyDeviceToHost));
+      if (!H) return rsg_fail(errbuf, "out of memory");
+      h.resize((size_t)g.nb);
+      HIPCHK(hipMemcpy(h.data(), d_hist + g.off, sizeof(unsigned long long) * g.nb, hipMemcpyDeviceToHost));
       for (int b = 0; b < g.nb; b++) H->obs[b] = h[(size_t)b];
       H->n = H->Nc = H->No = (uint64_t)g.npairs;
       H->imin = H->cmin = g.imin; H->imax = g.imax; H->xmin = g.xmin; H->xmax = g.xmax;

@@ -310,20 +310,29 @@ extern "C" int rsg_corr_GetCOV(rsg_ctx *c, rsg_msa *m, double *COV, double *minC
 // Wait for the completion stamp of the chain's last kernel in pinned memory (the result block is
 // complete once it shows); true when it was seen.  The stream is polled between spins, so a chain
 // that ends without writing the stamp (a failed launch) falls through to the caller's stream
-// synchronisation, which reports the failure; so does a chain still running after 2 ms (large
-// alignments), which then waits there instead of keeping a core spinning.
+// synchronisation, which reports the failure; so does a chain still running after
+// stamp_wait_us (2 ms by default: large alignments), which then waits there instead of keeping
+// a core spinning.  stamp_wait_us = 0 skips the spin.
 static bool wait_stamp(rsg_ctx *c)
 {
+  if (c->stamp_wait_us <= 0) return false;
   const unsigned long long want = c->hstamp;
   unsigned long long *slot = (unsigned long long *)(c->h_pinned + 63);
   const auto t0 = std::chrono::steady_clock::now();
+  const auto limit = std::chrono::microseconds(c->stamp_wait_us);
   for (unsigned spin = 1;; spin++) {
     if (__atomic_load_n(slot, __ATOMIC_ACQUIRE) == want) return true;
     if ((spin & 255) == 0) {
       if (hipStreamQuery(c->stream) != hipErrorNotReady) return __atomic_load_n(slot, __ATOMIC_ACQUIRE) == want;
-      if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) return false;
+      if (std::chrono::steady_clock::now() - t0 > limit) return false;
     }
   }
+}
+
+extern "C" int rsg_ctx_set_stamp_wait(rsg_ctx *c, int microseconds)
+{
+  c->stamp_wait_us = microseconds < 0 ? 0 : microseconds;
+  return RSG_OK;
 }
 
 extern "C" int rsg_cov_Calculate(rsg_ctx *c, rsg_msa *m, int covtype, int covclass, int counts_mode, int prec,

@@ -65,6 +65,110 @@ template <typename T> __device__ __forceinline__ T tlog(T v);
 template <> __device__ __forceinline__ double tlog<double>(double v) { return log(v); }
 template <> __device__ __forceinline__ float tlog<float>(float v) { return logf(v); }
 
+// phi(r) = r ln r - r + 1 >= 0 for r = pp / q > 0, given d = r - 1 (formed from an accurate
+// difference) and rr = r (formed as a ratio, never 0).  Near r = 1 the direct form cancels, so
+// for |v| < 1/4, v = (r - 1) / (r + 1), it uses Loader's series (C. Loader, "Fast and accurate
+// computation of binomial probabilities", 2000: x ln(x/m) - x + m = (x - m) v + 2 x sum_k
+// v^(2k+1) / (2k + 1)) through v^13 (truncation < 2e-8 relative); elsewhere the direct form loses
+// at most a factor 4.6 to cancellation.  Both branches are evaluated (no divergence).
+__device__ __forceinline__ float phi_ratio(float d, float rr)
+{
+  const float r = 1.f + d;
+  const float v = d / (2.f + d);
+  const float v2 = v * v;
+  float s = 1.f / 13.f;
+  s = s * v2 + 1.f / 11.f; s = s * v2 + 1.f / 9.f; s = s * v2 + 1.f / 7.f; s = s * v2 + 1.f / 5.f;
+  s = s * v2 + 1.f / 3.f;
+  const float ser = d * v + 2.f * r * (v * v2) * s;
+  const float dir = rr * logf(rr) - (rr - 1.f);
+  return (fabsf(v) < 0.25f) ? ser : dir;
+}
+
+// FP32 statistics (STAT_FP32): the sums that cancel are formed in FP64, the transcendental and
+// nonlinear parts in FP32.  Per cell k: pp_k and q_k = pm_i[x] pm_j[y] and delta_k = pp_k - q_k in
+// FP64; then
+//   pp ln(pp / q) = q phi(pp / q) + delta,  phi(r) = r ln r - r + 1 >= 0,
+// so GT / MI / MIg / MIr become sum_k q_k phi_k (FP32, every term >= 0: no cancellation) plus
+// sum_k delta_k (FP64, ~0 for the full 4 x 4 table) -- the same quantity as the reference's
+// sum pp (ln pp - ln pm_i - ln pm_j), without the cancellation of its large terms.  CHI and OMES
+// are sums of squares of delta (FP32).  The reference's cell conditions are kept (q > 0,
+// pp > 0; C2's pooled cells; CWC's allowed cells).  CCF has no logarithm: FP64.
+template <int TYPE, int CLS>
+__device__ double pair_stat_f32(const StatArgs &A, const double *cd, int64_t i, int64_t j)
+{
+#pragma clang fp contract(off)
+  double ns = 0.;
+#pragma unroll
+  for (int k = 0; k < 16; k++) ns += cd[k];
+  const int type = TYPE >= 0 ? TYPE : A.type;
+  const int cls = CLS >= 0 ? CLS : A.cls;
+  double pmi[4], pmj[4];
+#pragma unroll
+  for (int x = 0; x < 4; x++) { pmi[x] = A.pm[i * 4 + x]; pmj[x] = A.pm[j * 4 + x]; }
+  if (type == RSG_CCF) {
+    double sa = 0., sb = 0.;
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      const double a = ns * pmi[x] - A.meanp[x], b = ns * pmj[x] - A.meanp[x];
+      sa += a * a; sb += b * b;
+    }
+    return sqrt(sa * sb);
+  }
+  const double rden = 1. / (ns + 16e-10);
+  const bool logs = (type == RSG_GT || type == RSG_MI || type == RSG_MIg || type == RSG_MIr);
+  // one "cell" of the sums: probability p, expectation q (both FP64)
+  float sphi = 0.f, ssq = 0.f, sh = 0.f;
+  double sdel = 0.;
+  auto cell = [&](double p, double q) {
+    const double dl = p - q;
+    const float qf = (float)q, df = (float)dl;
+    if (logs) {
+      if (q > 0. && p > 0.) {
+        sphi += qf * phi_ratio(df / qf, (float)p / qf);
+        sdel += dl;
+      } else if (p > 0. && cls == RSG_C2 && (type == RSG_MI || type == RSG_MIg)) {
+        sphi += INFINITY;                  // C2 MI / MIg take ln q unchecked: q = 0 gives +inf
+      }
+      if (type == RSG_MIr && p > 0.) { const float pf = (float)p; sh -= pf * logf(pf); }
+    } else if (q > 0.) {
+      ssq += (type == RSG_CHI) ? df * df / qf : df * df;
+    }
+  };
+  if (cls == RSG_C2) {
+    double pw = 0., pn = 0., qw = 0., qn = 0.;
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        const double p = (cd[x * 4 + y] + 1e-10) * rden, q = pmi[x] * pmj[y];
+        if ((A.allow >> (x * 4 + y)) & 1u) { pw += p; qw += q; } else { pn += p; qn += q; }
+      }
+    if (type == RSG_CHI || type == RSG_OMES) {
+      if (!(ns > 0.)) return 0.;
+      cell(pw, qw); cell(pn, qn);
+      return ns * (double)ssq;
+    }
+    if (type == RSG_GT && !(ns > 0.)) return 0.;
+    cell(pw, qw); cell(pn, qn);
+  } else {
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        if (cls == RSG_CWC && !((A.allow >> (x * 4 + y)) & 1u)) continue;
+        cell((cd[x * 4 + y] + 1e-10) * rden, pmi[x] * pmj[y]);
+      }
+    if (type == RSG_CHI || type == RSG_OMES) return (ns > 0.) ? ns * (double)ssq : 0.;
+  }
+  const double s = (double)sphi + sdel;                    // sum pp ln(pp / q)
+  switch (type) {
+  case RSG_GT:  return (ns > 0.) ? 2. * ns * s : 0.;
+  case RSG_MI:  return s;
+  case RSG_MIg: return s - ((ns > 0.) ? (A.wsum - ns) / ns : 0.);
+  default:      return (sh > 1e-2f) ? s / (double)sh : 0.;  // MIr
+  }
+}
+
 // One pair. c: counts, pmi/pmj: marginals.  Returns the statistic (see the file header).
 // TYPE / CLS >= 0 fix the statistic and class at compile time (-1: read them from A).
 // tabc: ln(c + 1e-10) for counts c = 0..N (A.lgc, or a copy of it in LDS) when TAB; A.lgc holds
@@ -73,6 +177,7 @@ template <typename T, bool TAB, int TYPE = -1, int CLS = -1>
 __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int64_t j, const double *tabc)
 {
 #pragma clang fp contract(off)   // no FMA: every specialisation and the batched path round alike
+  if constexpr (sizeof(T) == sizeof(float)) return pair_stat_f32<TYPE, CLS>(A, cd, i, j);
   T ns = 0;
   double nsd = 0.;
 #pragma unroll

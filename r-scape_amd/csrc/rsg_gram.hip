@@ -688,7 +688,11 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, double *pm_part, int *pm
   constexpr int BI = 32 * WI, BJ = 32 * WJ;
   if (m->Lpad % BI || m->Lpad % BJ || m->Npad % KT) return rsg_fail(errbuf, "gram: padding does not fit the tile");
   int ntiles = 0;
-  const int2 *tiles = tile_list(c->device, m->L, BI, BJ, &ntiles, errbuf);
+  const int2 *tiles;
+  if (c->tiles_fixed) {                      // an allocation-free call brings its own tile list
+    if (BI != 32 || BJ != 32) return rsg_fail(errbuf, "gram: a fixed tile list needs 32 x 32-column tiles");
+    tiles = c->tiles_fixed; ntiles = c->ntiles_fixed;
+  } else tiles = tile_list(c->device, m->L, BI, BJ, &ntiles, errbuf);
   if (!tiles) return RSG_FAIL;
   if (ntiles == 0) return RSG_OK;
   if (OK == CNT_U16 && m->N > RSG_U16_MAXN) return rsg_fail(errbuf, "gram: uint16 records hold up to %d sequences", RSG_U16_MAXN);
@@ -709,6 +713,7 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, double *pm_part, int *pm
   uint8_t *part = nullptr;
   if (nrem > 0) {
     const size_t need = (size_t)nsu * BI * BJ * rsg_rec_bytes(OK);
+    if (c->gscr_cap < need && c->no_alloc) return rsg_fail(errbuf, "gram: split-K partials need an allocation");
     if (c->gscr_cap < need) {
       if (c->d_gscr) hipFree(c->d_gscr);
       c->d_gscr = nullptr; c->gscr_cap = 0;
@@ -793,7 +798,11 @@ static int launch_bf16(rsg_ctx *c, rsg_msa *m, float *out, char *errbuf)
   constexpr int BI = 32 * WI, BJ = 32 * WJ;
   if (m->Lpad % BI || m->Lpad % BJ || m->Npad % KT) return rsg_fail(errbuf, "gram: padding does not fit the tile");
   int ntiles = 0;
-  const int2 *tiles = tile_list(c->device, m->L, BI, BJ, &ntiles, errbuf);
+  const int2 *tiles;
+  if (c->tiles_fixed) {                      // an allocation-free call brings its own tile list
+    if (BI != 32 || BJ != 32) return rsg_fail(errbuf, "gram: a fixed tile list needs 32 x 32-column tiles");
+    tiles = c->tiles_fixed; ntiles = c->ntiles_fixed;
+  } else tiles = tile_list(c->device, m->L, BI, BJ, &ntiles, errbuf);
   if (!tiles) return RSG_FAIL;
   if (ntiles == 0) return RSG_OK;
   int per = (ntiles + 7) / 8;

@@ -91,7 +91,13 @@ struct rsg_ctx {
   // completion stamp: the min / max reduction writes hstamp to h_pinned[63] after the result
   // block (system-scope release), so the caller can wait for it without a stream synchronisation
   unsigned long long hstamp = 0;
+  int          stamp_wait_us = 2000;    // spin on the stamp this long, then synchronise the stream
   void        *d_small = nullptr;       // 64 KB of device scratch for small per-call tables
+  // set for the duration of an allocation-free call (rsg_noalloc.hip): the Gram takes this
+  // 32 x 32-column tile list instead of its cached one, and d_red is not grown past red_cap
+  const int2  *tiles_fixed = nullptr;
+  int          ntiles_fixed = 0;
+  int          no_alloc = 0;
 };
 
 struct rsg_msa {
@@ -116,6 +122,7 @@ struct rsg_msa {
   double  *d_pm = nullptr;      // L x 4
   double  *d_ps = nullptr;      // L x 5
   double  *d_lgc = nullptr;     // log table for integer counts, N+2 entries
+  int64_t  lgc_n = -1;          // the N the log table holds (rebuilt when it differs)
   double  *d_cov = nullptr;     // L x L (FP64)
   double  *d_tmp = nullptr;     // L x L scratch (RAFS source)
   double  *d_rowx = nullptr;    // L + 1 (row means, then COVavg)
@@ -157,6 +164,7 @@ static inline void rsg_tstop(rsg_ctx *c, int slot)
 }
 
 // ---- kernel launchers implemented in the .hip files ----
+int rsg_noalloc_init(int device, char *errbuf);   // rsg_noalloc.hip: the scratch array's address
 int rsg_pack(rsg_ctx *c, rsg_msa *m, char *errbuf);
 // pm_part (uint16 records only; nullptr = none): marginal partials of every 32 x 32 block
 // (pm32 layout, rsg_device.h); pmflag: cleared for the marginal reduction that follows
@@ -223,7 +231,7 @@ void rsg_fitch_perm(uint64_t seed, int64_t L, int *perm);
 int rsg_fitch_launch(rsg_ctx *c, const FitchPlan &P, const FitchDev &D, int R);
 
 // Batched pipeline: per-item descriptor of RSG_BDESC int64 fields
-// {N, L, Npad, Lpad, off_xt, off_cnt, off_cov, off_pm, off_rows, off_hist, off_leaf, blk0_pack,
+// {N, L, Npad, Lpad, off_xt, off_cnt, off_cov, off_pm, off_rows, (unused), off_leaf, blk0_pack,
 //  blk0_col, blk0_stat, blk0_el, nb32, group, blk0_hist, off_pm32 (marginal partials),
 //  blk0_col16 (16-column blocks)}
 #define RSG_BDESC 20

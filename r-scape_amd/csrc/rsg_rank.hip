@@ -120,23 +120,32 @@ extern "C" int rsg_cov_RankHist(rsg_ctx *c, rsg_msa *m, double bmin, double w, d
   if (!(w > 0.)) return rsg_fail(errbuf, "histogram width must be positive (w = %g)", w);
   double bmax = m->maxCOV + 5 * w;
   while (fabs(bmax - bmin) < tol) bmax += w;
+  // bmin above every score leaves no bin (Easel's histogram would grow; the reference never
+  // asks for this, its bmin is -10 and bmax = maxCOV + 5w): an error, before any allocation
+  if (!(bmax > bmin) || !std::isfinite(bmax))
+    return rsg_fail(errbuf, "histogram range is empty (bmin %g, bmax %g)", bmin, bmax);
   rsg_hist *h = rsg_hist_create(bmin, bmax, w);
   if (!h) return rsg_fail(errbuf, "out of memory");
   const int nb = h->nb;
+  if (nb < 1) { rsg_hist_destroy(h); return rsg_fail(errbuf, "histogram has no bins (bmin %g, bmax %g, w %g)", bmin, bmax, w); }
   unsigned long long *d_obs = nullptr;
-  HIPCHK(hipMalloc(&d_obs, sizeof(unsigned long long) * (size_t)nb));
-  HIPCHK(hipMemsetAsync(d_obs, 0, sizeof(unsigned long long) * (size_t)nb, c->stream));
   const int use_lds = (nb <= 16384);          // LDS-privatised bins up to 64 KB per work group
   const int64_t L = m->L;
   int blocks = (int)std::min<int64_t>(2048, (L * L + 255) / 256);
-  int t; rsg_tstart(c, "hist", &t);
-  hipLaunchKernelGGL(k_hist, dim3(blocks), dim3(256), use_lds ? sizeof(unsigned) * nb : 0, c->stream,
-                     m->d_cov, L, bmin, w, nb, use_lds, d_obs);
-  rsg_tstop(c, t);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(h->obs, d_obs, sizeof(uint64_t) * (size_t)nb, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  hipFree(d_obs);
+  int t;
+  hipError_t e = hipMalloc(&d_obs, sizeof(unsigned long long) * (size_t)nb);
+  if (e == hipSuccess) e = hipMemsetAsync(d_obs, 0, sizeof(unsigned long long) * (size_t)nb, c->stream);
+  if (e == hipSuccess) {
+    rsg_tstart(c, "hist", &t);
+    hipLaunchKernelGGL(k_hist, dim3(blocks), dim3(256), use_lds ? sizeof(unsigned) * nb : 0, c->stream,
+                       m->d_cov, L, bmin, w, nb, use_lds, d_obs);
+    rsg_tstop(c, t);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(h->obs, d_obs, sizeof(uint64_t) * (size_t)nb, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (d_obs) hipFree(d_obs);
+  if (e != hipSuccess) { rsg_hist_destroy(h); return rsg_fail(errbuf, "rsg_cov_RankHist: %s", hipGetErrorString(e)); }
   // scalar fields: the added values are max(COV, bmin + w), so their extremes follow
   // from the COV extremes of the upper triangle
   uint64_t npairs = (uint64_t)(L * (L - 1) / 2);

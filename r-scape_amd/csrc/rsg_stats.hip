@@ -16,6 +16,7 @@
 #include <algorithm>
 
 static int ensure_red(rsg_ctx *c, size_t n, char *errbuf);
+#define RSG_MINMAX_BLOCKS 512   // partials of k_minmax_upper (RAF / RAFS extremes)
 
 // Reduction scratch large enough for every stage of one scoring of an L-column alignment
 // (sized before a deferred chain, so no stage reallocates it while earlier work is queued).
@@ -23,7 +24,8 @@ int rsg_reserve_scratch(rsg_ctx *c, int64_t L, char *errbuf)
 {
   const int64_t nbt = (L + 63) / 64, nb = (L + 31) / 32;
   const size_t need = std::max<size_t>({(size_t)(pm_npi(nbt) + pm_npj(nbt)), (size_t)(2 * pm32_n(nb)),
-                                        (size_t)(2 * nb * nb + 8 + 4 * L), (size_t)(2 * L + 8)});
+                                        (size_t)(2 * nb * nb + 8 + 4 * L), (size_t)(2 * L + 8),
+                                        (size_t)(2 * RSG_MINMAX_BLOCKS + 8)});
   return ensure_red(c, need, errbuf);
 }
 
@@ -204,10 +206,11 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   }
-  if (!m->d_lgc) {
-    HIPCHK(hipMalloc(&m->d_lgc, sizeof(double) * 3 * (size_t)(m->N + 1)));
+  if (!m->d_lgc) HIPCHK(hipMalloc(&m->d_lgc, sizeof(double) * 3 * (size_t)(m->N + 1)));
+  if (m->lgc_n != m->N) {
     hipLaunchKernelGGL(k_logtab, dim3((unsigned)((m->N + 256) / 256)), dim3(256), 0, c->stream, m->N, m->d_lgc);
     HIPCHK(hipGetLastError());
+    m->lgc_n = m->N;
   }
   return RSG_OK;
 }
@@ -235,6 +238,19 @@ static int launch_stat_fixed(rsg_ctx *c, const StatArgs &A, int cls, int nb)
   const dim3 grid((unsigned)ntri);
   if (cls == RSG_C16) hipLaunchKernelGGL((k_stat<double, CNT_U16, true, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
   else                hipLaunchKernelGGL((k_stat<double, CNT_U16, true, TYPE, RSG_C2>), grid, dim3(256), 0, c->stream, A, nb);
+  return ntri;
+}
+
+// FP32 statistics (pair_stat_f32) of class C16 with the statistic fixed at compile time, from
+// uint16 or FP32 (weighted) records; 0 when not handled here.
+template <int TYPE>
+static int launch_stat_f32_fixed(rsg_ctx *c, const StatArgs &A, int kind, int cls, int nb)
+{
+  if (cls != RSG_C16 || (kind != CNT_U16 && kind != CNT_F32)) return 0;
+  const int ntri = nb * (nb + 1) / 2;
+  const dim3 grid((unsigned)ntri);
+  if (kind == CNT_U16) hipLaunchKernelGGL((k_stat<float, CNT_U16, false, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
+  else                 hipLaunchKernelGGL((k_stat<float, CNT_F32, false, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
   return ntri;
 }
 
@@ -277,6 +293,7 @@ __global__ void __launch_bounds__(1024) k_reduce_minmax(const double *__restrict
 static int ensure_red(rsg_ctx *c, size_t n, char *errbuf)
 {
   if (c->red_cap >= n) return RSG_OK;
+  if (c->no_alloc) return rsg_fail(errbuf, "reduction scratch of %zu doubles exceeds the fixed %zu", n, c->red_cap);
   if (c->d_red) hipFree(c->d_red);
   c->d_red = nullptr;
   HIPCHK(hipMalloc(&c->d_red, sizeof(double) * n));
@@ -362,6 +379,15 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
     case RSG_OMES: nparts = launch_stat_fixed<RSG_OMES>(c, A, cls, nb); break;
     }
   }
+  if (prec == RSG_STAT_FP32) {
+    switch (base) {
+    case RSG_GT:   nparts = launch_stat_f32_fixed<RSG_GT>(c, A, m->cnt_kind, cls, nb); break;
+    case RSG_MI:   nparts = launch_stat_f32_fixed<RSG_MI>(c, A, m->cnt_kind, cls, nb); break;
+    case RSG_MIr:  nparts = launch_stat_f32_fixed<RSG_MIr>(c, A, m->cnt_kind, cls, nb); break;
+    case RSG_MIg:  nparts = launch_stat_f32_fixed<RSG_MIg>(c, A, m->cnt_kind, cls, nb); break;
+    case RSG_CHI:  nparts = launch_stat_f32_fixed<RSG_CHI>(c, A, m->cnt_kind, cls, nb); break;
+    }
+  }
   if (nparts > 0) {
   } else if (prec == RSG_STAT_FP64) {
     if (m->cnt_kind == CNT_U16) { if (tab) LAUNCH(double, CNT_U16, true); else LAUNCH(double, CNT_U16, false); }
@@ -440,7 +466,7 @@ __global__ void k_minmax_upper(const double *__restrict__ cov, int64_t L, double
 
 int rsg_minmax(rsg_ctx *c, rsg_msa *m, char *errbuf)
 {
-  const int nblk = 512;
+  const int nblk = RSG_MINMAX_BLOCKS;
   if (ensure_red(c, 2 * nblk + 8, errbuf) != RSG_OK) return RSG_FAIL;
   hipLaunchKernelGGL(k_minmax_upper<0>, dim3(nblk), dim3(256), 0, c->stream, m->d_cov, m->L, c->d_red);
   HIPCHK(hipGetLastError());

@@ -56,6 +56,7 @@ def _declare(L):
         "rsg_gram_shape_name": (cp, [i32]),
         "rsg_ctx_set_gram_shape": (i32, [vp, i32]),
         "rsg_ctx_set_gram_split": (i32, [vp, i32]),
+        "rsg_ctx_set_stamp_wait": (i32, [vp, i32]),
         "rsg_msa_create": (i32, [vp, vp, vp, i64, i64, C.POINTER(vp), cp]),
         "rsg_msa_update": (i32, [vp, vp, vp, vp, cp]),
         "rsg_msa_destroy": (None, [vp, vp]),
@@ -102,7 +103,9 @@ class Hist(C.Structure):
                 ("obs", C.POINTER(C.c_uint64))]
 
     def counts(self):
-        return np.array([self.obs[b] for b in range(self.nb)], dtype=np.uint64)
+        if self.nb <= 0:
+            return np.zeros(0, dtype=np.uint64)
+        return np.ctypeslib.as_array(self.obs, shape=(self.nb,)).astype(np.uint64, copy=True)
 
 
 class Histogram:
@@ -241,6 +244,11 @@ class Context:
 
     def set_gram_split(self, on):
         self.L.rsg_ctx_set_gram_split(self.h, 1 if on else 0)
+
+    def set_stamp_wait(self, microseconds):
+        """How long a scoring call spins on its completion stamp before it synchronises the
+        stream instead (0: always synchronise)."""
+        self.L.rsg_ctx_set_stamp_wait(self.h, int(microseconds))
 
     def msa(self, ax, wgt=None):
         return Msa(self, ax, wgt)

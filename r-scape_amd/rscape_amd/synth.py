@@ -134,6 +134,36 @@ def plant_pairs(L, npairs, rng, hmin=4, hmax=7, loopmin=3):
     return ct
 
 
+def plant_crossing(ct, npairs, rng, hmin=4, hmax=7, loopmin=3, tries=100000):
+    """Pseudoknotted base pairs added to a nested ct: helices of hmin..hmax bp between free
+    columns, each crossing at least one pair already in ct (i < k < j < l or k < i < l < j).
+    Returns (ct, list of the added (i, j) pairs)."""
+    ct = ct.copy()
+    L = len(ct)
+    added = []
+    for _ in range(tries):
+        if len(added) >= npairs:
+            break
+        h = min(int(rng.integers(hmin, hmax + 1)), npairs - len(added))
+        a = int(rng.integers(0, L - h))
+        b = int(rng.integers(a + 2 * h + loopmin - 1, L)) if a + 2 * h + loopmin - 1 < L else -1
+        if b < 0:
+            continue
+        five = list(range(a, a + h))
+        three = list(range(b, b - h, -1))
+        if any(ct[k] >= 0 for k in five + three):
+            continue
+        ks = np.nonzero(ct > np.arange(L))[0]             # 5' ends of the pairs so far
+        ls = ct[ks]
+        i, j = a, b                                       # the helix's outermost pair
+        if not np.any(((i < ks) & (ks < j) & (j < ls)) | ((ks < i) & (i < ls) & (ls < j))):
+            continue
+        for i, j in zip(five, three):
+            ct[i], ct[j] = j, i
+            added.append((i, j))
+    return ct, added
+
+
 def evolve(tree, L, rng, pi=(0.25, 0.25, 0.25, 0.25), ct=None, exch=(1.0, 2.0, 1.0, 1.0, 2.0, 1.0)):
     """Evolve a root sequence down the tree; returns (all sequences (2N-1) x L, leaves N x L).
     Rows 0..N-2 are internal nodes, row N-1+l is leaf l (cov_simulate.c:288-324 layout)."""
@@ -196,7 +226,8 @@ def config(cfg, seed=42, N=None, L=None):
     """Alignments of the benchmark configurations (sizes can be overridden):
     0: N=200 L=120, 30 pairs, mean branch 0.1, 5% i.i.d. gaps, uniform pi;
     1: N=5000 L=1500, 400 pairs, pi A.25 C.22 G.30 U.23, mean branch 0.15, 10% gaps in runs (mean 4);
-    4: N=20000 L=5000, 1200 pairs (+ crossing pairs), mean branch 0.2, 15% gaps in runs."""
+    4: N=20000 L=5000, 1200 nested pairs + 40 pseudoknotted (crossing) pairs, mean branch 0.2,
+       15% gaps in runs."""
     rng = np.random.default_rng(seed)
     if cfg == 0:
         N, L, npairs, bl, gap, run, pi = N or 200, L or 120, 30, 0.1, 0.05, 1.0, (0.25,) * 4
@@ -209,9 +240,12 @@ def config(cfg, seed=42, N=None, L=None):
     npairs = min(npairs, max(0, (L - 3) // 3))
     tree = yule_tree(N, rng, bl)
     ct = plant_pairs(L, npairs, rng)
+    pk = []
+    if cfg == 4:                                   # 40 pseudoknotted pairs across the nested ones
+        ct, pk = plant_crossing(ct, min(40, max(0, (L - 3) // 20)), rng)
     _, leaves = evolve(tree, L, rng, pi, ct)
     ax = add_gaps(leaves, gap, rng, run)
-    return dict(ax=ax, ct=ct, tree=tree)
+    return dict(ax=ax, ct=ct, tree=tree, pk=pk)
 
 
 def random_msa(N, L, rng, gap=0.1, other=0.0):

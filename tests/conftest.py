@@ -11,6 +11,14 @@ for p in (os.path.join(ROOT, "r-scape_amd"), os.path.join(ROOT, "tests")):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
+    # the oracle's count loops may use the host's cores (at most 16: one GPU's CPU share);
+    # results do not depend on the thread count
+    import oracle
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    oracle.set_threads(min(16, n))
 
 
 def _has_gpu():

@@ -38,6 +38,11 @@ def lib():
     return _lib
 
 
+def set_threads(n):
+    """Threads of the oracle's pair-count loops (results are identical for any count)."""
+    lib().ref_set_threads(int(n))
+
+
 def P(a):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -85,6 +90,24 @@ def raf(ax, rafs=False, allowpair=None):
     COV, mm = np.zeros(L * L), np.zeros(2)
     f = lib().ref_rafs if rafs else lib().ref_raf
     f(P(ax), C.c_int64(N), C.c_int64(L), P(ap), P(COV), P(mm))
+    return COV, mm
+
+
+def raf_counts(cnt, N, L, allowpair=None):
+    """RAF from unweighted pair counts (closed form), as a symmetric matrix."""
+    from rscape_amd import ALLOW_WCGU
+    cnt = np.ascontiguousarray(cnt, dtype=np.int32)
+    ap = np.ascontiguousarray(ALLOW_WCGU if allowpair is None else allowpair, dtype=np.float64)
+    COV, mm = np.zeros(L * L), np.zeros(2)
+    lib().ref_raf_counts(P(cnt), C.c_int64(N), C.c_int64(L), P(ap), P(COV), P(mm))
+    return COV, mm
+
+
+def rafs_from(B, L):
+    """The RAFS stencil over a RAF matrix B (L*L, both triangles)."""
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    COV, mm = np.zeros(L * L), np.zeros(2)
+    lib().ref_rafs_from(P(B), C.c_int64(L), P(COV), P(mm))
     return COV, mm
 
 

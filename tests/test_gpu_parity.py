@@ -184,20 +184,42 @@ def test_raf_rafs(ctx, rafs):
     assert np.allclose([mn, mx], mm, rtol=1e-12, atol=1e-14)
 
 
-def test_fp32_statistics_error_bound(ctx):
-    """FP32 statistics: error against FP64 stated (bound 1e-3 with the near-zero floor)."""
-    N, L = 400, 60
-    ax = _msa(N, L, 51)
+@pytest.mark.parametrize("N,L,cls", [(400, 60, R.C16), (400, 60, R.C2), (5000, 160, R.C16)])
+def test_fp32_statistics_error_bound(ctx, N, L, cls):
+    """FP32 statistics (FP64 differences, FP32 logarithms): within 1e-5 of the FP64 oracle
+    (relative, with the near-zero floor), every statistic."""
+    print("fp32 stats N=%d L=%d class %d" % (N, L, cls), flush=True)
+    ax = _msa(N, L, 51 + N)
     m = ctx.msa(ax)
     m.probs(R.COUNTS_INT8)
     pr = oracle.probs(ax)
     for name, ot in STATS[:6]:
-        m.cov(name, R.C16, prec=R.STAT_FP32)
-        got, _, _ = m.get_cov()
-        ref, _ = oracle.cov(pr, ot, R.C16)
+        m.cov(name, cls, prec=R.STAT_FP32)
+        got, mn, mx = m.get_cov()
+        ref, mm = oracle.cov(pr, ot, cls)
         err = _relerr(_upper(got, L), _upper(ref, L))
         print("fp32 %s max rel err (floored) %.3e" % (name, err))
-        assert err < 1e-3
+        assert err < 1e-5, name
+        assert np.all(np.isneginf(np.diag(got)))
+
+
+def test_fp32_weighted_statistics(ctx):
+    """FP32 statistics on FP32 (split-BF16) weighted counts against FP64 statistics on the same
+    counts: the statistic adds at most 1e-5 to the count error."""
+    N, L = 700, 96
+    rng = np.random.default_rng(71)
+    ax = synth.random_msa(N, L, rng, gap=0.1)
+    w = rng.uniform(0.05, 3.0, size=N)
+    m = ctx.msa(ax, w)
+    m.probs(R.COUNTS_BF16X3)
+    for name in ("GT", "MI", "MIr", "MIg", "CHI", "OMES"):
+        m.cov(name, R.C16, prec=R.STAT_FP64)
+        ref, _, _ = m.get_cov()
+        m.cov(name, R.C16, prec=R.STAT_FP32)
+        got, _, _ = m.get_cov()
+        err = _relerr(_upper(got, L), _upper(ref, L))
+        print("weighted fp32 %s vs fp64 on the same counts: %.3e" % (name, err))
+        assert err < 1e-5, name
 
 
 @pytest.mark.parametrize("mode,bound", [(R.COUNTS_BF16, 2e-5), (R.COUNTS_BF16X3, 1e-6)])
@@ -227,18 +249,41 @@ def test_weighted_counts_split_bf16(ctx, mode, bound):
     assert e < 50 * bound
 
 
-def test_full_pipeline_calculate(ctx):
+PIPE = [("GTp", oracle.GT, R.APC), ("GT", oracle.GT, None), ("GTa", oracle.GT, R.ASC),
+        ("MIp", oracle.MI, R.APC), ("CCFp", oracle.CCF, R.APC), ("RAF", oracle.RAF, None),
+        ("RAFS", oracle.RAFS, None), ("CHIa", oracle.CHI, R.ASC)]
+
+
+@pytest.mark.parametrize("covtype,otype,corr", PIPE)
+@pytest.mark.parametrize("wait", [2000, 0])
+def test_full_pipeline_calculate(ctx, covtype, otype, corr, wait):
+    """rsg_cov_Calculate (one call: counts, probabilities, statistic, correction, extremes) for
+    every way its chain ends -- the correction's reduction writing the result block (APC, ASC),
+    the statistic's own extremes (no correction), RAF / RAFS (their min / max pass), CCF (a host
+    step mid-chain) -- with the completion stamp waited for, and with the stamp wait off (the
+    stream synchronisation fallback)."""
+    print("pipeline %s wait %d" % (covtype, wait), flush=True)
     N, L = 200, 120
     d = synth.config(0, seed=3)
     ax = d["ax"]
-    m = ctx.msa(ax)
-    mn, mx = m.calculate("GTp")
-    got, _, _ = m.get_cov()
-    pr = oracle.probs(ax)
-    ref, _ = oracle.cov(pr, oracle.GT, R.CSELECT)
-    ref2, mm = oracle.corrected(ref, L, R.APC)
-    assert _relerr(_upper(got, L), _upper(ref2, L)) < 1e-10
-    assert np.allclose([mn, mx], mm, rtol=1e-10)
+    ctx.set_stamp_wait(wait)
+    try:
+        m = ctx.msa(ax)
+        mn, mx = m.calculate(covtype)
+        got, gmn, gmx = m.get_cov()
+    finally:
+        ctx.set_stamp_wait(2000)
+    if otype in (oracle.RAF, oracle.RAFS):
+        ref, mm = oracle.raf(ax, otype == oracle.RAFS)
+    else:
+        pr = oracle.probs(ax)
+        ref, mm = oracle.cov(pr, otype, R.CSELECT)
+    if corr is not None:
+        ref, mm = oracle.corrected(ref, L, corr)
+    assert _relerr(_upper(got, L), _upper(ref, L)) < 1e-10
+    assert np.allclose([mn, mx], mm, rtol=1e-10, atol=1e-12)
+    assert (gmn, gmx) == (mn, mx)
+    assert np.all(np.isneginf(np.diag(got)))
 
 
 def test_tutorial_golden_gtp(ctx):

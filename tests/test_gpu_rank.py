@@ -115,3 +115,14 @@ def test_null_file_round_trip(ctx, tmp_path):
     assert back.h.n == cum.h.n and back.h.Nc == cum.h.Nc
     a = cum.h.counts(); b = back.h.counts()
     assert np.array_equal(a[a > 0], b[b > 0])          # same nonzero counts, in order
+
+
+def test_rank_hist_rejects_empty_range(ctx):
+    """bmin above every score leaves the histogram without bins: a clear error before any
+    allocation or launch, and the context keeps working."""
+    ax = synth.config(0, seed=5)["ax"]
+    m, cov, mn, mx = scored(ctx, ax)
+    with pytest.raises(R.RscapeError, match="empty|no bins"):
+        m.rank_hist(mx + 100.0, 0.05)
+    g = m.rank_hist(-10.0, R.width(mx, mn))
+    assert int(g.h.counts().sum()) == ax.shape[1] * (ax.shape[1] - 1) // 2

@@ -173,6 +173,22 @@ def test_raf_closed_form_from_counts():
             assert np.isclose(rafs[i, j], 0.25 * v)
 
 
+@pytest.mark.parametrize("N,L,seed", [(25, 8, 5), (2, 5, 6), (40, 13, 7), (1, 4, 8)])
+def test_raf_counts_matches_literal_loop(N, L, seed):
+    """The oracle's count-based RAF (used where the literal loop cannot finish) equals the literal
+    O(N^2) loop, and its RAFS stencil equals ref_rafs."""
+    rng = np.random.default_rng(seed)
+    ax = synth.random_msa(N, L, rng, gap=0.2, other=0.05)
+    lit, mm = oracle.raf(ax)
+    got, gm = oracle.raf_counts(oracle.pair_counts(ax), N, L)
+    off = ~np.eye(L, dtype=bool)
+    assert np.allclose(got.reshape(L, L)[off], lit.reshape(L, L)[off], rtol=1e-12, atol=1e-14)
+    assert np.allclose(gm, mm, rtol=1e-12, atol=1e-14)
+    lrafs, _ = oracle.raf(ax, rafs=True)
+    grafs, _ = oracle.rafs_from(got, L)
+    assert np.allclose(grafs.reshape(L, L)[off], lrafs.reshape(L, L)[off], rtol=1e-12, atol=1e-14)
+
+
 def test_tutorial_gtp_scores():
     """Published GTp scores of tutorial/updated_Arisong.sto (documentation/tutorial.tex:202-212)."""
     names, ax, _ = stockholm.read(os.path.join(HERE, "golden", "updated_Arisong.sto"))
