@@ -117,6 +117,23 @@ int rsg_cov_Calculate(rsg_ctx *ctx, rsg_msa *msa, int covtype, int covclass, int
                       int stat_precision, const double *allowpair, int nseqthresh, int alenthresh,
                       double tol, double *minCOV, double *maxCOV, char *errbuf);
 
+/* ---- cov_Calculate without any allocation (no reference counterpart) ----
+ * For callers that cannot allocate device memory: code under HIP stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The same chain as rsg_cov_Calculate (unit weights,
+ * int8 counts; covtype CHI/GT/MI/MIr/MIg/OMES with or without APC/ASC) on an alignment already
+ * in device memory: d_ax = nseq x alen codes (row-major, device), d_COV = alen x alen FP64
+ * output (device; diagonal -inf).  Every intermediate buffer comes from one fixed device array
+ * per GPU inside the library (about 42 MB, held until the process ends), so the call makes no
+ * hipMalloc / hipFree: at most RSG_NA_MAXN sequences and RSG_NA_MAXL columns (RSG_FAIL with a
+ * message beyond).  Calls on one GPU share that array: calls that overlap on one GPU (from two
+ * host threads or contexts) race.  Results equal rsg_cov_Calculate's. */
+#define RSG_NA_MAXN 4096
+#define RSG_NA_MAXL 1024
+int rsg_cov_Calculate_noalloc(rsg_ctx *ctx, const uint8_t *d_ax, int64_t nseq, int64_t alen, int covtype,
+                              int covclass, int stat_precision, const double *allowpair, int nseqthresh,
+                              int alenthresh, double tol, double *d_COV, double *minCOV, double *maxCOV,
+                              char *errbuf);
+
 /* ---- ranking, null histogram, tail fit, E-values (src/covariation.c:325-530, 1914-1975,
  *      2369-2506; src/R-scape.c:1280-1371, 1564-1612) ----
  * Histograms follow Easel's esl_histogram binning: bin b covers (bmin + b w, bmin + (b+1) w]. */

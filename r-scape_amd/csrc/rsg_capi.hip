@@ -31,6 +31,7 @@ extern "C" rsg_ctx *rsg_ctx_create(int device, char *errbuf)
     return nullptr;
   }
   memset(c->h_pinned, 0, 64 * sizeof(double));          // completion stamps start from 1
+  if (rsg_noalloc_init(device, errbuf) != RSG_OK) { rsg_ctx_destroy(c); return nullptr; }
   return c;
 }
 
