@@ -58,8 +58,29 @@ def run_steps(ctx, m, k, breakdown=5, **kw):
     return t1 - t0, allk
 
 
-def cpu_baseline(ax, Lwin):
-    """Oracle (serial FP64 restatement of the reference loops) on the first Lwin columns."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return min(16, n)                # one GPU's share of the host (16 cores)
+
+
+def cpu_baseline(ax, Lwin, threads=1):
+    """Oracle (FP64 restatement of the reference loops: counts, probabilities, marginals, GT,
+    APC) on the first Lwin columns, with `threads` threads over the column-pair rows (the
+    reference's own parallelism is independent runs over replicates or alignments; the rows of
+    one scoring are as independent)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     try:
         import oracle
@@ -68,15 +89,40 @@ def cpu_baseline(ax, Lwin):
         return {"value": None, "error": str(e)}
     sub = np.ascontiguousarray(ax[:, :Lwin])
     N = sub.shape[0]
+    oracle.set_threads(threads)
     t0 = time.perf_counter()
     pr = oracle.probs(sub)
     cv, _ = oracle.cov(pr, oracle.GT, oracle.CSELECT)
     oracle.corrected(cv, Lwin, oracle.APC)
     dt = time.perf_counter() - t0
+    oracle.set_threads(1)
     upd = N * Lwin * (Lwin - 1) / 2
-    return {"value": upd / dt, "unit": "column-pair*seq updates/s", "cores": 1, "kind": "port",
+    return {"value": upd / dt, "unit": "column-pair*seq updates/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
             "sample": "GT+APC (counts, probabilities, marginals, GT, APC) on columns 0..%d of the same "
-                      "N=%d alignment, %.1f s" % (Lwin - 1, N, dt)}
+                      "N=%d alignment, %.1f s, %d thread(s)" % (Lwin - 1, N, dt, threads)}
+
+
+def uneven_null_batch(ctx, reps=3):
+    """The batched null of 1000 unstructured alignments of uneven size (L uniform 80-600, N
+    log-uniform 50-4000, own Yule trees, mean branch length 0.05-0.3): simulate, score GTp,
+    histogram and exponential tail fit per alignment, all in one call; eager and as one captured
+    graph (capture included in the call)."""
+    from rscape_amd import null as NL
+    jobs = NL.uneven_jobs(1000, seed=42)
+    upd = float(sum(j.tree["n"] * j.L * (j.L - 1) / 2 for j in jobs))
+    out = {"workload": "1000 alignments, L ~ U(80,600), N ~ logU(50,4000), GTp, one group each",
+           "updates": upd}
+    for name, graph in (("eager", 0), ("graph", 1)):
+        NL.batch(ctx, jobs, "GTp", pmass=5e-4, graph=graph)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            NL.batch(ctx, jobs, "GTp", pmass=5e-4, graph=graph)
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        out[name] = {"s_per_batch": t, "alignments_per_s": 1000.0 / t, "updates_per_s": upd / t}
+    return out
 
 
 def evalue_runs(ctx, d, K=20, reps=3, acc_nulls=16):
@@ -287,6 +333,9 @@ def main():
     ap.add_argument("--calib", action="store_true", help="also run the counter calibration kernel")
     ap.add_argument("--no-evalue", action="store_true", help="skip the significance-run timing")
     ap.add_argument("--evalue-reps", type=int, default=3)
+    ap.add_argument("--stat", choices=("fp32", "fp64"), default="fp64",
+                    help="arithmetic of the per-pair statistics in the timed step")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 1000-alignment null batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path "
@@ -300,7 +349,8 @@ def main():
 
     ctx = R.Context(0)
     m = ctx.msa(ax)
-    kw = dict(covtype="GTp", counts=R.COUNTS_INT8, prec=R.STAT_FP64)
+    prec = R.STAT_FP32 if args.stat == "fp32" else R.STAT_FP64
+    kw = dict(covtype="GTp", counts=R.COUNTS_INT8, prec=prec)
     for _ in range(args.warmup):
         m.calculate(**kw)
     dt, kav = run_steps(ctx, m, args.steps, **kw)
@@ -323,9 +373,9 @@ def main():
         "vs_baseline": None,
         "dtype": "int8",
         "data": "synthetic",
-        "config": {"workload": "N=%d x L=%d synthetic rRNA-like MSA, GTp (unit weights, int8 counts, FP64 "
-                               "statistics), one full scoring per step" % (N, L),
-                   "N": N, "L": L, "covtype": "GTp", "counts": "int8 one-hot Gram", "statistic": "FP64"},
+        "config": {"workload": "N=%d x L=%d synthetic rRNA-like MSA, GTp (unit weights, int8 counts, %s "
+                               "statistics), one full scoring per step" % (N, L, args.stat.upper()),
+                   "N": N, "L": L, "covtype": "GTp", "counts": "int8 one-hot Gram", "statistic": args.stat.upper()},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
                      "frac": achieved / PEAK_INT8_TOPS,
                      "traffic": load_traffic("gram_i8_n5000_l1500_traffic.txt"),
@@ -339,21 +389,25 @@ def main():
         rng = np.random.default_rng(1)
         w = rng.uniform(0.2, 2.0, size=N)
         mw = ctx.msa(ax, w)
-        kw2 = dict(covtype="GTp", counts=R.COUNTS_BF16, prec=R.STAT_FP64)
-        mw.calculate(**kw2)
-        dtw, kw_av = run_steps(ctx, mw, max(3, args.steps // 4), **kw2)
-        b_ms = kw_av.get("gram_bf16", (float("nan"), 0))[0]
-        extra["weighted_bf16_hilo"] = {"value": upd * max(3, args.steps // 4) / dtw, "unit": "updates/s",
-                                       "gram_bf16_ms": b_ms,
-                                       "gram_bf16_tflops": 2 * 2 * 16 * upd / (b_ms * 1e-3) / 1e12,
-                                       "peak_tflops": PEAK_BF16_TFLOPS}
+        for sname, sprec in (("fp32", R.STAT_FP32), ("fp64", R.STAT_FP64)):
+            kw2 = dict(covtype="GTp", counts=R.COUNTS_BF16, prec=sprec)
+            mw.calculate(**kw2)
+            dtw, kw_av = run_steps(ctx, mw, max(3, args.steps // 4), **kw2)
+            b_ms = kw_av.get("gram_bf16", (float("nan"), 0))[0]
+            extra["weighted_bf16_hilo_" + sname + "_statistic"] = {
+                "value": upd * max(3, args.steps // 4) / dtw, "unit": "updates/s",
+                "gram_bf16_ms": b_ms, "gram_bf16_tflops": 2 * 2 * 16 * upd / (b_ms * 1e-3) / 1e12,
+                "peak_tflops": PEAK_BF16_TFLOPS, "stat_ms": kw_av.get("stat", (float("nan"), 0))[0]}
         mw.close()
         m = ctx.msa(ax)
-        kw3 = dict(covtype="GTp", counts=R.COUNTS_INT8, prec=R.STAT_FP32)
+        other = R.STAT_FP64 if prec == R.STAT_FP32 else R.STAT_FP32
+        kw3 = dict(covtype="GTp", counts=R.COUNTS_INT8, prec=other)
         m.calculate(**kw3)
-        dt3, _ = run_steps(ctx, m, max(3, args.steps // 4), **kw3)
-        extra["fp32_statistic"] = {"value": upd * max(3, args.steps // 4) / dt3, "unit": "updates/s",
-                                   "max_rel_err_vs_fp64": fp32_errors(ctx, ax)}
+        dt3, kav3 = run_steps(ctx, m, max(3, args.steps // 4), **kw3)
+        extra["fp64_statistic" if other == R.STAT_FP64 else "fp32_statistic"] = {
+            "value": upd * max(3, args.steps // 4) / dt3, "unit": "updates/s",
+            "stat_ms": kav3.get("stat", (float("nan"), 0))[0]}
+        extra["fp32_max_rel_err_vs_fp64"] = fp32_errors(ctx, ax)
         # PCIe-inclusive: the alignment uploaded from host memory before every scoring
         m.calculate(**kw)
         k = max(3, args.steps // 4)
@@ -366,10 +420,15 @@ def main():
         out["extra"] = extra
     if not args.no_evalue:
         out["evalue_runs"] = evalue_runs(ctx, d, reps=args.evalue_reps)
+    if not args.no_batch:
+        out["uneven_null_batch"] = uneven_null_batch(ctx)
     if args.calib:
         calibration_kernel(ctx)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ax, min(args.cpu_window, L))
+        one = cpu_baseline(ax, min(args.cpu_window, L), threads=1)
+        allc = cpu_baseline(ax, L, threads=cpu_cores())
+        allc["single_core"] = one
+        out["cpu_baseline"] = allc
     m.close()
     ctx.close()
     print(json.dumps(out))

@@ -65,24 +65,28 @@ template <typename T> __device__ __forceinline__ T tlog(T v);
 template <> __device__ __forceinline__ double tlog<double>(double v) { return log(v); }
 template <> __device__ __forceinline__ float tlog<float>(float v) { return logf(v); }
 
-// phi(r) = r ln r - r + 1 >= 0 for r = pp / q > 0, given d = r - 1 (formed from an accurate
-// difference) and rr = r (formed as a ratio, never 0).  Near r = 1 the direct form cancels, so
-// for |v| < 1/4, v = (r - 1) / (r + 1), it uses Loader's series (C. Loader, "Fast and accurate
-// computation of binomial probabilities", 2000: x ln(x/m) - x + m = (x - m) v + 2 x sum_k
-// v^(2k+1) / (2k + 1)) through v^13 (truncation < 2e-8 relative); elsewhere the direct form loses
-// at most a factor 4.6 to cancellation.  Both branches are evaluated (no divergence).
-__device__ __forceinline__ float phi_ratio(float d, float rr)
+// phi(r) = r ln r - r + 1 >= 0 for r = p / q > 0, from FP32 values of p, q and d = p - q (d formed
+// in FP64 from the unrounded values, so it carries no cancellation).  Near r = 1 the direct form
+// cancels, so for |v| < 1/4, v = (p - q) / (p + q), it uses Loader's series (C. Loader, "Fast and
+// accurate computation of binomial probabilities", 2000: x ln(x/m) - x + m = (x - m) v +
+// 2 x sum_k v^(2k+1) / (2k + 1)) through v^13 (truncation < 2e-8 relative); elsewhere the direct
+// form loses at most a factor 4.6 to cancellation.  Reciprocals and the logarithm are the
+// hardware's (v_rcp_f32, v_log_f32: about 1 ulp); both branches are evaluated (no divergence).
+__device__ __forceinline__ float phi_pq(float pf, float qf, float df)
 {
-  const float r = 1.f + d;
-  const float v = d / (2.f + d);
+  const float rq = __builtin_amdgcn_rcpf(qf);
+  const float d = df * rq, rr = pf * rq;
+  const float v = df * __builtin_amdgcn_rcpf(pf + qf);
   const float v2 = v * v;
   float s = 1.f / 13.f;
   s = s * v2 + 1.f / 11.f; s = s * v2 + 1.f / 9.f; s = s * v2 + 1.f / 7.f; s = s * v2 + 1.f / 5.f;
   s = s * v2 + 1.f / 3.f;
-  const float ser = d * v + 2.f * r * (v * v2) * s;
-  const float dir = rr * logf(rr) - (rr - 1.f);
+  const float ser = d * v + 2.f * rr * (v * v2) * s;
+  const float dir = rr * (__builtin_amdgcn_logf(rr) * 0.693147180559945309f) - (rr - 1.f);
   return (fabsf(v) < 0.25f) ? ser : dir;
 }
+// p ln p for p > 0 (FP32, hardware logarithm)
+__device__ __forceinline__ float plogp(float p) { return p * (__builtin_amdgcn_logf(p) * 0.693147180559945309f); }
 
 // FP32 statistics (STAT_FP32): the sums that cancel are formed in FP64, the transcendental and
 // nonlinear parts in FP32.  Per cell k: pp_k and q_k = pm_i[x] pm_j[y] and delta_k = pp_k - q_k in
@@ -96,7 +100,6 @@ __device__ __forceinline__ float phi_ratio(float d, float rr)
 template <int TYPE, int CLS>
 __device__ double pair_stat_f32(const StatArgs &A, const double *cd, int64_t i, int64_t j)
 {
-#pragma clang fp contract(off)
   double ns = 0.;
 #pragma unroll
   for (int k = 0; k < 16; k++) ns += cd[k];
@@ -124,14 +127,14 @@ __device__ double pair_stat_f32(const StatArgs &A, const double *cd, int64_t i, 
     const float qf = (float)q, df = (float)dl;
     if (logs) {
       if (q > 0. && p > 0.) {
-        sphi += qf * phi_ratio(df / qf, (float)p / qf);
+        sphi += qf * phi_pq((float)p, qf, df);
         sdel += dl;
       } else if (p > 0. && cls == RSG_C2 && (type == RSG_MI || type == RSG_MIg)) {
         sphi += INFINITY;                  // C2 MI / MIg take ln q unchecked: q = 0 gives +inf
       }
-      if (type == RSG_MIr && p > 0.) { const float pf = (float)p; sh -= pf * logf(pf); }
+      if (type == RSG_MIr && p > 0.) sh -= plogp((float)p);
     } else if (q > 0.) {
-      ssq += (type == RSG_CHI) ? df * df / qf : df * df;
+      ssq += (type == RSG_CHI) ? df * df * __builtin_amdgcn_rcpf(qf) : df * df;
     }
   };
   if (cls == RSG_C2) {

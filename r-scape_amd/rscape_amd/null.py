@@ -236,3 +236,19 @@ def hits(ctx, msa, thresh=0.05, cap=None):
         _check(rc, err, "rsg_cov_Hits")
         k = n.value
         return hi[:k], hj[:k], sc[:k], ev[:k]
+
+
+def uneven_jobs(n=1000, seed=42, Lrange=(80, 600), Nrange=(50, 4000), blrange=(0.05, 0.3)):
+    """The batched-null workload: n unstructured alignments of uneven size (L uniform over
+    Lrange, N log-uniform over Nrange), each on its own Yule tree with mean branch length
+    uniform over blrange and its own GTR composition, one replicate per alignment (group k)."""
+    rng = np.random.default_rng(seed)
+    Ls = rng.integers(Lrange[0], Lrange[1] + 1, size=n)
+    Ns = np.exp(rng.uniform(np.log(Nrange[0]), np.log(Nrange[1]), size=n)).astype(int)
+    bls = rng.uniform(blrange[0], blrange[1], size=n)
+    jobs = []
+    for k in range(n):
+        pi = rng.dirichlet(np.full(4, 20.0))
+        tree = yule_tree(int(Ns[k]), float(bls[k]), seed=seed * 7919 + k)
+        jobs.append(Job(tree, int(Ls[k]), gtr_model(pi), seed=seed, stream=k, gapfrac=0.0))
+    return jobs

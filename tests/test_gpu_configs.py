@@ -261,16 +261,7 @@ def test_c4_weighted_window(ctx, c4):
 # ---------------------------------------------------------------------------------- c2
 def c2_jobs(n=1000, seed=42):
     from rscape_amd import null as NL
-    rng = np.random.default_rng(seed)
-    Ls = rng.integers(80, 601, size=n)
-    Ns = np.exp(rng.uniform(np.log(50), np.log(4000), size=n)).astype(int)
-    bls = rng.uniform(0.05, 0.3, size=n)
-    jobs = []
-    for k in range(n):
-        pi = rng.dirichlet(np.full(4, 20.0))
-        tree = NL.yule_tree(int(Ns[k]), float(bls[k]), seed=seed * 7919 + k)
-        jobs.append(NL.Job(tree, int(Ls[k]), NL.gtr_model(pi), seed=seed, stream=k, gapfrac=0.0))
-    return jobs
+    return NL.uneven_jobs(n, seed)
 
 
 def test_c2_thousand_alignment_batch(ctx):
