@@ -375,3 +375,80 @@ int ref_evalues(int64_t L, const double *COV, int64_t Nc, const ref_hist *null,
     }
   return REF_OK;
 }
+
+
+/* cov_WriteNullHistogram (covariation.c:1847-1868): one "%f %llu" line per bin, the bin's lower
+ * bound bmin + i w and its count. */
+int ref_write_null(const char *path, const ref_hist *h)
+{
+  FILE *fp = fopen(path, "w");
+  if (!fp) return REF_FAIL;
+  for (int i = 0; i < h->nb; i++) fprintf(fp, "%f %llu\n", h->bmin + i * h->w, (unsigned long long)h->obs[i]);
+  fclose(fp);
+  return REF_OK;
+}
+
+/* cov_ReadNullHistogram (covariation.c:1717-1844).  Pass 1 over every line's first token: w is
+ * the smallest |score - previous score| (the first line compares with -inf), bmin / bmax the
+ * score extremes (bmax grown by w while within 1e-2 of bmin).  Pass 2: a line with exactly two
+ * tokens sets the bin Score2Bin(score) -- a written lower bound lands one bin below the bin it
+ * was written from -- to its count (atoi), growing the histogram below / above as Easel does. */
+ref_hist *ref_read_null(const char *path)
+{
+  FILE *fp = fopen(path, "r");
+  if (!fp) return NULL;
+  char line[1024];
+  double w = INFINITY, sc = -INFINITY, mx = -INFINITY, mn = INFINITY;
+  while (fgets(line, sizeof line, fp)) {
+    if (line[0] == '#') continue;
+    char *tok = strtok(line, " \t\n");
+    if (!tok) continue;
+    double prv = sc;
+    sc = atof(tok);
+    if (fabs(sc - prv) < w) w = fabs(sc - prv);
+    if (sc > mx) mx = sc;
+    if (sc < mn) mn = sc;
+  }
+  double bmax = mx, bmin = mn;
+  while (fabs(bmax - bmin) < 1e-2) bmax += w;
+  ref_hist *h = ref_hist_create(bmin, bmax, w);
+  rewind(fp);
+  while (fgets(line, sizeof line, fp)) {
+    if (line[0] == '#') continue;
+    int t = 0;
+    double x = 0.;
+    uint64_t counts = 0;
+    for (char *tok = strtok(line, " \t\n"); tok; tok = strtok(NULL, " \t\n"), t++) {
+      if (t == 0) x = atof(tok);
+      else        counts = (uint64_t)atoi(tok);
+    }
+    if (t != 2) continue;
+    int b;
+    if (ref_hist_score2bin(h, x, &b) != REF_OK) { ref_hist_destroy(h); fclose(fp); return NULL; }
+    if (b < 0) {
+      int nnew = -b * 2;
+      uint64_t *o = calloc((size_t)(nnew + h->nb), sizeof(uint64_t));
+      memcpy(o + nnew, h->obs, sizeof(uint64_t) * (size_t)h->nb);
+      free(h->obs); h->obs = o;
+      h->nb += nnew; b += nnew; h->bmin -= nnew * h->w;
+      h->imin += nnew; h->cmin += nnew;
+      if (h->imax > -1) h->imax += nnew;
+    } else if (b >= h->nb) {
+      int nnew = (b - h->nb + 1) * 2;
+      uint64_t *o = calloc((size_t)(nnew + h->nb), sizeof(uint64_t));
+      memcpy(o, h->obs, sizeof(uint64_t) * (size_t)h->nb);
+      free(h->obs); h->obs = o;
+      if (h->imin == h->nb) { h->imin += nnew; h->cmin += nnew; }
+      h->bmax += nnew * h->w;
+      h->nb   += nnew;
+    }
+    if (b < h->imin) { h->imin = b; h->cmin = b; }
+    if (b > h->imax) h->imax = b;
+    if (x < h->xmin) h->xmin = x;
+    if (x > h->xmax) h->xmax = x;
+    h->obs[b] = counts;
+    h->n += counts; h->Nc += counts; h->No += counts;
+  }
+  fclose(fp);
+  return h;
+}

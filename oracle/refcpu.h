@@ -73,6 +73,9 @@ typedef struct {
 } ref_hist;
 
 ref_hist *ref_hist_create(double bmin, double bmax, double w);
+/* the .null text format (cov_WriteNullHistogram / cov_ReadNullHistogram) */
+int       ref_write_null(const char *path, const ref_hist *h);
+ref_hist *ref_read_null(const char *path);
 void      ref_hist_destroy(ref_hist *h);
 int       ref_hist_score2bin(const ref_hist *h, double x, int *ret_b);
 double    ref_hist_lbound(const ref_hist *h, int b);

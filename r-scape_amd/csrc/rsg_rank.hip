@@ -507,11 +507,16 @@ extern "C" int rsg_hist_ReadNull(const char *path, rsg_hist **ret, char *errbuf)
     char *p = line;
     while (*p == ' ' || *p == '\t') p++;
     if (*p == '#' || *p == '\n' || *p == 0) continue;
-    char *tok1 = strtok(p, " \t\n"), *tok2 = strtok(nullptr, " \t\n");
-    if (!tok1) continue;
-    sc.push_back(atof(tok1));
-    cnt.push_back(tok2 ? (unsigned long long)atoi(tok2) : 0ull);
-    if (!tok2) cnt.back() = ~0ull;                          // marks a line without a count
+    // every line's first token is a score (it enters w and the range); only a line of exactly
+    // two tokens sets a bin, to its last token (atoi)
+    int t = 0;
+    unsigned long long c = 0ull;
+    for (char *tok = strtok(p, " \t\n"); tok; tok = strtok(nullptr, " \t\n"), t++) {
+      if (t == 0) sc.push_back(atof(tok));
+      else        c = (unsigned long long)atoi(tok);
+    }
+    if (t == 0) continue;
+    cnt.push_back(t == 2 ? c : ~0ull);                      // ~0: a line that sets no bin
   }
   fclose(fp);
   if (sc.empty()) return rsg_fail(errbuf, "empty null histogram file");

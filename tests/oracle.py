@@ -263,3 +263,16 @@ def fitch_null_rows(ax, tree, seed):
     f.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
     f(P(ax), N, L, P(left), P(right), seed, P(allr), P(sh))
     return allr, sh
+
+
+def write_null(path, h):
+    """cov_WriteNullHistogram of an oracle histogram."""
+    return lib().ref_write_null(path.encode(), h)
+
+
+def read_null(path):
+    """cov_ReadNullHistogram into an oracle histogram (pointer)."""
+    f = lib().ref_read_null
+    f.restype = C.c_void_p
+    f.argtypes = [C.c_char_p]
+    return f(path.encode())

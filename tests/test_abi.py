@@ -34,3 +34,58 @@ def test_host_histogram_entry_points_match_oracle():
     """rsg_width and rsg_cov2evalue are host-side FP64; compare with the oracle."""
     for mx, mn in [(50.0, -3.0), (0.5, 0.1), (1e-7, 0.0), (300.0, -50.0)]:
         assert R.width(mx, mn) == oracle.lib().ref_width(mx, mn, -10.0, 0.05, 400, 1e-6)
+
+
+def _lib_hist(bmin, bmax, w, counts):
+    """A library histogram (rsg_hist_create) with the given bin counts."""
+    L = R.lib()
+    p = L.rsg_hist_create(bmin, bmax, w)
+    h = p.contents
+    assert h.nb == len(counts)
+    for b, c in enumerate(counts):
+        h.obs[b] = int(c)
+    nz = np.nonzero(counts)[0]
+    h.imin, h.imax = int(nz[0]), int(nz[-1])
+    h.n = h.Nc = h.No = int(np.sum(counts))
+    return R.Histogram(p)
+
+
+def _fields(h):
+    return (h.nb, h.bmin, h.bmax, h.w, h.imin, h.imax, h.cmin, h.n, h.Nc, h.No, h.xmin, h.xmax)
+
+
+def test_null_file_format_and_read_back(tmp_path):
+    """cov_WriteNullHistogram / cov_ReadNullHistogram (covariation.c:1717-1868): one "%f %llu"
+    line per bin (its lower bound); reading back infers w as the smallest score gap and puts each
+    line's count into Score2Bin(score).  A lower bound lies on the boundary of the bin below, so
+    with exactly representable scores (w = 1/8) every count moves down one bin and the histogram
+    grows two bins below; the library's reader equals the oracle's restatement, also for w = 0.05
+    (whose printed scores round) and for lines that set no bin (one or three tokens)."""
+    rng = np.random.default_rng(4)
+    nb, w, bmin = 40, 0.125, -10.0
+    counts = rng.integers(0, 1000, size=nb).astype(np.uint64)
+    counts[[0, nb - 1]] = [3, 7]
+    h = _lib_hist(bmin, bmin + nb * w, w, counts)
+    f = str(tmp_path / "a.null")
+    h.write_null(f)
+    lines = open(f).read().splitlines()
+    assert lines == ["%f %d" % (bmin + i * w, c) for i, c in enumerate(counts)]
+    gh = R.Histogram.read_null(f)                 # keeps the library histogram alive
+    g = gh.h
+    assert g.w == w and g.nb == nb + 1 and g.bmin == bmin - 2 * w
+    got = g.counts()
+    assert got[0] == 0 and np.array_equal(got[1:], counts)       # bin i + 2 - 1: one bin lower
+    assert (g.imin, g.imax, g.n) == (1, nb, int(counts.sum()))
+    r = oracle._hp(oracle.read_null(f)).contents
+    assert _fields(g) == _fields(r) and np.array_equal(got, r.counts())
+    # rounded scores and malformed lines: library reader = oracle reader, field by field
+    counts = rng.integers(0, 50, size=300).astype(np.uint64)
+    h = _lib_hist(-10.0, -10.0 + 300 * 0.05, 0.05, counts)
+    f2 = str(tmp_path / "b.null")
+    h.write_null(f2)
+    with open(f2, "a") as fh:
+        fh.write("12.5\n3.0 4 5\n")
+    gh2 = R.Histogram.read_null(f2)
+    g, r = gh2.h, oracle._hp(oracle.read_null(f2)).contents
+    assert _fields(g) == _fields(r) and np.array_equal(g.counts(), r.counts())
+    assert int(g.n) == int(counts.sum())

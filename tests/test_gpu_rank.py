@@ -98,8 +98,15 @@ def test_evalues_match_oracle(ctx):
     # scalar helpers agree with the matrix
     i, j = np.unravel_index(np.argmax(np.where(off, cov, -np.inf)), cov.shape)
     assert np.isclose(gcum.cov2evalue(cov[i, j], Nc, gsf), ge[i, j], rtol=1e-12)
-    thr = gcum.evalue2cov(0.05, Nc, gsf)
-    assert np.isfinite(thr)
+    # evalue2cov (covariation.c:2404-2436) against the oracle's restatement, several thresholds
+    import ctypes as C
+    L_ = oracle.lib()
+    L_.ref_evalue2cov.argtypes = [C.c_double, C.c_int64, C.c_void_p, C.c_void_p]
+    for e in (1e-6, 1e-3, 0.05, 1.0, 10.0, 1e4):
+        thr = gcum.evalue2cov(e, Nc, gsf)
+        ref = L_.ref_evalue2cov(e, Nc, rcum, rsf.ctypes.data_as(C.c_void_p))
+        assert (thr == ref) or np.isclose(thr, ref, rtol=1e-12), (e, thr, ref)
+    assert np.isfinite(gcum.evalue2cov(0.05, Nc, gsf))
 
 
 def test_null_file_round_trip(ctx, tmp_path):
@@ -126,3 +133,21 @@ def test_rank_hist_rejects_empty_range(ctx):
         m.rank_hist(mx + 100.0, 0.05)
     g = m.rank_hist(-10.0, R.width(mx, mn))
     assert int(g.h.counts().sum()) == ax.shape[1] * (ax.shape[1] - 1) // 2
+
+
+def test_pm_validation_failure_is_reported(ctx):
+    """corr_ValidateProbs (correlators.c:1499-1545) on the device: a negative tolerance makes every
+    marginal fail validation; the device flag comes back as RSG_FAIL with a message from both
+    rsg_corr_Probs and rsg_cov_Calculate, and the context keeps working."""
+    ax = synth.config(0, seed=5)["ax"]
+    m = ctx.msa(ax)
+    with pytest.raises(R.RscapeError, match="pm validation"):
+        m.probs(R.COUNTS_INT8, tol=-1.0)
+    with pytest.raises(R.RscapeError, match="pm validation"):
+        m.calculate("GTp", tol=-1.0)
+    w = np.ones(ax.shape[0]); w[3] = np.nan                  # NaN weights fail the same check
+    mw = ctx.msa(ax, w)
+    with pytest.raises(R.RscapeError, match="pm validation"):
+        mw.calculate("GTp", counts=R.COUNTS_BF16)
+    mn, mx = m.calculate("GTp")
+    assert np.isfinite(mn) and np.isfinite(mx)
