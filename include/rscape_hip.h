@@ -178,6 +178,81 @@ void rsg_free(void *p);
 int  rsg_hist_WriteNull(const char *path, const rsg_hist *h, char *errbuf);
 int  rsg_hist_ReadNull(const char *path, rsg_hist **ret, char *errbuf);
 
+/* ---- substitutions on the tree and the power of base pairs to covary (the .cov and .power
+ *      columns; src/msatree.c:1423-1555, src/power.c:65-217, 532-650) ---- */
+enum { RSG_SINGLE_SUBS = 0, RSG_JOIN_SUBS = 1, RSG_DOUBLE_SUBS = 2 };   /* POWERTYPE */
+/* BPTYPE values the hit list uses (lib/R-view/src/rview_contacts.h:22-62): WWc, the first
+ * non-base-pair contact type, and none */
+enum { RSG_BP_WWC = 0, RSG_BP_STACKED = 32, RSG_BP_CONTACT = 33, RSG_BP_NONE = 34 };
+/* SAMPLESIZE (src/correlators.h:26-31) */
+enum { RSG_SAMPLE_CONTACTS = 0, RSG_SAMPLE_BP = 1, RSG_SAMPLE_WC = 2, RSG_SAMPLE_ALL = 3 };
+/* Tree_Substitutions: Fitch ancestral states on the tree (random tie-breaks keyed by seed, as
+ * rsg_null_fitch), then nsubs[c] (alen) = branches substituting at column c; ndouble / njoin
+ * (alen x alen, [i * alen + j] for i < j, 0 elsewhere) = branches substituting at both / at
+ * either of columns i and j.  Without includegaps a branch counts only where parent and child
+ * are canonical (--powergaps off).  Any output may be NULL. */
+int rsg_tree_Substitutions(rsg_ctx *ctx, const uint8_t *ax, int64_t nseq, int64_t alen, const int *left, const int *right,
+                           uint64_t seed, int includegaps, int32_t *nsubs, int32_t *ndouble, int32_t *njoin, char *errbuf);
+/* power_Read: a table of (substitutions, power) rows, two whitespace-separated columns; the
+ * power is clamped to [0, 1] and stays 1 after the first row that exceeds 1. */
+typedef struct { int ns; double *subs; double *prob; } rsg_power;
+int    rsg_power_Read(const char *path, rsg_power **ret, char *errbuf);
+void   rsg_power_Destroy(rsg_power *p);
+/* the power of a pair with nsubs substitutions: the power of the last row whose count it exceeds */
+double rsg_power_Prob(const rsg_power *p, int64_t nsubs);
+/* SPAIR (src/correlators.h / power.h): one entry per pair i < j in the order i, then j */
+typedef struct {
+  int64_t i, j, iabs, jabs;               /* analysed columns (0-based); original positions (1-based) */
+  double  nseff, ngap, wcf_f, wobble_f, noncan_f;
+  int32_t powertype;
+  int64_t nsubs, nsubs_join, nsubs_double;
+  double  power, power_join, power_double;
+  int32_t bptype_given;                   /* RSG_BP_WWC for a pair of the given structure */
+  int32_t covary;                         /* set by rsg_cov_CreateHitList */
+  double  sc, Pval, Eval;
+} rsg_spair;
+/* power_SPAIR_Create + power_SPAIR_AddMI after corr_Probs: alen (alen - 1) / 2 entries into spair;
+ * msamap[c] = original position of column c - 1 (NULL = identity); ct[c] = partner column or -1
+ * (the given structure, may be NULL); any of nsubs / njoin / ndouble (not all NULL) sets the
+ * power type as the reference does (double, else join, else single). */
+int rsg_power_SPAIR_Create(rsg_ctx *ctx, rsg_msa *msa, const int *msamap, const rsg_power *power, int powertype,
+                           const int *ct, const int32_t *nsubs, const int32_t *njoin, const int32_t *ndouble,
+                           rsg_spair *spair, char *errbuf);
+
+/* ---- hit list and output files (src/covariation.c:778-1006, 1139-1406, 1637-1673, 1870-1890;
+ *      src/power.c:261-378) ---- */
+typedef struct {
+  int64_t i, j;                           /* analysed columns, i < j */
+  double  sc, Eval, pval;
+  int64_t nsubs;
+  double  power;
+  int32_t bptype, is_compatible;
+} rsg_hit;
+/* cov_CreateHitList's selection over every pair i < j of the last scoring: pval =
+ * cov2evalue(COV, 1) against the null, E = pval * Nb for pairs counted as base pairs under
+ * samplesize, pval * expBP while fewer than expBP hits were found, else pval * Nt; a pair is a
+ * hit when E < thresh (every pair when thresh > 1000).  bptype (alen x alen, may be NULL) gives
+ * each pair's BPTYPE, else ct gives WWc for its pairs; is_compatible as the reference computes
+ * it (column i unpaired in ct).  spair (may be NULL) receives covary, sc, Pval, Eval.  *ret_hits
+ * is allocated by the library (rsg_free). */
+int rsg_cov_CreateHitList(rsg_ctx *ctx, rsg_msa *msa, const rsg_hist *null, const double *survfit, int samplesize,
+                          const int *ct, const uint8_t *bptype, int64_t Nt, int64_t Nb, double expBP, double thresh,
+                          rsg_spair *spair, rsg_hit **ret_hits, int *ret_nhit, char *errbuf);
+/* .cov (ranked = 0: cov_WriteHitList with its header) or .sorted.cov (ranked = 1:
+ * cov_WriteRankedHitList, by E-value, or by score when by_score), with the summary line
+ * "# covtype thresh [cov_min,cov_max] [FP | TP True Found | Sen PPV F]"; tnum = pairs of the
+ * structure counted under samplesize (True). */
+int rsg_cov_WriteHitList(const char *path, int ranked, int by_score, const char *covtype, double thresh,
+                         double cov_min, double cov_max, int tnum, int64_t Nb, int64_t Nt, const rsg_hit *hits,
+                         int nhit, const rsg_spair *spair, int64_t alen, const int *msamap, int firstpos,
+                         int samplesize, char *errbuf);
+/* .power: power_SPAIR_Write for the given structure's pairs (in_given = 1). */
+int rsg_power_SPAIR_Write(const char *path, const rsg_spair *spair, int64_t dim, int in_given, char *errbuf);
+/* .surv: survival curves of the base-pair and other-pair score histograms and of the null with
+ * its fit (cov_histogram_PlotSurvival); any histogram may be NULL. */
+int rsg_cov_WriteSurv(const char *path, const rsg_hist *hb, const rsg_hist *ht, const rsg_hist *null,
+                      const double *survfit, char *errbuf);
+
 /* ---- sequence weights and preprocessing (src/R-scape.c:1544-1562; src/msamanip.c:284-541) ---- */
 /* esl_msaweight_PB: position-based weights, FP64 on the device. */
 int rsg_weights_PB(rsg_ctx *ctx, const uint8_t *ax, int64_t nseq, int64_t alen, double *wgt, char *errbuf);

@@ -324,3 +324,62 @@ int ref_fitch_null_rows(const uint8_t *ax, int64_t N, int64_t L, const int *left
   free(all); free(sh); free(S); free(parent); free(perm); free(cols); free(cperm);
   return REF_OK;
 }
+
+
+/* Tree_Substitutions (src/msatree.c:1423-1555) on the Fitch reconstruction of
+ * ref_fitch_null_rows (leaves 0..N-1, internal node v at N+v): per column the branches whose
+ * child differs from the parent (both canonical unless includegaps), per pair i < j the
+ * branches differing at both columns (ndouble) or at either (njoin), [i * L + j]. */
+int ref_tree_substitutions(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right, uint64_t seed,
+                           int includegaps, int32_t *nsubs, int32_t *ndouble, int32_t *njoin)
+{
+  int64_t nn = 2 * N - 1;
+  uint8_t *all = malloc((size_t)(nn * L)), *sh = malloc((size_t)(nn * L));
+  if (!all || !sh) { free(all); free(sh); return REF_FAIL; }
+  ref_fitch_null_rows(ax, N, L, left, right, seed, all, sh);
+  free(sh);
+  const int K = 4;
+  if (nsubs) {
+    for (int64_t i = 0; i < L; i++) {
+      nsubs[i] = 0;
+      for (int v = 0; v < N - 1; v++) {
+        const uint8_t *a  = all + (N + v) * L;
+        const uint8_t *al = (left[v] > 0) ? all + (N + left[v]) * L : all + (int64_t)(-left[v]) * L;
+        const uint8_t *ar = (right[v] > 0) ? all + (N + right[v]) * L : all + (int64_t)(-right[v]) * L;
+        if (includegaps) {
+          if (al[i] != a[i]) nsubs[i]++;
+          if (ar[i] != a[i]) nsubs[i]++;
+        } else if (a[i] < K) {
+          if (al[i] < K && al[i] != a[i]) nsubs[i]++;
+          if (ar[i] < K && ar[i] != a[i]) nsubs[i]++;
+        }
+      }
+    }
+  }
+  for (int64_t i = 0; i < L; i++)
+    for (int64_t j = 0; j < L; j++) {
+      int64_t idx = i * L + j;
+      int32_t nd = 0, nj = 0;
+      if (i < j)
+        for (int v = 0; v < N - 1; v++) {
+          const uint8_t *a  = all + (N + v) * L;
+          const uint8_t *al = (left[v] > 0) ? all + (N + left[v]) * L : all + (int64_t)(-left[v]) * L;
+          const uint8_t *ar = (right[v] > 0) ? all + (N + right[v]) * L : all + (int64_t)(-right[v]) * L;
+          if (includegaps) {
+            if (al[i] != a[i] && al[j] != a[j]) nd++;
+            if (ar[i] != a[i] && ar[j] != a[j]) nd++;
+            if (al[i] != a[i] || al[j] != a[j]) nj++;
+            if (ar[i] != a[i] || ar[j] != a[j]) nj++;
+          } else if (a[i] < K && a[j] < K) {
+            if (al[i] < K && al[j] < K && al[i] != a[i] && al[j] != a[j]) nd++;
+            if (ar[i] < K && ar[j] < K && ar[i] != a[i] && ar[j] != a[j]) nd++;
+            if (al[i] < K && al[j] < K && (al[i] != a[i] || al[j] != a[j])) nj++;
+            if (ar[i] < K && ar[j] < K && (ar[i] != a[i] || ar[j] != a[j])) nj++;
+          }
+        }
+      if (ndouble) ndouble[idx] = nd;
+      if (njoin) njoin[idx] = nj;
+    }
+  free(all);
+  return REF_OK;
+}

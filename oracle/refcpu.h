@@ -118,6 +118,8 @@ int ref_evolve_pairs(int64_t nleaf, int64_t L, const int *left, const int *right
                      uint32_t stream, uint32_t gapthr, uint8_t *leaves);
 int ref_fitch_null(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
                    uint64_t seed, uint8_t *out);
+int ref_tree_substitutions(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right, uint64_t seed,
+                           int includegaps, int32_t *nsubs, int32_t *ndouble, int32_t *njoin);
 int ref_fitch_null_rows(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
                         uint64_t seed, uint8_t *all_out, uint8_t *sh_out);
 

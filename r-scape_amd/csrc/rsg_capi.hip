@@ -119,7 +119,8 @@ extern "C" int rsg_msa_update(rsg_ctx *c, rsg_msa *m, const uint8_t *ax, const d
   if (wgt) memcpy(w.data(), wgt, sizeof(double) * (size_t)m->N);
   m->weighted = (wgt != nullptr);
   m->wsum = 0.;
-  for (double v : w) m->wsum += v;
+  m->weights_ok = 1;
+  for (double v : w) { m->wsum += v; if (!(v >= 0.) || !std::isfinite(v)) m->weights_ok = 0; }
   HIPCHK(hipMemcpyAsync(m->d_w, w.data(), sizeof(double) * (size_t)m->N, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   m->nsplit_ready = 0;
@@ -183,6 +184,8 @@ extern "C" int rsg_corr_Probs(rsg_ctx *c, rsg_msa *m, int mode, double tol, char
     if (rsg_gram_counts_i8(c, m, m->d_cnt, kind, pm_part, (int *)(c->d_res + 4), errbuf) != RSG_OK) return RSG_FAIL;
     m->pm_fused = (pm_part != nullptr);
   } else if (mode == RSG_COUNTS_BF16 || mode == RSG_COUNTS_BF16X3) {
+    // a negative or non-finite weight makes pp fail corr_ValidateProbs (src/correlators.c:1499-1545)
+    if (!m->weights_ok) return rsg_fail(errbuf, "pp validation failed: sequence weights must be finite and >= 0");
     m->pm_fused = 0;
     if (alloc_counts(m, &m->d_cnt, CNT_F32, errbuf) != RSG_OK) return RSG_FAIL;
     m->cnt_kind = CNT_F32;

@@ -262,6 +262,27 @@ void rsg_fitch_perm(uint64_t seed, int64_t L, int *perm)
   }
 }
 
+// Fitch ancestral states only (Tree_FitchAlgorithmAncenstral): D.all receives the leaves' codes
+// and every internal node's state; D.mask, D.all, D.ax, the tree arrays and D.seeds are used.
+int rsg_fitch_ancestral(rsg_ctx *c, const FitchPlan &P, const FitchDev &D, int R)
+{
+  const int64_t N = P.N, L = P.L, rs = (2 * N - 1) * L;
+  const unsigned ry = (unsigned)R;
+  hipLaunchKernelGGL(k_fitch_leaves, dim3((unsigned)((N * L + 255) / 256), ry), dim3(256), 0, c->stream, D.ax, N, L,
+                     D.seeds, D.mask, D.all, rs);
+  for (int k = P.maxd; k >= 0; k--) {
+    const int nn = P.lev0[(size_t)k + 1] - P.lev0[(size_t)k];
+    hipLaunchKernelGGL(k_fitch_up, dim3((unsigned)(((int64_t)nn * L + 255) / 256), ry), dim3(256), 0, c->stream,
+                       D.nodes + P.lev0[(size_t)k], nn, N, L, D.left, D.right, D.mask, rs);
+  }
+  for (int k = 0; k <= P.maxd; k++) {
+    const int nn = P.lev0[(size_t)k + 1] - P.lev0[(size_t)k];
+    hipLaunchKernelGGL(k_fitch_down, dim3((unsigned)(((int64_t)nn * L + 255) / 256), ry), dim3(256), 0, c->stream,
+                       D.nodes + P.lev0[(size_t)k], nn, N, L, D.parent, D.seeds, D.mask, D.all, rs);
+  }
+  return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
+}
+
 int rsg_fitch_launch(rsg_ctx *c, const FitchPlan &P, const FitchDev &D, int R)
 {
   const int64_t N = P.N, L = P.L, rs = (2 * N - 1) * L;

@@ -108,6 +108,7 @@ struct rsg_msa {
   uint8_t *d_xt  = nullptr;     // Lpad x Npad one-hot nibble codes: 1<<c for c<4, else 0
   double  *d_w   = nullptr;     // N weights (FP64); all 1 when unweighted
   int      weighted = 0;
+  int      weights_ok = 1;    // every weight finite and >= 0 (else pp cannot validate)
   double   wsum = 0.;
   // split-BF16 weight parts of the weighted Gram: [3][Npad] bf16 bits
   uint16_t *d_wsplit = nullptr;
@@ -229,6 +230,7 @@ struct FitchDev {
 int rsg_fitch_plan(int64_t N, int64_t L, const int *left, const int *right, FitchPlan &P, char *errbuf);
 void rsg_fitch_perm(uint64_t seed, int64_t L, int *perm);
 int rsg_fitch_launch(rsg_ctx *c, const FitchPlan &P, const FitchDev &D, int R);
+int rsg_fitch_ancestral(rsg_ctx *c, const FitchPlan &P, const FitchDev &D, int R);
 
 // Batched pipeline: per-item descriptor of RSG_BDESC int64 fields
 // {N, L, Npad, Lpad, off_xt, off_cnt, off_cov, off_pm, off_rows, (unused), off_leaf, blk0_pack,

@@ -276,3 +276,16 @@ def read_null(path):
     f.restype = C.c_void_p
     f.argtypes = [C.c_char_p]
     return f(path.encode())
+
+
+def tree_substitutions(ax, tree, seed, includegaps=False):
+    """Tree_Substitutions: (nsubs (L,), ndouble (L, L), njoin (L, L)) on the oracle's Fitch states."""
+    ax = np.ascontiguousarray(ax, dtype=np.uint8)
+    N, L = ax.shape
+    ns = np.zeros(L, np.int32); nd = np.zeros(L * L, np.int32); nj = np.zeros(L * L, np.int32)
+    left = np.ascontiguousarray(tree["left"], dtype=np.int32); right = np.ascontiguousarray(tree["right"], dtype=np.int32)
+    f = lib().ref_tree_substitutions
+    f.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p,
+                  C.c_void_p]
+    f(P(ax), N, L, P(left), P(right), seed, int(includegaps), P(ns), P(nd), P(nj))
+    return ns, nd.reshape(L, L), nj.reshape(L, L)

@@ -138,16 +138,18 @@ def test_rank_hist_rejects_empty_range(ctx):
 def test_pm_validation_failure_is_reported(ctx):
     """corr_ValidateProbs (correlators.c:1499-1545) on the device: a negative tolerance makes every
     marginal fail validation; the device flag comes back as RSG_FAIL with a message from both
-    rsg_corr_Probs and rsg_cov_Calculate, and the context keeps working."""
+    rsg_corr_Probs and rsg_cov_Calculate, and the context keeps working.  Weights that are
+    negative or not finite fail as pp validation."""
     ax = synth.config(0, seed=5)["ax"]
     m = ctx.msa(ax)
     with pytest.raises(R.RscapeError, match="pm validation"):
         m.probs(R.COUNTS_INT8, tol=-1.0)
     with pytest.raises(R.RscapeError, match="pm validation"):
         m.calculate("GTp", tol=-1.0)
-    w = np.ones(ax.shape[0]); w[3] = np.nan                  # NaN weights fail the same check
-    mw = ctx.msa(ax, w)
-    with pytest.raises(R.RscapeError, match="pm validation"):
-        mw.calculate("GTp", counts=R.COUNTS_BF16)
+    for bad in (np.nan, -1.0, np.inf):                       # pp cannot validate with these weights
+        w = np.ones(ax.shape[0]); w[3] = bad
+        mw = ctx.msa(ax, w)
+        with pytest.raises(R.RscapeError, match="pp validation"):
+            mw.calculate("GTp", counts=R.COUNTS_BF16)
     mn, mx = m.calculate("GTp")
     assert np.isfinite(mn) and np.isfinite(mx)
