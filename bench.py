@@ -107,21 +107,41 @@ def uneven_null_batch(ctx, reps=3):
     """The batched null of 1000 unstructured alignments of uneven size (L uniform 80-600, N
     log-uniform 50-4000, own Yule trees, mean branch length 0.05-0.3): simulate, score GTp,
     histogram and exponential tail fit per alignment, all in one call; eager and as one captured
-    graph (capture included in the call)."""
+    graph.  The graph is captured by the first call (timed apart as `first_call_s`); each timed
+    call draws new seeds for the same sizes and trees and replays it."""
     from rscape_amd import null as NL
     jobs = NL.uneven_jobs(1000, seed=42)
     upd = float(sum(j.tree["n"] * j.L * (j.L - 1) / 2 for j in jobs))
     out = {"workload": "1000 alignments, L ~ U(80,600), N ~ logU(50,4000), GTp, one group each",
            "updates": upd}
-    for name, graph in (("eager", 0), ("graph", 1)):
+    for name, graph, persistent in (("eager_per_tile", 0, False), ("eager", 0, True), ("graph", 1, True)):
+        ctx.set_batch_persistent(persistent)
+        t0 = time.perf_counter()
         NL.batch(ctx, jobs, "GTp", pmass=5e-4, graph=graph)
+        first = time.perf_counter() - t0
         ts = []
-        for _ in range(reps):
+        for r in range(reps):
+            for j in jobs:
+                j.seed += 1
             t0 = time.perf_counter()
             NL.batch(ctx, jobs, "GTp", pmass=5e-4, graph=graph)
             ts.append(time.perf_counter() - t0)
         t = float(np.median(ts))
-        out[name] = {"s_per_batch": t, "alignments_per_s": 1000.0 / t, "updates_per_s": upd / t}
+        out[name] = {"s_per_batch": t, "alignments_per_s": 1000.0 / t, "updates_per_s": upd / t,
+                     "first_call_s": first}
+    ctx.set_batch_persistent(True)
+    out["graph_reuses"] = ctx.graph_reuses()
+    # where one eager call's time goes: the job records built in Python, the library's host
+    # planning, the uploads, and the device run (launch to completion)
+    t0 = time.perf_counter()
+    NL.job_array(jobs)
+    phases = {"python_job_records": (time.perf_counter() - t0) * 1e3}
+    ctx.set_timing(True)
+    NL.batch(ctx, jobs, "GTp", pmass=5e-4, graph=0)
+    for name, ms in ctx.timings():
+        phases[name] = phases.get(name, 0.0) + ms
+    ctx.set_timing(False)
+    out["phases_ms"] = phases
     return out
 
 
@@ -336,11 +356,18 @@ def main():
     ap.add_argument("--stat", choices=("fp32", "fp64"), default="fp64",
                     help="arithmetic of the per-pair statistics in the timed step")
     ap.add_argument("--no-batch", action="store_true", help="skip the 1000-alignment null batch")
+    ap.add_argument("--null-batch-only", action="store_true",
+                    help="time only the 1000-alignment null batch and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(a sharded design is described with the library). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.null_batch_only:
+        ctx = R.Context(0)
+        print(json.dumps({"uneven_null_batch": uneven_null_batch(ctx)}), flush=True)
+        ctx.close()
+        return
 
     d = synth.config(1, seed=42, N=args.N, L=args.L)
     ax = d["ax"]

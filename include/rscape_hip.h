@@ -67,6 +67,11 @@ int         rsg_ctx_set_gram_shape(rsg_ctx *ctx, int shape);
  * are split along the sequence axis over the idle CUs and their partial counts summed
  * (integers, so the result is identical); 0: one work group per tile. */
 int         rsg_ctx_set_gram_split(rsg_ctx *ctx, int on);
+/* Batched null (rsg_null_batch, rsg_null_batch_fitch): 1 (default) runs the batched Gram's
+ * 32 x 32-column tiles (alignments under 256 sequences or columns) in a persistent kernel fed
+ * by per-XCD tile queues (longest K first); 0 launches one work group per tile.  Results are
+ * identical (no reference counterpart). */
+int         rsg_ctx_set_batch_persistent(rsg_ctx *ctx, int on);
 /* How long rsg_cov_Calculate spins on its completion stamp in pinned host memory before it
  * waits in a stream synchronisation instead (default 2000 microseconds; 0 = always
  * synchronise).  Results are the same either way (no reference counterpart). */
@@ -262,6 +267,15 @@ int rsg_weights_GSC(rsg_ctx *ctx, const uint8_t *ax, int64_t nseq, int64_t alen,
 /* msamanip_RemoveGapColumns column mask (gapthresh, weights as given). Returns the kept count. */
 int rsg_gapcol_mask(const uint8_t *ax, const double *wgt, int64_t nseq, int64_t alen, double gapthresh,
                     int *useme);
+/* msamanip_RemoveGapColumns (src/msamanip.c:461-541) on a resident alignment: with
+ * gapthresh < 1 a column is kept iff its residues carry weight r > 0 and r / (residues + gaps)
+ * >= 1 - gapthresh (the msa's weights; R-scape filters before weighting, so with unit
+ * weights), computed on the device; with gapthresh >= 1 every column is kept.  *ret receives
+ * a new resident alignment of the kept columns, built on the device (same weights);
+ * useme (src alen ints) and map (new column -> src column, 0-based; up to src alen ints) may
+ * be NULL.  No column kept: RSG_FAIL "no positions left after gap trimming". */
+int rsg_msa_RemoveGapColumns(rsg_ctx *ctx, const rsg_msa *src, double gapthresh, int *useme, int *map,
+                             rsg_msa **ret, char *errbuf);
 
 /* ---- phylogenetic null simulation (cov_GenerateAlignment, src/cov_simulate.c:60-859;
  *      trees as R-scape-sim's create_tree, src/R-scape-sim.c:608-680) ----
@@ -320,10 +334,16 @@ int rsg_null_simulate_pairs(rsg_ctx *ctx, const rsg_null_job *job, const int *ct
  * fits[g*8 + {0..7}] = {mu, lambda, tau, newmass, pmass, w, minCOV, maxCOV}; hists (may be
  * NULL) receives each group's histogram; feats (may be NULL) receives 48 surrogate features
  * per group (rsg_null_features of its first replicate).  use_graph = k >= 1 records the whole batch in one
- * hipGraph and launches it k times (each launch recomputes everything); 0 runs eagerly. */
+ * hipGraph and launches it k times (each launch recomputes everything); 0 runs eagerly.  The
+ * instantiated graph stays with the context: a later call whose launch geometry is the same
+ * (job sizes, groups, tree shapes by depth, covtype and fit arguments) uploads its inputs
+ * (trees, branch lengths, models, seeds) and replays it without a new capture. */
 int rsg_null_batch(rsg_ctx *ctx, const rsg_null_job *jobs, int njobs, int covtype,
                    double bmin, double pmass, double fracfit, int doexpfit,
                    rsg_hist **hists, double *fits, float *feats, int use_graph, char *errbuf);
+/* Calls of rsg_null_batch / rsg_null_batch_fitch on this context that replayed the cached
+ * graph instead of capturing one (no counterpart in the reference: a diagnostic). */
+int rsg_ctx_graph_reuses(const rsg_ctx *ctx);
 /* R-scape's null of one alignment in one batch: null_rscape's nshuffle loop
  * (src/R-scape.c:1615-1724) with replicate r = rsg_null_fitch(ax, tree, seeds[r]) (Fitch
  * ancestors, column shuffle, substitution replay), every replicate scored (covtype, unit

@@ -44,6 +44,7 @@ extern "C" void rsg_ctx_destroy(rsg_ctx *c)
     if (c->timers[k].a) { hipEventDestroy(c->timers[k].a); hipEventDestroy(c->timers[k].b); }
   if (c->d_red) hipFree(c->d_red);
   if (c->d_gscr) hipFree(c->d_gscr);
+  if (c->bexec) hipGraphExecDestroy(c->bexec);
   if (c->d_arena) hipFree(c->d_arena);
   if (c->d_hits) hipFree(c->d_hits);
   if (c->h_hits) hipHostFree(c->h_hits);
@@ -89,7 +90,8 @@ extern "C" int rsg_ctx_timings(rsg_ctx *c, const char **names, double *ms, int *
   int k = 0;
   for (; k < c->ntimers && k < cap; k++) {
     float f = 0.f;
-    hipEventElapsedTime(&f, c->timers[k].a, c->timers[k].b);
+    if (c->timers[k].host_ms >= 0.) f = (float)c->timers[k].host_ms;
+    else hipEventElapsedTime(&f, c->timers[k].a, c->timers[k].b);
     names[k] = c->timers[k].name; ms[k] = f;
   }
   *n = k;
@@ -99,6 +101,7 @@ extern "C" int rsg_ctx_timings(rsg_ctx *c, const char **names, double *ms, int *
 
 extern "C" int rsg_ctx_set_gram_shape(rsg_ctx *c, int shape) { c->gram_shape = shape; return RSG_OK; }
 extern "C" int rsg_ctx_set_gram_split(rsg_ctx *c, int on) { c->gram_split = on ? 1 : 0; return RSG_OK; }
+extern "C" int rsg_ctx_set_batch_persistent(rsg_ctx *c, int on) { c->batch_persistent = on ? 1 : 0; return RSG_OK; }
 
 static void free_msa(rsg_msa *m)
 {
@@ -128,8 +131,8 @@ extern "C" int rsg_msa_update(rsg_ctx *c, rsg_msa *m, const uint8_t *ax, const d
   return RSG_OK;
 }
 
-extern "C" int rsg_msa_create(rsg_ctx *c, const uint8_t *ax, const double *wgt, int64_t nseq, int64_t alen,
-                              rsg_msa **ret, char *errbuf)
+// a resident alignment's buffers (codes, packed panels, weights), contents unset
+int rsg_msa_alloc(rsg_ctx *c, int64_t nseq, int64_t alen, rsg_msa **ret, char *errbuf)
 {
   *ret = nullptr;
   if (nseq < 1 || alen < 1) return rsg_fail(errbuf, "empty alignment (nseq %lld, alen %lld)", (long long)nseq, (long long)alen);
@@ -146,6 +149,15 @@ extern "C" int rsg_msa_create(rsg_ctx *c, const uint8_t *ax, const double *wgt, 
     free_msa(m);
     return rsg_fail(errbuf, "device allocation failed for a %lld x %lld alignment", (long long)nseq, (long long)alen);
   }
+  *ret = m;
+  return RSG_OK;
+}
+
+extern "C" int rsg_msa_create(rsg_ctx *c, const uint8_t *ax, const double *wgt, int64_t nseq, int64_t alen,
+                              rsg_msa **ret, char *errbuf)
+{
+  rsg_msa *m = nullptr;
+  if (rsg_msa_alloc(c, nseq, alen, &m, errbuf) != RSG_OK) return RSG_FAIL;
   if (rsg_msa_update(c, m, ax, wgt, errbuf) != RSG_OK) { free_msa(m); return RSG_FAIL; }
   *ret = m;
   return RSG_OK;

@@ -505,9 +505,65 @@ k_gram_i8_batch(const int4 *__restrict__ tiles, const int64_t *__restrict__ desc
                                     lds, 0, d[2] / KT, nullptr, pmi, pmi + pm32_n(nb32), nb32);
 }
 
-int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_big, int nbig, const int4 *d_small, int nsmall,
-                          const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt, double *d_pm_part, int pm_field)
+// The same tiles in a persistent grid (as many work groups as fit the device at once) fed by
+// per-XCD queues: work groups are dispatched to the 8 XCDs round-robin, so group g serves queue
+// x = g % 8, which holds the tiles t = x, x + 8, x + 16, ... of the list (each XCD gets the same
+// mix of long and short tiles); a pop takes the next index of its queue from an atomic counter.
+// The list is sorted by decreasing K length, so this is longest-first greedy scheduling of
+// uneven items; eight counters instead of one keep the pops from serialising on one address.
+// The counters are cleared by a fill kernel ahead of the launch (graph-safe).
+template <int WI, int WJ, int KT>
+__global__ void __launch_bounds__(128 * WI * WJ) __attribute__((amdgpu_waves_per_eu(2)))
+k_gram_i8_queue(const int4 *__restrict__ tiles, int ntiles, int *__restrict__ queue, const int64_t *__restrict__ desc,
+                const uint8_t *__restrict__ xt, uint16_t *__restrict__ cnt, double *__restrict__ pm_part, int pm_field)
 {
+  typedef CodeStage<32 * WI, 32 * WJ, KT, 128 * WI * WJ> St;
+  __shared__ __attribute__((aligned(16))) uint8_t lds[St::lds_bytes(true)];
+  __shared__ int next;
+  const int x = blockIdx.x & 7;
+  int *q = queue + 16 * x;                              // one counter per 64-byte line
+  if (threadIdx.x == 0) next = 8 * atomicAdd(q, 1) + x;
+  __syncthreads();
+  int t = next;
+  while (t < ntiles) {
+    int nxt = 0;                                        // the next pop overlaps this tile's first loads
+    if (threadIdx.x == 0) nxt = 8 * atomicAdd(q, 1) + x;
+    const int4 tl = tiles[t];
+    const int64_t *d = desc + (int64_t)tl.x * RSG_BDESC;
+    const int64_t nb32 = (d[1] + 31) / 32;
+    double *pmi = pm_part + d[pm_field];
+    gram_tile_i8<WI, WJ, KT, CNT_U16>(xt + d[4], d[3], d[1], cnt + d[5], (int64_t)tl.y * 32 * WI, (int64_t)tl.z * 32 * WJ,
+                                      lds, 0, d[2] / KT, nullptr, pmi, pmi + pm32_n(nb32), nb32);
+    __syncthreads();                                    // every wave is done with lds and has read next
+    if (threadIdx.x == 0) next = nxt;
+    __syncthreads();
+    t = next;
+  }
+}
+
+// the 8 queue counters (16 ints apart)
+__global__ void k_clear_queues(int *__restrict__ q) { if (threadIdx.x < 128) q[threadIdx.x] = 0; }
+
+int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_big, int nbig, const int4 *d_small, int nsmall,
+                          const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt, double *d_pm_part, int pm_field,
+                          int *d_queue)
+{
+  const int ncu = c->ncu > 0 ? c->ncu : 256;
+  if (c->batch_persistent && d_queue) {
+    // Large tiles (64 x 32 columns, tens of microseconds each) keep one work group per tile:
+    // the hardware dispatcher is already a longest-first queue over the sorted list, and the
+    // queue loop's extra live registers make that tile body spill (measured 30% slower).  The
+    // many short 32 x 32 tiles run persistent, saving a dispatch per tile.
+    if (nbig > 0)
+      hipLaunchKernelGGL((k_gram_i8_batch<2, 1, 256>), dim3((unsigned)nbig), dim3(256), 0, c->stream, d_big, d_desc, d_xt,
+                         d_cnt, d_pm_part, pm_field);
+    if (nsmall > 0) {                   // 2 waves per group, four groups per CU
+      hipLaunchKernelGGL(k_clear_queues, dim3(1), dim3(256), 0, c->stream, d_queue);
+      hipLaunchKernelGGL((k_gram_i8_queue<1, 1, 64>), dim3((unsigned)(std::min(nsmall, 4 * ncu) + 7) & ~7u), dim3(128), 0,
+                         c->stream, d_small, nsmall, d_queue, d_desc, d_xt, d_cnt, d_pm_part, pm_field);
+    }
+    return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
+  }
   if (nbig > 0)
     hipLaunchKernelGGL((k_gram_i8_batch<2, 1, 256>), dim3((unsigned)nbig), dim3(256), 0, c->stream, d_big, d_desc, d_xt, d_cnt,
                        d_pm_part, pm_field);

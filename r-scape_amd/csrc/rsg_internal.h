@@ -61,7 +61,7 @@ __device__ __forceinline__ void pack_xt_chunk(const uint8_t (*tile)[65], int64_t
   *(uint4 *)(xt + ((s0 / 16 + q) * Lpad + c0 + cc) * 16) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
-struct rsg_timer { const char *name; hipEvent_t a, b; };
+struct rsg_timer { const char *name; hipEvent_t a, b; double host_ms; };   // host_ms >= 0: a host phase
 
 struct rsg_ctx {
   int          device = 0;
@@ -71,11 +71,18 @@ struct rsg_ctx {
   rsg_timer    timers[RSG_MAX_TIMERS];
   int          gram_shape = 0;          // 0 = automatic
   int          gram_split = 1;          // split the last partial wave of Gram tiles along K
+  int          batch_persistent = 1;    // batched kernels: persistent grids (1) or one group per work unit (0)
   int          defer = 0;               // queue status/min/max copies; the caller synchronises once
   int          skip_minmax = 0;         // a correction follows: the statistic's min / max are not needed
   int          ncu = 0;                 // compute units of the device
   void        *d_gscr = nullptr;  size_t gscr_cap = 0;   // Gram split partials
   void        *d_arena = nullptr; size_t arena_cap = 0;  // batched-null buffers (grow-only)
+  // the last batched-null graph: instantiated once per launch geometry (sizes, tiles, tree
+  // levels, scalar arguments, arena address: the bytes in bexec_sig), replayed by later calls
+  // of the same geometry with new inputs
+  hipGraphExec_t bexec = nullptr;
+  std::vector<uint8_t> bexec_sig;
+  int          bexec_hits = 0;          // replays that reused the cached graph (for tests)
   void        *d_hits = nullptr;  size_t hits_cap = 0;   // hit-list compaction (grow-only, entries)
   void        *h_hits = nullptr;                          // its pinned host mirror, same capacity
   // small reusable device scratch (reductions)
@@ -155,7 +162,7 @@ static inline void rsg_tstart(rsg_ctx *c, const char *name, int *slot)
   if (c->timing == 2 && strncmp(name, "gram", 4) != 0) return;     // Gram launches only
   rsg_timer &t = c->timers[c->ntimers];
   if (!t.a) { hipEventCreate(&t.a); hipEventCreate(&t.b); }
-  t.name = name;
+  t.name = name; t.host_ms = -1.;
   hipEventRecord(t.a, c->stream);
   *slot = c->ntimers++;
 }
@@ -163,10 +170,18 @@ static inline void rsg_tstop(rsg_ctx *c, int slot)
 {
   if (slot >= 0) hipEventRecord(c->timers[slot].b, c->stream);
 }
+// a host-side phase measured by the caller (wall clock, ms), reported with the device timers
+static inline void rsg_thost(rsg_ctx *c, const char *name, double ms)
+{
+  if (c->timing != 1 || c->ntimers >= RSG_MAX_TIMERS) return;
+  rsg_timer &t = c->timers[c->ntimers++];
+  t.name = name; t.host_ms = ms;
+}
 
 // ---- kernel launchers implemented in the .hip files ----
 int rsg_noalloc_init(int device, char *errbuf);   // rsg_noalloc.hip: the scratch array's address
 int rsg_pack(rsg_ctx *c, rsg_msa *m, char *errbuf);
+int rsg_msa_alloc(rsg_ctx *c, int64_t nseq, int64_t alen, rsg_msa **ret, char *errbuf);   // rsg_capi.hip
 // pm_part (uint16 records only; nullptr = none): marginal partials of every 32 x 32 block
 // (pm32 layout, rsg_device.h); pmflag: cleared for the marginal reduction that follows
 int rsg_gram_counts_i8(rsg_ctx *c, rsg_msa *m, void *d_out, int kind, double *pm_part, int *pmflag, char *errbuf);
@@ -197,6 +212,7 @@ struct SimLevels {
   std::vector<int4> ent;
   std::vector<int64_t> lev_ent0;     // entry offset of each level (size nlev + 1)
   std::vector<int64_t> lev_blocks;   // 64-column blocks of each level
+  std::vector<int64_t> lev_maxl;     // longest alignment among each level's entries
 };
 int rsg_sim_levels(const SimItem *items, int nitems, const int *left, const int *right, SimLevels &lv, char *errbuf);
 int rsg_sim_launch(rsg_ctx *c, const SimItem *d_items, int nitems, int64_t nodes, int64_t blocks,
@@ -247,5 +263,8 @@ void rsg_feat_finish(const unsigned long long *cnt, int64_t N, int64_t L, const 
 // Large alignments of a batch use 64 x 32-column tiles (4 waves, 256-sequence K tiles), the
 // others 32 x 32-column tiles (2 waves, 64-sequence K tiles).
 static inline bool rsg_batch_big_tiles(int64_t N, int64_t L) { return L >= 256 && N >= 256; }
+// d_queue: 128 ints of device memory for the persistent kernel's per-XCD tile counters (unused
+// by the one-work-group-per-tile launch, rsg_ctx::batch_persistent = 0).
 int rsg_gram_batch_launch(rsg_ctx *c, const int4 *d_big, int nbig, const int4 *d_small, int nsmall,
-                          const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt, double *d_pm_part, int pm_field);
+                          const int64_t *d_desc, const uint8_t *d_xt, uint16_t *d_cnt, double *d_pm_part, int pm_field,
+                          int *d_queue);

@@ -205,19 +205,17 @@ k_root(const SimItem *__restrict__ items, int nitems, const double *__restrict__
   seqs[it.seq_off + col] = (uint8_t)a;
 }
 
-// One tree level: every internal node v at this depth draws its two children for 64
-// columns per block.  ent[k] = {item, v, first block of the entry within the level}.
-__global__ void __launch_bounds__(64)
-k_evolve_level(const SimItem *__restrict__ items, const int4 *__restrict__ ent, int nent,
+// One tree level: every internal node v at this depth draws its two children; block
+// (x, y) takes entry x of the level (ent[k] = {item, v, -, -}) and its columns y*256 ..
+// y*256 + 255 (blocks past the item's length return at once).
+__global__ void __launch_bounds__(256)
+k_evolve_level(const SimItem *__restrict__ items, const int4 *__restrict__ ent,
                const int *__restrict__ left, const int *__restrict__ right, const uint32_t *__restrict__ cdf,
                uint8_t *__restrict__ seqs, uint8_t *__restrict__ leaves)
 {
-  int lo = 0, hi = nent - 1;
-  const int blk = (int)blockIdx.x;
-  while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (ent[mid].z <= blk) lo = mid; else hi = mid - 1; }
-  const int4 e = ent[lo];
+  const int4 e = ent[blockIdx.x];
   const SimItem it = items[e.x];
-  const int64_t col = (int64_t)(blk - e.z) * 64 + threadIdx.x;
+  const int64_t col = (int64_t)blockIdx.y * 256 + threadIdx.x;
   if (col >= it.L) return;
   const int64_t N = it.N, L = it.L, v = e.y;
   uint8_t *S = seqs + it.seq_off;
@@ -253,43 +251,90 @@ __global__ void k_branch_tables(const SimItem *__restrict__ items, int nitems, c
 
 // Tree levels of all items (host).  Depths come from a walk down from each root, so any
 // node order is accepted; a node reached twice or never is an error.
-int rsg_sim_levels(const SimItem *items, int nitems, const int *left, const int *right, SimLevels &lv, char *errbuf)
+// Depth of every internal node of one tree (0 = root), checking that the tree is well formed:
+// every internal node reached exactly once from the root, leaves in range.  Trees whose
+// children always carry larger indices than their parent (as Yule simulation and UPGMA number
+// them) take one forward pass; any other tree a depth-first walk.
+static int tree_depths(const int *left, const int *right, int64_t N, int *depth, std::vector<int> &stack, int item,
+                       char *errbuf)
 {
-  std::vector<std::vector<int4>> per;                  // per level: entries without block offsets
-  std::vector<int> depth, stack;
-  for (int k = 0; k < nitems; k++) {
-    const SimItem &it = items[k];
-    const int64_t m = it.N - 1;
-    depth.assign((size_t)m, -1);
+  const int64_t m = N - 1;
+  for (int64_t v = 0; v < m; v++) depth[v] = -1;
+  depth[0] = 0;
+  int64_t seen = 1;
+  bool forward = true;
+  for (int64_t v = 0; v < m && forward; v++) {
+    if (depth[v] < 0) { forward = false; break; }          // not reached yet: not in topological order
+    for (int side = 0; side < 2; side++) {
+      const int d = side ? right[v] : left[v];
+      if (d > 0) {
+        if (d >= m || depth[d] >= 0) return rsg_fail(errbuf, "item %d: malformed tree at node %lld", item, (long long)v);
+        if (d <= v) { forward = false; break; }
+        depth[d] = depth[v] + 1; seen++;
+      } else if (-d >= N) return rsg_fail(errbuf, "item %d: leaf index %d out of range", item, -d);
+    }
+  }
+  if (!forward) {
+    for (int64_t v = 0; v < m; v++) depth[v] = -1;
+    depth[0] = 0; seen = 1;
     stack.assign(1, 0);
-    depth[0] = 0;
-    int64_t seen = 1;
     while (!stack.empty()) {
       const int v = stack.back(); stack.pop_back();
-      if ((int)per.size() <= depth[(size_t)v]) per.resize((size_t)depth[(size_t)v] + 1);
-      per[(size_t)depth[(size_t)v]].push_back(make_int4(k, v, 0, 0));
       for (int side = 0; side < 2; side++) {
-        const int d = side ? right[it.tree_off + v] : left[it.tree_off + v];
+        const int d = side ? right[v] : left[v];
         if (d > 0) {
-          if (d >= m || depth[(size_t)d] >= 0) return rsg_fail(errbuf, "item %d: malformed tree at node %d", k, v);
-          depth[(size_t)d] = depth[(size_t)v] + 1; stack.push_back(d); seen++;
-        } else if (-d >= it.N) return rsg_fail(errbuf, "item %d: leaf index %d out of range", k, -d);
+          if (d >= m || depth[d] >= 0) return rsg_fail(errbuf, "item %d: malformed tree at node %d", item, v);
+          depth[d] = depth[v] + 1; stack.push_back(d); seen++;
+        } else if (-d >= N) return rsg_fail(errbuf, "item %d: leaf index %d out of range", item, -d);
       }
     }
-    if (seen != m) return rsg_fail(errbuf, "item %d: %lld internal nodes unreachable from the root", k, (long long)(m - seen));
   }
-  lv.ent.clear(); lv.lev_ent0.assign(1, 0); lv.lev_blocks.clear();
-  for (auto &lev : per) {
-    std::sort(lev.begin(), lev.end(), [](const int4 &a, const int4 &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
-    int64_t blk = 0;
-    for (int4 &e : lev) {
-      if (blk > INT32_MAX) return rsg_fail(errbuf, "too many column blocks in one tree level");
-      e.z = (int)blk;
-      blk += (items[e.x].L + 63) / 64;
-      lv.ent.push_back(e);
+  if (seen != m) return rsg_fail(errbuf, "item %d: %lld internal nodes unreachable from the root", item, (long long)(m - seen));
+  return RSG_OK;
+}
+
+int rsg_sim_levels(const SimItem *items, int nitems, const int *left, const int *right, SimLevels &lv, char *errbuf)
+{
+  // pass 1: depths of all nodes and the number of nodes and 64-column blocks per depth
+  int64_t nodes = 0;
+  for (int k = 0; k < nitems; k++) nodes += items[k].N - 1;
+  std::vector<int> depth((size_t)nodes), stack;
+  std::vector<int64_t> cnt, blk, mxl;
+  for (int k = 0; k < nitems; k++) {
+    const SimItem &it = items[k];
+    int *dk = depth.data() + it.tree_off;
+    if (tree_depths(left + it.tree_off, right + it.tree_off, it.N, dk, stack, k, errbuf) != RSG_OK) return RSG_FAIL;
+    const int64_t nbk = (it.L + 63) / 64;
+    for (int64_t v = 0; v < it.N - 1; v++) {
+      const int dv = dk[v];
+      if ((int64_t)cnt.size() <= dv) { cnt.resize((size_t)dv + 1, 0); blk.resize((size_t)dv + 1, 0); mxl.resize((size_t)dv + 1, 0); }
+      cnt[(size_t)dv]++; blk[(size_t)dv] += nbk;
+      if (it.L > mxl[(size_t)dv]) mxl[(size_t)dv] = it.L;
     }
-    lv.lev_ent0.push_back((int64_t)lv.ent.size());
-    lv.lev_blocks.push_back(blk);
+  }
+  const size_t nlev = cnt.size();
+  lv.lev_ent0.assign(nlev + 1, 0);
+  lv.lev_blocks.assign(nlev, 0);
+  lv.lev_maxl.assign(nlev, 0);
+  for (size_t l = 0; l < nlev; l++) {
+    lv.lev_maxl[l] = mxl[l];
+    if (blk[l] > INT32_MAX) return rsg_fail(errbuf, "too many column blocks in one tree level");
+    lv.lev_ent0[l + 1] = lv.lev_ent0[l] + cnt[l];
+    lv.lev_blocks[l] = blk[l];
+  }
+  // pass 2: a node's draws are keyed by its item and row, so the order of a level's entries only
+  // sets which blocks simulate it: items ascending, each item's nodes ascending
+  lv.ent.resize((size_t)nodes);
+  std::vector<int64_t> fill(lv.lev_ent0.begin(), lv.lev_ent0.end() - 1), bfill(nlev, 0);
+  for (int k = 0; k < nitems; k++) {
+    const SimItem &it = items[k];
+    const int *dk = depth.data() + it.tree_off;
+    const int64_t nbk = (it.L + 63) / 64;
+    for (int64_t v = 0; v < it.N - 1; v++) {
+      const size_t dv = (size_t)dk[v];
+      lv.ent[(size_t)fill[dv]++] = make_int4(k, (int)v, (int)bfill[dv], 0);
+      bfill[dv] += nbk;
+    }
   }
   return RSG_OK;
 }
@@ -308,8 +353,8 @@ int rsg_sim_launch(rsg_ctx *c, const SimItem *d_items, int nitems, int64_t nodes
   for (size_t l = 0; l < lv.lev_blocks.size(); l++) {
     const int64_t e0 = lv.lev_ent0[l], ne = lv.lev_ent0[l + 1] - e0;
     if (lv.lev_blocks[l] > 0)
-      hipLaunchKernelGGL(k_evolve_level, dim3((unsigned)lv.lev_blocks[l]), dim3(64), 0, c->stream, d_items,
-                         d_ent + e0, (int)ne, d_left, d_right, d_cdf, d_seqs, d_leaves);
+      hipLaunchKernelGGL(k_evolve_level, dim3((unsigned)ne, (unsigned)((lv.lev_maxl[l] + 255) / 256)), dim3(256), 0,
+                         c->stream, d_items, d_ent + e0, d_left, d_right, d_cdf, d_seqs, d_leaves);
   }
   return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
 }
@@ -600,8 +645,8 @@ extern "C" int rsg_null_simulate_pairs(rsg_ctx *c, const rsg_null_job *job, cons
   for (size_t l = 0; l < lv.lev_blocks.size(); l++) {
     const int64_t e0 = lv.lev_ent0[l], ne = lv.lev_ent0[l + 1] - e0;
     if (lv.lev_blocks[l] > 0)
-      hipLaunchKernelGGL(k_evolve_level, dim3((unsigned)lv.lev_blocks[l]), dim3(64), 0, c->stream, d_items,
-                         d_ent + e0, (int)ne, d_left, d_right, d_cdf, d_seqs, d_leaves);
+      hipLaunchKernelGGL(k_evolve_level, dim3((unsigned)ne, (unsigned)((lv.lev_maxl[l] + 255) / 256)), dim3(256), 0,
+                         c->stream, d_items, d_ent + e0, d_left, d_right, d_cdf, d_seqs, d_leaves);
     if (np && ne > 0)
       hipLaunchKernelGGL(k_pair_level, dim3((unsigned)((np + 63) / 64), (unsigned)ne), dim3(64), 0, c->stream,
                          d_nodes + e0, d_pairs, (int)np, d_left, d_right, d_tab, N, L, it.seed, it.stream, it.gapthr,

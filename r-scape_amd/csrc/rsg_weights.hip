@@ -317,3 +317,83 @@ extern "C" int rsg_gapcol_mask(const uint8_t *ax, const double *wgt, int64_t N, 
   }
   return kept;
 }
+
+// ---------------- gap-column filter on the resident alignment ----------------
+// msamanip_RemoveGapColumns (src/msamanip.c:461-541) on the device: one thread per column sums
+// the weights of its residues (canonical, degenerate, N) and of residues + gaps in sequence
+// order (the host loop's order, so the FP64 sums and the decisions are the same); nonresidue
+// and missing symbols do not count.
+__global__ void __launch_bounds__(64) k_gapcol(const uint8_t *__restrict__ ax, const double *__restrict__ w, int64_t N,
+                                               int64_t L, double need, int *__restrict__ useme)
+{
+  const int64_t col = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (col >= L) return;
+  double r = 0., tot = 0.;
+  for (int64_t s = 0; s < N; s++) {
+    const uint8_t a = ax[s * L + col];
+    const double ws = w[s];
+    if (a < 4 || (a >= 5 && a <= 15)) { r += ws; tot += ws; }
+    else if (a == 4) tot += ws;
+  }
+  useme[col] = (r > 0. && r / tot >= need) ? 1 : 0;
+}
+
+// out (N x Lk) = the kept columns map[0..Lk) of ax (N x L)
+__global__ void __launch_bounds__(256) k_colsubset(const uint8_t *__restrict__ ax, int64_t N, int64_t L, const int *__restrict__ map,
+                                                   int64_t Lk, uint8_t *__restrict__ out)
+{
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= N * Lk) return;
+  const int64_t s = e / Lk, c = e - s * Lk;
+  out[e] = ax[s * L + map[c]];
+}
+
+extern "C" int rsg_msa_RemoveGapColumns(rsg_ctx *c, const rsg_msa *src, double gapthresh, int *useme, int *map,
+                                        rsg_msa **ret, char *errbuf)
+{
+  if (!c || !src || !ret) return rsg_fail(errbuf, "rsg_msa_RemoveGapColumns: bad arguments");
+  *ret = nullptr;
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t N = src->N, L = src->L;
+  std::vector<int> use((size_t)L, 1);
+  if (gapthresh < 1.0) {                                 // dofilter (useme is not an input here)
+    DevBuf db;
+    int *d_use;
+    HIPCHK(db.get(&d_use, sizeof(int) * (size_t)L));
+    int t; rsg_tstart(c, "gapcol", &t);
+    hipLaunchKernelGGL(k_gapcol, dim3((unsigned)((L + 63) / 64)), dim3(64), 0, c->stream, src->d_ax, src->d_w, N, L,
+                       1.0 - gapthresh, d_use);
+    rsg_tstop(c, t);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(use.data(), d_use, sizeof(int) * (size_t)L, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  std::vector<int> mp;
+  for (int64_t k = 0; k < L; k++) if (use[(size_t)k]) mp.push_back((int)k);
+  if (useme) memcpy(useme, use.data(), sizeof(int) * (size_t)L);
+  if (mp.empty()) return rsg_fail(errbuf, "no positions left after gap trimming");
+  if (map) memcpy(map, mp.data(), sizeof(int) * mp.size());
+  const int64_t Lk = (int64_t)mp.size();
+  rsg_msa *m = nullptr;
+  if (rsg_msa_alloc(c, N, Lk, &m, errbuf) != RSG_OK) return RSG_FAIL;
+  {
+    DevBuf db;
+    int *d_map;
+    if (db.get(&d_map, sizeof(int) * (size_t)Lk) != hipSuccess ||
+        hipMemcpyAsync(d_map, mp.data(), sizeof(int) * (size_t)Lk, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+      rsg_msa_destroy(c, m);
+      return rsg_fail(errbuf, "rsg_msa_RemoveGapColumns: column map upload failed");
+    }
+    hipLaunchKernelGGL(k_colsubset, dim3((unsigned)((N * Lk + 255) / 256)), dim3(256), 0, c->stream, src->d_ax, N, L, d_map,
+                       Lk, m->d_ax);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(m->d_w, src->d_w, sizeof(double) * (size_t)N, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+        rsg_pack(c, m, errbuf) != RSG_OK || hipStreamSynchronize(c->stream) != hipSuccess) {
+      rsg_msa_destroy(c, m);
+      return rsg_fail(errbuf, "rsg_msa_RemoveGapColumns: column subset failed");
+    }
+  }
+  m->weighted = src->weighted; m->wsum = src->wsum; m->weights_ok = src->weights_ok;
+  *ret = m;
+  return RSG_OK;
+}

@@ -56,7 +56,9 @@ def _declare(L):
         "rsg_gram_shape_name": (cp, [i32]),
         "rsg_ctx_set_gram_shape": (i32, [vp, i32]),
         "rsg_ctx_set_gram_split": (i32, [vp, i32]),
+        "rsg_ctx_set_batch_persistent": (i32, [vp, i32]),
         "rsg_ctx_set_stamp_wait": (i32, [vp, i32]),
+        "rsg_ctx_graph_reuses": (i32, [vp]),
         "rsg_msa_create": (i32, [vp, vp, vp, i64, i64, C.POINTER(vp), cp]),
         "rsg_msa_update": (i32, [vp, vp, vp, vp, cp]),
         "rsg_msa_destroy": (None, [vp, vp]),
@@ -87,6 +89,7 @@ def _declare(L):
         "rsg_weights_PB": (i32, [vp, vp, i64, i64, vp, cp]),
         "rsg_weights_GSC": (i32, [vp, vp, i64, i64, vp, cp]),
         "rsg_gapcol_mask": (i32, [vp, vp, i64, i64, d, vp]),
+        "rsg_msa_RemoveGapColumns": (i32, [vp, vp, d, vp, vp, C.POINTER(vp), cp]),
     })
     for name, (res, args) in sig.items():
         if hasattr(L, name):
@@ -245,10 +248,18 @@ class Context:
     def set_gram_split(self, on):
         self.L.rsg_ctx_set_gram_split(self.h, 1 if on else 0)
 
+    def set_batch_persistent(self, on):
+        """Batched null: persistent kernels (default) or one work group per work unit."""
+        self.L.rsg_ctx_set_batch_persistent(self.h, 1 if on else 0)
+
     def set_stamp_wait(self, microseconds):
         """How long a scoring call spins on its completion stamp before it synchronises the
         stream instead (0: always synchronise)."""
         self.L.rsg_ctx_set_stamp_wait(self.h, int(microseconds))
+
+    def graph_reuses(self):
+        """Batched-null calls that replayed the context's cached graph without a new capture."""
+        return int(self.L.rsg_ctx_graph_reuses(self.h))
 
     def msa(self, ax, wgt=None):
         return Msa(self, ax, wgt)
@@ -280,6 +291,24 @@ class Msa:
         _check(ctx.L.rsg_msa_create(ctx.h, _p(self.ax), _p(self.wgt), self.N, self.Lc, C.byref(h), err),
                err, "rsg_msa_create")
         self.h = h
+
+    @classmethod
+    def _wrap(cls, ctx, h, N, L, wgt):
+        m = cls.__new__(cls)
+        m.ctx, m.h, m.N, m.Lc, m.wgt, m.ax = ctx, h, N, L, wgt, None
+        return m
+
+    def remove_gap_columns(self, gapthresh):
+        """msamanip_RemoveGapColumns on the device: (resident alignment of the kept columns,
+        useme (L,), map (kept,) new -> old column)."""
+        useme = np.zeros(self.Lc, np.int32)
+        mp = np.zeros(self.Lc, np.int32)
+        h = C.c_void_p()
+        err = _err()
+        _check(self.ctx.L.rsg_msa_RemoveGapColumns(self.ctx.h, self.h, float(gapthresh), _p(useme), _p(mp), C.byref(h),
+                                                   err), err, "rsg_msa_RemoveGapColumns")
+        k = int(useme.sum())
+        return Msa._wrap(self.ctx, h, self.N, k, self.wgt), useme, mp[:k].copy()
 
     def close(self):
         if self.h:

@@ -24,6 +24,33 @@ class NullJob(C.Structure):
                 ("gapfrac", C.c_double), ("group", C.c_int32)]
 
 
+_JOB_DTYPE = np.dtype({"names": [f[0] for f in NullJob._fields_],
+                       "formats": ["<i8", "<i8", "<u8", "<u8", "<u8", "<u8", "<u8", "<u8", "<u4", "<f8", "<i4"],
+                       "offsets": [getattr(NullJob, f[0]).offset for f in NullJob._fields_],
+                       "itemsize": C.sizeof(NullJob)})
+
+
+def _addr(a):
+    return a.__array_interface__["data"][0]
+
+
+def job_array(jobs):
+    """The jobs as one rsg_null_job array (numpy records in the C layout; the jobs' arrays
+    must stay alive while it is used).  Jobs with group=None get group = their index."""
+    n = len(jobs)
+    rec = np.zeros(n, _JOB_DTYPE)
+    rec["nleaf"] = [j.tree["n"] for j in jobs]
+    rec["alen"] = [j.L for j in jobs]
+    for f in ("left", "right", "ld", "rd"):
+        rec[f] = [_addr(j.tree[f]) for j in jobs]
+    rec["model"] = [_addr(j.model) for j in jobs]
+    rec["seed"] = [j.seed for j in jobs]
+    rec["stream"] = [j.stream for j in jobs]
+    rec["gapfrac"] = [j.gapfrac for j in jobs]
+    rec["group"] = [k if j.group is None else j.group for k, j in enumerate(jobs)]
+    return rec
+
+
 def _declare():
     L = lib()
     if getattr(L, "_null_declared", False):
@@ -40,7 +67,7 @@ def _declare():
     L.rsg_null_fitch.restype = i32
     L.rsg_null_fitch.argtypes = [vp, vp, i64, i64, vp, vp, C.c_uint64, vp, cp]
     L.rsg_null_batch.restype = i32
-    L.rsg_null_batch.argtypes = [vp, C.POINTER(NullJob), i32, i32, d, d, d, i32, vp, vp, vp, i32, cp]
+    L.rsg_null_batch.argtypes = [vp, vp, i32, i32, d, d, d, i32, vp, vp, vp, i32, cp]
     L.rsg_null_batch_fitch.restype = i32
     L.rsg_null_batch_fitch.argtypes = [vp, vp, i64, i64, vp, vp, vp, i32, i32, d, d, d, i32, vp, vp, i32, cp]
     L.rsg_null_features.restype = i32
@@ -176,16 +203,15 @@ def batch(ctx, jobs, covtype="GTp", bmin=-10.0, pmass=5e-4, fracfit=1.0, expo=Tr
     minCOV, maxCOV.  feats=True also returns the (groups, 48) surrogate features."""
     from . import Hist, Histogram
     L = _declare()
-    structs = [j.struct(k) for k, j in enumerate(jobs)]
+    arr = job_array(jobs)
     n = len(jobs)
-    arr = (NullJob * n)(*structs)
-    ng = max(s.group for s in structs) + 1
+    ng = int(arr["group"].max()) + 1
     fits = np.zeros(ng * 8)
     hp = (C.POINTER(Hist) * ng)() if hists else None
     fe = np.zeros((ng, 48), np.float32) if feats else None
     err = _err()
     t = COV[covtype] if isinstance(covtype, str) else covtype
-    _check(L.rsg_null_batch(ctx.h, arr, n, t, bmin, pmass, fracfit, 1 if expo else 0,
+    _check(L.rsg_null_batch(ctx.h, _p(arr), n, t, bmin, pmass, fracfit, 1 if expo else 0,
                             C.cast(hp, C.c_void_p) if hists else None, _p(fits), _p(fe), int(graph), err),
            err, "rsg_null_batch")
     hs = [Histogram(hp[k]) if hp[k] else None for k in range(ng)] if hists else None

@@ -92,6 +92,23 @@ def test_batch_matches_single_path(ctx):
             assert np.isclose(fits[k][i], fit[key], rtol=1e-9, atol=1e-12), key
 
 
+def test_batch_persistent_equals_per_tile(ctx):
+    """The persistent queue Gram and the one-group-per-tile Gram give identical nulls, with
+    large (64 x 32-column) and small (32 x 32) tiles in one batch."""
+    print("persistent vs per-tile batch", flush=True)
+    sizes = [(300, 300), (1000, 420), (50, 80), (700, 257), (9, 64), (2000, 90)]
+    jobs = make_jobs(sizes, seed=21)
+    ctx.set_batch_persistent(False)
+    try:
+        a, ha = NL.batch(ctx, jobs, "GTp", pmass=0.05, hists=True)
+    finally:
+        ctx.set_batch_persistent(True)
+    b, hb = NL.batch(ctx, jobs, "GTp", pmass=0.05, hists=True)
+    assert np.array_equal(np.nan_to_num(a), np.nan_to_num(b))
+    for x, y in zip(ha, hb):
+        assert np.array_equal(x.h.counts(), y.h.counts())
+
+
 def test_batch_graph_replays_equal_eager(ctx):
     sizes = [(64, 100), (200, 150), (33, 80), (500, 60)]
     jobs = make_jobs(sizes, seed=5)
@@ -102,6 +119,46 @@ def test_batch_graph_replays_equal_eager(ctx):
         assert np.array_equal(np.nan_to_num(g), np.nan_to_num(eager))
         for a, b in zip(he, hg):
             assert np.array_equal(a.h.counts(), b.h.counts())
+
+
+def test_batch_graph_cache_replays_new_inputs(ctx):
+    """A graph call of a geometry already captured replays the cached graph with the new
+    call's inputs (seeds, branch lengths, models): equal to an eager run of those inputs.  A
+    call of another geometry captures afresh."""
+    print("graph cache: simulated batch", flush=True)
+    sizes = [(64, 100), (200, 150), (33, 80), (500, 60)]
+    first, _ = NL.batch(ctx, make_jobs(sizes, seed=5), "GTp", graph=1)
+    g0 = ctx.graph_reuses()
+    jobs = make_jobs(sizes, seed=5)
+    # same geometry (the tree shapes repeat), other branch lengths, models and seeds
+    for k, j in enumerate(jobs):
+        j.tree["ld"] = j.tree["ld"] * 1.7
+        j.tree["rd"] = j.tree["rd"] * 0.6
+        j.model = np.ascontiguousarray(NL.gtr_model((0.4, 0.1, 0.2, 0.3)))
+        j.seed += 99
+    eager, he = NL.batch(ctx, jobs, "GTp", graph=0, hists=True)
+    g, hg = NL.batch(ctx, jobs, "GTp", graph=1, hists=True)
+    assert ctx.graph_reuses() == g0 + 1
+    assert not np.array_equal(np.nan_to_num(g), np.nan_to_num(first))
+    assert np.array_equal(np.nan_to_num(g), np.nan_to_num(eager))
+    for a, b in zip(he, hg):
+        assert np.array_equal(a.h.counts(), b.h.counts())
+    other, _ = NL.batch(ctx, make_jobs(sizes[:3], seed=6), "GTp", graph=1)
+    assert ctx.graph_reuses() == g0 + 1
+    eager3, _ = NL.batch(ctx, make_jobs(sizes[:3], seed=6), "GTp", graph=0)
+    assert np.array_equal(np.nan_to_num(other), np.nan_to_num(eager3))
+    print("graph cache: R-scape null batch", flush=True)
+    from rscape_amd import synth
+    rng = np.random.default_rng(3)
+    tree = synth.yule_tree(80, rng, 0.15)
+    ax = synth.evolve(tree, 120, rng, (0.3, 0.2, 0.2, 0.3))[1].copy()
+    NL.batch_fitch(ctx, ax, tree, [1, 2, 3], "GTp", pmass=0.01, graph=1)
+    r0 = ctx.graph_reuses()
+    ax2 = synth.evolve(tree, 120, rng, (0.25, 0.25, 0.25, 0.25))[1].copy()
+    fit_e, h_e = NL.batch_fitch(ctx, ax2, tree, [7, 8, 9], "GTp", pmass=0.01)
+    fit_g, h_g = NL.batch_fitch(ctx, ax2, tree, [7, 8, 9], "GTp", pmass=0.01, graph=1)
+    assert ctx.graph_reuses() == r0 + 1
+    assert np.array_equal(fit_g, fit_e) and np.array_equal(h_g.h.counts(), h_e.h.counts())
 
 
 def test_batch_groups_match_cumulative_null(ctx):

@@ -81,3 +81,41 @@ def test_tutorial_scores_through_device_weights(ctx):
     pos = {c + 1: k for k, c in enumerate(cols)}
     errs = [abs(cov[pos[i], pos[j]] - s) / s for i, j, s in gold["pairs"]]
     assert max(errs) < 1e-3 and np.median(errs) < 5e-5          # as the oracle (test_oracle.py)
+
+
+@pytest.mark.parametrize("N,L,seed", [(1, 7, 0), (40, 90, 1), (513, 300, 2)])
+def test_device_gap_column_filter(ctx, N, L, seed):
+    """msamanip_RemoveGapColumns on the resident alignment: the column decisions equal the
+    oracle's (unit and random weights; residues, gaps, degenerate/N, nonresidue and missing
+    codes), and the kept-column alignment built on the device scores as one uploaded from the
+    host (counts bit-exact, GTp equal)."""
+    import rscape_amd as R
+    rng = np.random.default_rng(seed)
+    ax = rng.integers(0, 4, size=(N, L), dtype=np.uint8)
+    ax[rng.random((N, L)) < rng.random(L)[None, :] * 0.8] = 4
+    for code, p in ((9, 0.02), (15, 0.02), (16, 0.02), (17, 0.03)):
+        ax[rng.random((N, L)) < p] = code
+    for w in (None, rng.uniform(0.1, 3.0, size=N)):
+        m = ctx.msa(ax, w)
+        for thr in (0.0, 0.5, 0.75):
+            sub, use, mp = m.remove_gap_columns(thr)
+            ref = oracle.gapcol_filter(ax, np.ones(N) if w is None else w, thr)
+            assert np.array_equal(use.astype(bool), ref.astype(bool)), (thr, w is None)
+            assert np.array_equal(mp, np.nonzero(ref)[0])
+            if sub.Lc < 2:
+                sub.close()
+                continue
+            host = ctx.msa(np.ascontiguousarray(ax[:, ref.astype(bool)]), w)
+            kw = {} if w is None else dict(counts=R.COUNTS_BF16X3)
+            a = sub.calculate("GTp", **kw)
+            b = host.calculate("GTp", **kw)
+            assert a == b
+            assert np.array_equal(sub.counts(), host.counts())
+            assert np.array_equal(sub.get_cov()[0], host.get_cov()[0])
+            sub.close(); host.close()
+        whole, use, mp = m.remove_gap_columns(1.0)            # no filter: every column kept
+        assert whole.Lc == L and use.all() and np.array_equal(mp, np.arange(L))
+        whole.close(); m.close()
+    gaps = ctx.msa(np.full((5, 9), 4, np.uint8))
+    with pytest.raises(R.RscapeError, match="no positions left"):
+        gaps.remove_gap_columns(0.5)
