@@ -63,6 +63,11 @@ int      rsg_ctx_timings(rsg_ctx *ctx, const char **names, double *ms, int *n, i
 int         rsg_gram_nshapes(void);
 const char *rsg_gram_shape_name(int shape);
 int         rsg_ctx_set_gram_shape(rsg_ctx *ctx, int shape);
+/* The autotuner's table (no reference counterpart): "logN logL shape" lines giving the fastest
+ * shape for alignments with ceil(log2 nseq) = logN and ceil(log2 alen) = logL.  The library
+ * reads gram_tiles.tsv next to itself on first use; this replaces that table for the process.
+ * Sizes without an entry (and shape 0 contexts only) use the built-in choice. */
+int         rsg_gram_tiles_load(const char *path, char *errbuf);
 /* 1 (default): when the tile count leaves a last partial wave of work groups, those tiles
  * are split along the sequence axis over the idle CUs and their partial counts summed
  * (integers, so the result is identical); 0: one work group per tile. */

@@ -31,7 +31,9 @@
 // cell x*4+y (the layout of pp[i][j][IDX(x,y,4)] in the reference); unit-weight records hold
 // uint16 counts (N <= 65535), weighted ones FP32.
 #include "rsg_device.h"
+#include <dlfcn.h>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <algorithm>
 
@@ -792,9 +794,83 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, double *pm_part, int *pm
   return RSG_OK;
 }
 
+// The autotuner's table: the fastest shape per size bucket (ceil(log2 N), ceil(log2 L)), one
+// "logN logL shape" line each ('#' starts a comment).  Loaded once per process from
+// gram_tiles.tsv next to the library (or rsg_gram_tiles_load); sizes without an entry use
+// the built-in rule below.
+static std::mutex g_tiles_mu;
+static std::map<std::pair<int, int>, int> g_tiles;
+static bool g_tiles_tried = false;
+
+static int ceil_log2(int64_t v)
+{
+  int k = 0;
+  while (((int64_t)1 << k) < v) k++;
+  return k;
+}
+
+static int parse_tiles(const char *path, std::map<std::pair<int, int>, int> &t, char *errbuf)
+{
+  FILE *f = path ? fopen(path, "r") : nullptr;
+  if (!f) return rsg_fail(errbuf, "rsg_gram_tiles_load: cannot open %s", path ? path : "(null)");
+  char line[256];
+  int ln = 0;
+  while (fgets(line, sizeof line, f)) {
+    ln++;
+    char *h = strchr(line, '#');
+    if (h) *h = 0;
+    int a, b, sh;
+    char extra;
+    const int n = sscanf(line, "%d %d %d %c", &a, &b, &sh, &extra);
+    if (n <= 0) continue;                               // blank or comment
+    if (n != 3 || a < 0 || a > 40 || b < 0 || b > 40 || sh < 1 || sh > kNShapes) {
+      fclose(f);
+      return rsg_fail(errbuf, "rsg_gram_tiles_load: %s line %d: expected \"logN logL shape\" with shape 1..%d", path, ln,
+                      kNShapes);
+    }
+    t[std::make_pair(a, b)] = sh;
+  }
+  fclose(f);
+  return RSG_OK;
+}
+
+extern "C" int rsg_gram_tiles_load(const char *path, char *errbuf)
+{
+  std::map<std::pair<int, int>, int> t;
+  if (parse_tiles(path, t, errbuf) != RSG_OK) return RSG_FAIL;
+  std::lock_guard<std::mutex> g(g_tiles_mu);
+  g_tiles.swap(t);
+  g_tiles_tried = true;
+  return RSG_OK;
+}
+
+static int table_shape(int64_t N, int64_t L)
+{
+  std::lock_guard<std::mutex> g(g_tiles_mu);
+  if (!g_tiles_tried) {
+    g_tiles_tried = true;
+    Dl_info info;
+    if (dladdr((void *)&rsg_gram_tiles_load, &info) && info.dli_fname) {
+      std::string p(info.dli_fname);
+      const size_t k = p.rfind('/');
+      p = (k == std::string::npos ? std::string(".") : p.substr(0, k)) + "/gram_tiles.tsv";
+      std::map<std::pair<int, int>, int> t;
+      char eb[256];
+      if (FILE *f = fopen(p.c_str(), "r")) {
+        fclose(f);
+        if (parse_tiles(p.c_str(), t, eb) == RSG_OK) g_tiles.swap(t);   // a malformed table is ignored
+      }
+    }
+  }
+  auto it = g_tiles.find(std::make_pair(ceil_log2(N), ceil_log2(L)));
+  return it == g_tiles.end() ? 0 : it->second;
+}
+
 static int pick_shape(const rsg_ctx *c, const rsg_msa *m)
 {
   int s = c->gram_shape;
+  if (s >= 1 && s <= kNShapes) return s;
+  s = table_shape(m->N, m->L);
   if (s >= 1 && s <= kNShapes) return s;
   // autotuned on MI355X: 64 x 32-column tiles of four waves with 256-sequence K tiles win or tie
   // from N = 2000 to 20000 and L = 500 to 5000 (two work groups per CU, so one's epilogue runs

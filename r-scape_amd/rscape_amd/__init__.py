@@ -55,6 +55,7 @@ def _declare(L):
         "rsg_gram_nshapes": (i32, []),
         "rsg_gram_shape_name": (cp, [i32]),
         "rsg_ctx_set_gram_shape": (i32, [vp, i32]),
+        "rsg_gram_tiles_load": (i32, [cp, cp]),
         "rsg_ctx_set_gram_split": (i32, [vp, i32]),
         "rsg_ctx_set_batch_persistent": (i32, [vp, i32]),
         "rsg_ctx_set_stamp_wait": (i32, [vp, i32]),

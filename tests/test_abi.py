@@ -1,6 +1,7 @@
 """The C-ABI library loads without a GPU and exports every entry point the header declares;
 host-only entry points (no device needed) agree with the oracle."""
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -89,3 +90,25 @@ def test_null_file_format_and_read_back(tmp_path):
     g, r = gh2.h, oracle._hp(oracle.read_null(f2)).contents
     assert _fields(g) == _fields(r) and np.array_equal(g.counts(), r.counts())
     assert int(g.n) == int(counts.sum())
+
+
+def test_gram_tiles_table_load(tmp_path):
+    """The autotuner's table loads without a device; malformed lines are rejected with the line
+    number; the committed table (read by the library on first use) parses."""
+    from rscape_amd import autotune
+    L = R.lib()
+    err = C.create_string_buffer(256)
+    bad = tmp_path / "bad.tsv"
+    bad.write_text("# comment\n13 11 7\n13 11\n")
+    assert L.rsg_gram_tiles_load(str(bad).encode(), err) != 0 and b"line 3" in err.value
+    bad.write_text("13 11 99\n")
+    assert L.rsg_gram_tiles_load(str(bad).encode(), err) != 0
+    assert L.rsg_gram_tiles_load(str(tmp_path / "missing.tsv").encode(), err) != 0
+    good = tmp_path / "good.tsv"
+    autotune.write_table({(13, 11): 7, (10, 9): 4}, str(good))
+    assert L.rsg_gram_tiles_load(str(good).encode(), err) == 0
+    assert autotune.read_table(str(good)) == {(13, 11): 7, (10, 9): 4}
+    if os.path.exists(autotune.TABLE):                  # back to the committed table
+        assert L.rsg_gram_tiles_load(autotune.TABLE.encode(), err) == 0, err.value
+        tab = autotune.read_table()
+        assert tab and all(1 <= v <= L.rsg_gram_nshapes() for v in tab.values())
