@@ -516,22 +516,49 @@ __device__ __forceinline__ void stat_pairs(const StatArgs &A, int bi, int bj, co
     // four are done, so the loads (and the table reads they feed) of the four pairs overlap.
     const int ii = tid & 31;
     const int64_t i = (int64_t)bi * 32 + ii;
-    RawRec<KIND> raw[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
-      if (i < j && j < A.L) load_raw<KIND>(A.cnt, rsg_rec(i, j, A.L), raw[k]);
-    }
     double v[4];
+    if constexpr (KIND == CNT_F32 && sizeof(T) == sizeof(float)) {
+      // 64-byte records and the FP32 statistic's FP64 expectations: with two records in flight
+      // (the next one loads while this one is scored) the kernel needs ~100 VGPRs and runs four
+      // waves per SIMD; four records in flight took 176 VGPRs and two waves
+      RawRec<KIND> cur, nxt;
+      {
+        const int64_t j = (int64_t)bj * 32 + (tid >> 5);
+        if (i < j && j < A.L) load_raw<KIND>(A.cnt, rsg_rec(i, j, A.L), cur);
+      }
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
-      v[k] = -INFINITY;
-      if (i < j && j < A.L) {
-        double cd[16];
-        decode_rec<KIND>(raw[k], cd);
-        v[k] = pair_stat<T, TAB, TYPE, CLS>(A, cd, i, j, tabc);
-        lmin = fmin(lmin, v[k]); lmax = fmax(lmax, v[k]);
+      for (int k = 0; k < 4; k++) {
+        const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
+        if (k < 3) {
+          const int64_t j2 = j + 8;
+          if (i < j2 && j2 < A.L) load_raw<KIND>(A.cnt, rsg_rec(i, j2, A.L), nxt);
+        }
+        v[k] = -INFINITY;
+        if (i < j && j < A.L) {
+          double cd[16];
+          decode_rec<KIND>(cur, cd);
+          v[k] = pair_stat<T, TAB, TYPE, CLS>(A, cd, i, j, tabc);
+          lmin = fmin(lmin, v[k]); lmax = fmax(lmax, v[k]);
+        }
+        if (k < 3) cur = nxt;
+      }
+    } else {
+      RawRec<KIND> raw[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
+        if (i < j && j < A.L) load_raw<KIND>(A.cnt, rsg_rec(i, j, A.L), raw[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
+        v[k] = -INFINITY;
+        if (i < j && j < A.L) {
+          double cd[16];
+          decode_rec<KIND>(raw[k], cd);
+          v[k] = pair_stat<T, TAB, TYPE, CLS>(A, cd, i, j, tabc);
+          lmin = fmin(lmin, v[k]); lmax = fmax(lmax, v[k]);
+        }
       }
     }
 #pragma unroll
