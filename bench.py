@@ -425,6 +425,15 @@ def main():
                 "value": upd * max(3, args.steps // 4) / dtw, "unit": "updates/s",
                 "gram_bf16_ms": b_ms, "gram_bf16_tflops": 2 * 2 * 16 * upd / (b_ms * 1e-3) / 1e12,
                 "peak_tflops": PEAK_BF16_TFLOPS, "stat_ms": kw_av.get("stat", (float("nan"), 0))[0]}
+        # exact integer weighted counts: 21-bit weights in three 7-bit limbs through the int8 Gram
+        kw2 = dict(covtype="GTp", counts=R.COUNTS_W8, prec=R.STAT_FP32)
+        mw.calculate(**kw2)
+        dtw, kw_av = run_steps(ctx, mw, max(3, args.steps // 4), **kw2)
+        w_ms = kw_av.get("gram_w8", (float("nan"), 0))[0]
+        extra["weighted_int8_limbs_fp32_statistic"] = {
+            "value": upd * max(3, args.steps // 4) / dtw, "unit": "updates/s",
+            "gram_w8_ms": w_ms, "gram_w8_tops": 3 * 32 * upd / (w_ms * 1e-3) / 1e12,
+            "peak_tops": PEAK_INT8_TOPS, "stat_ms": kw_av.get("stat", (float("nan"), 0))[0]}
         mw.close()
         m = ctx.msa(ax)
         other = R.STAT_FP64 if prec == R.STAT_FP32 else R.STAT_FP32

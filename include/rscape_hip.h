@@ -42,7 +42,11 @@ enum {
   RSG_COUNTS_INT8  = 0,   /* unit weights: int8 one-hot Gram, int32 accumulation, bit-exact;
                              uint16 counts kept, so at most 65535 sequences */
   RSG_COUNTS_BF16  = 1,   /* sequence weights: split-BF16 (hi+lo) Gram, FP32 accumulation  */
-  RSG_COUNTS_BF16X3 = 2   /* sequence weights: 3-term BF16 split, FP32 accumulation        */
+  RSG_COUNTS_BF16X3 = 2,  /* sequence weights: 3-term BF16 split, FP32 accumulation        */
+  RSG_COUNTS_W8    = 3    /* sequence weights rounded to 21-bit fixed point (relative to the
+                             largest), three 7-bit limbs through the int8 Gram with exact int32
+                             accumulation, combined in FP64: counts carry only the weight
+                             rounding, |dw| <= 2^-22 max w; at most 2^21 sequences */
 };
 /* Arithmetic of the per-pair statistics. */
 enum { RSG_STAT_FP32 = 0, RSG_STAT_FP64 = 1 };

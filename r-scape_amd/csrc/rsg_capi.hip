@@ -105,7 +105,7 @@ extern "C" int rsg_ctx_set_batch_persistent(rsg_ctx *c, int on) { c->batch_persi
 
 static void free_msa(rsg_msa *m)
 {
-  void *ptrs[] = { m->d_ax, m->d_axT, m->d_xt, m->d_w, m->d_wsplit,
+  void *ptrs[] = { m->d_ax, m->d_axT, m->d_xt, m->d_w, m->d_wsplit, m->d_wq,
                    m->d_cnt, m->d_cnt_u, m->d_pm, m->d_ps, m->d_lgc, m->d_cov, m->d_tmp, m->d_rowx, m->d_eval };
   for (void *p : ptrs) if (p) hipFree(p);
   delete m;
@@ -124,9 +124,21 @@ extern "C" int rsg_msa_update(rsg_ctx *c, rsg_msa *m, const uint8_t *ax, const d
   m->wsum = 0.;
   m->weights_ok = 1;
   for (double v : w) { m->wsum += v; if (!(v >= 0.) || !std::isfinite(v)) m->weights_ok = 0; }
+  {                                 // the rounding of k_quant_weights, repeated for the rounded total
+    double wmax = 0.;
+    for (double v : w) wmax = std::fmax(wmax, v);
+    const double qmax = (double)((1 << 21) - 1), W = (wmax > 0.) ? wmax / qmax : 1.;
+    m->wsum_q = 0.;
+    for (double v : w) {
+      double q = (wmax > 0.) ? std::rint(v / W) : 0.;
+      q = std::fmin(std::fmax(q, 0.), qmax);
+      m->wsum_q += W * q;
+    }
+  }
   HIPCHK(hipMemcpyAsync(m->d_w, w.data(), sizeof(double) * (size_t)m->N, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   m->nsplit_ready = 0;
+  m->wq_ready = 0;
   m->have_probs = m->have_cov = m->have_eval = 0;
   return RSG_OK;
 }
@@ -185,6 +197,7 @@ extern "C" int rsg_corr_Probs(rsg_ctx *c, rsg_msa *m, int mode, double tol, char
 {
   HIPCHK(hipSetDevice(c->device));
   m->have_probs = m->have_cov = m->have_eval = 0;
+  m->wq_counts = 0;
   if (mode == RSG_COUNTS_INT8) {
     if (m->weighted) return rsg_fail(errbuf, "int8 counts need unit weights (alignment has weights)");
     const int kind = rsg_int_kind(m->N);
@@ -195,6 +208,13 @@ extern "C" int rsg_corr_Probs(rsg_ctx *c, rsg_msa *m, int mode, double tol, char
     m->pm_fused = 0;
     if (rsg_gram_counts_i8(c, m, m->d_cnt, kind, pm_part, (int *)(c->d_res + 4), errbuf) != RSG_OK) return RSG_FAIL;
     m->pm_fused = (pm_part != nullptr);
+  } else if (mode == RSG_COUNTS_W8) {
+    if (!m->weights_ok) return rsg_fail(errbuf, "pp validation failed: sequence weights must be finite and >= 0");
+    m->pm_fused = 0;
+    if (alloc_counts(m, &m->d_cnt, CNT_F32, errbuf) != RSG_OK) return RSG_FAIL;
+    m->cnt_kind = CNT_F32;
+    if (rsg_gram_counts_w8(c, m, (float *)m->d_cnt, errbuf) != RSG_OK) return RSG_FAIL;
+    m->wq_counts = 1;
   } else if (mode == RSG_COUNTS_BF16 || mode == RSG_COUNTS_BF16X3) {
     // a negative or non-finite weight makes pp fail corr_ValidateProbs (src/correlators.c:1499-1545)
     if (!m->weights_ok) return rsg_fail(errbuf, "pp validation failed: sequence weights must be finite and >= 0");
@@ -253,7 +273,7 @@ extern "C" int rsg_corr_GetProbs(rsg_ctx *c, rsg_msa *m, double *pp, double *pm,
         double ns = 0., tot = 0.;
         for (int k = 0; k < 16; k++) { ns += r[k]; tot += r[k] + 1e-10; }
         if (nseff) nseff[i * L + j] = nseff[j * L + i] = ns;
-        if (ngap) ngap[i * L + j] = m->wsum - ns;
+        if (ngap) ngap[i * L + j] = rsg_count_wsum(m) - ns;
         if (pp)
           for (int x = 0; x < 4; x++)
             for (int y = 0; y < 4; y++) {

@@ -517,10 +517,10 @@ __device__ __forceinline__ void stat_pairs(const StatArgs &A, int bi, int bj, co
     const int ii = tid & 31;
     const int64_t i = (int64_t)bi * 32 + ii;
     double v[4];
-    if constexpr (KIND == CNT_F32 && sizeof(T) == sizeof(float)) {
-      // 64-byte records and the FP32 statistic's FP64 expectations: with two records in flight
-      // (the next one loads while this one is scored) the kernel needs ~100 VGPRs and runs four
-      // waves per SIMD; four records in flight took 176 VGPRs and two waves
+    if constexpr (KIND == CNT_F32) {
+      // 64-byte (weighted) records: with two records in flight (the next one loads while this
+      // one is scored) the FP32 statistic needs ~100 VGPRs and runs four waves per SIMD; four
+      // records in flight took 176 VGPRs and two waves
       RawRec<KIND> cur, nxt;
       {
         const int64_t j = (int64_t)bj * 32 + (tid >> 5);

@@ -953,3 +953,195 @@ int rsg_gram_counts_bf16(rsg_ctx *c, rsg_msa *m, int nsplit, float *out, char *e
   if (nsplit > 2) return big ? launch_bf16<2, 2, 64, 3>(c, m, out, errbuf) : launch_bf16<1, 1, 64, 3>(c, m, out, errbuf);
   return big ? launch_bf16<2, 2, 64, 2>(c, m, out, errbuf) : launch_bf16<1, 1, 64, 2>(c, m, out, errbuf);
 }
+
+// ------------------------------------------------------------------------------------
+// Weighted Gram from int8 weight limbs (exact integer accumulation).  Each sequence weight is
+// rounded to 21-bit fixed point relative to the largest weight, q_s = round(w_s / W) with
+// W = max_s w_s / (2^21 - 1), and split into three 7-bit limbs q = q0 + 2^7 q1 + 2^14 q2.  The
+// int8 MFMA then forms, per limb k, G_k[x][y] = sum_s q_{s,k} [a_si = x][a_sj = y] exactly in
+// int32 (the j-side operand is the one-hot code masked for symbol y, scaled by 2^y; the i-side
+// operand is the limb byte where the code is symbol x, else 0), and the epilogue combines
+// W (G_0 + 2^7 G_1 + 2^14 G_2) in FP64 (exact) before the FP32 record store.  The only error
+// is the weight rounding: |w_s - W q_s| <= W / 2 = 2^-22 max_s w_s.
+//
+// Layout of the limb panel: [Npad / 16][4][16] bytes, limbs 0..2 of the 16 sequences of a
+// K-chunk (slot 3 zero), so one stage's limbs are NKC * 64 contiguous bytes, staged into LDS
+// behind the stage's codes by wave 0 (one more LDS-direct instruction for that wave).
+#define W8_BITS 21
+__global__ void __launch_bounds__(1024) k_quant_weights(const double *__restrict__ w, int64_t N, int64_t Npad,
+                                                         uint8_t *__restrict__ wq, double *__restrict__ scale)
+{
+  __shared__ double red[1024];
+  const int tid = threadIdx.x;
+  double mx = 0.;
+  for (int64_t s = tid; s < N; s += 1024) mx = fmax(mx, w[s]);
+  red[tid] = mx;
+  __syncthreads();
+  for (int st = 512; st > 0; st >>= 1) {
+    if (tid < st) red[tid] = fmax(red[tid], red[tid + st]);
+    __syncthreads();
+  }
+  const double wmax = red[0];
+  const double qmax = (double)((1 << W8_BITS) - 1);
+  const double W = (wmax > 0.) ? wmax / qmax : 1.;
+  if (tid == 0) *scale = W;
+  for (int64_t s = tid; s < Npad; s += 1024) {
+    double qd = (s < N && wmax > 0.) ? rint(w[s] / W) : 0.;
+    qd = fmin(fmax(qd, 0.), qmax);
+    const uint32_t q = (uint32_t)qd;
+    uint8_t *p = wq + (s / 16) * 64 + (s % 16);
+    p[0] = (uint8_t)(q & 127u); p[16] = (uint8_t)((q >> 7) & 127u); p[32] = (uint8_t)((q >> 14) & 127u); p[48] = 0;
+  }
+}
+
+// One 32 x 32-column tile, four waves: wave x owns the i-side symbol x and all four j-side
+// symbols for the three limbs (12 accumulators of 16 int32 per lane).  Staging as in the int8
+// Gram (three LDS images, LDS-direct loads two stages ahead), each image followed by the
+// stage's limb bytes.
+template <int KT>
+__device__ __forceinline__ void gram_tile_w8(const uint8_t *__restrict__ xt, const uint8_t *__restrict__ wq, double W,
+                                             int64_t Lpad, int64_t L, float *__restrict__ out, int64_t i0, int64_t j0,
+                                             uint8_t *lds, int64_t nkt)
+{
+#pragma clang fp contract(off)
+  constexpr int NT = 256, KS = KT / 32, NW = 4;
+  typedef CodeStage<32, 32, KT, NT> St;
+  constexpr int IS = St::BYTES + St::NKC * 64;     // one image: codes, then limbs
+  constexpr int WCH = St::NKC * 4;                 // 16-byte limb chunks per stage
+  constexpr int QW = St::NINST / NW;               // code loads per wave per stage (+1 for wave 0)
+  static_assert(QW >= 1 && QW + 1 <= 15 && WCH <= 64, "stage shape");
+  static_assert(KS >= 2, "the barrier precedes the last k-step");
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, x = wave;
+  const int col = lane & 31, khalf = lane >> 5;
+  const int offA = col * 16 + khalf * St::CSA;                           // j side
+  const int offB = St::NKC * St::CSA + col * 16 + khalf * St::CSB;       // i side
+  const int offW = St::BYTES + khalf * 64;                               // limbs of the lane's chunk
+
+  v16i acc[3][4];                                  // [limb][y]
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[k][y][r] = 0;
+
+  auto stage = [&](int64_t kt, uint8_t *base) {
+    stage_lds_direct<St, 32, 32, NW>(xt, Lpad, i0, j0, kt, base, wave, lane);
+    if (wave == 0 && lane < WCH) lds_direct16(wq + (kt * WCH + lane) * 16, base + St::BYTES);
+  };
+  auto wait_stage = [&]() {                        // this wave's loads of the older stage landed
+    if (wave == 0) __builtin_amdgcn_s_waitcnt(0x0F70 | (QW + 1));
+    else           __builtin_amdgcn_s_waitcnt(0x0F70 | QW);
+  };
+  const int nit = (int)nkt;
+  auto stage_of = [&](int it) -> int64_t { return (it < nit) ? it : nit - 1; };
+  stage(stage_of(0), lds);
+  stage(stage_of(1), lds + IS);
+  wait_stage();
+  __syncthreads();
+  uint4 ra = *(const uint4 *)(lds + offA);
+  uint4 rb = *(const uint4 *)(lds + offB);
+  for (int it = 0; it < nit; it++) {
+    stage(stage_of(it + 2), lds + ((it + 2) % 3) * IS);
+    const uint8_t *base = lds + (it % 3) * IS;
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      // this k-step's limbs: read before the last k-step's barrier, after which the image may
+      // be overwritten by the next iteration's loads
+      const uint4 *wp = (const uint4 *)(base + offW + ks * 128);
+      const uint4 w0 = wp[0], w1 = wp[1], w2 = wp[2];
+      uint4 na = ra, nb = rb;
+      if (ks + 1 < KS) {
+        na = *(const uint4 *)(base + offA + (ks + 1) * 2 * St::CSA);
+        nb = *(const uint4 *)(base + offB + (ks + 1) * 2 * St::CSB);
+      } else {
+        wait_stage();
+        __syncthreads();
+        const uint8_t *nbase = lds + ((it + 1) % 3) * IS;
+        na = *(const uint4 *)(nbase + offA);
+        nb = *(const uint4 *)(nbase + offB);
+      }
+      // i side: byte 0xFF where the code is symbol x, else 0
+      const unsigned t0 = (rb.x >> x) & 0x01010101u, t1 = (rb.y >> x) & 0x01010101u,
+                     t2 = (rb.z >> x) & 0x01010101u, t3 = (rb.w >> x) & 0x01010101u;
+      const unsigned m0 = (t0 << 8) - t0, m1 = (t1 << 8) - t1, m2 = (t2 << 8) - t2, m3 = (t3 << 8) - t3;
+      v4i a[4];
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        const unsigned mk = 0x01010101u << y;
+        a[y] = (v4i){(int)(ra.x & mk), (int)(ra.y & mk), (int)(ra.z & mk), (int)(ra.w & mk)};
+      }
+      const v4i b0 = (v4i){(int)(m0 & w0.x), (int)(m1 & w0.y), (int)(m2 & w0.z), (int)(m3 & w0.w)};
+#pragma unroll
+      for (int y = 0; y < 4; y++) acc[0][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[y], b0, acc[0][y], 0, 0, 0);
+      const v4i b1 = (v4i){(int)(m0 & w1.x), (int)(m1 & w1.y), (int)(m2 & w1.z), (int)(m3 & w1.w)};
+#pragma unroll
+      for (int y = 0; y < 4; y++) acc[1][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[y], b1, acc[1][y], 0, 0, 0);
+      const v4i b2 = (v4i){(int)(m0 & w2.x), (int)(m1 & w2.y), (int)(m2 & w2.z), (int)(m3 & w2.w)};
+#pragma unroll
+      for (int y = 0; y < 4; y++) acc[2][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[y], b2, acc[2][y], 0, 0, 0);
+      ra = na; rb = nb;
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0x0F70);             // no LDS-direct load outlives the loop
+
+  // epilogue: lane owns i column `col`; register r holds j = 8*(r/4) + 4*khalf + r%4; cells
+  // x*4 .. x*4+3 of each record are this wave's
+  const int64_t i = i0 + col;
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int64_t j = j0 + 8 * (r >> 2) + 4 * khalf + (r & 3);
+    if (!(i < j && j < L)) continue;
+    float v[4];
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const double g = (double)((unsigned)acc[0][y][r] >> y) + 128. * (double)((unsigned)acc[1][y][r] >> y) +
+                       16384. * (double)((unsigned)acc[2][y][r] >> y);
+      v[y] = (float)(g * W);
+    }
+    *(float4 *)(out + rsg_rec(i, j, L) * 16 + x * 4) = make_float4(v[0], v[1], v[2], v[3]);
+  }
+}
+
+template <int KT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_gram_w8(const uint8_t *__restrict__ xt, const uint8_t *__restrict__ wq, const double *__restrict__ wscale, int64_t Npad,
+          int64_t Lpad, int64_t L, float *__restrict__ out, const int2 *__restrict__ tiles, int ntiles, int per_xcd)
+{
+  typedef CodeStage<32, 32, KT, 256> St;
+  __shared__ __attribute__((aligned(16))) uint8_t lds[3 * (St::BYTES + St::NKC * 64)];
+  const int g = blockIdx.x;
+  const int t = (g & 7) * per_xcd + (g >> 3);            // neighbouring tiles on one XCD
+  if (t >= ntiles) return;
+  const int2 tl = tiles[t];
+  gram_tile_w8<KT>(xt, wq, *wscale, Lpad, L, out, (int64_t)tl.x * 32, (int64_t)tl.y * 32, lds, Npad / KT);
+}
+
+int rsg_gram_counts_w8(rsg_ctx *c, rsg_msa *m, float *out, char *errbuf)
+{
+  constexpr int KT = 256;
+  if (m->Npad % KT || m->Lpad % 32) return rsg_fail(errbuf, "gram: padding does not fit the tile");
+  if (m->N > (1 << 21)) return rsg_fail(errbuf, "weighted int8 counts: at most %d sequences", 1 << 21);
+  if (!m->d_wq) {
+    HIPCHK(hipMalloc(&m->d_wq, (size_t)m->Npad * 4 + 64));
+  }
+  double *d_scale = (double *)(m->d_wq + (size_t)m->Npad * 4);
+  if (!m->wq_ready) {
+    hipLaunchKernelGGL(k_quant_weights, dim3(1), dim3(1024), 0, c->stream, m->d_w, m->N, m->Npad, m->d_wq, d_scale);
+    HIPCHK(hipGetLastError());
+    m->wq_ready = 1;
+  }
+  int ntiles = 0;
+  const int2 *tiles;
+  if (c->tiles_fixed) { tiles = c->tiles_fixed; ntiles = c->ntiles_fixed; }
+  else tiles = tile_list(c->device, m->L, 32, 32, &ntiles, errbuf);
+  if (!tiles) return RSG_FAIL;
+  if (ntiles == 0) return RSG_OK;
+  const int per = (ntiles + 7) / 8;
+  int t; rsg_tstart(c, "gram_w8", &t);
+  hipLaunchKernelGGL((k_gram_w8<KT>), dim3((unsigned)(8 * per)), dim3(256), 0, c->stream, m->d_xt, m->d_wq, d_scale, m->Npad,
+                     m->Lpad, m->L, out, tiles, ntiles, per);
+  rsg_tstop(c, t);
+  HIPCHK(hipGetLastError());
+  return RSG_OK;
+}

@@ -120,6 +120,10 @@ struct rsg_msa {
   // split-BF16 weight parts of the weighted Gram: [3][Npad] bf16 bits
   uint16_t *d_wsplit = nullptr;
   int      nsplit_ready = 0;
+  uint8_t *d_wq = nullptr;      // int8 weight limbs [Npad/16][4][16] and (after them) the FP64 scale
+  int      wq_ready = 0;
+  int      wq_counts = 0;       // the current counts come from the rounded weights (RSG_COUNTS_W8)
+  double   wsum_q = 0.;         // sum of the rounded weights W * q_s
   // pair tables
   void    *d_cnt = nullptr;     // L x L x 16 (i<j records), kind below
   int      cnt_kind = -1;
@@ -139,6 +143,11 @@ struct rsg_msa {
   double   minCOV = 0., maxCOV = 0.;
   int      have_probs = 0, have_cov = 0;
 };
+
+// Total weight the counts were formed with (ngap = total - nseff): the rounded weights' sum for
+// counts from int8 weight limbs, so ngap is the gapped sequences' rounded weight
+static inline double rsg_count_wsum(const rsg_msa *m) { return m->wq_counts ? m->wsum_q : m->wsum; }
+
 
 // ---- error helpers ----
 static inline int rsg_fail(char *errbuf, const char *fmt, ...)
@@ -186,6 +195,7 @@ int rsg_msa_alloc(rsg_ctx *c, int64_t nseq, int64_t alen, rsg_msa **ret, char *e
 // (pm32 layout, rsg_device.h); pmflag: cleared for the marginal reduction that follows
 int rsg_gram_counts_i8(rsg_ctx *c, rsg_msa *m, void *d_out, int kind, double *pm_part, int *pmflag, char *errbuf);
 int rsg_gram_counts_bf16(rsg_ctx *c, rsg_msa *m, int nsplit, float *d_out, char *errbuf);
+int rsg_gram_counts_w8(rsg_ctx *c, rsg_msa *m, float *d_out, char *errbuf);
 int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf);
 double *rsg_pm32_scratch(rsg_ctx *c, int64_t L, char *errbuf);
 int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, int prec, char *errbuf);

@@ -237,9 +237,9 @@ extern "C" int rsg_power_SPAIR_Create(rsg_ctx *c, rsg_msa *m, const int *msamap,
     double4 *d_sum = nullptr;
     HIPCHK(hipMalloc(&d_sum, sizeof(double4) * (size_t)dim));
     const dim3 g((unsigned)((L * L + 255) / 256));
-    if (m->cnt_kind == CNT_U16)      hipLaunchKernelGGL(k_pair_summary<CNT_U16>, g, dim3(256), 0, c->stream, m->d_cnt, L, m->wsum, d_sum);
-    else if (m->cnt_kind == CNT_U32) hipLaunchKernelGGL(k_pair_summary<CNT_U32>, g, dim3(256), 0, c->stream, m->d_cnt, L, m->wsum, d_sum);
-    else                             hipLaunchKernelGGL(k_pair_summary<CNT_F32>, g, dim3(256), 0, c->stream, m->d_cnt, L, m->wsum, d_sum);
+    if (m->cnt_kind == CNT_U16)      hipLaunchKernelGGL(k_pair_summary<CNT_U16>, g, dim3(256), 0, c->stream, m->d_cnt, L, rsg_count_wsum(m), d_sum);
+    else if (m->cnt_kind == CNT_U32) hipLaunchKernelGGL(k_pair_summary<CNT_U32>, g, dim3(256), 0, c->stream, m->d_cnt, L, rsg_count_wsum(m), d_sum);
+    else                             hipLaunchKernelGGL(k_pair_summary<CNT_F32>, g, dim3(256), 0, c->stream, m->d_cnt, L, rsg_count_wsum(m), d_sum);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(sum.data(), d_sum, sizeof(double4) * (size_t)dim, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

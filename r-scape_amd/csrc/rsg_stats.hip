@@ -241,6 +241,17 @@ static int launch_stat_fixed(rsg_ctx *c, const StatArgs &A, int cls, int nb)
   return ntri;
 }
 
+// FP64 statistics of class C16 from FP32 (weighted) records with the statistic fixed at compile
+// time (inline log: non-integer counts have no table); 0 when not handled here.
+template <int TYPE>
+static int launch_stat_f64w_fixed(rsg_ctx *c, const StatArgs &A, int cls, int nb)
+{
+  if (cls != RSG_C16) return 0;
+  const int ntri = nb * (nb + 1) / 2;
+  hipLaunchKernelGGL((k_stat<double, CNT_F32, false, TYPE, RSG_C16>), dim3((unsigned)ntri), dim3(256), 0, c->stream, A, nb);
+  return ntri;
+}
+
 // FP32 statistics (pair_stat_f32) of class C16 with the statistic fixed at compile time, from
 // uint16 or FP32 (weighted) records; 0 when not handled here.
 template <int TYPE>
@@ -346,7 +357,7 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
   if (ensure_red(c, 2 * (size_t)nblk + 8 + 4 * (size_t)L, errbuf) != RSG_OK) return RSG_FAIL;
   StatArgs A;
   A.cnt = m->d_cnt; A.pm = m->d_pm; A.lpm = m->d_pm + 4 * L; A.lgc = m->d_lgc;
-  A.L = L; A.N = m->N; A.wsum = m->wsum; A.cov = m->d_cov; A.part = c->d_red;
+  A.L = L; A.N = m->N; A.wsum = rsg_count_wsum(m); A.cov = m->d_cov; A.part = c->d_red;
   A.type = base; A.cls = cls;
   A.allow = 0;
   for (int k = 0; k < 16; k++) if (allow16[k] > 0) A.allow |= 1u << k;
@@ -377,6 +388,15 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
     case RSG_MIg:  nparts = launch_stat_fixed<RSG_MIg>(c, A, cls, nb); break;
     case RSG_CHI:  nparts = launch_stat_fixed<RSG_CHI>(c, A, cls, nb); break;
     case RSG_OMES: nparts = launch_stat_fixed<RSG_OMES>(c, A, cls, nb); break;
+    }
+  }
+  if (prec == RSG_STAT_FP64 && m->cnt_kind == CNT_F32) {
+    switch (base) {
+    case RSG_GT:   nparts = launch_stat_f64w_fixed<RSG_GT>(c, A, cls, nb); break;
+    case RSG_MI:   nparts = launch_stat_f64w_fixed<RSG_MI>(c, A, cls, nb); break;
+    case RSG_MIr:  nparts = launch_stat_f64w_fixed<RSG_MIr>(c, A, cls, nb); break;
+    case RSG_MIg:  nparts = launch_stat_f64w_fixed<RSG_MIg>(c, A, cls, nb); break;
+    case RSG_CHI:  nparts = launch_stat_f64w_fixed<RSG_CHI>(c, A, cls, nb); break;
     }
   }
   if (prec == RSG_STAT_FP32) {

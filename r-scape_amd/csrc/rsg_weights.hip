@@ -393,7 +393,7 @@ extern "C" int rsg_msa_RemoveGapColumns(rsg_ctx *c, const rsg_msa *src, double g
       return rsg_fail(errbuf, "rsg_msa_RemoveGapColumns: column subset failed");
     }
   }
-  m->weighted = src->weighted; m->wsum = src->wsum; m->weights_ok = src->weights_ok;
+  m->weighted = src->weighted; m->wsum = src->wsum; m->wsum_q = src->wsum_q; m->weights_ok = src->weights_ok;
   *ret = m;
   return RSG_OK;
 }

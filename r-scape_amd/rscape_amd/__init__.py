@@ -19,7 +19,7 @@ COVTYPES = ["CHI", "CHIp", "CHIa", "GT", "GTp", "GTa", "MI", "MIp", "MIa", "MIr"
 COV = {n: k for k, n in enumerate(COVTYPES)}
 C16, C2, CWC, CSELECT = 0, 1, 2, 3
 APC, ASC = 0, 1
-COUNTS_INT8, COUNTS_BF16, COUNTS_BF16X3 = 0, 1, 2
+COUNTS_INT8, COUNTS_BF16, COUNTS_BF16X3, COUNTS_W8 = 0, 1, 2, 3
 STAT_FP32, STAT_FP64 = 0, 1
 
 # R-scape's default allowed pairs: Watson-Crick + G:U (src/R-scape.c:882-887)

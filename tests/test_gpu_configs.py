@@ -109,7 +109,14 @@ def test_c1_weighted(ctx, c1):
         rel = float(np.max(np.abs(g - ref_cnt) / np.maximum(np.abs(ref_cnt), 1.0)))
         print("c1 weighted counts mode %d: max rel err %.3e" % (mode, rel), flush=True)
         assert rel < bound
-    del ref_cnt, pp
+    # int8 weight limbs: only the 21-bit weight rounding (2^-22 max w per sequence) and one FP32
+    # rounding of the record
+    m.probs(R.COUNTS_W8)
+    g = m.counts().reshape(L, L, 16)[up]
+    rel8 = float(np.max(np.abs(g - ref_cnt) / np.maximum(np.abs(ref_cnt), 1.0)))
+    print("c1 weighted counts int8 limbs: max rel err %.3e" % rel8, flush=True)
+    assert rel8 < 2.0 ** -22 * w.max() / w.min() + 2.0 ** -22
+    del ref_cnt, pp, g
     for name, otype in (("GT", oracle.GT), ("MI", oracle.MI), ("MIr", oracle.MIr), ("MIg", oracle.MIg),
                         ("CHI", oracle.CHI)):
         ref, _ = oracle.cov(pr, otype, oracle.CSELECT)
@@ -121,6 +128,11 @@ def test_c1_weighted(ctx, c1):
         # the FP32-accumulated counts carry ~2e-6; MIg subtracts ngap / nseff with ngap formed as
         # sum w - nseff, which magnifies the count error where few sequences are gapped
         assert e < (1e-3 if name == "MIg" else 1e-4)
+        m.calculate(name + "p", counts=R.COUNTS_W8, prec=R.STAT_FP32)
+        e8 = _relerr(m.get_cov()[0][up], refc[up])
+        print("c1 weighted %sp (int8-limb counts, FP32 statistic) vs FP64: max rel err %.3e" % (name, e8), flush=True)
+        # MIg: ngap = total - nseff cancels ~90% of nseff, formed from the FP32 records
+        assert e8 < (1e-4 if name == "MIg" else 1e-5)
     m.close()
 
 

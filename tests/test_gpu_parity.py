@@ -249,6 +249,53 @@ def test_weighted_counts_split_bf16(ctx, mode, bound):
     assert e < 50 * bound
 
 
+def _weighted_counts_numpy(ax, w):
+    """Textbook weighted pair counts: X^T diag(w) X over the one-hot matrix X (N x 4L) in
+    float64; cnt[i, j, x*4+y] = sum_s w_s [a_si = x][a_sj = y]."""
+    N, L = ax.shape
+    X = np.zeros((N, L, 4))
+    for x in range(4):
+        X[:, :, x] = ax == x
+    X = X.reshape(N, 4 * L)
+    G = (X * w[:, None]).T @ X
+    return G.reshape(L, 4, L, 4).transpose(0, 2, 1, 3).reshape(L, L, 16)
+
+
+@pytest.mark.parametrize("spread", [(0.05, 3.0), (1e-3, 10.0)])
+def test_weighted_counts_int8_limbs(ctx, spread):
+    """COUNTS_W8: the weights rounded to 21-bit fixed point (q = rint(w / W), W = max w /
+    (2^21 - 1)), three 7-bit limbs through the int8 Gram.  Against numpy's FP64 counts with the
+    rounded weights the records are W * (an exact integer) rounded once to FP32 (<= 1 ulp);
+    against the true weights the difference is at most the rounding, 2^-22 max w per sequence.
+    Statistics from these counts are within 1e-5 of the FP64 oracle on the true weights."""
+    print("w8 counts, weights U%s" % (spread,), flush=True)
+    N, L = 700, 96
+    rng = np.random.default_rng(67)
+    ax = _msa(N, L, 67)
+    w = np.exp(rng.uniform(np.log(spread[0]), np.log(spread[1]), size=N))
+    m = ctx.msa(ax, w)
+    m.probs(R.COUNTS_W8)
+    got = m.counts().reshape(L, L, 16)
+    iu = np.triu_indices(L, 1)
+    g = got[iu]
+    W = w.max() / (2 ** 21 - 1)
+    wq = W * np.rint(w / W)
+    refq = _weighted_counts_numpy(ax, wq)[iu]
+    assert np.all(np.abs(g - refq) <= 2.0 ** -23 * np.abs(refq) + 1e-300)
+    ref = _weighted_counts_numpy(ax, w)[iu]
+    nunit = _weighted_counts_numpy(ax, np.ones(N))[iu]
+    assert np.all(np.abs(g - ref) <= 2.0 ** -22 * w.max() * nunit + 2.0 ** -23 * np.abs(ref))
+    print("w8 counts: max rel err vs true weights %.3e" % np.max(np.abs(g - ref) / np.maximum(ref, 1.0)))
+    pr = oracle.probs(ax, w)
+    for name, ot in STATS[:6]:
+        m.cov(name, R.C16, prec=R.STAT_FP32)
+        got_s, _, _ = m.get_cov()
+        ref_s, _ = oracle.cov(pr, ot, R.C16)
+        e = _relerr(_upper(got_s, L), _upper(ref_s, L))
+        print("w8 counts + FP32 %s vs FP64 weighted oracle: %.3e" % (name, e))
+        assert e < 1e-5, name
+
+
 PIPE = [("GTp", oracle.GT, R.APC), ("GT", oracle.GT, None), ("GTa", oracle.GT, R.ASC),
         ("MIp", oracle.MI, R.APC), ("CCFp", oracle.CCF, R.APC), ("RAF", oracle.RAF, None),
         ("RAFS", oracle.RAFS, None), ("CHIa", oracle.CHI, R.ASC)]
