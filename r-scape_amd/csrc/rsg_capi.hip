@@ -105,7 +105,7 @@ extern "C" int rsg_ctx_set_batch_persistent(rsg_ctx *c, int on) { c->batch_persi
 
 static void free_msa(rsg_msa *m)
 {
-  void *ptrs[] = { m->d_ax, m->d_axT, m->d_xt, m->d_w, m->d_wsplit, m->d_wq,
+  void *ptrs[] = { m->d_ax, m->d_axT, m->d_xt, m->d_w, m->d_wsplit, m->d_wq, m->d_nsx,
                    m->d_cnt, m->d_cnt_u, m->d_pm, m->d_ps, m->d_lgc, m->d_cov, m->d_tmp, m->d_rowx, m->d_eval };
   for (void *p : ptrs) if (p) hipFree(p);
   delete m;

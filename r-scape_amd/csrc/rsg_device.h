@@ -59,6 +59,7 @@ struct StatArgs {
   int64_t L, N; double wsum; double *cov; double *part;  // part: 2 per block (min, max)
   int type, cls; unsigned allow;                           // allow: 16-bit mask of allowed cells
   double meanp[4];                                         // CCF
+  const double *nsx;                                       // nseff per record (FP64), or nullptr: sum of the cells
 };
 
 template <typename T> __device__ __forceinline__ T tlog(T v);
@@ -167,7 +168,12 @@ __device__ double pair_stat_f32(const StatArgs &A, const double *cd, int64_t i, 
   switch (type) {
   case RSG_GT:  return (ns > 0.) ? 2. * ns * s : 0.;
   case RSG_MI:  return s;
-  case RSG_MIg: return s - ((ns > 0.) ? (A.wsum - ns) / ns : 0.);
+  case RSG_MIg: {
+    // ngap = total - nseff cancels most of nseff: take nseff exact where the counts provide it
+    // (the probabilities keep the cells' own sum, which makes sum p = sum q = 1)
+    const double ng = A.nsx ? A.nsx[rsg_rec(i, j, A.L)] : ns;
+    return s - ((ng > 0.) ? (A.wsum - ng) / ng : 0.);
+  }
   default:      return (sh > 1e-2f) ? s / (double)sh : 0.;  // MIr
   }
 }
@@ -185,6 +191,7 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
   double nsd = 0.;
 #pragma unroll
   for (int k = 0; k < 16; k++) { ns += (T)cd[k]; nsd += cd[k]; }
+  const double nsg = (!TAB && A.nsx) ? A.nsx[rsg_rec(i, j, A.L)] : nsd;   // MIg's nseff (exact where given)
   T pmi[4], pmj[4], lpi[4], lpj[4];
 #pragma unroll
   for (int x = 0; x < 4; x++) {
@@ -240,7 +247,7 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
     case RSG_MI: case RSG_MIg:
       if (pw > 0) v += pw * (tlog<T>(pw) - tlog<T>(qw));
       if (pn > 0) v += pn * (tlog<T>(pn) - tlog<T>(qn));
-      if (type == RSG_MIg && ns > 0) v -= (T)((A.wsum - nsd) / nsd);
+      if (type == RSG_MIg && ns > 0) v -= (T)((A.wsum - nsg) / nsg);
       break;
     case RSG_MIr:
       if (pw > 0) H -= pw * tlog<T>(pw);
@@ -309,7 +316,7 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
   case RSG_CHI: case RSG_OMES: return (ns > 0) ? (double)(ns * s) : 0.;
   case RSG_GT:   return (ns > 0) ? 2. * (double)ns * (double)s : 0.;
   case RSG_MI:   return (double)s;
-  case RSG_MIg:  return (double)s - ((nsd > 0) ? (A.wsum - nsd) / nsd : 0.);
+  case RSG_MIg:  return (double)s - ((nsg > 0) ? (A.wsum - nsg) / nsg : 0.);
   case RSG_MIr:  return (H > (T)1e-2) ? (double)(s / H) : 0.;
   }
   return 0.;

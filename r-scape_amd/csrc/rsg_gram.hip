@@ -1000,8 +1000,8 @@ __global__ void __launch_bounds__(1024) k_quant_weights(const double *__restrict
 // stage's limb bytes.
 template <int KT>
 __device__ __forceinline__ void gram_tile_w8(const uint8_t *__restrict__ xt, const uint8_t *__restrict__ wq, double W,
-                                             int64_t Lpad, int64_t L, float *__restrict__ out, int64_t i0, int64_t j0,
-                                             uint8_t *lds, int64_t nkt)
+                                             int64_t Lpad, int64_t L, float *__restrict__ out, double *__restrict__ nsx,
+                                             int64_t i0, int64_t j0, uint8_t *lds, int64_t nkt)
 {
 #pragma clang fp contract(off)
   constexpr int NT = 256, KS = KT / 32, NW = 4;
@@ -1086,27 +1086,45 @@ __device__ __forceinline__ void gram_tile_w8(const uint8_t *__restrict__ xt, con
   __builtin_amdgcn_s_waitcnt(0x0F70);             // no LDS-direct load outlives the loop
 
   // epilogue: lane owns i column `col`; register r holds j = 8*(r/4) + 4*khalf + r%4; cells
-  // x*4 .. x*4+3 of each record are this wave's
+  // x*4 .. x*4+3 of each record are this wave's.  nseff of every pair comes from the exact
+  // integer sums (the FP32 cells each carry a rounding, and ngap = total - nseff cancels most of
+  // nseff): the four waves' row sums meet in LDS, which every wave has finished with after the
+  // barrier, and one rounding W * sum goes to nsx.
+  __syncthreads();
+  int64_t *xs = (int64_t *)lds;                    // [wave][r][lane]
   const int64_t i = i0 + col;
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     const int64_t j = j0 + 8 * (r >> 2) + 4 * khalf + (r & 3);
-    if (!(i < j && j < L)) continue;
-    float v[4];
+    int64_t g[4], sx = 0;
 #pragma unroll
     for (int y = 0; y < 4; y++) {
-      const double g = (double)((unsigned)acc[0][y][r] >> y) + 128. * (double)((unsigned)acc[1][y][r] >> y) +
-                       16384. * (double)((unsigned)acc[2][y][r] >> y);
-      v[y] = (float)(g * W);
+      g[y] = (int64_t)((unsigned)acc[0][y][r] >> y) + 128 * (int64_t)((unsigned)acc[1][y][r] >> y) +
+             16384 * (int64_t)((unsigned)acc[2][y][r] >> y);
+      sx += g[y];
     }
-    *(float4 *)(out + rsg_rec(i, j, L) * 16 + x * 4) = make_float4(v[0], v[1], v[2], v[3]);
+    xs[(wave * 16 + r) * 64 + lane] = sx;
+    if (i < j && j < L)
+      *(float4 *)(out + rsg_rec(i, j, L) * 16 + x * 4) =
+          make_float4((float)((double)g[0] * W), (float)((double)g[1] * W), (float)((double)g[2] * W), (float)((double)g[3] * W));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int r = 4 * q + wave;
+    const int64_t j = j0 + 8 * (r >> 2) + 4 * khalf + (r & 3);
+    if (!(i < j && j < L)) continue;
+    const int64_t tot = xs[(0 * 16 + r) * 64 + lane] + xs[(1 * 16 + r) * 64 + lane] + xs[(2 * 16 + r) * 64 + lane] +
+                        xs[(3 * 16 + r) * 64 + lane];
+    nsx[rsg_rec(i, j, L)] = (double)tot * W;
   }
 }
 
 template <int KT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_gram_w8(const uint8_t *__restrict__ xt, const uint8_t *__restrict__ wq, const double *__restrict__ wscale, int64_t Npad,
-          int64_t Lpad, int64_t L, float *__restrict__ out, const int2 *__restrict__ tiles, int ntiles, int per_xcd)
+          int64_t Lpad, int64_t L, float *__restrict__ out, double *__restrict__ nsx, const int2 *__restrict__ tiles,
+          int ntiles, int per_xcd)
 {
   typedef CodeStage<32, 32, KT, 256> St;
   __shared__ __attribute__((aligned(16))) uint8_t lds[3 * (St::BYTES + St::NKC * 64)];
@@ -1114,7 +1132,7 @@ k_gram_w8(const uint8_t *__restrict__ xt, const uint8_t *__restrict__ wq, const 
   const int t = (g & 7) * per_xcd + (g >> 3);            // neighbouring tiles on one XCD
   if (t >= ntiles) return;
   const int2 tl = tiles[t];
-  gram_tile_w8<KT>(xt, wq, *wscale, Lpad, L, out, (int64_t)tl.x * 32, (int64_t)tl.y * 32, lds, Npad / KT);
+  gram_tile_w8<KT>(xt, wq, *wscale, Lpad, L, out, nsx, (int64_t)tl.x * 32, (int64_t)tl.y * 32, lds, Npad / KT);
 }
 
 int rsg_gram_counts_w8(rsg_ctx *c, rsg_msa *m, float *out, char *errbuf)
@@ -1131,6 +1149,7 @@ int rsg_gram_counts_w8(rsg_ctx *c, rsg_msa *m, float *out, char *errbuf)
     HIPCHK(hipGetLastError());
     m->wq_ready = 1;
   }
+  if (!m->d_nsx) HIPCHK(hipMalloc(&m->d_nsx, sizeof(double) * (size_t)m->L * (size_t)m->L));
   int ntiles = 0;
   const int2 *tiles;
   if (c->tiles_fixed) { tiles = c->tiles_fixed; ntiles = c->ntiles_fixed; }
@@ -1140,7 +1159,7 @@ int rsg_gram_counts_w8(rsg_ctx *c, rsg_msa *m, float *out, char *errbuf)
   const int per = (ntiles + 7) / 8;
   int t; rsg_tstart(c, "gram_w8", &t);
   hipLaunchKernelGGL((k_gram_w8<KT>), dim3((unsigned)(8 * per)), dim3(256), 0, c->stream, m->d_xt, m->d_wq, d_scale, m->Npad,
-                     m->Lpad, m->L, out, tiles, ntiles, per);
+                     m->Lpad, m->L, out, m->d_nsx, tiles, ntiles, per);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   return RSG_OK;

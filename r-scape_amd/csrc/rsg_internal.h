@@ -123,6 +123,7 @@ struct rsg_msa {
   uint8_t *d_wq = nullptr;      // int8 weight limbs [Npad/16][4][16] and (after them) the FP64 scale
   int      wq_ready = 0;
   int      wq_counts = 0;       // the current counts come from the rounded weights (RSG_COUNTS_W8)
+  double  *d_nsx = nullptr;     // with wq_counts: nseff of every pair (record index) in FP64
   double   wsum_q = 0.;         // sum of the rounded weights W * q_s
   // pair tables
   void    *d_cnt = nullptr;     // L x L x 16 (i<j records), kind below

@@ -359,6 +359,7 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
   A.cnt = m->d_cnt; A.pm = m->d_pm; A.lpm = m->d_pm + 4 * L; A.lgc = m->d_lgc;
   A.L = L; A.N = m->N; A.wsum = rsg_count_wsum(m); A.cov = m->d_cov; A.part = c->d_red;
   A.type = base; A.cls = cls;
+  A.nsx = m->wq_counts ? m->d_nsx : nullptr;
   A.allow = 0;
   for (int k = 0; k < 16; k++) if (allow16[k] > 0) A.allow |= 1u << k;
   for (int x = 0; x < 4; x++) A.meanp[x] = 0.;

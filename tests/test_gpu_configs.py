@@ -131,8 +131,7 @@ def test_c1_weighted(ctx, c1):
         m.calculate(name + "p", counts=R.COUNTS_W8, prec=R.STAT_FP32)
         e8 = _relerr(m.get_cov()[0][up], refc[up])
         print("c1 weighted %sp (int8-limb counts, FP32 statistic) vs FP64: max rel err %.3e" % (name, e8), flush=True)
-        # MIg: ngap = total - nseff cancels ~90% of nseff, formed from the FP32 records
-        assert e8 < (1e-4 if name == "MIg" else 1e-5)
+        assert e8 < 1e-5
     m.close()
 
 
