@@ -3,6 +3,10 @@
     python -m rscape_amd.profsum stats <rocprof dir> [--top K]
     python -m rscape_amd.profsum traffic <FETCH_SIZE dir> <WRITE_SIZE dir> --kernel NAME
         --calib NAME --calib-read BYTES --calib-write BYTES
+    python -m rscape_amd.profsum counters <pmc dir> --kernel NAME
+
+`counters` prints the mean per dispatch of every counter collected for the kernels whose name
+contains NAME.
 
 `traffic` averages FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the kernel, scales each by
 what the same counter reads for a calibration kernel of known bytes (known / measured), and
@@ -46,6 +50,20 @@ def counter_avg(d, counter, name):
     return sum(vals) / len(vals), len(vals)
 
 
+def counters(d, kernel):
+    acc = {}
+    for r in _rows(d, "counter_collection.csv"):
+        if kernel in r.get("Kernel_Name", ""):
+            acc.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    if not acc:
+        sys.exit("no counters for kernel %s in %s" % (kernel, d))
+    n = max(len(v) for v in acc.values())
+    lines = ["kernel %s  (%d dispatches)" % (kernel, n)]
+    for k in sorted(acc):
+        lines.append("  %-28s %.4g" % (k, sum(acc[k]) / len(acc[k])))
+    return "\n".join(lines)
+
+
 def traffic(fd, wd, kernel, calib, cread, cwrite):
     kf, nf = counter_avg(fd, "FETCH_SIZE", kernel)
     kw, nw = counter_avg(wd, "WRITE_SIZE", kernel)
@@ -72,9 +90,12 @@ def main():
     b = sub.add_parser("traffic"); b.add_argument("fetch_dir"); b.add_argument("write_dir")
     b.add_argument("--kernel", required=True); b.add_argument("--calib", required=True)
     b.add_argument("--calib-read", type=float, required=True); b.add_argument("--calib-write", type=float, required=True)
+    c = sub.add_parser("counters"); c.add_argument("dir"); c.add_argument("--kernel", required=True)
     args = ap.parse_args()
     if args.cmd == "stats":
         print(stats(args.dir, args.top))
+    elif args.cmd == "counters":
+        print(counters(args.dir, args.kernel))
     else:
         print(traffic(args.fetch_dir, args.write_dir, args.kernel, args.calib, args.calib_read, args.calib_write))
 
