@@ -195,6 +195,20 @@ def full_null(ctx, tree, L, pi, gapfrac, K=20, seed=5, covtype="GTp", graph=0, p
     return h, sf, fits[0]
 
 
+def full_nulls(ctx, tree, L, pi, gapfrac, seeds, K=20, covtype="GTp", pmass=5e-4):
+    """len(seeds) independent simulated nulls of one alignment in one batch (one group each, the
+    same jobs as full_null(seed=s) for every s): a list of (histogram, survfit, fit row)."""
+    model = NL.gtr_model(pi)
+    jobs = [NL.Job(tree, L, model, seed=s, stream=r, gapfrac=gapfrac, group=g)
+            for g, s in enumerate(seeds) for r in range(K)]
+    fits, hs = NL.batch(ctx, jobs, covtype, pmass=pmass, expo=True, hists=True)
+    out = []
+    for g in range(len(seeds)):
+        fit, sf = hs[g].fit(ctx, pmass, 1.0, True)
+        out.append((hs[g], sf, fits[g]))
+    return out
+
+
 def msa_null_params(ax):
     """Composition and gap fraction of an alignment (the null simulator's inputs)."""
     ax = np.asarray(ax)

@@ -124,8 +124,7 @@ def eval_one(ctx, sur, rng, Nr, Lr, K=None, seed=0, R=8):
     iu = np.triu_indices(L, 1)
     lg = lambda e: np.log10(np.maximum(e, 1e-300))
     lgs, caps, fits = [], [], []
-    for r in range(R):
-        h, sf, fit = S.full_null(ctx, tree, L, pi, gap, K=K, seed=1000 * seed + 17 + r)
+    for h, sf, fit in S.full_nulls(ctx, tree, L, pi, gap, [1000 * seed + 17 + r for r in range(R)], K=K):
         lgs.append(lg(msa.evalues(h, sf, Nc)[iu])); caps.append(full_cap(h)); fits.append(fit)
     lgs = np.array(lgs)
     ref = lgs.mean(0)

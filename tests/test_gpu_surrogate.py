@@ -179,3 +179,18 @@ def test_surrogate_beats_constant_tail(ctx):
     e, ce = np.median(np.concatenate(errs)), np.median(np.concatenate(cerrs))
     print("median log10 tail error: network %.3f, constant tail %.3f" % (e, ce))
     assert e < 0.5 * ce
+
+
+def test_full_nulls_batch_equals_single_nulls(ctx):
+    """Several independent nulls of one alignment in one batch (one group each) = each null on
+    its own (same seeds and streams): same histograms and fits."""
+    from rscape_amd import null as NL
+    from rscape_amd import surrogate as S
+    tree = NL.yule_tree(60, 0.1, seed=3)
+    pi, gap = (0.3, 0.2, 0.2, 0.3), 0.05
+    many = S.full_nulls(ctx, tree, 90, pi, gap, [11, 12, 13], K=4)
+    for s, (h, sf, fit) in zip([11, 12, 13], many):
+        h1, sf1, fit1 = S.full_null(ctx, tree, 90, pi, gap, K=4, seed=s)
+        assert np.array_equal(h.h.counts(), h1.h.counts())
+        assert np.array_equal(np.nan_to_num(fit), np.nan_to_num(fit1))
+        assert np.array_equal(np.nan_to_num(sf), np.nan_to_num(sf1))
