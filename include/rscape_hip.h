@@ -334,6 +334,26 @@ int rsg_pair_model(const double *Q16, const double *M16, double *model16);
  * of c (0-based) or -1 (the reference's 1-based ct with 0 = unpaired, shifted). */
 int rsg_null_simulate_pairs(rsg_ctx *ctx, const rsg_null_job *job, const int *ct, const double *model16,
                             uint8_t *leaves, char *errbuf);
+/* Rates of the e1 indel model (E1_RATE fields read by e1_model_transitions_AF, src/e1_model.c:784-868):
+ * for R-scape-sim's default AIF model (src/e1_rate.c:431-519) rM = rD = 0 and muAD = muAI = muAM, from
+ * the parameter file's rI, muAM, ldI, muI (data/training/Pfam.seed.S1000.trainGD.AIF.param). */
+typedef struct { double rI, rM, rD, muAM, muAD, muAI, ldI, muI; } rsg_e1_rates;
+/* e1_model_Create's transitions for a branch of length time in e2_GLOBAL mode, renormalised with
+ * no end state (e1_model_RenormStateE): t16 in the E1_MODEL order, rows B, S, D, I of {S, D, I, E}.
+ * L (the parent's length) matters only when the insertion term beta reaches 1. */
+int rsg_e1_transitions(const rsg_e1_rates *rates, double time, int64_t L, double *t16);
+/* cov_GenerateAlignment (src/cov_simulate.c:60-86): the job's tree evolves root (L = job->alen
+ * codes 0..3; NULL: drawn from the model's pi) without gaps -- with ct and model16 (rsg_pair_model)
+ * base-paired columns as 16-state characters, as rsg_null_simulate_pairs -- then, unless rates is
+ * NULL (noindels), the e1 indel model of cov_GenerateAlignmentAddGaps (:161-190, 326-361, 495-563,
+ * 634-722, 803-859): branch by branch in tree order an HMM walk over the current alignment deletes
+ * cells of the child and inserts columns (gap in every other row, the child's residue drawn from
+ * insf[4], NULL: the model's pi).  *ret_leaves receives N x *ret_alen codes and *ret_ct (may be
+ * NULL) the partner of every column or -1; both are freed with rsg_free.  Draws are Philox keyed as
+ * the oracle's (the reference's Easel generator is not restated: parity is in distribution). */
+int rsg_cov_GenerateAlignment(rsg_ctx *ctx, const rsg_null_job *job, const uint8_t *root, const int *ct,
+                              const double *model16, const rsg_e1_rates *rates, const double *insf,
+                              uint8_t **ret_leaves, int64_t *ret_alen, int **ret_ct, char *errbuf);
 /* Simulate and score a batch of alignments of any sizes (GT/MI/... + correction, unit
  * weights, CSELECT) entirely on the device, then build one null per group of jobs: jobs that
  * share a group id (ids contiguous, ascending from 0) are the replicates of one null, as

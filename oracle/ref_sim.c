@@ -82,7 +82,7 @@ static void row_cdf(const double *model, double t, int a, uint32_t *out)
  * symbol becomes a gap when the tag-1 draw is below gapthr.  leaves: N x L output. */
 int ref_evolve_gtr(int64_t nleaf, int64_t L, const int *left, const int *right, const double *ld,
                    const double *rd, const double *model, uint64_t seed, uint32_t stream, uint32_t gapthr,
-                   uint8_t *leaves)
+                   const uint8_t *root, uint8_t *leaves)
 {
   int64_t nrow = 2 * nleaf - 1;
   uint8_t *S = malloc((size_t)(nrow * L));
@@ -99,6 +99,7 @@ int ref_evolve_gtr(int64_t nleaf, int64_t L, const int *left, const int *right, 
     for (int b = 0; b < 3; b++) if (u < pic[b]) { a = b; break; }
     S[pos] = (uint8_t)a;
   }
+  if (root) memcpy(S, root, (size_t)L);            /* a given root row (cov_add_root, :204-238) */
   for (int64_t v = 0; v < nleaf - 1; v++)
     for (int side = 0; side < 2; side++) {
       int d = side ? right[v] : left[v];
@@ -141,7 +142,7 @@ static void row_cdf16(const double *m, double t, int a, uint32_t *out)
 
 int ref_evolve_pairs(int64_t nleaf, int64_t L, const int *left, const int *right, const double *ld,
                      const double *rd, const double *model, const double *m16, const int *ct, uint64_t seed,
-                     uint32_t stream, uint32_t gapthr, uint8_t *leaves)
+                     uint32_t stream, uint32_t gapthr, const uint8_t *root, uint8_t *leaves)
 {
   int64_t nrow = 2 * nleaf - 1;
   uint8_t *S = malloc((size_t)(nrow * L));
@@ -169,6 +170,7 @@ int ref_evolve_pairs(int64_t nleaf, int64_t L, const int *left, const int *right
     int st = icdf(rng_tag(seed, stream, (uint32_t)nrow, (uint32_t)pos, 2u), pth, 16);
     S[pos] = (uint8_t)(st >> 2); S[ct[pos]] = (uint8_t)(st & 3);
   }
+  if (root) memcpy(S, root, (size_t)L);
   for (int64_t v = 0; v < nleaf - 1; v++)
     for (int side = 0; side < 2; side++) {
       int d = side ? right[v] : left[v];
@@ -381,5 +383,124 @@ int ref_tree_substitutions(const uint8_t *ax, int64_t N, int64_t L, const int *l
       if (njoin) njoin[idx] = nj;
     }
   free(all);
+  return REF_OK;
+}
+
+/* ---- indels: cov_GenerateAlignmentAddGaps (cov_simulate.c:161-190) ----
+ * e1_model_transitions_AF (e1_model.c:784-868) in e2_GLOBAL mode, then e1_model_RenormStateE
+ * (:214-270): gamma_X = 1 - exp(-muA_X t) (gammaMDI_func, :1068-1080); beta from
+ * e1_model_LI_BetaFunc (:634-690).  rates = {rI, rM, rD, muAM, muAD, muAI, ldI, muI} (AIF:
+ * muAD = muAI = muAM, rM = rD = 0, e1_rate.c:431-519).  t16 in the E1_MODEL order: rows B, S, D, I
+ * of {S, D, I, E}.  L: the parent's length, used only when beta = 1. */
+int ref_e1_transitions(const double *rates, double time, int64_t L, double *t)
+{
+  const double rI = rates[0], rM = rates[1], rD = rates[2], muAM = rates[3], muAD = rates[4], muAI = rates[5];
+  const double ldI = rates[6], muI = rates[7];
+  const double gM = 1.0 - exp(-muAM * time), gD = 1.0 - exp(-muAD * time), gI = 1.0 - exp(-muAI * time);
+  double beta = 0.0;
+  if (time != 0.0 && ldI != 0.0) {
+    const double a = ldI - muI, alpha = fabs(a);
+    double num, den;
+    if (fabs(muI - ldI) < 1e-20) { num = muI * time; den = 1.0 + num; }
+    else {
+      const double expoI = -time * alpha;
+      if (a < 0.) {
+        const double A = (ldI > 0.) ? exp(expoI + log(ldI)) : 0.0;
+        num = ldI - A; den = muI - A;
+      } else {
+        const double Al = (ldI > 0.) ? exp(expoI + log(ldI)) : 0.0, Am = (muI > 0.) ? exp(expoI + log(muI)) : 0.0;
+        num = ldI - Al; den = ldI - Am;
+      }
+    }
+    beta = (fabs(den) > 0.) ? num / den : 1e-8;
+    if (isnan(beta)) return REF_FAIL;
+    if (beta > 1.) { if (beta < 1.0 + 1e-3) beta = 1.0; else return REF_FAIL; }
+    if (beta < 0.) { if (beta > -1e-4) beta = 0.0; else return REF_FAIL; }
+  }
+  const double move = 1.0 - (double)L / ((double)L + 1.0);
+  t[0] = (1.0 - beta) * (1.0 - gM);                     /* BS */
+  t[4] = (1.0 - rM) * (1.0 - beta) * (1.0 - gM) + rM;   /* SS */
+  t[8] = (1.0 - rD) * (1.0 - beta) * (1.0 - gD);        /* DS */
+  t[12] = (1.0 - rI) * (1.0 - beta) * (1.0 - gI);       /* IS */
+  t[1] = (1.0 - beta) * gM;                             /* BD */
+  t[5] = (1.0 - rM) * (1.0 - beta) * gM;                /* SD */
+  t[9] = (1.0 - rD) * (1.0 - beta) * gD + rD;           /* DD */
+  t[13] = (1.0 - rI) * (1.0 - beta) * gI;               /* ID */
+  t[2] = (beta == 1.0) ? 1.0 - move : beta;             /* BI */
+  t[6] = ((1.0 - rM) * beta == 1.0) ? 1.0 - move : (1.0 - rM) * beta;
+  t[10] = ((1.0 - rD) * beta == 1.0) ? 1.0 - move : (1.0 - rD) * beta;
+  t[14] = ((1.0 - rI) * beta == 1.0) ? 1.0 - move : (1.0 - rI) * beta + rI;
+  for (int r = 0; r < 4; r++) {                          /* RenormStateE: X -> E is 0 */
+    double sum = t[4 * r] + t[4 * r + 1] + t[4 * r + 2];
+    sum = (sum > 0.) ? 1.0 / sum : 1.0;
+    for (int k = 0; k < 3; k++) t[4 * r + k] *= sum;
+    t[4 * r + 3] = 0.0;
+  }
+  return REF_OK;
+}
+
+static uint32_t th32(double x) { return (x >= 1.0) ? 0xFFFFFFFFu : (uint32_t)(x * 4294967296.0); }
+
+/* The indel pass on the leaves (cov_evolve_indels_tree, :326-361; the branch step, :495-563;
+ * cov_emit_indels, :634-722; cov_insert, :803-859): branches in tree order (v = 0..N-2, left then
+ * right child), each walking the e1 HMM over the current alignment (all its columns, those earlier
+ * branches inserted included): from B, every step draws one uniform (key: child row, step, tag 3)
+ * and moves to S, D or I by the current state's row of the branch's transitions (esl_rnd_FChoose
+ * over the three); S and D consume a column (D gaps the child's cell), I inserts a column right
+ * after the current one (gap in every other row; the child's residue drawn from insf with key
+ * (child row, insertion ordinal, tag 4)); the walk ends when a step consumes column L + 1.  Only
+ * leaf rows are kept: internal rows are never output, and their branches' insertions add gap
+ * columns to every leaf.  ct follows the columns (inserted columns unpaired).  leaves_in N x L0,
+ * leaves_out N x cap; returns REF_FAIL when the alignment would outgrow cap. */
+int ref_add_indels(int64_t N, int64_t L0, const int *left, const int *right, const double *ld, const double *rd,
+                   const double *rates, const double *insf, uint64_t seed, uint32_t stream, const uint8_t *leaves_in,
+                   const int *ct_in, int64_t cap, uint8_t *leaves_out, int *ct_out, int64_t *alen)
+{
+  if (L0 > cap) return REF_FAIL;
+  int64_t len = L0;
+  for (int64_t l = 0; l < N; l++) memcpy(leaves_out + l * cap, leaves_in + l * L0, (size_t)L0);
+  if (ct_out) for (int64_t c = 0; c < L0; c++) ct_out[c] = ct_in ? ct_in[c] : -1;
+  uint32_t ith[3];
+  double cum = 0., tot = insf[0] + insf[1] + insf[2] + insf[3];
+  for (int b = 0; b < 3; b++) { cum += insf[b]; ith[b] = th32(cum / tot); }
+  for (int64_t v = 0; v < N - 1; v++)
+    for (int side = 0; side < 2; side++) {
+      const int d = side ? right[v] : left[v];
+      const int64_t row = (d > 0) ? d : (N - 1 - d), leaf = (d > 0) ? -1 : -d;
+      double t16[16];
+      if (ref_e1_transitions(rates, side ? rd[v] : ld[v], len, t16) != REF_OK) return REF_FAIL;
+      uint32_t th[4][2];
+      for (int r = 0; r < 4; r++) {
+        const double s = t16[4 * r] + t16[4 * r + 1] + t16[4 * r + 2];
+        th[r][0] = th32(t16[4 * r] / s); th[r][1] = th32((t16[4 * r] + t16[4 * r + 1]) / s);
+      }
+      const int64_t L = len;
+      int64_t i = 0, pos = 0, step = 0, nins = 0;
+      int st = 0;                                          /* B */
+      for (;;) {
+        const uint32_t u = rng_tag(seed, stream, (uint32_t)row, (uint32_t)step++, 3u);
+        st = (u < th[st][0]) ? 1 : (u < th[st][1]) ? 2 : 3;  /* S, D, I */
+        if (st == 1 || st == 2) { i++; pos++; }
+        if (i == L + 1) break;
+        if (st == 2 && leaf >= 0) leaves_out[leaf * cap + pos - 1] = 4;
+        if (st == 3) {                                     /* new column at index pos (after 1-based pos) */
+          if (len + 1 > cap) return REF_FAIL;
+          for (int64_t l = 0; l < N; l++) {
+            uint8_t *r8 = leaves_out + l * cap;
+            memmove(r8 + pos + 1, r8 + pos, (size_t)(len - pos));
+            r8[pos] = 4;
+          }
+          if (leaf >= 0)
+            leaves_out[leaf * cap + pos] = (uint8_t)icdf(rng_tag(seed, stream, (uint32_t)row, (uint32_t)nins, 4u), ith, 4);
+          if (ct_out) {
+            memmove(ct_out + pos + 1, ct_out + pos, sizeof(int) * (size_t)(len - pos));
+            ct_out[pos] = -1;
+            for (int64_t c = 0; c <= len; c++) if (ct_out[c] >= pos) ct_out[c]++;
+          }
+          len++; nins++; pos++;
+        }
+      }
+    }
+  *alen = len;
   return REF_OK;
 }

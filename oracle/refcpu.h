@@ -112,10 +112,15 @@ void     ref_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t o
 uint32_t ref_rng_u32(uint64_t seed, uint32_t stream, uint32_t node, uint32_t pos);
 int ref_evolve_gtr(int64_t nleaf, int64_t L, const int *left, const int *right, const double *ld,
                    const double *rd, const double *model, uint64_t seed, uint32_t stream, uint32_t gapthr,
-                   uint8_t *leaves);
+                   const uint8_t *root, uint8_t *leaves);
 int ref_evolve_pairs(int64_t nleaf, int64_t L, const int *left, const int *right, const double *ld,
                      const double *rd, const double *model, const double *m16, const int *ct, uint64_t seed,
-                     uint32_t stream, uint32_t gapthr, uint8_t *leaves);
+                     uint32_t stream, uint32_t gapthr, const uint8_t *root, uint8_t *leaves);
+/* e1 HMM transitions (AF family, e2_GLOBAL, RenormStateE); rates {rI, rM, rD, muAM, muAD, muAI, ldI, muI} */
+int ref_e1_transitions(const double *rates, double time, int64_t L, double *t16);
+int ref_add_indels(int64_t N, int64_t L0, const int *left, const int *right, const double *ld, const double *rd,
+                   const double *rates, const double *insf, uint64_t seed, uint32_t stream, const uint8_t *leaves_in,
+                   const int *ct_in, int64_t cap, uint8_t *leaves_out, int *ct_out, int64_t *alen);
 int ref_fitch_null(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right,
                    uint64_t seed, uint8_t *out);
 int ref_tree_substitutions(const uint8_t *ax, int64_t N, int64_t L, const int *left, const int *right, uint64_t seed,

@@ -575,7 +575,7 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
                        d_leaves, f_nsub, f_cols};
       if (rsg_fitch_launch(c, *fp, D, nj) != RSG_OK) return RSG_FAIL;
     } else if (rsg_sim_launch(c, d_sim, nj, nodes, b_sim, lv, d_ent, d_left, d_right, d_ld, d_rd, d_models, d_cdf, d_seqs,
-                              d_leaves) != RSG_OK) return RSG_FAIL;
+                              d_leaves, nullptr, 0) != RSG_OK) return RSG_FAIL;
     hipLaunchKernelGGL(k_pack_b, dim3((unsigned)b_pack), dim3(256), 0, c->stream, d_desc, nj, d_leaves, d_xt);
     if (feats && rsg_feat_launch(c, d_fitems, ng, fblocks, d_xt, d_fcnt) != RSG_OK) return RSG_FAIL;
     if (rsg_gram_batch_launch(c, d_btiles, (int)btiles.size(), d_tiles, (int)tiles.size(), d_desc, d_xt, d_cnt, d_pmp,

@@ -229,11 +229,15 @@ int rsg_sim_levels(const SimItem *items, int nitems, const int *left, const int 
 int rsg_sim_launch(rsg_ctx *c, const SimItem *d_items, int nitems, int64_t nodes, int64_t blocks,
                    const SimLevels &lv, const int4 *d_ent, const int *d_left, const int *d_right,
                    const double *d_ld, const double *d_rd, const double *d_models, uint32_t *d_cdf,
-                   uint8_t *d_seqs, uint8_t *d_leaves);
+                   uint8_t *d_seqs, uint8_t *d_leaves, const uint8_t *root0, int64_t root_len);
 int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t *L,
                         const int *left, const int *right, const double *ld, const double *rd,
                         const double *models, const uint64_t *seed, const uint32_t *stream, const double *gapfrac,
-                        uint8_t *d_leaves, const int64_t *leaf_off, char *errbuf);
+                        uint8_t *d_leaves, const int64_t *leaf_off, char *errbuf, const uint8_t *root0 = nullptr);
+// one alignment's leaves to the host, the root drawn from pi (root == nullptr) or given (L codes)
+int rsg_sim_one(rsg_ctx *c, const rsg_null_job *job, const uint8_t *root, uint8_t *leaves, char *errbuf);
+int rsg_sim_pairs(rsg_ctx *c, const rsg_null_job *job, const int *ct, const double *model16, const uint8_t *root,
+                  uint8_t *leaves, char *errbuf);
 
 // R-scape's null of one alignment for R replicates at once (rsg_fitch.hip): the tree's levels
 // (internal nodes grouped by depth, nodes[lev0[k] .. lev0[k+1]) at depth k), parents and the

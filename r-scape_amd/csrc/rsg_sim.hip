@@ -343,13 +343,15 @@ int rsg_sim_levels(const SimItem *items, int nitems, const int *left, const int 
 int rsg_sim_launch(rsg_ctx *c, const SimItem *d_items, int nitems, int64_t nodes, int64_t blocks,
                    const SimLevels &lv, const int4 *d_ent, const int *d_left, const int *d_right,
                    const double *d_ld, const double *d_rd, const double *d_models, uint32_t *d_cdf,
-                   uint8_t *d_seqs, uint8_t *d_leaves)
+                   uint8_t *d_seqs, uint8_t *d_leaves, const uint8_t *root0, int64_t root_len)
 {
   if (nodes > 0)
     hipLaunchKernelGGL(k_branch_tables, dim3((unsigned)((nodes * 8 + 255) / 256)), dim3(256), 0, c->stream,
                        d_items, nitems, d_ld, d_rd, d_models, nodes, d_cdf);
   if (blocks > 0)
     hipLaunchKernelGGL(k_root, dim3((unsigned)blocks), dim3(64), 0, c->stream, d_items, nitems, d_models, d_seqs);
+  // a given root row (cov_add_root, src/cov_simulate.c:204-238) replaces item 0's drawn one
+  if (root0 && hipMemcpyAsync(d_seqs, root0, (size_t)root_len, hipMemcpyHostToDevice, c->stream) != hipSuccess) return RSG_FAIL;
   for (size_t l = 0; l < lv.lev_blocks.size(); l++) {
     const int64_t e0 = lv.lev_ent0[l], ne = lv.lev_ent0[l + 1] - e0;
     if (lv.lev_blocks[l] > 0)
@@ -380,7 +382,7 @@ struct SimBuffers {
 int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t *L,
                         const int *left, const int *right, const double *ld, const double *rd,
                         const double *models, const uint64_t *seed, const uint32_t *stream, const double *gapfrac,
-                        uint8_t *d_leaves, const int64_t *leaf_off, char *errbuf)
+                        uint8_t *d_leaves, const int64_t *leaf_off, char *errbuf, const uint8_t *root0)
 {
   std::vector<SimItem> items((size_t)nitems);
   int64_t nodes = 0, seq = 0, blocks = 0;
@@ -410,7 +412,7 @@ int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t 
   HIPCHK(hipMemcpyAsync(d_ent, lv.ent.data(), sizeof(int4) * lv.ent.size(), hipMemcpyHostToDevice, c->stream));
   int t; rsg_tstart(c, "simulate", &t);
   const int lr = rsg_sim_launch(c, d_items, nitems, nodes, blocks, lv, d_ent, d_left, d_right, d_ld, d_rd, d_models,
-                                d_cdf, d_seqs, d_leaves);
+                                d_cdf, d_seqs, d_leaves, root0, root0 ? L[0] : 0);
   rsg_tstop(c, t);
   if (lr != RSG_OK) return rsg_fail(errbuf, "simulation launch failed");
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -420,13 +422,18 @@ int rsg_simulate_device(rsg_ctx *c, int nitems, const int64_t *N, const int64_t 
 // Simulate one alignment and return its leaves (N x L) to the host.
 extern "C" int rsg_null_simulate(rsg_ctx *c, const rsg_null_job *job, uint8_t *leaves, char *errbuf)
 {
+  return rsg_sim_one(c, job, nullptr, leaves, errbuf);
+}
+
+int rsg_sim_one(rsg_ctx *c, const rsg_null_job *job, const uint8_t *root, uint8_t *leaves, char *errbuf)
+{
   HIPCHK(hipSetDevice(c->device));
   if (job->nleaf < 2 || job->alen < 1) return rsg_fail(errbuf, "simulation needs >= 2 leaves and >= 1 column");
   uint8_t *d_leaves = nullptr;
   HIPCHK(hipMalloc(&d_leaves, (size_t)(job->nleaf * job->alen)));
   int64_t off = 0;
   int rc = rsg_simulate_device(c, 1, &job->nleaf, &job->alen, job->left, job->right, job->ld, job->rd,
-                               job->model, &job->seed, &job->stream, &job->gapfrac, d_leaves, &off, errbuf);
+                               job->model, &job->seed, &job->stream, &job->gapfrac, d_leaves, &off, errbuf, root);
   if (rc == RSG_OK) {
     if (hipMemcpy(leaves, d_leaves, (size_t)(job->nleaf * job->alen), hipMemcpyDeviceToHost) != hipSuccess)
       rc = rsg_fail(errbuf, "copy of simulated leaves failed");
@@ -595,6 +602,12 @@ k_pair_level(const int *__restrict__ nodes, const int2 *__restrict__ pairs, int 
 extern "C" int rsg_null_simulate_pairs(rsg_ctx *c, const rsg_null_job *job, const int *ct, const double *model16,
                                        uint8_t *leaves, char *errbuf)
 {
+  return rsg_sim_pairs(c, job, ct, model16, nullptr, leaves, errbuf);
+}
+
+int rsg_sim_pairs(rsg_ctx *c, const rsg_null_job *job, const int *ct, const double *model16, const uint8_t *root,
+                  uint8_t *leaves, char *errbuf)
+{
   if (!c || !job || !ct || !model16 || !leaves) return rsg_fail(errbuf, "rsg_null_simulate_pairs: bad arguments");
   HIPCHK(hipSetDevice(c->device));
   const int64_t N = job->nleaf, L = job->alen;
@@ -642,6 +655,7 @@ extern "C" int rsg_null_simulate_pairs(rsg_ctx *c, const rsg_null_job *job, cons
   hipLaunchKernelGGL(k_root, dim3((unsigned)blocks), dim3(64), 0, c->stream, d_items, 1, d_models, d_seqs);
   if (np) hipLaunchKernelGGL(k_pair_root, dim3((unsigned)((np + 63) / 64)), dim3(64), 0, c->stream, d_m16, d_pairs, (int)np, N,
                              it.seed, it.stream, d_seqs);
+  if (root) HIPCHK(hipMemcpyAsync(d_seqs, root, (size_t)L, hipMemcpyHostToDevice, c->stream));   // cov_add_root
   for (size_t l = 0; l < lv.lev_blocks.size(); l++) {
     const int64_t e0 = lv.lev_ent0[l], ne = lv.lev_ent0[l + 1] - e0;
     if (lv.lev_blocks[l] > 0)

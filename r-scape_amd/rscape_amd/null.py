@@ -51,6 +51,21 @@ def job_array(jobs):
     return rec
 
 
+class E1Rates(C.Structure):
+    """rsg_e1_rates: the E1_RATE fields of the e1 indel model's transitions."""
+    _fields_ = [(k, C.c_double) for k in ("rI", "rM", "rD", "muAM", "muAD", "muAI", "ldI", "muI")]
+
+
+def aif_rates(rI, muAM, ldI, muI):
+    """The AIF model's rates (src/e1_rate.c:431-519): rM = rD = 0, muAD = muAI = muAM."""
+    return E1Rates(rI, 0.0, 0.0, muAM, muAM, muAM, ldI, muI)
+
+
+# R-scape-sim's default evomodel parameters (data/training/Pfam.seed.S1000.trainGD.AIF.param:
+# rI 0.846315, ldI 0.0101, muI 0.306055, muAM 0.0102)
+AIF_PFAM = (0.846315, 0.0102, 0.0101, 0.306055)
+
+
 def _declare():
     L = lib()
     if getattr(L, "_null_declared", False):
@@ -70,6 +85,11 @@ def _declare():
     L.rsg_null_batch.argtypes = [vp, vp, i32, i32, d, d, d, i32, vp, vp, vp, i32, cp]
     L.rsg_null_batch_fitch.restype = i32
     L.rsg_null_batch_fitch.argtypes = [vp, vp, i64, i64, vp, vp, vp, i32, i32, d, d, d, i32, vp, vp, i32, cp]
+    L.rsg_e1_transitions.restype = i32
+    L.rsg_e1_transitions.argtypes = [C.POINTER(E1Rates), d, i64, vp]
+    L.rsg_cov_GenerateAlignment.restype = i32
+    L.rsg_cov_GenerateAlignment.argtypes = [vp, C.POINTER(NullJob), vp, vp, vp, C.POINTER(E1Rates), vp,
+                                            C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(i64), C.POINTER(C.POINTER(C.c_int)), cp]
     L.rsg_null_features.restype = i32
     L.rsg_null_features.argtypes = [vp, vp, vp, vp, vp, vp, vp, cp]
     L.rsg_cov_EvaluesTail.restype = i32
@@ -161,6 +181,36 @@ def simulate_pairs(ctx, job, ct, model16):
     err = _err()
     _check(L.rsg_null_simulate_pairs(ctx.h, C.byref(s), _p(ctv), _p(m16), _p(out), err), err, "rsg_null_simulate_pairs")
     return out
+
+
+def e1_transitions(rates, time, L=100):
+    """The 16 transitions (rows B, S, D, I of S, D, I, E) of a branch of length time."""
+    t = np.zeros(16)
+    if _declare().rsg_e1_transitions(C.byref(rates), float(time), int(L), _p(t)) != 0:
+        raise RuntimeError("e1 transitions failed")
+    return t
+
+
+def generate_alignment(ctx, job, root=None, ct=None, model16=None, rates=None, insf=None):
+    """cov_GenerateAlignment on the device: (leaves N x alen, ct alen).  rates None: no indels;
+    ct and model16 given: base pairs evolve under the 16-state model."""
+    L = _declare()
+    s = job.struct()
+    rt = None if root is None else np.ascontiguousarray(root, dtype=np.uint8)
+    ctv = None if ct is None else np.ascontiguousarray(ct, dtype=np.int32)
+    m16 = None if model16 is None else np.ascontiguousarray(model16, dtype=np.float64)
+    f = None if insf is None else np.ascontiguousarray(insf, dtype=np.float64)
+    out = C.POINTER(C.c_uint8)()
+    cto = C.POINTER(C.c_int)()
+    alen = C.c_int64(0)
+    err = _err()
+    _check(L.rsg_cov_GenerateAlignment(ctx.h, C.byref(s), _p(rt), _p(ctv), _p(m16), C.byref(rates) if rates is not None else None,
+                                       _p(f), C.byref(out), C.byref(alen), C.byref(cto), err), err, "rsg_cov_GenerateAlignment")
+    n, a = job.tree["n"], alen.value
+    leaves = np.ctypeslib.as_array(out, shape=(n, a)).copy()
+    ctn = np.ctypeslib.as_array(cto, shape=(a,)).copy()
+    lib().rsg_free(C.cast(out, C.c_void_p)); lib().rsg_free(C.cast(cto, C.c_void_p))
+    return leaves, ctn
 
 
 def fitch_null(ctx, ax, tree, seed=1):

@@ -200,31 +200,71 @@ def evalues(COV, L, Nc, h, survfit):
     return Ev.reshape(L, L)
 
 
-def evolve_gtr(tree, L, model, seed, stream, gapthr=0):
+def evolve_gtr(tree, L, model, seed, stream, gapthr=0, root=None):
     n = tree["n"]
     out = np.zeros((n, L), dtype=np.uint8)
     f = lib().ref_evolve_gtr
     f.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                  C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p]
+                  C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
     left = np.ascontiguousarray(tree["left"], dtype=np.int32); right = np.ascontiguousarray(tree["right"], dtype=np.int32)
     ld = np.ascontiguousarray(tree["ld"], dtype=np.float64); rd = np.ascontiguousarray(tree["rd"], dtype=np.float64)
-    f(n, L, P(left), P(right), P(ld), P(rd), P(np.ascontiguousarray(model, dtype=np.float64)), seed, stream, gapthr, P(out))
+    rt = None if root is None else np.ascontiguousarray(root, dtype=np.uint8)
+    f(n, L, P(left), P(right), P(ld), P(rd), P(np.ascontiguousarray(model, dtype=np.float64)), seed, stream, gapthr,
+      P(rt) if rt is not None else None, P(out))
     return out
 
 
-def evolve_pairs(tree, L, model, m16, ct, seed, stream, gapthr=0):
+def evolve_pairs(tree, L, model, m16, ct, seed, stream, gapthr=0, root=None):
     """Structured null: paired columns (ct[c] = partner or -1) evolve under the 16-state model."""
     n = tree["n"]
     out = np.zeros((n, L), dtype=np.uint8)
     f = lib().ref_evolve_pairs
     f.argtypes = [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                  C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p]
+                  C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
+    rt = None if root is None else np.ascontiguousarray(root, dtype=np.uint8)
     left = np.ascontiguousarray(tree["left"], dtype=np.int32); right = np.ascontiguousarray(tree["right"], dtype=np.int32)
     ld = np.ascontiguousarray(tree["ld"], dtype=np.float64); rd = np.ascontiguousarray(tree["rd"], dtype=np.float64)
     ctv = np.ascontiguousarray(ct, dtype=np.int32)
     f(n, L, P(left), P(right), P(ld), P(rd), P(np.ascontiguousarray(model, dtype=np.float64)),
-      P(np.ascontiguousarray(m16, dtype=np.float64)), P(ctv), seed, stream, gapthr, P(out))
+      P(np.ascontiguousarray(m16, dtype=np.float64)), P(ctv), seed, stream, gapthr, P(rt) if rt is not None else None,
+      P(out))
     return out
+
+
+def e1_transitions(rates, time, L=100):
+    """The 16 e1 HMM transitions (rows B, S, D, I of S, D, I, E) of one branch."""
+    f = lib().ref_e1_transitions
+    f.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.c_void_p]
+    t = np.zeros(16)
+    r = np.ascontiguousarray(rates, dtype=np.float64)
+    if f(P(r), float(time), int(L), P(t)) != 0:
+        raise RuntimeError("e1 transitions failed")
+    return t
+
+
+def add_indels(tree, leaves, rates, insf, seed, stream, ct=None, cap=None):
+    """The indel pass over ungapped leaves (N x L0): (leaves N x alen, ct alen)."""
+    n, L0 = leaves.shape
+    cap = cap or 4 * L0 + 256
+    f = lib().ref_add_indels
+    f.argtypes = [C.c_int64, C.c_int64] + [C.c_void_p] * 6 + [C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int64,
+                                                             C.c_void_p, C.c_void_p, C.c_void_p]
+    left = np.ascontiguousarray(tree["left"], dtype=np.int32); right = np.ascontiguousarray(tree["right"], dtype=np.int32)
+    ld = np.ascontiguousarray(tree["ld"], dtype=np.float64); rd = np.ascontiguousarray(tree["rd"], dtype=np.float64)
+    lv = np.ascontiguousarray(leaves, dtype=np.uint8)
+    ctv = None if ct is None else np.ascontiguousarray(ct, dtype=np.int32)
+    while True:
+        out = np.zeros((n, cap), np.uint8)
+        cto = np.zeros(cap, np.int32)
+        alen = C.c_int64(0)
+        rc = f(n, L0, P(left), P(right), P(ld), P(rd), P(np.ascontiguousarray(rates, dtype=np.float64)),
+               P(np.ascontiguousarray(insf, dtype=np.float64)), seed, stream, P(lv), P(ctv) if ctv is not None else None,
+               cap, P(out), P(cto), C.byref(alen))
+        if rc == 0:
+            return out[:, :alen.value].copy(), cto[:alen.value].copy()
+        if cap > 64 * L0 + 4096:
+            raise RuntimeError("indel pass failed")
+        cap *= 2
 
 
 def mlp_forward_backward(W, b, x, dy=None):

@@ -345,3 +345,64 @@ def test_simulated_null_scores_ks(ctx, structured):
 
     assert ks_2samp(got, textbook(41)).pvalue > 1e-3
     assert ks_2samp(got, textbook(42, scale=1.25)).pvalue < 1e-3
+
+
+def _rates8(r):
+    rI, muAM, ldI, muI = r
+    return [rI, 0.0, 0.0, muAM, muAM, muAM, ldI, muI]
+
+
+@pytest.mark.parametrize("N,L,rates,bl,use_root,npairs,gap", [
+    (2, 7, NL.AIF_PFAM, 0.1, False, 0, 0.0),
+    (5, 40, (0.5, 0.5, 0.3, 0.6), 0.5, True, 0, 0.0),
+    (40, 120, NL.AIF_PFAM, 0.15, True, 0, 0.0),
+    (40, 120, NL.AIF_PFAM, 0.15, False, 12, 0.05),
+    (200, 120, NL.AIF_PFAM, 0.05, True, 30, 0.0),
+    (300, 500, NL.AIF_PFAM, 0.05, False, 0, 0.0),
+])
+def test_generate_alignment_matches_oracle(ctx, N, L, rates, bl, use_root, npairs, gap):
+    """cov_GenerateAlignment with the e1 indel model on the device = the oracle's sequential
+    restatement of the reference loop (ungapped evolution from the given or drawn root, then the
+    branch-by-branch indel walks growing the alignment) on the same keys: same length, same
+    base-pair map, same gap pattern, residues identical up to the rare P(t)-table rounding the
+    ungapped simulators allow.  Every branch walks the whole alignment of its moment, inserted
+    columns included, so the length grows geometrically with the number of branches; the
+    branch lengths keep it to a few times the root's."""
+    from test_pair_model import pair_params
+    print("generate alignment N=%d L=%d rates=%s root=%s pairs=%d" % (N, L, rates, use_root, npairs), flush=True)
+    rng = np.random.default_rng(N + L)
+    m = NL.gtr_model(PI)
+    tree = NL.yule_tree(N, bl, seed=N + 1)
+    job = NL.Job(tree, L, m, seed=555 + N, stream=2, gapfrac=gap)
+    root = rng.integers(0, 4, size=L, dtype=np.uint8) if use_root else None
+    gapthr = 0 if gap <= 0 else int(gap * 2 ** 32)
+    if npairs:
+        Q, M = pair_params(N)
+        m16 = NL.pair_model(Q, M)
+        ct = _pairing(L, npairs, N)
+        got, gct = NL.generate_alignment(ctx, job, root=root, ct=ct, model16=m16, rates=NL.aif_rates(*rates))
+        ung = oracle.evolve_pairs(tree, L, m, m16, ct, 555 + N, 2, gapthr, root=root)
+    else:
+        ct = None
+        got, gct = NL.generate_alignment(ctx, job, root=root, rates=NL.aif_rates(*rates))
+        ung = oracle.evolve_gtr(tree, L, m, 555 + N, 2, gapthr, root=root)
+    ref, rct = oracle.add_indels(tree, ung, _rates8(rates), m[36:40], 555 + N, 2, ct=ct)
+    print("alignment length %d (root %d)" % (ref.shape[1], L), flush=True)
+    assert got.shape == ref.shape and np.array_equal(gct, rct)
+    assert np.array_equal(got == 4, ref == 4)
+    assert np.mean(got != ref) < 1e-5
+    # without indels: the ungapped leaves from the root
+    plain, pct = NL.generate_alignment(ctx, job, root=root, ct=ct, model16=m16 if npairs else None)
+    assert plain.shape == (N, L) and np.mean(plain != ung) < 1e-5
+    if use_root:
+        assert np.array_equal(pct, -np.ones(L, np.int32) if ct is None else ct)
+
+
+def test_generate_alignment_rejects_bad_input(ctx):
+    m = NL.gtr_model(PI)
+    tree = NL.yule_tree(6, 0.1, seed=1)
+    job = NL.Job(tree, 10, m, seed=1)
+    with pytest.raises(R.RscapeError, match="not canonical"):
+        NL.generate_alignment(ctx, job, root=np.full(10, 4, np.uint8))
+    with pytest.raises(R.RscapeError):
+        NL.generate_alignment(ctx, job, rates=NL.aif_rates(1.0, 0.1, 0.1, 0.3))
