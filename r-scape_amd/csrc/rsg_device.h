@@ -178,6 +178,104 @@ __device__ double pair_stat_f32(const StatArgs &A, const double *cd, int64_t i, 
   }
 }
 
+// ---- FP32 statistics of class C16 for GT, MI and MIg: the short form.
+// Every cell of the 4 x 4 table has q = pm_i[x] pm_j[y] > 0 (the marginals carry the 1e-10
+// pseudo-counts of every pair, or are uniform).  With c' = c + 1e-10, den = ns + 16e-10 and the
+// expected count B = den q, the cell's term of sum pp ln(pp / q) splits as
+//   pp ln(pp / q) = F / den + (pp - q),   F = c' ln(c' / B) - (c' - B) >= 0,
+// and the second parts sum to 1 - S_i S_j, S = sum_x pm[x] (FP64, ~1e-16).  B and e = c - B
+// are formed in FP64 (full-rate FMA units on this chip), so e carries no cancellation; the rest
+// is FP32.  With v = e / (c + B) = e / (2c - e) (C. Loader, "Fast and accurate computation of
+// binomial probabilities", 2000):
+//   |v| < 0.3:  F = v (e + c u S(u)),  u = v^2,  c u S(u) = 2 c sum_{k=1..6} v^(2k) / (2k + 1)
+//               (truncation < 2e-8 of F)
+//   otherwise:  F = c' ln(c' / B) - e   (c' / B with one Newton step, hardware logarithm:
+//               error below 6e-7 of F)
+// c' and c differ by 1e-10, which changes e and c + B by less than 1e-10 of any F that counts;
+// a zero count has v = -1 and takes the second form with c' = 1e-10.  The logarithm is
+// evaluated only in waves where some lane needs it (a wave-uniform branch; about 2% of the
+// cells of a wave at N = 5000, L = 1500).
+// c: the 16 counts as FP32 (exact for uint16 records, the record values for FP32 ones);
+// nsd: their FP64 sum; pi: pm_i; si: sum_x pm_i[x].
+template <int KIND, int TYPE>
+__device__ __forceinline__ double pair_stat_f32c16(const StatArgs &A, const float *c, double nsd,
+                                                   const double *pi, double si, int64_t i, int64_t j)
+{
+  const double den = nsd + 16e-10;
+  double pj[4];
+#pragma unroll
+  for (int y = 0; y < 4; y++) pj[y] = A.pm[j * 4 + y];
+  const double sj = (pj[0] + pj[1]) + (pj[2] + pj[3]);
+  float acc = 0.f;
+#pragma unroll
+  for (int x = 0; x < 4; x++) {
+    const double ax = den * pi[x];
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const float cf = c[x * 4 + y];
+      const double B = ax * pj[y];
+      const float e = (float)((double)cf - B);
+      const float v = e * __builtin_amdgcn_rcpf(fmaf(2.f, cf, -e));
+      const float u = v * v;
+      float s = 2.f / 13.f;
+      s = fmaf(s, u, 2.f / 11.f); s = fmaf(s, u, 2.f / 9.f); s = fmaf(s, u, 2.f / 7.f);
+      s = fmaf(s, u, 2.f / 5.f);  s = fmaf(s, u, 2.f / 3.f);
+      float F = v * fmaf(cf * u, s, e);
+      const bool far = fabsf(v) >= 0.3f;
+      if (__builtin_amdgcn_ballot_w64(far)) {
+        const float bh = (float)B, ch = fmaxf(cf, 1e-10f);
+        const float rb = __builtin_amdgcn_rcpf(bh);
+        float r = ch * rb;
+        r = fmaf(fmaf(-bh, r, ch), rb, r);
+        const float Fl = fmaf(ch * 0.693147180559945309f, __builtin_amdgcn_logf(r), -e);
+        F = far ? Fl : F;
+      }
+      acc += F;
+    }
+  }
+  const double s = (double)(acc * __builtin_amdgcn_rcpf((float)den)) + fma(-si, sj, 1.0);   // sum pp ln(pp / q)
+  if constexpr (TYPE == RSG_GT) return (nsd > 0.) ? 2. * nsd * s : 0.;
+  if constexpr (TYPE == RSG_MI) return s;
+  const double ng = A.nsx ? A.nsx[rsg_rec(i, j, A.L)] : nsd;        // MIg
+  return s - ((ng > 0.) ? (A.wsum - ng) / ng : 0.);
+}
+
+template <typename T, int KIND, int TYPE, int CLS>
+constexpr bool stat_short_form()
+{
+  return sizeof(T) == sizeof(float) && CLS == RSG_C16 && (KIND == CNT_U16 || KIND == CNT_F32) &&
+         (TYPE == RSG_GT || TYPE == RSG_MI || TYPE == RSG_MIg);
+}
+
+// A record as 16 FP32 counts and their FP64 sum (integer sum for uint16 records).
+template <int KIND> __device__ __forceinline__ double decode_rec_f32(const RawRec<KIND> &r, float *c)
+{
+  if constexpr (KIND == CNT_U16) {
+    unsigned ns = 0;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const unsigned w[4] = {r.q[h].x, r.q[h].y, r.q[h].z, r.q[h].w};
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        c[8 * h + 2 * q] = (float)(w[q] & 0xFFFFu); c[8 * h + 2 * q + 1] = (float)(w[q] >> 16);
+        ns += (w[q] & 0xFFFFu) + (w[q] >> 16);
+      }
+    }
+    return (double)ns;
+  } else {
+    double ns = 0.;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint4 v = r.q[k];
+      c[4 * k] = __uint_as_float(v.x); c[4 * k + 1] = __uint_as_float(v.y);
+      c[4 * k + 2] = __uint_as_float(v.z); c[4 * k + 3] = __uint_as_float(v.w);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) ns += (double)c[k];
+    return ns;
+  }
+}
+
 // One pair. c: counts, pmi/pmj: marginals.  Returns the statistic (see the file header).
 // TYPE / CLS >= 0 fix the statistic and class at compile time (-1: read them from A).
 // tabc: ln(c + 1e-10) for counts c = 0..N (A.lgc, or a copy of it in LDS) when TAB; A.lgc holds
@@ -524,7 +622,51 @@ __device__ __forceinline__ void stat_pairs(const StatArgs &A, int bi, int bj, co
     const int ii = tid & 31;
     const int64_t i = (int64_t)bi * 32 + ii;
     double v[4];
-    if constexpr (KIND == CNT_F32) {
+    if constexpr (stat_short_form<T, KIND, TYPE, CLS>()) {
+      // the i-side marginals and their sum once per thread
+      double pi[4] = {0., 0., 0., 0.}, si = 0.;
+      if (i < A.L) {
+#pragma unroll
+        for (int x = 0; x < 4; x++) pi[x] = A.pm[i * 4 + x];
+        si = (pi[0] + pi[1]) + (pi[2] + pi[3]);
+      }
+      auto score = [&](const RawRec<KIND> &r, int64_t j) -> double {
+        float cf[16];
+        const double nsd = decode_rec_f32<KIND>(r, cf);
+        return pair_stat_f32c16<KIND, TYPE>(A, cf, nsd, pi, si, i, j);
+      };
+      if constexpr (KIND == CNT_F32) {
+        RawRec<KIND> cur, nxt;
+        {
+          const int64_t j = (int64_t)bj * 32 + (tid >> 5);
+          if (i < j && j < A.L) load_raw<KIND>(A.cnt, rsg_rec(i, j, A.L), cur);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
+          if (k < 3) {
+            const int64_t j2 = j + 8;
+            if (i < j2 && j2 < A.L) load_raw<KIND>(A.cnt, rsg_rec(i, j2, A.L), nxt);
+          }
+          v[k] = -INFINITY;
+          if (i < j && j < A.L) { v[k] = score(cur, j); lmin = fmin(lmin, v[k]); lmax = fmax(lmax, v[k]); }
+          if (k < 3) cur = nxt;
+        }
+      } else {
+        RawRec<KIND> raw[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
+          if (i < j && j < A.L) load_raw<KIND>(A.cnt, rsg_rec(i, j, A.L), raw[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
+          v[k] = -INFINITY;
+          if (i < j && j < A.L) { v[k] = score(raw[k], j); lmin = fmin(lmin, v[k]); lmax = fmax(lmax, v[k]); }
+        }
+      }
+    } else if constexpr (KIND == CNT_F32) {
       // 64-byte (weighted) records: with two records in flight (the next one loads while this
       // one is scored) the FP32 statistic needs ~100 VGPRs and runs four waves per SIMD; four
       // records in flight took 176 VGPRs and two waves

@@ -138,7 +138,7 @@ struct rsg_msa {
   int64_t  lgc_n = -1;          // the N the log table holds (rebuilt when it differs)
   double  *d_cov = nullptr;     // L x L (FP64)
   double  *d_tmp = nullptr;     // L x L scratch (RAFS source)
-  double  *d_rowx = nullptr;    // L + 1 (row means, then COVavg)
+  double  *d_rowx = nullptr;    // 2 L + 2: row means, row sums, COVavg
   double  *d_eval = nullptr;    // L x L E-values (kept on the device for the hit list)
   int      have_eval = 0;
   double   minCOV = 0., maxCOV = 0.;

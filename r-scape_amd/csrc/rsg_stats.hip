@@ -521,8 +521,12 @@ int rsg_raf(rsg_ctx *c, rsg_msa *m, int rafs, const double *allow16, char *errbu
 }
 
 // ------------------------------------------------------------------------------------
-// corr_CalculateCOVCorrected: row means over j != i (FP64, fixed order), COVavg, then
-// the APC / ASC correction in place with a NaN flag and min / max.
+// corr_CalculateCOVCorrected (src/covariation.c): row sums over j != i (FP64, fixed order) and
+// the row means r / (L - 1), COVavg, then the APC / ASC correction in place with a NaN flag and
+// block min / max partials (k_reduce_minmax finishes them).
+__device__ __forceinline__ double row_dl(int64_t L) { return (L > 1) ? (double)L - 1. : 1.; }
+
+// Row sums from COV, one block per row (the order k_rowsum_b of the batched path reproduces).
 __global__ void __launch_bounds__(256) k_rowsum(const double *__restrict__ cov, int64_t L, double *__restrict__ rows,
                                                 int *__restrict__ flag)
 {
@@ -532,13 +536,13 @@ __global__ void __launch_bounds__(256) k_rowsum(const double *__restrict__ cov, 
   double s = 0.;
   for (int64_t j = threadIdx.x; j < L; j += 256) if (j != i) s += cov[i * L + j];
   s = block_sum_tree256(s, red);
-  if (threadIdx.x == 0) rows[L + i] = s;
+  if (threadIdx.x == 0) { rows[L + i] = s; rows[i] = s / row_dl(L); }
 }
 
-// APC / ASC in place, one block per row i (no per-element index division); block min / max
-// of the corrected row to part[2 i], part[2 i + 1].  Every block forms COVavg from the raw
-// row sums rows[L..2L) with the loop and tree of the batched k_rowmean_b (same order, same
-// value; block 0 publishes it) and the row means r / (L - 1) on the fly.
+// APC / ASC in place, one block per row i (no per-element index division), four elements per
+// thread in flight; block min / max of the corrected row to part[2 i], part[2 i + 1].  Every
+// block forms COVavg from the raw row sums rows[L..2L) with the loop and tree of the batched
+// k_rowmean_b (same order, same value; block 0 publishes it); the row means come from rows[0..L).
 __global__ void __launch_bounds__(256) k_correct(double *__restrict__ cov, int64_t L, const double *__restrict__ rows,
                                                  int actype, int *__restrict__ flag, double *__restrict__ part,
                                                  double *__restrict__ avg_out)
@@ -550,18 +554,27 @@ __global__ void __launch_bounds__(256) k_correct(double *__restrict__ cov, int64
   const double avg = (L > 1) ? s / ((double)L * ((double)L - 1.)) : s;
   __syncthreads();                                   // rmin is reused below
   if (blockIdx.x == 0 && threadIdx.x == 0) *avg_out = avg;
-  const double dl = (L > 1) ? (double)L - 1. : 1.;
   const int64_t i = blockIdx.x;
-  const double xi = rows[L + i] / dl;
+  const double xi = rows[i];
   double *C = cov + i * L;
   double a = INFINITY, b = -INFINITY;
-  for (int64_t j = threadIdx.x; j < L; j += 256) {
-    if (j == i) continue;
-    const double v = C[j], xj = rows[L + j] / dl;
-    const double r = (actype == RSG_APC) ? ((avg != 0.0) ? v - xi * xj / avg : 0.0) : v - (xi + xj - avg);
-    if (isnan(r)) atomicOr(flag, 1);
-    C[j] = r;
-    a = fmin(a, r); b = fmax(b, r);
+  for (int64_t j0 = threadIdx.x; j0 < L; j0 += 1024) {
+    double v[4], xj[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int64_t j = j0 + 256 * q;
+      if (j < L) { v[q] = C[j]; xj[q] = rows[j]; }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int64_t j = j0 + 256 * q;
+      if (j < L && j != i) {
+        const double r = (actype == RSG_APC) ? ((avg != 0.0) ? v[q] - xi * xj[q] / avg : 0.0) : v[q] - (xi + xj[q] - avg);
+        if (isnan(r)) atomicOr(flag, 1);
+        C[j] = r;
+        a = fmin(a, r); b = fmax(b, r);
+      }
+    }
   }
   block_minmax256(a, b, rmin, rmax);
   if (threadIdx.x == 0) { part[2 * i] = a; part[2 * i + 1] = b; }
