@@ -206,6 +206,12 @@ __device__ __forceinline__ double pair_stat_f32c16(const StatArgs &A, const floa
 #pragma unroll
   for (int y = 0; y < 4; y++) pj[y] = A.pm[j * 4 + y];
   const double sj = (pj[0] + pj[1]) + (pj[2] + pj[3]);
+  // the series coefficients 2 / (2k + 1) as register operands (empty asm statements pin them:
+  // 2 / 13 in a VGPR, the rest in SGPRs, one per instruction), so no cell spends vector moves
+  // re-creating them
+  float k3 = 2.f / 3.f, k5 = 2.f / 5.f, k7 = 2.f / 7.f, k9 = 2.f / 9.f, k11 = 2.f / 11.f, k13 = 2.f / 13.f;
+  asm("" : "+s"(k3), "+s"(k5), "+s"(k7), "+s"(k9), "+s"(k11));
+  asm("" : "+v"(k13));
   float acc = 0.f;
 #pragma unroll
   for (int x = 0; x < 4; x++) {
@@ -217,9 +223,8 @@ __device__ __forceinline__ double pair_stat_f32c16(const StatArgs &A, const floa
       const float e = (float)((double)cf - B);
       const float v = e * __builtin_amdgcn_rcpf(fmaf(2.f, cf, -e));
       const float u = v * v;
-      float s = 2.f / 13.f;
-      s = fmaf(s, u, 2.f / 11.f); s = fmaf(s, u, 2.f / 9.f); s = fmaf(s, u, 2.f / 7.f);
-      s = fmaf(s, u, 2.f / 5.f);  s = fmaf(s, u, 2.f / 3.f);
+      float s = fmaf(k13, u, k11);
+      s = fmaf(s, u, k9); s = fmaf(s, u, k7); s = fmaf(s, u, k5); s = fmaf(s, u, k3);
       float F = v * fmaf(cf * u, s, e);
       const bool far = fabsf(v) >= 0.3f;
       if (__builtin_amdgcn_ballot_w64(far)) {
@@ -238,13 +243,6 @@ __device__ __forceinline__ double pair_stat_f32c16(const StatArgs &A, const floa
   if constexpr (TYPE == RSG_MI) return s;
   const double ng = A.nsx ? A.nsx[rsg_rec(i, j, A.L)] : nsd;        // MIg
   return s - ((ng > 0.) ? (A.wsum - ng) / ng : 0.);
-}
-
-template <typename T, int KIND, int TYPE, int CLS>
-constexpr bool stat_short_form()
-{
-  return sizeof(T) == sizeof(float) && CLS == RSG_C16 && (KIND == CNT_U16 || KIND == CNT_F32) &&
-         (TYPE == RSG_GT || TYPE == RSG_MI || TYPE == RSG_MIg);
 }
 
 // A record as 16 FP32 counts and their FP64 sum (integer sum for uint16 records).
@@ -622,51 +620,7 @@ __device__ __forceinline__ void stat_pairs(const StatArgs &A, int bi, int bj, co
     const int ii = tid & 31;
     const int64_t i = (int64_t)bi * 32 + ii;
     double v[4];
-    if constexpr (stat_short_form<T, KIND, TYPE, CLS>()) {
-      // the i-side marginals and their sum once per thread
-      double pi[4] = {0., 0., 0., 0.}, si = 0.;
-      if (i < A.L) {
-#pragma unroll
-        for (int x = 0; x < 4; x++) pi[x] = A.pm[i * 4 + x];
-        si = (pi[0] + pi[1]) + (pi[2] + pi[3]);
-      }
-      auto score = [&](const RawRec<KIND> &r, int64_t j) -> double {
-        float cf[16];
-        const double nsd = decode_rec_f32<KIND>(r, cf);
-        return pair_stat_f32c16<KIND, TYPE>(A, cf, nsd, pi, si, i, j);
-      };
-      if constexpr (KIND == CNT_F32) {
-        RawRec<KIND> cur, nxt;
-        {
-          const int64_t j = (int64_t)bj * 32 + (tid >> 5);
-          if (i < j && j < A.L) load_raw<KIND>(A.cnt, rsg_rec(i, j, A.L), cur);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
-          if (k < 3) {
-            const int64_t j2 = j + 8;
-            if (i < j2 && j2 < A.L) load_raw<KIND>(A.cnt, rsg_rec(i, j2, A.L), nxt);
-          }
-          v[k] = -INFINITY;
-          if (i < j && j < A.L) { v[k] = score(cur, j); lmin = fmin(lmin, v[k]); lmax = fmax(lmax, v[k]); }
-          if (k < 3) cur = nxt;
-        }
-      } else {
-        RawRec<KIND> raw[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
-          if (i < j && j < A.L) load_raw<KIND>(A.cnt, rsg_rec(i, j, A.L), raw[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const int64_t j = (int64_t)bj * 32 + (tid >> 5) + 8 * k;
-          v[k] = -INFINITY;
-          if (i < j && j < A.L) { v[k] = score(raw[k], j); lmin = fmin(lmin, v[k]); lmax = fmax(lmax, v[k]); }
-        }
-      }
-    } else if constexpr (KIND == CNT_F32) {
+    if constexpr (KIND == CNT_F32) {
       // 64-byte (weighted) records: with two records in flight (the next one loads while this
       // one is scored) the FP32 statistic needs ~100 VGPRs and runs four waves per SIMD; four
       // records in flight took 176 VGPRs and two waves
@@ -726,6 +680,60 @@ __device__ __forceinline__ void stat_pairs(const StatArgs &A, int bi, int bj, co
       if (i < j && j < A.L) A.cov[i * A.L + j] = tile[jj][ii2];
     }
     __syncthreads();                      // the tile buffer is reused by the next call
+  }
+}
+
+// FP32 short-form statistics (GT, MI, MIg of class C16: pair_stat_f32c16) over half of a 32 x 32
+// pair tile (bi <= bj): columns j = bj*32 + 16h .. + 15 against i = bi*32 .. + 31, two pairs per
+// thread, so a grid has twice the blocks of whole tiles and its last blocks hold less work.
+// COV is written to both triangles (row j directly, row i through an LDS transpose); the
+// thread's min / max are folded into lmin / lmax.
+template <int KIND, int TYPE>
+__device__ __forceinline__ void stat_half_f32c16(const StatArgs &A, int bi, int bj, int h, double &lmin, double &lmax)
+{
+  __shared__ double tile[16][33];
+  const int tid = threadIdx.x, ii = tid & 31;
+  const int64_t i = (int64_t)bi * 32 + ii, jb = (int64_t)bj * 32 + 16 * h;
+  double pi[4] = {0., 0., 0., 0.}, si = 0.;
+  if (i < A.L) {
+#pragma unroll
+    for (int x = 0; x < 4; x++) pi[x] = A.pm[i * 4 + x];
+    si = (pi[0] + pi[1]) + (pi[2] + pi[3]);
+  }
+  RawRec<KIND> raw[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int64_t j = jb + (tid >> 5) + 8 * k;
+    if (i < j && j < A.L) load_raw<KIND>(A.cnt, rsg_rec(i, j, A.L), raw[k]);
+  }
+  double v[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int64_t j = jb + (tid >> 5) + 8 * k;
+    v[k] = -INFINITY;
+    if (i < j && j < A.L) {
+      float cf[16];
+      const double nsd = decode_rec_f32<KIND>(raw[k], cf);
+      v[k] = pair_stat_f32c16<KIND, TYPE>(A, cf, nsd, pi, si, i, j);
+      lmin = fmin(lmin, v[k]); lmax = fmax(lmax, v[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int jl = (tid >> 5) + 8 * k;
+    const int64_t j = jb + jl;
+    if (i < j && j < A.L) A.cov[j * A.L + i] = v[k];
+    else if (i == j && i < A.L) A.cov[i * A.L + j] = -INFINITY;
+    tile[jl][ii] = v[k];
+  }
+  __syncthreads();
+  const int jl = tid & 15;
+  const int64_t j = jb + jl;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int ii2 = (tid >> 4) + 16 * k;
+    const int64_t i2 = (int64_t)bi * 32 + ii2;
+    if (i2 < j && j < A.L) A.cov[i2 * A.L + j] = tile[jl][ii2];
   }
 }
 

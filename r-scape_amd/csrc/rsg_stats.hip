@@ -24,7 +24,7 @@ int rsg_reserve_scratch(rsg_ctx *c, int64_t L, char *errbuf)
 {
   const int64_t nbt = (L + 63) / 64, nb = (L + 31) / 32;
   const size_t need = std::max<size_t>({(size_t)(pm_npi(nbt) + pm_npj(nbt)), (size_t)(2 * pm32_n(nb)),
-                                        (size_t)(2 * nb * nb + 8 + 4 * L), (size_t)(2 * L + 8),
+                                        (size_t)(2 * nb * (nb + 1) + 8 + 4 * L), (size_t)(2 * L + 8),
                                         (size_t)(2 * RSG_MINMAX_BLOCKS + 8)});
   return ensure_red(c, need, errbuf);
 }
@@ -252,17 +252,37 @@ static int launch_stat_f64w_fixed(rsg_ctx *c, const StatArgs &A, int cls, int nb
   return ntri;
 }
 
-// FP32 statistics (pair_stat_f32) of class C16 with the statistic fixed at compile time, from
-// uint16 or FP32 (weighted) records; 0 when not handled here.
+// FP32 short-form statistics over half tiles: blocks 2t and 2t + 1 take the two halves of pair
+// tile t (bi <= bj); block min / max to part.
+template <int KIND, int TYPE>
+__global__ void __launch_bounds__(256) k_stat_half(StatArgs A, int nb)
+{
+  int bi, bj;
+  tri_tile(blockIdx.x >> 1, bi, bj);
+  double lmin = INFINITY, lmax = -INFINITY;
+  stat_half_f32c16<KIND, TYPE>(A, bi, bj, (int)(blockIdx.x & 1), lmin, lmax);
+  block_minmax(lmin, lmax, A.part, (int64_t)blockIdx.x);
+}
+
+// FP32 statistics of class C16 with the statistic fixed at compile time, from uint16 or FP32
+// (weighted) records: GT, MI and MIg in the short form over half tiles (pair_stat_f32c16),
+// the others through pair_stat_f32; 0 when not handled here.
 template <int TYPE>
 static int launch_stat_f32_fixed(rsg_ctx *c, const StatArgs &A, int kind, int cls, int nb)
 {
   if (cls != RSG_C16 || (kind != CNT_U16 && kind != CNT_F32)) return 0;
   const int ntri = nb * (nb + 1) / 2;
-  const dim3 grid((unsigned)ntri);
-  if (kind == CNT_U16) hipLaunchKernelGGL((k_stat<float, CNT_U16, false, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
-  else                 hipLaunchKernelGGL((k_stat<float, CNT_F32, false, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
-  return ntri;
+  if constexpr (TYPE == RSG_GT || TYPE == RSG_MI || TYPE == RSG_MIg) {
+    const dim3 grid((unsigned)(2 * ntri));
+    if (kind == CNT_U16) hipLaunchKernelGGL((k_stat_half<CNT_U16, TYPE>), grid, dim3(256), 0, c->stream, A, nb);
+    else                 hipLaunchKernelGGL((k_stat_half<CNT_F32, TYPE>), grid, dim3(256), 0, c->stream, A, nb);
+    return 2 * ntri;
+  } else {
+    const dim3 grid((unsigned)ntri);
+    if (kind == CNT_U16) hipLaunchKernelGGL((k_stat<float, CNT_U16, false, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
+    else                 hipLaunchKernelGGL((k_stat<float, CNT_F32, false, TYPE, RSG_C16>), grid, dim3(256), 0, c->stream, A, nb);
+    return ntri;
+  }
 }
 
 // (min, max) of n block partials into out[0..1]; one block of 1024 threads, four partials per
@@ -354,7 +374,8 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
   if (!m->d_cov) HIPCHK(hipMalloc(&m->d_cov, sizeof(double) * (size_t)(L * L)));
   const int nb = (int)((L + 31) / 32);
   const int nblk = nb * (nb + 1) / 2;             // pair tiles on or above the diagonal
-  if (ensure_red(c, 2 * (size_t)nblk + 8 + 4 * (size_t)L, errbuf) != RSG_OK) return RSG_FAIL;
+  // block min / max partials (2 per block; half-tile kernels have 2 nblk blocks), CCF rows after
+  if (ensure_red(c, 4 * (size_t)nblk + 8 + 4 * (size_t)L, errbuf) != RSG_OK) return RSG_FAIL;
   StatArgs A;
   A.cnt = m->d_cnt; A.pm = m->d_pm; A.lpm = m->d_pm + 4 * L; A.lgc = m->d_lgc;
   A.L = L; A.N = m->N; A.wsum = rsg_count_wsum(m); A.cov = m->d_cov; A.part = c->d_red;
