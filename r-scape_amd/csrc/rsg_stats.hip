@@ -285,35 +285,30 @@ static int launch_stat_f32_fixed(rsg_ctx *c, const StatArgs &A, int kind, int cl
   }
 }
 
-// (min, max) of n block partials into out[0..1]; one block of 1024 threads, four partials per
-// thread loaded at once.  With hout != nullptr out[0..4] (min, max and the flags and COVavg the
-// chain left in out[2..4]) also go to hout, host memory, followed (after a system-scope
-// release) by the completion stamp in hout[63], which the caller waits for.
-__global__ void __launch_bounds__(1024) k_reduce_minmax(const double *__restrict__ part, int n, double *__restrict__ out,
-                                                        double *__restrict__ hout, unsigned long long stamp)
+// (min, max) of n block partials into out[0..1]; one block of 256 threads, four partials per
+// thread loaded at once, then block_minmax256.  With hout != nullptr out[0..4] (min, max and the
+// flags and COVavg the chain left in out[2..4]) also go to hout, host memory, followed (after a
+// system-scope release) by the completion stamp in hout[63], which the caller waits for.
+__global__ void __launch_bounds__(256) k_reduce_minmax(const double *__restrict__ part, int n, double *__restrict__ out,
+                                                       double *__restrict__ hout, unsigned long long stamp)
 {
-  __shared__ double rmin[1024], rmax[1024];
+  __shared__ double rmin[256], rmax[256];
   double a = INFINITY, b = -INFINITY;
-  for (int k0 = threadIdx.x; k0 < n; k0 += 4096) {
+  for (int k0 = threadIdx.x; k0 < n; k0 += 1024) {
     double2 v[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int k = k0 + 1024 * q;
+      const int k = k0 + 256 * q;
       v[q] = (k < n) ? *(const double2 *)(part + 2 * k) : make_double2(INFINITY, -INFINITY);
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) { a = fmin(a, v[q].x); b = fmax(b, v[q].y); }
   }
-  rmin[threadIdx.x] = a; rmax[threadIdx.x] = b;
-  __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) { rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + s]); rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + s]); }
-    __syncthreads();
-  }
+  block_minmax256(a, b, rmin, rmax);
   if (threadIdx.x == 0) {
-    out[0] = rmin[0]; out[1] = rmax[0];
+    out[0] = a; out[1] = b;
     if (hout) {
-      hout[0] = rmin[0]; hout[1] = rmax[0];
+      hout[0] = a; hout[1] = b;
       hout[2] = out[2]; hout[3] = out[3]; hout[4] = out[4];
       __threadfence_system();
       *(volatile unsigned long long *)(hout + 63) = stamp;
@@ -338,7 +333,7 @@ static int fetch_minmax(rsg_ctx *c, rsg_msa *m, int nparts, char *errbuf)
 {
   if (c->skip_minmax) return RSG_OK;            // a correction recomputes them
   // a deferred chain ends here: the reduction also writes the result block to host memory
-  hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(1024), 0, c->stream, c->d_red, nparts, c->d_res,
+  hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(256), 0, c->stream, c->d_red, nparts, c->d_res,
                      c->defer ? c->d_hres : nullptr, ++c->hstamp);
   HIPCHK(hipGetLastError());
   if (c->defer) { c->res_on_host = 1; return RSG_OK; }
@@ -569,22 +564,31 @@ __global__ void __launch_bounds__(256) k_correct(double *__restrict__ cov, int64
                                                  double *__restrict__ avg_out)
 {
   __shared__ double rmin[257], rmax[256];
+  const int64_t i = blockIdx.x;
+  double *C = cov + i * L;
+  // the row's first four elements per thread are loaded before COVavg is formed, so their
+  // latency overlaps the reduction
+  double v[4], xj[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int64_t j = threadIdx.x + 256 * q;
+    if (j < L) { v[q] = C[j]; xj[q] = rows[j]; }
+  }
   double s = 0.;
   for (int64_t k = threadIdx.x; k < L; k += 256) s += rows[L + k];
   s = block_sum_tree256(s, rmin);
   const double avg = (L > 1) ? s / ((double)L * ((double)L - 1.)) : s;
   __syncthreads();                                   // rmin is reused below
   if (blockIdx.x == 0 && threadIdx.x == 0) *avg_out = avg;
-  const int64_t i = blockIdx.x;
   const double xi = rows[i];
-  double *C = cov + i * L;
   double a = INFINITY, b = -INFINITY;
   for (int64_t j0 = threadIdx.x; j0 < L; j0 += 1024) {
-    double v[4], xj[4];
+    if (j0 != threadIdx.x) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int64_t j = j0 + 256 * q;
-      if (j < L) { v[q] = C[j]; xj[q] = rows[j]; }
+      for (int q = 0; q < 4; q++) {
+        const int64_t j = j0 + 256 * q;
+        if (j < L) { v[q] = C[j]; xj[q] = rows[j]; }
+      }
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -626,7 +630,7 @@ int rsg_correct(rsg_ctx *c, rsg_msa *m, int actype, int shiftnonneg, char *errbu
                      c->d_res + 3);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(1024), 0, c->stream, c->d_red, nblk, c->d_res, direct ? c->d_hres : nullptr,
+  hipLaunchKernelGGL(k_reduce_minmax, dim3(1), dim3(256), 0, c->stream, c->d_red, nblk, c->d_res, direct ? c->d_hres : nullptr,
                      ++c->hstamp);
   if (direct) { c->res_on_host = 1; return RSG_OK; }
   if (shiftnonneg)

@@ -7,10 +7,13 @@ line per bucket.  Usage:
 
     python -m rscape_amd.autotune --N 5000 --L 1500 [--reps 10] [--save]
     python -m rscape_amd.autotune --grid [--save]      # every bucket of the default grid
+    python -m rscape_amd.autotune --from-log sweep.log [more.log ...] --save
+                                   # the table from the lines of earlier sweeps (later lines win)
 """
 import argparse
 import math
 import os
+import re
 
 import numpy as np
 
@@ -63,6 +66,23 @@ def write_table(tab, path=TABLE, notes=None):
             f.write("%d %d %d%s\n" % (a, b, s, ("  # " + notes[(a, b)]) if notes and (a, b) in notes else ""))
 
 
+_LINE = re.compile(r"N=(\d+) L=(\d+) best (\d+) (\S+) ([\d.]+) ms \((\d+) TOP/s\); built-in choice ([\d.]+) ms")
+
+
+def from_logs(paths):
+    """Table entries and notes from the result lines a sweep printed (later lines win)."""
+    tab, notes = {}, {}
+    for path in paths:
+        for line in open(path):
+            m = _LINE.search(line)
+            if not m:
+                continue
+            N, L, best = int(m.group(1)), int(m.group(2)), int(m.group(3))
+            tab[bucket(N, L)] = best
+            notes[bucket(N, L)] = "%s, N=%d L=%d: %s ms (built-in choice %s ms)" % (m.group(4), N, L, m.group(5), m.group(7))
+    return tab, notes
+
+
 def best_shape(N, L):
     """Recorded best shape for this size bucket, or 0 (the library default)."""
     return read_table().get(bucket(N, L), 0)
@@ -76,7 +96,14 @@ def main():
     ap.add_argument("--grid", action="store_true", help="sweep every bucket of the default grid")
     ap.add_argument("--save", action="store_true")
     ap.add_argument("--out", default=TABLE, help="table to update with --save")
+    ap.add_argument("--from-log", nargs="+", default=None, help="build the table from printed sweep lines")
     a = ap.parse_args()
+    if a.from_log:
+        tab, notes = from_logs(a.from_log)
+        print("%d buckets from %s" % (len(tab), ", ".join(a.from_log)))
+        if a.save:
+            write_table(tab, a.out, notes=notes)
+        return
     sizes = [(n, l) for n in GRID_N for l in GRID_L] if a.grid else [(a.N, a.L)]
     ctx = Context(0)
     tab, notes = read_table(a.out), {}

@@ -112,3 +112,22 @@ def test_gram_tiles_table_load(tmp_path):
         assert L.rsg_gram_tiles_load(autotune.TABLE.encode(), err) == 0, err.value
         tab = autotune.read_table()
         assert tab and all(1 <= v <= L.rsg_gram_nshapes() for v in tab.values())
+        # the committed table covers every bucket of the default sweep grid
+        assert {autotune.bucket(n, l) for n in autotune.GRID_N for l in autotune.GRID_L} <= set(tab)
+
+
+def test_autotune_table_from_sweep_lines(tmp_path):
+    """The table rebuilt from printed sweep lines: one entry per bucket, later lines win, other
+    lines ignored."""
+    from rscape_amd import autotune
+    a, b = tmp_path / "a.log", tmp_path / "b.log"
+    a.write_text("[gpurun] noise\n"
+                 "N=6144 L=1536 best 8 32x64c/4w/kt256 0.0933 ms (2484 TOP/s); built-in choice 0.0949 ms\n"
+                 "N=384 L=192 best 8 32x64c/4w/kt256 0.0123 ms (44 TOP/s); built-in choice 0.0130 ms\n")
+    b.write_text("N=5000 L=1500 best 7 64x32c/4w/kt256 0.0744 ms (2418 TOP/s); built-in choice 0.0744 ms\n")
+    tab, notes = autotune.from_logs([str(a), str(b)])
+    assert tab == {(13, 11): 7, (9, 8): 8}
+    assert "N=5000 L=1500" in notes[(13, 11)]
+    out = tmp_path / "t.tsv"
+    autotune.write_table(tab, str(out), notes=notes)
+    assert autotune.read_table(str(out)) == tab
