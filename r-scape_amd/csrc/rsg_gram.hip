@@ -349,12 +349,11 @@ __device__ __forceinline__ void split_unit(int u, int sbase, int sextra, int &t,
   else { t = sextra + (u - big) / sbase; sg = (u - big) % sbase; ns = sbase; }
 }
 
-// Grid: first the nsu split units, each over one K segment of one of the last tiles, writing
-// partial records for k_gram_fix (nsu8 groups: the count rounded up to a multiple of 8),
-// then 8 * per_xcd work groups for the first ndp tiles (whole K range).  The
-// split keeps the last, partial wave of tiles from leaving most CUs idle; running it first
-// lets its partial-record stores drain while the full tiles compute, and leaves the full
-// tiles' record stores as the only burst at the end.
+// Grid: first 8 * per_xcd work groups for the first ndp tiles (whole K range), then the nsu
+// split units, each over one K segment of one of the last tiles, writing partial records for
+// k_gram_fix.  The split keeps the last, partial wave of tiles from leaving most CUs idle; the
+// units go to the work-group slots as the full tiles free them, so the spread of the full
+// tiles' end times is filled rather than added to the kernel's length.
 template <int WI, int WJ, int KT, int OK>
 __global__ void __launch_bounds__(128 * WI * WJ) __attribute__((amdgpu_waves_per_eu(2)))
 k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t Lpad, int64_t L, void *__restrict__ out,
@@ -364,24 +363,24 @@ k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t Lpad, int64_t L,
   typedef CodeStage<32 * WI, 32 * WJ, KT, 128 * WI * WJ> St;
   __shared__ __attribute__((aligned(16))) uint8_t lds[St::lds_bytes(OK == CNT_U16)];
   const int64_t nkt = Npad / KT;
-  const int nsu = (ntiles - ndp) * sbase + sextra, nsu8 = (nsu + 7) & ~7;
+  const int nsu = (ntiles - ndp) * sbase + sextra;
   int t;
   int64_t k0 = 0, k1 = nkt;
   uint8_t *p = nullptr;
-  if ((int)blockIdx.x < nsu8) {
-    const int u = blockIdx.x;
+  if ((int)blockIdx.x < 8 * per_xcd) {
+    // XCD-aware mapping (work group g runs on XCD g & 7): consecutive tiles of the
+    // (bi, bj)-sorted list run on one XCD (one L2)
+    const int g = blockIdx.x;
+    t = (g & 7) * per_xcd + (g >> 3);
+    if (t >= ndp) return;
+  } else {
+    const int u = blockIdx.x - 8 * per_xcd;
     if (u >= nsu) return;
     int tr, sg, ns;
     split_unit(u, sbase, sextra, tr, sg, ns);
     t = ndp + tr;
     k0 = nkt * sg / ns; k1 = nkt * (sg + 1) / ns;
     p = part + (int64_t)u * (32 * WI) * (32 * WJ) * (OK == CNT_U16 ? 32 : 64);
-  } else {
-    // XCD-aware mapping (nsu8 is a multiple of 8, so g & 7 is still the XCD): consecutive
-    // tiles of the (bi, bj)-sorted list run on one XCD (one L2).
-    const int g = blockIdx.x - nsu8;
-    t = (g & 7) * per_xcd + (g >> 3);
-    if (t >= ndp) return;
   }
   if (pmflag && blockIdx.x == 0 && threadIdx.x == 0) *pmflag = 0;    // set by the marginal reduction
   const int2 tl = tiles[t];
@@ -755,14 +754,17 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, double *pm_part, int *pm
   if (ntiles == 0) return RSG_OK;
   if (OK == CNT_U16 && m->N > RSG_U16_MAXN) return rsg_fail(errbuf, "gram: uint16 records hold up to %d sequences", RSG_U16_MAXN);
   // whole waves of tiles run over the full K range; when there are few waves (under four
-  // tiles per CU) the last partial wave is split along K into enough segments to occupy every
-  // CU, then reduced (with many waves the tail is small and the partial traffic costs more)
+  // tiles per work-group slot) and the last wave is at most half full, its tiles are split along
+  // K into one segment per CU, which run after the full tiles in the slots the first of them
+  // free (the two work groups that share a CU finish far apart), then reduced (with many waves
+  // the tail is small and the partial traffic costs more).  A CU holds 8 waves of this kernel
+  // (two per SIMD): 4 / (WI WJ) work groups.
   const int ncu = c->ncu > 0 ? c->ncu : 256;
+  const int slots = std::max(ncu * 4 / (WI * WJ), 1);
   const int64_t nkt = m->Npad / KT;
   int ndp = ntiles, sbase = 1, sextra = 0;
-  const int rem = ntiles % ncu;
-  if (c->gram_split && ntiles < 4 * ncu && rem > 0 && rem <= ncu / 2 && nkt >= 2) {
-    // every CU takes one K segment of the last tiles before the full tiles start
+  const int rem = ntiles % slots;
+  if (c->gram_split && ntiles < 4 * slots && rem > 0 && rem <= slots / 2 && nkt >= 2) {
     ndp = ntiles - rem;
     const int nsu = (int)std::min<int64_t>((int64_t)ncu, (int64_t)rem * nkt);
     sbase = nsu / rem; sextra = nsu % rem;
@@ -781,7 +783,7 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, double *pm_part, int *pm
     part = (uint8_t *)c->d_gscr;
   }
   const int per = (ndp + 7) / 8;
-  dim3 grid((unsigned)(8 * per + ((nsu + 7) & ~7))), block(128 * WI * WJ);
+  dim3 grid((unsigned)(8 * per + nsu)), block(128 * WI * WJ);
   // one timer over the Gram and its split-K fix-up
   int t; rsg_tstart(c, "gram_i8", &t);
   hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->Lpad, m->L, out,
