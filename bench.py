@@ -344,8 +344,10 @@ def calibration_kernel(ctx):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # enough untimed steps for the GPU clocks to leave their idle state after the host-side data
+    # generation, and enough timed ones (~60 ms) for a steady rate
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--N", type=int, default=5000)
     ap.add_argument("--L", type=int, default=1500)
     ap.add_argument("--cpu-window", type=int, default=1000)
