@@ -424,6 +424,8 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
     b_sim += (L + 63) / 64; b_hist += (L * L + HIST_EPB - 1) / HIST_EPB;
     nmax = std::max(nmax, N);
   }
+  rsg_thost(c, "batch_plan_layout", since(h0));
+  h0 = clk();
   // Gram tiles, longest K loops first so the tail of each launch is short work: the jobs in
   // order of decreasing padded sequence count (ties in job order), each job's tiles row by row
   {
@@ -445,23 +447,31 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
       }
     }
   }
-  std::vector<int> left, right;
-  std::vector<double> ld, rd, models;
+  rsg_thost(c, "batch_plan_tiles", since(h0));
+  h0 = clk();
+  HostArr<int> left, right;
+  HostArr<double> ld, rd, models;
   SimLevels lv;
   std::vector<int> fperm, ftree;
   std::vector<uint64_t> fseed;
   if (!fp) {
-    left.resize((size_t)nodes); right.resize((size_t)nodes);
-    ld.resize((size_t)nodes); rd.resize((size_t)nodes); models.resize((size_t)nj * 40);
-    for (int k = 0; k < nj; k++) {
-      const int64_t off = sim[(size_t)k].tree_off, m1 = jobs[k].nleaf - 1;
-      memcpy(&left[(size_t)off], jobs[k].left, sizeof(int) * m1);
-      memcpy(&right[(size_t)off], jobs[k].right, sizeof(int) * m1);
-      memcpy(&ld[(size_t)off], jobs[k].ld, sizeof(double) * m1);
-      memcpy(&rd[(size_t)off], jobs[k].rd, sizeof(double) * m1);
-      memcpy(&models[(size_t)k * 40], jobs[k].model, sizeof(double) * 40);
-    }
+    left.alloc((size_t)nodes); right.alloc((size_t)nodes);
+    ld.alloc((size_t)nodes); rd.alloc((size_t)nodes); models.alloc((size_t)nj * 40);
+    rsg_parallel_ranges(nj, nodes, [&](int k) { return jobs[k].nleaf - 1; }, [&](int k0, int k1) {
+      for (int k = k0; k < k1; k++) {
+        const int64_t off = sim[(size_t)k].tree_off, m1 = jobs[k].nleaf - 1;
+        memcpy(&left[(size_t)off], jobs[k].left, sizeof(int) * m1);
+        memcpy(&right[(size_t)off], jobs[k].right, sizeof(int) * m1);
+        memcpy(&ld[(size_t)off], jobs[k].ld, sizeof(double) * m1);
+        memcpy(&rd[(size_t)off], jobs[k].rd, sizeof(double) * m1);
+        memcpy(&models[(size_t)k * 40], jobs[k].model, sizeof(double) * 40);
+      }
+    });
+    rsg_thost(c, "batch_plan_trees", since(h0));
+    h0 = clk();
     if (rsg_sim_levels(sim.data(), nj, left.data(), right.data(), lv, errbuf) != RSG_OK) return RSG_FAIL;
+    rsg_thost(c, "batch_plan_levels", since(h0));
+    h0 = clk();
   } else {
     // one column permutation per replicate; the tree as {left, right, parent, level-ordered nodes}
     for (int k = 0; k < nj; k++)
@@ -517,7 +527,7 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
       f_nsub = A.get<int>(R * 2 * (size_t)fp->m * 25); f_cols = A.get<int>(R * 2 * (size_t)fp->maxw * L);
     }
   };
-  rsg_thost(c, "batch_host_plan", since(h0));
+  rsg_thost(c, "batch_plan_rest", since(h0));
   h0 = clk();
   size_t plan_bytes = 0;
   {

@@ -8,6 +8,9 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <thread>
+#include <memory>
+#include <algorithm>
 #include "../../include/rscape_hip.h"
 
 #define RSG_MAX_TIMERS 64
@@ -150,6 +153,33 @@ struct rsg_msa {
 static inline double rsg_count_wsum(const rsg_msa *m) { return m->wq_counts ? m->wsum_q : m->wsum; }
 
 
+// Host threads for planning work: the hardware's, at most 16 (one GPU's share of a node).
+static inline int rsg_host_threads()
+{
+  const unsigned h = std::thread::hardware_concurrency();
+  return (int)std::min(16u, std::max(1u, h));
+}
+
+// fn(k0, k1) over contiguous ranges of [0, n) of about equal total weight w(k) (total: their
+// sum), one range per host thread (fewer when the total is under min_work per thread); the
+// calling thread takes the first range.
+template <class W, class F>
+static void rsg_parallel_ranges(int n, int64_t total, W w, F fn, int64_t min_work = 1 << 16)
+{
+  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)rsg_host_threads(), (int64_t)n, total / min_work + 1}));
+  std::vector<int> rb((size_t)nt + 1, n);
+  rb[0] = 0;
+  int64_t acc = 0;
+  for (int k = 0, t = 1; k < n && t < nt; k++) {
+    acc += w(k);
+    while (t < nt && acc * nt >= total * t) rb[(size_t)t++] = k + 1;
+  }
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; t++) th.emplace_back(fn, rb[(size_t)t], rb[(size_t)t + 1]);
+  fn(rb[0], rb[1]);
+  for (auto &x : th) x.join();
+}
+
 // ---- error helpers ----
 static inline int rsg_fail(char *errbuf, const char *fmt, ...)
 {
@@ -219,8 +249,20 @@ struct SimItem {
 };
 // Tree levels of a set of items: entries {item, internal node, first block within the level, 0}
 // grouped by depth.
+// Host staging array without value-initialisation (every element is written before it is read).
+template <class T> struct HostArr {
+  std::unique_ptr<T[]> p;
+  size_t n = 0;
+  void alloc(size_t k) { p.reset(new T[k]); n = k; }
+  T *data() { return p.get(); }
+  const T *data() const { return p.get(); }
+  size_t size() const { return n; }
+  T &operator[](size_t i) { return p[i]; }
+  const T &operator[](size_t i) const { return p[i]; }
+};
+
 struct SimLevels {
-  std::vector<int4> ent;
+  HostArr<int4> ent;
   std::vector<int64_t> lev_ent0;     // entry offset of each level (size nlev + 1)
   std::vector<int64_t> lev_blocks;   // 64-column blocks of each level
   std::vector<int64_t> lev_maxl;     // longest alignment among each level's entries

@@ -23,6 +23,8 @@
 #include <algorithm>
 #include <climits>
 #include <vector>
+#include <atomic>
+#include <thread>
 
 // Transition tables are rounded as the host computes them (no FMA contraction).
 #pragma clang fp contract(off)
@@ -295,48 +297,94 @@ static int tree_depths(const int *left, const int *right, int64_t N, int *depth,
 
 int rsg_sim_levels(const SimItem *items, int nitems, const int *left, const int *right, SimLevels &lv, char *errbuf)
 {
-  // pass 1: depths of all nodes and the number of nodes and 64-column blocks per depth
+  // Items in contiguous ranges of about equal node counts, one per host thread.  Pass 1: depths
+  // of the range's nodes and its node / 64-column block counts per depth.  Thread 0 then merges
+  // the counts (level offsets, and each range's start within every level: the ranges in item
+  // order), and pass 2 places the entries.  A node's draws are keyed by its item and row, so the
+  // order of a level's entries only sets which blocks simulate it: items ascending, each item's
+  // nodes ascending, as one thread would place them.
   int64_t nodes = 0;
   for (int k = 0; k < nitems; k++) nodes += items[k].N - 1;
-  std::vector<int> depth((size_t)nodes), stack;
-  std::vector<int64_t> cnt, blk, mxl;
-  for (int k = 0; k < nitems; k++) {
-    const SimItem &it = items[k];
-    int *dk = depth.data() + it.tree_off;
-    if (tree_depths(left + it.tree_off, right + it.tree_off, it.N, dk, stack, k, errbuf) != RSG_OK) return RSG_FAIL;
-    const int64_t nbk = (it.L + 63) / 64;
-    for (int64_t v = 0; v < it.N - 1; v++) {
-      const int dv = dk[v];
-      if ((int64_t)cnt.size() <= dv) { cnt.resize((size_t)dv + 1, 0); blk.resize((size_t)dv + 1, 0); mxl.resize((size_t)dv + 1, 0); }
-      cnt[(size_t)dv]++; blk[(size_t)dv] += nbk;
-      if (it.L > mxl[(size_t)dv]) mxl[(size_t)dv] = it.L;
+  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)rsg_host_threads(), (int64_t)nitems, nodes / 16384 + 1}));
+  std::vector<int> rb((size_t)nt + 1, nitems);
+  rb[0] = 0;
+  {
+    int64_t acc = 0;
+    int t = 1;
+    for (int k = 0; k < nitems && t < nt; k++) {
+      acc += items[k].N - 1;
+      while (t < nt && acc * nt >= nodes * t) rb[(size_t)t++] = k + 1;
     }
   }
-  const size_t nlev = cnt.size();
-  lv.lev_ent0.assign(nlev + 1, 0);
-  lv.lev_blocks.assign(nlev, 0);
-  lv.lev_maxl.assign(nlev, 0);
-  for (size_t l = 0; l < nlev; l++) {
-    lv.lev_maxl[l] = mxl[l];
-    if (blk[l] > INT32_MAX) return rsg_fail(errbuf, "too many column blocks in one tree level");
-    lv.lev_ent0[l + 1] = lv.lev_ent0[l] + cnt[l];
-    lv.lev_blocks[l] = blk[l];
-  }
-  // pass 2: a node's draws are keyed by its item and row, so the order of a level's entries only
-  // sets which blocks simulate it: items ascending, each item's nodes ascending
-  lv.ent.resize((size_t)nodes);
-  std::vector<int64_t> fill(lv.lev_ent0.begin(), lv.lev_ent0.end() - 1), bfill(nlev, 0);
-  for (int k = 0; k < nitems; k++) {
-    const SimItem &it = items[k];
-    const int *dk = depth.data() + it.tree_off;
-    const int64_t nbk = (it.L + 63) / 64;
-    for (int64_t v = 0; v < it.N - 1; v++) {
-      const size_t dv = (size_t)dk[v];
-      lv.ent[(size_t)fill[dv]++] = make_int4(k, (int)v, (int)bfill[dv], 0);
-      bfill[dv] += nbk;
+  struct Range { std::vector<int64_t> cnt, blk, mxl, fill, bfill; int st = RSG_OK; char err[RSG_ERRBUFSIZE]; };
+  std::vector<Range> R((size_t)nt);
+  std::vector<int> depth((size_t)nodes);
+  lv.ent.alloc((size_t)nodes);
+  std::atomic<int> arrived{0}, merged{0};
+  int status = RSG_OK;
+  auto work = [&](int t) {
+    Range &r = R[(size_t)t];
+    std::vector<int> stack;
+    for (int k = rb[(size_t)t]; k < rb[(size_t)t + 1] && r.st == RSG_OK; k++) {
+      const SimItem &it = items[k];
+      int *dk = depth.data() + it.tree_off;
+      if (tree_depths(left + it.tree_off, right + it.tree_off, it.N, dk, stack, k, r.err) != RSG_OK) { r.st = RSG_FAIL; break; }
+      int dmax = 0;
+      for (int64_t v = 0; v < it.N - 1; v++) dmax = std::max(dmax, dk[v]);
+      if ((int64_t)r.cnt.size() <= dmax) { r.cnt.resize((size_t)dmax + 1, 0); r.blk.resize((size_t)dmax + 1, 0); r.mxl.resize((size_t)dmax + 1, 0); }
+      const int64_t nbk = (it.L + 63) / 64;
+      for (int64_t v = 0; v < it.N - 1; v++) {
+        const size_t dv = (size_t)dk[v];
+        r.cnt[dv]++; r.blk[dv] += nbk;
+        if (it.L > r.mxl[dv]) r.mxl[dv] = it.L;
+      }
     }
-  }
-  return RSG_OK;
+    arrived.fetch_add(1);
+    if (t == 0) {
+      while (arrived.load() < nt) std::this_thread::yield();
+      for (int u = 0; u < nt && status == RSG_OK; u++)
+        if (R[(size_t)u].st != RSG_OK) status = rsg_fail(errbuf, "%s", R[(size_t)u].err);
+      size_t nlev = 0;
+      for (const Range &q : R) nlev = std::max(nlev, q.cnt.size());
+      if (status == RSG_OK) {
+        lv.lev_ent0.assign(nlev + 1, 0);
+        lv.lev_blocks.assign(nlev, 0);
+        lv.lev_maxl.assign(nlev, 0);
+        for (const Range &q : R)
+          for (size_t l = 0; l < q.cnt.size(); l++) {
+            lv.lev_ent0[l + 1] += q.cnt[l]; lv.lev_blocks[l] += q.blk[l]; lv.lev_maxl[l] = std::max(lv.lev_maxl[l], q.mxl[l]);
+          }
+        for (size_t l = 0; l < nlev; l++) {
+          if (lv.lev_blocks[l] > INT32_MAX) { status = rsg_fail(errbuf, "too many column blocks in one tree level"); break; }
+          lv.lev_ent0[l + 1] += lv.lev_ent0[l];
+        }
+        std::vector<int64_t> f(lv.lev_ent0.begin(), lv.lev_ent0.end() - 1), bf(nlev, 0);
+        for (Range &q : R) {
+          q.fill = f; q.bfill = bf;
+          for (size_t l = 0; l < q.cnt.size(); l++) { f[l] += q.cnt[l]; bf[l] += q.blk[l]; }
+        }
+      }
+      merged.store(1);
+    } else {
+      while (!merged.load()) std::this_thread::yield();
+    }
+    if (status != RSG_OK) return;
+    for (int k = rb[(size_t)t]; k < rb[(size_t)t + 1]; k++) {
+      const SimItem &it = items[k];
+      const int *dk = depth.data() + it.tree_off;
+      const int64_t nbk = (it.L + 63) / 64;
+      for (int64_t v = 0; v < it.N - 1; v++) {
+        const size_t dv = (size_t)dk[v];
+        lv.ent[(size_t)r.fill[dv]++] = make_int4(k, (int)v, (int)r.bfill[dv], 0);
+        r.bfill[dv] += nbk;
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto &x : th) x.join();
+  return status;
 }
 
 // Launch-only entry (no allocation, no synchronisation; usable under stream capture).

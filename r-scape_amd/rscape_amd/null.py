@@ -36,19 +36,11 @@ def _addr(a):
 
 def job_array(jobs):
     """The jobs as one rsg_null_job array (numpy records in the C layout; the jobs' arrays
-    must stay alive while it is used).  Jobs with group=None get group = their index."""
-    n = len(jobs)
-    rec = np.zeros(n, _JOB_DTYPE)
-    rec["nleaf"] = [j.tree["n"] for j in jobs]
-    rec["alen"] = [j.L for j in jobs]
-    for f in ("left", "right", "ld", "rd"):
-        rec[f] = [_addr(j.tree[f]) for j in jobs]
-    rec["model"] = [_addr(j.model) for j in jobs]
-    rec["seed"] = [j.seed for j in jobs]
-    rec["stream"] = [j.stream for j in jobs]
-    rec["gapfrac"] = [j.gapfrac for j in jobs]
-    rec["group"] = [k if j.group is None else j.group for k, j in enumerate(jobs)]
-    return rec
+    must stay alive while it is used).  Jobs with group=None get group = their index.  One
+    tuple per job (the array addresses are taken once, when the job is made)."""
+    rows = [(j.tree["n"], j.L) + j.addrs() + (j.seed, j.stream, j.gapfrac, k if j.group is None else j.group)
+            for k, j in enumerate(jobs)]
+    return np.array(rows, dtype=_JOB_DTYPE) if rows else np.zeros(0, _JOB_DTYPE)
 
 
 class E1Rates(C.Structure):
@@ -140,6 +132,14 @@ class Job:
                      for k, v in tree.items()}
         self.L, self.model = int(L), np.ascontiguousarray(model, dtype=np.float64)
         self.seed, self.stream, self.gapfrac, self.group = seed, stream, gapfrac, group
+        self._addrs = None
+
+    def addrs(self):
+        """Addresses of the tree arrays (left, right, ld, rd) and the model, for the C records."""
+        if self._addrs is None:
+            t = self.tree
+            self._addrs = (_addr(t["left"]), _addr(t["right"]), _addr(t["ld"]), _addr(t["rd"]), _addr(self.model))
+        return self._addrs
 
     def struct(self, group=0):
         t = self.tree
