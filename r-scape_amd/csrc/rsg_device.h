@@ -178,7 +178,7 @@ __device__ double pair_stat_f32(const StatArgs &A, const double *cd, int64_t i, 
   }
 }
 
-// ---- FP32 statistics of class C16 for GT, MI and MIg: the short form.
+// ---- FP32 statistics of class C16 (GT, MI, MIg, MIr, CHI, OMES): the short form.
 // Every cell of the 4 x 4 table has q = pm_i[x] pm_j[y] > 0 (the marginals carry the 1e-10
 // pseudo-counts of every pair, or are uniform).  With c' = c + 1e-10, den = ns + 16e-10 and the
 // expected count B = den q, the cell's term of sum pp ln(pp / q) splits as
@@ -194,7 +194,8 @@ __device__ double pair_stat_f32(const StatArgs &A, const double *cd, int64_t i, 
 // c' and c differ by 1e-10, which changes e and c + B by less than 1e-10 of any F that counts;
 // a zero count has v = -1 and takes the second form with c' = 1e-10.  The logarithm is
 // evaluated only in waves where some lane needs it (a wave-uniform branch; about 2% of the
-// cells of a wave at N = 5000, L = 1500).
+// cells of a wave at N = 5000, L = 1500).  MIr divides by H = -sum pp ln pp (hardware logarithm
+// per cell, as pair_stat_f32); CHI and OMES are (ns / den) sum e^2 / B and (ns / den^2) sum e^2.
 // c: the 16 counts as FP32 (exact for uint16 records, the record values for FP32 ones);
 // nsd: their FP64 sum; pi: pm_i; si: sum_x pm_i[x].
 template <int KIND, int TYPE>
@@ -205,7 +206,25 @@ __device__ __forceinline__ double pair_stat_f32c16(const StatArgs &A, const floa
   double pj[4];
 #pragma unroll
   for (int y = 0; y < 4; y++) pj[y] = A.pm[j * 4 + y];
+  if constexpr (TYPE == RSG_CHI || TYPE == RSG_OMES) {
+    // ns sum (pp - q)^2 / q = (ns / den) sum e^2 / B;  ns sum (pp - q)^2 = (ns / den^2) sum e^2
+    float ssq = 0.f;
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      const double ax = den * pi[x];
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        const double B = ax * pj[y];
+        const float e = (float)((double)c[x * 4 + y] - B);
+        ssq = (TYPE == RSG_CHI) ? fmaf(e * e, __builtin_amdgcn_rcpf((float)B), ssq) : fmaf(e, e, ssq);
+      }
+    }
+    if (!(nsd > 0.)) return 0.;
+    return (TYPE == RSG_CHI) ? nsd / den * (double)ssq : nsd / (den * den) * (double)ssq;
+  }
   const double sj = (pj[0] + pj[1]) + (pj[2] + pj[3]);
+  const float rdf = __builtin_amdgcn_rcpf((float)den);
+  float hsum = 0.f;                                      // MIr: sum pp log2 pp
   // the series coefficients 2 / (2k + 1) as register operands (empty asm statements pin them:
   // 2 / 13 in a VGPR, the rest in SGPRs, one per instruction), so no cell spends vector moves
   // re-creating them
@@ -236,11 +255,19 @@ __device__ __forceinline__ double pair_stat_f32c16(const StatArgs &A, const floa
         F = far ? Fl : F;
       }
       acc += F;
+      if constexpr (TYPE == RSG_MIr) {
+        const float pp = fmaxf(cf, 1e-10f) * rdf;
+        hsum = fmaf(pp, __builtin_amdgcn_logf(pp), hsum);
+      }
     }
   }
-  const double s = (double)(acc * __builtin_amdgcn_rcpf((float)den)) + fma(-si, sj, 1.0);   // sum pp ln(pp / q)
+  const double s = (double)(acc * rdf) + fma(-si, sj, 1.0);   // sum pp ln(pp / q)
   if constexpr (TYPE == RSG_GT) return (nsd > 0.) ? 2. * nsd * s : 0.;
   if constexpr (TYPE == RSG_MI) return s;
+  if constexpr (TYPE == RSG_MIr) {                                 // MI / H, H = -sum pp ln pp
+    const float H = -hsum * 0.693147180559945309f;
+    return (H > 1e-2f) ? s / (double)H : 0.;
+  }
   const double ng = A.nsx ? A.nsx[rsg_rec(i, j, A.L)] : nsd;        // MIg
   return s - ((ng > 0.) ? (A.wsum - ng) / ng : 0.);
 }
@@ -683,7 +710,7 @@ __device__ __forceinline__ void stat_pairs(const StatArgs &A, int bi, int bj, co
   }
 }
 
-// FP32 short-form statistics (GT, MI, MIg of class C16: pair_stat_f32c16) over half of a 32 x 32
+// FP32 short-form statistics (class C16: pair_stat_f32c16) over half of a 32 x 32
 // pair tile (bi <= bj): columns j = bj*32 + 16h .. + 15 against i = bi*32 .. + 31, two pairs per
 // thread, so a grid has twice the blocks of whole tiles and its last blocks hold less work.
 // COV is written to both triangles (row j directly, row i through an LDS transpose); the

@@ -265,14 +265,14 @@ __global__ void __launch_bounds__(256) k_stat_half(StatArgs A, int nb)
 }
 
 // FP32 statistics of class C16 with the statistic fixed at compile time, from uint16 or FP32
-// (weighted) records: GT, MI and MIg in the short form over half tiles (pair_stat_f32c16),
-// the others through pair_stat_f32; 0 when not handled here.
+// (weighted) records, in the short form over half tiles (pair_stat_f32c16); 0 when not handled
+// here (other classes go through pair_stat_f32).
 template <int TYPE>
 static int launch_stat_f32_fixed(rsg_ctx *c, const StatArgs &A, int kind, int cls, int nb)
 {
   if (cls != RSG_C16 || (kind != CNT_U16 && kind != CNT_F32)) return 0;
   const int ntri = nb * (nb + 1) / 2;
-  if constexpr (TYPE == RSG_GT || TYPE == RSG_MI || TYPE == RSG_MIg) {
+  if constexpr (TYPE == RSG_GT || TYPE == RSG_MI || TYPE == RSG_MIg || TYPE == RSG_MIr || TYPE == RSG_CHI || TYPE == RSG_OMES) {
     const dim3 grid((unsigned)(2 * ntri));
     if (kind == CNT_U16) hipLaunchKernelGGL((k_stat_half<CNT_U16, TYPE>), grid, dim3(256), 0, c->stream, A, nb);
     else                 hipLaunchKernelGGL((k_stat_half<CNT_F32, TYPE>), grid, dim3(256), 0, c->stream, A, nb);
@@ -423,6 +423,7 @@ int rsg_stat(rsg_ctx *c, rsg_msa *m, int base, int cls, const double *allow16, i
     case RSG_MIr:  nparts = launch_stat_f32_fixed<RSG_MIr>(c, A, m->cnt_kind, cls, nb); break;
     case RSG_MIg:  nparts = launch_stat_f32_fixed<RSG_MIg>(c, A, m->cnt_kind, cls, nb); break;
     case RSG_CHI:  nparts = launch_stat_f32_fixed<RSG_CHI>(c, A, m->cnt_kind, cls, nb); break;
+    case RSG_OMES: nparts = launch_stat_f32_fixed<RSG_OMES>(c, A, m->cnt_kind, cls, nb); break;
     }
   }
   if (nparts > 0) {
