@@ -51,6 +51,15 @@ static uint32_t rng_tag(uint64_t seed, uint32_t stream, uint32_t node, uint32_t 
   return o[0];
 }
 
+/* The per-node block of the GTR evolution (tag 5, keyed by the internal node v and the column):
+ * words 0 / 1 are the left / right child's symbol draws, 2 / 3 their leaf-gap draws. */
+static uint32_t node_word(uint64_t seed, uint32_t stream, uint32_t v, uint32_t pos, int k)
+{
+  uint32_t ctr[4] = { pos, v, stream, 5u }, key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) }, o[4];
+  ref_philox4x32(ctr, key, o);
+  return o[k];
+}
+
 /* Inverse CDF over n categories with n-1 uint32 thresholds: the reference's
  * "pdf += P[i]; if (pdf > x) break" (cov_simulate.c:756-763) with x = u / 2^32. */
 static int icdf(uint32_t u, const uint32_t *cdf, int n)
@@ -107,11 +116,11 @@ int ref_evolve_gtr(int64_t nleaf, int64_t L, const int *left, const int *right, 
       double t = side ? rd[v] : ld[v];
       for (int a = 0; a < 4; a++) row_cdf(model, t, a, tab + a * 3);
       for (int64_t pos = 0; pos < L; pos++) {
-        uint32_t u = ref_rng_u32(seed, stream, (uint32_t)row, (uint32_t)pos);
+        uint32_t u = node_word(seed, stream, (uint32_t)v, (uint32_t)pos, side);
         int sym = icdf(u, tab + S[v * L + pos] * 3, 4);
         S[row * L + pos] = (uint8_t)sym;
         if (d <= 0) {
-          if (gapthr && rng_tag(seed, stream, (uint32_t)row, (uint32_t)pos, 1u) < gapthr) sym = 4;
+          if (gapthr && node_word(seed, stream, (uint32_t)v, (uint32_t)pos, 2 + side) < gapthr) sym = 4;
           leaves[(-d) * L + pos] = (uint8_t)sym;
         }
       }
@@ -124,7 +133,7 @@ int ref_evolve_gtr(int64_t nleaf, int64_t L, const int *left, const int *right, 
  * column pair (i, ct[i]) evolves as one 16-state character, pair state a * 4 + b; the root pair
  * is drawn from the stationary pair frequencies, a child pair from its parent pair's P(t) row.
  * m16: [0..255] Pi^-1/2 V, [256..511] V^T Pi^1/2, [512..527] eigenvalues, [528..543] pair
- * frequencies.  Keys: (row, 5' column, tag 2); leaf gaps reuse the tag-1 draws.  Unpaired
+ * frequencies.  Keys: (row, 5' column, tag 2); leaf gaps reuse the per-node gap words.  Unpaired
  * columns (and, before they are overwritten, paired ones) evolve as in ref_evolve_gtr.
  * ct[c] = partner column or -1. */
 static void row_cdf16(const double *m, double t, int a, uint32_t *out)
@@ -179,11 +188,11 @@ int ref_evolve_pairs(int64_t nleaf, int64_t L, const int *left, const int *right
       for (int a = 0; a < 4; a++) row_cdf(model, t, a, tab + a * 3);
       for (int a = 0; a < 16; a++) row_cdf16(m16, t, a, tab16 + a * 15);
       for (int64_t pos = 0; pos < L; pos++) {
-        uint32_t u = ref_rng_u32(seed, stream, (uint32_t)row, (uint32_t)pos);
+        uint32_t u = node_word(seed, stream, (uint32_t)v, (uint32_t)pos, side);
         int sym = icdf(u, tab + S[v * L + pos] * 3, 4);
         S[row * L + pos] = (uint8_t)sym;
         if (d <= 0) {
-          if (gapthr && rng_tag(seed, stream, (uint32_t)row, (uint32_t)pos, 1u) < gapthr) sym = 4;
+          if (gapthr && node_word(seed, stream, (uint32_t)v, (uint32_t)pos, 2 + side) < gapthr) sym = 4;
           leaves[(-d) * L + pos] = (uint8_t)sym;
         }
       }
@@ -195,8 +204,8 @@ int ref_evolve_pairs(int64_t nleaf, int64_t L, const int *left, const int *right
         int x = st >> 2, y = st & 3;
         S[row * L + pos] = (uint8_t)x; S[row * L + q] = (uint8_t)y;
         if (d <= 0) {
-          if (gapthr && rng_tag(seed, stream, (uint32_t)row, (uint32_t)pos, 1u) < gapthr) x = 4;
-          if (gapthr && rng_tag(seed, stream, (uint32_t)row, (uint32_t)q, 1u) < gapthr) y = 4;
+          if (gapthr && node_word(seed, stream, (uint32_t)v, (uint32_t)pos, 2 + side) < gapthr) x = 4;
+          if (gapthr && node_word(seed, stream, (uint32_t)v, (uint32_t)q, 2 + side) < gapthr) y = 4;
           leaves[(-d) * L + pos] = (uint8_t)x; leaves[(-d) * L + q] = (uint8_t)y;
         }
       }

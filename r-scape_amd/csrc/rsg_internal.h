@@ -20,9 +20,10 @@
 __host__ __device__ static inline int64_t rsg_rec(int64_t i, int64_t j, int64_t L) { return j * L + i; }
 
 // Philox4x32-10 counter-based generator (Salmon, Moraes, Dror & Shaw, SC'11).  A draw is a
-// pure function of (seed, stream, node, position, tag), so simulation is order-free.
-__host__ __device__ static inline uint32_t rsg_philox(uint64_t seed, uint32_t stream, uint32_t node,
-                                                     uint32_t pos, uint32_t tag)
+// pure function of (seed, stream, node, position, tag), so simulation is order-free.  One block
+// gives four 32-bit words; rsg_philox returns the first.
+__host__ __device__ static inline void rsg_philox4(uint64_t seed, uint32_t stream, uint32_t node, uint32_t pos,
+                                                   uint32_t tag, uint32_t out[4])
 {
   uint32_t c0 = pos, c1 = node, c2 = stream, c3 = tag;
   uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
@@ -34,8 +35,18 @@ __host__ __device__ static inline uint32_t rsg_philox(uint64_t seed, uint32_t st
     c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
     k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
   }
-  return c0;
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
+__host__ __device__ static inline uint32_t rsg_philox(uint64_t seed, uint32_t stream, uint32_t node,
+                                                     uint32_t pos, uint32_t tag)
+{
+  uint32_t w[4];
+  rsg_philox4(seed, stream, node, pos, tag, w);
+  return w[0];
+}
+// Tag of the per-node draws of the GTR evolution: the block keyed (internal node v, column)
+// gives the left child's symbol draw, the right child's, and the two leaf-gap draws.
+#define RSG_TAG_CHILDREN 5u
 
 // Kinds of stored pair counts.
 //   CNT_U16: int8-Gram records, 16 uint16 counts (cell x*4+y), 32 bytes per record; for

@@ -207,32 +207,37 @@ k_root(const SimItem *__restrict__ items, int nitems, const double *__restrict__
   seqs[it.seq_off + col] = (uint8_t)a;
 }
 
-// One tree level: every internal node v at this depth draws its two children; block
-// (x, y) takes entry x of the level (ent[k] = {item, v, -, -}) and its columns y*256 ..
-// y*256 + 255 (blocks past the item's length return at once).
+// One tree level: every internal node v at this depth draws its two children.  Wave w of block
+// b takes entry 4 b + w of the level (ent[k] = {item, v, -, -}, ne entries) and walks the item's
+// columns in steps of 64 (one lane per column; every wave has work, and the node's item, table
+// and children are read once per wave).
 __global__ void __launch_bounds__(256)
-k_evolve_level(const SimItem *__restrict__ items, const int4 *__restrict__ ent,
+k_evolve_level(const SimItem *__restrict__ items, const int4 *__restrict__ ent, int64_t ne,
                const int *__restrict__ left, const int *__restrict__ right, const uint32_t *__restrict__ cdf,
                uint8_t *__restrict__ seqs, uint8_t *__restrict__ leaves)
 {
-  const int4 e = ent[blockIdx.x];
+  const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= ne) return;
+  const int lane = threadIdx.x & 63;
+  const int4 e = ent[k];
   const SimItem it = items[e.x];
-  const int64_t col = (int64_t)blockIdx.y * 256 + threadIdx.x;
-  if (col >= it.L) return;
-  const int64_t N = it.N, L = it.L, v = e.y;
+  const int64_t L = it.L, v = e.y;
   uint8_t *S = seqs + it.seq_off;
   const uint32_t *cd = cdf + (it.tree_off + v) * 24;
-  const int par = S[v * L + col];
+  const int dc[2] = {left[it.tree_off + v], right[it.tree_off + v]};
+  for (int64_t col = lane; col < L; col += 64) {
+    const int par = S[v * L + col];
+    uint32_t w[4];                                     // both children's draws, one block
+    rsg_philox4(it.seed, it.stream, (uint32_t)v, (uint32_t)col, RSG_TAG_CHILDREN, w);
 #pragma unroll
-  for (int side = 0; side < 2; side++) {
-    const int d = side ? right[it.tree_off + v] : left[it.tree_off + v];
-    const int64_t row = (d > 0) ? d : (N - 1 - d);
-    const uint32_t u = rsg_philox(it.seed, it.stream, (uint32_t)row, (uint32_t)col, 0u);
-    int sym = icdf3(u, cd + side * 12 + par * 3);
-    if (d > 0) S[(int64_t)d * L + col] = (uint8_t)sym;
-    else {
-      if (it.gapthr && rsg_philox(it.seed, it.stream, (uint32_t)row, (uint32_t)col, 1u) < it.gapthr) sym = 4;
-      leaves[it.leaf_off + (int64_t)(-d) * L + col] = (uint8_t)sym;
+    for (int side = 0; side < 2; side++) {
+      const int d = dc[side];
+      int sym = icdf3(w[side], cd + side * 12 + par * 3);
+      if (d > 0) S[(int64_t)d * L + col] = (uint8_t)sym;
+      else {
+        if (it.gapthr && w[2 + side] < it.gapthr) sym = 4;
+        leaves[it.leaf_off + (int64_t)(-d) * L + col] = (uint8_t)sym;
+      }
     }
   }
 }
@@ -403,8 +408,8 @@ int rsg_sim_launch(rsg_ctx *c, const SimItem *d_items, int nitems, int64_t nodes
   for (size_t l = 0; l < lv.lev_blocks.size(); l++) {
     const int64_t e0 = lv.lev_ent0[l], ne = lv.lev_ent0[l + 1] - e0;
     if (lv.lev_blocks[l] > 0)
-      hipLaunchKernelGGL(k_evolve_level, dim3((unsigned)ne, (unsigned)((lv.lev_maxl[l] + 255) / 256)), dim3(256), 0,
-                         c->stream, d_items, d_ent + e0, d_left, d_right, d_cdf, d_seqs, d_leaves);
+      hipLaunchKernelGGL(k_evolve_level, dim3((unsigned)((ne + 3) / 4)), dim3(256), 0,
+                         c->stream, d_items, d_ent + e0, ne, d_left, d_right, d_cdf, d_seqs, d_leaves);
   }
   return hipGetLastError() == hipSuccess ? RSG_OK : RSG_FAIL;
 }
@@ -498,7 +503,8 @@ int rsg_sim_one(rsg_ctx *c, const rsg_null_job *job, const uint8_t *root, uint8_
 // (cov_addpair, :774-798).  Unpaired columns evolve under the 4-state GTR model as before.
 // Device order: the 4-state level kernel draws every column of a level's children, then the
 // pair kernel overwrites both columns of every pair; draws are Philox keyed by (seed, stream,
-// child row, 5' column, tag 2), leaf gaps reuse the tag-1 draws of the 4-state path.
+// child row, 5' column, tag 2), leaf gaps reuse the leaf-gap words of the 4-state path's
+// per-node blocks (internal node, column, RSG_TAG_CHILDREN).
 
 // Eigendecomposition of a symmetric n x n (cyclic Jacobi; A, V row-major n x n).
 static void jacobi_n(int n, double *A, double *V, double *lam)
@@ -639,8 +645,15 @@ k_pair_level(const int *__restrict__ nodes, const int2 *__restrict__ pairs, int 
     int x = st >> 2, y = st & 3;
     if (d > 0) { S[(int64_t)d * L + pr.x] = (uint8_t)x; S[(int64_t)d * L + pr.y] = (uint8_t)y; }
     else {
-      if (gapthr && rsg_philox(seed, stream, (uint32_t)row, (uint32_t)pr.x, 1u) < gapthr) x = 4;
-      if (gapthr && rsg_philox(seed, stream, (uint32_t)row, (uint32_t)pr.y, 1u) < gapthr) y = 4;
+      // the leaf-gap draws of the unpaired evolution of these columns, so gaps do not depend
+      // on the pairing
+      if (gapthr) {
+        uint32_t wx[4], wy[4];
+        rsg_philox4(seed, stream, (uint32_t)v, (uint32_t)pr.x, RSG_TAG_CHILDREN, wx);
+        rsg_philox4(seed, stream, (uint32_t)v, (uint32_t)pr.y, RSG_TAG_CHILDREN, wy);
+        if (wx[2 + side] < gapthr) x = 4;
+        if (wy[2 + side] < gapthr) y = 4;
+      }
       leaves[(int64_t)(-d) * L + pr.x] = (uint8_t)x;
       leaves[(int64_t)(-d) * L + pr.y] = (uint8_t)y;
     }
@@ -707,8 +720,8 @@ int rsg_sim_pairs(rsg_ctx *c, const rsg_null_job *job, const int *ct, const doub
   for (size_t l = 0; l < lv.lev_blocks.size(); l++) {
     const int64_t e0 = lv.lev_ent0[l], ne = lv.lev_ent0[l + 1] - e0;
     if (lv.lev_blocks[l] > 0)
-      hipLaunchKernelGGL(k_evolve_level, dim3((unsigned)ne, (unsigned)((lv.lev_maxl[l] + 255) / 256)), dim3(256), 0,
-                         c->stream, d_items, d_ent + e0, d_left, d_right, d_cdf, d_seqs, d_leaves);
+      hipLaunchKernelGGL(k_evolve_level, dim3((unsigned)((ne + 3) / 4)), dim3(256), 0,
+                         c->stream, d_items, d_ent + e0, ne, d_left, d_right, d_cdf, d_seqs, d_leaves);
     if (np && ne > 0)
       hipLaunchKernelGGL(k_pair_level, dim3((unsigned)((np + 63) / 64), (unsigned)ne), dim3(64), 0, c->stream,
                          d_nodes + e0, d_pairs, (int)np, d_left, d_right, d_tab, N, L, it.seed, it.stream, it.gapthr,
