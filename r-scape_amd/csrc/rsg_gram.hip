@@ -1000,10 +1000,17 @@ __global__ void __launch_bounds__(1024) k_quant_weights(const double *__restrict
 // symbols for the three limbs (12 accumulators of 16 int32 per lane).  Staging as in the int8
 // Gram (three LDS images, LDS-direct loads two stages ahead), each image followed by the
 // stage's limb bytes.
-template <int KT>
+// K-tiles [kt0, kt1).  With SPLIT (a split-K segment) the combined limb sums
+// g = G_0 + 2^7 G_1 + 2^14 G_2 of every cell of every pair of the tile go, unconditionally, to
+// the int64 partial record jj * 32 + ii (16 cells) for k_gram_w8_fix, and no record or nseff is
+// written.
+// (SPLIT: the partial-sum epilogue, a template argument so that each variant of the loop gets its
+// own register allocation -- with both epilogues after one loop the kernel spilled.)
+template <int KT, bool SPLIT>
 __device__ __forceinline__ void gram_tile_w8(const uint8_t *__restrict__ xt, const uint8_t *__restrict__ wq, double W,
                                              int64_t Lpad, int64_t L, float *__restrict__ out, double *__restrict__ nsx,
-                                             int64_t i0, int64_t j0, uint8_t *lds, int64_t nkt)
+                                             int64_t i0, int64_t j0, uint8_t *lds, int64_t kt0, int64_t kt1,
+                                             int64_t *__restrict__ part)
 {
 #pragma clang fp contract(off)
   constexpr int NT = 256, KS = KT / 32, NW = 4;
@@ -1035,8 +1042,8 @@ __device__ __forceinline__ void gram_tile_w8(const uint8_t *__restrict__ xt, con
     if (wave == 0) __builtin_amdgcn_s_waitcnt(0x0F70 | (QW + 1));
     else           __builtin_amdgcn_s_waitcnt(0x0F70 | QW);
   };
-  const int nit = (int)nkt;
-  auto stage_of = [&](int it) -> int64_t { return (it < nit) ? it : nit - 1; };
+  const int nit = (int)(kt1 - kt0);
+  auto stage_of = [&](int it) -> int64_t { return kt0 + ((it < nit) ? it : nit - 1); };
   stage(stage_of(0), lds);
   stage(stage_of(1), lds + IS);
   wait_stage();
@@ -1087,6 +1094,22 @@ __device__ __forceinline__ void gram_tile_w8(const uint8_t *__restrict__ xt, con
   }
   __builtin_amdgcn_s_waitcnt(0x0F70);             // no LDS-direct load outlives the loop
 
+  if constexpr (SPLIT) {                           // a split-K segment: the exact partial sums
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int jj = 8 * (r >> 2) + 4 * khalf + (r & 3);
+      int64_t g[4];
+#pragma unroll
+      for (int y = 0; y < 4; y++)
+        g[y] = (int64_t)((unsigned)acc[0][y][r] >> y) + 128 * (int64_t)((unsigned)acc[1][y][r] >> y) +
+               16384 * (int64_t)((unsigned)acc[2][y][r] >> y);
+      longlong2 *p = (longlong2 *)(part + ((int64_t)jj * 32 + col) * 16 + x * 4);
+      p[0] = make_longlong2(g[0], g[1]);
+      p[1] = make_longlong2(g[2], g[3]);
+    }
+    return;
+  }
+
   // epilogue: lane owns i column `col`; register r holds j = 8*(r/4) + 4*khalf + r%4; cells
   // x*4 .. x*4+3 of each record are this wave's.  nseff of every pair comes from the exact
   // integer sums (the FP32 cells each carry a rounding, and ngap = total - nseff cancels most of
@@ -1122,19 +1145,76 @@ __device__ __forceinline__ void gram_tile_w8(const uint8_t *__restrict__ xt, con
   }
 }
 
+// Grid: 8 * per_xcd work groups for the first ndp tiles (whole K range, neighbouring tiles on one
+// XCD), then the split units of the last tiles (K segments as in k_gram_i8: tile ndp + t takes
+// sbase + 1 segments if t < sextra, else sbase), which run in the slots the full tiles free.
 template <int KT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_gram_w8(const uint8_t *__restrict__ xt, const uint8_t *__restrict__ wq, const double *__restrict__ wscale, int64_t Npad,
           int64_t Lpad, int64_t L, float *__restrict__ out, double *__restrict__ nsx, const int2 *__restrict__ tiles,
-          int ntiles, int per_xcd)
+          int ndp, int per_xcd, int ntiles, int sbase, int sextra, int64_t *__restrict__ part)
 {
   typedef CodeStage<32, 32, KT, 256> St;
   __shared__ __attribute__((aligned(16))) uint8_t lds[3 * (St::BYTES + St::NKC * 64)];
+  const int64_t nkt = Npad / KT;
   const int g = blockIdx.x;
-  const int t = (g & 7) * per_xcd + (g >> 3);            // neighbouring tiles on one XCD
-  if (t >= ntiles) return;
+  int t;
+  int64_t k0 = 0, k1 = nkt;
+  int64_t *p = nullptr;
+  if (g < 8 * per_xcd) {
+    t = (g & 7) * per_xcd + (g >> 3);
+    if (t >= ndp) return;
+  } else {
+    const int u = g - 8 * per_xcd, nsu = (ntiles - ndp) * sbase + sextra;
+    if (u >= nsu) return;
+    int tr, sg, ns;
+    split_unit(u, sbase, sextra, tr, sg, ns);
+    t = ndp + tr;
+    k0 = nkt * sg / ns; k1 = nkt * (sg + 1) / ns;
+    p = part + (int64_t)u * 1024 * 16;
+  }
   const int2 tl = tiles[t];
-  gram_tile_w8<KT>(xt, wq, *wscale, Lpad, L, out, nsx, (int64_t)tl.x * 32, (int64_t)tl.y * 32, lds, Npad / KT);
+  if (p) gram_tile_w8<KT, true>(xt, wq, *wscale, Lpad, L, out, nsx, (int64_t)tl.x * 32, (int64_t)tl.y * 32, lds, k0, k1, p);
+  else   gram_tile_w8<KT, false>(xt, wq, *wscale, Lpad, L, out, nsx, (int64_t)tl.x * 32, (int64_t)tl.y * 32, lds, k0, k1, p);
+}
+
+// Split tiles: one work group (16 waves) per tile adds the tile's int64 partial sums over its K
+// segments (exact) and writes the records and nseff with the arithmetic of the gram_tile_w8
+// epilogue (one FP64 product and one FP32 rounding per cell; nseff from the exact total), so
+// they equal the unsplit kernel's bit for bit.  Wave w takes row r = w of the MFMA layout.
+__global__ void __launch_bounds__(1024)
+k_gram_w8_fix(const int64_t *__restrict__ part, const int2 *__restrict__ tiles, int ndp, int sbase, int sextra,
+              const double *__restrict__ wscale, int64_t L, float *__restrict__ out, double *__restrict__ nsx)
+{
+#pragma clang fp contract(off)
+  const int tr = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31, khalf = lane >> 5;
+  const int2 tl = tiles[ndp + tr];
+  const int ns = sbase + (tr < sextra ? 1 : 0);
+  const double W = *wscale;
+  const int r = wave, jj = 8 * (r >> 2) + 4 * khalf + (r & 3);
+  const int64_t i = (int64_t)tl.x * 32 + col, j = (int64_t)tl.y * 32 + jj;
+  if (!(i < j && j < L)) return;
+  const longlong2 *p0 = (const longlong2 *)(part + (int64_t)split_first(tr, sbase, sextra) * 1024 * 16 + ((int64_t)jj * 32 + col) * 16);
+  int64_t g[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) g[c] = 0;
+  for (int k = 0; k < ns; k++) {
+    const longlong2 *q = p0 + (int64_t)k * 1024 * 8;
+#pragma unroll
+    for (int h = 0; h < 8; h++) { const longlong2 v = q[h]; g[2 * h] += v.x; g[2 * h + 1] += v.y; }
+  }
+  int64_t tot = 0;
+  float *o = out + rsg_rec(i, j, L) * 16;
+#pragma unroll
+  for (int x = 0; x < 4; x++) {
+    int64_t sx = 0;
+#pragma unroll
+    for (int y = 0; y < 4; y++) sx += g[x * 4 + y];
+    tot += sx;
+    *(float4 *)(o + x * 4) = make_float4((float)((double)g[x * 4] * W), (float)((double)g[x * 4 + 1] * W),
+                                         (float)((double)g[x * 4 + 2] * W), (float)((double)g[x * 4 + 3] * W));
+  }
+  nsx[rsg_rec(i, j, L)] = (double)tot * W;
 }
 
 int rsg_gram_counts_w8(rsg_ctx *c, rsg_msa *m, float *out, char *errbuf)
@@ -1158,10 +1238,38 @@ int rsg_gram_counts_w8(rsg_ctx *c, rsg_msa *m, float *out, char *errbuf)
   else tiles = tile_list(c->device, m->L, 32, 32, &ntiles, errbuf);
   if (!tiles) return RSG_FAIL;
   if (ntiles == 0) return RSG_OK;
-  const int per = (ntiles + 7) / 8;
+  // the tail as in launch_i8: a CU holds two of these 4-wave work groups; when the last wave of
+  // tiles is at most half full its tiles are split along K, one segment per CU, after the full
+  // tiles, and k_gram_w8_fix adds the exact partial sums
+  const int ncu = c->ncu > 0 ? c->ncu : 256, slots = 2 * ncu;
+  const int64_t nkt = m->Npad / KT;
+  int ndp = ntiles, sbase = 1, sextra = 0;
+  const int rem = ntiles % slots;
+  if (c->gram_split && ntiles < 4 * slots && rem > 0 && rem <= slots / 2 && nkt >= 2) {
+    ndp = ntiles - rem;
+    const int nsu = (int)std::min<int64_t>((int64_t)ncu, (int64_t)rem * nkt);
+    sbase = nsu / rem; sextra = nsu % rem;
+  }
+  const int nrem = ntiles - ndp, nsu = nrem * sbase + sextra;
+  int64_t *part = nullptr;
+  if (nrem > 0) {
+    const size_t need = (size_t)nsu * 1024 * 16 * sizeof(int64_t);
+    if (c->gscr_cap < need && c->no_alloc) return rsg_fail(errbuf, "gram: split-K partials need an allocation");
+    if (c->gscr_cap < need) {
+      if (c->d_gscr) hipFree(c->d_gscr);
+      c->d_gscr = nullptr; c->gscr_cap = 0;
+      HIPCHK(hipMalloc(&c->d_gscr, need));
+      c->gscr_cap = need;
+    }
+    part = (int64_t *)c->d_gscr;
+  }
+  const int per = (ndp + 7) / 8;
   int t; rsg_tstart(c, "gram_w8", &t);
-  hipLaunchKernelGGL((k_gram_w8<KT>), dim3((unsigned)(8 * per)), dim3(256), 0, c->stream, m->d_xt, m->d_wq, d_scale, m->Npad,
-                     m->Lpad, m->L, out, m->d_nsx, tiles, ntiles, per);
+  hipLaunchKernelGGL((k_gram_w8<KT>), dim3((unsigned)(8 * per + nsu)), dim3(256), 0, c->stream, m->d_xt, m->d_wq, d_scale,
+                     m->Npad, m->Lpad, m->L, out, m->d_nsx, tiles, ndp, per, ntiles, sbase, sextra, part);
+  if (nrem > 0)
+    hipLaunchKernelGGL(k_gram_w8_fix, dim3((unsigned)nrem), dim3(1024), 0, c->stream, part, tiles, ndp, sbase, sextra,
+                       d_scale, m->L, out, m->d_nsx);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   return RSG_OK;

@@ -296,6 +296,33 @@ def test_weighted_counts_int8_limbs(ctx, spread):
         assert e < 1e-5, name
 
 
+def test_weighted_counts_int8_limbs_split_matches_unsplit(ctx):
+    """COUNTS_W8 with the tiles of the last partial wave split along K (the exact int64 partial
+    sums added by k_gram_w8_fix) equals the unsplit kernel bit for bit: the records and the exact
+    nseff (seen through MIg).  L = 1100 gives 630 tiles of 32 x 32 columns, 118 past the 512
+    work-group slots of 256 CUs, so the tail is split."""
+    N, L = 300, 1100
+    print("w8 split N=%d L=%d" % (N, L), flush=True)
+    rng = np.random.default_rng(73)
+    ax = _msa(N, L, 73)
+    w = rng.uniform(0.05, 3.0, size=N)
+    m = ctx.msa(ax, w)
+    m.probs(R.COUNTS_W8)
+    a = m.counts()
+    m.cov("MIg", R.C16, prec=R.STAT_FP64)
+    ga = m.get_cov()[0]
+    ctx.set_gram_split(False)
+    try:
+        m.probs(R.COUNTS_W8)
+        b = m.counts()
+        m.cov("MIg", R.C16, prec=R.STAT_FP64)
+        gb = m.get_cov()[0]
+    finally:
+        ctx.set_gram_split(True)
+    assert np.array_equal(a, b)
+    assert np.array_equal(np.nan_to_num(ga), np.nan_to_num(gb))
+
+
 PIPE = [("GTp", oracle.GT, R.APC), ("GT", oracle.GT, None), ("GTa", oracle.GT, R.ASC),
         ("MIp", oracle.MI, R.APC), ("CCFp", oracle.CCF, R.APC), ("RAF", oracle.RAF, None),
         ("RAFS", oracle.RAFS, None), ("CHIa", oracle.CHI, R.ASC)]
