@@ -146,7 +146,7 @@ def uneven_null_batch(ctx, reps=3):
     return out
 
 
-def evalue_runs(ctx, d, K=20, reps=3, acc_nulls=16):
+def evalue_runs(ctx, d, K=20, reps=3, acc_nulls=64):
     """Complete significance runs of the alignment (GTp scoring + null + exponential tail fit +
     E-values + the list of pairs with E < 0.05), three ways: the plain per-replicate GPU path
     (simulate one replicate, score it, bin it, merge on the host: R-scape's loop with the GPU
@@ -211,7 +211,7 @@ def evalue_runs(ctx, d, K=20, reps=3, acc_nulls=16):
 
     def surrogate(matrix=False):
         m.calculate("GTp")
-        mu, lam = sur.predict(NL.features(ctx, m, tree))
+        mu, lam = sur.predict(S.inputs(ctx, m, tree, L, pi, gap, seed=7))
         if not np.isfinite(mu[0]):
             fallback[0] = True
             return batched(matrix)
@@ -296,6 +296,129 @@ def evalue_runs(ctx, d, K=20, reps=3, acc_nulls=16):
     return out
 
 
+def c4_significance(ctx, K=100, R_nulls=32, reps=3):
+    """Config 4 end to end: the N=20000 x L=5000 alignment (1200 nested + 40 pseudoknotted pairs,
+    15% gaps), position-based weights, weighted GT+APC scoring (int8-limb counts, FP32
+    statistics), then its significance three ways -- the plain per-replicate GPU path (R-scape's
+    loop), one batch of K replicates, and the surrogate -- each a complete run: scoring, null,
+    fit, E-values, hit list (E < 0.05).  The nulls score unit-weight GTp on simulated alignments
+    of the same size down the alignment's tree.  E-value accuracy is measured against the mean
+    log10 E over R independent K-replicate full nulls (a full null's E-value is itself random),
+    with one full null's distance from the mean of the others beside it."""
+    from rscape_amd import null as NL
+    from rscape_amd import surrogate as S
+    tstart = t0 = time.perf_counter()
+    d = synth.config(4, seed=42)
+    ax, tree = d["ax"], d["tree"]
+    N, L = ax.shape
+    Nc = L * (L - 1) // 2
+    out = {"workload": "N=%d x L=%d, PB weights, GTp (W8 counts, FP32 statistic), %d-replicate null" % (N, L, K),
+           "generate_s": time.perf_counter() - t0}
+    say = lambda msg: print("[c4] %s (%.1f s)" % (msg, time.perf_counter() - tstart), file=sys.stderr, flush=True)
+    say("alignment generated")
+    w = ctx.weights(ax, "PB")
+    m = ctx.msa(ax, w)
+    kw = dict(covtype="GTp", counts=R.COUNTS_W8, prec=R.STAT_FP32)
+    m.calculate(**kw)
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        m.calculate(**kw)
+        ts.append(time.perf_counter() - t0)
+    say("weighted scoring timed")
+    out["weighted_scoring_ms"] = float(np.median(ts)) * 1e3
+    out["weighted_scoring_updates_per_s"] = N * L * (L - 1) / 2 / float(np.median(ts))
+    pi, gap = S.msa_null_params(ax)
+    model = NL.gtr_model(pi)
+    sur = S.Surrogate.load(ctx) if os.path.exists(S.DATA) else None
+
+    def plain():
+        m.calculate(**kw)
+        cum, wd = None, None
+        for r in range(K):
+            mr = ctx.msa(NL.simulate(ctx, NL.Job(tree, L, model, seed=7, stream=r, gapfrac=gap)))
+            a, b = mr.calculate("GTp")
+            if wd is None:
+                wd = R.width(b, a)
+            cum = R.Histogram.null_add(cum, mr.rank_hist(-10.0, wd))
+            mr.close()
+        _, sf = cum.fit(ctx, 5e-4, 1.0, True)
+        return m.evalues(cum, sf, Nc, matrix=True)
+
+    def batched():
+        m.calculate(**kw)
+        h, sf, _ = S.full_null(ctx, tree, L, pi, gap, K=K, seed=7)
+        return m.evalues(h, sf, Nc, matrix=True)
+
+    def surrogate():
+        m.calculate(**kw)
+        mu, lam = sur.predict(S.inputs(ctx, m, tree, L, pi, gap, seed=7))
+        if not np.isfinite(mu[0]):
+            return batched(), True
+        xcap = sur.cap(float(mu[0]), float(lam[0]), K, Nc)
+        return NL.evalues_tail(ctx, m, float(mu[0]), float(lam[0]), S.P0, Nc, matrix=True, xcap=xcap), False
+
+    def timed(fn, n=reps):
+        r = fn()                                 # the first call allocates the buffers
+        ts = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            r = fn()
+            ts.append(time.perf_counter() - t0)
+        return r, float(np.median(ts))
+
+    iu = np.triu_indices(L, 1)
+    lg = lambda e: np.log10(np.maximum(e, 1e-300)).astype(np.float32)
+    eb, tb = timed(batched)
+    say("batched null timed")
+    ep, tp = timed(plain, 1)
+    say("plain per-replicate null timed")
+    out["plain_per_replicate_s"] = tp
+    out["batched_null_s"] = tb
+    out["speedup_batched_vs_plain"] = tp / tb
+    eb, ep = lg(eb[iu]), lg(ep[iu])
+    out["plain_vs_batched_max_abs_log10E"] = float(np.max(np.abs(ep - eb)))
+    del ep
+    if sur is not None:
+        (es, fb), tsur = timed(surrogate)
+        say("surrogate timed")
+        out["surrogate_s"] = tsur
+        out["surrogate_fallback"] = fb
+        out["speedup_surrogate_vs_plain"] = tp / tsur
+        out["speedup_surrogate_vs_batched"] = tb / tsur
+        es = lg(es[iu])
+    # accuracy reference: R independent full nulls (other seeds than the timed ones)
+    lgs = np.empty((R_nulls, len(iu[0])), np.float32)
+    m.calculate(**kw)
+    for r in range(R_nulls):
+        h, sf, _ = S.full_null(ctx, tree, L, pi, gap, K=K, seed=1000 + r)
+        lgs[r] = lg(m.evalues(h, sf, Nc, matrix=True)[iu])
+    say("%d reference nulls" % R_nulls)
+    ref = lgs.mean(0)
+    sig = ref < math.log10(0.05)
+    out["sig_pairs"] = int(sig.sum())
+    planted = np.zeros((L, L), bool)            # ct holds the nested and the pseudoknotted pairs
+    for i, j in enumerate(d["ct"]):
+        if j > i:
+            planted[i, j] = True
+    out["planted_pairs"] = int(planted.sum())
+    out["planted_called"] = int((planted[iu] & sig).sum())
+
+    def err(v):
+        dd = np.abs(v[sig] - ref[sig]).astype(np.float64)
+        return {"median": float(np.median(dd)), "p90": float(np.percentile(dd, 90)),
+                "frac_within_0p3": float(np.mean(dd <= 0.3))} if sig.any() else None
+    loo = np.abs(lgs[:, sig] - (lgs[:, sig].sum(0)[None] - lgs[:, sig]) / (R_nulls - 1))
+    out["accuracy_reference"] = "mean log10 E over %d independent %d-replicate full nulls" % (R_nulls, K)
+    out["one_full_null_log10E"] = {"median": float(np.median(loo)), "frac_within_0p3": float(np.mean(loo <= 0.3))}
+    out["batched_null_log10E"] = err(eb)
+    if sur is not None:
+        out["surrogate_log10E"] = err(es)
+        out["call_agreement"] = float(np.mean((es < math.log10(0.05)) == sig))
+    m.close()
+    return out
+
+
 def fp32_errors(ctx, ax):
     """Max relative error of each FP32 statistic against the FP64 path on the same counts
     (floored at 1e-3 of the largest |score|, as the parity tests do)."""
@@ -359,6 +482,8 @@ def main():
     ap.add_argument("--stat", choices=("fp32", "fp64"), default="fp32",
                     help="arithmetic of the per-pair statistics in the timed step")
     ap.add_argument("--no-batch", action="store_true", help="skip the 1000-alignment null batch")
+    ap.add_argument("--c4", action="store_true",
+                    help="only the N=20000 x L=5000 significance run (config 4: weighted scoring, 100-replicate null, surrogate)")
     ap.add_argument("--null-batch-only", action="store_true",
                     help="time only the 1000-alignment null batch and print its JSON")
     args = ap.parse_args()
@@ -369,6 +494,11 @@ def main():
     if args.null_batch_only:
         ctx = R.Context(0)
         print(json.dumps({"uneven_null_batch": uneven_null_batch(ctx)}), flush=True)
+        ctx.close()
+        return
+    if args.c4:
+        ctx = R.Context(0)
+        print(json.dumps({"config4_significance": c4_significance(ctx)}), flush=True)
         ctx.close()
         return
 

@@ -361,7 +361,7 @@ int rsg_cov_GenerateAlignment(rsg_ctx *ctx, const rsg_null_job *job, const uint8
  * (calculate_width_histo), every replicate's pairs accumulate into one histogram
  * (null_add2cumranklist) and one FP64 tail fit is made per group.  Outputs per group g:
  * fits[g*8 + {0..7}] = {mu, lambda, tau, newmass, pmass, w, minCOV, maxCOV}; hists (may be
- * NULL) receives each group's histogram; feats (may be NULL) receives 48 surrogate features
+ * NULL) receives each group's histogram; feats (may be NULL) receives RSG_NULL_NFEAT surrogate features
  * per group (rsg_null_features of its first replicate).  use_graph = k >= 1 records the whole batch in one
  * hipGraph and launches it k times (each launch recomputes everything); 0 runs eagerly.  The
  * instantiated graph stays with the context: a later call whose launch geometry is the same
@@ -385,9 +385,11 @@ int rsg_null_batch_fitch(rsg_ctx *ctx, const uint8_t *ax, int64_t nseq, int64_t 
                          int doexpfit, rsg_hist **hist, double *fit, int use_graph, char *errbuf);
 
 /* ---- learned null-tail surrogate (no reference counterpart) ---- */
-/* The surrogate's 48 inputs of an alignment and its tree (column statistics on the device,
- * tree moments on the host): ln N, ln L, ln total branch length, 16-bin column-entropy and
- * 16-bin gap histograms, A/C/G composition, 10 tree-shape moments. */
+/* The surrogate's RSG_NULL_NFEAT inputs of an alignment and its tree (column statistics on the
+ * device, tree statistics on the host): ln N, ln L, ln total branch length, 16-bin
+ * column-entropy and 16-bin gap histograms, A/C/G composition, 10 tree-shape moments, and the
+ * split spectrum (branch lengths by the minor clade share of their split, 8 bins, two sums). */
+#define RSG_NULL_NFEAT 64
 int  rsg_null_features(rsg_ctx *ctx, rsg_msa *msa, const int *left, const int *right,
                        const double *ld, const double *rd, float *feat, char *errbuf);
 /* E-values from a parametric tail P(score > x) = p0 exp(-lambda (x - m)) (the surrogate's

@@ -679,7 +679,7 @@ static int null_batch_impl(rsg_ctx *c, const rsg_null_job *jobs, int nj, int cov
     HIPCHK(hipMemcpy(cnt.data(), d_fcnt, sizeof(unsigned long long) * cnt.size(), hipMemcpyDeviceToHost));
     for (int g = 0; g < ng; g++) {
       const rsg_null_job &J = jobs[gfirst[(size_t)g]];
-      rsg_feat_finish(&cnt[(size_t)g * 36], J.nleaf, J.alen, J.left, J.right, J.ld, J.rd, feats + (size_t)g * 48);
+      rsg_feat_finish(&cnt[(size_t)g * 36], J.nleaf, J.alen, J.left, J.right, J.ld, J.rd, feats + (size_t)g * RSG_NULL_NFEAT);
     }
   }
   if (hists) {

@@ -1,4 +1,4 @@
-// rsg_features.hip -- the 48 alignment + tree features the null-tail surrogate reads.
+// rsg_features.hip -- the RSG_NULL_NFEAT (64) alignment + tree features the null-tail surrogate reads.
 // No reference counterpart.
 //
 //   f[0]  ln N            f[1]  ln L            f[2]  ln(total branch length)
@@ -11,6 +11,14 @@
 //              root-to-leaf depth; depth CV; max depth / mean depth; mean terminal branch /
 //              mean branch; mean internal branch / mean branch; mean root-to-leaf edge count /
 //              log2 N; cherries / (N/2); Colless imbalance / ((N-1)(N-2)/2)
+//   f[48..63]  split spectrum: every branch splits the leaves into its clade (n leaves) and the
+//              rest; with the minor side's share q = min(n, N - n) / N in bin k = 0..7
+//              (q in (1/4, 1/2], (1/8, 1/4], ..., (1/256, 1/128], then <= 1/256),
+//              f[48 + k] = ln(1e-3 + sum of those branches' lengths t) and
+//              f[56 + k] = ln(1e-4 + sum of (1 - e^-t)^2).  Two independent columns look
+//              covarying when both change on one branch (probability ~ (1 - e^-t)^2), and how
+//              far the score goes grows with the split's minor share; the tail of the null
+//              follows the long branches above large clades, which the moments above miss.
 //
 // The column statistics are one pass over the one-hot Gram operand on the device (one wave
 // per column; block histograms in LDS, one atomic per bin per block); the tree moments are
@@ -141,6 +149,18 @@ void rsg_feat_finish(const unsigned long long *cnt, int64_t N, int64_t L, const 
     nleaf[(size_t)v] = a + b;
     colless += (double)(a > b ? a - b : b - a);
   }
+  // split spectrum (nleaf is complete: a node's count follows its children's above)
+  double s1[8] = {0., 0., 0., 0., 0., 0., 0., 0.}, s2[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  for (int64_t v = 0; v < m; v++)
+    for (int side = 0; side < 2; side++) {
+      const int d = side ? right[v] : left[v];
+      const double t = side ? rd[v] : ld[v];
+      const int64_t n = d > 0 ? nleaf[(size_t)d] : 1, nmin = n < N - n ? n : N - n;
+      int k = 0;
+      while (k < 7 && (nmin << (k + 2)) <= N) k++;
+      const double e = 1. - exp(-t);
+      s1[k] += t; s2[k] += e * e;
+    }
   const double Nd = (double)N, md = sd / Nd;
   const double sdd = sqrt(fmax(0., sd2 / Nd - md * md));
   f[0] = (float)log(Nd); f[1] = (float)log((double)L); f[2] = (float)log(fmax(T, 1e-300));
@@ -157,6 +177,7 @@ void rsg_feat_finish(const unsigned long long *cnt, int64_t N, int64_t L, const 
   f[45] = (float)((se / Nd) / log2(Nd));
   f[46] = (float)((double)cherries / (Nd / 2.));
   f[47] = (float)(N > 2 ? colless / ((Nd - 1.) * (Nd - 2.) / 2.) : 0.);
+  for (int k = 0; k < 8; k++) { f[48 + k] = (float)log(1e-3 + s1[k]); f[56 + k] = (float)log(1e-4 + s2[k]); }
 }
 
 extern "C" int rsg_null_features(rsg_ctx *c, rsg_msa *msa, const int *left, const int *right, const double *ld,

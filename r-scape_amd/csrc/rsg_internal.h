@@ -323,7 +323,7 @@ int rsg_fitch_ancestral(rsg_ctx *c, const FitchPlan &P, const FitchDev &D, int R
 #define RSG_BDESC 20
 // Surrogate features (rsg_features.hip): column statistics of items {one-hot panel offset, N,
 // Npad, L, first 4-column block} into 36 uint64 counters per item; the host turns them into
-// 48 floats.
+// RSG_NULL_NFEAT floats.
 int rsg_feat_launch(rsg_ctx *c, const int64_t *d_items, int nitems, int64_t nblocks, const uint8_t *d_base,
                     unsigned long long *d_out);
 void rsg_feat_finish(const unsigned long long *cnt, int64_t N, int64_t L, const int *left, const int *right,

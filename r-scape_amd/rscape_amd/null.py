@@ -245,12 +245,15 @@ def batch_fitch(ctx, ax, tree, seeds, covtype="GTp", bmin=-10.0, pmass=5e-4, fra
     return fit, (Histogram(hp[0]) if hp[0] else None)
 
 
+NFEAT = 64          # RSG_NULL_NFEAT: the surrogate's inputs (rsg_null_features)
+
+
 def batch(ctx, jobs, covtype="GTp", bmin=-10.0, pmass=5e-4, fracfit=1.0, expo=True, graph=0, hists=False,
           feats=False):
     """Simulate + score + fit a batch on the device.  Jobs with group=None are nulls of one
     replicate each; jobs sharing a group id are the replicates of one null.  Returns (fits
     array (groups, 8), histograms); fits columns: mu, lambda, tau, newmass, pmass, w,
-    minCOV, maxCOV.  feats=True also returns the (groups, 48) surrogate features."""
+    minCOV, maxCOV.  feats=True also returns the (groups, NFEAT) surrogate features."""
     from . import Hist, Histogram
     L = _declare()
     arr = job_array(jobs)
@@ -258,7 +261,7 @@ def batch(ctx, jobs, covtype="GTp", bmin=-10.0, pmass=5e-4, fracfit=1.0, expo=Tr
     ng = int(arr["group"].max()) + 1
     fits = np.zeros(ng * 8)
     hp = (C.POINTER(Hist) * ng)() if hists else None
-    fe = np.zeros((ng, 48), np.float32) if feats else None
+    fe = np.zeros((ng, NFEAT), np.float32) if feats else None
     err = _err()
     t = COV[covtype] if isinstance(covtype, str) else covtype
     _check(L.rsg_null_batch(ctx.h, _p(arr), n, t, bmin, pmass, fracfit, 1 if expo else 0,
@@ -271,11 +274,11 @@ def batch(ctx, jobs, covtype="GTp", bmin=-10.0, pmass=5e-4, fracfit=1.0, expo=Tr
 
 
 def features(ctx, msa, tree):
-    """The surrogate's 48 inputs of a resident alignment and its tree."""
+    """The surrogate's NFEAT inputs of a resident alignment and its tree."""
     L = _declare()
     t = {k: (np.ascontiguousarray(v, dtype=np.int32) if k in ("left", "right") else
              np.ascontiguousarray(v, dtype=np.float64)) for k, v in tree.items() if k != "n"}
-    f = np.zeros(48, np.float32)
+    f = np.zeros(NFEAT, np.float32)
     err = _err()
     _check(L.rsg_null_features(ctx.h, msa.h, _p(t["left"]), _p(t["right"]), _p(t["ld"]), _p(t["rd"]), _p(f), err),
            err, "rsg_null_features")

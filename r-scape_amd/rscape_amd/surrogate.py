@@ -1,16 +1,25 @@
 """Learned null-tail surrogate.
 
-A small MLP (48 -> 128 -> 128 -> 128 -> 2, GELU; rsg_mlp_* kernels) maps the features of an
-alignment and its tree (rsg_null_features) to the exponential tail of its phylogenetic null,
-written as P(score > x) = P0 * exp(-lambda (x - m)).  The network outputs (lambda m, ln lambda):
-an error in lambda m moves log E by the same amount at every score, where an error in m
-would be weighted by lambda, and lambda ranges over more than an order of magnitude.  It is
-trained on tail fits that the batched GPU null simulator produces (rsg_null_batch with
-replicate groups and features), and falls back to the full simulated null when an
-alignment's features leave the training range.
+A small MLP (67 -> 128 -> 128 -> 128 -> 2, GELU; rsg_mlp_* kernels) maps the features of an
+alignment and its tree (rsg_null_features: 64) and a calibration null (3) to the exponential
+tail of its phylogenetic null, written as P(score > x) = P0 * exp(-lambda (x - m)).  The
+network outputs (lambda m, ln lambda): an error in lambda m moves log E by the same amount at
+every score, where an error in m would be weighted by lambda, and lambda ranges over more than
+an order of magnitude.  It is trained on tail fits that the batched GPU null simulator
+produces (rsg_null_batch with replicate groups and features), and falls back to the full
+simulated null when an alignment's inputs leave the training range.
+
+The calibration null is a few replicates (cal_replicates: about 1e6 null pairs, at most 4)
+fitted with a deeper mass (5e-3).  Past the last survival bin an E-value extrapolates the tail
+by about 30 e-folds, so lambda has to be right to a few percent; two Yule trees of the same
+size and mean branch length give nulls whose lambda differs by more than that (the long
+branches above large clades set it), and the features of the tree alone do not pin it.  The
+calibration fit sees the alignment's own tree; the network maps it, with the features, to
+the tail of the full null.
 
 Training data: random setups with N log-uniform in [20, 6000], L log-uniform in [50, 3500]
-(a share `big` of them drawn from N in [1000, 6000], L in [300, 3500] instead), Yule trees
+(a share `big` of them drawn from N in [1000, 6000], L in [300, 3500] instead, and a share `xl`
+from N in [3000, 24000], L in [1500, 6000], the sizes of large rRNA alignments), Yule trees
 with mean branch length U(0.05, 0.3), random base composition and gap fraction.  Each setup
 is simulated as a group of K replicates, K = clamp(ceil(T / pairs), 1, 200) for T null pairs,
 so a target fit sees T * 5e-4 pairs in its tail (fewer only for the smallest alignments); the
@@ -34,7 +43,12 @@ from . import null as NL
 from .mlp import MLP
 
 P0 = 5e-4
-DIMS = (48, 128, 128, 128, 2)
+# Calibration null: up to CAL_MAXK simulated replicates of the alignment (about CAL_PAIRS null
+# pairs), fitted with the deeper mass CAL_PMASS.  Its tail enters the network as three inputs
+# (valid flag, lambda m, ln lambda at P0; zeros when the fit failed).
+CAL_PMASS, CAL_PAIRS, CAL_MAXK, CAL_STREAM0 = 5e-3, 1e6, 4, 1 << 20
+NCAL = 3
+DIMS = (NL.NFEAT + NCAL, 128, 128, 128, 2)
 DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_gtp.npz")
 
 
@@ -48,8 +62,14 @@ def replicates_for(L, target_pairs=2e5, cap=200):
     return int(min(cap, max(1, math.ceil(target_pairs / pairs))))
 
 
-def random_setup(rng, Nr=(20, 6000), Lr=(50, 3500), big=0.0):
-    if big > 0 and rng.random() < big:
+XL_N, XL_L = (3000, 24000), (1500, 6000)
+
+
+def random_setup(rng, Nr=(20, 6000), Lr=(50, 3500), big=0.0, xl=0.0):
+    u = rng.random() if (big > 0 or xl > 0) else 1.0
+    if u < xl:
+        Nr, Lr = XL_N, XL_L
+    elif u < xl + big:
         Nr = (max(Nr[0], min(1000, Nr[1])), Nr[1]); Lr = (max(Lr[0], min(300, Lr[1])), Lr[1])
     N = int(round(math.exp(rng.uniform(math.log(Nr[0]), math.log(Nr[1])))))
     L = int(round(math.exp(rng.uniform(math.log(Lr[0]), math.log(Lr[1])))))
@@ -62,11 +82,45 @@ def _record_bytes(L, N, K):
     return K * (48 * L * L + 3 * N * L)
 
 
+def cal_replicates(L):
+    pairs = L * (L - 1) // 2
+    return int(min(CAL_MAXK, max(1, math.ceil(CAL_PAIRS / max(pairs, 1)))))
+
+
+def cal_jobs(tree, L, model, seed, gapfrac, group=0):
+    """The calibration replicates of one alignment (streams apart from the full null's)."""
+    return [NL.Job(tree, L, model, seed=seed, stream=CAL_STREAM0 + r, gapfrac=gapfrac, group=group)
+            for r in range(cal_replicates(L))]
+
+
+def cal_inputs(fit):
+    """Network inputs from a calibration fit row (mu, lambda, tau, newmass, ...)."""
+    mu, lam, newmass = float(fit[0]), float(fit[1]), float(fit[3])
+    if not (math.isfinite(mu) and math.isfinite(lam) and lam > 0 and newmass > 0):
+        return np.zeros(NCAL, np.float32)
+    m, lam = tail_params(mu, lam, newmass)
+    return np.array([1.0, lam * m, math.log(lam)], np.float32)
+
+
+def calibrate(ctx, tree, L, pi, gapfrac, seed=5, graph=0):
+    """The calibration null of one alignment on the device: its network inputs."""
+    fits, _ = NL.batch(ctx, cal_jobs(tree, L, NL.gtr_model(pi), seed, gapfrac), "GTp", pmass=CAL_PMASS,
+                       expo=True, graph=graph)
+    return cal_inputs(fits[0])
+
+
+def inputs(ctx, msa, tree, L, pi, gapfrac, seed=5):
+    """The network's inputs for a resident alignment: its features and its calibration null."""
+    return np.concatenate([NL.features(ctx, msa, tree), calibrate(ctx, tree, L, pi, gapfrac, seed)])
+
+
 def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1, log=None, covtype="GTp",
-             target_pairs=2e5, big=0.0):
-    """n training pairs: X (n, 48) features, Y (n, 2) = (lambda m, ln lambda); setups whose fit
-    failed are dropped.  Returns X, Y, list of setups (each with its replicate count K, null
-    pair count npairs, bin width w and the group's smallest and largest null score)."""
+             target_pairs=2e5, big=0.0, xl=0.0):
+    """n training pairs: X (n, NFEAT + NCAL) inputs (features of the group's first replicate and
+    the calibration inputs of a separate calibration null), Y (n, 2) = (lambda m, ln lambda)
+    of the group's fit; setups whose fit failed are dropped.  Returns X, Y, list of setups
+    (each with its replicate count K, null pair count npairs, bin width w and the group's
+    smallest and largest null score)."""
     rng = np.random.default_rng(seed)
     X, Y, S = [], [], []
     pending, used = [], 0.0
@@ -80,6 +134,10 @@ def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1,
             K = replicates_for(st["L"], target_pairs)
             jobs += [NL.Job(tree, st["L"], model, seed=st["sseed"], stream=r, gapfrac=st["gap"], group=g) for r in range(K)]
         fits, _, fe = NL.batch(ctx, jobs, covtype, pmass=5e-4, expo=True, graph=graph, feats=True)
+        cj = []
+        for g, (st, tree, model) in enumerate(pending):
+            cj += cal_jobs(tree, st["L"], model, st["sseed"], st["gap"], group=g)
+        cfits, _ = NL.batch(ctx, cj, covtype, pmass=CAL_PMASS, expo=True, graph=graph)
         for g, (st, _, _) in enumerate(pending):
             mu, lam, newmass = fits[g][0], fits[g][1], fits[g][3]
             if not (np.isfinite(mu) and np.isfinite(lam) and lam > 0 and newmass > 0):
@@ -88,17 +146,17 @@ def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1,
             K = replicates_for(st["L"], target_pairs)
             st = dict(st, K=K, npairs=K * (st["L"] * (st["L"] - 1) // 2), w=float(fits[g][5]),
                       mn=float(fits[g][6]), mx=float(fits[g][7]))
-            X.append(fe[g]); Y.append((lam * m, math.log(lam))); S.append(st)
+            X.append(np.concatenate([fe[g], cal_inputs(cfits[g])])); Y.append((lam * m, math.log(lam))); S.append(st)
         if log:
             log("generated %d / %d" % (len(X), n))
         pending, used = [], 0.0
 
     made = 0
     while made < n:
-        st = random_setup(rng, Nr, Lr, big)
+        st = random_setup(rng, Nr, Lr, big, xl)
         tree = NL.yule_tree(st["N"], st["bl"], seed=st["tseed"])
         model = NL.gtr_model(st["pi"])
-        b = _record_bytes(st["L"], st["N"], replicates_for(st["L"], target_pairs))
+        b = _record_bytes(st["L"], st["N"], max(replicates_for(st["L"], target_pairs), cal_replicates(st["L"])))
         if used + b > budget:
             flush()
         pending.append((st, tree, model)); used += b; made += 1
@@ -140,20 +198,31 @@ class Surrogate:
         nb = int((mx + 5 * w - bmin) / w)
         return bmin + 2 * nb * w
 
+    @staticmethod
+    def base(X):
+        """The calibration tail (lambda m, ln lambda) of each input row, zero where the
+        calibration fit failed.  The network learns the target's difference from it: the
+        calibration carries the alignment's own tree, the network the step from its mass to
+        P0 and its noise, so inputs unlike the training alignments can move the prediction only
+        as far as the differences it has seen."""
+        X = np.atleast_2d(np.asarray(X))
+        return X[:, NL.NFEAT + 1:NL.NFEAT + 3].astype(np.float64) * X[:, NL.NFEAT:NL.NFEAT + 1]
+
     def fit(self, X, Y, epochs=60, batch=512, lr=1e-3, seed=7):
+        Yr = (np.asarray(Y, np.float64) - self.base(X)).astype(np.float32)
         xm, xs = X.mean(0), X.std(0)
         xs = np.where(xs > 1e-6, xs, 1.0)
-        ym, ys = Y.mean(0), Y.std(0)
+        ym, ys = Yr.mean(0), Yr.std(0)
         self.net.set_scaling(xm, xs, ym, ys)
         self.xmin, self.xmax = X.min(0), X.max(0)
         self.net.set_range(self.xmin, self.xmax)
         self.scaling = (xm, xs, ym, ys)
-        return self.net.train(X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+        return self.net.train(X, Yr, epochs=epochs, batch=batch, lr=lr, seed=seed)
 
     def predict(self, feats):
-        """(m, lambda) per row of feats; rows outside the training range get NaN."""
+        """(m, lambda) per row of inputs (inputs()); rows outside the training range get NaN."""
         feats = np.atleast_2d(np.asarray(feats, np.float32))
-        out = self.net.forward(feats).astype(np.float64)
+        out = self.net.forward(feats).astype(np.float64) + self.base(feats)
         ok = np.array([self.net.in_range(f) for f in feats])
         lam = np.exp(out[:, 1])
         m = out[:, 0] / lam

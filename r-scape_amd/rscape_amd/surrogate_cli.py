@@ -51,7 +51,8 @@ def load_data(path):
 def cmd_gen(a):
     ctx = Context(0)
     t0 = time.time()
-    X, Y, setups = S.generate(ctx, a.n, seed=a.seed, budget=a.budget, log=log, target_pairs=a.target_pairs, big=a.big)
+    X, Y, setups = S.generate(ctx, a.n, seed=a.seed, budget=a.budget, log=log, target_pairs=a.target_pairs, big=a.big,
+                                  xl=a.xl)
     log("training pairs: %d (%.1f s to simulate, score and fit)" % (len(X), time.time() - t0))
     save_data(a.out, X, Y, setups)
 
@@ -71,7 +72,8 @@ def cmd_train(a):
     if a.data:
         X, Y, setups = load_data(a.data)
     else:
-        X, Y, setups = S.generate(ctx, a.n, seed=a.seed, budget=a.budget, log=log, target_pairs=a.target_pairs, big=a.big)
+        X, Y, setups = S.generate(ctx, a.n, seed=a.seed, budget=a.budget, log=log, target_pairs=a.target_pairs, big=a.big,
+                                  xl=a.xl)
     t1 = time.time()
     log("training pairs: %d (%.1f s to simulate, score and fit or load)" % (len(X), t1 - t0))
     rng = np.random.default_rng(a.seed + 1)
@@ -91,7 +93,7 @@ def cmd_train(a):
     e13 = tail_error_at(Y[va][ok], pm[ok], pl[ok])
     Nv = np.array([setups[i]["N"] for i in va])[ok]
     by_n = {}
-    for lo, hi in ((20, 100), (100, 1000), (1000, 3000), (3000, 7000)):
+    for lo, hi in ((20, 100), (100, 1000), (1000, 3000), (3000, 7000), (7000, 25000)):
         s = (Nv >= lo) & (Nv < hi)
         if s.any():
             by_n["N%d-%d" % (lo, hi)] = dict(n=int(s.sum()), median=float(np.median(e13[s])),
@@ -135,7 +137,7 @@ def eval_one(ctx, sur, rng, Nr, Lr, K=None, seed=0, R=8):
     loo = np.abs(lgs - (lgs.sum(0)[None] - lgs) / (R - 1))[:, sig]
     res["noise"] = loo.mean(0).tolist()                       # one full null vs the others, per pair
     res["tail_full"] = [float(np.mean([f[0] for f in fits])), float(np.mean([f[1] for f in fits]))]
-    f = NL.features(ctx, msa, tree)
+    f = S.inputs(ctx, msa, tree, L, pi, gap, seed=1000 * seed + 5)
     m, lam = sur.predict(f)
     res["in_range"] = bool(np.isfinite(m[0]))
     if res["in_range"]:
@@ -208,10 +210,12 @@ def main(argv=None):
     g.add_argument("--budget", type=float, default=24e9)
     g.add_argument("--target-pairs", type=float, default=2e6)
     g.add_argument("--big", type=float, default=0.5)
+    g.add_argument("--xl", type=float, default=0.03)
     t = sub.add_parser("train")
     t.add_argument("--data", default=None)
     t.add_argument("--n", type=int, default=100000)
     t.add_argument("--big", type=float, default=0.5)
+    t.add_argument("--xl", type=float, default=0.03)
     t.add_argument("--out", default=S.DATA)
     t.add_argument("--report", default=None)
     t.add_argument("--epochs", type=int, default=120)

@@ -1,4 +1,4 @@
-"""Surrogate inputs and outputs on the device: the 48 features against a numpy statement of
+"""Surrogate inputs and outputs on the device: the 64 features against a numpy statement of
 their definition, the batch's per-group features against the single-alignment entry point,
 the network's scalers, and tail E-values against numpy."""
 import numpy as np
@@ -16,7 +16,7 @@ def features_np(ax, tree):
     ax = np.asarray(ax)
     N, L = ax.shape
     left, right, ld, rd = (np.asarray(tree[k]) for k in ("left", "right", "ld", "rd"))
-    f = np.zeros(48)
+    f = np.zeros(64)
     bl = np.concatenate([ld, rd])
     T = bl.sum()
     f[0], f[1], f[2] = np.log(N), np.log(L), np.log(T)
@@ -58,6 +58,18 @@ def features_np(ax, tree):
         nleaf[v] = a + b
         coll += abs(a - b)
     f[47] = coll / ((N - 1) * (N - 2) / 2) if N > 2 else 0.0
+    # split spectrum: branch sums by the minor clade share q of the branch's split,
+    # bins q in (1/4, 1/2], (1/8, 1/4], ..., (1/256, 1/128], <= 1/256
+    s1, s2 = np.zeros(8), np.zeros(8)
+    for v in range(N - 1):
+        for d, t in ((left[v], ld[v]), (right[v], rd[v])):
+            n = nleaf[d] if d > 0 else 1
+            q = min(n, N - n) / N
+            k = min(7, int(np.floor(-np.log2(q))) - 1)          # q in (2^-(k+2), 2^-(k+1)]
+            s1[k] += t
+            s2[k] += (1 - np.exp(-t)) ** 2
+    f[48:56] = np.log(1e-3 + s1)
+    f[56:64] = np.log(1e-4 + s2)
     return f
 
 
