@@ -146,6 +146,75 @@ def uneven_null_batch(ctx, reps=3):
     return out
 
 
+def significance_accuracy(ctx, m, tree, L, pi, gap, K, R):
+    """The reference for E-value errors of a scored alignment m (GTp): the full null's E-value of
+    a pair is itself random (it depends on the replicates drawn), most of all past the last
+    survival bin, where it is set by the largest null score (src/covariation.c:2380).  The
+    reference log10 E is therefore the mean over R independent K-replicate full nulls; one full
+    null's own error is its distance from the mean of the other R - 1 (the replicate noise any
+    single run of R-scape carries).  Returns (fields, fn) with fn(E matrix) -> the fields of
+    that matrix's errors on the pairs the reference calls (E < 0.05), split into the fitted
+    range and the capped range (past the last survival bin)."""
+    from rscape_amd import surrogate as S
+    Nc = L * (L - 1) // 2
+    iu = np.triu_indices(L, 1)
+    lg = lambda e: np.log10(np.maximum(e, 1e-300)).astype(np.float32)
+    cov = m.get_cov()[0][iu]
+    lgs, capped_from = np.empty((R, len(iu[0])), np.float32), []
+    for r in range(R):
+        h, sf, _ = S.full_null(ctx, tree, L, pi, gap, K=K, seed=1000 + r)
+        lgs[r] = lg(m.evalues(h, sf, Nc)[iu])
+        capped_from.append(h.h.bmin + (2 * h.h.nb - 2) * h.h.w)
+    ref = lgs.mean(0)
+    sig = ref < math.log10(0.05)
+    capped = sig & (cov > np.median(capped_from))
+    fitr = sig & ~capped
+
+    def errs(v, sel):
+        v = v[sel]
+        return dict(pairs=int(sel.sum()), median=float(np.median(v)) if len(v) else None,
+                    frac_within_0p3=float(np.mean(v <= 0.3)) if len(v) else None)
+    loo = np.abs(lgs[:, sig] - (lgs[:, sig].sum(0)[None] - lgs[:, sig]) / (R - 1)).reshape(-1)
+    til = lambda s: np.tile(s[sig], R)
+    out = {"accuracy_reference": "mean log10 E over %d independent %d-replicate full nulls" % (R, K),
+           "sig_pairs": int(sig.sum()),
+           "one_full_null_log10E": {"all": errs(loo, np.ones(len(loo), bool)), "fit_range": errs(loo, til(fitr)),
+                                    "capped": errs(loo, til(capped))}}
+
+    def of(ev):
+        d = np.abs(lg(ev[iu]) - ref)
+        return {"surrogate_log10E": {"all": errs(d, sig), "fit_range": errs(d, fitr), "capped": errs(d, capped)},
+                "call_agreement": float(np.mean((ev[iu] < 0.05) == sig))}
+    return out, of
+
+
+def fitted_range_variant(ctx, N=200, K=20, R=64):
+    """Surrogate accuracy on the headline's shape with few sequences (N=200 x L=1500, 400
+    planted pairs): the planted pairs score inside the fitted tail instead of past the last
+    survival bin, so this tests the tail the network predicts rather than the cap."""
+    from rscape_amd import null as NL
+    from rscape_amd import surrogate as S
+    d = synth.config(1, seed=42, N=N)
+    ax, tree = d["ax"], d["tree"]
+    L = ax.shape[1]
+    Nc = L * (L - 1) // 2
+    pi, gap = S.msa_null_params(ax)
+    m = ctx.msa(ax)
+    m.calculate("GTp")
+    out, of = significance_accuracy(ctx, m, tree, L, pi, gap, K, R)
+    out["workload"] = "N=%d x L=%d, GTp, %d planted pairs, %d-replicate nulls" % (N, L, int((d["ct"] >= 0).sum()) // 2, K)
+    if os.path.exists(S.DATA):
+        sur = S.Surrogate.load(ctx)
+        mu, lam = sur.predict(S.inputs(ctx, m, tree, L, pi, gap, seed=7))
+        if np.isfinite(mu[0]):
+            xcap = sur.cap(float(mu[0]), float(lam[0]), K, Nc)
+            out.update(of(NL.evalues_tail(ctx, m, float(mu[0]), float(lam[0]), S.P0, Nc, matrix=True, xcap=xcap)))
+        else:
+            out["surrogate_fallback"] = True
+    m.close()
+    return out
+
+
 def evalue_runs(ctx, d, K=20, reps=3, acc_nulls=64):
     """Complete significance runs of the alignment (GTp scoring + null + exponential tail fit +
     E-values + the list of pairs with E < 0.05), three ways: the plain per-replicate GPU path
@@ -252,46 +321,17 @@ def evalue_runs(ctx, d, K=20, reps=3, acc_nulls=64):
     sig = eb < 0.05
     lg = lambda e: np.log10(np.maximum(e, 1e-300))
     out["plain_vs_batched_max_abs_log10E"] = float(np.max(np.abs(lg(ep[sig]) - lg(eb[sig])))) if sig.any() else 0.0
-    # The full null's E-value of a pair is itself random (it depends on the replicates drawn),
-    # most of all past the last survival bin, where it is set by the largest null score
-    # (src/covariation.c:2380).  The reference value of log10 E is therefore the mean over R
-    # independent full nulls; one full null's own error is its distance from the mean of the
-    # other R - 1 (the replicate noise any single run of R-scape carries).
     m.calculate("GTp")
-    cov = m.get_cov()[0][iu]
-    lgs, capped_from = [], []
-    for r in range(acc_nulls):
-        h, sf, _ = S.full_null(ctx, tree, L, pi, gap, K=K, seed=1000 + r)
-        lgs.append(lg(m.evalues(h, sf, Nc)[iu]))
-        capped_from.append(h.h.bmin + (2 * h.h.nb - 2) * h.h.w)
-    lgs = np.array(lgs)
-    ref = lgs.mean(0)
-    sig = ref < math.log10(0.05)
-    capped = sig & (cov > np.median(capped_from))
-    fitr = sig & ~capped
-
-    def errs(v, sel):
-        v = v[sel]
-        return dict(pairs=int(sel.sum()), median=float(np.median(v)) if len(v) else None,
-                    frac_within_0p3=float(np.mean(v <= 0.3)) if len(v) else None)
-    loo = np.abs(lgs - (lgs.sum(0)[None] - lgs) / (acc_nulls - 1))            # R x pairs
-    single = loo.reshape(-1)
-    sel = lambda s: np.tile(s, acc_nulls)
-    out["accuracy_reference"] = "mean log10 E over %d independent %d-replicate full nulls" % (acc_nulls, K)
-    out["sig_pairs"] = int(sig.sum())
-    out["one_full_null_log10E"] = {"all": errs(single, sel(sig)), "fit_range": errs(single, sel(fitr)),
-                                   "capped": errs(single, sel(capped))}
+    acc, ref_sig = significance_accuracy(ctx, m, tree, L, pi, gap, K, acc_nulls)
+    out.update(acc)
     if sur is not None:
         ts = timed(surrogate)
-        es = surrogate(matrix=True)[iu]
         out["surrogate"] = 1.0 / ts
         out["surrogate_fallback"] = fallback[0]
         out["speedup_surrogate_vs_plain"] = tp / ts
         out["speedup_surrogate_vs_rscape_null_plain"] = tpf / ts
         out["speedup_surrogate_vs_rscape_null_batched"] = tbf / ts
-        d = np.abs(lg(es) - ref)
-        out["surrogate_log10E"] = {"all": errs(d, sig), "fit_range": errs(d, fitr), "capped": errs(d, capped)}
-        out["call_agreement"] = float(np.mean((es < 0.05) == sig))
+        out.update(ref_sig(surrogate(matrix=True)))
     m.close()
     return out
 
@@ -589,6 +629,7 @@ def main():
         out["extra"] = extra
     if not args.no_evalue:
         out["evalue_runs"] = evalue_runs(ctx, d, reps=args.evalue_reps)
+        out["evalue_runs"]["fitted_range_variant"] = fitted_range_variant(ctx)
     if not args.no_batch:
         out["uneven_null_batch"] = uneven_null_batch(ctx)
     if args.calib:
