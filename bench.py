@@ -205,10 +205,9 @@ def fitted_range_variant(ctx, N=200, K=20, R=64):
     out["workload"] = "N=%d x L=%d, GTp, %d planted pairs, %d-replicate nulls" % (N, L, int((d["ct"] >= 0).sum()) // 2, K)
     if os.path.exists(S.DATA):
         sur = S.Surrogate.load(ctx)
-        mu, lam = sur.predict(S.inputs(ctx, m, tree, L, pi, gap, seed=7))
-        if np.isfinite(mu[0]):
-            xcap = sur.cap(float(mu[0]), float(lam[0]), K, Nc)
-            out.update(of(NL.evalues_tail(ctx, m, float(mu[0]), float(lam[0]), S.P0, Nc, matrix=True, xcap=xcap)))
+        mu, lam, xcap = sur.tail(S.inputs(ctx, m, tree, L, pi, gap, seed=7), K, Nc)
+        if np.isfinite(mu):
+            out.update(of(NL.evalues_tail(ctx, m, mu, lam, S.P0, Nc, matrix=True, xcap=xcap)))
         else:
             out["surrogate_fallback"] = True
     m.close()
@@ -280,12 +279,11 @@ def evalue_runs(ctx, d, K=20, reps=3, acc_nulls=64):
 
     def surrogate(matrix=False):
         m.calculate("GTp")
-        mu, lam = sur.predict(S.inputs(ctx, m, tree, L, pi, gap, seed=7))
-        if not np.isfinite(mu[0]):
+        mu, lam, xcap = sur.tail(S.inputs(ctx, m, tree, L, pi, gap, seed=7), K, Nc)
+        if not np.isfinite(mu):
             fallback[0] = True
             return batched(matrix)
-        xcap = sur.cap(float(mu[0]), float(lam[0]), K, Nc)
-        ev = NL.evalues_tail(ctx, m, float(mu[0]), float(lam[0]), S.P0, Nc, matrix=matrix, xcap=xcap)
+        ev = NL.evalues_tail(ctx, m, mu, lam, S.P0, Nc, matrix=matrix, xcap=xcap)
         return ev if matrix else NL.hits(ctx, m, 0.05)
 
     def timed(fn):
@@ -392,11 +390,10 @@ def c4_significance(ctx, K=100, R_nulls=32, reps=3):
 
     def surrogate():
         m.calculate(**kw)
-        mu, lam = sur.predict(S.inputs(ctx, m, tree, L, pi, gap, seed=7))
-        if not np.isfinite(mu[0]):
+        mu, lam, xcap = sur.tail(S.inputs(ctx, m, tree, L, pi, gap, seed=7), K, Nc)
+        if not np.isfinite(mu):
             return batched(), True
-        xcap = sur.cap(float(mu[0]), float(lam[0]), K, Nc)
-        return NL.evalues_tail(ctx, m, float(mu[0]), float(lam[0]), S.P0, Nc, matrix=True, xcap=xcap), False
+        return NL.evalues_tail(ctx, m, mu, lam, S.P0, Nc, matrix=True, xcap=xcap), False
 
     def timed(fn, n=reps):
         r = fn()                                 # the first call allocates the buffers

@@ -49,6 +49,7 @@ P0 = 5e-4
 CAL_PMASS, CAL_PAIRS, CAL_MAXK, CAL_STREAM0 = 5e-3, 1e6, 4, 1 << 20
 NCAL = 3
 DIMS = (NL.NFEAT + NCAL, 128, 128, 128, 2)
+CAP_DIMS = (NL.NFEAT + NCAL, 64, 64, 1)
 DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_gtp.npz")
 
 
@@ -172,26 +173,68 @@ class Surrogate:
         self.net = net or MLP(ctx, DIMS, seed=seed, bf16=True)
         self.xmin = self.xmax = None
         self.zab = (-math.log(math.log(2.0)), 1.0)     # exponential tail, median maximum
+        self.cap_net, self.cap_scaling, self.cap_pairs = None, None, math.inf
 
-    def fit_cap(self, setups, Y):
-        """(a, b) of z = lambda (max - m) = a + b ln(npairs P0) from the training groups."""
+    @staticmethod
+    def zu(setups, Y):
+        """Each training group's largest null score in tail units, z = lambda (max - m), and
+        u = ln(npairs P0)."""
         lam = np.exp(Y[:, 1].astype(np.float64))
         m = Y[:, 0] / lam
         z = lam * (np.array([s["mx"] for s in setups]) - m)
         u = np.log(np.array([s["npairs"] for s in setups], np.float64) * P0)
+        return z, u
+
+    def fit_cap(self, setups, Y, X=None, epochs=40, batch=256, lr=3e-4, seed=11):
+        """(a, b) of z = a + b u over the training groups; with their inputs X, also a small
+        network for each alignment's offset delta = z - (a + b u).  One group's maximum carries
+        a Gumbel spread (sd ~1.2 in z) that no input explains; under it sits an offset set by
+        the tree (the split spectrum above all), which the network learns as a conditional
+        mean.  The offset is learned where the training groups' maxima lie (about
+        cap_pairs = their median null pair count); tail() applies it to nulls at least that
+        large, whose cap extrapolates past it, and the line alone below.  Returns (a, b)."""
+        z, u = self.zu(setups, Y)
         ok = np.isfinite(z) & np.isfinite(u)
         b, a = np.polyfit(u[ok], z[ok], 1)
         self.zab = (float(a), float(b))
+        if X is not None:
+            d = (z - (a + b * u))[ok].astype(np.float32)[:, None]
+            Xo = np.asarray(X, np.float32)[ok]
+            xm, xs = Xo.mean(0), Xo.std(0)
+            xs = np.where(xs > 1e-6, xs, 1.0)
+            self.cap_net = MLP(self.ctx, CAP_DIMS, seed=seed, bf16=True)
+            self.cap_scaling = (xm, xs, d.mean(0), d.std(0))
+            self.cap_net.set_scaling(*self.cap_scaling)
+            self.cap_net.train(Xo, d, epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
+            self.cap_pairs = float(np.median([s["npairs"] for s in setups]))
         return self.zab
 
-    def cap(self, m, lam, K, Nc, bmin=-10.0, w0=0.05, hpts=400, tol=1e-6):
+    def delta(self, feats):
+        """The cap network's offset of z per input row (0 without a cap network)."""
+        feats = np.atleast_2d(np.asarray(feats, np.float32))
+        if self.cap_net is None:
+            return np.zeros(len(feats))
+        return self.cap_net.forward(feats)[:, 0].astype(np.float64)
+
+    def tail(self, feats, K, Nc):
+        """(m, lambda, xcap) of one alignment's inputs for a K-replicate null of Nc pairs; NaN,
+        NaN, inf outside the training range (the caller runs the full null)."""
+        mu, lam = self.predict(feats)
+        if not np.isfinite(mu[0]):
+            return math.nan, math.nan, math.inf
+        m, lam = float(mu[0]), float(lam[0])
+        delta = float(self.delta(feats)[0]) if K * Nc >= self.cap_pairs else 0.0
+        return m, lam, self.cap(m, lam, K, Nc, delta=delta)
+
+    def cap(self, m, lam, K, Nc, bmin=-10.0, w0=0.05, hpts=400, tol=1e-6, delta=0.0):
         """Score from which R-scape's E-values stop following the tail, for a K-replicate null
         of an alignment with Nc pairs: bmin + 2 nb w (src/covariation.c:2380), with w from the
         first replicate (calculate_width_histo, src/R-scape.c:1353-1360: its largest score
-        against bmin) and nb = (max + 5 w - bmin) / w bins (cov_GrowRankList)."""
+        against bmin) and nb = (max + 5 w - bmin) / w bins (cov_GrowRankList); the maxima
+        from z = a + b u + delta."""
         a, b = self.zab
-        mx1 = m + (a + b * math.log(max(Nc * P0, 1e-300))) / lam
-        mx = m + (a + b * math.log(max(K * Nc * P0, 1e-300))) / lam
+        mx1 = m + (a + b * math.log(max(Nc * P0, 1e-300)) + delta) / lam
+        mx = m + (a + b * math.log(max(K * Nc * P0, 1e-300)) + delta) / lam
         w = min(w0, (mx1 - bmin) / hpts)
         if not (w >= tol and mx > bmin):
             return math.inf
@@ -232,9 +275,15 @@ class Surrogate:
     def save(self, path=DATA):
         W, b = self.net.get_params()
         xm, xs, ym, ys = self.scaling
+        cap = {}
+        if self.cap_net is not None:
+            cW, cb = self.cap_net.get_params()
+            cxm, cxs, cym, cys = self.cap_scaling
+            cap = dict(cdims=np.array(CAP_DIMS), cxm=cxm, cxs=cxs, cym=cym, cys=cys, cpairs=np.array(self.cap_pairs),
+                       **{"cW%d" % k: w for k, w in enumerate(cW)}, **{"cb%d" % k: v for k, v in enumerate(cb)})
         os.makedirs(os.path.dirname(path), exist_ok=True)
         np.savez(path, dims=np.array(DIMS), xm=xm, xs=xs, ym=ym, ys=ys, xmin=self.xmin, xmax=self.xmax,
-                 zab=np.array(self.zab),
+                 zab=np.array(self.zab), **cap,
                  **{"W%d" % k: w for k, w in enumerate(W)}, **{"b%d" % k: v for k, v in enumerate(b)})
 
     @classmethod
@@ -249,6 +298,13 @@ class Surrogate:
         s.net.set_range(s.xmin, s.xmax)
         if "zab" in z.files:
             s.zab = (float(z["zab"][0]), float(z["zab"][1]))
+        if "cdims" in z.files:
+            cd = tuple(int(v) for v in z["cdims"])
+            s.cap_net = MLP(ctx, cd, bf16=True)
+            s.cap_net.set_params([z["cW%d" % k] for k in range(len(cd) - 1)], [z["cb%d" % k] for k in range(len(cd) - 1)])
+            s.cap_scaling = (z["cxm"], z["cxs"], z["cym"], z["cys"])
+            s.cap_net.set_scaling(*s.cap_scaling)
+            s.cap_pairs = float(z["cpairs"])
         return s
 
 

@@ -82,7 +82,12 @@ def cmd_train(a):
     va, tr = idx[:nval], idx[nval:]
     sur = S.Surrogate(ctx, seed=a.seed)
     loss = sur.fit(X[tr], Y[tr], epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed + 2)
-    zab = sur.fit_cap([setups[i] for i in tr], Y[tr])
+    zab = sur.fit_cap([setups[i] for i in tr], Y[tr], X=X[tr], epochs=a.cap_epochs)
+    zv, uv = S.Surrogate.zu([setups[i] for i in va], Y[va])
+    dv = zv - (zab[0] + zab[1] * uv)
+    pv = sur.delta(X[va])
+    okv = np.isfinite(dv)
+    cap_r2 = float(1 - np.mean((dv[okv] - pv[okv]) ** 2) / np.var(dv[okv]))
     t2 = time.time()
     pm, pl = sur.predict(X[va])
     ok = np.isfinite(pm)
@@ -102,6 +107,7 @@ def cmd_train(a):
     rep = dict(pairs=int(len(X)), simulate_s=t1 - t0, train_s=t2 - t1, final_loss=loss,
                val_rmse_lambda_m=err_u, val_rmse_lnlambda=err_l, const_rmse_lambda_m=const_u,
                const_rmse_lnlambda=const_l, val_log10E_err_at_depth13=by_n, cap_model_ab=list(zab),
+               cap_offset_val_r2=cap_r2, cap_offset_val_sd=float(np.std(pv)),
                epochs=a.epochs, batch=a.batch, lr=a.lr)
     log(json.dumps(rep))
     if a.report:
@@ -138,14 +144,13 @@ def eval_one(ctx, sur, rng, Nr, Lr, K=None, seed=0, R=8):
     res["noise"] = loo.mean(0).tolist()                       # one full null vs the others, per pair
     res["tail_full"] = [float(np.mean([f[0] for f in fits])), float(np.mean([f[1] for f in fits]))]
     f = S.inputs(ctx, msa, tree, L, pi, gap, seed=1000 * seed + 5)
-    m, lam = sur.predict(f)
-    res["in_range"] = bool(np.isfinite(m[0]))
+    m, lam, xcap = sur.tail(f, K, Nc)
+    res["in_range"] = bool(np.isfinite(m))
     if res["in_range"]:
-        xcap = sur.cap(float(m[0]), float(lam[0]), K, Nc)
-        es = NL.evalues_tail(ctx, msa, float(m[0]), float(lam[0]), S.P0, Nc, xcap=xcap)[iu]
+        es = NL.evalues_tail(ctx, msa, m, lam, S.P0, Nc, xcap=xcap)[iu]
         res["err"] = np.abs(lg(es[sig]) - ref[sig]).tolist()
         res["agree"] = float(np.mean((es < 0.05) == sig))
-        res["tail_sur"] = [float(m[0]), float(lam[0])]
+        res["tail_sur"] = [m, lam]
         res["err_median"] = float(np.median(res["err"])) if res["err"] else None
     msa.close()
     return res
@@ -219,6 +224,7 @@ def main(argv=None):
     t.add_argument("--out", default=S.DATA)
     t.add_argument("--report", default=None)
     t.add_argument("--epochs", type=int, default=120)
+    t.add_argument("--cap-epochs", type=int, default=40)
     t.add_argument("--batch", type=int, default=256)
     t.add_argument("--lr", type=float, default=3e-4)
     t.add_argument("--seed", type=int, default=1)
