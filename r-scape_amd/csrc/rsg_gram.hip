@@ -168,7 +168,7 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
                                              void *__restrict__ out, int64_t i0, int64_t j0, uint8_t *lds,
                                              int64_t kt0, int64_t kt1, void *__restrict__ part,
                                              double *__restrict__ pm_i = nullptr, double *__restrict__ pm_j = nullptr,
-                                             int64_t nb32 = 0)
+                                             int64_t nb32 = 0, int64_t tailc = -1)
 {
 #pragma clang fp contract(off)
   constexpr int BI = 32 * WI, BJ = 32 * WJ, NT = 128 * WI * WJ, KS = KT / 32;
@@ -187,6 +187,14 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
     for (int y = 0; y < 4; y++)
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[x][y][r] = 0;
+  // tailc >= 0: the sequences past the last whole K-tile fill one 32-sequence k-step (chunks
+  // tailc, tailc + 1; zero codes past N), loaded here straight into the lane's fragments and
+  // multiplied after the loop, so the loop runs whole K-tiles only
+  uint4 ta = make_uint4(0, 0, 0, 0), tb = make_uint4(0, 0, 0, 0);
+  if (tailc >= 0) {
+    ta = *(const uint4 *)(xt + ((tailc + khalf) * Lpad + j0 + wj * 32 + col) * 16);
+    tb = *(const uint4 *)(xt + ((tailc + khalf) * Lpad + i0 + wi * 32 + col) * 16);
+  }
 
   // three LDS images, loads two stages ahead: stage kt0 + it is read from image it % 3 while
   // stage kt0 + it + 2 lands in image (it + 2) % 3, which was last read in iteration it - 1
@@ -242,6 +250,24 @@ __device__ __forceinline__ void gram_tile_i8(const uint8_t *__restrict__ xt, int
     }
   }
   __builtin_amdgcn_s_waitcnt(0x0F70);          // no LDS-direct load outlives the loop
+  if (tailc >= 0) {
+    v4i a[4], b[2];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const unsigned mk = 0x01010101u << s;
+      a[s] = (v4i){(int)(ta.x & mk), (int)(ta.y & mk), (int)(ta.z & mk), (int)(ta.w & mk)};
+    }
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      const unsigned mk = 0x01010101u << (x0 + x);
+      b[x] = (v4i){(int)(tb.x & mk), (int)(tb.y & mk), (int)(tb.z & mk), (int)(tb.w & mk)};
+    }
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++)
+        acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[y], b[x], acc[x][y], 0, 0, 0);
+  }
 
   // epilogue: lane owns i column `col`; register r holds j = 8*(r/4) + 4*khalf + r%4.
   // Both destinations are affine in (ii, jj): base + jj * sj + ii * 16 cells.
@@ -356,13 +382,12 @@ __device__ __forceinline__ void split_unit(int u, int sbase, int sextra, int &t,
 // tiles' end times is filled rather than added to the kernel's length.
 template <int WI, int WJ, int KT, int OK>
 __global__ void __launch_bounds__(128 * WI * WJ) __attribute__((amdgpu_waves_per_eu(2)))
-k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t Lpad, int64_t L, void *__restrict__ out,
+k_gram_i8(const uint8_t *__restrict__ xt, int64_t nkt, int64_t tailc, int64_t Lpad, int64_t L, void *__restrict__ out,
           const int2 *__restrict__ tiles, int ndp, int per_xcd, int ntiles, int sbase, int sextra,
           uint8_t *__restrict__ part, double *__restrict__ pm_part, int *__restrict__ pmflag)
 {
   typedef CodeStage<32 * WI, 32 * WJ, KT, 128 * WI * WJ> St;
   __shared__ __attribute__((aligned(16))) uint8_t lds[St::lds_bytes(OK == CNT_U16)];
-  const int64_t nkt = Npad / KT;
   const int nsu = (ntiles - ndp) * sbase + sextra;
   int t;
   int64_t k0 = 0, k1 = nkt;
@@ -387,7 +412,7 @@ k_gram_i8(const uint8_t *__restrict__ xt, int64_t Npad, int64_t Lpad, int64_t L,
   const int64_t nb32 = (L + 31) / 32;
   double *pmi = (pm_part && !p) ? pm_part : nullptr;               // split segments: in k_gram_fix
   gram_tile_i8<WI, WJ, KT, OK>(xt, Lpad, L, out, (int64_t)tl.x * 32 * WI, (int64_t)tl.y * 32 * WJ, lds, k0, k1, p,
-                               pmi, pmi ? pmi + pm32_n(nb32) : nullptr, nb32);
+                               pmi, pmi ? pmi + pm32_n(nb32) : nullptr, nb32, k1 == nkt ? tailc : -1);
 }
 
 // Split tiles: one work group (16 waves) per 32 x 32 block sums the block's partial records over
@@ -761,7 +786,13 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, double *pm_part, int *pm
   // (two per SIMD): 4 / (WI WJ) work groups.
   const int ncu = c->ncu > 0 ? c->ncu : 256;
   const int slots = std::max(ncu * 4 / (WI * WJ), 1);
-  const int64_t nkt = m->Npad / KT;
+  // whole K-tiles; when the sequences past the last whole one fit one 32-sequence k-step, that
+  // step runs from registers after the loop instead of as a K-tile of mostly padding
+  int64_t nkt = m->Npad / KT, tailc = -1;
+  {
+    const int64_t nfull = m->N / KT, r = m->N - nfull * KT;
+    if (nfull >= 1 && r > 0 && r <= 32) { nkt = nfull; tailc = nfull * (KT / 16); }
+  }
   int ndp = ntiles, sbase = 1, sextra = 0;
   const int rem = ntiles % slots;
   if (c->gram_split && ntiles < 4 * slots && rem > 0 && rem <= slots / 2 && nkt >= 2) {
@@ -786,7 +817,7 @@ static int launch_i8(rsg_ctx *c, rsg_msa *m, void *out, double *pm_part, int *pm
   dim3 grid((unsigned)(8 * per + nsu)), block(128 * WI * WJ);
   // one timer over the Gram and its split-K fix-up
   int t; rsg_tstart(c, "gram_i8", &t);
-  hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, m->Npad, m->Lpad, m->L, out,
+  hipLaunchKernelGGL((k_gram_i8<WI, WJ, KT, OK>), grid, block, 0, c->stream, m->d_xt, nkt, tailc, m->Lpad, m->L, out,
                      tiles, ndp, per, ntiles, sbase, sextra, part, pm_part, pmflag);
   if (nrem > 0)
     hipLaunchKernelGGL((k_gram_fix<BI, BJ, OK>), dim3((unsigned)(nrem * (BI / 32) * (BJ / 32))), dim3(64 * FIX_WAVES), 0, c->stream,

@@ -70,6 +70,29 @@ def test_gram_shapes_bit_exact(ctx, shape):
     assert np.allclose(pm, oracle.probs(ax)["pm"], rtol=1e-12, atol=1e-15)
 
 
+@pytest.mark.parametrize("N", [257, 288, 289])
+@pytest.mark.parametrize("shape", range(1, 10))
+def test_gram_tail_step_bit_exact(ctx, shape, N):
+    """N = 257 and 288 leave 1 and 32 sequences past the last whole K-tile of every shape (64,
+    128, 256): the int8 Gram runs them as one k-step from registers after its loop; N = 289
+    (33 past) takes a whole padded K-tile instead.  Counts and marginals as the oracle's."""
+    print("gram tail: shape %d N=%d" % (shape, N), flush=True)
+    L = 203
+    ax = _msa(N, L, 31 + shape + N)
+    ctx.set_gram_shape(shape)
+    try:
+        m = ctx.msa(ax)
+        m.probs(R.COUNTS_INT8)
+        got = m.counts().reshape(L, L, 16)
+        pm = m.get_probs()["pm"]
+    finally:
+        ctx.set_gram_shape(0)
+    ref = oracle.pair_counts(ax).reshape(L, L, 16)
+    iu = np.triu_indices(L, 1)
+    assert np.array_equal(got[iu], ref[iu].astype(float))
+    assert np.allclose(pm, oracle.probs(ax)["pm"], rtol=1e-12, atol=1e-15)
+
+
 @pytest.mark.parametrize("N,L", [(600, 1500), (300, 203), (5000, 90)])
 def test_gram_split_matches_unsplit(ctx, N, L):
     """Tiles of the last partial wave split along K (partials summed) give the same counts."""
