@@ -1209,26 +1209,28 @@ k_gram_w8(const uint8_t *__restrict__ xt, const uint8_t *__restrict__ wq, const 
   else   gram_tile_w8<KT, false>(xt, wq, *wscale, Lpad, L, out, nsx, (int64_t)tl.x * 32, (int64_t)tl.y * 32, lds, k0, k1, p);
 }
 
-// Split tiles: one work group (16 waves) per tile adds the tile's int64 partial sums over its K
-// segments (exact) and writes the records and nseff with the arithmetic of the gram_tile_w8
-// epilogue (one FP64 product and one FP32 rounding per cell; nseff from the exact total), so
-// they equal the unsplit kernel's bit for bit.  Wave w takes row r = w of the MFMA layout.
-__global__ void __launch_bounds__(1024)
+// Split tiles: four work groups (4 waves each) per tile add the tile's int64 partial sums over
+// its K segments (exact) and write the records and nseff with the arithmetic of the
+// gram_tile_w8 epilogue (one FP64 product and one FP32 rounding per cell; nseff from the exact
+// total), so they equal the unsplit kernel's bit for bit.  Wave w of group q takes row
+// r = 4 q + w of the MFMA layout.
+__global__ void __launch_bounds__(256)
 k_gram_w8_fix(const int64_t *__restrict__ part, const int2 *__restrict__ tiles, int ndp, int sbase, int sextra,
               const double *__restrict__ wscale, int64_t L, float *__restrict__ out, double *__restrict__ nsx)
 {
 #pragma clang fp contract(off)
-  const int tr = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31, khalf = lane >> 5;
+  const int tr = blockIdx.x >> 2, wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31, khalf = lane >> 5;
   const int2 tl = tiles[ndp + tr];
   const int ns = sbase + (tr < sextra ? 1 : 0);
   const double W = *wscale;
-  const int r = wave, jj = 8 * (r >> 2) + 4 * khalf + (r & 3);
+  const int r = 4 * (blockIdx.x & 3) + wave, jj = 8 * (r >> 2) + 4 * khalf + (r & 3);
   const int64_t i = (int64_t)tl.x * 32 + col, j = (int64_t)tl.y * 32 + jj;
   if (!(i < j && j < L)) return;
   const longlong2 *p0 = (const longlong2 *)(part + (int64_t)split_first(tr, sbase, sextra) * 1024 * 16 + ((int64_t)jj * 32 + col) * 16);
   int64_t g[16];
 #pragma unroll
   for (int c = 0; c < 16; c++) g[c] = 0;
+#pragma unroll 4
   for (int k = 0; k < ns; k++) {
     const longlong2 *q = p0 + (int64_t)k * 1024 * 8;
 #pragma unroll
@@ -1299,7 +1301,7 @@ int rsg_gram_counts_w8(rsg_ctx *c, rsg_msa *m, float *out, char *errbuf)
   hipLaunchKernelGGL((k_gram_w8<KT>), dim3((unsigned)(8 * per + nsu)), dim3(256), 0, c->stream, m->d_xt, m->d_wq, d_scale,
                      m->Npad, m->Lpad, m->L, out, m->d_nsx, tiles, ndp, per, ntiles, sbase, sextra, part);
   if (nrem > 0)
-    hipLaunchKernelGGL(k_gram_w8_fix, dim3((unsigned)nrem), dim3(1024), 0, c->stream, part, tiles, ndp, sbase, sextra,
+    hipLaunchKernelGGL(k_gram_w8_fix, dim3((unsigned)(4 * nrem)), dim3(256), 0, c->stream, part, tiles, ndp, sbase, sextra,
                        d_scale, m->L, out, m->d_nsx);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
