@@ -49,7 +49,7 @@ P0 = 5e-4
 CAL_PMASS, CAL_PAIRS, CAL_MAXK, CAL_STREAM0 = 5e-3, 1e6, 4, 1 << 20
 NCAL = 3
 DIMS = (NL.NFEAT + NCAL, 128, 128, 128, 2)
-CAP_DIMS = (NL.NFEAT + NCAL, 64, 64, 1)
+CAP_DIMS = (NL.NFEAT + NCAL + 1, 64, 64, 1)     # the inputs and u = ln(null pairs x P0)
 DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_gtp.npz")
 
 
@@ -76,7 +76,8 @@ def random_setup(rng, Nr=(20, 6000), Lr=(50, 3500), big=0.0, xl=0.0):
     L = int(round(math.exp(rng.uniform(math.log(Lr[0]), math.log(Lr[1])))))
     pi = rng.dirichlet(np.full(4, 8.0))
     return dict(N=N, L=L, bl=float(rng.uniform(0.05, 0.3)), pi=tuple(float(p) for p in pi),
-                gap=float(rng.uniform(0.0, 0.3)), tseed=int(rng.integers(1 << 31)), sseed=int(rng.integers(1 << 31)))
+                gap=float(rng.uniform(0.0, 0.3)), tseed=int(rng.integers(1 << 31)), sseed=int(rng.integers(1 << 31)),
+                tq=float(rng.random()))
 
 
 def _record_bytes(L, N, K):
@@ -115,6 +116,15 @@ def inputs(ctx, msa, tree, L, pi, gapfrac, seed=5):
     return np.concatenate([NL.features(ctx, msa, tree), calibrate(ctx, tree, L, pi, gapfrac, seed)])
 
 
+def setup_target(st, target_pairs):
+    """The null pair count a setup's group aims at: target_pairs, or for a (lo, hi) pair the
+    setup's own draw, log-uniform between them (its `tq` in [0, 1))."""
+    if isinstance(target_pairs, (tuple, list)):
+        lo, hi = target_pairs
+        return math.exp(math.log(lo) + st["tq"] * (math.log(hi) - math.log(lo)))
+    return target_pairs
+
+
 def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1, log=None, covtype="GTp",
              target_pairs=2e5, big=0.0, xl=0.0):
     """n training pairs: X (n, NFEAT + NCAL) inputs (features of the group's first replicate and
@@ -132,7 +142,7 @@ def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1,
             return
         jobs = []
         for g, (st, tree, model) in enumerate(pending):
-            K = replicates_for(st["L"], target_pairs)
+            K = replicates_for(st["L"], setup_target(st, target_pairs))
             jobs += [NL.Job(tree, st["L"], model, seed=st["sseed"], stream=r, gapfrac=st["gap"], group=g) for r in range(K)]
         fits, _, fe = NL.batch(ctx, jobs, covtype, pmass=5e-4, expo=True, graph=graph, feats=True)
         cj = []
@@ -144,7 +154,7 @@ def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1,
             if not (np.isfinite(mu) and np.isfinite(lam) and lam > 0 and newmass > 0):
                 continue
             m, lam = tail_params(mu, lam, newmass)
-            K = replicates_for(st["L"], target_pairs)
+            K = replicates_for(st["L"], setup_target(st, target_pairs))
             st = dict(st, K=K, npairs=K * (st["L"] * (st["L"] - 1) // 2), w=float(fits[g][5]),
                       mn=float(fits[g][6]), mx=float(fits[g][7]))
             X.append(np.concatenate([fe[g], cal_inputs(cfits[g])])); Y.append((lam * m, math.log(lam))); S.append(st)
@@ -157,7 +167,8 @@ def generate(ctx, n, seed=1, Nr=(20, 6000), Lr=(50, 3500), budget=24e9, graph=1,
         st = random_setup(rng, Nr, Lr, big, xl)
         tree = NL.yule_tree(st["N"], st["bl"], seed=st["tseed"])
         model = NL.gtr_model(st["pi"])
-        b = _record_bytes(st["L"], st["N"], max(replicates_for(st["L"], target_pairs), cal_replicates(st["L"])))
+        b = _record_bytes(st["L"], st["N"], max(replicates_for(st["L"], setup_target(st, target_pairs)),
+                                                cal_replicates(st["L"])))
         if used + b > budget:
             flush()
         pending.append((st, tree, model)); used += b; made += 1
@@ -173,7 +184,7 @@ class Surrogate:
         self.net = net or MLP(ctx, DIMS, seed=seed, bf16=True)
         self.xmin = self.xmax = None
         self.zab = (-math.log(math.log(2.0)), 1.0)     # exponential tail, median maximum
-        self.cap_net, self.cap_scaling, self.cap_pairs = None, None, math.inf
+        self.cap_net, self.cap_scaling, self.cap_urange = None, None, None
 
     @staticmethod
     def zu(setups, Y):
@@ -187,34 +198,37 @@ class Surrogate:
 
     def fit_cap(self, setups, Y, X=None, epochs=40, batch=256, lr=3e-4, seed=11):
         """(a, b) of z = a + b u over the training groups; with their inputs X, also a small
-        network for each alignment's offset delta = z - (a + b u).  One group's maximum carries
-        a Gumbel spread (sd ~1.2 in z) that no input explains; under it sits an offset set by
-        the tree (the split spectrum above all), which the network learns as a conditional
-        mean.  The offset is learned where the training groups' maxima lie (about
-        cap_pairs = their median null pair count); tail() applies it to nulls at least that
-        large, whose cap extrapolates past it, and the line alone below.  Returns (a, b)."""
+        network for each alignment's offset delta(inputs, u) = z - (a + b u).  One group's
+        maximum carries a Gumbel spread (sd ~1.2 in z) that no input explains; under it sits an
+        offset set by the tree (the split spectrum above all) and by how far past the fitted
+        tail the maximum lies (u), which the network learns as a conditional mean.  Training
+        groups of several sizes per alignment size (generate with a target_pairs range) give it
+        u to learn from; delta() clamps u to the range it saw.  Returns (a, b)."""
         z, u = self.zu(setups, Y)
         ok = np.isfinite(z) & np.isfinite(u)
         b, a = np.polyfit(u[ok], z[ok], 1)
         self.zab = (float(a), float(b))
         if X is not None:
             d = (z - (a + b * u))[ok].astype(np.float32)[:, None]
-            Xo = np.asarray(X, np.float32)[ok]
+            Xo = np.hstack([np.asarray(X, np.float32), u[:, None].astype(np.float32)])[ok]
             xm, xs = Xo.mean(0), Xo.std(0)
             xs = np.where(xs > 1e-6, xs, 1.0)
             self.cap_net = MLP(self.ctx, CAP_DIMS, seed=seed, bf16=True)
             self.cap_scaling = (xm, xs, d.mean(0), d.std(0))
             self.cap_net.set_scaling(*self.cap_scaling)
             self.cap_net.train(Xo, d, epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
-            self.cap_pairs = float(np.median([s["npairs"] for s in setups]))
+            self.cap_urange = (float(u[ok].min()), float(u[ok].max()))
         return self.zab
 
-    def delta(self, feats):
-        """The cap network's offset of z per input row (0 without a cap network)."""
+    def delta(self, feats, u):
+        """The cap network's offset of z per input row at u = ln(null pairs x P0) (scalar or per
+        row; clamped to the training range), 0 without a cap network."""
         feats = np.atleast_2d(np.asarray(feats, np.float32))
         if self.cap_net is None:
             return np.zeros(len(feats))
-        return self.cap_net.forward(feats)[:, 0].astype(np.float64)
+        u = np.clip(np.broadcast_to(np.asarray(u, np.float64), (len(feats),)), *self.cap_urange)
+        x = np.hstack([feats, u[:, None].astype(np.float32)])
+        return self.cap_net.forward(x)[:, 0].astype(np.float64)
 
     def tail(self, feats, K, Nc):
         """(m, lambda, xcap) of one alignment's inputs for a K-replicate null of Nc pairs; NaN,
@@ -223,7 +237,7 @@ class Surrogate:
         if not np.isfinite(mu[0]):
             return math.nan, math.nan, math.inf
         m, lam = float(mu[0]), float(lam[0])
-        delta = float(self.delta(feats)[0]) if K * Nc >= self.cap_pairs else 0.0
+        delta = float(self.delta(feats, math.log(max(K * Nc * P0, 1e-300)))[0])
         return m, lam, self.cap(m, lam, K, Nc, delta=delta)
 
     def cap(self, m, lam, K, Nc, bmin=-10.0, w0=0.05, hpts=400, tol=1e-6, delta=0.0):
@@ -279,7 +293,7 @@ class Surrogate:
         if self.cap_net is not None:
             cW, cb = self.cap_net.get_params()
             cxm, cxs, cym, cys = self.cap_scaling
-            cap = dict(cdims=np.array(CAP_DIMS), cxm=cxm, cxs=cxs, cym=cym, cys=cys, cpairs=np.array(self.cap_pairs),
+            cap = dict(cdims=np.array(CAP_DIMS), cxm=cxm, cxs=cxs, cym=cym, cys=cys, curange=np.array(self.cap_urange),
                        **{"cW%d" % k: w for k, w in enumerate(cW)}, **{"cb%d" % k: v for k, v in enumerate(cb)})
         os.makedirs(os.path.dirname(path), exist_ok=True)
         np.savez(path, dims=np.array(DIMS), xm=xm, xs=xs, ym=ym, ys=ys, xmin=self.xmin, xmax=self.xmax,
@@ -304,7 +318,7 @@ class Surrogate:
             s.cap_net.set_params([z["cW%d" % k] for k in range(len(cd) - 1)], [z["cb%d" % k] for k in range(len(cd) - 1)])
             s.cap_scaling = (z["cxm"], z["cxs"], z["cym"], z["cys"])
             s.cap_net.set_scaling(*s.cap_scaling)
-            s.cap_pairs = float(z["cpairs"])
+            s.cap_urange = (float(z["curange"][0]), float(z["curange"][1]))
         return s
 
 

@@ -48,10 +48,14 @@ def load_data(path):
     return arr["X"], arr["Y"], setups
 
 
+def _tp(v):
+    return v[0] if len(v) == 1 else (v[0], v[1])
+
+
 def cmd_gen(a):
     ctx = Context(0)
     t0 = time.time()
-    X, Y, setups = S.generate(ctx, a.n, seed=a.seed, budget=a.budget, log=log, target_pairs=a.target_pairs, big=a.big,
+    X, Y, setups = S.generate(ctx, a.n, seed=a.seed, budget=a.budget, log=log, target_pairs=_tp(a.target_pairs), big=a.big,
                                   xl=a.xl)
     log("training pairs: %d (%.1f s to simulate, score and fit)" % (len(X), time.time() - t0))
     save_data(a.out, X, Y, setups)
@@ -72,7 +76,7 @@ def cmd_train(a):
     if a.data:
         X, Y, setups = load_data(a.data)
     else:
-        X, Y, setups = S.generate(ctx, a.n, seed=a.seed, budget=a.budget, log=log, target_pairs=a.target_pairs, big=a.big,
+        X, Y, setups = S.generate(ctx, a.n, seed=a.seed, budget=a.budget, log=log, target_pairs=_tp(a.target_pairs), big=a.big,
                                   xl=a.xl)
     t1 = time.time()
     log("training pairs: %d (%.1f s to simulate, score and fit or load)" % (len(X), t1 - t0))
@@ -82,10 +86,17 @@ def cmd_train(a):
     va, tr = idx[:nval], idx[nval:]
     sur = S.Surrogate(ctx, seed=a.seed)
     loss = sur.fit(X[tr], Y[tr], epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed + 2)
-    zab = sur.fit_cap([setups[i] for i in tr], Y[tr], X=X[tr], epochs=a.cap_epochs)
-    zv, uv = S.Surrogate.zu([setups[i] for i in va], Y[va])
+    # the cap model learns from its own data set when one is given (groups of many sizes per
+    # alignment size, for the offset's dependence on u), else from the tail's
+    Xc, Yc, sc, trc, vac = X, Y, setups, tr, va
+    if a.cap_data:
+        Xc, Yc, sc = load_data(a.cap_data)
+        ic = np.random.default_rng(a.seed + 3).permutation(len(Xc))
+        vac, trc = ic[:max(1, len(Xc) // 20)], ic[max(1, len(Xc) // 20):]
+    zab = sur.fit_cap([sc[i] for i in trc], Yc[trc], X=Xc[trc], epochs=a.cap_epochs)
+    zv, uv = S.Surrogate.zu([sc[i] for i in vac], Yc[vac])
     dv = zv - (zab[0] + zab[1] * uv)
-    pv = sur.delta(X[va])
+    pv = sur.delta(Xc[vac], uv)
     okv = np.isfinite(dv)
     cap_r2 = float(1 - np.mean((dv[okv] - pv[okv]) ** 2) / np.var(dv[okv]))
     t2 = time.time()
@@ -213,7 +224,8 @@ def main(argv=None):
     g.add_argument("--out", required=True)
     g.add_argument("--seed", type=int, default=1)
     g.add_argument("--budget", type=float, default=24e9)
-    g.add_argument("--target-pairs", type=float, default=2e6)
+    g.add_argument("--target-pairs", type=float, nargs="+", default=[2e6],
+                   help="null pairs per training group, or a range LO HI drawn log-uniformly per setup")
     g.add_argument("--big", type=float, default=0.5)
     g.add_argument("--xl", type=float, default=0.03)
     t = sub.add_parser("train")
@@ -225,11 +237,12 @@ def main(argv=None):
     t.add_argument("--report", default=None)
     t.add_argument("--epochs", type=int, default=120)
     t.add_argument("--cap-epochs", type=int, default=40)
+    t.add_argument("--cap-data", default=None, help="training data of the cap model (default: --data)")
     t.add_argument("--batch", type=int, default=256)
     t.add_argument("--lr", type=float, default=3e-4)
     t.add_argument("--seed", type=int, default=1)
     t.add_argument("--budget", type=float, default=24e9)
-    t.add_argument("--target-pairs", type=float, default=2e6)
+    t.add_argument("--target-pairs", type=float, nargs="+", default=[2e6])
     e = sub.add_parser("eval")
     e.add_argument("--model", default=S.DATA)
     e.add_argument("--n", type=int, default=100)

@@ -210,10 +210,10 @@ def test_full_nulls_batch_equals_single_nulls(ctx):
 
 def test_surrogate_cap_network_and_save_load(ctx, tmp_path):
     """The cap network learns each group's offset of the largest null score from the linear cap
-    model, a saved and reloaded surrogate gives the same tail and cap, and tail() applies the
-    offset to nulls of at least cap_pairs null pairs only."""
+    model from the inputs and the null size, a saved and reloaded surrogate gives the same tail
+    and cap, and tail() takes the offset at the null's own size (clamped to the training range)."""
     from rscape_amd import surrogate as S
-    X, Y, st = S.generate(ctx, 300, seed=4, Nr=(20, 300), Lr=(50, 160), budget=4e9)
+    X, Y, st = S.generate(ctx, 300, seed=4, Nr=(20, 300), Lr=(50, 160), budget=4e9, target_pairs=(2e4, 4e5))
     assert X.shape[1] == S.DIMS[0] and np.all(np.isfinite(X))
     sur = S.Surrogate(ctx, seed=2)
     sur.fit(X, Y, epochs=20, batch=64, lr=2e-3, seed=5)
@@ -221,8 +221,11 @@ def test_surrogate_cap_network_and_save_load(ctx, tmp_path):
     z, u = S.Surrogate.zu(st, Y)
     ok = np.isfinite(z)
     assert abs(np.mean(z[ok] - (a + b * u[ok]))) < 1e-6           # least-squares line
-    dl = sur.delta(X[:16])
+    assert len(set(s["K"] for s in st)) > 3                           # groups of several sizes
+    dl = sur.delta(X[:16], 5.0)
     assert dl.shape == (16,) and np.all(np.isfinite(dl))
+    lo, hi = sur.cap_urange
+    assert np.array_equal(sur.delta(X[:4], hi + 5.0), sur.delta(X[:4], hi))     # u is clamped
     path = str(tmp_path / "sur.npz")
     sur.save(path)
     back = S.Surrogate.load(ctx, path)
@@ -231,7 +234,5 @@ def test_surrogate_cap_network_and_save_load(ctx, tmp_path):
         assert np.allclose(t1, t2, rtol=1e-6, equal_nan=True)
         if np.isfinite(t1[0]):
             m, lam, _ = t1
-            dl = float(sur.delta(X[k])[0]) if 20 * 5000 >= sur.cap_pairs else 0.0
+            dl = float(sur.delta(X[k], np.log(20 * 5000 * S.P0))[0])
             assert t1[2] == sur.cap(m, lam, 20, 5000, delta=dl)
-            big = sur.tail(X[k], int(np.ceil(sur.cap_pairs / 5000)) + 1, 5000)       # a null past cap_pairs
-            assert big[2] == sur.cap(m, lam, int(np.ceil(sur.cap_pairs / 5000)) + 1, 5000, delta=float(sur.delta(X[k])[0]))
