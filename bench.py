@@ -9,9 +9,12 @@ extremes.  Workload: synthetic rRNA-like alignment N=5000 sequences x L=1500
 columns (400 planted nested pairs, 10% gaps in runs).  Metric: column-pair x sequence
 updates per second, updates = N * L(L-1)/2 per step.
 
-Prints ONE JSON line.  Extra fields report the weighted (split-BF16) variant, the FP32
-statistic variant, a per-kernel time breakdown, the roofline of the dominant kernel
-(the int8 Gram) and the CPU oracle timed on a column window of the same alignment.
+Prints ONE JSON line.  Extra fields report the weighted variants (int8 weight limbs and
+split-BF16), the FP32 and FP64 statistics, a per-kernel time breakdown, the roofline of the
+dominant kernel (the int8 Gram), the CPU oracle on the same alignment, complete significance
+runs (per-replicate null, batched null, surrogate; their E-value accuracy), the batched null of
+1000 uneven alignments, and the N=20000 x L=5000 weighted significance run (--no-evalue,
+--no-batch, --no-c4 skip the last three).
 """
 import argparse
 import json
@@ -521,6 +524,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 1000-alignment null batch")
     ap.add_argument("--c4", action="store_true",
                     help="only the N=20000 x L=5000 significance run (config 4: weighted scoring, 100-replicate null, surrogate)")
+    ap.add_argument("--no-c4", action="store_true", help="skip the N=20000 x L=5000 significance run")
     ap.add_argument("--null-batch-only", action="store_true",
                     help="time only the 1000-alignment null batch and print its JSON")
     args = ap.parse_args()
@@ -629,6 +633,8 @@ def main():
         out["evalue_runs"]["fitted_range_variant"] = fitted_range_variant(ctx)
     if not args.no_batch:
         out["uneven_null_batch"] = uneven_null_batch(ctx)
+    if not args.no_c4:
+        out["config4_significance"] = c4_significance(ctx)
     if args.calib:
         calibration_kernel(ctx)
     if not args.no_cpu:
