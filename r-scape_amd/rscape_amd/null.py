@@ -123,6 +123,32 @@ def yule_tree(n, mean_bl, seed=42, stream=0):
     return dict(left=left, right=right, ld=ld, rd=rd, n=n)
 
 
+def star_tree(n, bl):
+    """R-scape-sim's star tree (cov_evolve_root_ungapped_star, src/cov_simulate.c:241-262: every
+    leaf evolves from the root over the same time bl) as a binary tree the simulator walks by
+    depth: a complete binary tree in heap order (internal nodes 0..n-2, node v's children
+    2v+1 and 2v+2, leaves numbered in order), internal branches of length 0 (P(0) = I, no
+    change), leaf branches of length bl.  Depth ceil(log2 n) instead of n - 1 for a comb."""
+    if n < 2:
+        raise ValueError("a tree needs n >= 2")
+    m = n - 1
+    left = np.zeros(m, dtype=np.int32); right = np.zeros(m, dtype=np.int32)
+    ld = np.zeros(m); rd = np.zeros(m)
+    for v in range(m):
+        for side, c in ((0, 2 * v + 1), (1, 2 * v + 2)):
+            d, t = (c, 0.0) if c < m else (-(c - m), float(bl))
+            (left if side == 0 else right)[v] = d
+            (ld if side == 0 else rd)[v] = t
+    return dict(left=left, right=right, ld=ld, rd=rd, n=n)
+
+
+def rand_tree(n):
+    """R-scape-sim's random alignments (cov_evolve_root_ungapped_rand, src/cov_simulate.c:265-285:
+    every residue drawn from the background frequencies, independently): a star tree whose leaf
+    branches are long enough that P(t) is the stationary distribution to double precision."""
+    return star_tree(n, 1e3)
+
+
 class Job:
     """One alignment to simulate: tree, length, model and RNG key."""
 

@@ -59,6 +59,34 @@ def test_simulation_distribution(ctx):
     assert abs(np.mean(lv[0] != lv[1]) - expect) < 0.005
 
 
+@pytest.mark.parametrize("kind", ["star", "rand"])
+def test_star_and_random_trees(ctx, kind):
+    """R-scape-sim's star and random trees: leaves exchangeable, each pair of leaves the same as
+    on a two-leaf tree, P(same symbol) = sum_a pi_a [P(2t)]_aa for a star of leaf branch t and
+    sum_a pi_a^2 for the random tree (independent residues); composition stationary."""
+    from rscape_amd import synth
+    m = NL.gtr_model(PI)
+    N, L, t = 300, 3000, 0.3
+    tree = NL.star_tree(N, t) if kind == "star" else NL.rand_tree(N)
+    inner = np.concatenate([tree["ld"][tree["left"] > 0], tree["rd"][tree["right"] > 0]])
+    assert np.all(inner == 0) and len(inner) == N - 2                # the star's zero-length spine
+    lv = NL.simulate(ctx, NL.Job(tree, L, m, seed=21))
+    assert lv.shape == (N, L) and lv.max() <= 3
+    comp = np.bincount(lv.ravel(), minlength=4)[:4] / lv.size
+    assert np.allclose(comp, PI, atol=0.01)
+    P = expm(synth.gtr_rate(PI) * 2 * t) if kind == "star" else np.tile(np.array(PI), (4, 1))
+    expect = float(np.sum(np.array(PI) * np.diag(P)))
+    one = np.eye(4, dtype=np.float32)[lv].reshape(N, 4 * L)   # one-hot rows
+    same = (one @ one.T).astype(np.float64) / L              # P(same) for every leaf pair
+    iu = np.triu_indices(N, 1)
+    sd = np.sqrt(expect * (1 - expect) / L)
+    assert abs(same[iu].mean() - expect) < 0.003
+    # exchangeable: no pair further from the expectation than chance allows (about 45k pairs)
+    assert np.max(np.abs(same[iu] - expect)) < 6 * sd
+    # two leaves under one node are no closer than two from opposite halves of the tree
+    assert abs(same[0, 1] - same[0, N - 1]) < 6 * sd
+
+
 def single_path_fit(ctx, leaves, covtype="GTp", pmass=5e-4, expo=True):
     msa = ctx.msa(leaves)
     mn, mx = msa.calculate(covtype)
