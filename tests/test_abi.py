@@ -121,7 +121,7 @@ def test_autotune_table_from_sweep_lines(tmp_path):
     lines ignored."""
     from rscape_amd import autotune
     a, b = tmp_path / "a.log", tmp_path / "b.log"
-    a.write_text("[gpurun] noise\n"
+    a.write_text("[sweep] unrelated line\n"
                  "N=6144 L=1536 best 8 32x64c/4w/kt256 0.0933 ms (2484 TOP/s); built-in choice 0.0949 ms\n"
                  "N=384 L=192 best 8 32x64c/4w/kt256 0.0123 ms (44 TOP/s); built-in choice 0.0130 ms\n")
     b.write_text("N=5000 L=1500 best 7 64x32c/4w/kt256 0.0744 ms (2418 TOP/s); built-in choice 0.0744 ms\n")
