@@ -66,53 +66,67 @@ __device__ __forceinline__ unsigned bytes_eq(unsigned w, unsigned pat)
 }
 
 // corr_NaivePS: ps[i][x] = normalize(1e-5 + sum_s w_s [a_si == x]), x over A, C, G, U, gap.
-// One wave per column (4 per block), 16 codes per load; W: weighted, else unit weights (exact
-// integer counts: four loads in flight per lane, the padding codes (4) of the last load taken
-// off the gap count).
-template <bool W>
-__device__ __forceinline__ void ps_cols(const uint8_t *__restrict__ axT, const double *__restrict__ w, int64_t N,
-                                        int64_t Npad, int64_t L, double *__restrict__ ps, int64_t blk)
+// Unit weights: one wave per column (4 per block), exact integer counts, 16 codes per load, four
+// loads in flight per lane, the padding codes (4) of the last load taken off the gap count.
+__device__ __forceinline__ void ps_cols(const uint8_t *__restrict__ axT, int64_t N, int64_t Npad, int64_t L,
+                                        double *__restrict__ ps, int64_t blk)
 {
   const int lane = threadIdx.x & 63;
   const int64_t i = blk * 4 + (threadIdx.x >> 6);
   if (i >= L) return;
   const uint4 *col = (const uint4 *)(axT + i * Npad);
-  if (!W) {
-    const int64_t nch = (N + 15) / 16;
-    unsigned n[5] = {0u, 0u, 0u, 0u, 0u};
-    auto count = [&](const uint4 v) {
-#pragma unroll
-      for (int x = 0; x < 5; x++)
-        n[x] += bytes_eq(v.x, 0x01010101u * x) + bytes_eq(v.y, 0x01010101u * x) + bytes_eq(v.z, 0x01010101u * x) +
-                bytes_eq(v.w, 0x01010101u * x);
-    };
-    int64_t k = lane;
-    for (; k + 192 < nch; k += 256) {
-      const uint4 v0 = col[k], v1 = col[k + 64], v2 = col[k + 128], v3 = col[k + 192];
-      count(v0); count(v1); count(v2); count(v3);
-    }
-    for (; k < nch; k += 64) count(col[k]);
+  const int64_t nch = (N + 15) / 16;
+  unsigned n[5] = {0u, 0u, 0u, 0u, 0u};
+  auto count = [&](const uint4 v) {
 #pragma unroll
     for (int x = 0; x < 5; x++)
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) n[x] += __shfl_xor(n[x], o, 64);
-    if (lane == 0) {
-      n[4] -= (unsigned)(16 * nch - N);
-      double v[5], sum = 0.;
-      for (int x = 0; x < 5; x++) { v[x] = 1e-5 + (double)n[x]; sum += v[x]; }
-      for (int x = 0; x < 5; x++) ps[i * 5 + x] = v[x] / sum;
-    }
-    return;
+      n[x] += bytes_eq(v.x, 0x01010101u * x) + bytes_eq(v.y, 0x01010101u * x) + bytes_eq(v.z, 0x01010101u * x) +
+              bytes_eq(v.w, 0x01010101u * x);
+  };
+  int64_t k = lane;
+  for (; k + 192 < nch; k += 256) {
+    const uint4 v0 = col[k], v1 = col[k + 64], v2 = col[k + 128], v3 = col[k + 192];
+    count(v0); count(v1); count(v2); count(v3);
   }
-  double acc[5] = {0., 0., 0., 0., 0.};
-  for (int64_t k = lane; k * 16 < N; k += 64) {
-    const uint4 v = col[k];
-    const unsigned wd[4] = {v.x, v.y, v.z, v.w};
+  for (; k < nch; k += 64) count(col[k]);
 #pragma unroll
-    for (int b = 0; b < 16; b++) {
-      const int64_t sq = k * 16 + b;
-      const unsigned a = (sq < N) ? (wd[b >> 2] >> (8 * (b & 3))) & 0xFFu : 0xFFu;
-      const double ws = (a < 5u) ? w[sq] : 0.0;
+  for (int x = 0; x < 5; x++)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) n[x] += __shfl_xor(n[x], o, 64);
+  if (lane == 0) {
+    n[4] -= (unsigned)(16 * nch - N);
+    double v[5], sum = 0.;
+    for (int x = 0; x < 5; x++) { v[x] = 1e-5 + (double)n[x]; sum += v[x]; }
+    for (int x = 0; x < 5; x++) ps[i * 5 + x] = v[x] / sum;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, int64_t N, int64_t Npad, int64_t L,
+                                            double *__restrict__ ps)
+{
+  ps_cols(axT, N, Npad, L, ps, blockIdx.x);
+}
+
+// Weighted: one work group per column (more groups in flight than with several columns per
+// group, which measured slower); wave q of 4 takes sequences 4 l .. 4 l + 3 (l: lane) of every
+// 1024 from 256 q on (one 32-bit word of codes and the four weights of consecutive sequences per
+// step, so a wave's loads are contiguous), FP64 sums per lane, then the wave sums and the four
+// waves' sums in a fixed order.
+__global__ void __launch_bounds__(256) k_ps_w(const uint8_t *__restrict__ axT, const double *__restrict__ w,
+                                              int64_t N, int64_t Npad, int64_t L, double *__restrict__ ps)
+{
+  __shared__ double sh[4][5];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int64_t i = blockIdx.x;
+  const uint32_t *c32 = (const uint32_t *)(axT + i * Npad);
+  double acc[5] = {0., 0., 0., 0., 0.};
+  for (int64_t s0 = 4 * (int64_t)lane + 256 * q; s0 < N; s0 += 1024) {
+    const uint32_t c4 = c32[s0 >> 2];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int64_t sq = s0 + b;
+      const unsigned a = (c4 >> (8 * b)) & 0xFFu;
+      const double ws = (sq < N && a < 5u) ? w[sq] : 0.0;
 #pragma unroll
       for (int x = 0; x < 5; x++) acc[x] += (a == (unsigned)x) ? ws : 0.0;
     }
@@ -120,18 +134,15 @@ __device__ __forceinline__ void ps_cols(const uint8_t *__restrict__ axT, const d
 #pragma unroll
   for (int x = 0; x < 5; x++)
     for (int o = 32; o > 0; o >>= 1) acc[x] += __shfl_down(acc[x], o, 64);
-  if (lane == 0) {
+  if (lane == 0)
+#pragma unroll
+    for (int x = 0; x < 5; x++) sh[q][x] = acc[x];
+  __syncthreads();
+  if (threadIdx.x == 0) {
     double v[5], sum = 0.;
-    for (int x = 0; x < 5; x++) { v[x] = 1e-5 + acc[x]; sum += v[x]; }
+    for (int x = 0; x < 5; x++) { v[x] = 1e-5 + ((sh[0][x] + sh[1][x]) + (sh[2][x] + sh[3][x])); sum += v[x]; }
     for (int x = 0; x < 5; x++) ps[i * 5 + x] = v[x] / sum;
   }
-}
-
-template <bool W>
-__global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, const double *__restrict__ w,
-                                            int64_t N, int64_t Npad, int64_t L, double *__restrict__ ps)
-{
-  ps_cols<W>(axT, w, N, Npad, L, ps, blockIdx.x);
 }
 
 // Unit weights after the int8 Gram, in one launch: the marginals from the Gram's 32 x 32 block
@@ -147,7 +158,7 @@ __global__ void __launch_bounds__(256) k_pm32_ps(const double *__restrict__ part
     if (col < L) pm32_reduce_col(part, part + pm32_n(nb32), nb32, L, col, threadIdx.x & 15, tol, pm, lpm, flag);
     return;
   }
-  ps_cols<false>(axT, nullptr, N, Npad, L, ps, (int64_t)blockIdx.x - nbm);
+  ps_cols(axT, N, Npad, L, ps, (int64_t)blockIdx.x - nbm);
 }
 
 // ln(k + 1e-10), ln(k + 16e-10) and (k + 1e-10) ln(k + 1e-10) for integer k in [0, N]
@@ -200,9 +211,9 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
   if (!ps_done) {
   rsg_tstart(c, "ps", &t);
   if (m->weighted)
-    hipLaunchKernelGGL(k_ps<true>, dim3((unsigned)((L + 3) / 4)), dim3(256), 0, c->stream, m->d_axT, m->d_w, m->N, m->Npad, L, m->d_ps);
+    hipLaunchKernelGGL(k_ps_w, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_axT, m->d_w, m->N, m->Npad, L, m->d_ps);
   else
-    hipLaunchKernelGGL(k_ps<false>, dim3((unsigned)((L + 3) / 4)), dim3(256), 0, c->stream, m->d_axT, m->d_w, m->N, m->Npad, L, m->d_ps);
+    hipLaunchKernelGGL(k_ps, dim3((unsigned)((L + 3) / 4)), dim3(256), 0, c->stream, m->d_axT, m->N, m->Npad, L, m->d_ps);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   }
