@@ -453,9 +453,9 @@ __device__ double pair_stat(const StatArgs &A, const double *cd, int64_t i, int6
 // partials in a fixed order (deterministic) and normalises.  The single-alignment and the
 // batched paths share both, so they round identically.
 #define PMT 64
-#define PMS 4           // j sub-blocks per 64 x 64 tile (one work group each)
+#define PMS 8           // j sub-blocks per 64 x 64 tile (one work group each)
 #define PMJ (PMT / PMS) // j-columns per work group
-#define PMW 4           // waves per work group: PMJ / PMW = 4 j-columns per thread
+#define PMW 4           // waves per work group: PMJ / PMW = 2 j-columns per thread
 // partials of upper-triangle tile (bi <= bj) of nbt x nbt tiles, row-major
 __host__ __device__ __forceinline__ int64_t pm_tri(int64_t bi, int64_t bj, int64_t nbt)
 {
