@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(64 * PMW) k_pm_tile(const void *__restrict__ c
                                                       double *__restrict__ part_i, double *__restrict__ part_j,
                                                       int *__restrict__ flag)
 {
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *flag = 0;   // k_pm_reduce sets it
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *flag = 0;   // k_pm_reduce_ps sets it
   const int bj = (int)blockIdx.x / PMS, sub = (int)blockIdx.x % PMS, bi = (int)blockIdx.y;
   if (bi > bj) return;
   pm_tile<KIND>(cnt, L, nbt, bi, bj, sub, part_i, part_j);
@@ -49,13 +49,6 @@ double *rsg_pm32_scratch(rsg_ctx *c, int64_t L, char *errbuf)
   return c->d_red;
 }
 
-__global__ void __launch_bounds__(256) k_pm_reduce(const double *__restrict__ part_i, const double *__restrict__ part_j,
-                                                   int nbt, int64_t L, double tol, double *__restrict__ pm,
-                                                   double *__restrict__ lpm, int *__restrict__ flag)
-{
-  const int64_t col = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (col < L) pm_reduce_col(part_i, part_j, nbt, L, col, threadIdx.x & 63, tol, pm, lpm, flag);
-}
 
 // bytes of w equal to the byte pattern pat (0x01010101 x): a byte of w ^ pat is zero exactly
 // there, and ((b & 0x7F) + 0x7F) | b has its high bit set exactly when b != 0
@@ -101,23 +94,16 @@ __device__ __forceinline__ void ps_cols(const uint8_t *__restrict__ axT, int64_t
   }
 }
 
-__global__ void __launch_bounds__(256) k_ps(const uint8_t *__restrict__ axT, int64_t N, int64_t Npad, int64_t L,
-                                            double *__restrict__ ps)
-{
-  ps_cols(axT, N, Npad, L, ps, blockIdx.x);
-}
-
-// Weighted: one work group per column (more groups in flight than with several columns per
-// group, which measured slower); wave q of 4 takes sequences 4 l .. 4 l + 3 (l: lane) of every
+// Weighted (the caller's work group covers one column, i; more groups in flight than with several
+// columns per group, which measured slower): wave q of 4 takes sequences 4 l .. 4 l + 3 (l: lane) of every
 // 1024 from 256 q on (one 32-bit word of codes and the four weights of consecutive sequences per
 // step, so a wave's loads are contiguous), FP64 sums per lane, then the wave sums and the four
 // waves' sums in a fixed order.
-__global__ void __launch_bounds__(256) k_ps_w(const uint8_t *__restrict__ axT, const double *__restrict__ w,
-                                              int64_t N, int64_t Npad, int64_t L, double *__restrict__ ps)
+__device__ __forceinline__ void ps_col_w(const uint8_t *__restrict__ axT, const double *__restrict__ w, int64_t N,
+                                         int64_t Npad, double *__restrict__ ps, int64_t i)
 {
   __shared__ double sh[4][5];
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int64_t i = blockIdx.x;
   const uint32_t *c32 = (const uint32_t *)(axT + i * Npad);
   double acc[5] = {0., 0., 0., 0., 0.};
   for (int64_t s0 = 4 * (int64_t)lane + 256 * q; s0 < N; s0 += 1024) {
@@ -145,9 +131,30 @@ __global__ void __launch_bounds__(256) k_ps_w(const uint8_t *__restrict__ axT, c
   }
 }
 
+// The marginals' second pass and the single-residue frequencies in one launch (they are
+// independent): blocks [0, nbr) reduce four columns' marginal partials each (pm_reduce_col),
+// the rest compute ps, one column per block weighted (ps_col_w), four per block unweighted
+// (ps_cols).
+template <bool W>
+__global__ void __launch_bounds__(256) k_pm_reduce_ps(const double *__restrict__ part_i, const double *__restrict__ part_j,
+                                                      int nbt, int64_t L, double tol, double *__restrict__ pm,
+                                                      double *__restrict__ lpm, int *__restrict__ flag, int nbr,
+                                                      const uint8_t *__restrict__ axT, const double *__restrict__ w,
+                                                      int64_t N, int64_t Npad, double *__restrict__ ps)
+{
+  if ((int)blockIdx.x < nbr) {
+    const int64_t col = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (col < L) pm_reduce_col(part_i, part_j, nbt, L, col, threadIdx.x & 63, tol, pm, lpm, flag);
+    return;
+  }
+  const int64_t b = (int64_t)blockIdx.x - nbr;
+  if (W) ps_col_w(axT, w, N, Npad, ps, b);
+  else   ps_cols(axT, N, Npad, L, ps, b);
+}
+
 // Unit weights after the int8 Gram, in one launch: the marginals from the Gram's 32 x 32 block
 // partials (blocks [0, nbm), 16 lanes per column) and the single-residue frequencies (the rest,
-// as k_ps<false>).
+// ps_cols).
 __global__ void __launch_bounds__(256) k_pm32_ps(const double *__restrict__ part, int nb32, int64_t L, double tol,
                                                  double *__restrict__ pm, double *__restrict__ lpm, int *__restrict__ flag,
                                                  const uint8_t *__restrict__ axT, int64_t N, int64_t Npad,
@@ -177,7 +184,6 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
   if (!m->d_pm) HIPCHK(hipMalloc(&m->d_pm, sizeof(double) * 8 * (size_t)L));
   if (!m->d_ps) HIPCHK(hipMalloc(&m->d_ps, sizeof(double) * 5 * (size_t)L));
   int t;
-  bool ps_done = false;
   if (m->pm_fused) {                 // unit weights (int8 counts)
     // the Gram left 32 x 32 block partials in d_red (rsg_pm32_scratch) and cleared the flag
     m->pm_fused = 0;
@@ -187,13 +193,12 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
     hipLaunchKernelGGL(k_pm32_ps, dim3((unsigned)(nbm + nbp)), dim3(256), 0, c->stream, c->d_red, nb32, L, tol,
                        m->d_pm, m->d_pm + 4 * L, (int *)(c->d_res + 4), m->d_axT, m->N, m->Npad, m->d_ps, nbm);
     rsg_tstop(c, t);
-    ps_done = true;
     HIPCHK(hipGetLastError());
   } else {
   const int nbt = (int)((L + PMT - 1) / PMT);
   if (ensure_red(c, (size_t)(pm_npi(nbt) + pm_npj(nbt)), errbuf) != RSG_OK) return RSG_FAIL;
   double *part_i = c->d_red, *part_j = c->d_red + pm_npi(nbt);
-  rsg_tstart(c, "marginals", &t);
+  rsg_tstart(c, "marginals+ps", &t);
   if (m->cnt_kind == CNT_U16)
     hipLaunchKernelGGL(k_pm_tile<CNT_U16>, dim3((unsigned)(nbt * PMS), (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
                        (int *)(c->d_res + 4));
@@ -203,17 +208,14 @@ int rsg_probs_from_counts(rsg_ctx *c, rsg_msa *m, double tol, char *errbuf)
   else
     hipLaunchKernelGGL(k_pm_tile<CNT_F32>, dim3((unsigned)(nbt * PMS), (unsigned)nbt), dim3(64 * PMW), 0, c->stream, m->d_cnt, L, nbt, part_i, part_j,
                        (int *)(c->d_res + 4));
-  hipLaunchKernelGGL(k_pm_reduce, dim3((unsigned)((L + 3) / 4)), dim3(256), 0, c->stream, part_i, part_j, nbt, L, tol,
-                     m->d_pm, m->d_pm + 4 * L, (int *)(c->d_res + 4));
-  rsg_tstop(c, t);
-  HIPCHK(hipGetLastError());
-  }
-  if (!ps_done) {
-  rsg_tstart(c, "ps", &t);
+  // the second pass of the marginals with the single-residue frequencies, in one launch
+  const int nbr = (int)((L + 3) / 4), nps = m->weighted ? (int)L : (int)((L + 3) / 4);
   if (m->weighted)
-    hipLaunchKernelGGL(k_ps_w, dim3((unsigned)L), dim3(256), 0, c->stream, m->d_axT, m->d_w, m->N, m->Npad, L, m->d_ps);
+    hipLaunchKernelGGL(k_pm_reduce_ps<true>, dim3((unsigned)(nbr + nps)), dim3(256), 0, c->stream, part_i, part_j, nbt, L, tol,
+                       m->d_pm, m->d_pm + 4 * L, (int *)(c->d_res + 4), nbr, m->d_axT, m->d_w, m->N, m->Npad, m->d_ps);
   else
-    hipLaunchKernelGGL(k_ps, dim3((unsigned)((L + 3) / 4)), dim3(256), 0, c->stream, m->d_axT, m->N, m->Npad, L, m->d_ps);
+    hipLaunchKernelGGL(k_pm_reduce_ps<false>, dim3((unsigned)(nbr + nps)), dim3(256), 0, c->stream, part_i, part_j, nbt, L, tol,
+                       m->d_pm, m->d_pm + 4 * L, (int *)(c->d_res + 4), nbr, m->d_axT, m->d_w, m->N, m->Npad, m->d_ps);
   rsg_tstop(c, t);
   HIPCHK(hipGetLastError());
   }
