@@ -369,6 +369,16 @@ def c4_significance(ctx, K=100, R_nulls=32, reps=3):
     say("weighted scoring timed")
     out["weighted_scoring_ms"] = float(np.median(ts)) * 1e3
     out["weighted_scoring_updates_per_s"] = N * L * (L - 1) / 2 / float(np.median(ts))
+    # the int8-limb Gram alone (HIP events around its launches): 3 limbs x 16 int8 MACs per update
+    ctx.set_timing("gram")
+    for _ in range(3):
+        m.calculate(**kw)
+    tg = [ms for name, ms in ctx.timings() if name == "gram_w8"]
+    ctx.set_timing(False)
+    if tg:
+        gms = float(np.mean(tg))
+        out["weighted_gram_w8_ms"] = gms
+        out["weighted_gram_w8_frac_int8_peak"] = 3 * 32 * N * L * (L - 1) / 2 / (gms * 1e-3) / (PEAK_INT8_TOPS * 1e12)
     pi, gap = S.msa_null_params(ax)
     model = NL.gtr_model(pi)
     sur = S.Surrogate.load(ctx) if os.path.exists(S.DATA) else None
